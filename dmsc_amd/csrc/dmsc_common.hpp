@@ -1,0 +1,198 @@
+// Shared definitions for the gfx950 compute_dmsc kernels: cell codec, the (value, id) total order,
+// and the lower-star face/edge pairing table.
+//
+// Cell complex (reference dmsc/csrc/cell_complex.hxx:13-51): slot q = y*Nx + x with Nx = W+1 over an
+// (H+1) x (W+1) grid; cell id = 4*q + type, type 0 vertex (pixel, valid y<H,x<W), 1 H-edge (valid
+// y<H), 2 V-edge (valid x<W), 3 face (always valid). Invalid padding cells are never critical and
+// stay -1 in the gradient.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define DMSC_HD __host__ __device__ __forceinline__
+
+namespace dmsc {
+
+// ------------------------------------------------------------------ (value, pixel id) order
+// std::tie semantics of the reference (cpu/cell_compare.hxx:9-31): when neither a<b nor b<a (equal
+// values, -0 == +0, or a NaN operand) the pixel id decides. `higher(a, ia, b, ib)` is
+// value_greater<DUAL>: in the dual complex every comparison is reversed.
+template <bool DUAL>
+DMSC_HD bool higher(float a, int ia, float b, int ib) {
+  if (!DUAL) return (b < a) || (!(a < b) && ib < ia);
+  return (a < b) || (!(b < a) && ia < ib);
+}
+
+// Same relation when the id order of the two pixels is known statically (a_id_larger = id(a) > id(b)).
+template <bool DUAL>
+DMSC_HD bool higher_known(float a, float b, bool a_id_larger) {
+  if (!DUAL) return a_id_larger ? !(a < b) : (b < a);
+  return a_id_larger ? (a < b) : !(b < a);
+}
+
+// float -> uint32 whose unsigned order equals the float order; -0 is folded onto +0 (the reference
+// compares them equal) and every NaN maps to one key above +inf.
+DMSC_HD uint32_t sortable_key(float f) {
+  if (f != f) return 0xFFFFFFFFu;
+  if (f == 0.0f) f = 0.0f;
+  uint32_t u = __builtin_bit_cast(uint32_t, f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+// ------------------------------------------------------------------ lower-star pairing table
+// Lower star of vertex v = (y, x) (reference cpu/gradient.hxx:55-74):
+//   edges in the reference's L_edges order: 0 = H(y,x) "left", 1 = H(y,x+1) "right",
+//                                           2 = V(y,x) "up",   3 = V(y+1,x) "down";
+//   faces in L_faces order:  0 = F(y,x) "UL", 1 = F(y,x+1) "UR", 2 = F(y+1,x) "DL", 3 = F(y+1,x+1) "DR".
+// Every incident edge and face of a valid vertex is a valid cell, so membership is the only
+// variable. A face in the lower star always has both of its v-incident edges in it.
+// Face j's v-incident edges, (first, second):
+constexpr DMSC_HD int face_edge(int j, int which) {
+  // UL: left, up   UR: up, right   DL: left, down   DR: right, down
+  const int t0[4] = {0, 2, 0, 1};
+  const int t1[4] = {2, 1, 3, 3};
+  return which ? t1[j] : t0[j];
+}
+// The two faces bounding edge e, in the order the reference pushes them (gradient.hxx:174-181):
+// H edge (ey,ex): F(ey,ex), F(ey+1,ex);  V edge (ey,ex): F(ey,ex), F(ey,ex+1).
+constexpr DMSC_HD int edge_face(int e, int which) {
+  const int t0[4] = {0, 1, 0, 2};  // left: UL,DL  right: UR,DR  up: UL,UR  down: DL,DR
+  const int t1[4] = {2, 3, 1, 3};
+  return which ? t1[e] : t0[e];
+}
+
+// Result of the reference FIFO face queue (gradient.hxx:115-191) for one lower star, as a byte:
+// bits [2j, 2j+1] = 0 if face j stays unpaired, 1 if paired with face_edge(j,0), 2 with face_edge(j,1).
+// index = edge_mask | face_mask << 4 | estar << 8, estar in 0..3 (edge paired with v) or 4 (none).
+constexpr int kLutSize = 5 * 256;
+
+struct PairLut {
+  uint8_t v[kLutSize];
+};
+
+constexpr uint8_t lower_star_queue(int emask, int fmask, int estar) {
+  int epaired[4] = {0, 0, 0, 0};
+  int fpaired[4] = {0, 0, 0, 0};  // 0 = free, else 1 + which edge slot (0/1)
+  if (estar < 4) epaired[estar] = 1;
+  int q[16] = {};
+  int qn = 0, head = 0;
+  for (int j = 0; j < 4; ++j)
+    if (fmask >> j & 1) q[qn++] = j;
+  while (head < qn) {
+    int f = q[head++];
+    if (fpaired[f]) continue;
+    int nfree = 0, freew = -1;
+    for (int w = 0; w < 2; ++w) {
+      int e = face_edge(f, w);
+      if ((emask >> e & 1) && !epaired[e]) {
+        ++nfree;
+        freew = w;
+      }
+    }
+    if (nfree != 1) continue;
+    int e = face_edge(f, freew);
+    fpaired[f] = 1 + freew;
+    epaired[e] = 1;
+    for (int w = 0; w < 2; ++w) {
+      int nf = edge_face(e, w);
+      if (nf == f || !(fmask >> nf & 1) || fpaired[nf]) continue;
+      bool queued = false;
+      for (int k = head; k < qn; ++k) queued = queued || (q[k] == nf);
+      if (!queued && qn < 16) q[qn++] = nf;
+    }
+  }
+  uint8_t out = 0;
+  for (int j = 0; j < 4; ++j) out |= (uint8_t)(fpaired[j] << (2 * j));
+  return out;
+}
+
+constexpr PairLut make_pair_lut() {
+  PairLut t{};
+  for (int es = 0; es < 5; ++es)
+    for (int fm = 0; fm < 16; ++fm)
+      for (int em = 0; em < 16; ++em) t.v[em | fm << 4 | es << 8] = lower_star_queue(em, fm, es);
+  return t;
+}
+
+// ------------------------------------------------------------------ lower star of one vertex
+// nb[dy+1][dx+1] = neighbour values, ex[..] = neighbour exists in the image. Returns
+//   edge_mask (bits in L_edges order), face_mask (L_faces order), estar (0..3 or 4 = none).
+// Every comparison follows the reference's candidate order (cell_compare.hxx:33-63 for
+// "highest vertex", gradient.hxx:76-112 for the vertex pairing) so the result also matches when
+// NaN makes the order non-transitive. Pixel ids of two neighbours compare as (dy, dx)
+// lexicographically, which lets each comparison be a single float compare.
+struct LowerStar {
+  int emask, fmask, estar;
+};
+
+template <bool DUAL>
+DMSC_HD LowerStar lower_star(const float nb[3][3], const bool ex[3][3]) {
+  const float v = nb[1][1];
+  // g(dy,dx): v higher than neighbour (v's id is larger iff the neighbour precedes it)
+  auto g = [&](int dy, int dx) { return higher_known<DUAL>(v, nb[dy + 1][dx + 1], (dy < 0) || (dy == 0 && dx < 0)); };
+  // h(a, b): neighbour a higher than neighbour b
+  auto h = [&](int ay, int ax, int by, int bx) {
+    bool a_larger = (ay > by) || (ay == by && ax > bx);
+    return higher_known<DUAL>(nb[ay + 1][ax + 1], nb[by + 1][bx + 1], a_larger);
+  };
+  const bool eL = ex[1][0], eR = ex[1][2], eU = ex[0][1], eD = ex[2][1];
+  const bool gL = eL ? g(0, -1) : true, gR = eR ? g(0, 1) : true;
+  const bool gU = eU ? g(-1, 0) : true, gD = eD ? g(1, 0) : true;
+  int emask = (gL ? 1 : 0) | (gR ? 2 : 0) | (gU ? 4 : 0) | (gD ? 8 : 0);
+
+  int fmask = 0;
+  // UL F(y,x): candidates (-1,-1), (-1,0), (0,-1), v  -> v wins iff v beats the running best.
+  {
+    int by = 2, bx = 2;  // none yet
+    auto cand = [&](int cy, int cx) {
+      if (!ex[cy + 1][cx + 1]) return;
+      if (by == 2 || h(cy, cx, by, bx)) { by = cy; bx = cx; }
+    };
+    cand(-1, -1);
+    cand(-1, 0);
+    cand(0, -1);
+    if (by == 2 || g(by, bx)) fmask |= 1;
+  }
+  // UR F(y,x+1): (-1,0), (-1,1), v, (0,1)
+  {
+    int by = 2, bx = 2;
+    if (ex[0][1]) { by = -1; bx = 0; }
+    if (ex[0][2] && (by == 2 || h(-1, 1, by, bx))) { by = -1; bx = 1; }
+    bool vin = (by == 2) || g(by, bx);
+    if (vin && gR) fmask |= 2;
+  }
+  // DL F(y+1,x): (0,-1), v, (1,-1), (1,0)
+  {
+    bool vin = gL;
+    if (vin && ex[2][0] && !g(1, -1)) vin = false;
+    if (vin && !gD) vin = false;
+    if (vin) fmask |= 4;
+  }
+  // DR F(y+1,x+1): v, (0,1), (1,0), (1,1)
+  {
+    bool vin = gR && gD;
+    if (vin && ex[2][2] && !g(1, 1)) vin = false;
+    if (vin) fmask |= 8;
+  }
+
+  // vertex pairing: among lower-star edges with a second vertex, in L_edges order, keep the one
+  // whose other vertex is lowest (replace when the kept one is higher than the candidate).
+  int estar = 4, sy = 0, sx = 0;
+  const int oy[4] = {0, 0, -1, 1}, ox[4] = {-1, 1, 0, 0};
+  const bool oex[4] = {eL, eR, eU, eD};
+  for (int e = 0; e < 4; ++e) {
+    if (!(emask >> e & 1) || !oex[e]) continue;
+    if (estar == 4 || h(sy, sx, oy[e], ox[e])) {
+      estar = e;
+      sy = oy[e];
+      sx = ox[e];
+    }
+  }
+  return LowerStar{emask, fmask, estar};
+}
+
+// ------------------------------------------------------------------ misc
+DMSC_HD int ceil_div(int a, int b) { return (a + b - 1) / b; }
+
+}  // namespace dmsc

@@ -1,0 +1,172 @@
+// Thin PyTorch-ROCm binding of the C-ABI (include/dmsc_hip.h): module dmsc_amd.csrc.dmsc_gpu.
+//
+// Mirrors the reference binding dmsc.csrc.dmsc_gpu (dmsc/csrc/dmsc_gpu.cpp:128-254): same function
+// name, argument names, defaults and error texts, and a DMSComplex object with 19 read-only fields,
+// __getitem__ and __len__. Outputs are torch tensors created by the allocator callback on the
+// input's device (shape is a CPU int32[2], as in the reference GPU path); kernels run on the
+// current HIP stream of that device.
+#include <c10/hip/HIPGuard.h>
+#include <c10/hip/HIPStream.h>
+#include <pybind11/stl.h>
+#include <torch/extension.h>
+
+#include <limits>
+#include <mutex>
+#include <unordered_map>
+#include <vector>
+
+#include "dmsc_hip.h"
+
+namespace {
+
+struct DMSComplex {
+  torch::Tensor t[DMSC_NUM_OUTPUTS];
+};
+
+const char* kNames[DMSC_NUM_OUTPUTS] = {
+    "shape",   "max_pts",    "min_pts",    "sad_pts",     "e_max",          "e_min",
+    "p_max",   "p_min",      "ppairs_max", "ppairs_min",  "grad_indices",   "peaks",
+    "ridges",  "ridge_offsets", "ridge_arc_offsets", "basins", "valleys", "valley_offsets",
+    "valley_arc_offsets"};
+
+struct AllocUser {
+  torch::Device dev;
+  DMSComplex* res;
+};
+
+void* alloc_cb(void* user, int32_t slot, int32_t ndim, const int64_t* dims, int32_t dtype) {
+  auto* a = static_cast<AllocUser*>(user);
+  auto opts = torch::TensorOptions().dtype(dtype == DMSC_DTYPE_FLOAT32 ? torch::kFloat32 : torch::kInt32).device(a->dev);
+  std::vector<int64_t> shape(dims, dims + ndim);
+  a->res->t[slot] = torch::empty(shape, opts);
+  return a->res->t[slot].data_ptr();
+}
+
+dmsc_context* context_for(int device) {
+  static std::mutex mu;
+  static std::unordered_map<int, dmsc_context*> ctxs;  // one per device, kept for the process
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = ctxs.find(device);
+  if (it != ctxs.end()) return it->second;
+  dmsc_context* c = nullptr;
+  TORCH_CHECK(dmsc_context_create(device, &c) == DMSC_OK, "dmsc: could not create a HIP context on device ", device);
+  ctxs[device] = c;
+  return c;
+}
+
+// reference: dmsc/csrc/input_validation.hxx:8-27
+void validate(const torch::Tensor& f) {
+  TORCH_CHECK(f.defined(), "Input tensor must be defined");
+  TORCH_CHECK(f.layout() == c10::kStrided, "Input tensor must use strided layout");
+  TORCH_CHECK(f.scalar_type() == torch::kFloat32, "Input tensor must have dtype torch.float32, got ", f.scalar_type());
+  TORCH_CHECK(f.dim() == 2 || f.dim() == 3, "Input tensor must be 2D [H, W] or 3D [B, H, W], got ", f.dim(), "D");
+  if (f.dim() == 3) TORCH_CHECK(f.size(0) > 0, "Batch dimension must be non-empty");
+  const int64_t h = f.size(-2), w = f.size(-1);
+  TORCH_CHECK(h > 0 && w > 0, "Spatial dimensions must be non-empty, got ", h, "x", w);
+  const int64_t max_cells = std::numeric_limits<int32_t>::max();
+  TORCH_CHECK(h + 1 <= max_cells / 4 / (w + 1), "Input is too large for 32-bit cell IDs: ", h, "x", w);
+}
+
+uint32_t make_flags(bool grad, bool dual, bool tmax, bool tmin, bool gmax, bool gmin) {
+  return (grad ? DMSC_RETURN_GRADIENT : 0u) | (dual ? DMSC_DUAL : 0u) | (tmax ? DMSC_TRACE_MAX_ARCS : 0u) |
+         (tmin ? DMSC_TRACE_MIN_ARCS : 0u) | (gmax ? DMSC_TRACE_MAX_GROUPS : 0u) | (gmin ? DMSC_TRACE_MIN_GROUPS : 0u);
+}
+
+DMSComplex run_field(dmsc_context* ctx, const torch::Tensor& f, float thr, uint32_t flags, hipStream_t stream) {
+  DMSComplex r;
+  AllocUser u{f.device(), &r};
+  dmsc_outputs out;
+  dmsc_status st;
+  {
+    pybind11::gil_scoped_release release;
+    st = dmsc_extract(ctx, f.data_ptr<float>(), (int32_t)f.size(0), (int32_t)f.size(1), thr, flags,
+                      (dmsc_stream_t)stream, alloc_cb, &u, &out);
+  }
+  TORCH_CHECK(st == DMSC_OK, dmsc_last_error(ctx));
+  r.t[0] = torch::empty({2}, torch::TensorOptions().dtype(torch::kInt32));
+  r.t[0][0] = out.shape[0];
+  r.t[0][1] = out.shape[1];
+  return r;
+}
+
+pybind11::object extract_dmsc(torch::Tensor scalar_field, float persistence_threshold, bool return_gradient,
+                              bool is_dual, bool trace_max_arcs, bool trace_min_arcs, bool trace_max_groups,
+                              bool trace_min_groups) {
+  validate(scalar_field);
+  TORCH_CHECK(scalar_field.device().is_cuda(), "GPU backend requires a CUDA or MPS tensor, got ",
+              scalar_field.device());
+  c10::hip::HIPGuard guard(scalar_field.device().index());
+  scalar_field = scalar_field.contiguous();
+  dmsc_context* ctx = context_for(scalar_field.device().index());
+  hipStream_t stream = c10::hip::getCurrentHIPStream(scalar_field.device().index()).stream();
+  const uint32_t flags = make_flags(return_gradient, is_dual, trace_max_arcs, trace_min_arcs, trace_max_groups,
+                                    trace_min_groups);
+  if (scalar_field.dim() == 2) return pybind11::cast(run_field(ctx, scalar_field, persistence_threshold, flags, stream));
+  std::vector<DMSComplex> res;
+  res.reserve(scalar_field.size(0));
+  for (int64_t b = 0; b < scalar_field.size(0); ++b)
+    res.push_back(run_field(ctx, scalar_field[b], persistence_threshold, flags, stream));
+  return pybind11::cast(res);
+}
+
+// Mixed-size batch: a list of 2-D fields with one threshold each (no reference counterpart; the
+// reference batches only one shape with one threshold, dmsc_gpu.cpp:84-126).
+std::vector<DMSComplex> extract_dmsc_list(std::vector<torch::Tensor> fields, std::vector<double> thresholds,
+                                          bool return_gradient, bool is_dual, bool trace_max_arcs,
+                                          bool trace_min_arcs, bool trace_max_groups, bool trace_min_groups) {
+  TORCH_CHECK(!fields.empty(), "Batch dimension must be non-empty");
+  TORCH_CHECK(thresholds.size() == fields.size(), "one persistence threshold per field is required");
+  const auto dev = fields[0].device();
+  TORCH_CHECK(dev.is_cuda(), "GPU backend requires a CUDA or MPS tensor, got ", dev);
+  for (auto& f : fields) {
+    validate(f);
+    TORCH_CHECK(f.dim() == 2, "each field of a list batch must be 2D [H, W]");
+    TORCH_CHECK(f.device() == dev, "all fields of a batch must be on one device");
+  }
+  c10::hip::HIPGuard guard(dev.index());
+  dmsc_context* ctx = context_for(dev.index());
+  hipStream_t stream = c10::hip::getCurrentHIPStream(dev.index()).stream();
+  const uint32_t flags = make_flags(return_gradient, is_dual, trace_max_arcs, trace_min_arcs, trace_max_groups,
+                                    trace_min_groups);
+  std::vector<DMSComplex> res;
+  res.reserve(fields.size());
+  for (size_t b = 0; b < fields.size(); ++b)
+    res.push_back(run_field(ctx, fields[b].contiguous(), (float)thresholds[b], flags, stream));
+  return res;
+}
+
+pybind11::object debug_fetch(int device, const std::string& name, int64_t capacity, bool as_float, bool as_u64) {
+  dmsc_context* ctx = context_for(device);
+  auto dt = as_u64 ? torch::kInt64 : (as_float ? torch::kFloat32 : torch::kInt32);
+  torch::Tensor t = torch::empty({capacity}, torch::TensorOptions().dtype(dt));
+  int64_t n = dmsc_debug_fetch(ctx, name.c_str(), t.data_ptr(), capacity);
+  if (n < 0) return pybind11::none();
+  return pybind11::cast(t.slice(0, 0, std::min(n, capacity)));
+}
+
+}  // namespace
+
+PYBIND11_MODULE(dmsc_gpu, m) {
+  auto cls = pybind11::class_<DMSComplex>(m, "DMSComplex", pybind11::module_local(),
+                                          "Discrete Morse-Smale complex (19 tensors, reference slot order)");
+  for (int i = 0; i < DMSC_NUM_OUTPUTS; ++i)
+    cls.def_property_readonly(kNames[i], [i](const DMSComplex& r) { return r.t[i]; });
+  cls.def("__getitem__", [](const DMSComplex& r, int idx) {
+    if (idx < 0 || idx >= DMSC_NUM_OUTPUTS) throw pybind11::index_error();
+    return r.t[idx];
+  });
+  cls.def("__len__", [](const DMSComplex&) { return DMSC_NUM_OUTPUTS; });
+
+  m.def("extract_dmsc", &extract_dmsc, "MI355X Discrete Morse-Smale complex computation",
+        pybind11::arg("scalar_field"), pybind11::arg("persistence_threshold"), pybind11::arg("return_gradient") = false,
+        pybind11::arg("is_dual") = false, pybind11::arg("trace_max_arcs") = false,
+        pybind11::arg("trace_min_arcs") = false, pybind11::arg("trace_max_groups") = false,
+        pybind11::arg("trace_min_groups") = false);
+  m.def("extract_dmsc_list", &extract_dmsc_list, "Mixed-size batch of fields with per-field thresholds",
+        pybind11::arg("fields"), pybind11::arg("persistence_thresholds"), pybind11::arg("return_gradient") = false,
+        pybind11::arg("is_dual") = false, pybind11::arg("trace_max_arcs") = false,
+        pybind11::arg("trace_min_arcs") = false, pybind11::arg("trace_max_groups") = false,
+        pybind11::arg("trace_min_groups") = false);
+  m.def("debug_fetch", &debug_fetch, pybind11::arg("device"), pybind11::arg("name"), pybind11::arg("capacity"),
+        pybind11::arg("as_float") = false, pybind11::arg("as_u64") = false);
+}

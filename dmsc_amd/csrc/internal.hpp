@@ -1,0 +1,137 @@
+// Host-side plumbing shared by the kernel translation units: device scratch arena, launch
+// helpers and the launcher declarations of every kernel.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "dmsc_common.hpp"
+
+namespace dmsc {
+
+struct HipError {
+  hipError_t err;
+  const char* what;
+};
+
+#define DMSC_CHECK(call)                                \
+  do {                                                  \
+    hipError_t e_ = (call);                             \
+    if (e_ != hipSuccess) throw ::dmsc::HipError{e_, #call}; \
+  } while (0)
+
+#define DMSC_LAUNCH_CHECK() DMSC_CHECK(hipGetLastError())
+
+// Grow-only device scratch, one slot per named buffer. Buffers keep their contents between calls
+// only by accident: every stage initialises what it reads.
+class Arena {
+ public:
+  ~Arena() { release(); }
+  void* get(int slot, size_t bytes) {
+    if ((size_t)slot >= slots_.size()) slots_.resize(slot + 1);
+    Slot& s = slots_[slot];
+    if (bytes == 0) bytes = 256;
+    if (s.cap < bytes) {
+      if (s.p) DMSC_CHECK(hipFree(s.p));
+      s.p = nullptr;
+      size_t cap = bytes + bytes / 4 + 4096;
+      DMSC_CHECK(hipMalloc(&s.p, cap));
+      s.cap = cap;
+    }
+    return s.p;
+  }
+  template <class T>
+  T* get(int slot, size_t count) {
+    return static_cast<T*>(get(slot, count * sizeof(T)));
+  }
+  void release() {
+    for (auto& s : slots_)
+      if (s.p) (void)hipFree(s.p);
+    slots_.clear();
+  }
+
+ private:
+  struct Slot {
+    void* p = nullptr;
+    size_t cap = 0;
+  };
+  std::vector<Slot> slots_;
+};
+
+// Device-side error flags (bit set by kernels that hit a loop cap or a capacity bound).
+enum : int32_t { ERR_PJ_CAP = 1, ERR_UF_CAP = 2, ERR_SEG_CAP = 4, ERR_GEO_CAP = 8 };
+constexpr int kLoopCap = 1 << 22;
+__device__ __forceinline__ void report(int32_t* err, int32_t code) {
+  if (err) atomicOr(err, code);
+}
+
+inline int grid_for(int64_t n, int block, int max_blocks = 1 << 20) {
+  int64_t g = (n + block - 1) / block;
+  if (g < 1) g = 1;
+  if (g > max_blocks) g = max_blocks;
+  return (int)g;
+}
+
+// Geometry of one field.
+struct Dims {
+  int H, W, Nx, Ny;
+  int64_t S, N;  // slots (H+1)(W+1), pixels H*W
+  int WPR;       // 32-slot words per slot row
+  int64_t PW;    // words per bit plane
+};
+inline Dims make_dims(int H, int W) {
+  Dims d;
+  d.H = H;
+  d.W = W;
+  d.Nx = W + 1;
+  d.Ny = H + 1;
+  d.S = (int64_t)d.Nx * d.Ny;
+  d.N = (int64_t)H * W;
+  d.WPR = (d.Nx + 31) / 32;
+  d.PW = (int64_t)d.Ny * d.WPR;
+  return d;
+}
+
+// Critical-cell rank structure: bit planes (type t at planes + t*PW) plus exclusive popcount
+// prefixes per word for minima (type 0), saddles (types 1+2) and maxima (type 3).
+struct RankView {
+  const uint32_t* planes;
+  const int32_t* pre_min;
+  const int32_t* pre_sad;
+  const int32_t* pre_max;
+  int64_t PW;
+  int WPR, Nx, W;
+};
+
+// ------------------------------------------------------------------ launchers (k_*.hip)
+// primitives
+template <class T>
+void scan_exclusive(const T* in, T* out, int64_t n, T* d_total, Arena& ar, int slot, hipStream_t st);
+void scan_plane_popcounts(const uint32_t* plane_a, const uint32_t* plane_b, int32_t* out, int64_t n, int32_t* d_total,
+                          Arena& ar, int slot, hipStream_t st);
+void radix_sort_pairs(uint32_t* keys, int32_t* vals, uint32_t* keys_tmp, int32_t* vals_tmp, int64_t n, Arena& ar,
+                      int slot, hipStream_t st);
+void pointer_jump(uint64_t* pj, int64_t n, int32_t* err, hipStream_t st);
+void pointer_jump_init(const int32_t* succ, uint64_t* pj, int64_t n, hipStream_t st);
+void fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t st);
+void fill_u64(uint64_t* p, uint64_t v, int64_t n, hipStream_t st);
+void iota_i32(int32_t* p, int64_t n, hipStream_t st);
+
+// grid stages
+void launch_gradient(bool dual, const float* f, const Dims& d, int32_t* pw, int32_t* fsucc, int32_t* vsucc,
+                     uint32_t* planes, hipStream_t st);
+void launch_compact_lists(bool dual, const float* f, const Dims& d, const RankView& rv, int32_t* min_ids,
+                          float* min_val, int32_t* sad_ids, float* sad_val, int32_t* max_ids, float* max_val,
+                          hipStream_t st);
+void launch_saddle_records(const Dims& d, const RankView& rv, const int32_t* sad_ids, int64_t nsad,
+                           const uint64_t* pj_face, const uint64_t* pj_vert, int32_t* cmax, int32_t* cmin,
+                           int32_t* lmax, int32_t* lmin, hipStream_t st);
+void launch_labels_faces(const Dims& d, const RankView& rv, const uint64_t* pj_face, const int32_t* root_t,
+                         const int32_t* to_out, int32_t* out, hipStream_t st);
+void launch_labels_verts(const Dims& d, const RankView& rv, const uint64_t* pj_vert, const int32_t* root_t,
+                         const int32_t* to_out, int32_t* out, hipStream_t st);
+
+}  // namespace dmsc

@@ -1,0 +1,329 @@
+// Grid-shaped stages: discrete gradient + critical bit planes (one LDS-tiled stencil pass),
+// order-preserving critical lists, saddle records and basin/peak labels.
+#include "internal.hpp"
+
+namespace dmsc {
+
+__constant__ constexpr PairLut c_lut = make_pair_lut();
+
+// ------------------------------------------------------------------ value of a cell
+// Pixel index of the "highest" vertex of cell (t, y, x) in the reference's candidate order
+// (cpu/cell_compare.hxx:33-63); the cell value is the field value there.
+template <bool DUAL>
+__device__ __forceinline__ int top_pixel(int t, int y, int x, int H, int W, const float* __restrict__ f) {
+  int best = -1;
+  float bv = 0.f;
+  auto consider = [&](int vy, int vx) {
+    int id = vy * W + vx;
+    float v = f[id];
+    if (best == -1 || higher<DUAL>(v, id, bv, best)) {
+      best = id;
+      bv = v;
+    }
+  };
+  if (t == 0) {
+    consider(y, x);
+  } else if (t == 1) {
+    if (x - 1 >= 0) consider(y, x - 1);
+    if (x < W) consider(y, x);
+  } else if (t == 2) {
+    if (y - 1 >= 0) consider(y - 1, x);
+    if (y < H) consider(y, x);
+  } else {
+    if (y - 1 >= 0 && x - 1 >= 0) consider(y - 1, x - 1);
+    if (y - 1 >= 0 && x < W) consider(y - 1, x);
+    if (y < H && x - 1 >= 0) consider(y, x - 1);
+    if (y < H && x < W) consider(y, x);
+  }
+  return best;
+}
+
+// ------------------------------------------------------------------ K1: gradient
+// One 64 x 16 tile of slots per 256-thread block (a wave per slot row). Phase 1 evaluates the lower
+// star of every vertex whose star touches the tile ((16+1) x (64+1) vertices, pixels staged in
+// LDS with a 2-pixel halo) and scatters the pairing of each lower-star cell into an LDS cell tile;
+// every valid cell lies in exactly one lower star, so there are no write conflicts. Phase 2 writes
+// each slot's four partners as one 16-byte store, the face / vertex V-path successors, and the
+// four critical bit planes via wave ballots.
+constexpr int GT_X = 64, GT_Y = 16;
+constexpr int GP_W = GT_X + 3, GP_H = GT_Y + 3;
+
+template <bool DUAL>
+__global__ __launch_bounds__(256) void k_gradient(const float* __restrict__ f, int H, int W,
+                                                  int32_t* __restrict__ pw, int32_t* __restrict__ fsucc,
+                                                  int32_t* __restrict__ vsucc, uint32_t* __restrict__ planes,
+                                                  int64_t PW, int WPR) {
+  __shared__ float pix[GP_H][GP_W];
+  __shared__ int32_t cell[GT_Y][GT_X][4];
+  __shared__ uint8_t lut[kLutSize];
+  const int Nx = W + 1, Ny = H + 1;
+  const int x0 = blockIdx.x * GT_X, y0 = blockIdx.y * GT_Y;
+  const int tid = threadIdx.x;
+
+  for (int i = tid; i < kLutSize; i += 256) lut[i] = c_lut.v[i];
+  for (int i = tid; i < GP_H * GP_W; i += 256) {
+    int r = i / GP_W, c = i - r * GP_W;
+    int py = y0 - 2 + r, px = x0 - 2 + c;
+    pix[r][c] = (py >= 0 && py < H && px >= 0 && px < W) ? f[(int64_t)py * W + px] : 0.0f;
+  }
+  for (int i = tid; i < GT_Y * GT_X * 4; i += 256) (&cell[0][0][0])[i] = -1;
+  __syncthreads();
+
+  for (int i = tid; i < (GT_Y + 1) * (GT_X + 1); i += 256) {
+    const int ly = i / (GT_X + 1), lx = i - ly * (GT_X + 1);
+    const int vy = y0 - 1 + ly, vx = x0 - 1 + lx;
+    if (vy < 0 || vy >= H || vx < 0 || vx >= W) continue;
+    float nb[3][3];
+    bool ex[3][3];
+#pragma unroll
+    for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+      for (int dx = -1; dx <= 1; ++dx) {
+        int py = vy + dy, px = vx + dx;
+        ex[dy + 1][dx + 1] = (py >= 0 && py < H && px >= 0 && px < W);
+        nb[dy + 1][dx + 1] = pix[ly + dy + 1][lx + dx + 1];
+      }
+    const LowerStar ls = lower_star<DUAL>(nb, ex);
+    const int code = lut[ls.emask | ls.fmask << 4 | ls.estar << 8];
+    const int q = vy * Nx + vx;
+    const int eid[4] = {4 * q + 1, 4 * (q + 1) + 1, 4 * q + 2, 4 * (q + Nx) + 2};
+    const int fid[4] = {4 * q + 3, 4 * (q + 1) + 3, 4 * (q + Nx) + 3, 4 * (q + Nx + 1) + 3};
+    const int sy = vy - y0, sx = vx - x0;
+    if (sy >= 0 && sx >= 0) cell[sy][sx][0] = ls.estar < 4 ? eid[ls.estar] : -1;
+    const int esy[4] = {0, 0, 0, 1}, esx[4] = {0, 1, 0, 0};
+    const int fsy[4] = {0, 0, 1, 1}, fsx[4] = {0, 1, 0, 1};
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      if (!(ls.emask >> e & 1)) continue;
+      int ty = sy + esy[e], tx = sx + esx[e];
+      if (ty < 0 || ty >= GT_Y || tx < 0 || tx >= GT_X) continue;
+      int partner = -1;
+      if (e == ls.estar) partner = 4 * q;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        int cj = (code >> (2 * j)) & 3;
+        if (cj && face_edge(j, cj - 1) == e) partner = fid[j];
+      }
+      cell[ty][tx][e < 2 ? 1 : 2] = partner;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (!(ls.fmask >> j & 1)) continue;
+      int ty = sy + fsy[j], tx = sx + fsx[j];
+      if (ty < 0 || ty >= GT_Y || tx < 0 || tx >= GT_X) continue;
+      int cj = (code >> (2 * j)) & 3;
+      cell[ty][tx][3] = cj ? eid[face_edge(j, cj - 1)] : -1;
+    }
+  }
+  __syncthreads();
+
+  const int lane = tid & 63, wv = tid >> 6;
+  for (int r = wv; r < GT_Y; r += 4) {
+    const int y = y0 + r, x = x0 + lane;
+    const bool in = (y < Ny) && (x < Nx);
+    const int c0 = cell[r][lane][0], c1 = cell[r][lane][1], c2 = cell[r][lane][2], c3 = cell[r][lane][3];
+    const bool k0 = in && c0 < 0 && y < H && x < W;
+    const bool k1 = in && c1 < 0 && y < H;
+    const bool k2 = in && c2 < 0 && x < W;
+    const bool k3 = in && c3 < 0;
+    if (in) {
+      const int q = y * Nx + x;
+      if (pw) *reinterpret_cast<int4*>(pw + 4 * (int64_t)q) = make_int4(c0, c1, c2, c3);
+      int fs = q;
+      if (c3 >= 0) {
+        if (c3 == 4 * (q - Nx) + 1) fs = q - Nx;
+        else if (c3 == 4 * q + 1) fs = q + Nx;
+        else if (c3 == 4 * (q - 1) + 2) fs = q - 1;
+        else fs = q + 1;
+      }
+      fsucc[q] = fs;
+      if (y < H && x < W) {
+        const int p = y * W + x;
+        int vs = p;
+        if (c0 >= 0) {
+          if (c0 == 4 * q + 1) vs = p - 1;
+          else if (c0 == 4 * (q + 1) + 1) vs = p + 1;
+          else if (c0 == 4 * q + 2) vs = p - W;
+          else vs = p + W;
+        }
+        vsucc[p] = vs;
+      }
+    }
+    const uint64_t b0 = __ballot(k0), b1 = __ballot(k1), b2 = __ballot(k2), b3 = __ballot(k3);
+    if (y < Ny && lane < 8) {
+      const int t = lane >> 1, half = lane & 1;
+      const uint64_t bb = t == 0 ? b0 : t == 1 ? b1 : t == 2 ? b2 : b3;
+      const int wi = (x0 >> 5) + half;
+      if (wi < WPR) planes[t * PW + (int64_t)y * WPR + wi] = (uint32_t)(bb >> (32 * half));
+    }
+  }
+}
+
+void launch_gradient(bool dual, const float* f, const Dims& d, int32_t* pw, int32_t* fsucc, int32_t* vsucc,
+                     uint32_t* planes, hipStream_t st) {
+  dim3 grid(ceil_div(d.Nx, GT_X), ceil_div(d.Ny, GT_Y));
+  if (dual)
+    hipLaunchKernelGGL(k_gradient<true>, grid, dim3(256), 0, st, f, d.H, d.W, pw, fsucc, vsucc, planes, d.PW, d.WPR);
+  else
+    hipLaunchKernelGGL(k_gradient<false>, grid, dim3(256), 0, st, f, d.H, d.W, pw, fsucc, vsucc, planes, d.PW, d.WPR);
+  DMSC_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------------------------ ranks
+// index of a critical cell within its list (the reference's fast_crit_map, gradient.hxx:198-231)
+__device__ __forceinline__ int rank_in_plane(const uint32_t* plane, const int32_t* pre, int WPR, int y, int x) {
+  const int64_t w = (int64_t)y * WPR + (x >> 5);
+  const uint32_t below = (1u << (x & 31)) - 1u;
+  return pre[w] + __popc(plane[w] & below);
+}
+__device__ __forceinline__ int rank_max_slot(const RankView& rv, int q) {
+  int y = q / rv.Nx, x = q - y * rv.Nx;
+  return rank_in_plane(rv.planes + 3 * rv.PW, rv.pre_max, rv.WPR, y, x);
+}
+__device__ __forceinline__ int rank_min_pixel(const RankView& rv, int p) {
+  int y = p / rv.W, x = p - y * rv.W;
+  return rank_in_plane(rv.planes, rv.pre_min, rv.WPR, y, x);
+}
+
+// ------------------------------------------------------------------ critical lists
+// One thread per plane word; set bits are emitted in ascending cell id (the reference's single-
+// thread order, critical_points.hxx:14-65), with the extremum / saddle value.
+template <bool DUAL>
+__global__ void k_compact_lists(const float* __restrict__ f, int H, int W, RankView rv,
+                                int32_t* __restrict__ min_ids, float* __restrict__ min_val,
+                                int32_t* __restrict__ sad_ids, float* __restrict__ sad_val,
+                                int32_t* __restrict__ max_ids, float* __restrict__ max_val) {
+  const int Nx = W + 1;
+  for (int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; w < rv.PW; w += (int64_t)gridDim.x * blockDim.x) {
+    const int y = (int)(w / rv.WPR), xb = (int)(w - (int64_t)y * rv.WPR) * 32;
+    uint32_t m0 = rv.planes[w], m1 = rv.planes[rv.PW + w], m2 = rv.planes[2 * rv.PW + w], m3 = rv.planes[3 * rv.PW + w];
+    int o = rv.pre_min[w];
+    while (m0) {
+      int b = __builtin_ctz(m0);
+      m0 &= m0 - 1;
+      int x = xb + b;
+      min_ids[o] = 4 * (y * Nx + x);
+      min_val[o] = f[(int64_t)y * W + x];
+      ++o;
+    }
+    o = rv.pre_max[w];
+    while (m3) {
+      int b = __builtin_ctz(m3);
+      m3 &= m3 - 1;
+      int x = xb + b;
+      max_ids[o] = 4 * (y * Nx + x) + 3;
+      max_val[o] = f[top_pixel<DUAL>(3, y, x, H, W, f)];
+      ++o;
+    }
+    o = rv.pre_sad[w];
+    uint32_t m12 = m1 | m2;
+    while (m12) {
+      int b = __builtin_ctz(m12);
+      m12 &= m12 - 1;
+      int x = xb + b;
+      if (m1 >> b & 1) {
+        sad_ids[o] = 4 * (y * Nx + x) + 1;
+        sad_val[o] = f[top_pixel<DUAL>(1, y, x, H, W, f)];
+        ++o;
+      }
+      if (m2 >> b & 1) {
+        sad_ids[o] = 4 * (y * Nx + x) + 2;
+        sad_val[o] = f[top_pixel<DUAL>(2, y, x, H, W, f)];
+        ++o;
+      }
+    }
+  }
+}
+
+void launch_compact_lists(bool dual, const float* f, const Dims& d, const RankView& rv, int32_t* min_ids,
+                          float* min_val, int32_t* sad_ids, float* sad_val, int32_t* max_ids, float* max_val,
+                          hipStream_t st) {
+  int g = grid_for(d.PW, 256, 8192);
+  if (dual)
+    hipLaunchKernelGGL(k_compact_lists<true>, dim3(g), dim3(256), 0, st, f, d.H, d.W, rv, min_ids, min_val, sad_ids,
+                       sad_val, max_ids, max_val);
+  else
+    hipLaunchKernelGGL(k_compact_lists<false>, dim3(g), dim3(256), 0, st, f, d.H, d.W, rv, min_ids, min_val, sad_ids,
+                       sad_val, max_ids, max_val);
+  DMSC_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------------------------ saddle records
+// Per saddle (list order): the extremum reached by each of its four V-paths and the step count
+// (reference arcs_topology.hxx:67-172). Arc k of a side is its k-th existing start cell; a missing
+// start (border edge, vertex side only) gives target -1 and length 0.
+__global__ void k_saddle_records(int H, int W, RankView rv, const int32_t* __restrict__ sad_ids, int64_t nsad,
+                                 const uint64_t* __restrict__ pj_face, const uint64_t* __restrict__ pj_vert,
+                                 int32_t* __restrict__ cmax, int32_t* __restrict__ cmin, int32_t* __restrict__ lmax,
+                                 int32_t* __restrict__ lmin) {
+  const int Nx = W + 1;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nsad; i += (int64_t)gridDim.x * blockDim.x) {
+    const int id = sad_ids[i], q = id >> 2, t = id & 3;
+    const int y = q / Nx, x = q - y * Nx;
+    const int f1 = q, f2 = (t == 1) ? q + Nx : q + 1;
+    const uint64_t a = pj_face[f1], b = pj_face[f2];
+    cmax[2 * i] = rank_max_slot(rv, (int)(uint32_t)a);
+    cmax[2 * i + 1] = rank_max_slot(rv, (int)(uint32_t)b);
+    lmax[2 * i] = (int)(a >> 32);
+    lmax[2 * i + 1] = (int)(b >> 32);
+    int v[2], nv = 0;
+    if (t == 1) {
+      if (x - 1 >= 0) v[nv++] = y * W + x - 1;
+      if (x < W) v[nv++] = y * W + x;
+    } else {
+      if (y - 1 >= 0) v[nv++] = (y - 1) * W + x;
+      if (y < H) v[nv++] = y * W + x;
+    }
+    for (int k = 0; k < 2; ++k) {
+      if (k < nv) {
+        const uint64_t c = pj_vert[v[k]];
+        cmin[2 * i + k] = rank_min_pixel(rv, (int)(uint32_t)c);
+        lmin[2 * i + k] = (int)(c >> 32);
+      } else {
+        cmin[2 * i + k] = -1;
+        lmin[2 * i + k] = 0;
+      }
+    }
+  }
+}
+
+void launch_saddle_records(const Dims& d, const RankView& rv, const int32_t* sad_ids, int64_t nsad,
+                           const uint64_t* pj_face, const uint64_t* pj_vert, int32_t* cmax, int32_t* cmin,
+                           int32_t* lmax, int32_t* lmin, hipStream_t st) {
+  if (nsad == 0) return;
+  hipLaunchKernelGGL(k_saddle_records, dim3(grid_for(nsad, 256, 16384)), dim3(256), 0, st, d.H, d.W, rv, sad_ids, nsad,
+                     pj_face, pj_vert, cmax, cmin, lmax, lmin);
+  DMSC_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------------------------ labels
+// label(cell) = output index of the thresholded-UF root of its V-path terminal, -1 for the void
+// or a cancelled root (reference cell_groups.hxx:96-229).
+__global__ void k_labels_faces(int64_t S, RankView rv, const uint64_t* __restrict__ pj, const int32_t* __restrict__ root_t,
+                               const int32_t* __restrict__ to_out, int32_t* __restrict__ out) {
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < S; q += (int64_t)gridDim.x * blockDim.x) {
+    int r = root_t[rank_max_slot(rv, (int)(uint32_t)pj[q])];
+    out[q] = r >= 0 ? to_out[r] : -1;
+  }
+}
+__global__ void k_labels_verts(int64_t N, RankView rv, const uint64_t* __restrict__ pj, const int32_t* __restrict__ root_t,
+                               const int32_t* __restrict__ to_out, int32_t* __restrict__ out) {
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < N; p += (int64_t)gridDim.x * blockDim.x) {
+    int r = root_t[rank_min_pixel(rv, (int)(uint32_t)pj[p])];
+    out[p] = r >= 0 ? to_out[r] : -1;
+  }
+}
+void launch_labels_faces(const Dims& d, const RankView& rv, const uint64_t* pj_face, const int32_t* root_t,
+                         const int32_t* to_out, int32_t* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_labels_faces, dim3(grid_for(d.S, 256, 16384)), dim3(256), 0, st, d.S, rv, pj_face, root_t, to_out,
+                     out);
+  DMSC_LAUNCH_CHECK();
+}
+void launch_labels_verts(const Dims& d, const RankView& rv, const uint64_t* pj_vert, const int32_t* root_t,
+                         const int32_t* to_out, int32_t* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_labels_verts, dim3(grid_for(d.N, 256, 16384)), dim3(256), 0, st, d.N, rv, pj_vert, root_t, to_out,
+                     out);
+  DMSC_LAUNCH_CHECK();
+}
+
+}  // namespace dmsc

@@ -1,0 +1,312 @@
+// Device-wide primitives written for wave64: exclusive scans (reduce / scan-of-sums / down-sweep),
+// a stable LSD radix sort of (uint32 key, int32 value) pairs, and asynchronous pointer jumping
+// over successor forests.
+#include "internal.hpp"
+
+namespace dmsc {
+
+constexpr int SCAN_THREADS = 256, SCAN_ITEMS = 8, SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
+
+template <class T>
+__device__ __forceinline__ T wave_inclusive_scan(T v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    T t = __shfl_up(v, o, 64);
+    if (lane >= o) v += t;
+  }
+  return v;
+}
+
+// exclusive scan of one value per thread over a 256-thread block
+template <class T>
+__device__ __forceinline__ T block_exclusive_scan(T v, T* lds4, T& total) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  T inc = wave_inclusive_scan(v);
+  if (lane == 63) lds4[wid] = inc;
+  __syncthreads();
+  T base = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < SCAN_THREADS / 64; ++w) {
+    T s = lds4[w];
+    if (w < wid) base += s;
+    tot += s;
+  }
+  __syncthreads();
+  total = tot;
+  return base + inc - v;
+}
+
+template <class T>
+struct LoadArr {
+  const T* p;
+  __device__ T operator()(int64_t i) const { return p[i]; }
+};
+struct LoadPop {
+  const uint32_t* a;
+  const uint32_t* b;
+  __device__ int32_t operator()(int64_t i) const { return __popc(a[i]) + (b ? __popc(b[i]) : 0); }
+};
+
+template <class T, class Load>
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(Load ld, int64_t n, T* __restrict__ bsums) {
+  __shared__ T lds4[4];
+  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+  T s = 0;
+#pragma unroll
+  for (int j = 0; j < SCAN_ITEMS; ++j)
+    if (base + j < n) s += ld(base + j);
+  T tot;
+  block_exclusive_scan(s, lds4, tot);
+  if (threadIdx.x == 0) bsums[blockIdx.x] = tot;
+}
+
+// single block: in-place exclusive scan of the block sums; grand total to *total
+template <class T>
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_sums(T* __restrict__ bsums, int64_t nb, T* __restrict__ total) {
+  __shared__ T lds4[4];
+  T carry = 0;
+  for (int64_t c = 0; c < nb; c += SCAN_TILE) {
+    const int64_t base = c + threadIdx.x * SCAN_ITEMS;
+    T v[SCAN_ITEMS];
+    T s = 0;
+#pragma unroll
+    for (int j = 0; j < SCAN_ITEMS; ++j) {
+      v[j] = (base + j < nb) ? bsums[base + j] : (T)0;
+      s += v[j];
+    }
+    T tot;
+    T ex = block_exclusive_scan(s, lds4, tot) + carry;
+#pragma unroll
+    for (int j = 0; j < SCAN_ITEMS; ++j) {
+      if (base + j < nb) bsums[base + j] = ex;
+      ex += v[j];
+    }
+    carry += tot;
+  }
+  if (threadIdx.x == 0 && total) *total = carry;
+}
+
+template <class T, class Load>
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(Load ld, int64_t n, const T* __restrict__ bsums,
+                                                            T* __restrict__ out) {
+  __shared__ T lds4[4];
+  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+  T v[SCAN_ITEMS];
+  T s = 0;
+#pragma unroll
+  for (int j = 0; j < SCAN_ITEMS; ++j) {
+    v[j] = (base + j < n) ? ld(base + j) : (T)0;
+    s += v[j];
+  }
+  T tot;
+  T ex = block_exclusive_scan(s, lds4, tot) + bsums[blockIdx.x];
+#pragma unroll
+  for (int j = 0; j < SCAN_ITEMS; ++j) {
+    if (base + j < n) out[base + j] = ex;
+    ex += v[j];
+  }
+}
+
+template <class T, class Load>
+static void scan_impl(Load ld, T* out, int64_t n, T* d_total, Arena& ar, int slot, hipStream_t st) {
+  const int64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+  T* bsums = ar.get<T>(slot, (size_t)(nb > 0 ? nb : 1));
+  if (n > 0) {
+    hipLaunchKernelGGL((k_scan_reduce<T, Load>), dim3((unsigned)nb), dim3(SCAN_THREADS), 0, st, ld, n, bsums);
+    DMSC_LAUNCH_CHECK();
+  }
+  hipLaunchKernelGGL(k_scan_sums<T>, dim3(1), dim3(SCAN_THREADS), 0, st, bsums, nb, d_total);
+  DMSC_LAUNCH_CHECK();
+  if (n > 0) {
+    hipLaunchKernelGGL((k_scan_down<T, Load>), dim3((unsigned)nb), dim3(SCAN_THREADS), 0, st, ld, n, bsums, out);
+    DMSC_LAUNCH_CHECK();
+  }
+}
+
+template <class T>
+void scan_exclusive(const T* in, T* out, int64_t n, T* d_total, Arena& ar, int slot, hipStream_t st) {
+  scan_impl<T>(LoadArr<T>{in}, out, n, d_total, ar, slot, st);
+}
+template void scan_exclusive<int32_t>(const int32_t*, int32_t*, int64_t, int32_t*, Arena&, int, hipStream_t);
+template void scan_exclusive<int64_t>(const int64_t*, int64_t*, int64_t, int64_t*, Arena&, int, hipStream_t);
+
+void scan_plane_popcounts(const uint32_t* plane_a, const uint32_t* plane_b, int32_t* out, int64_t n, int32_t* d_total,
+                          Arena& ar, int slot, hipStream_t st) {
+  scan_impl<int32_t>(LoadPop{plane_a, plane_b}, out, n, d_total, ar, slot, st);
+}
+
+struct LoadWiden {
+  const int32_t* p;
+  __device__ int64_t operator()(int64_t i) const { return (int64_t)p[i]; }
+};
+void scan_i32_to_i64(const int32_t* in, int64_t* out, int64_t n, int64_t* d_total, Arena& ar, int slot, hipStream_t st) {
+  scan_impl<int64_t>(LoadWiden{in}, out, n, d_total, ar, slot, st);
+}
+
+// ------------------------------------------------------------------ radix sort
+// Stable LSD sort, 8-bit digits, 4 passes. Per pass: per-tile digit histograms (digit-major), one
+// exclusive scan, then a scatter that ranks equal digits in element order: chunks of 256 in
+// order, within a chunk wave-level peer masks from 8 ballots, waves in order via LDS counts.
+constexpr int RS_THREADS = 256, RS_ITEMS = 16, RS_TILE = RS_THREADS * RS_ITEMS;
+
+__global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint32_t* __restrict__ keys, int64_t n, int shift,
+                                                        int32_t* __restrict__ counts, int64_t nb) {
+  __shared__ int32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+  for (int j = 0; j < RS_ITEMS; ++j) {
+    int64_t i = base + j * RS_THREADS + threadIdx.x;
+    if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255], 1);
+  }
+  __syncthreads();
+  counts[(int64_t)threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint32_t* __restrict__ kin, const int32_t* __restrict__ vin,
+                                                           uint32_t* __restrict__ kout, int32_t* __restrict__ vout,
+                                                           int64_t n, int shift, const int32_t* __restrict__ offs,
+                                                           int64_t nb) {
+  __shared__ int32_t base_off[256];
+  __shared__ int32_t running[256];
+  __shared__ int32_t wcount[4][256];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  base_off[tid] = offs[(int64_t)tid * nb + blockIdx.x];
+  running[tid] = 0;
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+  for (int j = 0; j < RS_ITEMS; ++j) {
+#pragma unroll
+    for (int w = 0; w < 4; ++w) wcount[w][tid] = 0;
+    __syncthreads();
+    const int64_t i = base + j * RS_THREADS + tid;
+    const bool valid = i < n;
+    uint32_t k = valid ? kin[i] : 0u;
+    int32_t v = valid ? vin[i] : 0;
+    const int d = (k >> shift) & 255;
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const bool bit = (d >> b) & 1;
+      const uint64_t bal = __ballot(bit);
+      peers &= bit ? bal : ~bal;
+    }
+    const int rank = __popcll(peers & lt);
+    if (valid && rank == 0) wcount[wid][d] = __popcll(peers);
+    __syncthreads();
+    if (valid) {
+      int prior = 0;
+      for (int w = 0; w < wid; ++w) prior += wcount[w][d];
+      const int pos = base_off[d] + running[d] + prior + rank;
+      kout[pos] = k;
+      vout[pos] = v;
+    }
+    __syncthreads();
+    running[tid] += wcount[0][tid] + wcount[1][tid] + wcount[2][tid] + wcount[3][tid];
+    __syncthreads();
+  }
+}
+
+void radix_sort_pairs(uint32_t* keys, int32_t* vals, uint32_t* keys_tmp, int32_t* vals_tmp, int64_t n, Arena& ar,
+                      int slot, hipStream_t st) {
+  if (n <= 1) return;
+  const int64_t nb = (n + RS_TILE - 1) / RS_TILE;
+  int32_t* counts = ar.get<int32_t>(slot, (size_t)(256 * nb));
+  uint32_t* ka = keys;
+  int32_t* va = vals;
+  uint32_t* kb = keys_tmp;
+  int32_t* vb = vals_tmp;
+  for (int pass = 0; pass < 4; ++pass) {
+    const int shift = 8 * pass;
+    hipLaunchKernelGGL(k_rs_hist, dim3((unsigned)nb), dim3(RS_THREADS), 0, st, ka, n, shift, counts, nb);
+    DMSC_LAUNCH_CHECK();
+    scan_exclusive<int32_t>(counts, counts, 256 * nb, nullptr, ar, slot + 1, st);
+    hipLaunchKernelGGL(k_rs_scatter, dim3((unsigned)nb), dim3(RS_THREADS), 0, st, ka, va, kb, vb, n, shift, counts, nb);
+    DMSC_LAUNCH_CHECK();
+    std::swap(ka, kb);
+    std::swap(va, vb);
+  }
+  // four passes: the result is back in (keys, vals)
+}
+
+// ------------------------------------------------------------------ pointer jumping
+// pj[i] packs (ancestor J in the low 32 bits, distance D in the high 32 bits); a root has J == i.
+// Each thread repeatedly replaces (J, D) by (J', D + D') read from pj[J] until J is a root. Reads
+// and writes are single 64-bit atomics, so every observed pair is a consistent (ancestor,
+// distance); each step moves strictly closer to the root, so every thread terminates on its own
+// without waiting for any other (no barrier, no spin).
+__global__ void k_pj_init(const int32_t* __restrict__ succ, uint64_t* __restrict__ pj, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t s = (uint32_t)succ[i];
+    pj[i] = (uint64_t)s | ((uint64_t)(s != (uint32_t)i ? 1u : 0u) << 32);
+  }
+}
+
+__global__ void k_pj(uint64_t* __restrict__ pj, int64_t n, int32_t* __restrict__ err) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    uint64_t cur = __hip_atomic_load(&pj[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t J = (uint32_t)cur, D = (uint32_t)(cur >> 32);
+    if (J == (uint32_t)i) continue;
+    for (int it = 0;; ++it) {
+      if (it >= kLoopCap) {  // only a cyclic successor array can get here
+        report(err, ERR_PJ_CAP);
+        break;
+      }
+      const uint64_t nx = __hip_atomic_load(&pj[J], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const uint32_t JJ = (uint32_t)nx;
+      if (JJ == J) break;
+      D += (uint32_t)(nx >> 32);
+      J = JJ;
+      __hip_atomic_store(&pj[i], (uint64_t)J | ((uint64_t)D << 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
+void pointer_jump_init(const int32_t* succ, uint64_t* pj, int64_t n, hipStream_t st) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_pj_init, dim3(grid_for(n, 256, 16384)), dim3(256), 0, st, succ, pj, n);
+  DMSC_LAUNCH_CHECK();
+}
+void pointer_jump(uint64_t* pj, int64_t n, int32_t* err, hipStream_t st) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_pj, dim3(grid_for(n, 256, 1 << 20)), dim3(256), 0, st, pj, n, err);
+  DMSC_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------------------------ fills
+__global__ void k_fill_i32(int32_t* p, int32_t v, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_fill_u64(uint64_t* p, uint64_t v, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_fill_u8(uint8_t* p, uint8_t v, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+void fill_u8(uint8_t* p, uint8_t v, int64_t n, hipStream_t st) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_fill_u8, dim3(grid_for(n, 256, 16384)), dim3(256), 0, st, p, v, n);
+  DMSC_LAUNCH_CHECK();
+}
+__global__ void k_iota_i32(int32_t* p, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = (int32_t)i;
+}
+void fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t st) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(n, 256, 16384)), dim3(256), 0, st, p, v, n);
+  DMSC_LAUNCH_CHECK();
+}
+void fill_u64(uint64_t* p, uint64_t v, int64_t n, hipStream_t st) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(n, 256, 16384)), dim3(256), 0, st, p, v, n);
+  DMSC_LAUNCH_CHECK();
+}
+void iota_i32(int32_t* p, int64_t n, hipStream_t st) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_iota_i32, dim3(grid_for(n, 256, 16384)), dim3(256), 0, st, p, n);
+  DMSC_LAUNCH_CHECK();
+}
+
+}  // namespace dmsc

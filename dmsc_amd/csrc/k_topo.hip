@@ -1,0 +1,296 @@
+// Persistence (elder rule with a void root, thresholded union-find, cancellation records) as
+// rounds of independent saddle applications, and the assembly of the graph outputs.
+//
+// Sequential reference: persistence.hxx:50-163 visits each side's saddles in order and merges the
+// younger of the two extremum components into the older one. Here every round:
+//   A) each pending saddle finds the two current component roots (the root of a component is its
+//      oldest extremum); equal roots => a cycle (pair -1, persistence -1). Otherwise it posts its
+//      order position into minkey[root] of both roots (64-bit atomicMin, tagged by round);
+//   B) a saddle is applied when it holds minkey of the root that dies at it. No earlier pending
+//      saddle touches that component, and the other component can only grow older before this
+//      saddle, so the dying extremum, the persistence and the thresholded merge are exactly those
+//      of the sequential sweep. Dying roots are distinct within a round.
+// The smallest pending saddle is always applicable, so every round makes progress.
+#include "internal.hpp"
+#include "k_topo.hpp"
+
+namespace dmsc {
+
+__device__ __forceinline__ int uf_find(int32_t* __restrict__ p, int x, int32_t* err = nullptr) {
+  if (x < 0) return -1;
+  for (int it = 0;; ++it) {
+    if (it >= kLoopCap) {  // only a cyclic parent array can get here
+      report(err, ERR_UF_CAP);
+      return -1;
+    }
+    const int px = p[x];
+    if (px == x) return x;
+    if (px < 0) return -1;
+    const int ppx = p[px];
+    if (ppx == px) return px;
+    p[x] = ppx;  // path halving: always an ancestor, so concurrent finds stay valid
+    if (ppx < 0) return -1;
+    x = ppx;
+  }
+}
+
+__global__ void k_persist_find(PersistSide s0, PersistSide s1, int round) {
+  const int cur = round & 1;
+  const int n0 = s0.cnt[cur], n1 = s1.cnt[cur];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    s0.cnt[cur ^ 1] = 0;
+    s1.cnt[cur ^ 1] = 0;
+  }
+  const uint64_t tag = (uint64_t)(0xFFFFFFFFu - (uint32_t)round) << 32;
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n0 + n1; j += (int64_t)gridDim.x * blockDim.x) {
+    const bool side1 = j >= n0;
+    const PersistSide& s = side1 ? s1 : s0;
+    const int k = s.wl[cur][side1 ? j - n0 : j];
+    const int i = s.ord[k];
+    const int ra = uf_find(s.par_g, s.c[2 * i], s.err);
+    const int rb = uf_find(s.par_g, s.c[2 * i + 1], s.err);
+    if (ra == rb) {
+      s.pair[i] = -1;
+      s.pers[i] = -1.0f;
+      s.rr[2 * k] = -2;
+      continue;
+    }
+    s.rr[2 * k] = ra;
+    s.rr[2 * k + 1] = rb;
+    const uint64_t key = tag | (uint32_t)k;
+    if (ra >= 0) atomicMin((unsigned long long*)&s.minkey[ra], (unsigned long long)key);
+    if (rb >= 0) atomicMin((unsigned long long*)&s.minkey[rb], (unsigned long long)key);
+  }
+}
+
+template <bool DUAL>
+__global__ void k_persist_apply(PersistSide s0, PersistSide s1, int round, float thr, uint8_t* __restrict__ removed) {
+  const int cur = round & 1;
+  const int n0 = s0.cnt[cur], n1 = s1.cnt[cur];
+  const uint64_t tag = (uint64_t)(0xFFFFFFFFu - (uint32_t)round) << 32;
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n0 + n1; j += (int64_t)gridDim.x * blockDim.x) {
+    const bool side1 = j >= n0;
+    const PersistSide& s = side1 ? s1 : s0;
+    const int k = s.wl[cur][side1 ? j - n0 : j];
+    const int ra = s.rr[2 * k];
+    if (ra == -2) continue;
+    const int rb = s.rr[2 * k + 1];
+    const int i = s.ord[k];
+    int dying, surv;
+    if (ra == -1) {
+      dying = rb;
+      surv = -1;
+    } else if (rb == -1) {
+      dying = ra;
+      surv = -1;
+    } else {
+      // min side: value_greater -> r1 dies (persistence.hxx:73-80); max side: value_less (:130-137)
+      const bool r1_dies = s.maxside ? higher<DUAL>(s.ev[rb], s.eid[rb], s.ev[ra], s.eid[ra])
+                                     : higher<DUAL>(s.ev[ra], s.eid[ra], s.ev[rb], s.eid[rb]);
+      dying = r1_dies ? ra : rb;
+      surv = r1_dies ? rb : ra;
+    }
+    const uint64_t key = tag | (uint32_t)k;
+    if (s.minkey[dying] != key) {
+      const int pos = atomicAdd(&s.cnt[cur ^ 1], 1);
+      s.wl[cur ^ 1][pos] = k;
+      continue;
+    }
+    const float sv = s.sval[i], dv = s.ev[dying];
+    float p;
+    if (s.maxside) p = DUAL ? sv - dv : dv - sv;
+    else p = DUAL ? dv - sv : sv - dv;
+    s.pers[i] = p;
+    s.pair[i] = dying;
+    s.par_g[dying] = surv;
+    if (p <= thr) {
+      s.alive[dying] = 0;
+      removed[i] = 1;
+      const int a = s.c[2 * i], b = s.c[2 * i + 1];
+      int d, t;
+      if (dying == rb) {
+        d = uf_find(s.par_t, b, s.err);
+        t = uf_find(s.par_t, a, s.err);
+      } else {
+        d = uf_find(s.par_t, a, s.err);
+        t = uf_find(s.par_t, b, s.err);
+      }
+      if (d != t && d != -1) s.par_t[d] = t;
+      if (s.cancel_d) s.cancel_d[i] = d;
+    }
+  }
+}
+
+void launch_persist_round(bool dual, const PersistSide& s0, const PersistSide& s1, int round, float thr,
+                          uint8_t* removed, int64_t max_items, hipStream_t st) {
+  const int g = grid_for(max_items, 256, 2048);
+  hipLaunchKernelGGL(k_persist_find, dim3(g), dim3(256), 0, st, s0, s1, round);
+  DMSC_LAUNCH_CHECK();
+  if (dual)
+    hipLaunchKernelGGL(k_persist_apply<true>, dim3(g), dim3(256), 0, st, s0, s1, round, thr, removed);
+  else
+    hipLaunchKernelGGL(k_persist_apply<false>, dim3(g), dim3(256), 0, st, s0, s1, round, thr, removed);
+  DMSC_LAUNCH_CHECK();
+}
+
+// root of every extremum in the thresholded union-find (reference flattening, cell_groups.hxx:128-146)
+__global__ void k_uf_flatten(int32_t* __restrict__ par, int32_t* __restrict__ root, int64_t n) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+    root[e] = uf_find(par, (int)e);
+}
+void launch_uf_flatten(int32_t* par, int32_t* root, int64_t n, hipStream_t st) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_uf_flatten, dim3(grid_for(n, 256, 16384)), dim3(256), 0, st, par, root, n);
+  DMSC_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------------------------ assembly (cpu/workspace.hxx:86-365)
+__global__ void k_u8_to_i32(const uint8_t* __restrict__ a, int32_t* __restrict__ o, int64_t n, int invert) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    o[i] = invert ? (a[i] ? 0 : 1) : (a[i] ? 1 : 0);
+}
+void launch_u8_to_i32(const uint8_t* a, int32_t* o, int64_t n, bool invert, hipStream_t st) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_u8_to_i32, dim3(grid_for(n, 256, 16384)), dim3(256), 0, st, a, o, n, invert ? 1 : 0);
+  DMSC_LAUNCH_CHECK();
+}
+
+// to_out[e] = prefix[e] if alive else -1 (in place on the prefix array)
+__global__ void k_mask_prefix(const uint8_t* __restrict__ alive, int32_t* __restrict__ pre, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    if (!alive[i]) pre[i] = -1;
+}
+void launch_mask_prefix(const uint8_t* alive, int32_t* pre, int64_t n, hipStream_t st) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_mask_prefix, dim3(grid_for(n, 256, 16384)), dim3(256), 0, st, alive, pre, n);
+  DMSC_LAUNCH_CHECK();
+}
+
+// Per order position: survivor flag (min order) and edge-row count of one side.
+__global__ void k_side_counts(int64_t ns, const int32_t* __restrict__ ord, const uint8_t* __restrict__ removed,
+                              const int32_t* __restrict__ c, const int32_t* __restrict__ root_t,
+                              const int32_t* __restrict__ to_out, int32_t* __restrict__ surv,
+                              int32_t* __restrict__ rows) {
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < ns; k += (int64_t)gridDim.x * blockDim.x) {
+    const int i = ord[k];
+    const bool alive = !removed[i];
+    if (surv) surv[k] = alive ? 1 : 0;
+    int n = 0;
+    if (alive) {
+      const int a = c[2 * i], b = c[2 * i + 1];
+      const int r1 = a >= 0 ? root_t[a] : -1, r2 = b >= 0 ? root_t[b] : -1;
+      if (r1 != -1 && to_out[r1] != -1) ++n;
+      if (r1 != r2 && r2 != -1 && to_out[r2] != -1) ++n;
+    }
+    rows[k] = n;
+  }
+}
+void launch_side_counts(int64_t ns, const int32_t* ord, const uint8_t* removed, const int32_t* c, const int32_t* root_t,
+                        const int32_t* to_out, int32_t* surv, int32_t* rows, hipStream_t st) {
+  if (ns == 0) return;
+  hipLaunchKernelGGL(k_side_counts, dim3(grid_for(ns, 256, 16384)), dim3(256), 0, st, ns, ord, removed, c, root_t,
+                     to_out, surv, rows);
+  DMSC_LAUNCH_CHECK();
+}
+
+// sad_to_out[i] for surviving saddles, from the survivor prefix over the min order
+__global__ void k_sad_to_out(int64_t ns, const int32_t* __restrict__ ord, const uint8_t* __restrict__ removed,
+                             const int32_t* __restrict__ surv_pre, int32_t* __restrict__ sad_to_out) {
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < ns; k += (int64_t)gridDim.x * blockDim.x) {
+    const int i = ord[k];
+    sad_to_out[i] = removed[i] ? -1 : surv_pre[k];
+  }
+}
+void launch_sad_to_out(int64_t ns, const int32_t* ord, const uint8_t* removed, const int32_t* surv_pre,
+                       int32_t* sad_to_out, hipStream_t st) {
+  if (ns == 0) return;
+  hipLaunchKernelGGL(k_sad_to_out, dim3(grid_for(ns, 256, 16384)), dim3(256), 0, st, ns, ord, removed, surv_pre,
+                     sad_to_out);
+  DMSC_LAUNCH_CHECK();
+}
+
+// extremum lists -> output points
+__global__ void k_emit_points(int64_t n, const int32_t* __restrict__ ids, const int32_t* __restrict__ to_out,
+                              int32_t* __restrict__ out) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int o = to_out[e];
+    if (o >= 0) out[o] = ids[e];
+  }
+}
+void launch_emit_points(int64_t n, const int32_t* ids, const int32_t* to_out, int32_t* out, hipStream_t st) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_emit_points, dim3(grid_for(n, 256, 16384)), dim3(256), 0, st, n, ids, to_out, out);
+  DMSC_LAUNCH_CHECK();
+}
+
+// One side's per-saddle outputs, visiting that side's order: persistence, paired extremum (through
+// the thresholded root), and edge rows (saddle output index, extremum output index).
+__global__ void k_emit_side(int64_t ns, const int32_t* __restrict__ ord, const uint8_t* __restrict__ removed,
+                            const int32_t* __restrict__ sad_to_out, const int32_t* __restrict__ c,
+                            const int32_t* __restrict__ root_t, const int32_t* __restrict__ to_out,
+                            const float* __restrict__ pers, const int32_t* __restrict__ pair,
+                            const int32_t* __restrict__ row_pre, const int32_t* __restrict__ sad_ids,
+                            float* __restrict__ out_p, int32_t* __restrict__ out_pp, int32_t* __restrict__ out_e,
+                            int32_t* __restrict__ out_sad) {
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < ns; k += (int64_t)gridDim.x * blockDim.x) {
+    const int i = ord[k];
+    if (removed[i]) continue;
+    const int so = sad_to_out[i];
+    if (out_sad) out_sad[so] = sad_ids[i];
+    out_p[so] = pers[i];
+    const int pe = pair[i];
+    int pp = -1;
+    if (pe >= 0) {
+      const int r = root_t[pe];
+      pp = r >= 0 ? to_out[r] : -1;
+    }
+    out_pp[so] = pp;
+    const int a = c[2 * i], b = c[2 * i + 1];
+    const int r1 = a >= 0 ? root_t[a] : -1, r2 = b >= 0 ? root_t[b] : -1;
+    int row = row_pre[k];
+    if (r1 != -1 && to_out[r1] != -1) {
+      out_e[2 * row] = so;
+      out_e[2 * row + 1] = to_out[r1];
+      ++row;
+    }
+    if (r1 != r2 && r2 != -1 && to_out[r2] != -1) {
+      out_e[2 * row] = so;
+      out_e[2 * row + 1] = to_out[r2];
+    }
+  }
+}
+void launch_emit_side(int64_t ns, const int32_t* ord, const uint8_t* removed, const int32_t* sad_to_out, const int32_t* c,
+                      const int32_t* root_t, const int32_t* to_out, const float* pers, const int32_t* pair,
+                      const int32_t* row_pre, const int32_t* sad_ids, float* out_p, int32_t* out_pp, int32_t* out_e,
+                      int32_t* out_sad, hipStream_t st) {
+  if (ns == 0) return;
+  hipLaunchKernelGGL(k_emit_side, dim3(grid_for(ns, 256, 16384)), dim3(256), 0, st, ns, ord, removed, sad_to_out, c,
+                     root_t, to_out, pers, pair, row_pre, sad_ids, out_p, out_pp, out_e, out_sad);
+  DMSC_LAUNCH_CHECK();
+}
+
+// side order arrays from the ascending (value, id) order: forward or reversed
+__global__ void k_orders(int64_t ns, const int32_t* __restrict__ asc, int32_t* __restrict__ fwd, int32_t* __restrict__ rev) {
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < ns; k += (int64_t)gridDim.x * blockDim.x) {
+    const int v = asc[k];
+    fwd[k] = v;
+    rev[ns - 1 - k] = v;
+  }
+}
+void launch_orders(int64_t ns, const int32_t* asc, int32_t* fwd, int32_t* rev, hipStream_t st) {
+  if (ns == 0) return;
+  hipLaunchKernelGGL(k_orders, dim3(grid_for(ns, 256, 16384)), dim3(256), 0, st, ns, asc, fwd, rev);
+  DMSC_LAUNCH_CHECK();
+}
+
+__global__ void k_sort_keys(int64_t n, const float* __restrict__ v, uint32_t* __restrict__ keys) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    keys[i] = sortable_key(v[i]);
+}
+void launch_sort_keys(int64_t n, const float* v, uint32_t* keys, hipStream_t st) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_sort_keys, dim3(grid_for(n, 256, 16384)), dim3(256), 0, st, n, v, keys);
+  DMSC_LAUNCH_CHECK();
+}
+
+}  // namespace dmsc

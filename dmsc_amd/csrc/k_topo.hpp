@@ -1,0 +1,110 @@
+// Launchers for persistence, assembly and arc geometry stages.
+#pragma once
+
+#include "internal.hpp"
+
+namespace dmsc {
+
+// One side (vertex-minima or face-maxima) of the persistence sweep.
+struct PersistSide {
+  const int32_t* ord;   // order position -> saddle index (the side's sweep order)
+  const int32_t* c;     // per saddle: c[2i], c[2i+1] = extremum indices of its two V-paths (-1 = none)
+  const float* sval;    // per saddle value
+  const float* ev;      // per extremum value
+  const int32_t* eid;   // per extremum cell id
+  int32_t* par_g;       // global union-find parents (-1 = joined the void)
+  int32_t* par_t;       // thresholded union-find parents
+  uint8_t* alive;       // per extremum
+  uint64_t* minkey;     // per extremum: (round tag << 32 | order position), atomic min
+  int32_t* pair;        // per saddle: dying extremum index, -1 for a cycle
+  float* pers;          // per saddle persistence (-1 for a cycle)
+  int32_t* cancel_d;    // per saddle thresholded dead extremum (null when arcs are not traced)
+  int32_t* rr;          // per order position: the two roots seen this round (-2 = cycle)
+  int32_t* wl[2];       // ping-pong worklists of order positions
+  int32_t* cnt;         // [2] worklist sizes (device)
+  int32_t* err;         // device error flags
+  int maxside;
+};
+
+void launch_persist_round(bool dual, const PersistSide& s0, const PersistSide& s1, int round, float thr,
+                          uint8_t* removed, int64_t max_items, hipStream_t st);
+void launch_uf_flatten(int32_t* par, int32_t* root, int64_t n, hipStream_t st);
+void launch_u8_to_i32(const uint8_t* a, int32_t* o, int64_t n, bool invert, hipStream_t st);
+void launch_mask_prefix(const uint8_t* alive, int32_t* pre, int64_t n, hipStream_t st);
+void launch_side_counts(int64_t ns, const int32_t* ord, const uint8_t* removed, const int32_t* c, const int32_t* root_t,
+                        const int32_t* to_out, int32_t* surv, int32_t* rows, hipStream_t st);
+void launch_sad_to_out(int64_t ns, const int32_t* ord, const uint8_t* removed, const int32_t* surv_pre,
+                       int32_t* sad_to_out, hipStream_t st);
+void launch_emit_points(int64_t n, const int32_t* ids, const int32_t* to_out, int32_t* out, hipStream_t st);
+void launch_emit_side(int64_t ns, const int32_t* ord, const uint8_t* removed, const int32_t* sad_to_out, const int32_t* c,
+                      const int32_t* root_t, const int32_t* to_out, const float* pers, const int32_t* pair,
+                      const int32_t* row_pre, const int32_t* sad_ids, float* out_p, int32_t* out_pp, int32_t* out_e,
+                      int32_t* out_sad, hipStream_t st);
+void launch_orders(int64_t ns, const int32_t* asc, int32_t* fwd, int32_t* rev, hipStream_t st);
+void launch_sort_keys(int64_t n, const float* v, uint32_t* keys, hipStream_t st);
+
+// ------------------------------------------------------------------ arc geometry (k_arcs.hip)
+// One traced side of the arc geometry (faces for ridges/"max", vertices for valleys/"min").
+struct ArcSide {
+  int faces;               // 1: face V-paths (slot-indexed successors), 0: vertex V-paths (pixel-indexed)
+  const int32_t* succ;     // successor array of the forest
+  int Nx, W, H;            // geometry for cell-id encoding
+  int64_t ns;              // saddles
+  int64_t next;            // extrema of this side
+  const int32_t* sad_ids;
+  const int32_t* tinit;    // per arc (2s+k): initial target extremum, -1 = no arc
+  const int32_t* blen;     // per arc: raw length in cells (steps + 1), 0 = no arc
+  const int64_t* boff;     // per arc: offset in the raw arena
+  int32_t* arena;          // raw arcs, cell ids from the saddle outwards
+};
+
+void launch_arc_lengths(int64_t n2, const int32_t* c, const int32_t* steps, int32_t* blen, hipStream_t st);
+void launch_raw_arcs(const ArcSide& a, hipStream_t st);
+
+struct GeoState {
+  // cancellations sorted by (persistence, saddle id): position = key
+  int64_t ncancel;
+  const int32_t* cs;       // cancel -> saddle index
+  const int32_t* cd;       // cancel -> dead extremum
+  const int32_t* deadc;    // extremum -> cancel index whose dead it is (-1)
+  int32_t* gparent;        // geometry union-find (-1 = void)
+  int32_t* linklen;        // length of the link path of a linked extremum (saturating int32)
+  int32_t* linkc;          // cancel that linked the extremum
+  int32_t* linkkd;         // which arc (0/1) of that cancel's saddle led to the dead extremum
+  int32_t* done;           // per cancel: round processed (INT_MAX while pending)
+  uint64_t* watch;         // per extremum: (tag << 32 | cancel) atomic min
+  int32_t* rr;             // per cancel: R0, R1, len0, len1
+  int32_t* wl[2];
+  int32_t* cnt;
+  int32_t* err;
+};
+void launch_geo_round(const ArcSide& a, const GeoState& g, int round, int64_t max_items, hipStream_t st);
+void launch_cancel_flags(int64_t ns, const int32_t* cancel_d, int32_t* flag, hipStream_t st);
+void launch_cancel_compact(int64_t ns, const int32_t* cancel_d, const float* pers, const int32_t* pre,
+                           int32_t* cs, int32_t* cd, uint32_t* keys, hipStream_t st);
+void launch_cancel_finish(int64_t nc, const int32_t* sorted_cancel_slot, const int32_t* cs_in, const int32_t* cd_in,
+                          int32_t* cs, int32_t* cd, int32_t* deadc, hipStream_t st);
+void launch_geo_q(const GeoState& g, int64_t next, uint64_t* pjq, hipStream_t st);
+
+// Output arc lengths for surviving saddles in min order; then segment expansion.
+void launch_final_lengths(int64_t ns, const int32_t* ord, const uint8_t* removed, const ArcSide& a,
+                          const uint64_t* pjq, int64_t next, int64_t* flen /*2 per position*/,
+                          int64_t* per_pos /*per position total*/, hipStream_t st);
+struct Seg {
+  int32_t kind;  // 0 RAW (a = arc index), 1 CHAIN (a = from extremum, b = up-to ancestor, -1 = void), 2 LINK (a = dead)
+  int32_t a, b;
+  int32_t dir;   // 1 forward, 0 reversed
+  int64_t off;   // output offset
+};
+void launch_seed_segments(int64_t ns, const int32_t* ord, const uint8_t* removed, const ArcSide& a,
+                          const uint64_t* pjq, int64_t next, const int64_t* flen, const int64_t* pos_pre,
+                          const int32_t* sad_to_out, Seg* segs, int32_t* nseg, Seg* raw, int32_t* nraw, int64_t cap,
+                          int32_t* err, int32_t* out_offsets, int32_t* out_arc_offsets, int64_t n_out,
+                          const int64_t* d_total, hipStream_t st);
+void launch_expand_segments(const ArcSide& a, const GeoState& g, const uint64_t* pjq, const Seg* in, const int32_t* nin,
+                            Seg* out, int32_t* nout, Seg* raw, int32_t* nraw, int64_t max_in, int64_t cap,
+                            hipStream_t st);
+void launch_copy_raw(const ArcSide& a, const Seg* raw, const int32_t* nraw, int64_t max_raw, int32_t* out,
+                     int64_t out_total, int32_t* err, hipStream_t st);
+
+}  // namespace dmsc

@@ -1,0 +1,100 @@
+/* C-ABI of the MI355X (gfx950) compute_dmsc implementation.
+ *
+ * Plain pointers and sizes only; no torch types. Device buffers are HIP device pointers on the
+ * context's device; `stream` is the HIP stream the work is ordered on (NULL = default stream).
+ *
+ * Output slots follow the reference DMSComplex order (reference dmsc/csrc/dmsc_gpu.cpp:202-247):
+ *   0 shape, 1 max_pts, 2 min_pts, 3 sad_pts, 4 e_max, 5 e_min, 6 p_max, 7 p_min, 8 ppairs_max,
+ *   9 ppairs_min, 10 grad_indices, 11 peaks, 12 ridges, 13 ridge_offsets, 14 ridge_arc_offsets,
+ *   15 basins, 16 valleys, 17 valley_offsets, 18 valley_arc_offsets.
+ * All int32 except p_max/p_min (float32). Outputs are allocated through the caller's allocator
+ * callback, so the caller owns them (the reference returns freshly allocated tensors,
+ * dmsc_gpu.cpp:141-158); slot 0 (shape) is returned by value in `shape`.
+ */
+#ifndef DMSC_HIP_H
+#define DMSC_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t* dmsc_stream_t; /* == hipStream_t */
+
+typedef enum {
+  DMSC_OK = 0,
+  DMSC_ERR_INVALID_ARGUMENT = 1, /* reference: TORCH_CHECK in input_validation.hxx:9-26 */
+  DMSC_ERR_TOO_LARGE = 2,        /* 32-bit cell ids, input_validation.hxx:24-26; or int32 arc offsets */
+  DMSC_ERR_HIP = 3,              /* a HIP runtime call or kernel launch failed */
+  DMSC_ERR_ALLOC = 4,            /* device scratch or the caller's allocator failed */
+  DMSC_ERR_INTERNAL = 5          /* an internal consistency check failed */
+} dmsc_status;
+
+/* Flags, with the meaning of the reference extract_dmsc arguments (dmsc_gpu.cpp:128-130). The
+ * trace flags are given in primal terms; the dual swap (dmsc_gpu.cpp:71-74) happens inside. */
+enum {
+  DMSC_RETURN_GRADIENT = 1u << 0,
+  DMSC_DUAL = 1u << 1,
+  DMSC_TRACE_MAX_ARCS = 1u << 2,   /* ridges   */
+  DMSC_TRACE_MIN_ARCS = 1u << 3,   /* valleys  */
+  DMSC_TRACE_MAX_GROUPS = 1u << 4, /* peaks    */
+  DMSC_TRACE_MIN_GROUPS = 1u << 5  /* basins   */
+};
+
+enum { DMSC_NUM_OUTPUTS = 19 };
+enum { DMSC_DTYPE_INT32 = 0, DMSC_DTYPE_FLOAT32 = 1 };
+
+/* Allocator callback: return device memory for output `slot` with the given shape and dtype
+ * (ndim 1 or 2, row-major). Called once per slot 1..18 per field, possibly with zero elements
+ * (the pointer is then never dereferenced). Return NULL on failure. */
+typedef void* (*dmsc_alloc_fn)(void* user, int32_t slot, int32_t ndim, const int64_t* dims, int32_t dtype);
+
+typedef struct {
+  void* data[DMSC_NUM_OUTPUTS]; /* device pointers from the allocator (data[0] unused) */
+  int64_t dims[DMSC_NUM_OUTPUTS][2];
+  int32_t ndim[DMSC_NUM_OUTPUTS];
+  int32_t dtype[DMSC_NUM_OUTPUTS];
+  int32_t shape[2]; /* slot 0: {H, W} */
+} dmsc_outputs;
+
+typedef struct dmsc_context dmsc_context;
+
+/* Creates a context bound to HIP device `device`; it owns reusable device scratch (grow-only).
+ * Replaces the per-call gpu::Workspace of the reference (dmsc_gpu.cpp:148, gpu/workspace.hxx). */
+dmsc_status dmsc_context_create(int device, dmsc_context** out);
+void dmsc_context_destroy(dmsc_context* ctx);
+
+/* Full pipeline on one contiguous H x W float32 field resident on the device.
+ * Replaces extract_single_dmsc_gpu_t (reference dmsc_gpu.cpp:53-82) with the whole pipeline on
+ * the GPU (gradient, critical cells, V-path forests, persistence, arc geometry, labels, assembly).
+ * Synchronous: the outputs are complete when it returns (as in the reference). Calls on one
+ * context must not overlap. */
+dmsc_status dmsc_extract(dmsc_context* ctx, const float* d_field, int32_t H, int32_t W, float persistence_threshold,
+                         uint32_t flags, dmsc_stream_t stream, dmsc_alloc_fn alloc, void* alloc_user,
+                         dmsc_outputs* out);
+
+/* Batch of fields of independent sizes and thresholds (fields[b] is a contiguous H[b] x W[b]
+ * device array). Generalises extract_batched_dmsc_gpu (reference dmsc_gpu.cpp:84-126), which
+ * requires one shape and one threshold. out must hold B entries. */
+dmsc_status dmsc_extract_batch(dmsc_context* ctx, int32_t B, const float* const* d_fields, const int32_t* H,
+                               const int32_t* W, const float* persistence_thresholds, uint32_t flags,
+                               dmsc_stream_t stream, dmsc_alloc_fn alloc, void* alloc_user, dmsc_outputs* out);
+
+/* Message for the last failing call on this context (valid until the next call). */
+const char* dmsc_last_error(dmsc_context* ctx);
+
+/* Copies an internal intermediate of the last dmsc_extract call into host memory, for stage-by-
+ * stage parity tests. No reference counterpart. Returns the element count (int32 unless stated),
+ * or -1 for an unknown name; copies min(count, capacity) elements. Names: "counts" (n_min, n_sad,
+ * n_max), "min_ids", "sad_ids", "max_ids", "face_term", "face_depth", "vert_term", "vert_depth",
+ * "asc_order", "sad_cmax", "sad_cmin", "pers_min" (float), "pers_max" (float), "pair_min",
+ * "pair_max", "rounds" (persistence rounds, geometry rounds). */
+int64_t dmsc_debug_fetch(dmsc_context* ctx, const char* name, void* host_dst, int64_t capacity);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DMSC_HIP_H */
