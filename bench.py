@@ -1,0 +1,208 @@
+"""Benchmark of the MI355X compute_dmsc path.
+
+Default workload (one GPU): a 4096x4096 float32 Gaussian-mixture field (256 bumps, sigma 16-256 px,
+noise 0.02, seeded), full DMSC (gradient returned, ridges, valleys, peaks, basins) for primal and
+dual at persistence thresholds {0, 0.05, 0.2}. One step = those 6 extractions, each on its own
+device copy of the field (no cache reuse between extractions). Metric: Mpix/s = pixels processed /
+wall time of the timed steps.
+
+Prints one JSON line with the metric, the roofline object of the dominant kernel (algorithmic
+bytes / average launch time from HIP events recorded around its launches during the timed steps),
+the end-to-end algorithmic HBM rate, and the CPU oracle timed on a bounded sample of the same
+workload on this host.
+
+Other workloads (--workload): noise8192 (8192^2 uniform noise quantised to 256 levels, thr 0.05),
+mixed (512 fields of sizes 64..1024 with per-field thresholds, looped per field).
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic bytes per launch of each instrumented kernel, per unit it processes:
+#   gradient:      4 B field read per pixel + 16 B gradient write per slot
+#   pointer_jump:  8 B (terminal, depth) read + 8 B written per forest node
+#   labels_*:      8 B terminal read + 4 B label written per cell
+KERNEL_MODELS = {
+    "gradient": lambda H, W: 4 * H * W + 16 * (H + 1) * (W + 1),
+}
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=5)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--workload", default="gauss4096", choices=["gauss4096", "noise8192", "mixed"])
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    p.add_argument("--profile", action="store_true", help="print per-kernel event timings to stderr")
+    return p.parse_args()
+
+
+def workload(name):
+    from dmsc_amd import synthetic
+
+    if name == "gauss4096":
+        f = synthetic.gaussian_mixture(4096, 4096, 256, (16.0, 256.0), 0.02, seed=4096)
+        runs = [(d, t) for d in (False, True) for t in (0.0, 0.05, 0.2)]
+        desc = ("gaussian-mixture 4096x4096 fp32, 256 bumps, sigma 16-256 px, noise 0.02, seed 4096; primal+dual x "
+                "thr {0, 0.05, 0.2}; full geometry + gradient")
+        return [f], runs, desc
+    if name == "noise8192":
+        f = synthetic.quantized_noise(8192, 8192, 256, seed=8192)
+        desc = "uniform noise 8192x8192 quantised to 256 levels, seed 8192; primal thr 0.05; full geometry + gradient"
+        return [f], [(False, 0.05)], desc
+    fields, thrs = synthetic.mixed_batch(512, seed=512)
+    desc = "512 gaussian-mixture fields, H,W in {64..1024 step 32}, 8-64 bumps, thr U[0.02,0.2], seed 512"
+    return fields, list(zip([False] * len(fields), thrs)), desc
+
+
+def out_bytes(ms):
+    return sum(t.numel() * t.element_size() for t in ms)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    import dmsc_amd
+    from dmsc_amd.csrc import dmsc_gpu
+
+    dev = torch.device("cuda:0")
+    fields, runs, desc = workload(args.workload)
+    mixed = args.workload == "mixed"
+    n_ext = len(runs)  # extractions per step
+
+    def field_of(j):
+        return fields[j] if mixed else fields[0]
+
+    # distinct device copies per extraction and step (no cache reuse across extractions)
+    total_steps = args.warmup + args.steps
+    copies = [[field_of(j).to(dev) if (mixed or s == 0) else None for j in range(n_ext)] for s in range(total_steps)]
+    if not mixed:
+        for s in range(total_steps):
+            copies[s] = [fields[0].to(dev) for _ in range(n_ext)]
+    torch.cuda.synchronize()
+
+    def step(s, record=None, plain=False):
+        if mixed and not plain:  # one batched call over all fields (per-field thresholds)
+            res = dmsc_amd.compute_dmsc_batch(copies[s], [t for _, t in runs], return_gradient=True)
+            if record is not None:
+                record.extend((copies[s][j].shape, out_bytes(ms)) for j, ms in enumerate(res))
+            return
+        for j, (dual, thr) in enumerate(runs):
+            ms = dmsc_amd.compute_dmsc(copies[s][j], thr, return_gradient=True, is_dual=dual)
+            if record is not None:
+                record.append((copies[s][j].shape, out_bytes(ms)))
+
+    for s in range(args.warmup):
+        step(s)
+    torch.cuda.synchronize()
+
+    dmsc_gpu.profile_enable(0, True)
+    sizes = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(args.warmup, total_steps):
+        step(s, sizes if s == args.warmup else None)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    prof = dmsc_gpu.profile_read(0)
+    dmsc_gpu.profile_enable(0, False)
+
+    elapsed = t1 - t0
+    px_step = sum(int(sh[0]) * int(sh[1]) for sh, _ in sizes)
+    value = px_step * args.steps / elapsed / 1e6
+    b_alg_step = sum(4 * int(sh[0]) * int(sh[1]) + ob for sh, ob in sizes)
+    e2e_gbs = b_alg_step * args.steps / elapsed / 1e9
+
+    # dominant kernel (largest total event time) and its roofline
+    ranked = sorted(prof.items(), key=lambda kv: -kv[1][0])
+    if args.profile:
+        for k, (ms_tot, n) in ranked:
+            print(f"  {k:20s} total {ms_tot:9.3f} ms  launches {n:6d}  avg {ms_tot / max(n, 1) * 1e3:9.1f} us",
+                  file=sys.stderr)
+    dom = ranked[0][0] if ranked else None
+    roof = None
+    grad_ms, grad_n = prof.get("gradient", (0.0, 0))
+    if grad_n:
+        # bytes per gradient launch: the field of that launch plus its gradient output
+        per_launch = sum(KERNEL_MODELS["gradient"](int(sh[0]), int(sh[1])) for sh, _ in sizes) / len(sizes)
+        avg_s = grad_ms / grad_n / 1e3
+        ach = per_launch / avg_s / 1e9
+        roof = {"bound": "hbm", "kernel": "gradient", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "avg_launch_us": round(avg_s * 1e6, 2),
+                "bytes_per_launch": int(per_launch), "dominant_kernel": dom,
+                "dominant_share": round(ranked[0][1][0] / max(sum(v[0] for v in prof.values()), 1e-9), 3)}
+
+    extra = {}
+    if mixed:  # the same fields through the plain per-field path, for the batched-vs-plain ratio
+        torch.cuda.synchronize()
+        tp = time.perf_counter()
+        step(args.warmup, plain=True)
+        torch.cuda.synchronize()
+        plain = px_step / (time.perf_counter() - tp) / 1e6
+        extra = {"plain_loop_Mpix_s": round(plain, 2), "batched_over_plain": round(value / plain, 2)}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(fields, runs, mixed)
+
+    line = {
+        "metric": "Mpix/s full DMSC (1 GPU) + achieved HBM GB/s, bit-exact",
+        "value": round(value, 2),
+        "unit": "Mpix/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": desc, "extractions_per_step": n_ext, "pixels_per_step": px_step, **extra},
+        "end_to_end": {"algorithmic_bytes_per_step": b_alg_step, "achieved_GBs": round(e2e_gbs, 1),
+                       "frac": round(e2e_gbs / HBM_PEAK_GBS, 4)},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+def cpu_baseline(fields, runs, mixed):
+    """The oracle (C++ restatement of the reference CPU path) on a bounded sample of the workload."""
+    import numpy as np
+
+    import oracle
+
+    cores = min(os.cpu_count() or 1, 16)
+    if mixed:
+        sample = [(fields[j].numpy(), runs[j][1], runs[j][0]) for j in range(16)]
+        what = "first 16 fields of the mixed batch, primal, their thresholds"
+    else:
+        sample = [(fields[0].numpy(), runs[1][1] if len(runs) > 1 else runs[0][1], False)]
+        what = f"one primal extraction of the {fields[0].shape[0]}x{fields[0].shape[1]} field at thr {sample[0][1]}"
+    px = 0
+    t0 = time.perf_counter()
+    for f, thr, dual in sample:
+        oracle.extract(np.ascontiguousarray(f), thr, True, dual, True, True, True, True, nthreads=cores)
+        px += f.shape[0] * f.shape[1]
+    dt = time.perf_counter() - t0
+    return {"value": round(px / dt / 1e6, 3), "unit": "Mpix/s", "cores": cores, "kind": "port",
+            "sample": what + f" ({dt:.1f} s)"}
+
+
+if __name__ == "__main__":
+    main()

@@ -31,15 +31,23 @@ const char* kNames[DMSC_NUM_OUTPUTS] = {
 
 struct AllocUser {
   torch::Device dev;
-  DMSComplex* res;
+  DMSComplex* res;  // one per field; the batch entry fills them from several threads (distinct fields)
 };
 
-void* alloc_cb(void* user, int32_t slot, int32_t ndim, const int64_t* dims, int32_t dtype) {
+void* alloc_cb(void* user, int32_t field, int32_t slot, int32_t ndim, const int64_t* dims, int32_t dtype) {
   auto* a = static_cast<AllocUser*>(user);
   auto opts = torch::TensorOptions().dtype(dtype == DMSC_DTYPE_FLOAT32 ? torch::kFloat32 : torch::kInt32).device(a->dev);
   std::vector<int64_t> shape(dims, dims + ndim);
-  a->res->t[slot] = torch::empty(shape, opts);
-  return a->res->t[slot].data_ptr();
+  torch::Tensor& t = a->res[field].t[slot];
+  t = torch::empty(shape, opts);
+  return t.data_ptr();
+}
+
+torch::Tensor shape_tensor(const dmsc_outputs& out) {
+  torch::Tensor s = torch::empty({2}, torch::TensorOptions().dtype(torch::kInt32));
+  s.data_ptr<int32_t>()[0] = out.shape[0];
+  s.data_ptr<int32_t>()[1] = out.shape[1];
+  return s;
 }
 
 dmsc_context* context_for(int device) {
@@ -83,10 +91,33 @@ DMSComplex run_field(dmsc_context* ctx, const torch::Tensor& f, float thr, uint3
                       (dmsc_stream_t)stream, alloc_cb, &u, &out);
   }
   TORCH_CHECK(st == DMSC_OK, dmsc_last_error(ctx));
-  r.t[0] = torch::empty({2}, torch::TensorOptions().dtype(torch::kInt32));
-  r.t[0][0] = out.shape[0];
-  r.t[0][1] = out.shape[1];
+  r.t[0] = shape_tensor(out);
   return r;
+}
+
+// Fields of one device processed by the C-ABI batch entry (concurrent worker streams).
+std::vector<DMSComplex> run_batch(dmsc_context* ctx, const std::vector<torch::Tensor>& fields,
+                                  const std::vector<float>& thr, uint32_t flags, hipStream_t stream) {
+  const int B = (int)fields.size();
+  std::vector<DMSComplex> res(B);
+  std::vector<const float*> ptrs(B);
+  std::vector<int32_t> hs(B), ws(B);
+  for (int b = 0; b < B; ++b) {
+    ptrs[b] = fields[b].data_ptr<float>();
+    hs[b] = (int32_t)fields[b].size(0);
+    ws[b] = (int32_t)fields[b].size(1);
+  }
+  AllocUser u{fields[0].device(), res.data()};
+  std::vector<dmsc_outputs> outs(B);
+  dmsc_status st;
+  {
+    pybind11::gil_scoped_release release;
+    st = dmsc_extract_batch(ctx, B, ptrs.data(), hs.data(), ws.data(), thr.data(), flags, (dmsc_stream_t)stream,
+                            alloc_cb, &u, outs.data());
+  }
+  TORCH_CHECK(st == DMSC_OK, dmsc_last_error(ctx));
+  for (int b = 0; b < B; ++b) res[b].t[0] = shape_tensor(outs[b]);
+  return res;
 }
 
 pybind11::object extract_dmsc(torch::Tensor scalar_field, float persistence_threshold, bool return_gradient,
@@ -102,11 +133,10 @@ pybind11::object extract_dmsc(torch::Tensor scalar_field, float persistence_thre
   const uint32_t flags = make_flags(return_gradient, is_dual, trace_max_arcs, trace_min_arcs, trace_max_groups,
                                     trace_min_groups);
   if (scalar_field.dim() == 2) return pybind11::cast(run_field(ctx, scalar_field, persistence_threshold, flags, stream));
-  std::vector<DMSComplex> res;
-  res.reserve(scalar_field.size(0));
-  for (int64_t b = 0; b < scalar_field.size(0); ++b)
-    res.push_back(run_field(ctx, scalar_field[b], persistence_threshold, flags, stream));
-  return pybind11::cast(res);
+  std::vector<torch::Tensor> fields;
+  for (int64_t b = 0; b < scalar_field.size(0); ++b) fields.push_back(scalar_field[b]);
+  std::vector<float> thr(fields.size(), persistence_threshold);
+  return pybind11::cast(run_batch(ctx, fields, thr, flags, stream));
 }
 
 // Mixed-size batch: a list of 2-D fields with one threshold each (no reference counterpart; the
@@ -128,11 +158,10 @@ std::vector<DMSComplex> extract_dmsc_list(std::vector<torch::Tensor> fields, std
   hipStream_t stream = c10::hip::getCurrentHIPStream(dev.index()).stream();
   const uint32_t flags = make_flags(return_gradient, is_dual, trace_max_arcs, trace_min_arcs, trace_max_groups,
                                     trace_min_groups);
-  std::vector<DMSComplex> res;
-  res.reserve(fields.size());
-  for (size_t b = 0; b < fields.size(); ++b)
-    res.push_back(run_field(ctx, fields[b].contiguous(), (float)thresholds[b], flags, stream));
-  return res;
+  std::vector<torch::Tensor> contig;
+  for (auto& f : fields) contig.push_back(f.contiguous());
+  std::vector<float> thr(thresholds.begin(), thresholds.end());
+  return run_batch(ctx, contig, thr, flags, stream);
 }
 
 pybind11::object debug_fetch(int device, const std::string& name, int64_t capacity, bool as_float, bool as_u64) {
@@ -144,9 +173,25 @@ pybind11::object debug_fetch(int device, const std::string& name, int64_t capaci
   return pybind11::cast(t.slice(0, 0, std::min(n, capacity)));
 }
 
+void profile_enable(int device, bool on) { dmsc_profile_enable(context_for(device), on ? 1 : 0); }
+
+pybind11::dict profile_read(int device) {
+  dmsc_context* ctx = context_for(device);
+  pybind11::dict d;
+  for (int32_t i = 0; i < dmsc_profile_count(ctx); ++i) {
+    const char* name = nullptr;
+    double ms = 0;
+    int64_t n = 0;
+    if (dmsc_profile_entry(ctx, i, &name, &ms, &n) == DMSC_OK) d[name] = pybind11::make_tuple(ms, n);
+  }
+  return d;
+}
+
 }  // namespace
 
 PYBIND11_MODULE(dmsc_gpu, m) {
+  m.def("profile_enable", &profile_enable, pybind11::arg("device"), pybind11::arg("on"));
+  m.def("profile_read", &profile_read, pybind11::arg("device"));
   auto cls = pybind11::class_<DMSComplex>(m, "DMSComplex", pybind11::module_local(),
                                           "Discrete Morse-Smale complex (19 tensors, reference slot order)");
   for (int i = 0; i < DMSC_NUM_OUTPUTS; ++i)

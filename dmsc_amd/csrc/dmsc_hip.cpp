@@ -7,11 +7,17 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <climits>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
+#include <memory>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <utility>
+#include <vector>
 
 #include "dmsc_hip.h"
 #include "k_topo.hpp"
@@ -28,7 +34,7 @@ enum : int {
   A_C32, A_C64,
   A_MINIDS, A_MINVAL, A_SADIDS, A_SADVAL, A_MAXIDS, A_MAXVAL,
   A_CMAX, A_CMIN, A_LMAX, A_LMIN,
-  A_KEYS, A_KEYS2, A_VALS2, A_ASC, A_ORDMIN, A_ORDMAX, A_REMOVED,
+  A_KEYS, A_KEYS2, A_VALS2, A_ASC, A_ORDMIN, A_ORDMAX, A_REMOVED, A_POSMIN, A_POSMAX,
   A_SIDE0,  // per persistence side: 13 slots each
   A_SIDE_END = A_SIDE0 + 2 * 13,
   A_ASM0 = A_SIDE_END,  // assembly: surv, surv_pre, rows x2, rows_pre x2, sad_to_out, to_out x2
@@ -46,10 +52,15 @@ enum ArcSlot2 { H_PJQ = 0, H_FLEN, H_PERPOS, H_POSPRE, H_SEGA, H_SEGB, H_RAW, H_
 
 }  // namespace dmsc
 
+namespace dmsc {
+thread_local KernelTimes* tl_times = nullptr;
+}
+
 using namespace dmsc;
 
 struct dmsc_context {
   int device = 0;
+  KernelTimes times;
   Arena ar;
   Arena ar2;  // arc emission scratch
   int64_t* pinned = nullptr;
@@ -58,12 +69,24 @@ struct dmsc_context {
   Dims dims{};
   int64_t nmin = 0, nsad = 0, nmax = 0;
   int rounds_persist = 0, rounds_geo[2] = {0, 0};
+  std::vector<int32_t> pending_log;  // (round, pending min side, pending max side) at each readback
   bool dual = false;
   const int32_t *min_ids = nullptr, *sad_ids = nullptr, *max_ids = nullptr, *asc = nullptr;
   const int32_t *cmax = nullptr, *cmin = nullptr, *pair_min = nullptr, *pair_max = nullptr;
   const float *pers_min = nullptr, *pers_max = nullptr;
   const uint64_t *pjf = nullptr, *pjv = nullptr;
+  // batch workers (own scratch + stream each)
+  std::vector<std::unique_ptr<dmsc_context>> workers;
+  hipStream_t stream = nullptr;
 };
+
+namespace {
+int batch_workers() {
+  const char* e = std::getenv("DMSC_BATCH_WORKERS");
+  int n = e ? std::atoi(e) : 8;
+  return n < 1 ? 1 : (n > 64 ? 64 : n);
+}
+}  // namespace
 
 namespace {
 
@@ -79,14 +102,14 @@ inline int slot_pp(bool faces, bool dual) { return (faces != dual) ? 8 : 9; }
 inline int slot_labels(bool faces, bool dual) { return (faces != dual) ? 11 : 15; }
 inline int slot_arcs(bool faces, bool dual) { return (faces != dual) ? 12 : 16; }
 
-void* alloc_out(dmsc_alloc_fn alloc, void* user, dmsc_outputs* out, int slot, int ndim, int64_t d0, int64_t d1,
+void* alloc_out(dmsc_alloc_fn alloc, void* user, int field, dmsc_outputs* out, int slot, int ndim, int64_t d0, int64_t d1,
                 int dtype) {
   int64_t dims[2] = {d0, d1};
   out->ndim[slot] = ndim;
   out->dims[slot][0] = d0;
   out->dims[slot][1] = ndim == 2 ? d1 : 0;
   out->dtype[slot] = dtype;
-  void* p = alloc(user, slot, ndim, dims, dtype);
+  void* p = alloc(user, field, slot, ndim, dims, dtype);
   int64_t n = ndim == 2 ? d0 * d1 : d0;
   if (!p && n > 0) throw Fail{DMSC_ERR_ALLOC, "output allocator returned NULL for slot " + std::to_string(slot)};
   out->data[slot] = p;
@@ -139,7 +162,7 @@ struct ArcWork {
 };
 
 void run_one(dmsc_context* ctx, const float* f, int H, int W, float thr, uint32_t flags, hipStream_t st,
-             dmsc_alloc_fn alloc, void* user, dmsc_outputs* out) {
+             dmsc_alloc_fn alloc, void* user, int field, dmsc_outputs* out) {
   if (!f) throw Fail{DMSC_ERR_INVALID_ARGUMENT, "Input tensor must be defined"};
   if (H <= 0 || W <= 0)
     throw Fail{DMSC_ERR_INVALID_ARGUMENT,
@@ -163,10 +186,10 @@ void run_one(dmsc_context* ctx, const float* f, int H, int W, float thr, uint32_
   out->dims[0][0] = 2;
 
   // outputs whose sizes are known up front
-  int32_t* o_grad = (int32_t*)alloc_out(alloc, user, out, 10, 1, grad ? 4 * d.S : 0, 0, DMSC_DTYPE_INT32);
-  int32_t* o_flab = (int32_t*)alloc_out(alloc, user, out, slot_labels(true, dual), 2, gmax ? d.Ny : 0,
+  int32_t* o_grad = (int32_t*)alloc_out(alloc, user, field, out,10, 1, grad ? 4 * d.S : 0, 0, DMSC_DTYPE_INT32);
+  int32_t* o_flab = (int32_t*)alloc_out(alloc, user, field, out,slot_labels(true, dual), 2, gmax ? d.Ny : 0,
                                         gmax ? d.Nx : 0, DMSC_DTYPE_INT32);
-  int32_t* o_vlab = (int32_t*)alloc_out(alloc, user, out, slot_labels(false, dual), 2, gmin ? H : 0, gmin ? W : 0,
+  int32_t* o_vlab = (int32_t*)alloc_out(alloc, user, field, out,slot_labels(false, dual), 2, gmin ? H : 0, gmin ? W : 0,
                                         DMSC_DTYPE_INT32);
 
   // ---- gradient, critical bit planes, successor forests
@@ -223,8 +246,10 @@ void run_one(dmsc_context* ctx, const float* f, int H, int W, float thr, uint32_
   int32_t* ord_min = ar.get<int32_t>(A_ORDMIN, nsad);
   int32_t* ord_max = ar.get<int32_t>(A_ORDMAX, nsad);
   // primal: min side ascending, max side descending; dual: the reverse (arcs_topology.hxx:178-181)
-  if (!dual) launch_orders(nsad, asc, ord_min, ord_max, st);
-  else launch_orders(nsad, asc, ord_max, ord_min, st);
+  int32_t* pos_min = ar.get<int32_t>(A_POSMIN, nsad);
+  int32_t* pos_max = ar.get<int32_t>(A_POSMAX, nsad);
+  if (!dual) launch_orders(nsad, asc, ord_min, ord_max, pos_min, pos_max, st);
+  else launch_orders(nsad, asc, ord_max, ord_min, pos_max, pos_min, st);
 
   // ---- persistence rounds (both sides together)
   uint8_t* removed = ar.get<uint8_t>(A_REMOVED, nsad);
@@ -244,7 +269,7 @@ void run_one(dmsc_context* ctx, const float* f, int H, int W, float thr, uint32_
     fill_i32(sb[s].cnt, (int32_t)nsad, 1, st);
     fill_i32(sb[s].cnt + 1, 0, 1, st);
     PersistSide& p = ps[s];
-    p.ord = s ? ord_max : ord_min;
+    p.pos = s ? pos_max : pos_min;
     p.c = s ? cmax : cmin;
     p.sval = sad_val;
     p.ev = s ? max_val : min_val;
@@ -264,6 +289,7 @@ void run_one(dmsc_context* ctx, const float* f, int H, int W, float thr, uint32_
     p.maxside = s;
   }
   int round = 0;
+  ctx->pending_log.clear();
   if (nsad > 0) {
     while (true) {
       const int batch = round < 8 ? 2 : 8;
@@ -272,6 +298,7 @@ void run_one(dmsc_context* ctx, const float* f, int H, int W, float thr, uint32_
       DMSC_CHECK(hipMemcpyAsync((char*)ctx->pinned + 4, sb[1].cnt + (round & 1), 4, hipMemcpyDeviceToHost, st));
       DMSC_CHECK(hipStreamSynchronize(st));
       const int32_t* pc = (const int32_t*)ctx->pinned;
+      ctx->pending_log.insert(ctx->pending_log.end(), {round, pc[0], pc[1]});
       if (pc[0] == 0 && pc[1] == 0) break;
       if (round > 4 * nsad + 64) throw Fail{DMSC_ERR_INTERNAL, "persistence rounds did not converge"};
     }
@@ -357,13 +384,13 @@ void run_one(dmsc_context* ctx, const float* f, int H, int W, float thr, uint32_
     radix_sort_pairs(ck, cslot, ar.get<uint32_t>(b + G_KEYS2, nc), ar.get<int32_t>(b + G_SLOT2, nc), nc, ar, A_RS0, st);
     GeoState& g = w.G;
     g.ncancel = nc;
-    int32_t* cs = ar.get<int32_t>(b + G_CS, nc);
-    int32_t* cd = ar.get<int32_t>(b + G_CD, nc);
+    int32_t* ckey = ar.get<int32_t>(b + G_CS, nc);
     int32_t* deadc = ar.get<int32_t>(b + G_DEADC, next);
     fill_i32(deadc, -1, next, st);
-    launch_cancel_finish(nc, cslot, cs0, cd0, cs, cd, deadc, st);
-    g.cs = cs;
-    g.cd = cd;
+    launch_cancel_finish(nc, cslot, cd0, ckey, deadc, st);
+    g.cs = cs0;
+    g.cd = cd0;
+    g.key = ckey;
     g.deadc = deadc;
     g.gparent = ar.get<int32_t>(b + G_GPAR, next);
     g.linklen = ar.get<int32_t>(b + G_LLEN, next);
@@ -417,21 +444,21 @@ void run_one(dmsc_context* ctx, const float* f, int H, int W, float thr, uint32_
   // ---- outputs (cpu/workspace.hxx:291-362 layout, dual swap by slot)
   int32_t* o_pts[2];
   for (int s = 0; s < 2; ++s)
-    o_pts[s] = (int32_t*)alloc_out(alloc, user, out, slot_pts(s == 1, dual), 1, n_out_ext[s], 0, DMSC_DTYPE_INT32);
-  int32_t* o_sad = (int32_t*)alloc_out(alloc, user, out, 3, 1, n_sad_out, 0, DMSC_DTYPE_INT32);
+    o_pts[s] = (int32_t*)alloc_out(alloc, user, field, out,slot_pts(s == 1, dual), 1, n_out_ext[s], 0, DMSC_DTYPE_INT32);
+  int32_t* o_sad = (int32_t*)alloc_out(alloc, user, field, out,3, 1, n_sad_out, 0, DMSC_DTYPE_INT32);
   int32_t *o_e[2], *o_pp[2];
   float* o_p[2];
   for (int s = 0; s < 2; ++s) {
-    o_e[s] = (int32_t*)alloc_out(alloc, user, out, slot_e(s == 1, dual), 2, n_rows[s], 2, DMSC_DTYPE_INT32);
-    o_p[s] = (float*)alloc_out(alloc, user, out, slot_p(s == 1, dual), 1, n_sad_out, 0, DMSC_DTYPE_FLOAT32);
-    o_pp[s] = (int32_t*)alloc_out(alloc, user, out, slot_pp(s == 1, dual), 1, n_sad_out, 0, DMSC_DTYPE_INT32);
+    o_e[s] = (int32_t*)alloc_out(alloc, user, field, out,slot_e(s == 1, dual), 2, n_rows[s], 2, DMSC_DTYPE_INT32);
+    o_p[s] = (float*)alloc_out(alloc, user, field, out,slot_p(s == 1, dual), 1, n_sad_out, 0, DMSC_DTYPE_FLOAT32);
+    o_pp[s] = (int32_t*)alloc_out(alloc, user, field, out,slot_pp(s == 1, dual), 1, n_sad_out, 0, DMSC_DTYPE_INT32);
   }
   for (int s = 0; s < 2; ++s) {
     const int sl = slot_arcs(s == 1, dual);
     ArcWork& w = aw[s];
-    w.out_cells = (int32_t*)alloc_out(alloc, user, out, sl, 1, w.on ? w.out_total : 0, 0, DMSC_DTYPE_INT32);
-    w.out_off = (int32_t*)alloc_out(alloc, user, out, sl + 1, 1, w.on ? n_sad_out + 1 : 0, 0, DMSC_DTYPE_INT32);
-    w.out_arc_off = (int32_t*)alloc_out(alloc, user, out, sl + 2, 1, w.on ? n_sad_out : 0, 0, DMSC_DTYPE_INT32);
+    w.out_cells = (int32_t*)alloc_out(alloc, user, field, out,sl, 1, w.on ? w.out_total : 0, 0, DMSC_DTYPE_INT32);
+    w.out_off = (int32_t*)alloc_out(alloc, user, field, out,sl + 1, 1, w.on ? n_sad_out + 1 : 0, 0, DMSC_DTYPE_INT32);
+    w.out_arc_off = (int32_t*)alloc_out(alloc, user, field, out,sl + 2, 1, w.on ? n_sad_out : 0, 0, DMSC_DTYPE_INT32);
   }
 
   launch_emit_points(nmin, min_ids, sb[0].to_out, o_pts[0], st);
@@ -479,6 +506,7 @@ void run_one(dmsc_context* ctx, const float* f, int H, int W, float thr, uint32_
     if (nraw > 0) launch_copy_raw(w.A, raw, nseg + 2, nraw, w.out_cells, w.out_total, d_err, st);
   }
   readback(ctx, d_err, sizeof(int32_t), st);
+  if (ctx->times.on) ctx->times.collect();
   if (*(const int32_t*)ctx->pinned != 0)
     throw Fail{DMSC_ERR_INTERNAL, "device consistency check failed (flags " +
                                       std::to_string(*(const int32_t*)ctx->pinned) + ")"};
@@ -507,6 +535,7 @@ void run_one(dmsc_context* ctx, const float* f, int H, int W, float thr, uint32_
 dmsc_status guarded(dmsc_context* ctx, const std::function<void()>& fn) {
   if (!ctx) return DMSC_ERR_INVALID_ARGUMENT;
   ctx->err.clear();
+  tl_times = &ctx->times;
   try {
     DMSC_CHECK(hipSetDevice(ctx->device));
     fn();
@@ -543,6 +572,13 @@ void dmsc_context_destroy(dmsc_context* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
   (void)hipDeviceSynchronize();
+  for (auto& w : ctx->workers) {
+    w->ar.release();
+    w->ar2.release();
+    if (w->pinned) (void)hipHostFree(w->pinned);
+    if (w->stream) (void)hipStreamDestroy(w->stream);
+  }
+  ctx->workers.clear();
   ctx->ar.release();
   ctx->ar2.release();
   if (ctx->pinned) (void)hipHostFree(ctx->pinned);
@@ -552,7 +588,7 @@ void dmsc_context_destroy(dmsc_context* ctx) {
 dmsc_status dmsc_extract(dmsc_context* ctx, const float* d_field, int32_t H, int32_t W, float thr, uint32_t flags,
                          dmsc_stream_t stream, dmsc_alloc_fn alloc, void* user, dmsc_outputs* out) {
   if (!alloc || !out) return DMSC_ERR_INVALID_ARGUMENT;
-  return guarded(ctx, [&] { run_one(ctx, d_field, H, W, thr, flags, (hipStream_t)stream, alloc, user, out); });
+  return guarded(ctx, [&] { run_one(ctx, d_field, H, W, thr, flags, (hipStream_t)stream, alloc, user, 0, out); });
 }
 
 dmsc_status dmsc_extract_batch(dmsc_context* ctx, int32_t B, const float* const* d_fields, const int32_t* H,
@@ -560,11 +596,89 @@ dmsc_status dmsc_extract_batch(dmsc_context* ctx, int32_t B, const float* const*
                                dmsc_alloc_fn alloc, void* user, dmsc_outputs* out) {
   if (!alloc || !out || B < 0 || (B > 0 && (!d_fields || !H || !W || !thr))) return DMSC_ERR_INVALID_ARGUMENT;
   return guarded(ctx, [&] {
-    for (int b = 0; b < B; ++b) run_one(ctx, d_fields[b], H[b], W[b], thr[b], flags, (hipStream_t)stream, alloc, user, &out[b]);
+    const int nw = std::min<int>(B, batch_workers());
+    if (nw <= 1) {
+      for (int b = 0; b < B; ++b)
+        run_one(ctx, d_fields[b], H[b], W[b], thr[b], flags, (hipStream_t)stream, alloc, user, b, &out[b]);
+      return;
+    }
+    // Worker contexts, each with its own scratch and its own (default-priority) HIP stream; fields
+    // are taken from a shared counter, so small fields' latency-bound stages overlap on the device.
+    while ((int)ctx->workers.size() < nw) {
+      auto* w = new dmsc_context();
+      w->device = ctx->device;
+      DMSC_CHECK(hipHostMalloc((void**)&w->pinned, 4096));
+      DMSC_CHECK(hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking));
+      ctx->workers.emplace_back(w);
+    }
+    hipEvent_t ready;
+    DMSC_CHECK(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+    DMSC_CHECK(hipEventRecord(ready, (hipStream_t)stream));  // inputs were produced on the caller's stream
+    for (int k = 0; k < nw; ++k) DMSC_CHECK(hipStreamWaitEvent(ctx->workers[k]->stream, ready, 0));
+    std::atomic<int> next{0};
+    std::mutex mu;
+    Fail first{DMSC_OK, ""};
+    std::vector<std::thread> th;
+    for (int k = 0; k < nw; ++k) {
+      dmsc_context* w = ctx->workers[k].get();
+      w->times.on = ctx->times.on;
+      th.emplace_back([&, w] {
+        tl_times = &w->times;
+        if (hipSetDevice(w->device) != hipSuccess) return;
+        for (int b; (b = next.fetch_add(1)) < B;) {
+          try {
+            run_one(w, d_fields[b], H[b], W[b], thr[b], flags, w->stream, alloc, user, b, &out[b]);
+          } catch (const Fail& e) {
+            std::lock_guard<std::mutex> lk(mu);
+            if (first.st == DMSC_OK) first = e;
+            next = B;
+          } catch (const HipError& e) {
+            std::lock_guard<std::mutex> lk(mu);
+            if (first.st == DMSC_OK) first = Fail{DMSC_ERR_HIP, std::string(e.what) + ": " + hipGetErrorString(e.err)};
+            next = B;
+          } catch (const std::exception& e) {
+            std::lock_guard<std::mutex> lk(mu);
+            if (first.st == DMSC_OK) first = Fail{DMSC_ERR_INTERNAL, e.what()};
+            next = B;
+          }
+        }
+      });
+    }
+    for (auto& t : th) t.join();
+    (void)hipEventDestroy(ready);
+    for (auto& w : ctx->workers) {
+      for (auto& kv : w->times.acc) {
+        ctx->times.add(kv.first.c_str(), 0.0);
+        for (auto& mine : ctx->times.acc)
+          if (mine.first == kv.first) {
+            mine.second.first += kv.second.first;
+            mine.second.second += kv.second.second - 1;
+          }
+      }
+      w->times.acc.clear();
+    }
+    tl_times = &ctx->times;
+    if (first.st != DMSC_OK) throw first;
   });
 }
 
 const char* dmsc_last_error(dmsc_context* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+dmsc_status dmsc_profile_enable(dmsc_context* ctx, int32_t on) {
+  if (!ctx) return DMSC_ERR_INVALID_ARGUMENT;
+  ctx->times.on = on != 0;
+  ctx->times.acc.clear();
+  return DMSC_OK;
+}
+int32_t dmsc_profile_count(dmsc_context* ctx) { return ctx ? (int32_t)ctx->times.acc.size() : 0; }
+dmsc_status dmsc_profile_entry(dmsc_context* ctx, int32_t i, const char** name, double* total_ms, int64_t* launches) {
+  if (!ctx || i < 0 || i >= (int32_t)ctx->times.acc.size()) return DMSC_ERR_INVALID_ARGUMENT;
+  const auto& kv = ctx->times.acc[i];
+  if (name) *name = kv.first.c_str();
+  if (total_ms) *total_ms = kv.second.first;
+  if (launches) *launches = kv.second.second;
+  return DMSC_OK;
+}
 
 int64_t dmsc_debug_fetch(dmsc_context* ctx, const char* name, void* dst, int64_t cap) {
   if (!ctx || !name) return -1;
@@ -585,6 +699,11 @@ int64_t dmsc_debug_fetch(dmsc_context* ctx, const char* name, void* dst, int64_t
     int32_t v[3] = {ctx->rounds_persist, ctx->rounds_geo[0], ctx->rounds_geo[1]};
     if (dst) std::memcpy(dst, v, sizeof(int32_t) * (size_t)std::min<int64_t>(3, cap));
     return 3;
+  }
+  if (n == "persist_pending") {
+    int64_t m = std::min<int64_t>((int64_t)ctx->pending_log.size(), cap);
+    if (dst && m > 0) std::memcpy(dst, ctx->pending_log.data(), (size_t)m * 4);
+    return (int64_t)ctx->pending_log.size();
   }
   if (n == "min_ids") return copy(ctx->min_ids, ctx->nmin, 4);
   if (n == "sad_ids") return copy(ctx->sad_ids, ctx->nsad, 4);

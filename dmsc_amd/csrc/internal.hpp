@@ -68,6 +68,89 @@ __device__ __forceinline__ void report(int32_t* err, int32_t code) {
   if (err) atomicOr(err, code);
 }
 
+// Optional per-kernel timing with HIP events on the launch stream (enabled per context; used by
+// bench.py to measure the dominant kernel's average duration live). Events come from a reusable pool.
+struct KernelTimes {
+  bool on = false;
+  struct Rec {
+    const char* name;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> pending;
+  std::vector<hipEvent_t> pool;
+  std::vector<std::pair<std::string, std::pair<double, int64_t>>> acc;  // name -> (total ms, launches)
+  hipEvent_t take() {
+    if (pool.empty()) {
+      hipEvent_t e;
+      DMSC_CHECK(hipEventCreate(&e));
+      return e;
+    }
+    hipEvent_t e = pool.back();
+    pool.pop_back();
+    return e;
+  }
+  void add(const char* name, double ms) {
+    for (auto& kv : acc)
+      if (kv.first == name) {
+        kv.second.first += ms;
+        kv.second.second += 1;
+        return;
+      }
+    acc.push_back({name, {ms, 1}});
+  }
+  // call after the stream has been synchronised
+  void collect() {
+    for (auto& r : pending) {
+      float ms = 0.f;
+      DMSC_CHECK(hipEventElapsedTime(&ms, r.a, r.b));
+      add(r.name, ms);
+      pool.push_back(r.a);
+      pool.push_back(r.b);
+    }
+    pending.clear();
+  }
+};
+extern thread_local KernelTimes* tl_times;
+
+struct KScope {
+  KernelTimes* t;
+  const char* name;
+  hipStream_t st;
+  hipEvent_t a{};
+  KScope(const char* n, hipStream_t s) : t(tl_times && tl_times->on ? tl_times : nullptr), name(n), st(s) {
+    if (t) {
+      a = t->take();
+      DMSC_CHECK(hipEventRecord(a, st));
+    }
+  }
+  ~KScope() noexcept(false) {
+    if (t) {
+      hipEvent_t b = t->take();
+      DMSC_CHECK(hipEventRecord(b, st));
+      t->pending.push_back({name, a, b});
+    }
+  }
+};
+
+// Wave-aggregated append: one atomicAdd per wave for all lanes with pred set; returns the lane's
+// slot (or -1). Must be reached by every lane of the wave (wave-uniform control flow).
+__device__ __forceinline__ int wave_append(int32_t* counter, bool pred) {
+  const uint64_t m = __ballot(pred);
+  if (m == 0) return -1;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((unsigned long long)m) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(counter, __popcll(m));
+  base = __shfl(base, leader, 64);
+  return pred ? base + __popcll(m & ((1ull << lane) - 1ull)) : -1;
+}
+
+// Wave-uniform grid-stride loop: every lane of a wave runs the same iterations; `active` marks
+// lanes with an element. Use with wave_append.
+#define DMSC_WAVE_LOOP(j, n)                                                                        \
+  for (int64_t j##_base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); j##_base < (n);   \
+       j##_base += (int64_t)gridDim.x * blockDim.x)
+
 inline int grid_for(int64_t n, int block, int max_blocks = 1 << 20) {
   int64_t g = (n + block - 1) / block;
   if (g < 1) g = 1;

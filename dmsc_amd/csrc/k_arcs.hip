@@ -73,6 +73,7 @@ __global__ void k_raw_arcs(ArcSide A) {
   }
 }
 void launch_raw_arcs(const ArcSide& a, hipStream_t st) {
+  KScope ks_("raw_arcs", st);
   if (a.ns == 0) return;
   hipLaunchKernelGGL(k_raw_arcs, dim3(grid_for(2 * a.ns, 256, 16384)), dim3(256), 0, st, a);
   DMSC_LAUNCH_CHECK();
@@ -108,22 +109,20 @@ void launch_cancel_compact(int64_t ns, const int32_t* cancel_d, const float* per
                      keys);
   DMSC_LAUNCH_CHECK();
 }
-__global__ void k_cancel_finish(int64_t nc, const int32_t* __restrict__ slot, const int32_t* __restrict__ cs_in,
-                                const int32_t* __restrict__ cd_in, int32_t* __restrict__ cs, int32_t* __restrict__ cd,
-                                int32_t* __restrict__ deadc) {
+// Cancellations stay in saddle-id order (index j, spatially coherent); their sorted position is the
+// merge key: key[j] = position of j in (persistence, saddle id) order; deadc[d] = j.
+__global__ void k_cancel_finish(int64_t nc, const int32_t* __restrict__ slot, const int32_t* __restrict__ cd,
+                                int32_t* __restrict__ key, int32_t* __restrict__ deadc) {
   for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x) {
     const int j = slot[c];
-    cs[c] = cs_in[j];
-    const int d = cd_in[j];
-    cd[c] = d;
-    deadc[d] = (int)c;
+    key[j] = (int)c;
+    deadc[cd[j]] = j;
   }
 }
-void launch_cancel_finish(int64_t nc, const int32_t* slot, const int32_t* cs_in, const int32_t* cd_in, int32_t* cs,
-                          int32_t* cd, int32_t* deadc, hipStream_t st) {
+void launch_cancel_finish(int64_t nc, const int32_t* slot, const int32_t* cd, int32_t* key, int32_t* deadc,
+                          hipStream_t st) {
   if (nc == 0) return;
-  hipLaunchKernelGGL(k_cancel_finish, dim3(grid_for(nc, 256, 16384)), dim3(256), 0, st, nc, slot, cs_in, cd_in, cs, cd,
-                     deadc);
+  hipLaunchKernelGGL(k_cancel_finish, dim3(grid_for(nc, 256, 16384)), dim3(256), 0, st, nc, slot, cd, key, deadc);
   DMSC_LAUNCH_CHECK();
 }
 
@@ -160,51 +159,65 @@ __global__ void k_geo_find(ArcSide A, GeoState g, int round) {
     g.rr[4 * c + 1] = R1;
     g.rr[4 * c + 2] = L0;
     g.rr[4 * c + 3] = L1;
-    const uint64_t key = tag | (uint32_t)c;
+    const uint64_t key = tag | (uint32_t)g.key[c];
     if (R0 >= 0) atomicMin((unsigned long long*)&g.watch[R0], (unsigned long long)key);
     if (R1 >= 0) atomicMin((unsigned long long*)&g.watch[R1], (unsigned long long)key);
   }
 }
 
-__device__ __forceinline__ bool root_settled(const GeoState& g, int R, int c, int round) {
+// R stays a root until after cancellation `kc` (key): the cancellation that could link it is done,
+// absent, or comes later in (persistence, saddle id) order.
+__device__ __forceinline__ bool root_settled(const GeoState& g, int R, int kc, int round) {
   if (R < 0) return true;
   const int dk = g.deadc[R];
-  return dk < 0 || g.done[dk] < round || dk >= c;
+  return dk < 0 || g.done[dk] < round || g.key[dk] >= kc;
+}
+
+// Decides one pending cancellation (index c, saddle-id order); returns true when it must wait.
+__device__ __forceinline__ bool geo_apply_one(const ArcSide& A, const GeoState& g, int c, int round, uint64_t tag) {
+  const int R0 = g.rr[4 * c], R1 = g.rr[4 * c + 1];
+  const int32_t L0 = g.rr[4 * c + 2], L1 = g.rr[4 * c + 3];
+  const int d = g.cd[c];
+  const int kc = g.key[c];
+  int kd = -1;
+  if (R0 == d && R0 != R1) kd = 0;
+  else if (R1 == d && R0 != R1) kd = 1;
+  bool ready = root_settled(g, R0, kc, round) && root_settled(g, R1, kc, round);
+  if (ready && kd >= 0) ready = g.watch[d] == (tag | (uint32_t)kc);
+  if (!ready) return true;
+  g.done[c] = round;
+  if (kd >= 0) {
+    const int s = g.cs[c], ko = 1 - kd;
+    const int Ro = kd == 0 ? R1 : R0;
+    const int32_t Ld = kd == 0 ? L0 : L1, Lo = kd == 0 ? L1 : L0;
+    g.gparent[d] = Ro;
+    g.linklen[d] = sat_add(sat_add(Ld, A.blen[2 * s + kd]), sat_add(A.blen[2 * s + ko], Lo));
+    g.linkc[d] = c;
+    g.linkkd[d] = kd;
+  }
+  return false;
 }
 
 __global__ void k_geo_apply(ArcSide A, GeoState g, int round) {
   const int cur = round & 1;
   const int n = g.cnt[cur];
   const uint64_t tag = (uint64_t)(0xFFFFFFFFu - (uint32_t)round) << 32;
-  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
-    const int c = g.wl[cur][j];
-    const int R0 = g.rr[4 * c], R1 = g.rr[4 * c + 1];
-    const int32_t L0 = g.rr[4 * c + 2], L1 = g.rr[4 * c + 3];
-    const int d = g.cd[c];
-    int kd = -1;
-    if (R0 == d && R0 != R1) kd = 0;
-    else if (R1 == d && R0 != R1) kd = 1;
-    bool ready = root_settled(g, R0, c, round) && root_settled(g, R1, c, round);
-    if (ready && kd >= 0) ready = g.watch[d] == (tag | (uint32_t)c);
-    if (!ready) {
-      const int pos = atomicAdd(&g.cnt[cur ^ 1], 1);
-      g.wl[cur ^ 1][pos] = c;
-      continue;
+  const int lane = threadIdx.x & 63;
+  DMSC_WAVE_LOOP(j, (int64_t)n) {
+    const int64_t jj = j_base + lane;
+    bool wait = false;
+    int c = 0;
+    if (jj < n) {
+      c = g.wl[cur][jj];
+      wait = geo_apply_one(A, g, c, round, tag);
     }
-    g.done[c] = round;
-    if (kd >= 0) {
-      const int s = g.cs[c], ko = 1 - kd;
-      const int Ro = kd == 0 ? R1 : R0;
-      const int32_t Ld = kd == 0 ? L0 : L1, Lo = kd == 0 ? L1 : L0;
-      g.gparent[d] = Ro;
-      g.linklen[d] = sat_add(sat_add(Ld, A.blen[2 * s + kd]), sat_add(A.blen[2 * s + ko], Lo));
-      g.linkc[d] = c;
-      g.linkkd[d] = kd;
-    }
+    const int pos = wave_append(&g.cnt[cur ^ 1], wait);
+    if (wait) g.wl[cur ^ 1][pos] = c;
   }
 }
 
 void launch_geo_round(const ArcSide& a, const GeoState& g, int round, int64_t max_items, hipStream_t st) {
+  KScope ks_("geo_round", st);
   const int grid = grid_for(max_items, 256, 2048);
   hipLaunchKernelGGL(k_geo_find, dim3(grid), dim3(256), 0, st, a, g, round);
   DMSC_LAUNCH_CHECK();
@@ -246,6 +259,7 @@ __global__ void k_pj_sat(uint64_t* __restrict__ pj, int64_t n) {
   }
 }
 void launch_geo_q(const GeoState& g, int64_t next, uint64_t* pjq, hipStream_t st) {
+  KScope ks_("geo_q", st);
   hipLaunchKernelGGL(k_geo_q_init, dim3(grid_for(next + 1, 256, 16384)), dim3(256), 0, st, g, next, pjq);
   DMSC_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_pj_sat, dim3(grid_for(next + 1, 256, 16384)), dim3(256), 0, st, pjq, next + 1);
@@ -274,6 +288,7 @@ __global__ void k_final_lengths(int64_t ns, const int32_t* __restrict__ ord, con
 }
 void launch_final_lengths(int64_t ns, const int32_t* ord, const uint8_t* removed, const ArcSide& a, const uint64_t* pjq,
                           int64_t /*next*/, int64_t* flen, int64_t* per_pos, hipStream_t st) {
+  KScope ks_("final_lengths", st);
   if (ns == 0) return;
   hipLaunchKernelGGL(k_final_lengths, dim3(grid_for(ns, 256, 16384)), dim3(256), 0, st, ns, ord, removed, a, pjq, flen,
                      per_pos);
@@ -298,21 +313,43 @@ __global__ void k_seed_segments(int64_t ns, const int32_t* __restrict__ ord, con
                                 int32_t* __restrict__ nseg, Seg* __restrict__ raw, int32_t* __restrict__ nraw,
                                 int64_t cap, int32_t* __restrict__ g_err, int32_t* __restrict__ out_off,
                                 int32_t* __restrict__ out_arc_off) {
-  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < ns; k += (int64_t)gridDim.x * blockDim.x) {
-    const int i = ord[k];
-    if (removed[i]) continue;
-    const int o = sad_to_out[i];
-    const int64_t base = pos_pre[k];
-    out_off[o] = (int32_t)base;
-    out_arc_off[o] = (int32_t)(base + flen[2 * k]);
+  const int lane = threadIdx.x & 63;
+  DMSC_WAVE_LOOP(k, ns) {
+    const int64_t kk0 = k_base + lane;
+    bool has_raw[2] = {false, false}, has_chain[2] = {false, false};
+    Seg rs[2], cs[2];
+    if (kk0 < ns) {
+      const int i = ord[kk0];
+      if (!removed[i]) {
+        const int o = sad_to_out[i];
+        const int64_t base = pos_pre[kk0];
+        out_off[o] = (int32_t)base;
+        out_arc_off[o] = (int32_t)(base + flen[2 * kk0]);
+        for (int kk = 0; kk < 2; ++kk) {
+          const int a = 2 * i + kk, T = A.tinit[a];
+          if (T < 0) continue;
+          const int64_t off = base + (kk ? flen[2 * kk0] : 0);
+          has_raw[kk] = true;
+          rs[kk] = Seg{0, a, 0, 1, off};
+          if (qlen(pjq, T) > 0) {
+            const uint32_t root = (uint32_t)pjq[T];
+            has_chain[kk] = true;
+            cs[kk] = Seg{1, T, root == (uint32_t)next ? -1 : (int)root, 1, off + A.blen[a]};
+          }
+        }
+      }
+    }
+#pragma unroll
     for (int kk = 0; kk < 2; ++kk) {
-      const int a = 2 * i + kk, T = A.tinit[a];
-      if (T < 0) continue;
-      const int64_t off = base + (kk ? flen[2 * k] : 0);
-      push_seg(raw, nraw, cap, g_err, 0, a, 0, 1, off);
-      if (qlen(pjq, T) > 0) {
-        const uint32_t root = (uint32_t)pjq[T];
-        push_seg(segs, nseg, cap, g_err, 1, T, root == (uint32_t)next ? -1 : (int)root, 1, off + A.blen[a]);
+      const int pr = wave_append(nraw, has_raw[kk]);
+      if (has_raw[kk]) {
+        if (pr < cap) raw[pr] = rs[kk];
+        else report(g_err, ERR_SEG_CAP);
+      }
+      const int pc = wave_append(nseg, has_chain[kk]);
+      if (has_chain[kk]) {
+        if (pc < cap) segs[pc] = cs[kk];
+        else report(g_err, ERR_SEG_CAP);
       }
     }
   }
@@ -378,6 +415,7 @@ __global__ void k_expand(ArcSide A, GeoState g, const uint64_t* __restrict__ pjq
 void launch_expand_segments(const ArcSide& a, const GeoState& g, const uint64_t* pjq, const Seg* in, const int32_t* nin,
                             Seg* out, int32_t* nout, Seg* raw, int32_t* nraw, int64_t max_in, int64_t cap,
                             hipStream_t st) {
+  KScope ks_("expand_segments", st);
   hipLaunchKernelGGL(k_expand, dim3(grid_for(max_in, 256, 4096)), dim3(256), 0, st, a, g, pjq, in, nin, out, nout, raw,
                      nraw, cap);
   DMSC_LAUNCH_CHECK();
@@ -407,6 +445,7 @@ __global__ void k_copy_raw(ArcSide A, const Seg* __restrict__ raw, const int32_t
 }
 void launch_copy_raw(const ArcSide& a, const Seg* raw, const int32_t* nraw, int64_t max_raw, int32_t* out,
                      int64_t out_total, int32_t* err, hipStream_t st) {
+  KScope ks_("copy_raw", st);
   hipLaunchKernelGGL(k_copy_raw, dim3(grid_for(max_raw * 64, 256, 8192)), dim3(256), 0, st, a, raw, nraw, out, out_total,
                      err);
   DMSC_LAUNCH_CHECK();

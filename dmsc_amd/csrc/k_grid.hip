@@ -161,6 +161,7 @@ __global__ __launch_bounds__(256) void k_gradient(const float* __restrict__ f, i
 
 void launch_gradient(bool dual, const float* f, const Dims& d, int32_t* pw, int32_t* fsucc, int32_t* vsucc,
                      uint32_t* planes, hipStream_t st) {
+  KScope ks_("gradient", st);
   dim3 grid(ceil_div(d.Nx, GT_X), ceil_div(d.Ny, GT_Y));
   if (dual)
     hipLaunchKernelGGL(k_gradient<true>, grid, dim3(256), 0, st, f, d.H, d.W, pw, fsucc, vsucc, planes, d.PW, d.WPR);
@@ -238,6 +239,7 @@ __global__ void k_compact_lists(const float* __restrict__ f, int H, int W, RankV
 void launch_compact_lists(bool dual, const float* f, const Dims& d, const RankView& rv, int32_t* min_ids,
                           float* min_val, int32_t* sad_ids, float* sad_val, int32_t* max_ids, float* max_val,
                           hipStream_t st) {
+  KScope ks_("compact_lists", st);
   int g = grid_for(d.PW, 256, 8192);
   if (dual)
     hipLaunchKernelGGL(k_compact_lists<true>, dim3(g), dim3(256), 0, st, f, d.H, d.W, rv, min_ids, min_val, sad_ids,
@@ -290,6 +292,7 @@ __global__ void k_saddle_records(int H, int W, RankView rv, const int32_t* __res
 void launch_saddle_records(const Dims& d, const RankView& rv, const int32_t* sad_ids, int64_t nsad,
                            const uint64_t* pj_face, const uint64_t* pj_vert, int32_t* cmax, int32_t* cmin,
                            int32_t* lmax, int32_t* lmin, hipStream_t st) {
+  KScope ks_("saddle_records", st);
   if (nsad == 0) return;
   hipLaunchKernelGGL(k_saddle_records, dim3(grid_for(nsad, 256, 16384)), dim3(256), 0, st, d.H, d.W, rv, sad_ids, nsad,
                      pj_face, pj_vert, cmax, cmin, lmax, lmin);
@@ -315,12 +318,14 @@ __global__ void k_labels_verts(int64_t N, RankView rv, const uint64_t* __restric
 }
 void launch_labels_faces(const Dims& d, const RankView& rv, const uint64_t* pj_face, const int32_t* root_t,
                          const int32_t* to_out, int32_t* out, hipStream_t st) {
+  KScope ks_("labels_faces", st);
   hipLaunchKernelGGL(k_labels_faces, dim3(grid_for(d.S, 256, 16384)), dim3(256), 0, st, d.S, rv, pj_face, root_t, to_out,
                      out);
   DMSC_LAUNCH_CHECK();
 }
 void launch_labels_verts(const Dims& d, const RankView& rv, const uint64_t* pj_vert, const int32_t* root_t,
                          const int32_t* to_out, int32_t* out, hipStream_t st) {
+  KScope ks_("labels_verts", st);
   hipLaunchKernelGGL(k_labels_verts, dim3(grid_for(d.N, 256, 16384)), dim3(256), 0, st, d.N, rv, pj_vert, root_t, to_out,
                      out);
   DMSC_LAUNCH_CHECK();

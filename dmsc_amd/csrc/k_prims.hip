@@ -110,6 +110,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(Load ld, int64_t n, 
 
 template <class T, class Load>
 static void scan_impl(Load ld, T* out, int64_t n, T* d_total, Arena& ar, int slot, hipStream_t st) {
+  KScope ks_("scan", st);
   const int64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
   T* bsums = ar.get<T>(slot, (size_t)(nb > 0 ? nb : 1));
   if (n > 0) {
@@ -210,6 +211,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint32_t* __res
 
 void radix_sort_pairs(uint32_t* keys, int32_t* vals, uint32_t* keys_tmp, int32_t* vals_tmp, int64_t n, Arena& ar,
                       int slot, hipStream_t st) {
+  KScope ks_("radix_sort", st);
   if (n <= 1) return;
   const int64_t nb = (n + RS_TILE - 1) / RS_TILE;
   int32_t* counts = ar.get<int32_t>(slot, (size_t)(256 * nb));
@@ -264,11 +266,13 @@ __global__ void k_pj(uint64_t* __restrict__ pj, int64_t n, int32_t* __restrict__
 }
 
 void pointer_jump_init(const int32_t* succ, uint64_t* pj, int64_t n, hipStream_t st) {
+  KScope ks_("pointer_jump_init", st);
   if (n == 0) return;
   hipLaunchKernelGGL(k_pj_init, dim3(grid_for(n, 256, 16384)), dim3(256), 0, st, succ, pj, n);
   DMSC_LAUNCH_CHECK();
 }
 void pointer_jump(uint64_t* pj, int64_t n, int32_t* err, hipStream_t st) {
+  KScope ks_("pointer_jump", st);
   if (n == 0) return;
   hipLaunchKernelGGL(k_pj, dim3(grid_for(n, 256, 1 << 20)), dim3(256), 0, st, pj, n, err);
   DMSC_LAUNCH_CHECK();

@@ -45,57 +45,48 @@ __global__ void k_persist_find(PersistSide s0, PersistSide s1, int round) {
   for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n0 + n1; j += (int64_t)gridDim.x * blockDim.x) {
     const bool side1 = j >= n0;
     const PersistSide& s = side1 ? s1 : s0;
-    const int k = s.wl[cur][side1 ? j - n0 : j];
-    const int i = s.ord[k];
+    const int i = s.wl[cur][side1 ? j - n0 : j];  // saddle index (worklists run in cell-id order)
     const int ra = uf_find(s.par_g, s.c[2 * i], s.err);
     const int rb = uf_find(s.par_g, s.c[2 * i + 1], s.err);
     if (ra == rb) {
       s.pair[i] = -1;
       s.pers[i] = -1.0f;
-      s.rr[2 * k] = -2;
+      s.rr[2 * i] = -2;
       continue;
     }
-    s.rr[2 * k] = ra;
-    s.rr[2 * k + 1] = rb;
-    const uint64_t key = tag | (uint32_t)k;
+    s.rr[2 * i] = ra;
+    s.rr[2 * i + 1] = rb;
+    const uint64_t key = tag | (uint32_t)s.pos[i];
     if (ra >= 0) atomicMin((unsigned long long*)&s.minkey[ra], (unsigned long long)key);
     if (rb >= 0) atomicMin((unsigned long long*)&s.minkey[rb], (unsigned long long)key);
   }
 }
 
+// Decides one pending saddle i; returns true when it must wait for a later round.
 template <bool DUAL>
-__global__ void k_persist_apply(PersistSide s0, PersistSide s1, int round, float thr, uint8_t* __restrict__ removed) {
-  const int cur = round & 1;
-  const int n0 = s0.cnt[cur], n1 = s1.cnt[cur];
-  const uint64_t tag = (uint64_t)(0xFFFFFFFFu - (uint32_t)round) << 32;
-  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n0 + n1; j += (int64_t)gridDim.x * blockDim.x) {
-    const bool side1 = j >= n0;
-    const PersistSide& s = side1 ? s1 : s0;
-    const int k = s.wl[cur][side1 ? j - n0 : j];
-    const int ra = s.rr[2 * k];
-    if (ra == -2) continue;
-    const int rb = s.rr[2 * k + 1];
-    const int i = s.ord[k];
-    int dying, surv;
-    if (ra == -1) {
-      dying = rb;
-      surv = -1;
-    } else if (rb == -1) {
-      dying = ra;
-      surv = -1;
-    } else {
-      // min side: value_greater -> r1 dies (persistence.hxx:73-80); max side: value_less (:130-137)
-      const bool r1_dies = s.maxside ? higher<DUAL>(s.ev[rb], s.eid[rb], s.ev[ra], s.eid[ra])
-                                     : higher<DUAL>(s.ev[ra], s.eid[ra], s.ev[rb], s.eid[rb]);
-      dying = r1_dies ? ra : rb;
-      surv = r1_dies ? rb : ra;
-    }
-    const uint64_t key = tag | (uint32_t)k;
-    if (s.minkey[dying] != key) {
-      const int pos = atomicAdd(&s.cnt[cur ^ 1], 1);
-      s.wl[cur ^ 1][pos] = k;
-      continue;
-    }
+__device__ __forceinline__ bool persist_apply_one(const PersistSide& s, int i, uint64_t tag, float thr,
+                                                  uint8_t* __restrict__ removed) {
+  const int ra = s.rr[2 * i];
+  if (ra == -2) return false;
+  const int rb = s.rr[2 * i + 1];
+  const int k = s.pos[i];
+  int dying, surv;
+  if (ra == -1) {
+    dying = rb;
+    surv = -1;
+  } else if (rb == -1) {
+    dying = ra;
+    surv = -1;
+  } else {
+    // min side: value_greater -> r1 dies (persistence.hxx:73-80); max side: value_less (:130-137)
+    const bool r1_dies = s.maxside ? higher<DUAL>(s.ev[rb], s.eid[rb], s.ev[ra], s.eid[ra])
+                                   : higher<DUAL>(s.ev[ra], s.eid[ra], s.ev[rb], s.eid[rb]);
+    dying = r1_dies ? ra : rb;
+    surv = r1_dies ? rb : ra;
+  }
+  const uint64_t key = tag | (uint32_t)k;
+  if (s.minkey[dying] != key) return true;
+  {
     const float sv = s.sval[i], dv = s.ev[dying];
     float p;
     if (s.maxside) p = DUAL ? sv - dv : dv - sv;
@@ -119,10 +110,37 @@ __global__ void k_persist_apply(PersistSide s0, PersistSide s1, int round, float
       if (s.cancel_d) s.cancel_d[i] = d;
     }
   }
+  return false;
+}
+
+template <bool DUAL>
+__global__ void k_persist_apply(PersistSide s0, PersistSide s1, int round, float thr, uint8_t* __restrict__ removed) {
+  const int cur = round & 1;
+  const int n0 = s0.cnt[cur], n1 = s1.cnt[cur];
+  const uint64_t tag = (uint64_t)(0xFFFFFFFFu - (uint32_t)round) << 32;
+  const int lane = threadIdx.x & 63;
+  DMSC_WAVE_LOOP(j, (int64_t)n0 + n1) {
+    const int64_t jj = j_base + lane;
+    const bool side1 = jj >= n0;
+    bool wait = false;
+    int k = 0;
+    if (jj < (int64_t)n0 + n1) {
+      const PersistSide& s = side1 ? s1 : s0;
+      k = s.wl[cur][side1 ? jj - n0 : jj];
+      wait = persist_apply_one<DUAL>(s, k, tag, thr, removed);
+    }
+    const int p0 = wave_append(&s0.cnt[cur ^ 1], wait && !side1);
+    const int p1 = wave_append(&s1.cnt[cur ^ 1], wait && side1);
+    if (wait) {
+      if (side1) s1.wl[cur ^ 1][p1] = k;
+      else s0.wl[cur ^ 1][p0] = k;
+    }
+  }
 }
 
 void launch_persist_round(bool dual, const PersistSide& s0, const PersistSide& s1, int round, float thr,
                           uint8_t* removed, int64_t max_items, hipStream_t st) {
+  KScope ks_("persist_round", st);
   const int g = grid_for(max_items, 256, 2048);
   hipLaunchKernelGGL(k_persist_find, dim3(g), dim3(256), 0, st, s0, s1, round);
   DMSC_LAUNCH_CHECK();
@@ -139,6 +157,7 @@ __global__ void k_uf_flatten(int32_t* __restrict__ par, int32_t* __restrict__ ro
     root[e] = uf_find(par, (int)e);
 }
 void launch_uf_flatten(int32_t* par, int32_t* root, int64_t n, hipStream_t st) {
+  KScope ks_("uf_flatten", st);
   if (n == 0) return;
   hipLaunchKernelGGL(k_uf_flatten, dim3(grid_for(n, 256, 16384)), dim3(256), 0, st, par, root, n);
   DMSC_LAUNCH_CHECK();
@@ -263,6 +282,7 @@ void launch_emit_side(int64_t ns, const int32_t* ord, const uint8_t* removed, co
                       const int32_t* root_t, const int32_t* to_out, const float* pers, const int32_t* pair,
                       const int32_t* row_pre, const int32_t* sad_ids, float* out_p, int32_t* out_pp, int32_t* out_e,
                       int32_t* out_sad, hipStream_t st) {
+  KScope ks_("emit_side", st);
   if (ns == 0) return;
   hipLaunchKernelGGL(k_emit_side, dim3(grid_for(ns, 256, 16384)), dim3(256), 0, st, ns, ord, removed, sad_to_out, c,
                      root_t, to_out, pers, pair, row_pre, sad_ids, out_p, out_pp, out_e, out_sad);
@@ -270,16 +290,21 @@ void launch_emit_side(int64_t ns, const int32_t* ord, const uint8_t* removed, co
 }
 
 // side order arrays from the ascending (value, id) order: forward or reversed
-__global__ void k_orders(int64_t ns, const int32_t* __restrict__ asc, int32_t* __restrict__ fwd, int32_t* __restrict__ rev) {
+// Both sweep orders (ascending / descending) and their inverses (saddle index -> position).
+__global__ void k_orders(int64_t ns, const int32_t* __restrict__ asc, int32_t* __restrict__ fwd, int32_t* __restrict__ rev,
+                         int32_t* __restrict__ pos_fwd, int32_t* __restrict__ pos_rev) {
   for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < ns; k += (int64_t)gridDim.x * blockDim.x) {
     const int v = asc[k];
     fwd[k] = v;
     rev[ns - 1 - k] = v;
+    pos_fwd[v] = (int)k;
+    pos_rev[v] = (int)(ns - 1 - k);
   }
 }
-void launch_orders(int64_t ns, const int32_t* asc, int32_t* fwd, int32_t* rev, hipStream_t st) {
+void launch_orders(int64_t ns, const int32_t* asc, int32_t* fwd, int32_t* rev, int32_t* pos_fwd, int32_t* pos_rev,
+                   hipStream_t st) {
   if (ns == 0) return;
-  hipLaunchKernelGGL(k_orders, dim3(grid_for(ns, 256, 16384)), dim3(256), 0, st, ns, asc, fwd, rev);
+  hipLaunchKernelGGL(k_orders, dim3(grid_for(ns, 256, 16384)), dim3(256), 0, st, ns, asc, fwd, rev, pos_fwd, pos_rev);
   DMSC_LAUNCH_CHECK();
 }
 

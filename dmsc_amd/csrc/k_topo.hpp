@@ -7,7 +7,7 @@ namespace dmsc {
 
 // One side (vertex-minima or face-maxima) of the persistence sweep.
 struct PersistSide {
-  const int32_t* ord;   // order position -> saddle index (the side's sweep order)
+  const int32_t* pos;   // saddle index -> position in the side's sweep order (the merge key)
   const int32_t* c;     // per saddle: c[2i], c[2i+1] = extremum indices of its two V-paths (-1 = none)
   const float* sval;    // per saddle value
   const float* ev;      // per extremum value
@@ -19,8 +19,8 @@ struct PersistSide {
   int32_t* pair;        // per saddle: dying extremum index, -1 for a cycle
   float* pers;          // per saddle persistence (-1 for a cycle)
   int32_t* cancel_d;    // per saddle thresholded dead extremum (null when arcs are not traced)
-  int32_t* rr;          // per order position: the two roots seen this round (-2 = cycle)
-  int32_t* wl[2];       // ping-pong worklists of order positions
+  int32_t* rr;          // per saddle: the two roots seen this round (-2 = cycle)
+  int32_t* wl[2];       // ping-pong worklists of pending saddle indices
   int32_t* cnt;         // [2] worklist sizes (device)
   int32_t* err;         // device error flags
   int maxside;
@@ -40,7 +40,8 @@ void launch_emit_side(int64_t ns, const int32_t* ord, const uint8_t* removed, co
                       const int32_t* root_t, const int32_t* to_out, const float* pers, const int32_t* pair,
                       const int32_t* row_pre, const int32_t* sad_ids, float* out_p, int32_t* out_pp, int32_t* out_e,
                       int32_t* out_sad, hipStream_t st);
-void launch_orders(int64_t ns, const int32_t* asc, int32_t* fwd, int32_t* rev, hipStream_t st);
+void launch_orders(int64_t ns, const int32_t* asc, int32_t* fwd, int32_t* rev, int32_t* pos_fwd, int32_t* pos_rev,
+                   hipStream_t st);
 void launch_sort_keys(int64_t n, const float* v, uint32_t* keys, hipStream_t st);
 
 // ------------------------------------------------------------------ arc geometry (k_arcs.hip)
@@ -62,11 +63,12 @@ void launch_arc_lengths(int64_t n2, const int32_t* c, const int32_t* steps, int3
 void launch_raw_arcs(const ArcSide& a, hipStream_t st);
 
 struct GeoState {
-  // cancellations sorted by (persistence, saddle id): position = key
+  // cancellations indexed in saddle-id order; key = position in (persistence, saddle id) order
   int64_t ncancel;
   const int32_t* cs;       // cancel -> saddle index
   const int32_t* cd;       // cancel -> dead extremum
-  const int32_t* deadc;    // extremum -> cancel index whose dead it is (-1)
+  const int32_t* key;      // cancel -> sorted position
+  const int32_t* deadc;    // extremum -> cancel whose dead it is (-1)
   int32_t* gparent;        // geometry union-find (-1 = void)
   int32_t* linklen;        // length of the link path of a linked extremum (saturating int32)
   int32_t* linkc;          // cancel that linked the extremum
@@ -82,8 +84,8 @@ void launch_geo_round(const ArcSide& a, const GeoState& g, int round, int64_t ma
 void launch_cancel_flags(int64_t ns, const int32_t* cancel_d, int32_t* flag, hipStream_t st);
 void launch_cancel_compact(int64_t ns, const int32_t* cancel_d, const float* pers, const int32_t* pre,
                            int32_t* cs, int32_t* cd, uint32_t* keys, hipStream_t st);
-void launch_cancel_finish(int64_t nc, const int32_t* sorted_cancel_slot, const int32_t* cs_in, const int32_t* cd_in,
-                          int32_t* cs, int32_t* cd, int32_t* deadc, hipStream_t st);
+void launch_cancel_finish(int64_t nc, const int32_t* slot, const int32_t* cd, int32_t* key, int32_t* deadc,
+                          hipStream_t st);
 void launch_geo_q(const GeoState& g, int64_t next, uint64_t* pjq, hipStream_t st);
 
 // Output arc lengths for surviving saddles in min order; then segment expansion.

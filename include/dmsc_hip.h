@@ -46,10 +46,13 @@ enum {
 enum { DMSC_NUM_OUTPUTS = 19 };
 enum { DMSC_DTYPE_INT32 = 0, DMSC_DTYPE_FLOAT32 = 1 };
 
-/* Allocator callback: return device memory for output `slot` with the given shape and dtype
- * (ndim 1 or 2, row-major). Called once per slot 1..18 per field, possibly with zero elements
- * (the pointer is then never dereferenced). Return NULL on failure. */
-typedef void* (*dmsc_alloc_fn)(void* user, int32_t slot, int32_t ndim, const int64_t* dims, int32_t dtype);
+/* Allocator callback: return device memory for output `slot` of field `field` (0 for
+ * dmsc_extract, the batch index for dmsc_extract_batch) with the given shape and dtype (ndim 1 or
+ * 2, row-major). Called once per slot 1..18 per field, possibly with zero elements (the pointer is
+ * then never dereferenced). Return NULL on failure. dmsc_extract_batch may call it concurrently
+ * from several host threads (for different fields). */
+typedef void* (*dmsc_alloc_fn)(void* user, int32_t field, int32_t slot, int32_t ndim, const int64_t* dims,
+                               int32_t dtype);
 
 typedef struct {
   void* data[DMSC_NUM_OUTPUTS]; /* device pointers from the allocator (data[0] unused) */
@@ -81,6 +84,15 @@ dmsc_status dmsc_extract(dmsc_context* ctx, const float* d_field, int32_t H, int
 dmsc_status dmsc_extract_batch(dmsc_context* ctx, int32_t B, const float* const* d_fields, const int32_t* H,
                                const int32_t* W, const float* persistence_thresholds, uint32_t flags,
                                dmsc_stream_t stream, dmsc_alloc_fn alloc, void* alloc_user, dmsc_outputs* out);
+
+/* Per-kernel timing with HIP events recorded on the work stream around each kernel launch (no
+ * reference counterpart; the benchmark uses it to measure the dominant kernel's duration live).
+ * dmsc_profile_enable(ctx, 1) clears and starts accumulating; entries are (kernel name, total ms,
+ * launches) over all calls since. */
+dmsc_status dmsc_profile_enable(dmsc_context* ctx, int32_t on);
+int32_t dmsc_profile_count(dmsc_context* ctx);
+dmsc_status dmsc_profile_entry(dmsc_context* ctx, int32_t index, const char** name, double* total_ms,
+                               int64_t* launches);
 
 /* Message for the last failing call on this context (valid until the next call). */
 const char* dmsc_last_error(dmsc_context* ctx);
