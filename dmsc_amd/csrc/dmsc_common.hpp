@@ -49,17 +49,14 @@ DMSC_HD uint32_t sortable_key(float f) {
 // variable. A face in the lower star always has both of its v-incident edges in it.
 // Face j's v-incident edges, (first, second):
 constexpr DMSC_HD int face_edge(int j, int which) {
-  // UL: left, up   UR: up, right   DL: left, down   DR: right, down
-  const int t0[4] = {0, 2, 0, 1};
-  const int t1[4] = {2, 1, 3, 3};
-  return which ? t1[j] : t0[j];
+  // UL: left, up   UR: up, right   DL: left, down   DR: right, down  (2-bit entries, packed)
+  return ((which ? 0xF6 : 0x48) >> (2 * j)) & 3;
 }
 // The two faces bounding edge e, in the order the reference pushes them (gradient.hxx:174-181):
 // H edge (ey,ex): F(ey,ex), F(ey+1,ex);  V edge (ey,ex): F(ey,ex), F(ey,ex+1).
 constexpr DMSC_HD int edge_face(int e, int which) {
-  const int t0[4] = {0, 1, 0, 2};  // left: UL,DL  right: UR,DR  up: UL,UR  down: DL,DR
-  const int t1[4] = {2, 3, 1, 3};
-  return which ? t1[e] : t0[e];
+  // left: UL,DL  right: UR,DR  up: UL,UR  down: DL,DR  (2-bit entries, packed)
+  return ((which ? 0xDE : 0x84) >> (2 * e)) & 3;
 }
 
 // Result of the reference FIFO face queue (gradient.hxx:115-191) for one lower star, as a byte:
@@ -128,67 +125,56 @@ struct LowerStar {
 
 template <bool DUAL>
 DMSC_HD LowerStar lower_star(const float nb[3][3], const bool ex[3][3]) {
+  // Neighbour n at (dy, dx) has static rank r = 3*(dy+1) + (dx+1); ranks order like pixel ids.
+  // All array indices below are compile-time constants; running "best" candidates are carried as
+  // (value, rank) in registers, so nothing is indexed at run time.
   const float v = nb[1][1];
-  // g(dy,dx): v higher than neighbour (v's id is larger iff the neighbour precedes it)
-  auto g = [&](int dy, int dx) { return higher_known<DUAL>(v, nb[dy + 1][dx + 1], (dy < 0) || (dy == 0 && dx < 0)); };
-  // h(a, b): neighbour a higher than neighbour b
-  auto h = [&](int ay, int ax, int by, int bx) {
-    bool a_larger = (ay > by) || (ay == by && ax > bx);
-    return higher_known<DUAL>(nb[ay + 1][ax + 1], nb[by + 1][bx + 1], a_larger);
-  };
+  auto hk = [](float a, int ra, float b, int rb) { return higher_known<DUAL>(a, b, ra > rb); };
+  // v higher than neighbour of rank r (v has rank 4)
   const bool eL = ex[1][0], eR = ex[1][2], eU = ex[0][1], eD = ex[2][1];
-  const bool gL = eL ? g(0, -1) : true, gR = eR ? g(0, 1) : true;
-  const bool gU = eU ? g(-1, 0) : true, gD = eD ? g(1, 0) : true;
-  int emask = (gL ? 1 : 0) | (gR ? 2 : 0) | (gU ? 4 : 0) | (gD ? 8 : 0);
+  const bool gL = eL ? hk(v, 4, nb[1][0], 3) : true, gR = eR ? hk(v, 4, nb[1][2], 5) : true;
+  const bool gU = eU ? hk(v, 4, nb[0][1], 1) : true, gD = eD ? hk(v, 4, nb[2][1], 7) : true;
+  const int emask = (gL ? 1 : 0) | (gR ? 2 : 0) | (gU ? 4 : 0) | (gD ? 8 : 0);
 
   int fmask = 0;
-  // UL F(y,x): candidates (-1,-1), (-1,0), (0,-1), v  -> v wins iff v beats the running best.
+  // UL F(y,x): candidates (-1,-1), (-1,0), (0,-1), then v; v wins iff it beats the running best.
   {
-    int by = 2, bx = 2;  // none yet
-    auto cand = [&](int cy, int cx) {
-      if (!ex[cy + 1][cx + 1]) return;
-      if (by == 2 || h(cy, cx, by, bx)) { by = cy; bx = cx; }
-    };
-    cand(-1, -1);
-    cand(-1, 0);
-    cand(0, -1);
-    if (by == 2 || g(by, bx)) fmask |= 1;
+    bool have = false;
+    float bv = 0.f;
+    int br = 0;
+    if (ex[0][0]) { have = true; bv = nb[0][0]; br = 0; }
+    if (ex[0][1] && (!have || hk(nb[0][1], 1, bv, br))) { have = true; bv = nb[0][1]; br = 1; }
+    if (ex[1][0] && (!have || hk(nb[1][0], 3, bv, br))) { have = true; bv = nb[1][0]; br = 3; }
+    if (!have || hk(v, 4, bv, br)) fmask |= 1;
   }
   // UR F(y,x+1): (-1,0), (-1,1), v, (0,1)
   {
-    int by = 2, bx = 2;
-    if (ex[0][1]) { by = -1; bx = 0; }
-    if (ex[0][2] && (by == 2 || h(-1, 1, by, bx))) { by = -1; bx = 1; }
-    bool vin = (by == 2) || g(by, bx);
+    bool have = false;
+    float bv = 0.f;
+    int br = 0;
+    if (ex[0][1]) { have = true; bv = nb[0][1]; br = 1; }
+    if (ex[0][2] && (!have || hk(nb[0][2], 2, bv, br))) { have = true; bv = nb[0][2]; br = 2; }
+    const bool vin = !have || hk(v, 4, bv, br);
     if (vin && gR) fmask |= 2;
   }
   // DL F(y+1,x): (0,-1), v, (1,-1), (1,0)
-  {
-    bool vin = gL;
-    if (vin && ex[2][0] && !g(1, -1)) vin = false;
-    if (vin && !gD) vin = false;
-    if (vin) fmask |= 4;
-  }
+  if (gL && (!ex[2][0] || hk(v, 4, nb[2][0], 6)) && gD) fmask |= 4;
   // DR F(y+1,x+1): v, (0,1), (1,0), (1,1)
-  {
-    bool vin = gR && gD;
-    if (vin && ex[2][2] && !g(1, 1)) vin = false;
-    if (vin) fmask |= 8;
-  }
+  if (gR && gD && (!ex[2][2] || hk(v, 4, nb[2][2], 8))) fmask |= 8;
 
-  // vertex pairing: among lower-star edges with a second vertex, in L_edges order, keep the one
-  // whose other vertex is lowest (replace when the kept one is higher than the candidate).
-  int estar = 4, sy = 0, sx = 0;
-  const int oy[4] = {0, 0, -1, 1}, ox[4] = {-1, 1, 0, 0};
-  const bool oex[4] = {eL, eR, eU, eD};
-  for (int e = 0; e < 4; ++e) {
-    if (!(emask >> e & 1) || !oex[e]) continue;
-    if (estar == 4 || h(sy, sx, oy[e], ox[e])) {
-      estar = e;
-      sy = oy[e];
-      sx = ox[e];
-    }
-  }
+  // vertex pairing: among lower-star edges with a second vertex, in L_edges order (left, right,
+  // up, down), keep the one whose other vertex is lowest (replace when the kept one is higher).
+  int estar = 4;
+  float sv = 0.f;
+  int sr = 0;
+  auto cand = [&](int e, bool exists, float ov, int orank) {
+    if (!(emask >> e & 1) || !exists) return;
+    if (estar == 4 || hk(sv, sr, ov, orank)) { estar = e; sv = ov; sr = orank; }
+  };
+  cand(0, eL, nb[1][0], 3);
+  cand(1, eR, nb[1][2], 5);
+  cand(2, eU, nb[0][1], 1);
+  cand(3, eD, nb[2][1], 7);
   return LowerStar{emask, fmask, estar};
 }
 

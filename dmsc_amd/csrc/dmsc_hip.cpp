@@ -48,7 +48,7 @@ enum ArcSlot {
   G_BLEN, G_BOFF, G_ARENA, G_FLAG, G_PRE, G_CS0, G_CD0, G_KEYS, G_SLOT, G_KEYS2, G_SLOT2, G_CS, G_CD, G_DEADC,
   G_GPAR, G_LLEN, G_LC, G_LKD, G_DONE, G_WATCH, G_RR, G_WL0, G_WL1, G_CNT
 };
-enum ArcSlot2 { H_PJQ = 0, H_FLEN, H_PERPOS, H_POSPRE, H_SEGA, H_SEGB, H_RAW, H_NSEG };
+enum ArcSlot2 { H_PJQ = 0, H_FLEN, H_PERPOS, H_POSPRE, H_SEGA, H_SEGB, H_RAW, H_NSEG, H_LONG };
 
 }  // namespace dmsc
 
@@ -423,7 +423,7 @@ void run_one(dmsc_context* ctx, const float* f, int H, int W, float thr, uint32_
     ctx->rounds_geo[s] = r;
     // path lengths to the final roots and per-output-arc lengths (min order)
     Arena& a2 = ctx->ar2;
-    const int hb = 8 * s;
+    const int hb = 16 * s;
     w.pjq = a2.get<uint64_t>(hb + H_PJQ, next + 1);
     launch_geo_q(g, next, w.pjq, st);
     w.flen = a2.get<int64_t>(hb + H_FLEN, 2 * nsad);
@@ -474,7 +474,7 @@ void run_one(dmsc_context* ctx, const float* f, int H, int W, float thr, uint32_
     ArcWork& w = aw[s];
     if (!w.on) continue;
     Arena& a2 = ctx->ar2;
-    const int hb = 8 * s;
+    const int hb = 16 * s;
     const int64_t next = s ? nmax : nmin;
     const int64_t cap = w.out_total + 16;
     Seg* sa = a2.get<Seg>(hb + H_SEGA, cap);
@@ -503,7 +503,9 @@ void run_one(dmsc_context* ctx, const float* f, int H, int W, float thr, uint32_
     }
     const int32_t nraw = ((const int32_t*)ctx->pinned)[2];
     if (nraw > cap) throw Fail{DMSC_ERR_INTERNAL, "arc segment list overflow"};
-    if (nraw > 0) launch_copy_raw(w.A, raw, nseg + 2, nraw, w.out_cells, w.out_total, d_err, st);
+    if (nraw > 0)
+      launch_copy_raw(w.A, raw, nseg + 2, nraw, w.out_cells, w.out_total, d_err, a2.get<int32_t>(hb + H_LONG, nraw),
+                      nseg + 3, st);
   }
   readback(ctx, d_err, sizeof(int32_t), st);
   if (ctx->times.on) ctx->times.collect();

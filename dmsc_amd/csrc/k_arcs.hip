@@ -421,20 +421,48 @@ void launch_expand_segments(const ArcSide& a, const GeoState& g, const uint64_t*
   DMSC_LAUNCH_CHECK();
 }
 
-// one wave per RAW segment: copy the raw arc (forward or reversed) into the output
-__global__ void k_copy_raw(ArcSide A, const Seg* __restrict__ raw, const int32_t* __restrict__ nraw,
-                           int32_t* __restrict__ out, int64_t out_total, int32_t* __restrict__ err) {
+// RAW segments: copy the raw arc (forward or reversed) into the output. Short arcs (the common case)
+// are copied by one thread each; arcs longer than kLongArc are deferred to a wave-per-arc pass.
+constexpr int kLongArc = 48;
+
+__global__ void k_copy_raw_short(ArcSide A, const Seg* __restrict__ raw, const int32_t* __restrict__ nraw,
+                                 int32_t* __restrict__ out, int64_t out_total, int32_t* __restrict__ err,
+                                 int32_t* __restrict__ longlist, int32_t* __restrict__ nlong) {
   const int n = *nraw;
+  const int lane = threadIdx.x & 63;
+  DMSC_WAVE_LOOP(j, (int64_t)n) {
+    const int64_t jj = j_base + lane;
+    bool is_long = false;
+    if (jj < n) {
+      const Seg sg = raw[jj];
+      const int len = A.blen[sg.a];
+      if (sg.off < 0 || sg.off + len > out_total) {  // inconsistent offsets: never write out of bounds
+        report(err, ERR_SEG_CAP);
+      } else if (len > kLongArc) {
+        is_long = true;
+      } else {
+        const int32_t* src = A.arena + A.boff[sg.a];
+        int32_t* dst = out + sg.off;
+        if (sg.dir)
+          for (int t = 0; t < len; ++t) dst[t] = src[t];
+        else
+          for (int t = 0; t < len; ++t) dst[t] = src[len - 1 - t];
+      }
+    }
+    const int p = wave_append(nlong, is_long);
+    if (is_long) longlist[p] = (int)jj;
+  }
+}
+
+__global__ void k_copy_raw_long(ArcSide A, const Seg* __restrict__ raw, const int32_t* __restrict__ longlist,
+                                const int32_t* __restrict__ nlong, int32_t* __restrict__ out) {
+  const int n = *nlong;
   const int lane = threadIdx.x & 63;
   const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
   for (int64_t j = wave; j < n; j += nwaves) {
-    const Seg sg = raw[j];
+    const Seg sg = raw[longlist[j]];
     const int len = A.blen[sg.a];
-    if (sg.off < 0 || sg.off + len > out_total) {  // inconsistent offsets: never write out of bounds
-      if (lane == 0) report(err, ERR_SEG_CAP);
-      continue;
-    }
     const int32_t* src = A.arena + A.boff[sg.a];
     int32_t* dst = out + sg.off;
     if (sg.dir)
@@ -443,11 +471,15 @@ __global__ void k_copy_raw(ArcSide A, const Seg* __restrict__ raw, const int32_t
       for (int t = lane; t < len; t += 64) dst[t] = src[len - 1 - t];
   }
 }
+
 void launch_copy_raw(const ArcSide& a, const Seg* raw, const int32_t* nraw, int64_t max_raw, int32_t* out,
-                     int64_t out_total, int32_t* err, hipStream_t st) {
+                     int64_t out_total, int32_t* err, int32_t* longlist, int32_t* nlong, hipStream_t st) {
   KScope ks_("copy_raw", st);
-  hipLaunchKernelGGL(k_copy_raw, dim3(grid_for(max_raw * 64, 256, 8192)), dim3(256), 0, st, a, raw, nraw, out, out_total,
-                     err);
+  fill_i32(nlong, 0, 1, st);
+  hipLaunchKernelGGL(k_copy_raw_short, dim3(grid_for(max_raw, 256, 16384)), dim3(256), 0, st, a, raw, nraw, out,
+                     out_total, err, longlist, nlong);
+  DMSC_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_copy_raw_long, dim3(2048), dim3(256), 0, st, a, raw, longlist, nlong, out);
   DMSC_LAUNCH_CHECK();
 }
 

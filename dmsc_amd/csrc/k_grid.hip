@@ -86,33 +86,42 @@ __global__ __launch_bounds__(256) void k_gradient(const float* __restrict__ f, i
     const LowerStar ls = lower_star<DUAL>(nb, ex);
     const int code = lut[ls.emask | ls.fmask << 4 | ls.estar << 8];
     const int q = vy * Nx + vx;
-    const int eid[4] = {4 * q + 1, 4 * (q + 1) + 1, 4 * q + 2, 4 * (q + Nx) + 2};
-    const int fid[4] = {4 * q + 3, 4 * (q + 1) + 3, 4 * (q + Nx) + 3, 4 * (q + Nx + 1) + 3};
+    // edge e of v: left H(y,x), right H(y,x+1), up V(y,x), down V(y+1,x); face j: UL, UR, DL, DR
+    auto eid = [&](int e) { return 4 * (q + (e == 1 ? 1 : 0) + (e == 3 ? Nx : 0)) + (e < 2 ? 1 : 2); };
+    auto fid = [&](int j) { return 4 * (q + (j & 1) + (j >> 1) * Nx) + 3; };
     const int sy = vy - y0, sx = vx - x0;
-    if (sy >= 0 && sx >= 0) cell[sy][sx][0] = ls.estar < 4 ? eid[ls.estar] : -1;
-    const int esy[4] = {0, 0, 0, 1}, esx[4] = {0, 1, 0, 0};
-    const int fsy[4] = {0, 0, 1, 1}, fsx[4] = {0, 1, 0, 1};
+    if (sy >= 0 && sx >= 0) {
+      int p = -1;
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        if (ls.estar == e) p = eid(e);
+      cell[sy][sx][0] = p;
+    }
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       if (!(ls.emask >> e & 1)) continue;
-      int ty = sy + esy[e], tx = sx + esx[e];
+      const int ty = sy + (e == 3 ? 1 : 0), tx = sx + (e == 1 ? 1 : 0);
       if (ty < 0 || ty >= GT_Y || tx < 0 || tx >= GT_X) continue;
       int partner = -1;
       if (e == ls.estar) partner = 4 * q;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        int cj = (code >> (2 * j)) & 3;
-        if (cj && face_edge(j, cj - 1) == e) partner = fid[j];
+        const int cj = (code >> (2 * j)) & 3;
+        if (cj && face_edge(j, cj - 1) == e) partner = fid(j);
       }
       cell[ty][tx][e < 2 ? 1 : 2] = partner;
     }
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       if (!(ls.fmask >> j & 1)) continue;
-      int ty = sy + fsy[j], tx = sx + fsx[j];
+      const int ty = sy + (j >> 1), tx = sx + (j & 1);
       if (ty < 0 || ty >= GT_Y || tx < 0 || tx >= GT_X) continue;
-      int cj = (code >> (2 * j)) & 3;
-      cell[ty][tx][3] = cj ? eid[face_edge(j, cj - 1)] : -1;
+      const int cj = (code >> (2 * j)) & 3;
+      int partner = -1;
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        if (cj && face_edge(j, cj - 1) == e) partner = eid(e);
+      cell[ty][tx][3] = partner;
     }
   }
   __syncthreads();

@@ -34,6 +34,22 @@ __device__ __forceinline__ int uf_find(int32_t* __restrict__ p, int x, int32_t* 
   }
 }
 
+__device__ __forceinline__ void persist_find_one(const PersistSide& s, int i, uint64_t tag) {
+  const int ra = uf_find(s.par_g, s.c[2 * i], s.err);
+  const int rb = uf_find(s.par_g, s.c[2 * i + 1], s.err);
+  if (ra == rb) {
+    s.pair[i] = -1;
+    s.pers[i] = -1.0f;
+    s.rr[2 * i] = -2;
+    return;
+  }
+  s.rr[2 * i] = ra;
+  s.rr[2 * i + 1] = rb;
+  const uint64_t key = tag | (uint32_t)s.pos[i];
+  if (ra >= 0) atomicMin((unsigned long long*)&s.minkey[ra], (unsigned long long)key);
+  if (rb >= 0) atomicMin((unsigned long long*)&s.minkey[rb], (unsigned long long)key);
+}
+
 __global__ void k_persist_find(PersistSide s0, PersistSide s1, int round) {
   const int cur = round & 1;
   const int n0 = s0.cnt[cur], n1 = s1.cnt[cur];
@@ -43,22 +59,9 @@ __global__ void k_persist_find(PersistSide s0, PersistSide s1, int round) {
   }
   const uint64_t tag = (uint64_t)(0xFFFFFFFFu - (uint32_t)round) << 32;
   for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n0 + n1; j += (int64_t)gridDim.x * blockDim.x) {
-    const bool side1 = j >= n0;
-    const PersistSide& s = side1 ? s1 : s0;
-    const int i = s.wl[cur][side1 ? j - n0 : j];  // saddle index (worklists run in cell-id order)
-    const int ra = uf_find(s.par_g, s.c[2 * i], s.err);
-    const int rb = uf_find(s.par_g, s.c[2 * i + 1], s.err);
-    if (ra == rb) {
-      s.pair[i] = -1;
-      s.pers[i] = -1.0f;
-      s.rr[2 * i] = -2;
-      continue;
-    }
-    s.rr[2 * i] = ra;
-    s.rr[2 * i + 1] = rb;
-    const uint64_t key = tag | (uint32_t)s.pos[i];
-    if (ra >= 0) atomicMin((unsigned long long*)&s.minkey[ra], (unsigned long long)key);
-    if (rb >= 0) atomicMin((unsigned long long*)&s.minkey[rb], (unsigned long long)key);
+    // worklists hold saddle indices (cell-id order); the sweep position is the merge key
+    if (j >= n0) persist_find_one(s1, s1.wl[cur][j - n0], tag);
+    else persist_find_one(s0, s0.wl[cur][j], tag);
   }
 }
 
@@ -125,9 +128,14 @@ __global__ void k_persist_apply(PersistSide s0, PersistSide s1, int round, float
     bool wait = false;
     int k = 0;
     if (jj < (int64_t)n0 + n1) {
-      const PersistSide& s = side1 ? s1 : s0;
-      k = s.wl[cur][side1 ? jj - n0 : jj];
-      wait = persist_apply_one<DUAL>(s, k, tag, thr, removed);
+      // two inlined copies instead of selecting a struct at run time (which would spill it)
+      if (side1) {
+        k = s1.wl[cur][jj - n0];
+        wait = persist_apply_one<DUAL>(s1, k, tag, thr, removed);
+      } else {
+        k = s0.wl[cur][jj];
+        wait = persist_apply_one<DUAL>(s0, k, tag, thr, removed);
+      }
     }
     const int p0 = wave_append(&s0.cnt[cur ^ 1], wait && !side1);
     const int p1 = wave_append(&s1.cnt[cur ^ 1], wait && side1);

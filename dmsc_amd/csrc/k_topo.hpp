@@ -107,6 +107,6 @@ void launch_expand_segments(const ArcSide& a, const GeoState& g, const uint64_t*
                             Seg* out, int32_t* nout, Seg* raw, int32_t* nraw, int64_t max_in, int64_t cap,
                             hipStream_t st);
 void launch_copy_raw(const ArcSide& a, const Seg* raw, const int32_t* nraw, int64_t max_raw, int32_t* out,
-                     int64_t out_total, int32_t* err, hipStream_t st);
+                     int64_t out_total, int32_t* err, int32_t* longlist, int32_t* nlong, hipStream_t st);
 
 }  // namespace dmsc
