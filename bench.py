@@ -43,7 +43,24 @@ def parse():
     p.add_argument("--workload", default="gauss4096", choices=["gauss4096", "noise8192", "mixed"])
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     p.add_argument("--profile", action="store_true", help="print per-kernel event timings to stderr")
+    p.add_argument("--copy-calibration", action="store_true",
+                   help="only run plain 256 MiB device-to-device copies (reference kernel for PMC byte counters)")
     return p.parse_args()
+
+
+def copy_calibration():
+    """Five copies of 256 MiB between two device buffers: each moves exactly 256 MiB in and out, so
+    FETCH_SIZE / WRITE_SIZE of these dispatches calibrate the counters' units."""
+    import torch
+
+    n = 64 * 1024 * 1024  # float32 elements = 256 MiB
+    a = torch.ones(n, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_per_dispatch": 4 * n, "dispatches": 5}))
 
 
 def workload(name):
@@ -70,6 +87,9 @@ def out_bytes(ms):
 
 def main():
     args = parse()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
               file=sys.stderr)
@@ -105,6 +125,7 @@ def main():
             ms = dmsc_amd.compute_dmsc(copies[s][j], thr, return_gradient=True, is_dual=dual)
             if record is not None:
                 record.append((copies[s][j].shape, out_bytes(ms)))
+                counts.append([int(v) for v in dmsc_gpu.debug_fetch(0, "counts", 3).tolist()])
 
     for s in range(args.warmup):
         step(s)
@@ -112,6 +133,7 @@ def main():
 
     dmsc_gpu.profile_enable(0, True)
     sizes = []
+    counts = []  # (minima, saddles, maxima) per extraction of the first timed step
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for s in range(args.warmup, total_steps):
@@ -134,17 +156,40 @@ def main():
             print(f"  {k:20s} total {ms_tot:9.3f} ms  launches {n:6d}  avg {ms_tot / max(n, 1) * 1e3:9.1f} us",
                   file=sys.stderr)
     dom = ranked[0][0] if ranked else None
+    total_ms = max(sum(v[0] for v in prof.values()), 1e-9)
+    n_extr = args.steps * n_ext
     roof = None
+    pers_ms, _ = prof.get("persist_round", (0.0, 0))
+    if pers_ms and counts:
+        # Persistence sweep (the dominant stage), one sweep per extraction. Compulsory bytes per sweep:
+        # per saddle and side 20 B (two extremum indices, saddle value in; pair, persistence out),
+        # per extremum 5 B (value in, alive flag out).
+        b_sweep = sum(2 * 20 * ns + 5 * (nmi + nma) for nmi, ns, nma in counts) / len(counts)
+        sweep_s = pers_ms / n_extr / 1e3
+        ach = b_sweep / sweep_s / 1e9
+        roof = {"bound": "hbm", "kernel": "persistence sweep (k_persist_find + k_persist_apply rounds)",
+                "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 5),
+                "traffic": None, "bytes_per_launch": int(b_sweep), "avg_launch_us": round(sweep_s * 1e6, 1),
+                "launch_unit": "one sweep (all rounds) per extraction",
+                "share_of_kernel_time": round(pers_ms / total_ms, 3), "dominant_kernel": dom}
     grad_ms, grad_n = prof.get("gradient", (0.0, 0))
     if grad_n:
-        # bytes per gradient launch: the field of that launch plus its gradient output
+        # gradient kernel: the field of that launch plus its gradient output
         per_launch = sum(KERNEL_MODELS["gradient"](int(sh[0]), int(sh[1])) for sh, _ in sizes) / len(sizes)
         avg_s = grad_ms / grad_n / 1e3
         ach = per_launch / avg_s / 1e9
-        roof = {"bound": "hbm", "kernel": "gradient", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "avg_launch_us": round(avg_s * 1e6, 2),
-                "bytes_per_launch": int(per_launch), "dominant_kernel": dom,
-                "dominant_share": round(ranked[0][1][0] / max(sum(v[0] for v in prof.values()), 1e-9), 3)}
+        measured = None
+        try:
+            with open(os.path.join(ROOT, "profiles", "traffic.json")) as fh:
+                measured = json.load(fh).get(args.workload, {}).get("gradient", {}).get("traffic_bytes_per_launch")
+        except OSError:
+            pass
+        g = {"achieved": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4), "avg_launch_us": round(avg_s * 1e6, 2),
+             "bytes_per_launch": int(per_launch), "traffic": measured, "share_of_kernel_time": round(grad_ms / total_ms, 4)}
+        if roof is None:
+            roof = {"bound": "hbm", "kernel": "gradient", "peak": HBM_PEAK_GBS, "unit": "GB/s", **g}
+        else:
+            roof["gradient_kernel"] = g
 
     extra = {}
     if mixed:  # the same fields through the plain per-field path, for the batched-vs-plain ratio

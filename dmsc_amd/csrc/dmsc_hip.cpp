@@ -292,7 +292,7 @@ void run_one(dmsc_context* ctx, const float* f, int H, int W, float thr, uint32_
   ctx->pending_log.clear();
   if (nsad > 0) {
     while (true) {
-      const int batch = round < 8 ? 2 : 8;
+      const int batch = round < 4 ? 4 : 8;
       for (int b = 0; b < batch; ++b, ++round) launch_persist_round(dual, ps[0], ps[1], round, thr, removed, nsad, st);
       DMSC_CHECK(hipMemcpyAsync(ctx->pinned, sb[0].cnt + (round & 1), 4, hipMemcpyDeviceToHost, st));
       DMSC_CHECK(hipMemcpyAsync((char*)ctx->pinned + 4, sb[1].cnt + (round & 1), 4, hipMemcpyDeviceToHost, st));
@@ -410,17 +410,34 @@ void run_one(dmsc_context* ctx, const float* f, int H, int W, float thr, uint32_
     iota_i32(g.wl[0], nc, st);
     fill_i32(g.cnt, (int32_t)nc, 1, st);
     fill_i32(g.cnt + 1, 0, 1, st);
+  }
+  // geometry rounds of both sides, interleaved, one readback per batch of rounds
+  {
+    bool live[2] = {aw[0].on && aw[0].ncancel > 0, aw[1].on && aw[1].ncancel > 0};
     int r = 0;
-    if (nc > 0) {
-      while (true) {
-        const int batch = r < 8 ? 2 : 8;
-        for (int k = 0; k < batch; ++k, ++r) launch_geo_round(w.A, g, r, nc, st);
-        readback(ctx, g.cnt + (r & 1), 4, st);
-        if (*(const int32_t*)ctx->pinned == 0) break;
-        if (r > 4 * nc + 64) throw Fail{DMSC_ERR_INTERNAL, "geometry rounds did not converge"};
-      }
+    ctx->rounds_geo[0] = ctx->rounds_geo[1] = 0;
+    while (live[0] || live[1]) {
+      const int batch = r < 4 ? 4 : 8;
+      for (int k = 0; k < batch; ++k, ++r)
+        for (int s = 0; s < 2; ++s)
+          if (live[s]) launch_geo_round(aw[s].A, aw[s].G, r, aw[s].ncancel, st);
+      for (int s = 0; s < 2; ++s)
+        if (live[s]) DMSC_CHECK(hipMemcpyAsync((char*)ctx->pinned + 4 * s, aw[s].G.cnt + (r & 1), 4,
+                                               hipMemcpyDeviceToHost, st));
+      DMSC_CHECK(hipStreamSynchronize(st));
+      for (int s = 0; s < 2; ++s)
+        if (live[s]) {
+          ctx->rounds_geo[s] = r;
+          if (((const int32_t*)ctx->pinned)[s] == 0) live[s] = false;
+          else if (r > 4 * aw[s].ncancel + 64) throw Fail{DMSC_ERR_INTERNAL, "geometry rounds did not converge"};
+        }
     }
-    ctx->rounds_geo[s] = r;
+  }
+  for (int s = 0; s < 2; ++s) {
+    if (!aw[s].on) continue;
+    ArcWork& w = aw[s];
+    GeoState& g = w.G;
+    const int64_t next = s ? nmax : nmin;
     // path lengths to the final roots and per-output-arc lengths (min order)
     Arena& a2 = ctx->ar2;
     const int hb = 16 * s;
