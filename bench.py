@@ -12,7 +12,8 @@ the end-to-end algorithmic HBM rate, and the CPU oracle timed on a bounded sampl
 workload on this host.
 
 Other workloads (--workload): noise8192 (8192^2 uniform noise quantised to 256 levels, thr 0.05),
-mixed (512 fields of sizes 64..1024 with per-field thresholds, looped per field).
+mixed (512 fields of sizes 64..1024 with per-field thresholds: one batched call per step, with the
+per-field loop of the plain path timed beside it for the batched/plain ratio).
 """
 
 import argparse
@@ -115,29 +116,35 @@ def main():
             copies[s] = [fields[0].to(dev) for _ in range(n_ext)]
     torch.cuda.synchronize()
 
-    def step(s, record=None, plain=False):
+    def step(s, plain=False):
         if mixed and not plain:  # one batched call over all fields (per-field thresholds)
-            res = dmsc_amd.compute_dmsc_batch(copies[s], [t for _, t in runs], return_gradient=True)
-            if record is not None:
-                record.extend((copies[s][j].shape, out_bytes(ms)) for j, ms in enumerate(res))
-            return
-        for j, (dual, thr) in enumerate(runs):
-            ms = dmsc_amd.compute_dmsc(copies[s][j], thr, return_gradient=True, is_dual=dual)
-            if record is not None:
-                record.append((copies[s][j].shape, out_bytes(ms)))
-                counts.append([int(v) for v in dmsc_gpu.debug_fetch(0, "counts", 3).tolist()])
+            return [dmsc_amd.compute_dmsc_batch(copies[s], [t for _, t in runs], return_gradient=True)]
+        return [dmsc_amd.compute_dmsc(copies[s][j], thr, return_gradient=True, is_dual=dual)
+                for j, (dual, thr) in enumerate(runs)]
 
-    for s in range(args.warmup):
-        step(s)
+    # bookkeeping (output sizes, critical-cell counts per call) on the first warmup step, untimed
+    sizes = []   # (shape, output bytes) per extracted field
+    counts = []  # (minima, saddles, maxima) per call (a batched call counts all its fields)
+    for s in range(max(args.warmup, 1)):
+        if mixed and not s:
+            res = dmsc_amd.compute_dmsc_batch(copies[s], [t for _, t in runs], return_gradient=True)
+            sizes = [(copies[s][j].shape, out_bytes(ms)) for j, ms in enumerate(res)]
+            counts.append([int(v) for v in dmsc_gpu.debug_fetch(0, "counts", 3).tolist()])
+        elif not s:
+            for j, (dual, thr) in enumerate(runs):
+                ms = dmsc_amd.compute_dmsc(copies[s][j], thr, return_gradient=True, is_dual=dual)
+                sizes.append((copies[s][j].shape, out_bytes(ms)))
+                counts.append([int(v) for v in dmsc_gpu.debug_fetch(0, "counts", 3).tolist()])
+        else:
+            step(s)
     torch.cuda.synchronize()
+    calls_per_step = len(counts)
 
     dmsc_gpu.profile_enable(0, True)
-    sizes = []
-    counts = []  # (minima, saddles, maxima) per extraction of the first timed step
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for s in range(args.warmup, total_steps):
-        step(s, sizes if s == args.warmup else None)
+        step(s)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     prof = dmsc_gpu.profile_read(0)
@@ -157,25 +164,26 @@ def main():
                   file=sys.stderr)
     dom = ranked[0][0] if ranked else None
     total_ms = max(sum(v[0] for v in prof.values()), 1e-9)
-    n_extr = args.steps * n_ext
+    n_calls = args.steps * calls_per_step
     roof = None
     pers_ms, _ = prof.get("persist_round", (0.0, 0))
     if pers_ms and counts:
-        # Persistence sweep (the dominant stage), one sweep per extraction. Compulsory bytes per sweep:
-        # per saddle and side 20 B (two extremum indices, saddle value in; pair, persistence out),
-        # per extremum 5 B (value in, alive flag out).
+        # Persistence sweep (all rounds of one call). Compulsory bytes per sweep: per saddle and
+        # side 20 B (two extremum indices, saddle value in; pair, persistence out), per extremum
+        # 5 B (value in, alive flag out).
         b_sweep = sum(2 * 20 * ns + 5 * (nmi + nma) for nmi, ns, nma in counts) / len(counts)
-        sweep_s = pers_ms / n_extr / 1e3
+        sweep_s = pers_ms / n_calls / 1e3
         ach = b_sweep / sweep_s / 1e9
         roof = {"bound": "hbm", "kernel": "persistence sweep (k_persist_find + k_persist_apply rounds)",
                 "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 5),
                 "traffic": None, "bytes_per_launch": int(b_sweep), "avg_launch_us": round(sweep_s * 1e6, 1),
-                "launch_unit": "one sweep (all rounds) per extraction",
+                "launch_unit": "one sweep (all rounds) per call",
                 "share_of_kernel_time": round(pers_ms / total_ms, 3), "dominant_kernel": dom}
     grad_ms, grad_n = prof.get("gradient", (0.0, 0))
     if grad_n:
-        # gradient kernel: the field of that launch plus its gradient output
-        per_launch = sum(KERNEL_MODELS["gradient"](int(sh[0]), int(sh[1])) for sh, _ in sizes) / len(sizes)
+        # gradient kernel: all fields of a launch (one field per plain call, the whole batch per
+        # batched call) plus their gradient output
+        per_launch = sum(KERNEL_MODELS["gradient"](int(sh[0]), int(sh[1])) for sh, _ in sizes) * args.steps / grad_n
         avg_s = grad_ms / grad_n / 1e3
         ach = per_launch / avg_s / 1e9
         measured = None

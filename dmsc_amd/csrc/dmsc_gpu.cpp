@@ -31,16 +31,50 @@ const char* kNames[DMSC_NUM_OUTPUTS] = {
 
 struct AllocUser {
   torch::Device dev;
-  DMSComplex* res;  // one per field; the batch entry fills them from several threads (distinct fields)
+  DMSComplex* res;                    // one per field
+  std::vector<torch::Tensor> shared;  // buffers holding one slot of several fields (field = -1)
 };
+
+torch::TensorOptions opts_for(int32_t dtype, torch::Device dev) {
+  return torch::TensorOptions().dtype(dtype == DMSC_DTYPE_FLOAT32 ? torch::kFloat32 : torch::kInt32).device(dev);
+}
 
 void* alloc_cb(void* user, int32_t field, int32_t slot, int32_t ndim, const int64_t* dims, int32_t dtype) {
   auto* a = static_cast<AllocUser*>(user);
-  auto opts = torch::TensorOptions().dtype(dtype == DMSC_DTYPE_FLOAT32 ? torch::kFloat32 : torch::kInt32).device(a->dev);
   std::vector<int64_t> shape(dims, dims + ndim);
+  if (field < 0) {
+    a->shared.push_back(torch::empty(shape, opts_for(dtype, a->dev)));
+    return a->shared.back().data_ptr();
+  }
   torch::Tensor& t = a->res[field].t[slot];
-  t = torch::empty(shape, opts);
+  t = torch::empty(shape, opts_for(dtype, a->dev));
   return t.data_ptr();
+}
+
+// Outputs that live in a shared buffer become views of it (each field's slice, in its own shape).
+void attach_views(AllocUser& u, const dmsc_outputs& out, DMSComplex& r) {
+  for (int k = 1; k < DMSC_NUM_OUTPUTS; ++k) {
+    if (r.t[k].defined()) continue;
+    std::vector<int64_t> shape(out.dims[k], out.dims[k] + out.ndim[k]);
+    int64_t numel = 1;
+    for (int64_t d : shape) numel *= d;
+    if (numel == 0 || !out.data[k]) {
+      r.t[k] = torch::empty(shape, opts_for(out.dtype[k], u.dev));
+      continue;
+    }
+    const char* p = static_cast<const char*>(out.data[k]);
+    bool found = false;
+    for (const torch::Tensor& buf : u.shared) {
+      const char* b = static_cast<const char*>(buf.data_ptr());
+      const int64_t bytes = buf.numel() * buf.element_size();
+      if (b && p >= b && p < b + bytes) {
+        r.t[k] = buf.narrow(0, (p - b) / buf.element_size(), numel).view(shape);
+        found = true;
+        break;
+      }
+    }
+    TORCH_CHECK(found, "dmsc: output slot ", k, " does not point into an allocated buffer");
+  }
 }
 
 torch::Tensor shape_tensor(const dmsc_outputs& out) {
@@ -82,7 +116,7 @@ uint32_t make_flags(bool grad, bool dual, bool tmax, bool tmin, bool gmax, bool 
 
 DMSComplex run_field(dmsc_context* ctx, const torch::Tensor& f, float thr, uint32_t flags, hipStream_t stream) {
   DMSComplex r;
-  AllocUser u{f.device(), &r};
+  AllocUser u{f.device(), &r, {}};
   dmsc_outputs out;
   dmsc_status st;
   {
@@ -95,19 +129,13 @@ DMSComplex run_field(dmsc_context* ctx, const torch::Tensor& f, float thr, uint3
   return r;
 }
 
-// Fields of one device processed by the C-ABI batch entry (concurrent worker streams).
-std::vector<DMSComplex> run_batch(dmsc_context* ctx, const std::vector<torch::Tensor>& fields,
+// Fields of one device processed together by the C-ABI batch entry (one pipeline for all fields).
+std::vector<DMSComplex> run_batch(dmsc_context* ctx, torch::Device dev, const std::vector<const float*>& ptrs,
+                                  const std::vector<int32_t>& hs, const std::vector<int32_t>& ws,
                                   const std::vector<float>& thr, uint32_t flags, hipStream_t stream) {
-  const int B = (int)fields.size();
+  const int B = (int)ptrs.size();
   std::vector<DMSComplex> res(B);
-  std::vector<const float*> ptrs(B);
-  std::vector<int32_t> hs(B), ws(B);
-  for (int b = 0; b < B; ++b) {
-    ptrs[b] = fields[b].data_ptr<float>();
-    hs[b] = (int32_t)fields[b].size(0);
-    ws[b] = (int32_t)fields[b].size(1);
-  }
-  AllocUser u{fields[0].device(), res.data()};
+  AllocUser u{dev, res.data(), {}};
   std::vector<dmsc_outputs> outs(B);
   dmsc_status st;
   {
@@ -116,7 +144,10 @@ std::vector<DMSComplex> run_batch(dmsc_context* ctx, const std::vector<torch::Te
                             alloc_cb, &u, outs.data());
   }
   TORCH_CHECK(st == DMSC_OK, dmsc_last_error(ctx));
-  for (int b = 0; b < B; ++b) res[b].t[0] = shape_tensor(outs[b]);
+  for (int b = 0; b < B; ++b) {
+    attach_views(u, outs[b], res[b]);
+    res[b].t[0] = shape_tensor(outs[b]);
+  }
   return res;
 }
 
@@ -133,10 +164,13 @@ pybind11::object extract_dmsc(torch::Tensor scalar_field, float persistence_thre
   const uint32_t flags = make_flags(return_gradient, is_dual, trace_max_arcs, trace_min_arcs, trace_max_groups,
                                     trace_min_groups);
   if (scalar_field.dim() == 2) return pybind11::cast(run_field(ctx, scalar_field, persistence_threshold, flags, stream));
-  std::vector<torch::Tensor> fields;
-  for (int64_t b = 0; b < scalar_field.size(0); ++b) fields.push_back(scalar_field[b]);
-  std::vector<float> thr(fields.size(), persistence_threshold);
-  return pybind11::cast(run_batch(ctx, fields, thr, flags, stream));
+  const int64_t B = scalar_field.size(0), H = scalar_field.size(1), W = scalar_field.size(2);
+  TORCH_CHECK(B <= std::numeric_limits<int32_t>::max(), "batch too large");
+  std::vector<const float*> ptrs(B);
+  for (int64_t b = 0; b < B; ++b) ptrs[b] = scalar_field.data_ptr<float>() + b * H * W;
+  std::vector<int32_t> hs(B, (int32_t)H), ws(B, (int32_t)W);
+  std::vector<float> thr(B, persistence_threshold);
+  return pybind11::cast(run_batch(ctx, scalar_field.device(), ptrs, hs, ws, thr, flags, stream));
 }
 
 // Mixed-size batch: a list of 2-D fields with one threshold each (no reference counterpart; the
@@ -159,9 +193,16 @@ std::vector<DMSComplex> extract_dmsc_list(std::vector<torch::Tensor> fields, std
   const uint32_t flags = make_flags(return_gradient, is_dual, trace_max_arcs, trace_min_arcs, trace_max_groups,
                                     trace_min_groups);
   std::vector<torch::Tensor> contig;
-  for (auto& f : fields) contig.push_back(f.contiguous());
+  std::vector<const float*> ptrs;
+  std::vector<int32_t> hs, ws;
+  for (auto& f : fields) {
+    contig.push_back(f.contiguous());
+    ptrs.push_back(contig.back().data_ptr<float>());
+    hs.push_back((int32_t)f.size(0));
+    ws.push_back((int32_t)f.size(1));
+  }
   std::vector<float> thr(thresholds.begin(), thresholds.end());
-  return run_batch(ctx, contig, thr, flags, stream);
+  return run_batch(ctx, dev, ptrs, hs, ws, thr, flags, stream);
 }
 
 pybind11::object debug_fetch(int device, const std::string& name, int64_t capacity, bool as_float, bool as_u64) {

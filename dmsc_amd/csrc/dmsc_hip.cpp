@@ -33,7 +33,7 @@ enum : int {
   A_FSUCC, A_VSUCC, A_PJF, A_PJV, A_PLANES, A_PREMIN, A_PRESAD, A_PREMAX,
   A_C32, A_C64,
   A_MINIDS, A_MINVAL, A_SADIDS, A_SADVAL, A_MAXIDS, A_MAXVAL,
-  A_CMAX, A_CMIN, A_LMAX, A_LMIN,
+  A_CMAX, A_CMIN, A_LMAX, A_LMIN, A_META, A_BASES32, A_SADFIELD,
   A_KEYS, A_KEYS2, A_VALS2, A_ASC, A_ORDMIN, A_ORDMAX, A_REMOVED, A_POSMIN, A_POSMAX,
   A_SIDE0,  // per persistence side: 13 slots each
   A_SIDE_END = A_SIDE0 + 2 * 13,
@@ -48,7 +48,7 @@ enum ArcSlot {
   G_BLEN, G_BOFF, G_ARENA, G_FLAG, G_PRE, G_CS0, G_CD0, G_KEYS, G_SLOT, G_KEYS2, G_SLOT2, G_CS, G_CD, G_DEADC,
   G_GPAR, G_LLEN, G_LC, G_LKD, G_DONE, G_WATCH, G_RR, G_WL0, G_WL1, G_CNT
 };
-enum ArcSlot2 { H_PJQ = 0, H_FLEN, H_PERPOS, H_POSPRE, H_SEGA, H_SEGB, H_RAW, H_NSEG, H_LONG };
+enum ArcSlot2 { H_PJQ = 0, H_FLEN, H_PERPOS, H_POSPRE, H_SEGA, H_SEGB, H_RAW, H_NSEG, H_CUMR, H_CUMC, H_ARCB };
 
 }  // namespace dmsc
 
@@ -64,9 +64,10 @@ struct dmsc_context {
   Arena ar;
   Arena ar2;  // arc emission scratch
   int64_t* pinned = nullptr;
+  size_t pinned_cap = 0;
   std::string err;
   // last-call record for dmsc_debug_fetch
-  Dims dims{};
+  int64_t S = 0, N = 0;
   int64_t nmin = 0, nsad = 0, nmax = 0;
   int rounds_persist = 0, rounds_geo[2] = {0, 0};
   std::vector<int32_t> pending_log;  // (round, pending min side, pending max side) at each readback
@@ -75,18 +76,7 @@ struct dmsc_context {
   const int32_t *cmax = nullptr, *cmin = nullptr, *pair_min = nullptr, *pair_max = nullptr;
   const float *pers_min = nullptr, *pers_max = nullptr;
   const uint64_t *pjf = nullptr, *pjv = nullptr;
-  // batch workers (own scratch + stream each)
-  std::vector<std::unique_ptr<dmsc_context>> workers;
-  hipStream_t stream = nullptr;
 };
-
-namespace {
-int batch_workers() {
-  const char* e = std::getenv("DMSC_BATCH_WORKERS");
-  int n = e ? std::atoi(e) : 8;
-  return n < 1 ? 1 : (n > 64 ? 64 : n);
-}
-}  // namespace
 
 namespace {
 
@@ -155,21 +145,91 @@ struct ArcWork {
   int64_t arena_total = 0, ncancel = 0, out_total = 0;
   int64_t* flen = nullptr;
   int64_t* pos_pre = nullptr;
+  int64_t* arc_base = nullptr;  // [B+1] device
   uint64_t* pjq = nullptr;
+  uint64_t* cum_r = nullptr;  // per extremum: raw / chain pieces of the links on its path to the root
+  uint64_t* cum_c = nullptr;
   int32_t* out_cells = nullptr;
   int32_t* out_off = nullptr;
   int32_t* out_arc_off = nullptr;
+  std::vector<int64_t> h_arc_base;
 };
 
-void run_one(dmsc_context* ctx, const float* f, int H, int W, float thr, uint32_t flags, hipStream_t st,
-             dmsc_alloc_fn alloc, void* user, int field, dmsc_outputs* out) {
-  if (!f) throw Fail{DMSC_ERR_INVALID_ARGUMENT, "Input tensor must be defined"};
-  if (H <= 0 || W <= 0)
+// One field of a call, with the caller's index (the allocator's `field` argument).
+struct FieldIn {
+  const float* f;
+  int H, W;
+  float thr;
+  int index;
+};
+
+void validate_field(const FieldIn& in) {
+  if (!in.f) throw Fail{DMSC_ERR_INVALID_ARGUMENT, "Input tensor must be defined"};
+  if (in.H <= 0 || in.W <= 0)
     throw Fail{DMSC_ERR_INVALID_ARGUMENT,
-               "Spatial dimensions must be non-empty, got " + std::to_string(H) + "x" + std::to_string(W)};
-  if ((int64_t)H + 1 > (int64_t)INT32_MAX / 4 / ((int64_t)W + 1))
+               "Spatial dimensions must be non-empty, got " + std::to_string(in.H) + "x" + std::to_string(in.W)};
+  if ((int64_t)in.H + 1 > (int64_t)INT32_MAX / 4 / ((int64_t)in.W + 1))
     throw Fail{DMSC_ERR_TOO_LARGE,
-               "Input is too large for 32-bit cell IDs: " + std::to_string(H) + "x" + std::to_string(W)};
+               "Input is too large for 32-bit cell IDs: " + std::to_string(in.H) + "x" + std::to_string(in.W)};
+}
+
+void ensure_pinned(dmsc_context* c, size_t bytes) {
+  if (bytes <= c->pinned_cap) return;
+  if (c->pinned) DMSC_CHECK(hipHostFree(c->pinned));
+  c->pinned = nullptr;
+  c->pinned_cap = 0;
+  size_t cap = std::max<size_t>(bytes + bytes / 2, 4096);
+  DMSC_CHECK(hipHostMalloc((void**)&c->pinned, cap));
+  c->pinned_cap = cap;
+}
+
+constexpr size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// Output allocation for a group of fields. A one-field group gets the field's own arrays in their
+// final shapes; a larger group gets one buffer per slot (allocator called with field = -1) holding
+// the fields' arrays back to back in field order, and each field's output points into it.
+struct OutAlloc {
+  dmsc_alloc_fn alloc;
+  void* user;
+  const FieldIn* fin;
+  dmsc_outputs* outs;
+  int B;
+  // numel(f) and shape of slot k for field f
+  void* slot(int k, int ndim, const std::function<std::pair<int64_t, int64_t>(int)>& shape, int dtype) {
+    if (B == 1) {
+      auto sh = shape(0);
+      return alloc_out(alloc, user, fin[0].index, &outs[fin[0].index], k, ndim, sh.first, sh.second, dtype);
+    }
+    std::vector<int64_t> numel(B);
+    int64_t total = 0;
+    for (int f = 0; f < B; ++f) {
+      auto sh = shape(f);
+      numel[f] = ndim == 2 ? sh.first * sh.second : sh.first;
+      dmsc_outputs* o = &outs[fin[f].index];
+      o->ndim[k] = ndim;
+      o->dims[k][0] = sh.first;
+      o->dims[k][1] = ndim == 2 ? sh.second : 0;
+      o->dtype[k] = dtype;
+      total += numel[f];
+    }
+    int64_t dims[2] = {total, 0};
+    char* p = (char*)alloc(user, -1, k, 1, dims, dtype);
+    if (!p && total > 0) throw Fail{DMSC_ERR_ALLOC, "output allocator returned NULL for slot " + std::to_string(k)};
+    int64_t off = 0;
+    for (int f = 0; f < B; ++f) {
+      outs[fin[f].index].data[k] = numel[f] > 0 ? p + 4 * off : nullptr;
+      off += numel[f];
+    }
+    return p;
+  }
+};
+
+// All fields of a group go through the same launches: per-cell arrays are the concatenation of the
+// fields' arrays, critical-cell lists are concatenated in field order (ids stay field-local), and
+// the persistence and geometry rounds run for every field at once (a saddle only ever meets
+// extrema of its own field, so fields never interact).
+void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hipStream_t st, dmsc_alloc_fn alloc,
+               void* user, dmsc_outputs* outs) {
   const bool dual = flags & DMSC_DUAL, grad = flags & DMSC_RETURN_GRADIENT;
   bool tmax = flags & DMSC_TRACE_MAX_ARCS, tmin = flags & DMSC_TRACE_MIN_ARCS;
   bool gmax = flags & DMSC_TRACE_MAX_GROUPS, gmin = flags & DMSC_TRACE_MIN_GROUPS;
@@ -178,63 +238,128 @@ void run_one(dmsc_context* ctx, const float* f, int H, int W, float thr, uint32_
     std::swap(gmax, gmin);
   }
   Arena& ar = ctx->ar;
-  const Dims d = make_dims(H, W);
-  std::memset(out, 0, sizeof(*out));
-  out->shape[0] = H;
-  out->shape[1] = W;
-  out->ndim[0] = 1;
-  out->dims[0][0] = 2;
+
+  // ---- field table, bases and tiles
+  std::vector<FieldDesc> fd(B);
+  std::vector<Dims> dims(B);
+  std::vector<int64_t> slot_b(B + 1), pix_b(B + 1), word_b(B + 1);
+  std::vector<float> thr(B);
+  std::vector<TileRef> tiles;
+  int64_t S = 0, N = 0, PW = 0;
+  for (int f = 0; f < B; ++f) {
+    const Dims d = make_dims(fin[f].H, fin[f].W);
+    dims[f] = d;
+    fd[f] = FieldDesc{fin[f].f, d.H, d.W, d.Nx, d.Ny, d.WPR, fin[f].thr, N, S, PW};
+    slot_b[f] = S;
+    pix_b[f] = N;
+    word_b[f] = PW;
+    thr[f] = fin[f].thr;
+    for (int ty = 0; ty < ceil_div(d.Ny, 16); ++ty)
+      for (int tx = 0; tx < ceil_div(d.Nx, 64); ++tx) tiles.push_back(TileRef{f, tx, ty});
+    S += d.S;
+    N += d.N;
+    PW += d.PW;
+  }
+  slot_b[B] = S;
+  pix_b[B] = N;
+  word_b[B] = PW;
+  if (S > (int64_t)INT32_MAX - 1) throw Fail{DMSC_ERR_TOO_LARGE, "batch group exceeds 32-bit cell indices"};
+
+  const size_t o_fd = 0, o_sb = align_up(o_fd + sizeof(FieldDesc) * B, 16),
+               o_pb = align_up(o_sb + 8 * (B + 1), 16), o_wb = align_up(o_pb + 8 * (B + 1), 16),
+               o_thr = align_up(o_wb + 8 * (B + 1), 16), o_tl = align_up(o_thr + 4 * B, 16),
+               meta_bytes = align_up(o_tl + sizeof(TileRef) * tiles.size(), 16);
+  ensure_pinned(ctx, std::max<size_t>(meta_bytes, 64 * (size_t)(B + 1) + 1024));
+  {
+    char* h = (char*)ctx->pinned;
+    std::memcpy(h + o_fd, fd.data(), sizeof(FieldDesc) * B);
+    std::memcpy(h + o_sb, slot_b.data(), 8 * (B + 1));
+    std::memcpy(h + o_pb, pix_b.data(), 8 * (B + 1));
+    std::memcpy(h + o_wb, word_b.data(), 8 * (B + 1));
+    std::memcpy(h + o_thr, thr.data(), 4 * B);
+    if (!tiles.empty()) std::memcpy(h + o_tl, tiles.data(), sizeof(TileRef) * tiles.size());
+  }
+  char* meta = ar.get<char>(A_META, meta_bytes);
+  DMSC_CHECK(hipMemcpyAsync(meta, ctx->pinned, meta_bytes, hipMemcpyHostToDevice, st));
+  Batch bt{(const FieldDesc*)(meta + o_fd), (const int64_t*)(meta + o_sb), (const int64_t*)(meta + o_pb),
+           (const int64_t*)(meta + o_wb), B};
+  const float* d_thr = (const float*)(meta + o_thr);
+  const TileRef* d_tiles = (const TileRef*)(meta + o_tl);
+
+  for (int f = 0; f < B; ++f) {
+    dmsc_outputs* o = &outs[fin[f].index];
+    std::memset(o, 0, sizeof(*o));
+    o->shape[0] = fin[f].H;
+    o->shape[1] = fin[f].W;
+    o->ndim[0] = 1;
+    o->dims[0][0] = 2;
+  }
+  OutAlloc oa{alloc, user, fin, outs, B};
 
   // outputs whose sizes are known up front
-  int32_t* o_grad = (int32_t*)alloc_out(alloc, user, field, out,10, 1, grad ? 4 * d.S : 0, 0, DMSC_DTYPE_INT32);
-  int32_t* o_flab = (int32_t*)alloc_out(alloc, user, field, out,slot_labels(true, dual), 2, gmax ? d.Ny : 0,
-                                        gmax ? d.Nx : 0, DMSC_DTYPE_INT32);
-  int32_t* o_vlab = (int32_t*)alloc_out(alloc, user, field, out,slot_labels(false, dual), 2, gmin ? H : 0, gmin ? W : 0,
-                                        DMSC_DTYPE_INT32);
+  int32_t* o_grad = (int32_t*)oa.slot(10, 1, [&](int f) { return std::make_pair(grad ? 4 * dims[f].S : 0, (int64_t)0); },
+                                      DMSC_DTYPE_INT32);
+  int32_t* o_flab = (int32_t*)oa.slot(
+      slot_labels(true, dual), 2,
+      [&](int f) { return gmax ? std::make_pair((int64_t)dims[f].Ny, (int64_t)dims[f].Nx) : std::make_pair(0L, 0L); },
+      DMSC_DTYPE_INT32);
+  int32_t* o_vlab = (int32_t*)oa.slot(
+      slot_labels(false, dual), 2,
+      [&](int f) { return gmin ? std::make_pair((int64_t)dims[f].H, (int64_t)dims[f].W) : std::make_pair(0L, 0L); },
+      DMSC_DTYPE_INT32);
 
   // ---- gradient, critical bit planes, successor forests
-  int32_t* fsucc = ar.get<int32_t>(A_FSUCC, d.S);
-  int32_t* vsucc = ar.get<int32_t>(A_VSUCC, d.N);
-  uint32_t* planes = ar.get<uint32_t>(A_PLANES, 4 * d.PW);
-  launch_gradient(dual, f, d, grad ? o_grad : nullptr, fsucc, vsucc, planes, st);
+  int32_t* fsucc = ar.get<int32_t>(A_FSUCC, S);
+  int32_t* vsucc = ar.get<int32_t>(A_VSUCC, N);
+  uint32_t* planes = ar.get<uint32_t>(A_PLANES, 4 * PW);
+  launch_gradient(dual, bt, d_tiles, (int64_t)tiles.size(), grad ? o_grad : nullptr, fsucc, vsucc, planes, PW, st);
 
   int32_t* c32 = ar.get<int32_t>(A_C32, 64);
   int64_t* c64 = ar.get<int64_t>(A_C64, 64);
   int32_t* d_err = c32 + 40;
   fill_i32(d_err, 0, 1, st);
-  int32_t* pre_min = ar.get<int32_t>(A_PREMIN, d.PW);
-  int32_t* pre_sad = ar.get<int32_t>(A_PRESAD, d.PW);
-  int32_t* pre_max = ar.get<int32_t>(A_PREMAX, d.PW);
-  scan_plane_popcounts(planes, nullptr, pre_min, d.PW, c32 + 0, ar, A_SCAN, st);
-  scan_plane_popcounts(planes + d.PW, planes + 2 * d.PW, pre_sad, d.PW, c32 + 1, ar, A_SCAN, st);
-  scan_plane_popcounts(planes + 3 * d.PW, nullptr, pre_max, d.PW, c32 + 2, ar, A_SCAN, st);
+  int32_t* pre_min = ar.get<int32_t>(A_PREMIN, PW);
+  int32_t* pre_sad = ar.get<int32_t>(A_PRESAD, PW);
+  int32_t* pre_max = ar.get<int32_t>(A_PREMAX, PW);
+  scan_plane_popcounts(planes, nullptr, pre_min, PW, c32 + 0, ar, A_SCAN, st);
+  scan_plane_popcounts(planes + PW, planes + 2 * PW, pre_sad, PW, c32 + 1, ar, A_SCAN, st);
+  scan_plane_popcounts(planes + 3 * PW, nullptr, pre_max, PW, c32 + 2, ar, A_SCAN, st);
+  // per-field list bases: b32 rows 0 min, 1 saddles, 2 max, 3-4 output extrema (min, max side),
+  // 5 output saddles, 6-7 edge rows (min, max side); each row B+1 entries
+  int32_t* b32 = ar.get<int32_t>(A_BASES32, 8 * (size_t)(B + 1));
+  auto row32 = [&](int r) { return b32 + (size_t)r * (B + 1); };
+  bases_at<int32_t, int64_t>(pre_min, bt.word_b, PW, c32 + 0, B, row32(0), st);
+  bases_at<int32_t, int64_t>(pre_sad, bt.word_b, PW, c32 + 1, B, row32(1), st);
+  bases_at<int32_t, int64_t>(pre_max, bt.word_b, PW, c32 + 2, B, row32(2), st);
   readback(ctx, c32, 3 * sizeof(int32_t), st);
   const int32_t* hc = (const int32_t*)ctx->pinned;
   const int64_t nmin = hc[0], nsad = hc[1], nmax = hc[2];
-  const RankView rv{planes, pre_min, pre_sad, pre_max, d.PW, d.WPR, d.Nx, d.W};
+  const RankView rv{planes, pre_min, pre_sad, pre_max, PW, bt};
+  const int32_t* sad_b = row32(1);
 
   int32_t* min_ids = ar.get<int32_t>(A_MINIDS, nmin);
   float* min_val = ar.get<float>(A_MINVAL, nmin);
   int32_t* sad_ids = ar.get<int32_t>(A_SADIDS, nsad);
   float* sad_val = ar.get<float>(A_SADVAL, nsad);
+  int32_t* sad_field = B > 1 ? ar.get<int32_t>(A_SADFIELD, nsad) : nullptr;
   int32_t* max_ids = ar.get<int32_t>(A_MAXIDS, nmax);
   float* max_val = ar.get<float>(A_MAXVAL, nmax);
-  launch_compact_lists(dual, f, d, rv, min_ids, min_val, sad_ids, sad_val, max_ids, max_val, st);
+  launch_compact_lists(dual, rv, min_ids, min_val, sad_ids, sad_val, sad_field, max_ids, max_val, st);
 
   // ---- V-path forests: terminal extremum and depth of every face / vertex
-  uint64_t* pjf = ar.get<uint64_t>(A_PJF, d.S);
-  uint64_t* pjv = ar.get<uint64_t>(A_PJV, d.N);
-  pointer_jump_init(fsucc, pjf, d.S, st);
-  pointer_jump_init(vsucc, pjv, d.N, st);
-  pointer_jump(pjf, d.S, d_err, st);
-  pointer_jump(pjv, d.N, d_err, st);
+  uint64_t* pjf = ar.get<uint64_t>(A_PJF, S);
+  uint64_t* pjv = ar.get<uint64_t>(A_PJV, N);
+  pointer_jump_init(fsucc, pjf, S, st);
+  pointer_jump_init(vsucc, pjv, N, st);
+  pointer_jump(pjf, S, d_err, st);
+  pointer_jump(pjv, N, d_err, st);
 
   // ---- saddle records and sweep orders
   int32_t* cmax = ar.get<int32_t>(A_CMAX, 2 * nsad);
   int32_t* cmin = ar.get<int32_t>(A_CMIN, 2 * nsad);
   int32_t* lmax = ar.get<int32_t>(A_LMAX, 2 * nsad);
   int32_t* lmin = ar.get<int32_t>(A_LMIN, 2 * nsad);
-  launch_saddle_records(d, rv, sad_ids, nsad, pjf, pjv, cmax, cmin, lmax, lmin, st);
+  launch_saddle_records(rv, sad_ids, sad_field, nsad, pjf, pjv, cmax, cmin, lmax, lmin, st);
 
   uint32_t* keys = ar.get<uint32_t>(A_KEYS, nsad);
   uint32_t* keys2 = ar.get<uint32_t>(A_KEYS2, nsad);
@@ -243,13 +368,19 @@ void run_one(dmsc_context* ctx, const float* f, int H, int W, float thr, uint32_
   launch_sort_keys(nsad, sad_val, keys, st);
   iota_i32(asc, nsad, st);
   radix_sort_pairs(keys, asc, keys2, vals2, nsad, ar, A_RS0, st);
+  if (B > 1) {  // stable second pass on the field: (field, value, id) order
+    launch_gather_field(nsad, asc, sad_field, keys, st);
+    int bits = 0;
+    while ((1 << bits) < B) ++bits;
+    radix_sort_pairs_bits(keys, asc, keys2, vals2, nsad, bits, ar, A_RS0, st);
+  }
   int32_t* ord_min = ar.get<int32_t>(A_ORDMIN, nsad);
   int32_t* ord_max = ar.get<int32_t>(A_ORDMAX, nsad);
   // primal: min side ascending, max side descending; dual: the reverse (arcs_topology.hxx:178-181)
   int32_t* pos_min = ar.get<int32_t>(A_POSMIN, nsad);
   int32_t* pos_max = ar.get<int32_t>(A_POSMAX, nsad);
-  if (!dual) launch_orders(nsad, asc, ord_min, ord_max, pos_min, pos_max, st);
-  else launch_orders(nsad, asc, ord_max, ord_min, pos_max, pos_min, st);
+  if (!dual) launch_orders(nsad, asc, sad_field, sad_b, ord_min, ord_max, pos_min, pos_max, st);
+  else launch_orders(nsad, asc, sad_field, sad_b, ord_max, ord_min, pos_max, pos_min, st);
 
   // ---- persistence rounds (both sides together)
   uint8_t* removed = ar.get<uint8_t>(A_REMOVED, nsad);
@@ -286,6 +417,8 @@ void run_one(dmsc_context* ctx, const float* f, int H, int W, float thr, uint32_
     p.wl[1] = sb[s].wl1;
     p.cnt = sb[s].cnt;
     p.err = d_err;
+    p.thr = d_thr;
+    p.sfield = sad_field;
     p.maxside = s;
   }
   int round = 0;
@@ -293,7 +426,7 @@ void run_one(dmsc_context* ctx, const float* f, int H, int W, float thr, uint32_
   if (nsad > 0) {
     while (true) {
       const int batch = round < 4 ? 4 : 8;
-      for (int b = 0; b < batch; ++b, ++round) launch_persist_round(dual, ps[0], ps[1], round, thr, removed, nsad, st);
+      for (int b = 0; b < batch; ++b, ++round) launch_persist_round(dual, ps[0], ps[1], round, removed, nsad, st);
       DMSC_CHECK(hipMemcpyAsync(ctx->pinned, sb[0].cnt + (round & 1), 4, hipMemcpyDeviceToHost, st));
       DMSC_CHECK(hipMemcpyAsync((char*)ctx->pinned + 4, sb[1].cnt + (round & 1), 4, hipMemcpyDeviceToHost, st));
       DMSC_CHECK(hipStreamSynchronize(st));
@@ -317,6 +450,8 @@ void run_one(dmsc_context* ctx, const float* f, int H, int W, float thr, uint32_
     const int64_t next = s ? nmax : nmin;
     launch_u8_to_i32(sb[s].alive, tmp_i32, next, false, st);
     scan_exclusive<int32_t>(tmp_i32, sb[s].to_out, next, c32 + 3 + s, ar, A_SCAN, st);
+    // first output extremum of each field (sampled before dead extrema are masked)
+    bases_at<int32_t, int32_t>(sb[s].to_out, row32(s ? 2 : 0), next, c32 + 3 + s, B, row32(3 + s), st);
     launch_mask_prefix(sb[s].alive, sb[s].to_out, next, st);
   }
   launch_side_counts(nsad, ord_min, removed, cmin, sb[0].root, sb[0].to_out, surv, rows[0], st);
@@ -324,6 +459,10 @@ void run_one(dmsc_context* ctx, const float* f, int H, int W, float thr, uint32_
   scan_exclusive<int32_t>(surv, surv_pre, nsad, c32 + 5, ar, A_SCAN, st);
   scan_exclusive<int32_t>(rows[0], rows_pre[0], nsad, c32 + 6, ar, A_SCAN, st);
   scan_exclusive<int32_t>(rows[1], rows_pre[1], nsad, c32 + 7, ar, A_SCAN, st);
+  // orders are grouped by field, so field f owns positions [sad_b[f], sad_b[f+1]) of each order
+  bases_at<int32_t, int32_t>(surv_pre, sad_b, nsad, c32 + 5, B, row32(5), st);
+  bases_at<int32_t, int32_t>(rows_pre[0], sad_b, nsad, c32 + 6, B, row32(6), st);
+  bases_at<int32_t, int32_t>(rows_pre[1], sad_b, nsad, c32 + 7, B, row32(7), st);
   launch_sad_to_out(nsad, ord_min, removed, surv_pre, sad_to_out, st);
 
   // ---- arc geometry, per traced side: raw arcs and cancellation lists
@@ -344,9 +483,8 @@ void run_one(dmsc_context* ctx, const float* f, int H, int W, float thr, uint32_
     scan_exclusive<int32_t>(flag, pre, nsad, c32 + 8 + s, ar, A_SCAN, st);
     w.A.faces = s;
     w.A.succ = s ? fsucc : vsucc;
-    w.A.Nx = d.Nx;
-    w.A.W = W;
-    w.A.H = H;
+    w.A.fd = bt.fd;
+    w.A.sfield = sad_field;
     w.A.ns = nsad;
     w.A.next = s ? nmax : nmin;
     w.A.sad_ids = sad_ids;
@@ -354,15 +492,17 @@ void run_one(dmsc_context* ctx, const float* f, int H, int W, float thr, uint32_
     w.A.blen = blen;
     w.A.boff = boff;
   }
-  readback(ctx, c32, 16 * sizeof(int32_t), st);
+  DMSC_CHECK(hipMemcpyAsync(ctx->pinned, c32, 16 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  DMSC_CHECK(hipMemcpyAsync((char*)ctx->pinned + 64, c64, 4 * sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  DMSC_CHECK(hipMemcpyAsync((char*)ctx->pinned + 128, b32, 8 * sizeof(int32_t) * (B + 1), hipMemcpyDeviceToHost, st));
+  DMSC_CHECK(hipStreamSynchronize(st));
   int32_t h32[16];
-  std::memcpy(h32, ctx->pinned, sizeof(h32));
-  readback(ctx, c64, 4 * sizeof(int64_t), st);
   int64_t h64[4];
-  std::memcpy(h64, ctx->pinned, sizeof(h64));
-  const int64_t n_out_ext[2] = {h32[3], h32[4]};
-  const int64_t n_sad_out = h32[5];
-  const int64_t n_rows[2] = {h32[6], h32[7]};
+  std::memcpy(h32, ctx->pinned, sizeof(h32));
+  std::memcpy(h64, (char*)ctx->pinned + 64, sizeof(h64));
+  std::vector<int32_t> hb(8 * (size_t)(B + 1));
+  std::memcpy(hb.data(), (char*)ctx->pinned + 128, sizeof(int32_t) * hb.size());
+  auto hrow = [&](int r, int f) -> int64_t { return hb[(size_t)r * (B + 1) + f]; };
 
   for (int s = 0; s < 2; ++s) {
     if (!aw[s].on) continue;
@@ -373,7 +513,8 @@ void run_one(dmsc_context* ctx, const float* f, int H, int W, float thr, uint32_
     w.ncancel = h32[8 + s];
     w.A.arena = ar.get<int32_t>(b + G_ARENA, w.arena_total);
     launch_raw_arcs(w.A, st);
-    // cancellations sorted by (persistence, saddle id) (persistence_struct.hxx:15-19)
+    // cancellations sorted by (persistence, saddle id) (persistence_struct.hxx:15-19); fields never
+    // meet, so one batch-wide stable sort orders every field's cancellations correctly
     const int64_t nc = w.ncancel;
     int32_t* cs0 = ar.get<int32_t>(b + G_CS0, nc);
     int32_t* cd0 = ar.get<int32_t>(b + G_CD0, nc);
@@ -440,67 +581,91 @@ void run_one(dmsc_context* ctx, const float* f, int H, int W, float thr, uint32_
     const int64_t next = s ? nmax : nmin;
     // path lengths to the final roots and per-output-arc lengths (min order)
     Arena& a2 = ctx->ar2;
-    const int hb = 16 * s;
-    w.pjq = a2.get<uint64_t>(hb + H_PJQ, next + 1);
+    const int hb2 = 16 * s;
+    w.pjq = a2.get<uint64_t>(hb2 + H_PJQ, next + 1);
     launch_geo_q(g, next, w.pjq, st);
-    w.flen = a2.get<int64_t>(hb + H_FLEN, 2 * nsad);
-    int64_t* per_pos = a2.get<int64_t>(hb + H_PERPOS, nsad);
-    w.pos_pre = a2.get<int64_t>(hb + H_POSPRE, nsad);
+    w.cum_r = a2.get<uint64_t>(hb2 + H_CUMR, next + 1);
+    w.cum_c = a2.get<uint64_t>(hb2 + H_CUMC, next + 1);
+    launch_link_counts(w.A, g, w.pjq, next, w.cum_r, w.cum_c, st);
+    w.flen = a2.get<int64_t>(hb2 + H_FLEN, 2 * nsad);
+    int64_t* per_pos = a2.get<int64_t>(hb2 + H_PERPOS, nsad);
+    w.pos_pre = a2.get<int64_t>(hb2 + H_POSPRE, nsad);
     launch_final_lengths(nsad, ord_min, removed, w.A, w.pjq, next, w.flen, per_pos, st);
     scan_exclusive<int64_t>(per_pos, w.pos_pre, nsad, c64 + 2 * s + 1, ar, A_SCAN, st);
+    w.arc_base = a2.get<int64_t>(hb2 + H_ARCB, B + 1);
+    bases_at<int64_t, int32_t>(w.pos_pre, sad_b, nsad, c64 + 2 * s + 1, B, w.arc_base, st);
   }
-  readback(ctx, c64, 4 * sizeof(int64_t), st);
-  std::memcpy(h64, ctx->pinned, sizeof(h64));
-  for (int s = 0; s < 2; ++s) {
-    if (!aw[s].on) continue;
-    aw[s].out_total = h64[2 * s + 1];
-    if (aw[s].out_total > INT32_MAX)
-      throw Fail{DMSC_ERR_TOO_LARGE, "merged arc geometry exceeds 32-bit offsets"};
+  if (aw[0].on || aw[1].on) {
+    for (int s = 0; s < 2; ++s)
+      if (aw[s].on)
+        DMSC_CHECK(hipMemcpyAsync((char*)ctx->pinned + 8 * (size_t)(B + 1) * s, aw[s].arc_base, 8 * (size_t)(B + 1),
+                                  hipMemcpyDeviceToHost, st));
+    DMSC_CHECK(hipStreamSynchronize(st));
+    for (int s = 0; s < 2; ++s) {
+      if (!aw[s].on) continue;
+      ArcWork& w = aw[s];
+      w.h_arc_base.resize(B + 1);
+      std::memcpy(w.h_arc_base.data(), (char*)ctx->pinned + 8 * (size_t)(B + 1) * s, 8 * (size_t)(B + 1));
+      w.out_total = w.h_arc_base[B];
+      for (int f = 0; f < B; ++f)
+        if (w.h_arc_base[f + 1] - w.h_arc_base[f] > INT32_MAX)
+          throw Fail{DMSC_ERR_TOO_LARGE, "merged arc geometry exceeds 32-bit offsets"};
+      if (w.out_total > INT32_MAX - 64)
+        throw Fail{DMSC_ERR_TOO_LARGE, "merged arc geometry of the batch exceeds 32-bit segment counts"};
+    }
   }
 
   // ---- outputs (cpu/workspace.hxx:291-362 layout, dual swap by slot)
+  auto n_out_ext = [&](int s, int f) { return hrow(3 + s, f + 1) - hrow(3 + s, f); };
+  auto n_sad_out = [&](int f) { return hrow(5, f + 1) - hrow(5, f); };
+  auto n_rows = [&](int s, int f) { return hrow(6 + s, f + 1) - hrow(6 + s, f); };
+  auto one = [](int64_t n) { return std::make_pair(n, (int64_t)0); };
   int32_t* o_pts[2];
   for (int s = 0; s < 2; ++s)
-    o_pts[s] = (int32_t*)alloc_out(alloc, user, field, out,slot_pts(s == 1, dual), 1, n_out_ext[s], 0, DMSC_DTYPE_INT32);
-  int32_t* o_sad = (int32_t*)alloc_out(alloc, user, field, out,3, 1, n_sad_out, 0, DMSC_DTYPE_INT32);
+    o_pts[s] = (int32_t*)oa.slot(slot_pts(s == 1, dual), 1, [&](int f) { return one(n_out_ext(s, f)); },
+                                 DMSC_DTYPE_INT32);
+  int32_t* o_sad = (int32_t*)oa.slot(3, 1, [&](int f) { return one(n_sad_out(f)); }, DMSC_DTYPE_INT32);
   int32_t *o_e[2], *o_pp[2];
   float* o_p[2];
   for (int s = 0; s < 2; ++s) {
-    o_e[s] = (int32_t*)alloc_out(alloc, user, field, out,slot_e(s == 1, dual), 2, n_rows[s], 2, DMSC_DTYPE_INT32);
-    o_p[s] = (float*)alloc_out(alloc, user, field, out,slot_p(s == 1, dual), 1, n_sad_out, 0, DMSC_DTYPE_FLOAT32);
-    o_pp[s] = (int32_t*)alloc_out(alloc, user, field, out,slot_pp(s == 1, dual), 1, n_sad_out, 0, DMSC_DTYPE_INT32);
+    o_e[s] = (int32_t*)oa.slot(slot_e(s == 1, dual), 2, [&](int f) { return std::make_pair(n_rows(s, f), (int64_t)2); },
+                               DMSC_DTYPE_INT32);
+    o_p[s] = (float*)oa.slot(slot_p(s == 1, dual), 1, [&](int f) { return one(n_sad_out(f)); }, DMSC_DTYPE_FLOAT32);
+    o_pp[s] = (int32_t*)oa.slot(slot_pp(s == 1, dual), 1, [&](int f) { return one(n_sad_out(f)); }, DMSC_DTYPE_INT32);
   }
   for (int s = 0; s < 2; ++s) {
     const int sl = slot_arcs(s == 1, dual);
     ArcWork& w = aw[s];
-    w.out_cells = (int32_t*)alloc_out(alloc, user, field, out,sl, 1, w.on ? w.out_total : 0, 0, DMSC_DTYPE_INT32);
-    w.out_off = (int32_t*)alloc_out(alloc, user, field, out,sl + 1, 1, w.on ? n_sad_out + 1 : 0, 0, DMSC_DTYPE_INT32);
-    w.out_arc_off = (int32_t*)alloc_out(alloc, user, field, out,sl + 2, 1, w.on ? n_sad_out : 0, 0, DMSC_DTYPE_INT32);
+    w.out_cells = (int32_t*)oa.slot(
+        sl, 1, [&](int f) { return one(w.on ? w.h_arc_base[f + 1] - w.h_arc_base[f] : 0); }, DMSC_DTYPE_INT32);
+    w.out_off = (int32_t*)oa.slot(sl + 1, 1, [&](int f) { return one(w.on ? n_sad_out(f) + 1 : 0); }, DMSC_DTYPE_INT32);
+    w.out_arc_off = (int32_t*)oa.slot(sl + 2, 1, [&](int f) { return one(w.on ? n_sad_out(f) : 0); }, DMSC_DTYPE_INT32);
   }
 
   launch_emit_points(nmin, min_ids, sb[0].to_out, o_pts[0], st);
   launch_emit_points(nmax, max_ids, sb[1].to_out, o_pts[1], st);
+  const EmitBases eb[2] = {{sad_field, row32(5), row32(3)}, {sad_field, row32(5), row32(4)}};
   launch_emit_side(nsad, ord_min, removed, sad_to_out, cmin, sb[0].root, sb[0].to_out, sb[0].pers, sb[0].pair,
-                   rows_pre[0], sad_ids, o_p[0], o_pp[0], o_e[0], o_sad, st);
+                   rows_pre[0], sad_ids, eb[0], o_p[0], o_pp[0], o_e[0], o_sad, st);
   launch_emit_side(nsad, ord_max, removed, sad_to_out, cmax, sb[1].root, sb[1].to_out, sb[1].pers, sb[1].pair,
-                   rows_pre[1], sad_ids, o_p[1], o_pp[1], o_e[1], nullptr, st);
-  if (gmax) launch_labels_faces(d, rv, pjf, sb[1].root, sb[1].to_out, o_flab, st);
-  if (gmin) launch_labels_verts(d, rv, pjv, sb[0].root, sb[0].to_out, o_vlab, st);
+                   rows_pre[1], sad_ids, eb[1], o_p[1], o_pp[1], o_e[1], nullptr, st);
+  if (gmax) launch_labels_faces(rv, S, pjf, sb[1].root, sb[1].to_out, row32(4), o_flab, st);
+  if (gmin) launch_labels_verts(rv, N, pjv, sb[0].root, sb[0].to_out, row32(3), o_vlab, st);
 
   for (int s = 0; s < 2; ++s) {
     ArcWork& w = aw[s];
     if (!w.on) continue;
     Arena& a2 = ctx->ar2;
-    const int hb = 16 * s;
+    const int hb2 = 16 * s;
     const int64_t next = s ? nmax : nmin;
     const int64_t cap = w.out_total + 16;
-    Seg* sa = a2.get<Seg>(hb + H_SEGA, cap);
-    Seg* sbuf = a2.get<Seg>(hb + H_SEGB, cap);
-    Seg* raw = a2.get<Seg>(hb + H_RAW, cap);
-    int32_t* nseg = a2.get<int32_t>(hb + H_NSEG, 4);  // [0] list A, [1] list B, [2] raw
+    Seg* sa = a2.get<Seg>(hb2 + H_SEGA, cap);
+    Seg* sbuf = a2.get<Seg>(hb2 + H_SEGB, cap);
+    Seg* raw = a2.get<Seg>(hb2 + H_RAW, cap);
+    int32_t* nseg = a2.get<int32_t>(hb2 + H_NSEG, 4);  // [0] list A, [1] list B, [2] raw
     fill_i32(nseg, 0, 4, st);
     launch_seed_segments(nsad, ord_min, removed, w.A, w.pjq, next, w.flen, w.pos_pre, sad_to_out, sa, nseg, raw,
-                         nseg + 2, cap, d_err, w.out_off, w.out_arc_off, n_sad_out, c64 + 2 * s + 1, st);
+                         nseg + 2, cap, d_err, w.out_off, w.out_arc_off, w.arc_base, row32(5), B, st);
     readback(ctx, nseg, 3 * sizeof(int32_t), st);
     int32_t n_in = ((const int32_t*)ctx->pinned)[0];
     Seg* in = sa;
@@ -511,7 +676,8 @@ void run_one(dmsc_context* ctx, const float* f, int H, int W, float thr, uint32_
     while (n_in > 0) {
       if (n_in > cap) throw Fail{DMSC_ERR_INTERNAL, "arc segment list overflow"};
       fill_i32(cnt_out, 0, 1, st);
-      launch_expand_segments(w.A, w.G, w.pjq, in, cnt_in, nxt, cnt_out, raw, nseg + 2, n_in, cap, st);
+      launch_expand_segments(w.A, w.G, w.pjq, w.cum_r, w.cum_c, in, cnt_in, nxt, cnt_out, raw, nseg + 2, n_in, cap,
+                             st);
       readback(ctx, nseg, 3 * sizeof(int32_t), st);
       n_in = ((const int32_t*)ctx->pinned)[cnt_out - nseg];
       std::swap(in, nxt);
@@ -520,9 +686,7 @@ void run_one(dmsc_context* ctx, const float* f, int H, int W, float thr, uint32_
     }
     const int32_t nraw = ((const int32_t*)ctx->pinned)[2];
     if (nraw > cap) throw Fail{DMSC_ERR_INTERNAL, "arc segment list overflow"};
-    if (nraw > 0)
-      launch_copy_raw(w.A, raw, nseg + 2, nraw, w.out_cells, w.out_total, d_err, a2.get<int32_t>(hb + H_LONG, nraw),
-                      nseg + 3, st);
+    if (nraw > 0) launch_copy_raw(w.A, raw, nseg + 2, nraw, w.out_cells, w.out_total, d_err, st);
   }
   readback(ctx, d_err, sizeof(int32_t), st);
   if (ctx->times.on) ctx->times.collect();
@@ -530,8 +694,9 @@ void run_one(dmsc_context* ctx, const float* f, int H, int W, float thr, uint32_
     throw Fail{DMSC_ERR_INTERNAL, "device consistency check failed (flags " +
                                       std::to_string(*(const int32_t*)ctx->pinned) + ")"};
 
-  // record for dmsc_debug_fetch
-  ctx->dims = d;
+  // record for dmsc_debug_fetch (batch-wide arrays; for one field, that field's)
+  ctx->S = S;
+  ctx->N = N;
   ctx->nmin = nmin;
   ctx->nsad = nsad;
   ctx->nmax = nmax;
@@ -549,6 +714,38 @@ void run_one(dmsc_context* ctx, const float* f, int H, int W, float thr, uint32_
   ctx->pers_max = sb[1].pers;
   ctx->pjf = pjf;
   ctx->pjv = pjv;
+}
+
+// Splits a call's fields into groups whose concatenated cell arrays stay within 32-bit indices and
+// a bounded scratch footprint, and runs each group through the batch-wide pipeline.
+constexpr int64_t kGroupSlots = int64_t(1) << 28;
+constexpr int kGroupFields = 1 << 16;
+
+// slots per group: DMSC_GROUP_SLOTS when set (smaller groups, e.g. to bound scratch memory)
+int64_t group_slots() {
+  const char* e = std::getenv("DMSC_GROUP_SLOTS");
+  if (!e || !*e) return kGroupSlots;
+  const long long v = std::atoll(e);
+  return v < 1 ? 1 : std::min<int64_t>(v, kGroupSlots);
+}
+
+void run_fields(dmsc_context* ctx, const std::vector<FieldIn>& fields, uint32_t flags, hipStream_t st,
+                dmsc_alloc_fn alloc, void* user, dmsc_outputs* outs) {
+  for (const FieldIn& f : fields) validate_field(f);
+  const int64_t cap = group_slots();
+  size_t i = 0;
+  while (i < fields.size()) {
+    size_t j = i;
+    int64_t slots = 0;
+    while (j < fields.size() && (int)(j - i) < kGroupFields) {
+      const int64_t s = ((int64_t)fields[j].H + 1) * ((int64_t)fields[j].W + 1);
+      if (j > i && slots + s > cap) break;
+      slots += s;
+      ++j;
+    }
+    run_group(ctx, fields.data() + i, (int)(j - i), flags, st, alloc, user, outs);
+    i = j;
+  }
 }
 
 dmsc_status guarded(dmsc_context* ctx, const std::function<void()>& fn) {
@@ -583,6 +780,7 @@ dmsc_status dmsc_context_create(int device, dmsc_context** out) {
     delete c;
     return DMSC_ERR_HIP;
   }
+  c->pinned_cap = 4096;
   *out = c;
   return DMSC_OK;
 }
@@ -591,13 +789,6 @@ void dmsc_context_destroy(dmsc_context* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
   (void)hipDeviceSynchronize();
-  for (auto& w : ctx->workers) {
-    w->ar.release();
-    w->ar2.release();
-    if (w->pinned) (void)hipHostFree(w->pinned);
-    if (w->stream) (void)hipStreamDestroy(w->stream);
-  }
-  ctx->workers.clear();
   ctx->ar.release();
   ctx->ar2.release();
   if (ctx->pinned) (void)hipHostFree(ctx->pinned);
@@ -607,7 +798,9 @@ void dmsc_context_destroy(dmsc_context* ctx) {
 dmsc_status dmsc_extract(dmsc_context* ctx, const float* d_field, int32_t H, int32_t W, float thr, uint32_t flags,
                          dmsc_stream_t stream, dmsc_alloc_fn alloc, void* user, dmsc_outputs* out) {
   if (!alloc || !out) return DMSC_ERR_INVALID_ARGUMENT;
-  return guarded(ctx, [&] { run_one(ctx, d_field, H, W, thr, flags, (hipStream_t)stream, alloc, user, 0, out); });
+  return guarded(ctx, [&] {
+    run_fields(ctx, {FieldIn{d_field, H, W, thr, 0}}, flags, (hipStream_t)stream, alloc, user, out);
+  });
 }
 
 dmsc_status dmsc_extract_batch(dmsc_context* ctx, int32_t B, const float* const* d_fields, const int32_t* H,
@@ -615,69 +808,9 @@ dmsc_status dmsc_extract_batch(dmsc_context* ctx, int32_t B, const float* const*
                                dmsc_alloc_fn alloc, void* user, dmsc_outputs* out) {
   if (!alloc || !out || B < 0 || (B > 0 && (!d_fields || !H || !W || !thr))) return DMSC_ERR_INVALID_ARGUMENT;
   return guarded(ctx, [&] {
-    const int nw = std::min<int>(B, batch_workers());
-    if (nw <= 1) {
-      for (int b = 0; b < B; ++b)
-        run_one(ctx, d_fields[b], H[b], W[b], thr[b], flags, (hipStream_t)stream, alloc, user, b, &out[b]);
-      return;
-    }
-    // Worker contexts, each with its own scratch and its own (default-priority) HIP stream; fields
-    // are taken from a shared counter, so small fields' latency-bound stages overlap on the device.
-    while ((int)ctx->workers.size() < nw) {
-      auto* w = new dmsc_context();
-      w->device = ctx->device;
-      DMSC_CHECK(hipHostMalloc((void**)&w->pinned, 4096));
-      DMSC_CHECK(hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking));
-      ctx->workers.emplace_back(w);
-    }
-    hipEvent_t ready;
-    DMSC_CHECK(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
-    DMSC_CHECK(hipEventRecord(ready, (hipStream_t)stream));  // inputs were produced on the caller's stream
-    for (int k = 0; k < nw; ++k) DMSC_CHECK(hipStreamWaitEvent(ctx->workers[k]->stream, ready, 0));
-    std::atomic<int> next{0};
-    std::mutex mu;
-    Fail first{DMSC_OK, ""};
-    std::vector<std::thread> th;
-    for (int k = 0; k < nw; ++k) {
-      dmsc_context* w = ctx->workers[k].get();
-      w->times.on = ctx->times.on;
-      th.emplace_back([&, w] {
-        tl_times = &w->times;
-        if (hipSetDevice(w->device) != hipSuccess) return;
-        for (int b; (b = next.fetch_add(1)) < B;) {
-          try {
-            run_one(w, d_fields[b], H[b], W[b], thr[b], flags, w->stream, alloc, user, b, &out[b]);
-          } catch (const Fail& e) {
-            std::lock_guard<std::mutex> lk(mu);
-            if (first.st == DMSC_OK) first = e;
-            next = B;
-          } catch (const HipError& e) {
-            std::lock_guard<std::mutex> lk(mu);
-            if (first.st == DMSC_OK) first = Fail{DMSC_ERR_HIP, std::string(e.what) + ": " + hipGetErrorString(e.err)};
-            next = B;
-          } catch (const std::exception& e) {
-            std::lock_guard<std::mutex> lk(mu);
-            if (first.st == DMSC_OK) first = Fail{DMSC_ERR_INTERNAL, e.what()};
-            next = B;
-          }
-        }
-      });
-    }
-    for (auto& t : th) t.join();
-    (void)hipEventDestroy(ready);
-    for (auto& w : ctx->workers) {
-      for (auto& kv : w->times.acc) {
-        ctx->times.add(kv.first.c_str(), 0.0);
-        for (auto& mine : ctx->times.acc)
-          if (mine.first == kv.first) {
-            mine.second.first += kv.second.first;
-            mine.second.second += kv.second.second - 1;
-          }
-      }
-      w->times.acc.clear();
-    }
-    tl_times = &ctx->times;
-    if (first.st != DMSC_OK) throw first;
+    std::vector<FieldIn> fields((size_t)B);
+    for (int b = 0; b < B; ++b) fields[b] = FieldIn{d_fields[b], H[b], W[b], thr[b], b};
+    run_fields(ctx, fields, flags, (hipStream_t)stream, alloc, user, out);
   });
 }
 
@@ -734,8 +867,8 @@ int64_t dmsc_debug_fetch(dmsc_context* ctx, const char* name, void* dst, int64_t
   if (n == "pair_max") return copy(ctx->pair_max, ctx->nsad, 4);
   if (n == "pers_min") return copy(ctx->pers_min, ctx->nsad, 4);
   if (n == "pers_max") return copy(ctx->pers_max, ctx->nsad, 4);
-  if (n == "face_pj") return copy(ctx->pjf, ctx->dims.S, 8);
-  if (n == "vert_pj") return copy(ctx->pjv, ctx->dims.N, 8);
+  if (n == "face_pj") return copy(ctx->pjf, ctx->S, 8);
+  if (n == "vert_pj") return copy(ctx->pjv, ctx->N, 8);
   return -1;
 }
 

@@ -145,6 +145,28 @@ __device__ __forceinline__ int wave_append(int32_t* counter, bool pred) {
   return pred ? base + __popcll(m & ((1ull << lane) - 1ull)) : -1;
 }
 
+// Inclusive prefix sum over the 64 lanes of a wave (every lane must take part).
+__device__ __forceinline__ int32_t wave_inclusive_sum(int32_t v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int32_t t = __shfl_up(v, o, 64);
+    if (lane >= o) v += t;
+  }
+  return v;
+}
+
+// Wave-aggregated reservation of n >= 0 consecutive slots per lane: one atomicAdd per wave;
+// returns the lane's first slot. Must be reached by every lane of the wave.
+__device__ __forceinline__ int32_t wave_reserve(int32_t* counter, int32_t n) {
+  const int32_t incl = wave_inclusive_sum(n);
+  const int32_t total = __shfl(incl, 63, 64);
+  int32_t base = 0;
+  if ((threadIdx.x & 63) == 0 && total > 0) base = atomicAdd(counter, total);
+  base = __shfl(base, 0, 64);
+  return base + incl - n;
+}
+
 // Wave-uniform grid-stride loop: every lane of a wave runs the same iterations; `active` marks
 // lanes with an element. Use with wave_append.
 #define DMSC_WAVE_LOOP(j, n)                                                                        \
@@ -178,15 +200,46 @@ inline Dims make_dims(int H, int W) {
   return d;
 }
 
-// Critical-cell rank structure: bit planes (type t at planes + t*PW) plus exclusive popcount
-// prefixes per word for minima (type 0), saddles (types 1+2) and maxima (type 3).
+// ------------------------------------------------------------------ batches of fields
+// All fields of a call are processed by the same launches. Every per-cell array is the concatenation
+// of the fields' arrays; a field's slice starts at its base offset. Cell ids stay field-local.
+struct FieldDesc {
+  const float* f;  // H x W input
+  int H, W, Nx, Ny, WPR;
+  float thr;
+  int64_t pix_base, slot_base, word_base;
+};
+struct TileRef {
+  int field, tx, ty;
+};
+struct Batch {
+  const FieldDesc* fd;    // [B]
+  const int64_t* slot_b;  // [B+1] slot bases
+  const int64_t* pix_b;   // [B+1] pixel bases
+  const int64_t* word_b;  // [B+1] bit-plane word bases
+  int B;
+};
+// index of the field whose [base[f], base[f+1]) range holds idx
+__device__ __forceinline__ int field_of(const int64_t* __restrict__ base, int B, int64_t idx) {
+  int lo = 0, hi = B - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (base[mid] <= idx) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Critical-cell rank structure over the batch: bit planes (type t at planes + t*PW, PW = total
+// words) plus exclusive popcount prefixes per word for minima (type 0), saddles (types 1+2) and
+// maxima (type 3). Ranks are global list indices (fields' lists are concatenated in field order).
 struct RankView {
   const uint32_t* planes;
   const int32_t* pre_min;
   const int32_t* pre_sad;
   const int32_t* pre_max;
   int64_t PW;
-  int WPR, Nx, W;
+  Batch bt;
 };
 
 // ------------------------------------------------------------------ launchers (k_*.hip)
@@ -202,19 +255,23 @@ void pointer_jump_init(const int32_t* succ, uint64_t* pj, int64_t n, hipStream_t
 void fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t st);
 void fill_u64(uint64_t* p, uint64_t v, int64_t n, hipStream_t st);
 void iota_i32(int32_t* p, int64_t n, hipStream_t st);
+template <class T, class S>
+void bases_at(const T* pre, const S* starts, int64_t n, const T* total, int B, T* out, hipStream_t st);
 
-// grid stages
-void launch_gradient(bool dual, const float* f, const Dims& d, int32_t* pw, int32_t* fsucc, int32_t* vsucc,
-                     uint32_t* planes, hipStream_t st);
-void launch_compact_lists(bool dual, const float* f, const Dims& d, const RankView& rv, int32_t* min_ids,
-                          float* min_val, int32_t* sad_ids, float* sad_val, int32_t* max_ids, float* max_val,
-                          hipStream_t st);
-void launch_saddle_records(const Dims& d, const RankView& rv, const int32_t* sad_ids, int64_t nsad,
+// grid stages (over all fields of a batch)
+void launch_gradient(bool dual, const Batch& bt, const TileRef* tiles, int64_t ntiles, int32_t* pw, int32_t* fsucc,
+                     int32_t* vsucc, uint32_t* planes, int64_t PW, hipStream_t st);
+void launch_compact_lists(bool dual, const RankView& rv, int32_t* min_ids, float* min_val, int32_t* sad_ids,
+                          float* sad_val, int32_t* sad_field, int32_t* max_ids, float* max_val, hipStream_t st);
+void launch_saddle_records(const RankView& rv, const int32_t* sad_ids, const int32_t* sad_field, int64_t nsad,
                            const uint64_t* pj_face, const uint64_t* pj_vert, int32_t* cmax, int32_t* cmin,
                            int32_t* lmax, int32_t* lmin, hipStream_t st);
-void launch_labels_faces(const Dims& d, const RankView& rv, const uint64_t* pj_face, const int32_t* root_t,
-                         const int32_t* to_out, int32_t* out, hipStream_t st);
-void launch_labels_verts(const Dims& d, const RankView& rv, const uint64_t* pj_vert, const int32_t* root_t,
-                         const int32_t* to_out, int32_t* out, hipStream_t st);
+// labels: out_base[f] = output index of field f's first extremum of that side
+void launch_labels_faces(const RankView& rv, int64_t S, const uint64_t* pj_face, const int32_t* root_t,
+                         const int32_t* to_out, const int32_t* out_base, int32_t* out, hipStream_t st);
+void launch_labels_verts(const RankView& rv, int64_t N, const uint64_t* pj_vert, const int32_t* root_t,
+                         const int32_t* to_out, const int32_t* out_base, int32_t* out, hipStream_t st);
+void radix_sort_pairs_bits(uint32_t* keys, int32_t* vals, uint32_t* keys_tmp, int32_t* vals_tmp, int64_t n, int bits,
+                           Arena& ar, int slot, hipStream_t st);
 
 }  // namespace dmsc

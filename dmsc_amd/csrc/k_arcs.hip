@@ -38,25 +38,27 @@ void launch_arc_lengths(int64_t n2, const int32_t* c, const int32_t* steps, int3
   DMSC_LAUNCH_CHECK();
 }
 
-// start cell (forest index) of arc k of saddle s; -1 if absent
-__device__ __forceinline__ int arc_start(const ArcSide& A, int s, int k) {
+// start cell (field-local forest index) of arc k of saddle s; -1 if absent
+__device__ __forceinline__ int arc_start(const ArcSide& A, const FieldDesc& fd, int s, int k) {
   const int id = A.sad_ids[s], q = id >> 2, t = id & 3;
-  if (A.faces) return k == 0 ? q : (t == 1 ? q + A.Nx : q + 1);
-  const int y = q / A.Nx, x = q - y * A.Nx;
+  const int Nx = fd.Nx, W = fd.W;
+  if (A.faces) return k == 0 ? q : (t == 1 ? q + Nx : q + 1);
+  const int y = q / Nx, x = q - y * Nx;
   int v[2], n = 0;
   if (t == 1) {
-    if (x - 1 >= 0) v[n++] = y * A.W + x - 1;
-    if (x < A.W) v[n++] = y * A.W + x;
+    if (x - 1 >= 0) v[n++] = y * W + x - 1;
+    if (x < W) v[n++] = y * W + x;
   } else {
-    if (y - 1 >= 0) v[n++] = (y - 1) * A.W + x;
-    if (y < A.H) v[n++] = y * A.W + x;
+    if (y - 1 >= 0) v[n++] = (y - 1) * W + x;
+    if (y < fd.H) v[n++] = y * W + x;
   }
   return k < n ? v[k] : -1;
 }
-__device__ __forceinline__ int32_t cell_of(const ArcSide& A, int node) {
+// cell id of a field-local forest node
+__device__ __forceinline__ int32_t cell_of(const ArcSide& A, const FieldDesc& fd, int node) {
   if (A.faces) return 4 * node + 3;
-  const int y = node / A.W, x = node - y * A.W;
-  return 4 * (y * A.Nx + x);
+  const int y = node / fd.W, x = node - y * fd.W;
+  return 4 * (y * fd.Nx + x);
 }
 
 // one thread per arc: walk the successor forest from the start cell to the extremum
@@ -64,10 +66,13 @@ __global__ void k_raw_arcs(ArcSide A) {
   for (int64_t a = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; a < 2 * A.ns; a += (int64_t)gridDim.x * blockDim.x) {
     const int len = A.blen[a];
     if (len == 0) continue;
-    int cur = arc_start(A, (int)(a >> 1), (int)(a & 1));
+    const int s = (int)(a >> 1);
+    const FieldDesc& fd = A.fd[A.sfield ? A.sfield[s] : 0];
+    const int base = (int)(A.faces ? fd.slot_base : fd.pix_base);
+    int cur = base + arc_start(A, fd, s, (int)(a & 1));
     int32_t* o = A.arena + A.boff[a];
     for (int j = 0; j < len; ++j) {
-      o[j] = cell_of(A, cur);
+      o[j] = cell_of(A, fd, cur - base);
       cur = A.succ[cur];
     }
   }
@@ -295,24 +300,20 @@ void launch_final_lengths(int64_t ns, const int32_t* ord, const uint8_t* removed
   DMSC_LAUNCH_CHECK();
 }
 
-__device__ __forceinline__ void push_seg(Seg* list, int32_t* n, int64_t cap, int32_t* err, int kind, int a, int b,
-                                         int dir, int64_t off) {
-  const int pos = atomicAdd(n, 1);
-  if (pos >= cap) {
-    report(err, ERR_SEG_CAP);
-    return;
-  }
-  list[pos] = Seg{kind, a, b, dir, off};
+__device__ __forceinline__ int64_t seg_od(int64_t off, int fwd) { return off << 1 | (fwd ? 1 : 0); }
+__device__ __forceinline__ int32_t cum_count(const uint64_t* __restrict__ cum, int x) {
+  return x >= 0 ? (int32_t)(cum[x] >> 32) : 0;
 }
 
-// Seeds: per surviving saddle arc, RAW(arc) then CHAIN(target -> final root); also the CSR offsets.
+// Seeds: per surviving saddle arc, its raw arc then CHAIN(target -> final root); also the field-local
+// CSR offsets.
 __global__ void k_seed_segments(int64_t ns, const int32_t* __restrict__ ord, const uint8_t* __restrict__ removed,
                                 ArcSide A, const uint64_t* __restrict__ pjq, int64_t next,
                                 const int64_t* __restrict__ flen, const int64_t* __restrict__ pos_pre,
-                                const int32_t* __restrict__ sad_to_out, Seg* __restrict__ segs,
-                                int32_t* __restrict__ nseg, Seg* __restrict__ raw, int32_t* __restrict__ nraw,
+                                const int32_t* __restrict__ sad_to_out, Seg* __restrict__ chains,
+                                int32_t* __restrict__ nchain, Seg* __restrict__ raw, int32_t* __restrict__ nraw,
                                 int64_t cap, int32_t* __restrict__ g_err, int32_t* __restrict__ out_off,
-                                int32_t* __restrict__ out_arc_off) {
+                                int32_t* __restrict__ out_arc_off, const int64_t* __restrict__ arc_base) {
   const int lane = threadIdx.x & 63;
   DMSC_WAVE_LOOP(k, ns) {
     const int64_t kk0 = k_base + lane;
@@ -322,19 +323,21 @@ __global__ void k_seed_segments(int64_t ns, const int32_t* __restrict__ ord, con
       const int i = ord[kk0];
       if (!removed[i]) {
         const int o = sad_to_out[i];
+        const int fi = A.sfield ? A.sfield[i] : 0;
         const int64_t base = pos_pre[kk0];
-        out_off[o] = (int32_t)base;
-        out_arc_off[o] = (int32_t)(base + flen[2 * kk0]);
+        const int64_t local = base - arc_base[fi];  // field-local output offset
+        out_off[o + fi] = (int32_t)local;
+        out_arc_off[o] = (int32_t)(local + flen[2 * kk0]);
         for (int kk = 0; kk < 2; ++kk) {
           const int a = 2 * i + kk, T = A.tinit[a];
           if (T < 0) continue;
           const int64_t off = base + (kk ? flen[2 * kk0] : 0);
           has_raw[kk] = true;
-          rs[kk] = Seg{0, a, 0, 1, off};
+          rs[kk] = Seg{a, 0, seg_od(off, 1)};
           if (qlen(pjq, T) > 0) {
             const uint32_t root = (uint32_t)pjq[T];
             has_chain[kk] = true;
-            cs[kk] = Seg{1, T, root == (uint32_t)next ? -1 : (int)root, 1, off + A.blen[a]};
+            cs[kk] = Seg{T, root == (uint32_t)next ? -1 : (int)root, seg_od(off + A.blen[a], 1)};
           }
         }
       }
@@ -346,154 +349,216 @@ __global__ void k_seed_segments(int64_t ns, const int32_t* __restrict__ ord, con
         if (pr < cap) raw[pr] = rs[kk];
         else report(g_err, ERR_SEG_CAP);
       }
-      const int pc = wave_append(nseg, has_chain[kk]);
+      const int pc = wave_append(nchain, has_chain[kk]);
       if (has_chain[kk]) {
-        if (pc < cap) segs[pc] = cs[kk];
+        if (pc < cap) chains[pc] = cs[kk];
         else report(g_err, ERR_SEG_CAP);
       }
     }
   }
 }
-__global__ void k_set_last_offset(int32_t* out_off, int64_t n_out, const int64_t* total) {
-  out_off[n_out] = (int32_t)*total;
+// the closing entry of each field's offsets: its total merged length
+__global__ void k_set_last_offsets(int32_t* __restrict__ out_off, const int64_t* __restrict__ arc_base,
+                                   const int32_t* __restrict__ sad_out, int B) {
+  for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < B; f += gridDim.x * blockDim.x)
+    out_off[sad_out[f + 1] + f] = (int32_t)(arc_base[f + 1] - arc_base[f]);
 }
 
-// One expansion round: CHAIN -> LINK segments; LINK -> two RAW and up to two CHAIN segments.
-__global__ void k_expand(ArcSide A, GeoState g, const uint64_t* __restrict__ pjq, const Seg* __restrict__ in,
-                         const int32_t* __restrict__ nin, Seg* __restrict__ out, int32_t* __restrict__ nout,
-                         Seg* __restrict__ raw, int32_t* __restrict__ nraw, int64_t cap) {
+// ------------------------------------------------------------------ link expansion
+// The link of extremum d (made by cancellation c, saddle s, dead arc kd, other arc ko) expands to
+//   reverse(CHAIN(T_kd -> d)) ++ reverse(raw kd) ++ raw ko ++ CHAIN(T_ko -> R_o)
+// with the chains present when their length is positive and raw ko when the arc exists.
+struct LinkParts {
+  int s, kd, ko, Td, To, Ro;
+  int32_t Alen, Bk, Bo, Clen;
+};
+__device__ __forceinline__ LinkParts link_parts(const ArcSide& A, const GeoState& g, const uint64_t* __restrict__ pjq,
+                                                int d) {
+  LinkParts L;
+  const int c = g.linkc[d];
+  L.kd = g.linkkd[d];
+  L.ko = 1 - L.kd;
+  L.s = g.cs[c];
+  L.Td = A.tinit[2 * L.s + L.kd];
+  L.To = A.tinit[2 * L.s + L.ko];
+  L.Ro = g.gparent[d];
+  L.Alen = qlen(pjq, L.Td) - qlen(pjq, d);
+  L.Bk = A.blen[2 * L.s + L.kd];
+  L.Bo = A.blen[2 * L.s + L.ko];
+  L.Clen = L.To >= 0 ? qlen(pjq, L.To) - qlen(pjq, L.Ro) : 0;
+  return L;
+}
+
+// cum_r / cum_c: (parent or the void node `next`, raw / chain pieces of the link) for linked
+// extrema, (self, 0) for roots; pointer jumping then turns them into path sums.
+__global__ void k_link_counts(ArcSide A, GeoState g, const uint64_t* __restrict__ pjq, int64_t next,
+                              uint64_t* __restrict__ cum_r, uint64_t* __restrict__ cum_c) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e <= next; e += (int64_t)gridDim.x * blockDim.x) {
+    uint32_t J = (uint32_t)e, nr = 0, nc = 0;
+    if (e < next) {
+      const int p = g.gparent[e];
+      if (p != (int)e) {
+        J = p < 0 ? (uint32_t)next : (uint32_t)p;
+        const LinkParts L = link_parts(A, g, pjq, (int)e);
+        nr = 1 + (L.Bo > 0 ? 1 : 0);
+        nc = (L.Alen > 0 ? 1 : 0) + (L.Clen > 0 ? 1 : 0);
+      }
+    }
+    cum_r[e] = (uint64_t)J | ((uint64_t)nr << 32);
+    cum_c[e] = (uint64_t)J | ((uint64_t)nc << 32);
+  }
+}
+void launch_link_counts(const ArcSide& a, const GeoState& g, const uint64_t* pjq, int64_t next, uint64_t* cum_r,
+                        uint64_t* cum_c, hipStream_t st) {
+  KScope ks_("link_counts", st);
+  hipLaunchKernelGGL(k_link_counts, dim3(grid_for(next + 1, 256, 16384)), dim3(256), 0, st, a, g, pjq, next, cum_r,
+                     cum_c);
+  DMSC_LAUNCH_CHECK();
+  pointer_jump(cum_r, next + 1, g.err, st);
+  pointer_jump(cum_c, next + 1, g.err, st);
+}
+
+// One expansion level. Each chain (a -> b) reserves exactly its links' pieces (path-sum
+// differences of cum_r / cum_c) with one atomic per wave, then walks its links writing them.
+__global__ void k_expand(ArcSide A, GeoState g, const uint64_t* __restrict__ pjq, const uint64_t* __restrict__ cum_r,
+                         const uint64_t* __restrict__ cum_c, const Seg* __restrict__ in, const int32_t* __restrict__ nin,
+                         Seg* __restrict__ out, int32_t* __restrict__ nout, Seg* __restrict__ raw,
+                         int32_t* __restrict__ nraw, int64_t cap) {
   const int n = *nin;
-  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
-    const Seg sg = in[j];
-    if (sg.kind == 1) {
-      const int b = sg.b;
-      const int32_t Lc = qlen(pjq, sg.a) - qlen(pjq, b);
-      int y = sg.a;
-      int64_t acc = 0;
-      for (int it = 0; y >= 0 && y != b && g.gparent[y] != y; ++it) {
-        if (it >= kLoopCap) {
-          report(g.err, ERR_GEO_CAP);
-          break;
-        }
-        const int32_t len = g.linklen[y];
-        const int64_t o = sg.dir ? sg.off + acc : sg.off + Lc - acc - len;
-        push_seg(out, nout, cap, g.err, 2, y, 0, sg.dir, o);
-        acc += len;
-        y = g.gparent[y];
+  const int lane = threadIdx.x & 63;
+  DMSC_WAVE_LOOP(j, (int64_t)n) {
+    const int64_t jj = j_base + lane;
+    Seg sg{0, 0, 0};
+    int32_t nr = 0, nc = 0;
+    if (jj < n) {
+      sg = in[jj];
+      nr = cum_count(cum_r, sg.a) - cum_count(cum_r, sg.b);
+      nc = cum_count(cum_c, sg.a) - cum_count(cum_c, sg.b);
+    }
+    int32_t pr = wave_reserve(nraw, nr);
+    int32_t pc = wave_reserve(nout, nc);
+    if (jj >= n) continue;
+    if ((int64_t)pr + nr > cap || (int64_t)pc + nc > cap) {
+      report(g.err, ERR_SEG_CAP);
+      continue;
+    }
+    const int b = sg.b;
+    const int fwd = (int)(sg.od & 1);
+    const int64_t off = sg.od >> 1;
+    const int32_t Lc = qlen(pjq, sg.a) - qlen(pjq, b);
+    const int pr_end = pr + nr, pc_end = pc + nc;
+    int y = sg.a;
+    int64_t acc = 0;
+    for (int it = 0; y >= 0 && y != b && g.gparent[y] != y; ++it) {
+      if (it >= kLoopCap) {
+        report(g.err, ERR_GEO_CAP);
+        break;
       }
-    } else {
-      const int d = sg.a;
-      const int c = g.linkc[d], kd = g.linkkd[d], ko = 1 - kd;
-      const int s = g.cs[c];
-      const int Td = A.tinit[2 * s + kd], To = A.tinit[2 * s + ko];
-      const int Ro = g.gparent[d];
-      const int32_t Alen = qlen(pjq, Td) - qlen(pjq, d);
-      const int32_t Bk = A.blen[2 * s + kd], Bo = A.blen[2 * s + ko];
-      const int32_t Clen = To >= 0 ? qlen(pjq, To) - qlen(pjq, Ro) : 0;
-      if (sg.dir) {
-        int64_t o = sg.off;
-        if (Alen > 0) push_seg(out, nout, cap, g.err, 1, Td, d, 0, o);
-        o += Alen;
-        push_seg(raw, nraw, cap, g.err, 0, 2 * s + kd, 0, 0, o);
-        o += Bk;
-        if (Bo > 0) push_seg(raw, nraw, cap, g.err, 0, 2 * s + ko, 0, 1, o);
-        o += Bo;
-        if (Clen > 0) push_seg(out, nout, cap, g.err, 1, To, Ro, 1, o);
+      const int32_t len = g.linklen[y];
+      const LinkParts L = link_parts(A, g, pjq, y);
+      const int need_r = 1 + (L.Bo > 0 ? 1 : 0), need_c = (L.Alen > 0 ? 1 : 0) + (L.Clen > 0 ? 1 : 0);
+      if (pr + need_r > pr_end || pc + need_c > pc_end) {  // counts disagree with the walk: never overrun
+        report(g.err, ERR_SEG_CAP);
+        break;
+      }
+      int64_t o = fwd ? off + acc : off + Lc - acc - len;
+      if (fwd) {
+        if (L.Alen > 0) out[pc++] = Seg{L.Td, y, seg_od(o, 0)};
+        o += L.Alen;
+        raw[pr++] = Seg{2 * L.s + L.kd, 0, seg_od(o, 0)};
+        o += L.Bk;
+        if (L.Bo > 0) raw[pr++] = Seg{2 * L.s + L.ko, 0, seg_od(o, 1)};
+        o += L.Bo;
+        if (L.Clen > 0) out[pc++] = Seg{L.To, L.Ro, seg_od(o, 1)};
       } else {
-        int64_t o = sg.off;
-        if (Clen > 0) push_seg(out, nout, cap, g.err, 1, To, Ro, 0, o);
-        o += Clen;
-        if (Bo > 0) push_seg(raw, nraw, cap, g.err, 0, 2 * s + ko, 0, 0, o);
-        o += Bo;
-        push_seg(raw, nraw, cap, g.err, 0, 2 * s + kd, 0, 1, o);
-        o += Bk;
-        if (Alen > 0) push_seg(out, nout, cap, g.err, 1, Td, d, 1, o);
+        if (L.Clen > 0) out[pc++] = Seg{L.To, L.Ro, seg_od(o, 0)};
+        o += L.Clen;
+        if (L.Bo > 0) raw[pr++] = Seg{2 * L.s + L.ko, 0, seg_od(o, 0)};
+        o += L.Bo;
+        raw[pr++] = Seg{2 * L.s + L.kd, 0, seg_od(o, 1)};
+        o += L.Bk;
+        if (L.Alen > 0) out[pc++] = Seg{L.Td, y, seg_od(o, 1)};
       }
+      acc += len;
+      y = g.gparent[y];
     }
   }
 }
-void launch_expand_segments(const ArcSide& a, const GeoState& g, const uint64_t* pjq, const Seg* in, const int32_t* nin,
-                            Seg* out, int32_t* nout, Seg* raw, int32_t* nraw, int64_t max_in, int64_t cap,
-                            hipStream_t st) {
+void launch_expand_segments(const ArcSide& a, const GeoState& g, const uint64_t* pjq, const uint64_t* cum_r,
+                            const uint64_t* cum_c, const Seg* in, const int32_t* nin, Seg* out, int32_t* nout, Seg* raw,
+                            int32_t* nraw, int64_t max_in, int64_t cap, hipStream_t st) {
   KScope ks_("expand_segments", st);
-  hipLaunchKernelGGL(k_expand, dim3(grid_for(max_in, 256, 4096)), dim3(256), 0, st, a, g, pjq, in, nin, out, nout, raw,
-                     nraw, cap);
+  hipLaunchKernelGGL(k_expand, dim3(grid_for(max_in, 256, 8192)), dim3(256), 0, st, a, g, pjq, cum_r, cum_c, in, nin,
+                     out, nout, raw, nraw, cap);
   DMSC_LAUNCH_CHECK();
 }
 
-// RAW segments: copy the raw arc (forward or reversed) into the output. Short arcs (the common case)
-// are copied by one thread each; arcs longer than kLongArc are deferred to a wave-per-arc pass.
-constexpr int kLongArc = 48;
-
-__global__ void k_copy_raw_short(ArcSide A, const Seg* __restrict__ raw, const int32_t* __restrict__ nraw,
-                                 int32_t* __restrict__ out, int64_t out_total, int32_t* __restrict__ err,
-                                 int32_t* __restrict__ longlist, int32_t* __restrict__ nlong) {
+// Raw pieces -> output cells. A wave takes 64 pieces, lays their cells end to end (prefix sum of
+// lengths) and copies that run 64 cells at a time, so reads and writes are contiguous within each
+// piece whatever the piece lengths.
+__global__ __launch_bounds__(256) void k_copy_raw(ArcSide A, const Seg* __restrict__ raw,
+                                                  const int32_t* __restrict__ nraw, int32_t* __restrict__ out,
+                                                  int64_t out_total, int32_t* __restrict__ err) {
   const int n = *nraw;
   const int lane = threadIdx.x & 63;
   DMSC_WAVE_LOOP(j, (int64_t)n) {
     const int64_t jj = j_base + lane;
-    bool is_long = false;
+    int32_t len = 0;
+    int64_t dst = 0, src = 0;
+    int fwd = 1;
     if (jj < n) {
       const Seg sg = raw[jj];
-      const int len = A.blen[sg.a];
-      if (sg.off < 0 || sg.off + len > out_total) {  // inconsistent offsets: never write out of bounds
+      len = A.blen[sg.a];
+      dst = sg.od >> 1;
+      fwd = (int)(sg.od & 1);
+      src = A.boff[sg.a];
+      if (dst < 0 || dst + len > out_total) {  // inconsistent offsets: never write out of bounds
         report(err, ERR_SEG_CAP);
-      } else if (len > kLongArc) {
-        is_long = true;
-      } else {
-        const int32_t* src = A.arena + A.boff[sg.a];
-        int32_t* dst = out + sg.off;
-        if (sg.dir)
-          for (int t = 0; t < len; ++t) dst[t] = src[t];
-        else
-          for (int t = 0; t < len; ++t) dst[t] = src[len - 1 - t];
+        len = 0;
       }
     }
-    const int p = wave_append(nlong, is_long);
-    if (is_long) longlist[p] = (int)jj;
-  }
-}
-
-__global__ void k_copy_raw_long(ArcSide A, const Seg* __restrict__ raw, const int32_t* __restrict__ longlist,
-                                const int32_t* __restrict__ nlong, int32_t* __restrict__ out) {
-  const int n = *nlong;
-  const int lane = threadIdx.x & 63;
-  const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
-  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-  for (int64_t j = wave; j < n; j += nwaves) {
-    const Seg sg = raw[longlist[j]];
-    const int len = A.blen[sg.a];
-    const int32_t* src = A.arena + A.boff[sg.a];
-    int32_t* dst = out + sg.off;
-    if (sg.dir)
-      for (int t = lane; t < len; t += 64) dst[t] = src[t];
-    else
-      for (int t = lane; t < len; t += 64) dst[t] = src[len - 1 - t];
+    // reversed pieces read src + len - 1 - t: encode as a negative start
+    const int64_t srcv = fwd ? src : ~(src + len - 1);
+    const int32_t incl = wave_inclusive_sum(len);
+    const int32_t start = incl - len;
+    const int32_t total = __shfl(incl, 63, 64);
+    for (int32_t base = 0; base < total; base += 64) {
+      const int32_t idx = base + lane;
+      int k = 0;
+#pragma unroll
+      for (int step = 32; step > 0; step >>= 1) {
+        const int32_t sc = __shfl(start, k + step, 64);
+        if (sc <= idx) k += step;
+      }
+      const int32_t t = idx - __shfl(start, k, 64);
+      const int64_t d = __shfl(dst, k, 64);
+      const int64_t sv = __shfl(srcv, k, 64);
+      if (idx < total) out[d + t] = A.arena[sv >= 0 ? sv + t : ~sv - t];
+    }
   }
 }
 
 void launch_copy_raw(const ArcSide& a, const Seg* raw, const int32_t* nraw, int64_t max_raw, int32_t* out,
-                     int64_t out_total, int32_t* err, int32_t* longlist, int32_t* nlong, hipStream_t st) {
+                     int64_t out_total, int32_t* err, hipStream_t st) {
   KScope ks_("copy_raw", st);
-  fill_i32(nlong, 0, 1, st);
-  hipLaunchKernelGGL(k_copy_raw_short, dim3(grid_for(max_raw, 256, 16384)), dim3(256), 0, st, a, raw, nraw, out,
-                     out_total, err, longlist, nlong);
-  DMSC_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_copy_raw_long, dim3(2048), dim3(256), 0, st, a, raw, longlist, nlong, out);
+  hipLaunchKernelGGL(k_copy_raw, dim3(grid_for(max_raw, 256, 16384)), dim3(256), 0, st, a, raw, nraw, out, out_total,
+                     err);
   DMSC_LAUNCH_CHECK();
 }
 
 void launch_seed_segments(int64_t ns, const int32_t* ord, const uint8_t* removed, const ArcSide& a,
                           const uint64_t* pjq, int64_t next, const int64_t* flen, const int64_t* pos_pre,
-                          const int32_t* sad_to_out, Seg* segs, int32_t* nseg, Seg* raw, int32_t* nraw, int64_t cap,
-                          int32_t* err, int32_t* out_offsets, int32_t* out_arc_offsets, int64_t n_out,
-                          const int64_t* d_total, hipStream_t st) {
+                          const int32_t* sad_to_out, Seg* chains, int32_t* nchain, Seg* raw, int32_t* nraw, int64_t cap,
+                          int32_t* err, int32_t* out_offsets, int32_t* out_arc_offsets, const int64_t* arc_base,
+                          const int32_t* sad_out, int B, hipStream_t st) {
+  KScope ks_("seed_segments", st);
   if (ns > 0) {
     hipLaunchKernelGGL(k_seed_segments, dim3(grid_for(ns, 256, 16384)), dim3(256), 0, st, ns, ord, removed, a, pjq, next,
-                       flen, pos_pre, sad_to_out, segs, nseg, raw, nraw, cap, err, out_offsets, out_arc_offsets);
+                       flen, pos_pre, sad_to_out, chains, nchain, raw, nraw, cap, err, out_offsets, out_arc_offsets,
+                       arc_base);
     DMSC_LAUNCH_CHECK();
   }
-  hipLaunchKernelGGL(k_set_last_offset, dim3(1), dim3(1), 0, st, out_offsets, n_out, d_total);
+  hipLaunchKernelGGL(k_set_last_offsets, dim3(grid_for(B, 256, 64)), dim3(256), 0, st, out_offsets, arc_base, sad_out, B);
   DMSC_LAUNCH_CHECK();
 }
 

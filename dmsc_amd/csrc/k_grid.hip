@@ -49,15 +49,24 @@ constexpr int GT_X = 64, GT_Y = 16;
 constexpr int GP_W = GT_X + 3, GP_H = GT_Y + 3;
 
 template <bool DUAL>
-__global__ __launch_bounds__(256) void k_gradient(const float* __restrict__ f, int H, int W,
-                                                  int32_t* __restrict__ pw, int32_t* __restrict__ fsucc,
-                                                  int32_t* __restrict__ vsucc, uint32_t* __restrict__ planes,
-                                                  int64_t PW, int WPR) {
+__global__ __launch_bounds__(256) void k_gradient(Batch bt, const TileRef* __restrict__ tiles,
+                                                  int32_t* __restrict__ pw_all, int32_t* __restrict__ fsucc_all,
+                                                  int32_t* __restrict__ vsucc_all, uint32_t* __restrict__ planes,
+                                                  int64_t PW) {
   __shared__ float pix[GP_H][GP_W];
   __shared__ int32_t cell[GT_Y][GT_X][4];
   __shared__ uint8_t lut[kLutSize];
+  const TileRef tr = tiles[blockIdx.x];
+  const FieldDesc fd = bt.fd[tr.field];
+  const float* __restrict__ f = fd.f;
+  const int H = fd.H, W = fd.W, WPR = fd.WPR;
   const int Nx = W + 1, Ny = H + 1;
-  const int x0 = blockIdx.x * GT_X, y0 = blockIdx.y * GT_Y;
+  const int x0 = tr.tx * GT_X, y0 = tr.ty * GT_Y;
+  const int sbase = (int)fd.slot_base, pbase = (int)fd.pix_base;  // batch totals fit int32 (host check)
+  int32_t* __restrict__ pw = pw_all ? pw_all + 4 * fd.slot_base : nullptr;
+  int32_t* __restrict__ fsucc = fsucc_all + fd.slot_base;
+  int32_t* __restrict__ vsucc = vsucc_all + fd.pix_base;
+  const int64_t wbase = fd.word_base;
   const int tid = threadIdx.x;
 
   for (int i = tid; i < kLutSize; i += 256) lut[i] = c_lut.v[i];
@@ -145,7 +154,7 @@ __global__ __launch_bounds__(256) void k_gradient(const float* __restrict__ f, i
         else if (c3 == 4 * (q - 1) + 2) fs = q - 1;
         else fs = q + 1;
       }
-      fsucc[q] = fs;
+      fsucc[q] = sbase + fs;
       if (y < H && x < W) {
         const int p = y * W + x;
         int vs = p;
@@ -155,7 +164,7 @@ __global__ __launch_bounds__(256) void k_gradient(const float* __restrict__ f, i
           else if (c0 == 4 * q + 2) vs = p - W;
           else vs = p + W;
         }
-        vsucc[p] = vs;
+        vsucc[p] = pbase + vs;
       }
     }
     const uint64_t b0 = __ballot(k0), b1 = __ballot(k1), b2 = __ballot(k2), b3 = __ballot(k3);
@@ -163,49 +172,61 @@ __global__ __launch_bounds__(256) void k_gradient(const float* __restrict__ f, i
       const int t = lane >> 1, half = lane & 1;
       const uint64_t bb = t == 0 ? b0 : t == 1 ? b1 : t == 2 ? b2 : b3;
       const int wi = (x0 >> 5) + half;
-      if (wi < WPR) planes[t * PW + (int64_t)y * WPR + wi] = (uint32_t)(bb >> (32 * half));
+      if (wi < WPR) planes[t * PW + wbase + (int64_t)y * WPR + wi] = (uint32_t)(bb >> (32 * half));
     }
   }
 }
 
-void launch_gradient(bool dual, const float* f, const Dims& d, int32_t* pw, int32_t* fsucc, int32_t* vsucc,
-                     uint32_t* planes, hipStream_t st) {
+void launch_gradient(bool dual, const Batch& bt, const TileRef* tiles, int64_t ntiles, int32_t* pw, int32_t* fsucc,
+                     int32_t* vsucc, uint32_t* planes, int64_t PW, hipStream_t st) {
   KScope ks_("gradient", st);
-  dim3 grid(ceil_div(d.Nx, GT_X), ceil_div(d.Ny, GT_Y));
+  if (ntiles == 0) return;
   if (dual)
-    hipLaunchKernelGGL(k_gradient<true>, grid, dim3(256), 0, st, f, d.H, d.W, pw, fsucc, vsucc, planes, d.PW, d.WPR);
+    hipLaunchKernelGGL(k_gradient<true>, dim3((unsigned)ntiles), dim3(256), 0, st, bt, tiles, pw, fsucc, vsucc, planes, PW);
   else
-    hipLaunchKernelGGL(k_gradient<false>, grid, dim3(256), 0, st, f, d.H, d.W, pw, fsucc, vsucc, planes, d.PW, d.WPR);
+    hipLaunchKernelGGL(k_gradient<false>, dim3((unsigned)ntiles), dim3(256), 0, st, bt, tiles, pw, fsucc, vsucc, planes, PW);
   DMSC_LAUNCH_CHECK();
 }
 
 // ------------------------------------------------------------------ ranks
-// index of a critical cell within its list (the reference's fast_crit_map, gradient.hxx:198-231)
-__device__ __forceinline__ int rank_in_plane(const uint32_t* plane, const int32_t* pre, int WPR, int y, int x) {
-  const int64_t w = (int64_t)y * WPR + (x >> 5);
+// index of a critical cell within its (batch-wide) list: the reference's fast_crit_map,
+// gradient.hxx:198-231, as a popcount over the field's bit plane plus the word prefix
+__device__ __forceinline__ int rank_in_plane(const uint32_t* plane, const int32_t* pre, int64_t row_word, int x) {
+  const int64_t w = row_word + (x >> 5);
   const uint32_t below = (1u << (x & 31)) - 1u;
   return pre[w] + __popc(plane[w] & below);
 }
-__device__ __forceinline__ int rank_max_slot(const RankView& rv, int q) {
-  int y = q / rv.Nx, x = q - y * rv.Nx;
-  return rank_in_plane(rv.planes + 3 * rv.PW, rv.pre_max, rv.WPR, y, x);
+// q: global slot index of a face of field f
+__device__ __forceinline__ int rank_max_slot(const RankView& rv, int f, int q) {
+  const FieldDesc& fd = rv.bt.fd[f];
+  const int ql = q - (int)fd.slot_base;
+  const int y = ql / fd.Nx, x = ql - y * fd.Nx;
+  return rank_in_plane(rv.planes + 3 * rv.PW, rv.pre_max, fd.word_base + (int64_t)y * fd.WPR, x);
 }
-__device__ __forceinline__ int rank_min_pixel(const RankView& rv, int p) {
-  int y = p / rv.W, x = p - y * rv.W;
-  return rank_in_plane(rv.planes, rv.pre_min, rv.WPR, y, x);
+// p: global pixel index of a vertex of field f
+__device__ __forceinline__ int rank_min_pixel(const RankView& rv, int f, int p) {
+  const FieldDesc& fd = rv.bt.fd[f];
+  const int pl = p - (int)fd.pix_base;
+  const int y = pl / fd.W, x = pl - y * fd.W;
+  return rank_in_plane(rv.planes, rv.pre_min, fd.word_base + (int64_t)y * fd.WPR, x);
 }
 
 // ------------------------------------------------------------------ critical lists
 // One thread per plane word; set bits are emitted in ascending cell id (the reference's single-
-// thread order, critical_points.hxx:14-65), with the extremum / saddle value.
+// thread order, critical_points.hxx:14-65), with the extremum / saddle value. Ids are field-local;
+// lists are concatenated in field order.
 template <bool DUAL>
-__global__ void k_compact_lists(const float* __restrict__ f, int H, int W, RankView rv,
-                                int32_t* __restrict__ min_ids, float* __restrict__ min_val,
+__global__ void k_compact_lists(RankView rv, int32_t* __restrict__ min_ids, float* __restrict__ min_val,
                                 int32_t* __restrict__ sad_ids, float* __restrict__ sad_val,
-                                int32_t* __restrict__ max_ids, float* __restrict__ max_val) {
-  const int Nx = W + 1;
+                                int32_t* __restrict__ sad_field, int32_t* __restrict__ max_ids,
+                                float* __restrict__ max_val) {
   for (int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; w < rv.PW; w += (int64_t)gridDim.x * blockDim.x) {
-    const int y = (int)(w / rv.WPR), xb = (int)(w - (int64_t)y * rv.WPR) * 32;
+    const int fi = field_of(rv.bt.word_b, rv.bt.B, w);
+    const FieldDesc& fd = rv.bt.fd[fi];
+    const float* __restrict__ f = fd.f;
+    const int H = fd.H, W = fd.W, Nx = fd.Nx;
+    const int64_t lw = w - fd.word_base;
+    const int y = (int)(lw / fd.WPR), xb = (int)(lw - (int64_t)y * fd.WPR) * 32;
     uint32_t m0 = rv.planes[w], m1 = rv.planes[rv.PW + w], m2 = rv.planes[2 * rv.PW + w], m3 = rv.planes[3 * rv.PW + w];
     int o = rv.pre_min[w];
     while (m0) {
@@ -234,47 +255,53 @@ __global__ void k_compact_lists(const float* __restrict__ f, int H, int W, RankV
       if (m1 >> b & 1) {
         sad_ids[o] = 4 * (y * Nx + x) + 1;
         sad_val[o] = f[top_pixel<DUAL>(1, y, x, H, W, f)];
+        if (sad_field) sad_field[o] = fi;
         ++o;
       }
       if (m2 >> b & 1) {
         sad_ids[o] = 4 * (y * Nx + x) + 2;
         sad_val[o] = f[top_pixel<DUAL>(2, y, x, H, W, f)];
+        if (sad_field) sad_field[o] = fi;
         ++o;
       }
     }
   }
 }
 
-void launch_compact_lists(bool dual, const float* f, const Dims& d, const RankView& rv, int32_t* min_ids,
-                          float* min_val, int32_t* sad_ids, float* sad_val, int32_t* max_ids, float* max_val,
-                          hipStream_t st) {
+void launch_compact_lists(bool dual, const RankView& rv, int32_t* min_ids, float* min_val, int32_t* sad_ids,
+                          float* sad_val, int32_t* sad_field, int32_t* max_ids, float* max_val, hipStream_t st) {
   KScope ks_("compact_lists", st);
-  int g = grid_for(d.PW, 256, 8192);
+  if (rv.PW == 0) return;
+  int g = grid_for(rv.PW, 256, 8192);
   if (dual)
-    hipLaunchKernelGGL(k_compact_lists<true>, dim3(g), dim3(256), 0, st, f, d.H, d.W, rv, min_ids, min_val, sad_ids,
-                       sad_val, max_ids, max_val);
+    hipLaunchKernelGGL(k_compact_lists<true>, dim3(g), dim3(256), 0, st, rv, min_ids, min_val, sad_ids, sad_val, sad_field,
+                       max_ids, max_val);
   else
-    hipLaunchKernelGGL(k_compact_lists<false>, dim3(g), dim3(256), 0, st, f, d.H, d.W, rv, min_ids, min_val, sad_ids,
-                       sad_val, max_ids, max_val);
+    hipLaunchKernelGGL(k_compact_lists<false>, dim3(g), dim3(256), 0, st, rv, min_ids, min_val, sad_ids, sad_val,
+                       sad_field, max_ids, max_val);
   DMSC_LAUNCH_CHECK();
 }
 
 // ------------------------------------------------------------------ saddle records
 // Per saddle (list order): the extremum reached by each of its four V-paths and the step count
 // (reference arcs_topology.hxx:67-172). Arc k of a side is its k-th existing start cell; a missing
-// start (border edge, vertex side only) gives target -1 and length 0.
-__global__ void k_saddle_records(int H, int W, RankView rv, const int32_t* __restrict__ sad_ids, int64_t nsad,
-                                 const uint64_t* __restrict__ pj_face, const uint64_t* __restrict__ pj_vert,
-                                 int32_t* __restrict__ cmax, int32_t* __restrict__ cmin, int32_t* __restrict__ lmax,
-                                 int32_t* __restrict__ lmin) {
-  const int Nx = W + 1;
+// start (border edge, vertex side only) gives target -1 and length 0. Forest indices are global
+// (field base + local slot / pixel); extremum indices are batch-wide list ranks.
+__global__ void k_saddle_records(RankView rv, const int32_t* __restrict__ sad_ids, const int32_t* __restrict__ sad_field,
+                                 int64_t nsad, const uint64_t* __restrict__ pj_face,
+                                 const uint64_t* __restrict__ pj_vert, int32_t* __restrict__ cmax,
+                                 int32_t* __restrict__ cmin, int32_t* __restrict__ lmax, int32_t* __restrict__ lmin) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nsad; i += (int64_t)gridDim.x * blockDim.x) {
+    const int fi = sad_field ? sad_field[i] : 0;
+    const FieldDesc& fd = rv.bt.fd[fi];
+    const int Nx = fd.Nx, W = fd.W, H = fd.H;
     const int id = sad_ids[i], q = id >> 2, t = id & 3;
     const int y = q / Nx, x = q - y * Nx;
-    const int f1 = q, f2 = (t == 1) ? q + Nx : q + 1;
+    const int sb = (int)fd.slot_base, pb = (int)fd.pix_base;
+    const int f1 = sb + q, f2 = sb + ((t == 1) ? q + Nx : q + 1);
     const uint64_t a = pj_face[f1], b = pj_face[f2];
-    cmax[2 * i] = rank_max_slot(rv, (int)(uint32_t)a);
-    cmax[2 * i + 1] = rank_max_slot(rv, (int)(uint32_t)b);
+    cmax[2 * i] = rank_max_slot(rv, fi, (int)(uint32_t)a);
+    cmax[2 * i + 1] = rank_max_slot(rv, fi, (int)(uint32_t)b);
     lmax[2 * i] = (int)(a >> 32);
     lmax[2 * i + 1] = (int)(b >> 32);
     int v[2], nv = 0;
@@ -287,8 +314,8 @@ __global__ void k_saddle_records(int H, int W, RankView rv, const int32_t* __res
     }
     for (int k = 0; k < 2; ++k) {
       if (k < nv) {
-        const uint64_t c = pj_vert[v[k]];
-        cmin[2 * i + k] = rank_min_pixel(rv, (int)(uint32_t)c);
+        const uint64_t c = pj_vert[pb + v[k]];
+        cmin[2 * i + k] = rank_min_pixel(rv, fi, (int)(uint32_t)c);
         lmin[2 * i + k] = (int)(c >> 32);
       } else {
         cmin[2 * i + k] = -1;
@@ -298,45 +325,55 @@ __global__ void k_saddle_records(int H, int W, RankView rv, const int32_t* __res
   }
 }
 
-void launch_saddle_records(const Dims& d, const RankView& rv, const int32_t* sad_ids, int64_t nsad,
+void launch_saddle_records(const RankView& rv, const int32_t* sad_ids, const int32_t* sad_field, int64_t nsad,
                            const uint64_t* pj_face, const uint64_t* pj_vert, int32_t* cmax, int32_t* cmin,
                            int32_t* lmax, int32_t* lmin, hipStream_t st) {
   KScope ks_("saddle_records", st);
   if (nsad == 0) return;
-  hipLaunchKernelGGL(k_saddle_records, dim3(grid_for(nsad, 256, 16384)), dim3(256), 0, st, d.H, d.W, rv, sad_ids, nsad,
+  hipLaunchKernelGGL(k_saddle_records, dim3(grid_for(nsad, 256, 16384)), dim3(256), 0, st, rv, sad_ids, sad_field, nsad,
                      pj_face, pj_vert, cmax, cmin, lmax, lmin);
   DMSC_LAUNCH_CHECK();
 }
 
 // ------------------------------------------------------------------ labels
-// label(cell) = output index of the thresholded-UF root of its V-path terminal, -1 for the void
-// or a cancelled root (reference cell_groups.hxx:96-229).
+// label(cell) = field-local output index of the thresholded-UF root of its V-path terminal, -1 for
+// the void or a cancelled root (reference cell_groups.hxx:96-229). The output is the concatenation
+// of the fields' label arrays (slot / pixel order).
+__device__ __forceinline__ int32_t label_of(int r, const int32_t* __restrict__ to_out, int32_t base) {
+  if (r < 0) return -1;
+  const int32_t o = to_out[r];
+  return o >= 0 ? o - base : -1;
+}
 __global__ void k_labels_faces(int64_t S, RankView rv, const uint64_t* __restrict__ pj, const int32_t* __restrict__ root_t,
-                               const int32_t* __restrict__ to_out, int32_t* __restrict__ out) {
+                               const int32_t* __restrict__ to_out, const int32_t* __restrict__ out_base,
+                               int32_t* __restrict__ out) {
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < S; q += (int64_t)gridDim.x * blockDim.x) {
-    int r = root_t[rank_max_slot(rv, (int)(uint32_t)pj[q])];
-    out[q] = r >= 0 ? to_out[r] : -1;
+    const int fi = field_of(rv.bt.slot_b, rv.bt.B, q);
+    out[q] = label_of(root_t[rank_max_slot(rv, fi, (int)(uint32_t)pj[q])], to_out, out_base[fi]);
   }
 }
 __global__ void k_labels_verts(int64_t N, RankView rv, const uint64_t* __restrict__ pj, const int32_t* __restrict__ root_t,
-                               const int32_t* __restrict__ to_out, int32_t* __restrict__ out) {
+                               const int32_t* __restrict__ to_out, const int32_t* __restrict__ out_base,
+                               int32_t* __restrict__ out) {
   for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < N; p += (int64_t)gridDim.x * blockDim.x) {
-    int r = root_t[rank_min_pixel(rv, (int)(uint32_t)pj[p])];
-    out[p] = r >= 0 ? to_out[r] : -1;
+    const int fi = field_of(rv.bt.pix_b, rv.bt.B, p);
+    out[p] = label_of(root_t[rank_min_pixel(rv, fi, (int)(uint32_t)pj[p])], to_out, out_base[fi]);
   }
 }
-void launch_labels_faces(const Dims& d, const RankView& rv, const uint64_t* pj_face, const int32_t* root_t,
-                         const int32_t* to_out, int32_t* out, hipStream_t st) {
+void launch_labels_faces(const RankView& rv, int64_t S, const uint64_t* pj_face, const int32_t* root_t,
+                         const int32_t* to_out, const int32_t* out_base, int32_t* out, hipStream_t st) {
   KScope ks_("labels_faces", st);
-  hipLaunchKernelGGL(k_labels_faces, dim3(grid_for(d.S, 256, 16384)), dim3(256), 0, st, d.S, rv, pj_face, root_t, to_out,
-                     out);
+  if (S == 0) return;
+  hipLaunchKernelGGL(k_labels_faces, dim3(grid_for(S, 256, 16384)), dim3(256), 0, st, S, rv, pj_face, root_t, to_out,
+                     out_base, out);
   DMSC_LAUNCH_CHECK();
 }
-void launch_labels_verts(const Dims& d, const RankView& rv, const uint64_t* pj_vert, const int32_t* root_t,
-                         const int32_t* to_out, int32_t* out, hipStream_t st) {
+void launch_labels_verts(const RankView& rv, int64_t N, const uint64_t* pj_vert, const int32_t* root_t,
+                         const int32_t* to_out, const int32_t* out_base, int32_t* out, hipStream_t st) {
   KScope ks_("labels_verts", st);
-  hipLaunchKernelGGL(k_labels_verts, dim3(grid_for(d.N, 256, 16384)), dim3(256), 0, st, d.N, rv, pj_vert, root_t, to_out,
-                     out);
+  if (N == 0) return;
+  hipLaunchKernelGGL(k_labels_verts, dim3(grid_for(N, 256, 16384)), dim3(256), 0, st, N, rv, pj_vert, root_t, to_out,
+                     out_base, out);
   DMSC_LAUNCH_CHECK();
 }
 

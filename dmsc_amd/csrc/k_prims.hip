@@ -209,17 +209,27 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint32_t* __res
   }
 }
 
-void radix_sort_pairs(uint32_t* keys, int32_t* vals, uint32_t* keys_tmp, int32_t* vals_tmp, int64_t n, Arena& ar,
-                      int slot, hipStream_t st) {
+__global__ void k_copy_pairs(const uint32_t* __restrict__ ks, const int32_t* __restrict__ vs, uint32_t* __restrict__ kd,
+                             int32_t* __restrict__ vd, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    kd[i] = ks[i];
+    vd[i] = vs[i];
+  }
+}
+
+// Sorts on the low `bits` bits of the keys (ceil(bits / 8) passes); the result is in (keys, vals).
+void radix_sort_pairs_bits(uint32_t* keys, int32_t* vals, uint32_t* keys_tmp, int32_t* vals_tmp, int64_t n, int bits,
+                           Arena& ar, int slot, hipStream_t st) {
   KScope ks_("radix_sort", st);
-  if (n <= 1) return;
+  if (n <= 1 || bits <= 0) return;
+  const int npass = (bits + 7) / 8 > 4 ? 4 : (bits + 7) / 8;
   const int64_t nb = (n + RS_TILE - 1) / RS_TILE;
   int32_t* counts = ar.get<int32_t>(slot, (size_t)(256 * nb));
   uint32_t* ka = keys;
   int32_t* va = vals;
   uint32_t* kb = keys_tmp;
   int32_t* vb = vals_tmp;
-  for (int pass = 0; pass < 4; ++pass) {
+  for (int pass = 0; pass < npass; ++pass) {
     const int shift = 8 * pass;
     hipLaunchKernelGGL(k_rs_hist, dim3((unsigned)nb), dim3(RS_THREADS), 0, st, ka, n, shift, counts, nb);
     DMSC_LAUNCH_CHECK();
@@ -229,7 +239,15 @@ void radix_sort_pairs(uint32_t* keys, int32_t* vals, uint32_t* keys_tmp, int32_t
     std::swap(ka, kb);
     std::swap(va, vb);
   }
-  // four passes: the result is back in (keys, vals)
+  if (ka != keys) {  // odd number of passes
+    hipLaunchKernelGGL(k_copy_pairs, dim3(grid_for(n, 256, 16384)), dim3(256), 0, st, ka, va, keys, vals, n);
+    DMSC_LAUNCH_CHECK();
+  }
+}
+
+void radix_sort_pairs(uint32_t* keys, int32_t* vals, uint32_t* keys_tmp, int32_t* vals_tmp, int64_t n, Arena& ar,
+                      int slot, hipStream_t st) {
+  radix_sort_pairs_bits(keys, vals, keys_tmp, vals_tmp, n, 32, ar, slot, st);
 }
 
 // ------------------------------------------------------------------ pointer jumping
@@ -277,6 +295,26 @@ void pointer_jump(uint64_t* pj, int64_t n, int32_t* err, hipStream_t st) {
   hipLaunchKernelGGL(k_pj, dim3(grid_for(n, 256, 1 << 20)), dim3(256), 0, st, pj, n, err);
   DMSC_LAUNCH_CHECK();
 }
+
+// ------------------------------------------------------------------ per-field bases
+// out[f] = pre[starts[f]] (an exclusive prefix sampled at each field's first element), or *total
+// when the field starts at the end; f = 0..B.
+template <class T, class S>
+__global__ void k_bases_at(const T* __restrict__ pre, const S* __restrict__ starts, int64_t n, const T* __restrict__ total,
+                           int B, T* __restrict__ out) {
+  for (int f = blockIdx.x * blockDim.x + threadIdx.x; f <= B; f += gridDim.x * blockDim.x) {
+    const int64_t s = (int64_t)starts[f];
+    out[f] = s < n ? pre[s] : *total;
+  }
+}
+template <class T, class S>
+void bases_at(const T* pre, const S* starts, int64_t n, const T* total, int B, T* out, hipStream_t st) {
+  hipLaunchKernelGGL((k_bases_at<T, S>), dim3(grid_for(B + 1, 256, 64)), dim3(256), 0, st, pre, starts, n, total, B, out);
+  DMSC_LAUNCH_CHECK();
+}
+template void bases_at<int32_t, int64_t>(const int32_t*, const int64_t*, int64_t, const int32_t*, int, int32_t*, hipStream_t);
+template void bases_at<int32_t, int32_t>(const int32_t*, const int32_t*, int64_t, const int32_t*, int, int32_t*, hipStream_t);
+template void bases_at<int64_t, int32_t>(const int64_t*, const int32_t*, int64_t, const int64_t*, int, int64_t*, hipStream_t);
 
 // ------------------------------------------------------------------ fills
 __global__ void k_fill_i32(int32_t* p, int32_t v, int64_t n) {

@@ -67,7 +67,7 @@ __global__ void k_persist_find(PersistSide s0, PersistSide s1, int round) {
 
 // Decides one pending saddle i; returns true when it must wait for a later round.
 template <bool DUAL>
-__device__ __forceinline__ bool persist_apply_one(const PersistSide& s, int i, uint64_t tag, float thr,
+__device__ __forceinline__ bool persist_apply_one(const PersistSide& s, int i, uint64_t tag,
                                                   uint8_t* __restrict__ removed) {
   const int ra = s.rr[2 * i];
   if (ra == -2) return false;
@@ -97,6 +97,7 @@ __device__ __forceinline__ bool persist_apply_one(const PersistSide& s, int i, u
     s.pers[i] = p;
     s.pair[i] = dying;
     s.par_g[dying] = surv;
+    const float thr = s.thr[s.sfield ? s.sfield[i] : 0];
     if (p <= thr) {
       s.alive[dying] = 0;
       removed[i] = 1;
@@ -117,7 +118,7 @@ __device__ __forceinline__ bool persist_apply_one(const PersistSide& s, int i, u
 }
 
 template <bool DUAL>
-__global__ void k_persist_apply(PersistSide s0, PersistSide s1, int round, float thr, uint8_t* __restrict__ removed) {
+__global__ void k_persist_apply(PersistSide s0, PersistSide s1, int round, uint8_t* __restrict__ removed) {
   const int cur = round & 1;
   const int n0 = s0.cnt[cur], n1 = s1.cnt[cur];
   const uint64_t tag = (uint64_t)(0xFFFFFFFFu - (uint32_t)round) << 32;
@@ -131,10 +132,10 @@ __global__ void k_persist_apply(PersistSide s0, PersistSide s1, int round, float
       // two inlined copies instead of selecting a struct at run time (which would spill it)
       if (side1) {
         k = s1.wl[cur][jj - n0];
-        wait = persist_apply_one<DUAL>(s1, k, tag, thr, removed);
+        wait = persist_apply_one<DUAL>(s1, k, tag, removed);
       } else {
         k = s0.wl[cur][jj];
-        wait = persist_apply_one<DUAL>(s0, k, tag, thr, removed);
+        wait = persist_apply_one<DUAL>(s0, k, tag, removed);
       }
     }
     const int p0 = wave_append(&s0.cnt[cur ^ 1], wait && !side1);
@@ -146,16 +147,16 @@ __global__ void k_persist_apply(PersistSide s0, PersistSide s1, int round, float
   }
 }
 
-void launch_persist_round(bool dual, const PersistSide& s0, const PersistSide& s1, int round, float thr,
-                          uint8_t* removed, int64_t max_items, hipStream_t st) {
+void launch_persist_round(bool dual, const PersistSide& s0, const PersistSide& s1, int round, uint8_t* removed,
+                          int64_t max_items, hipStream_t st) {
   KScope ks_("persist_round", st);
   const int g = grid_for(max_items, 256, 2048);
   hipLaunchKernelGGL(k_persist_find, dim3(g), dim3(256), 0, st, s0, s1, round);
   DMSC_LAUNCH_CHECK();
   if (dual)
-    hipLaunchKernelGGL(k_persist_apply<true>, dim3(g), dim3(256), 0, st, s0, s1, round, thr, removed);
+    hipLaunchKernelGGL(k_persist_apply<true>, dim3(g), dim3(256), 0, st, s0, s1, round, removed);
   else
-    hipLaunchKernelGGL(k_persist_apply<false>, dim3(g), dim3(256), 0, st, s0, s1, round, thr, removed);
+    hipLaunchKernelGGL(k_persist_apply<false>, dim3(g), dim3(256), 0, st, s0, s1, round, removed);
   DMSC_LAUNCH_CHECK();
 }
 
@@ -256,12 +257,18 @@ __global__ void k_emit_side(int64_t ns, const int32_t* __restrict__ ord, const u
                             const int32_t* __restrict__ sad_to_out, const int32_t* __restrict__ c,
                             const int32_t* __restrict__ root_t, const int32_t* __restrict__ to_out,
                             const float* __restrict__ pers, const int32_t* __restrict__ pair,
-                            const int32_t* __restrict__ row_pre, const int32_t* __restrict__ sad_ids,
+                            const int32_t* __restrict__ row_pre, const int32_t* __restrict__ sad_ids, EmitBases eb,
                             float* __restrict__ out_p, int32_t* __restrict__ out_pp, int32_t* __restrict__ out_e,
                             int32_t* __restrict__ out_sad) {
   for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < ns; k += (int64_t)gridDim.x * blockDim.x) {
     const int i = ord[k];
     if (removed[i]) continue;
+    int sbase = 0, ebase = 0;
+    if (eb.sfield) {
+      const int fi = eb.sfield[i];
+      sbase = eb.sad_out[fi];
+      ebase = eb.ext_out[fi];
+    }
     const int so = sad_to_out[i];
     if (out_sad) out_sad[so] = sad_ids[i];
     out_p[so] = pers[i];
@@ -269,50 +276,71 @@ __global__ void k_emit_side(int64_t ns, const int32_t* __restrict__ ord, const u
     int pp = -1;
     if (pe >= 0) {
       const int r = root_t[pe];
-      pp = r >= 0 ? to_out[r] : -1;
+      pp = (r >= 0 && to_out[r] >= 0) ? to_out[r] - ebase : -1;
     }
     out_pp[so] = pp;
     const int a = c[2 * i], b = c[2 * i + 1];
     const int r1 = a >= 0 ? root_t[a] : -1, r2 = b >= 0 ? root_t[b] : -1;
     int row = row_pre[k];
     if (r1 != -1 && to_out[r1] != -1) {
-      out_e[2 * row] = so;
-      out_e[2 * row + 1] = to_out[r1];
+      out_e[2 * (int64_t)row] = so - sbase;
+      out_e[2 * (int64_t)row + 1] = to_out[r1] - ebase;
       ++row;
     }
     if (r1 != r2 && r2 != -1 && to_out[r2] != -1) {
-      out_e[2 * row] = so;
-      out_e[2 * row + 1] = to_out[r2];
+      out_e[2 * (int64_t)row] = so - sbase;
+      out_e[2 * (int64_t)row + 1] = to_out[r2] - ebase;
     }
   }
 }
 void launch_emit_side(int64_t ns, const int32_t* ord, const uint8_t* removed, const int32_t* sad_to_out, const int32_t* c,
                       const int32_t* root_t, const int32_t* to_out, const float* pers, const int32_t* pair,
-                      const int32_t* row_pre, const int32_t* sad_ids, float* out_p, int32_t* out_pp, int32_t* out_e,
-                      int32_t* out_sad, hipStream_t st) {
+                      const int32_t* row_pre, const int32_t* sad_ids, const EmitBases& eb, float* out_p, int32_t* out_pp,
+                      int32_t* out_e, int32_t* out_sad, hipStream_t st) {
   KScope ks_("emit_side", st);
   if (ns == 0) return;
   hipLaunchKernelGGL(k_emit_side, dim3(grid_for(ns, 256, 16384)), dim3(256), 0, st, ns, ord, removed, sad_to_out, c,
-                     root_t, to_out, pers, pair, row_pre, sad_ids, out_p, out_pp, out_e, out_sad);
+                     root_t, to_out, pers, pair, row_pre, sad_ids, eb, out_p, out_pp, out_e, out_sad);
   DMSC_LAUNCH_CHECK();
 }
 
-// side order arrays from the ascending (value, id) order: forward or reversed
-// Both sweep orders (ascending / descending) and their inverses (saddle index -> position).
-__global__ void k_orders(int64_t ns, const int32_t* __restrict__ asc, int32_t* __restrict__ fwd, int32_t* __restrict__ rev,
+// Both sweep orders (ascending / descending within each field) and their inverses (saddle index ->
+// position). `asc` is grouped by field, ascending (value, id) inside each group.
+__global__ void k_orders(int64_t ns, const int32_t* __restrict__ asc, const int32_t* __restrict__ sfield,
+                         const int32_t* __restrict__ sad_b, int32_t* __restrict__ fwd, int32_t* __restrict__ rev,
                          int32_t* __restrict__ pos_fwd, int32_t* __restrict__ pos_rev) {
   for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < ns; k += (int64_t)gridDim.x * blockDim.x) {
     const int v = asc[k];
+    int64_t lo = 0, hi = ns;
+    if (sfield) {
+      const int fi = sfield[v];
+      lo = sad_b[fi];
+      hi = sad_b[fi + 1];
+    }
+    const int64_t r = lo + hi - 1 - k;
     fwd[k] = v;
-    rev[ns - 1 - k] = v;
+    rev[r] = v;
     pos_fwd[v] = (int)k;
-    pos_rev[v] = (int)(ns - 1 - k);
+    pos_rev[v] = (int)r;
   }
 }
-void launch_orders(int64_t ns, const int32_t* asc, int32_t* fwd, int32_t* rev, int32_t* pos_fwd, int32_t* pos_rev,
-                   hipStream_t st) {
+void launch_orders(int64_t ns, const int32_t* asc, const int32_t* sfield, const int32_t* sad_b, int32_t* fwd,
+                   int32_t* rev, int32_t* pos_fwd, int32_t* pos_rev, hipStream_t st) {
   if (ns == 0) return;
-  hipLaunchKernelGGL(k_orders, dim3(grid_for(ns, 256, 16384)), dim3(256), 0, st, ns, asc, fwd, rev, pos_fwd, pos_rev);
+  hipLaunchKernelGGL(k_orders, dim3(grid_for(ns, 256, 16384)), dim3(256), 0, st, ns, asc, sfield, sad_b, fwd, rev,
+                     pos_fwd, pos_rev);
+  DMSC_LAUNCH_CHECK();
+}
+
+// keys[k] = field of saddle idx[k] (the second, field-major pass of the saddle sort)
+__global__ void k_gather_field(int64_t n, const int32_t* __restrict__ idx, const int32_t* __restrict__ sfield,
+                               uint32_t* __restrict__ keys) {
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x)
+    keys[k] = (uint32_t)sfield[idx[k]];
+}
+void launch_gather_field(int64_t n, const int32_t* idx, const int32_t* sfield, uint32_t* keys, hipStream_t st) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_gather_field, dim3(grid_for(n, 256, 16384)), dim3(256), 0, st, n, idx, sfield, keys);
   DMSC_LAUNCH_CHECK();
 }
 

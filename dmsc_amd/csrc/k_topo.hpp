@@ -23,11 +23,13 @@ struct PersistSide {
   int32_t* wl[2];       // ping-pong worklists of pending saddle indices
   int32_t* cnt;         // [2] worklist sizes (device)
   int32_t* err;         // device error flags
+  const float* thr;     // per field persistence threshold
+  const int32_t* sfield;  // per saddle field index (null: a single field)
   int maxside;
 };
 
-void launch_persist_round(bool dual, const PersistSide& s0, const PersistSide& s1, int round, float thr,
-                          uint8_t* removed, int64_t max_items, hipStream_t st);
+void launch_persist_round(bool dual, const PersistSide& s0, const PersistSide& s1, int round, uint8_t* removed,
+                          int64_t max_items, hipStream_t st);
 void launch_uf_flatten(int32_t* par, int32_t* root, int64_t n, hipStream_t st);
 void launch_u8_to_i32(const uint8_t* a, int32_t* o, int64_t n, bool invert, hipStream_t st);
 void launch_mask_prefix(const uint8_t* alive, int32_t* pre, int64_t n, hipStream_t st);
@@ -36,20 +38,31 @@ void launch_side_counts(int64_t ns, const int32_t* ord, const uint8_t* removed, 
 void launch_sad_to_out(int64_t ns, const int32_t* ord, const uint8_t* removed, const int32_t* surv_pre,
                        int32_t* sad_to_out, hipStream_t st);
 void launch_emit_points(int64_t n, const int32_t* ids, const int32_t* to_out, int32_t* out, hipStream_t st);
+// Output indices are batch-wide positions; index values written into the outputs are made
+// field-local with the per-field bases (sfield null: a single field, bases unused).
+struct EmitBases {
+  const int32_t* sfield;    // per saddle field
+  const int32_t* sad_out;   // [B+1] first output saddle of each field
+  const int32_t* ext_out;   // [B+1] first output extremum (this side) of each field
+};
 void launch_emit_side(int64_t ns, const int32_t* ord, const uint8_t* removed, const int32_t* sad_to_out, const int32_t* c,
                       const int32_t* root_t, const int32_t* to_out, const float* pers, const int32_t* pair,
-                      const int32_t* row_pre, const int32_t* sad_ids, float* out_p, int32_t* out_pp, int32_t* out_e,
-                      int32_t* out_sad, hipStream_t st);
-void launch_orders(int64_t ns, const int32_t* asc, int32_t* fwd, int32_t* rev, int32_t* pos_fwd, int32_t* pos_rev,
-                   hipStream_t st);
+                      const int32_t* row_pre, const int32_t* sad_ids, const EmitBases& eb, float* out_p, int32_t* out_pp,
+                      int32_t* out_e, int32_t* out_sad, hipStream_t st);
+// Sweep orders from the ascending order `asc`, grouped by field (sad_b: [B+1] saddle bases):
+// fwd = asc, rev = asc reversed within each field's range, and their inverses.
+void launch_orders(int64_t ns, const int32_t* asc, const int32_t* sfield, const int32_t* sad_b, int32_t* fwd,
+                   int32_t* rev, int32_t* pos_fwd, int32_t* pos_rev, hipStream_t st);
+void launch_gather_field(int64_t n, const int32_t* idx, const int32_t* sfield, uint32_t* keys, hipStream_t st);
 void launch_sort_keys(int64_t n, const float* v, uint32_t* keys, hipStream_t st);
 
 // ------------------------------------------------------------------ arc geometry (k_arcs.hip)
 // One traced side of the arc geometry (faces for ridges/"max", vertices for valleys/"min").
 struct ArcSide {
   int faces;               // 1: face V-paths (slot-indexed successors), 0: vertex V-paths (pixel-indexed)
-  const int32_t* succ;     // successor array of the forest
-  int Nx, W, H;            // geometry for cell-id encoding
+  const int32_t* succ;     // successor array of the forest (global slot / pixel indices)
+  const FieldDesc* fd;     // per field geometry and bases
+  const int32_t* sfield;   // per saddle field (null: a single field)
   int64_t ns;              // saddles
   int64_t next;            // extrema of this side
   const int32_t* sad_ids;
@@ -92,21 +105,29 @@ void launch_geo_q(const GeoState& g, int64_t next, uint64_t* pjq, hipStream_t st
 void launch_final_lengths(int64_t ns, const int32_t* ord, const uint8_t* removed, const ArcSide& a,
                           const uint64_t* pjq, int64_t next, int64_t* flen /*2 per position*/,
                           int64_t* per_pos /*per position total*/, hipStream_t st);
+// A piece of a merged arc: a raw arc (a = arc index) or a chain of links (from extremum a up to its
+// ancestor b, -1 = the void). od = output offset << 1 | forward bit.
 struct Seg {
-  int32_t kind;  // 0 RAW (a = arc index), 1 CHAIN (a = from extremum, b = up-to ancestor, -1 = void), 2 LINK (a = dead)
   int32_t a, b;
-  int32_t dir;   // 1 forward, 0 reversed
-  int64_t off;   // output offset
+  int64_t od;
 };
+// Per extremum, after the geometry rounds: cumulative counts of the raw / chain pieces that the
+// links on its path to the final root expand into (pointer-jumped (root, count) pairs).
+void launch_link_counts(const ArcSide& a, const GeoState& g, const uint64_t* pjq, int64_t next, uint64_t* cum_r,
+                        uint64_t* cum_c, hipStream_t st);
+// arc_base: [B+1] first output cell of each field; sad_out: [B+1] first output saddle of each field.
+// A field's offsets array has one more entry than its saddles, so field f's entries start at
+// sad_out[f] + f in the concatenated offsets output.
 void launch_seed_segments(int64_t ns, const int32_t* ord, const uint8_t* removed, const ArcSide& a,
                           const uint64_t* pjq, int64_t next, const int64_t* flen, const int64_t* pos_pre,
-                          const int32_t* sad_to_out, Seg* segs, int32_t* nseg, Seg* raw, int32_t* nraw, int64_t cap,
-                          int32_t* err, int32_t* out_offsets, int32_t* out_arc_offsets, int64_t n_out,
-                          const int64_t* d_total, hipStream_t st);
-void launch_expand_segments(const ArcSide& a, const GeoState& g, const uint64_t* pjq, const Seg* in, const int32_t* nin,
-                            Seg* out, int32_t* nout, Seg* raw, int32_t* nraw, int64_t max_in, int64_t cap,
-                            hipStream_t st);
+                          const int32_t* sad_to_out, Seg* chains, int32_t* nchain, Seg* raw, int32_t* nraw, int64_t cap,
+                          int32_t* err, int32_t* out_offsets, int32_t* out_arc_offsets, const int64_t* arc_base,
+                          const int32_t* sad_out, int B, hipStream_t st);
+// One level of expansion: every chain becomes the raw pieces and sub-chains of its links.
+void launch_expand_segments(const ArcSide& a, const GeoState& g, const uint64_t* pjq, const uint64_t* cum_r,
+                            const uint64_t* cum_c, const Seg* in, const int32_t* nin, Seg* out, int32_t* nout, Seg* raw,
+                            int32_t* nraw, int64_t max_in, int64_t cap, hipStream_t st);
 void launch_copy_raw(const ArcSide& a, const Seg* raw, const int32_t* nraw, int64_t max_raw, int32_t* out,
-                     int64_t out_total, int32_t* err, int32_t* longlist, int32_t* nlong, hipStream_t st);
+                     int64_t out_total, int32_t* err, hipStream_t st);
 
 }  // namespace dmsc

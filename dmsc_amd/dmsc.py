@@ -7,7 +7,7 @@ gfx950 kernels behind ``dmsc_amd.csrc.dmsc_gpu``; there is no CPU fallback -- a 
 """
 
 import time
-from dataclasses import astuple, dataclass, fields
+from dataclasses import dataclass, fields
 
 import torch
 
@@ -103,7 +103,8 @@ class MSComplex:
         return self._csr_slice(self.valleys, self.valley_offsets, self.valley_arc_offsets, saddle_idx, split_arcs)
 
     def __iter__(self):
-        return iter(astuple(self))
+        # the result tensors themselves, in field order (no copies)
+        return iter(tuple(getattr(self, f.name) for f in fields(self)))
 
     def plot(self, *args, **kwargs):
         raise NotImplementedError("plotting dashboards are not part of dmsc_amd (use the reference's dmsc.plots)")

@@ -46,11 +46,14 @@ enum {
 enum { DMSC_NUM_OUTPUTS = 19 };
 enum { DMSC_DTYPE_INT32 = 0, DMSC_DTYPE_FLOAT32 = 1 };
 
-/* Allocator callback: return device memory for output `slot` of field `field` (0 for
- * dmsc_extract, the batch index for dmsc_extract_batch) with the given shape and dtype (ndim 1 or
- * 2, row-major). Called once per slot 1..18 per field, possibly with zero elements (the pointer is
- * then never dereferenced). Return NULL on failure. dmsc_extract_batch may call it concurrently
- * from several host threads (for different fields). */
+/* Allocator callback: return device memory for output `slot` with the given shape and dtype (ndim
+ * 1 or 2, row-major), possibly with zero elements (the pointer is then never dereferenced). Return
+ * NULL on failure. Called from the calling thread only.
+ *   field >= 0: the outputs of one field (0 for dmsc_extract, the batch index when a batch group
+ *     holds a single field); called once per slot 1..18 with that field's shape.
+ *   field == -1 (dmsc_extract_batch): one buffer of dims[0] elements holding `slot` of several
+ *     consecutive fields back to back in field order; out[b].data[slot] then points into it and
+ *     out[b].dims / ndim give field b's own shape. */
 typedef void* (*dmsc_alloc_fn)(void* user, int32_t field, int32_t slot, int32_t ndim, const int64_t* dims,
                                int32_t dtype);
 
@@ -80,7 +83,10 @@ dmsc_status dmsc_extract(dmsc_context* ctx, const float* d_field, int32_t H, int
 
 /* Batch of fields of independent sizes and thresholds (fields[b] is a contiguous H[b] x W[b]
  * device array). Generalises extract_batched_dmsc_gpu (reference dmsc_gpu.cpp:84-126), which
- * requires one shape and one threshold. out must hold B entries. */
+ * requires one shape and one threshold. out must hold B entries. Fields are processed in groups
+ * (up to 2^28 cells per group): all fields of a group share every kernel launch and every
+ * persistence / geometry round, so a batch of small fields costs about as many launches as one
+ * field. Results equal dmsc_extract on each field alone. */
 dmsc_status dmsc_extract_batch(dmsc_context* ctx, int32_t B, const float* const* d_fields, const int32_t* H,
                                const int32_t* W, const float* persistence_thresholds, uint32_t flags,
                                dmsc_stream_t stream, dmsc_alloc_fn alloc, void* alloc_user, dmsc_outputs* out);

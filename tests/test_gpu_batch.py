@@ -1,0 +1,97 @@
+"""Batched extraction (all fields of a call share every launch and every persistence / geometry
+round) against the oracle, field by field: mixed shapes and thresholds, all flag combinations,
+heavy ties across field boundaries, the 3-D batch entry, and batches split into several groups."""
+
+import itertools
+import os
+
+import numpy as np
+import pytest
+
+from dmsc_amd import synthetic
+from tests.parity_util import diff_outputs, oracle_extract
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(1, 1), (1, 2), (2, 1), (1, 9), (9, 1), (2, 2), (3, 5), (7, 11), (16, 64), (17, 65), (31, 63), (64, 129),
+          (100, 37), (40, 33), (5, 200)]
+THRS = (-1.0, 0.0, 0.1, 0.5)
+
+
+def run_list(fields, thrs, **kw):
+    import torch
+
+    from dmsc_amd.csrc import dmsc_gpu
+
+    args = [kw.get(k, True) for k in ("return_gradient",)] + [kw.get("is_dual", False)] + \
+        [kw.get(k, True) for k in ("tmax", "tmin", "gmax", "gmin")]
+    res = dmsc_gpu.extract_dmsc_list([torch.from_numpy(np.ascontiguousarray(f)).cuda() for f in fields],
+                                     [float(t) for t in thrs], *args)
+    torch.cuda.synchronize()
+    return [[r[i].cpu().numpy() for i in range(19)] for r in res]
+
+
+def check_batch(fields, thrs, **kw):
+    print(f"[batch] {len(fields)} fields {kw}", flush=True)
+    got = run_list(fields, thrs, **kw)
+    assert len(got) == len(fields)
+    for j, (f, t, g) in enumerate(zip(fields, thrs, got)):
+        bad = diff_outputs(g, oracle_extract(f, t, **kw))
+        assert not bad, f"field {j} shape={f.shape} thr={t} {kw}: {bad}"
+
+
+@pytest.mark.parametrize("is_dual", [False, True])
+def test_mixed_shapes_and_thresholds(is_dual):
+    rng = np.random.default_rng(11)
+    fields = [rng.random(s).astype(np.float32) for s in SHAPES]
+    thrs = [THRS[j % len(THRS)] for j in range(len(fields))]
+    check_batch(fields, thrs, is_dual=is_dual)
+
+
+@pytest.mark.parametrize("is_dual", [False, True])
+@pytest.mark.parametrize("flags", list(itertools.product([True, False], repeat=4)))
+def test_flag_combinations(is_dual, flags):
+    tmax, tmin, gmax, gmin = flags
+    rng = np.random.default_rng(5)
+    fields = [rng.random(s).astype(np.float32) for s in [(9, 1), (33, 20), (12, 70), (1, 1), (48, 48)]]
+    thrs = [0.0, 0.2, -1.0, 0.1, 0.05]
+    check_batch(fields, thrs, return_gradient=not gmin, is_dual=is_dual, tmax=tmax, tmin=tmin, gmax=gmax, gmin=gmin)
+
+
+@pytest.mark.parametrize("is_dual", [False, True])
+def test_ties_across_fields(is_dual):
+    # identical values in every field: the batch-wide sorts must keep each field's (value, id) order
+    fields = [synthetic.quantized_noise(h, w, lv, seed=lv).numpy() for h, w, lv in
+              [(30, 41, 2), (30, 41, 2), (17, 9, 3), (64, 64, 2), (5, 5, 1)]]
+    fields.append(np.zeros((6, 7), np.float32))
+    check_batch(fields, [0.0, 0.3, 0.05, 0.0, 0.0, 0.0], is_dual=is_dual)
+
+
+def test_gaussian_fields_with_per_field_thresholds():
+    fields, thrs = synthetic.mixed_batch(12, seed=9, sizes=tuple(range(32, 161, 32)))
+    check_batch([f.numpy() for f in fields], thrs)
+
+
+def test_3d_batch_entry():
+    import torch
+
+    from dmsc_amd.csrc import dmsc_gpu
+
+    stack = np.random.default_rng(2).random((5, 33, 47)).astype(np.float32)
+    res = dmsc_gpu.extract_dmsc(torch.from_numpy(stack).cuda(), 0.1, True, True, True, True, True, True)
+    torch.cuda.synchronize()
+    assert len(res) == 5
+    for b in range(5):
+        bad = diff_outputs([res[b][i].cpu().numpy() for i in range(19)], oracle_extract(stack[b], 0.1, is_dual=True))
+        assert not bad, f"field {b}: {bad}"
+
+
+def test_batch_split_into_groups():
+    rng = np.random.default_rng(7)
+    fields = [rng.random(s).astype(np.float32) for s in [(30, 40), (1, 1), (50, 50), (20, 90), (3, 3), (45, 31)]]
+    thrs = [0.1, 0.0, 0.2, 0.05, 0.0, 0.3]
+    os.environ["DMSC_GROUP_SLOTS"] = "2000"  # several groups, one oversize field alone
+    try:
+        check_batch(fields, thrs)
+    finally:
+        del os.environ["DMSC_GROUP_SLOTS"]
