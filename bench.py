@@ -13,7 +13,8 @@ workload on this host.
 
 Other workloads (--workload): noise8192 (8192^2 uniform noise quantised to 256 levels, thr 0.05),
 mixed (512 fields of sizes 64..1024 with per-field thresholds: one batched call per step, with the
-per-field loop of the plain path timed beside it for the batched/plain ratio).
+per-field loop of the plain path timed beside it for the batched/plain ratio), diff (persistence
+loss over 64 fields of 512x512: batched forward + hand-written backward per step).
 """
 
 import argparse
@@ -27,7 +28,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic bytes per launch of each instrumented kernel, per unit it processes (see DESIGN.md):
+# Algorithmic bytes per launch of each instrumented kernel, per unit it processes:
 #   gradient:      4 B field read per pixel + 16 B gradient write per slot
 #   pointer_jump:  8 B (terminal, depth) read + 8 B written per forest node
 #   labels_*:      8 B terminal read + 4 B label written per cell
@@ -41,7 +42,7 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=5)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--workload", default="gauss4096", choices=["gauss4096", "noise8192", "mixed"])
+    p.add_argument("--workload", default="gauss4096", choices=["gauss4096", "noise8192", "mixed", "diff"])
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     p.add_argument("--profile", action="store_true", help="print per-kernel event timings to stderr")
     p.add_argument("--copy-calibration", action="store_true",
@@ -101,6 +102,9 @@ def main():
     from dmsc_amd.csrc import dmsc_gpu
 
     dev = torch.device("cuda:0")
+    if args.workload == "diff":
+        bench_diff(args, torch, dmsc_amd, dmsc_gpu)
+        return
     fields, runs, desc = workload(args.workload)
     mixed = args.workload == "mixed"
     n_ext = len(runs)  # extractions per step
@@ -232,6 +236,98 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))
+
+
+def bench_diff(args, torch, dmsc_amd, dmsc_gpu):
+    """Persistence loss sum(pers^2, 0 <= pers < 0.1) of 64 fields of 512x512 and its gradient: one
+    step = the batched forward (unsimplified complex, persistence only) + the backward kernel."""
+    from dmsc_amd import synthetic
+
+    B, H, W, maxp = 64, 512, 512, 0.1
+    stack = torch.stack([synthetic.gaussian_mixture(H, W, 24, (8.0, 32.0), 0.05, seed=5000 + b) for b in range(B)])
+    total_steps = max(args.warmup, 1) + args.steps
+    copies = [stack.cuda() for _ in range(total_steps)]
+    torch.cuda.synchronize()
+
+    def step(s):
+        x = copies[s].requires_grad_(True)
+        loss = dmsc_amd.persistence_loss(x, maxp).sum()
+        loss.backward()
+        return x
+
+    for s in range(max(args.warmup, 1)):
+        step(s)
+    torch.cuda.synchronize()
+    dmsc_gpu.profile_enable(0, True)
+    t0 = time.perf_counter()
+    for s in range(max(args.warmup, 1), total_steps):
+        step(s)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    prof = dmsc_gpu.profile_read(0)
+    dmsc_gpu.profile_enable(0, False)
+    px = B * H * W
+    bw_ms, bw_n = prof.get("persistence_backward", (0.0, 0))
+    # backward kernels per field: contributions zeroed + scattered (16 B per slot) and one gather
+    # pass (reads the field, 4 B per pixel, writes the gradient, 4 B per pixel)
+    b_bw = B * (16 * (H + 1) * (W + 1) + 8 * H * W)
+    roof = None
+    if bw_n:
+        avg_s = bw_ms / args.steps / 1e3
+        ach = b_bw / avg_s / 1e9
+        roof = {"bound": "hbm", "kernel": "persistence backward (all fields of a step)", "achieved": round(ach, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "bytes_per_launch": b_bw, "avg_launch_us": round(avg_s * 1e6, 1)}
+    cpu = None if args.no_cpu else cpu_baseline_diff(stack[:2].numpy(), maxp)
+    print(json.dumps({
+        "metric": "Mpix/s persistence-loss forward + backward (1 GPU)", "value": round(px * args.steps / elapsed / 1e6, 2),
+        "unit": "Mpix/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "64 gaussian-mixture fields 512x512 (24 bumps, sigma 8-32 px, noise 0.05, seeds "
+                               "5000-5063); loss = sum of pers^2 over pairs with 0 <= pers < 0.1, both sides",
+                   "backward_ms_per_step": round(bw_ms / args.steps, 3)},
+        "roofline": roof, "cpu_baseline": cpu}))
+
+
+def cpu_baseline_diff(fields, maxp):
+    """The oracle's unsimplified pairs plus a numpy gradient assembly, on a sample of fields."""
+    import numpy as np
+
+    import oracle
+
+    cores = min(os.cpu_count() or 1, 16)
+    t0 = time.perf_counter()
+    for f in fields:
+        H, W = f.shape
+        out = oracle.extract(np.ascontiguousarray(f), -1.0, False, False, False, False, False, False, nthreads=cores)
+        g = np.zeros(H * W)
+        ff = f.reshape(-1)
+
+        def px(cells):  # highest vertex of each cell (primal)
+            t, q = cells & 3, cells >> 2
+            y, x = q // (W + 1), q % (W + 1)
+            best = np.full(cells.shape, -1)
+            for dy, dx, ok in ((0, 0, t == 0), (0, -1, t == 1), (0, 0, t == 1), (-1, 0, t == 2), (0, 0, t == 2),
+                               (-1, -1, t == 3), (-1, 0, t == 3), (0, -1, t == 3), (0, 0, t == 3)):
+                yy, xx = y + dy, x + dx
+                v = ok & (yy >= 0) & (yy < H) & (xx >= 0) & (xx < W)
+                cand = np.where(v, yy * W + xx, 0)
+                better = v & ((best < 0) | (ff[cand] > ff[np.maximum(best, 0)]) |
+                              ((ff[cand] == ff[np.maximum(best, 0)]) & (cand > best)))
+                best = np.where(better, cand, best)
+            return best
+
+        sad, pmin, ppmin, mins, pmax, ppmax, maxs = (np.asarray(out[k]) for k in (3, 7, 9, 2, 6, 8, 1))
+        sp = px(sad)
+        for p, pp, ext, sgn in ((pmin, ppmin, mins, 1.0), (pmax, ppmax, maxs, -1.0)):
+            m = (pp >= 0) & (p >= 0) & (p < maxp)
+            np.add.at(g, sp[m], sgn * 2 * p[m])
+            np.add.at(g, px(ext[pp[m]]), -sgn * 2 * p[m])
+    dt = time.perf_counter() - t0
+    px_total = sum(f.size for f in fields)
+    return {"value": round(px_total / dt / 1e6, 3), "unit": "Mpix/s", "cores": cores, "kind": "port",
+            "sample": f"{len(fields)} of the 512x512 fields: oracle pairs + numpy gradient ({dt:.1f} s)"}
 
 
 def cpu_baseline(fields, runs, mixed):

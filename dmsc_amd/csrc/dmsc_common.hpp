@@ -40,6 +40,38 @@ DMSC_HD uint32_t sortable_key(float f) {
   return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
 
+// ------------------------------------------------------------------ value of a cell
+// Pixel index of the "highest" vertex of cell (t, y, x) in the reference's candidate order
+// (cpu/cell_compare.hxx:33-63); the cell value is the field value there.
+template <bool DUAL>
+__device__ __forceinline__ int top_pixel(int t, int y, int x, int H, int W, const float* __restrict__ f) {
+  int best = -1;
+  float bv = 0.f;
+  auto consider = [&](int vy, int vx) {
+    int id = vy * W + vx;
+    float v = f[id];
+    if (best == -1 || higher<DUAL>(v, id, bv, best)) {
+      best = id;
+      bv = v;
+    }
+  };
+  if (t == 0) {
+    consider(y, x);
+  } else if (t == 1) {
+    if (x - 1 >= 0) consider(y, x - 1);
+    if (x < W) consider(y, x);
+  } else if (t == 2) {
+    if (y - 1 >= 0) consider(y - 1, x);
+    if (y < H) consider(y, x);
+  } else {
+    if (y - 1 >= 0 && x - 1 >= 0) consider(y - 1, x - 1);
+    if (y - 1 >= 0 && x < W) consider(y - 1, x);
+    if (y < H && x - 1 >= 0) consider(y, x - 1);
+    if (y < H && x < W) consider(y, x);
+  }
+  return best;
+}
+
 // ------------------------------------------------------------------ lower-star pairing table
 // Lower star of vertex v = (y, x) (reference cpu/gradient.hxx:55-74):
 //   edges in the reference's L_edges order: 0 = H(y,x) "left", 1 = H(y,x+1) "right",

@@ -205,6 +205,67 @@ std::vector<DMSComplex> extract_dmsc_list(std::vector<torch::Tensor> fields, std
   return run_batch(ctx, dev, ptrs, hs, ws, thr, flags, stream);
 }
 
+// Backward of the persistence loss for a list of fields; pairs[b] = (sad_pts, p_min, ppairs_min,
+// min_pts, p_max, ppairs_max, max_pts) of field b's unsimplified complex.
+std::vector<torch::Tensor> persistence_backward(std::vector<torch::Tensor> fields,
+                                                std::vector<std::vector<torch::Tensor>> pairs,
+                                                torch::Tensor grad_scale, double max_persistence, bool is_dual) {
+  const int64_t B = (int64_t)fields.size();
+  TORCH_CHECK(B > 0 && (int64_t)pairs.size() == B, "one set of pairs per field is required");
+  const auto dev = fields[0].device();
+  TORCH_CHECK(dev.is_cuda(), "persistence_backward requires GPU tensors");
+  TORCH_CHECK(grad_scale.device() == dev && grad_scale.scalar_type() == torch::kFloat32 && grad_scale.numel() == B,
+              "grad_scale must be a float32 tensor with one entry per field on the fields' device");
+  grad_scale = grad_scale.contiguous();
+  c10::hip::HIPGuard guard(dev.index());
+  dmsc_context* ctx = context_for(dev.index());
+  hipStream_t stream = c10::hip::getCurrentHIPStream(dev.index()).stream();
+  std::vector<torch::Tensor> keep, grads;
+  std::vector<const float*> ptrs;
+  std::vector<float*> gptrs;
+  std::vector<int32_t> hs, ws;
+  std::vector<dmsc_pairs> pv(B);
+  for (int64_t b = 0; b < B; ++b) {
+    torch::Tensor f = fields[b];
+    validate(f);
+    TORCH_CHECK(f.dim() == 2 && f.device() == dev, "each field must be a 2D tensor on the same device");
+    f = f.contiguous();
+    keep.push_back(f);
+    ptrs.push_back(f.data_ptr<float>());
+    hs.push_back((int32_t)f.size(0));
+    ws.push_back((int32_t)f.size(1));
+    grads.push_back(torch::empty_like(f));
+    gptrs.push_back(grads.back().data_ptr<float>());
+    auto& p = pairs[b];
+    TORCH_CHECK(p.size() == 7, "pairs must hold (sad_pts, p_min, ppairs_min, min_pts, p_max, ppairs_max, max_pts)");
+    for (int k = 0; k < 7; ++k) {
+      TORCH_CHECK(p[k].device() == dev, "pair tensors must be on the fields' device");
+      p[k] = p[k].contiguous();
+      keep.push_back(p[k]);
+      const bool fl = (k == 1 || k == 4);
+      TORCH_CHECK(p[k].scalar_type() == (fl ? torch::kFloat32 : torch::kInt32), "pair tensor ", k, " has the wrong dtype");
+    }
+    const int64_t ns = p[0].numel();
+    TORCH_CHECK(p[1].numel() == ns && p[2].numel() == ns && p[4].numel() == ns && p[5].numel() == ns,
+                "per-saddle pair tensors must have one entry per saddle");
+    pv[b] = dmsc_pairs{ns,
+                       p[3].numel(),
+                       p[6].numel(),
+                       ns ? p[0].data_ptr<int32_t>() : nullptr,
+                       ns ? p[1].data_ptr<float>() : nullptr,
+                       ns ? p[2].data_ptr<int32_t>() : nullptr,
+                       p[3].numel() ? p[3].data_ptr<int32_t>() : nullptr,
+                       ns ? p[4].data_ptr<float>() : nullptr,
+                       ns ? p[5].data_ptr<int32_t>() : nullptr,
+                       p[6].numel() ? p[6].data_ptr<int32_t>() : nullptr};
+  }
+  const dmsc_status st = dmsc_persistence_backward(ctx, (int32_t)B, ptrs.data(), hs.data(), ws.data(),
+                                                   is_dual ? DMSC_DUAL : 0u, pv.data(), (float)max_persistence,
+                                                   grad_scale.data_ptr<float>(), gptrs.data(), (dmsc_stream_t)stream);
+  TORCH_CHECK(st == DMSC_OK, dmsc_last_error(ctx));
+  return grads;
+}
+
 pybind11::object debug_fetch(int device, const std::string& name, int64_t capacity, bool as_float, bool as_u64) {
   dmsc_context* ctx = context_for(device);
   auto dt = as_u64 ? torch::kInt64 : (as_float ? torch::kFloat32 : torch::kInt32);
@@ -253,6 +314,9 @@ PYBIND11_MODULE(dmsc_gpu, m) {
         pybind11::arg("is_dual") = false, pybind11::arg("trace_max_arcs") = false,
         pybind11::arg("trace_min_arcs") = false, pybind11::arg("trace_max_groups") = false,
         pybind11::arg("trace_min_groups") = false);
+  m.def("persistence_backward", &persistence_backward, "Gradient of sum(pers^2 for 0 <= pers < max) per field",
+        pybind11::arg("fields"), pybind11::arg("pairs"), pybind11::arg("grad_scale"),
+        pybind11::arg("max_persistence"), pybind11::arg("is_dual") = false);
   m.def("debug_fetch", &debug_fetch, pybind11::arg("device"), pybind11::arg("name"), pybind11::arg("capacity"),
         pybind11::arg("as_float") = false, pybind11::arg("as_u64") = false);
 }

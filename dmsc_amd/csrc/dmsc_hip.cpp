@@ -33,7 +33,7 @@ enum : int {
   A_FSUCC, A_VSUCC, A_PJF, A_PJV, A_PLANES, A_PREMIN, A_PRESAD, A_PREMAX,
   A_C32, A_C64,
   A_MINIDS, A_MINVAL, A_SADIDS, A_SADVAL, A_MAXIDS, A_MAXVAL,
-  A_CMAX, A_CMIN, A_LMAX, A_LMIN, A_META, A_BASES32, A_SADFIELD,
+  A_CMAX, A_CMIN, A_LMAX, A_LMIN, A_META, A_BASES32, A_SADFIELD, A_CONTRIB,
   A_KEYS, A_KEYS2, A_VALS2, A_ASC, A_ORDMIN, A_ORDMAX, A_REMOVED, A_POSMIN, A_POSMAX,
   A_SIDE0,  // per persistence side: 13 slots each
   A_SIDE_END = A_SIDE0 + 2 * 13,
@@ -811,6 +811,38 @@ dmsc_status dmsc_extract_batch(dmsc_context* ctx, int32_t B, const float* const*
     std::vector<FieldIn> fields((size_t)B);
     for (int b = 0; b < B; ++b) fields[b] = FieldIn{d_fields[b], H[b], W[b], thr[b], b};
     run_fields(ctx, fields, flags, (hipStream_t)stream, alloc, user, out);
+  });
+}
+
+dmsc_status dmsc_persistence_backward(dmsc_context* ctx, int32_t B, const float* const* d_fields, const int32_t* H,
+                                      const int32_t* W, uint32_t flags, const dmsc_pairs* pairs,
+                                      float max_persistence, const float* d_grad_scale, float* const* d_grad,
+                                      dmsc_stream_t stream) {
+  if (B < 0 || (B > 0 && (!d_fields || !H || !W || !pairs || !d_grad_scale || !d_grad))) return DMSC_ERR_INVALID_ARGUMENT;
+  return guarded(ctx, [&] {
+    int64_t smax = 0;
+    for (int b = 0; b < B; ++b) {
+      validate_field(FieldIn{d_fields[b], H[b], W[b], 0.f, b});
+      if (!d_grad[b]) throw Fail{DMSC_ERR_INVALID_ARGUMENT, "gradient output must be defined"};
+      const dmsc_pairs& p = pairs[b];
+      if (p.n_sad < 0 || p.n_min < 0 || p.n_max < 0 ||
+          (p.n_sad > 0 && (!p.sad_pts || !p.p_min || !p.ppairs_min || !p.p_max || !p.ppairs_max)) ||
+          (p.n_min > 0 && !p.min_pts) || (p.n_max > 0 && !p.max_pts))
+        throw Fail{DMSC_ERR_INVALID_ARGUMENT, "persistence pairs of field " + std::to_string(b) + " are incomplete"};
+      smax = std::max<int64_t>(smax, ((int64_t)H[b] + 1) * ((int64_t)W[b] + 1));
+    }
+    float* contrib = ctx->ar.get<float>(A_CONTRIB, 4 * smax);  // one scratch reused field after field
+    for (int b = 0; b < B; ++b) {
+      const dmsc_pairs& p = pairs[b];
+      const PairView pv{p.n_sad, p.n_min, p.n_max, p.sad_pts, p.p_min, p.ppairs_min, p.min_pts,
+                        p.p_max,   p.ppairs_max, p.max_pts};
+      launch_persistence_backward((flags & DMSC_DUAL) != 0, d_fields[b], H[b], W[b], pv, max_persistence,
+                                  d_grad_scale + b, contrib, d_grad[b], (hipStream_t)stream);
+    }
+    if (ctx->times.on) {
+      DMSC_CHECK(hipStreamSynchronize((hipStream_t)stream));
+      ctx->times.collect();
+    }
   });
 }
 

@@ -6,38 +6,6 @@ namespace dmsc {
 
 __constant__ constexpr PairLut c_lut = make_pair_lut();
 
-// ------------------------------------------------------------------ value of a cell
-// Pixel index of the "highest" vertex of cell (t, y, x) in the reference's candidate order
-// (cpu/cell_compare.hxx:33-63); the cell value is the field value there.
-template <bool DUAL>
-__device__ __forceinline__ int top_pixel(int t, int y, int x, int H, int W, const float* __restrict__ f) {
-  int best = -1;
-  float bv = 0.f;
-  auto consider = [&](int vy, int vx) {
-    int id = vy * W + vx;
-    float v = f[id];
-    if (best == -1 || higher<DUAL>(v, id, bv, best)) {
-      best = id;
-      bv = v;
-    }
-  };
-  if (t == 0) {
-    consider(y, x);
-  } else if (t == 1) {
-    if (x - 1 >= 0) consider(y, x - 1);
-    if (x < W) consider(y, x);
-  } else if (t == 2) {
-    if (y - 1 >= 0) consider(y - 1, x);
-    if (y < H) consider(y, x);
-  } else {
-    if (y - 1 >= 0 && x - 1 >= 0) consider(y - 1, x - 1);
-    if (y - 1 >= 0 && x < W) consider(y - 1, x);
-    if (y < H && x - 1 >= 0) consider(y, x - 1);
-    if (y < H && x < W) consider(y, x);
-  }
-  return best;
-}
-
 // ------------------------------------------------------------------ K1: gradient
 // One 64 x 16 tile of slots per 256-thread block (a wave per slot row). Phase 1 evaluates the lower
 // star of every vertex whose star touches the tile ((16+1) x (64+1) vertices, pixels staged in

@@ -335,6 +335,15 @@ __global__ void k_iota_i32(int32_t* p, int64_t n) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     p[i] = (int32_t)i;
 }
+__global__ void k_fill_f32_zero(float* p, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0.0f;
+}
+void fill_f32_zero(float* p, int64_t n, hipStream_t st) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_fill_f32_zero, dim3(grid_for(n, 256, 16384)), dim3(256), 0, st, p, n);
+  DMSC_LAUNCH_CHECK();
+}
 void fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t st) {
   if (n == 0) return;
   hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(n, 256, 16384)), dim3(256), 0, st, p, v, n);

@@ -91,6 +91,30 @@ dmsc_status dmsc_extract_batch(dmsc_context* ctx, int32_t B, const float* const*
                                const int32_t* W, const float* persistence_thresholds, uint32_t flags,
                                dmsc_stream_t stream, dmsc_alloc_fn alloc, void* alloc_user, dmsc_outputs* out);
 
+/* The persistence pairs of one field's complex, as produced in the output slots (device pointers). */
+typedef struct {
+  int64_t n_sad;              /* number of output saddles (slot 3 length) */
+  int64_t n_min, n_max;       /* slot 2 / slot 1 lengths */
+  const int32_t* sad_pts;     /* slot 3 */
+  const float* p_min;         /* slot 7 */
+  const int32_t* ppairs_min;  /* slot 9 */
+  const int32_t* min_pts;     /* slot 2 */
+  const float* p_max;         /* slot 6 */
+  const int32_t* ppairs_max;  /* slot 8 */
+  const int32_t* max_pts;     /* slot 1 */
+} dmsc_pairs;
+
+/* Gradient of loss_b = sum over the pairs of field b with 0 <= persistence < max_persistence of
+ * persistence^2, scaled by d_grad_scale[b] (device float), written to d_grad[b] (H[b] x W[b]
+ * float32, fully overwritten). The pairs must come from an unsimplified complex (threshold < 0),
+ * in which every extremum dies in at most one pair; flags: DMSC_DUAL as in the extraction.
+ * Deterministic (no float atomics) and asynchronous on `stream`. No reference counterpart (the
+ * reference lists differentiation as future work, readme.md:237-242). */
+dmsc_status dmsc_persistence_backward(dmsc_context* ctx, int32_t B, const float* const* d_fields, const int32_t* H,
+                                      const int32_t* W, uint32_t flags, const dmsc_pairs* pairs,
+                                      float max_persistence, const float* d_grad_scale, float* const* d_grad,
+                                      dmsc_stream_t stream);
+
 /* Per-kernel timing with HIP events recorded on the work stream around each kernel launch (no
  * reference counterpart; the benchmark uses it to measure the dominant kernel's duration live).
  * dmsc_profile_enable(ctx, 1) clears and starts accumulating; entries are (kernel name, total ms,
