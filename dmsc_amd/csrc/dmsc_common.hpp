@@ -72,6 +72,13 @@ __device__ __forceinline__ int top_pixel(int t, int y, int x, int H, int W, cons
   return best;
 }
 
+// An extremum's (value, cell id) in one 64-bit word (value bits low, id high): one load per compare.
+DMSC_HD uint64_t pack_key(float v, int32_t id) {
+  return (uint64_t)__builtin_bit_cast(uint32_t, v) | ((uint64_t)(uint32_t)id << 32);
+}
+DMSC_HD float key_value(uint64_t k) { return __builtin_bit_cast(float, (uint32_t)k); }
+DMSC_HD int32_t key_id(uint64_t k) { return (int32_t)(k >> 32); }
+
 // ------------------------------------------------------------------ lower-star pairing table
 // Lower star of vertex v = (y, x) (reference cpu/gradient.hxx:55-74):
 //   edges in the reference's L_edges order: 0 = H(y,x) "left", 1 = H(y,x+1) "right",

@@ -39,14 +39,14 @@ enum : int {
   A_SIDE_END = A_SIDE0 + 2 * 13,
   A_ASM0 = A_SIDE_END,  // assembly: surv, surv_pre, rows x2, rows_pre x2, sad_to_out, to_out x2
   A_ASM_END = A_ASM0 + 12,
-  A_ARC0 = A_ASM_END,  // per arc side: 24 slots each
-  A_ARC_END = A_ARC0 + 2 * 24,
+  A_ARC0 = A_ASM_END,  // per arc side: 25 slots each
+  A_ARC_END = A_ARC0 + 2 * 25,
 };
 
 enum SideSlot { P_PARG, P_PART, P_ALIVE, P_MINKEY, P_PAIR, P_PERS, P_CANCEL, P_RR, P_WL0, P_WL1, P_CNT, P_ROOT, P_TOOUT };
 enum ArcSlot {
   G_BLEN, G_BOFF, G_ARENA, G_FLAG, G_PRE, G_CS0, G_CD0, G_KEYS, G_SLOT, G_KEYS2, G_SLOT2, G_CS, G_CD, G_DEADC,
-  G_GPAR, G_LLEN, G_LC, G_LKD, G_DONE, G_WATCH, G_RR, G_WL0, G_WL1, G_CNT
+  G_GPAR, G_LLEN, G_LC, G_LKD, G_DONE, G_WATCH, G_RR, G_WL0, G_WL1, G_CNT, G_JUMP
 };
 enum ArcSlot2 { H_PJQ = 0, H_FLEN, H_PERPOS, H_POSPRE, H_SEGA, H_SEGB, H_RAW, H_NSEG, H_CUMR, H_CUMC, H_ARCB };
 
@@ -338,13 +338,13 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
   const int32_t* sad_b = row32(1);
 
   int32_t* min_ids = ar.get<int32_t>(A_MINIDS, nmin);
-  float* min_val = ar.get<float>(A_MINVAL, nmin);
+  uint64_t* min_key = ar.get<uint64_t>(A_MINVAL, nmin);
   int32_t* sad_ids = ar.get<int32_t>(A_SADIDS, nsad);
   float* sad_val = ar.get<float>(A_SADVAL, nsad);
   int32_t* sad_field = B > 1 ? ar.get<int32_t>(A_SADFIELD, nsad) : nullptr;
   int32_t* max_ids = ar.get<int32_t>(A_MAXIDS, nmax);
-  float* max_val = ar.get<float>(A_MAXVAL, nmax);
-  launch_compact_lists(dual, rv, min_ids, min_val, sad_ids, sad_val, sad_field, max_ids, max_val, st);
+  uint64_t* max_key = ar.get<uint64_t>(A_MAXVAL, nmax);
+  launch_compact_lists(dual, rv, min_ids, min_key, sad_ids, sad_val, sad_field, max_ids, max_key, st);
 
   // ---- V-path forests: terminal extremum and depth of every face / vertex
   uint64_t* pjf = ar.get<uint64_t>(A_PJF, S);
@@ -403,8 +403,7 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     p.pos = s ? pos_max : pos_min;
     p.c = s ? cmax : cmin;
     p.sval = sad_val;
-    p.ev = s ? max_val : min_val;
-    p.eid = s ? max_ids : min_ids;
+    p.ekey = s ? max_key : min_key;
     p.par_g = sb[s].par_g;
     p.par_t = sb[s].par_t;
     p.alive = sb[s].alive;
@@ -472,7 +471,7 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     if (!traced[s]) continue;
     ArcWork& w = aw[s];
     w.on = true;
-    const int b = A_ARC0 + 24 * s;
+    const int b = A_ARC0 + 25 * s;
     int32_t* blen = ar.get<int32_t>(b + G_BLEN, 2 * nsad);
     int64_t* boff = ar.get<int64_t>(b + G_BOFF, 2 * nsad);
     launch_arc_lengths(2 * nsad, s ? cmax : cmin, s ? lmax : lmin, blen, st);
@@ -507,7 +506,7 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
   for (int s = 0; s < 2; ++s) {
     if (!aw[s].on) continue;
     ArcWork& w = aw[s];
-    const int b = A_ARC0 + 24 * s;
+    const int b = A_ARC0 + 25 * s;
     const int64_t next = s ? nmax : nmin;
     w.arena_total = h64[2 * s];
     w.ncancel = h32[8 + s];
@@ -545,6 +544,8 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     g.cnt = ar.get<int32_t>(b + G_CNT, 2);
     g.err = d_err;
     iota_i32(g.gparent, next, st);
+    g.gjump = ar.get<uint64_t>(b + G_JUMP, next);
+    pointer_jump_init(g.gparent, g.gjump, next, st);  // (self, 0) for every extremum
     fill_i32(g.linklen, 0, next, st);
     fill_i32(g.done, INT_MAX, nc, st);
     fill_u64(g.watch, ~0ull, next, st);

@@ -132,19 +132,6 @@ struct KScope {
   }
 };
 
-// Wave-aggregated append: one atomicAdd per wave for all lanes with pred set; returns the lane's
-// slot (or -1). Must be reached by every lane of the wave (wave-uniform control flow).
-__device__ __forceinline__ int wave_append(int32_t* counter, bool pred) {
-  const uint64_t m = __ballot(pred);
-  if (m == 0) return -1;
-  const int lane = threadIdx.x & 63;
-  const int leader = __ffsll((unsigned long long)m) - 1;
-  int base = 0;
-  if (lane == leader) base = atomicAdd(counter, __popcll(m));
-  base = __shfl(base, leader, 64);
-  return pred ? base + __popcll(m & ((1ull << lane) - 1ull)) : -1;
-}
-
 // Inclusive prefix sum over the 64 lanes of a wave (every lane must take part).
 __device__ __forceinline__ int32_t wave_inclusive_sum(int32_t v) {
   const int lane = threadIdx.x & 63;
@@ -156,19 +143,49 @@ __device__ __forceinline__ int32_t wave_inclusive_sum(int32_t v) {
   return v;
 }
 
-// Wave-aggregated reservation of n >= 0 consecutive slots per lane: one atomicAdd per wave;
-// returns the lane's first slot. Must be reached by every lane of the wave.
-__device__ __forceinline__ int32_t wave_reserve(int32_t* counter, int32_t n) {
-  const int32_t incl = wave_inclusive_sum(n);
-  const int32_t total = __shfl(incl, 63, 64);
-  int32_t base = 0;
-  if ((threadIdx.x & 63) == 0 && total > 0) base = atomicAdd(counter, total);
-  base = __shfl(base, 0, 64);
-  return base + incl - n;
+// Block-aggregated reservation for 256-thread blocks: every thread asks for n[l] >= 0 consecutive
+// slots of list l; one atomicAdd per list and block (same-address atomics from every wave would
+// serialise on one L2 channel). Returns the thread's first slot per list; slots follow thread
+// order within the block. Must be reached by every thread of the block.
+template <int NL>
+__device__ __forceinline__ void block_reserve(int32_t* const (&counter)[NL], const int32_t (&n)[NL],
+                                              int32_t (&first)[NL]) {
+  __shared__ int32_t s_tot[NL][4];
+  __shared__ int32_t s_base[NL];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int32_t incl[NL];
+#pragma unroll
+  for (int l = 0; l < NL; ++l) {
+    incl[l] = wave_inclusive_sum(n[l]);
+    if (lane == 63) s_tot[l][wid] = incl[l];
+  }
+  __syncthreads();
+  if (threadIdx.x < NL) {
+    int32_t* cp = counter[0];
+#pragma unroll
+    for (int k = 1; k < NL; ++k)
+      if ((int)threadIdx.x == k) cp = counter[k];
+    const int l = threadIdx.x;
+    const int32_t t = s_tot[l][0] + s_tot[l][1] + s_tot[l][2] + s_tot[l][3];
+    s_base[l] = t > 0 ? atomicAdd(cp, t) : 0;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int l = 0; l < NL; ++l) {
+    int32_t b = s_base[l];
+    for (int w = 0; w < wid; ++w) b += s_tot[l][w];
+    first[l] = b + incl[l] - n[l];
+  }
+  __syncthreads();
 }
 
-// Wave-uniform grid-stride loop: every lane of a wave runs the same iterations; `active` marks
-// lanes with an element. Use with wave_append.
+// Block-uniform grid-stride loop over chunks of `per_block` items (every thread of a block runs the
+// same iterations; use with block_reserve).
+#define DMSC_BLOCK_LOOP(b, n, per_block)                                                       \
+  for (int64_t b = (int64_t)blockIdx.x * (per_block); b < (n); b += (int64_t)gridDim.x * (per_block))
+
+// Wave-uniform grid-stride loop: every lane of a wave runs the same iterations (for wave-wide
+// shuffles over a group of 64 consecutive items).
 #define DMSC_WAVE_LOOP(j, n)                                                                        \
   for (int64_t j##_base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); j##_base < (n);   \
        j##_base += (int64_t)gridDim.x * blockDim.x)
@@ -261,8 +278,9 @@ void bases_at(const T* pre, const S* starts, int64_t n, const T* total, int B, T
 // grid stages (over all fields of a batch)
 void launch_gradient(bool dual, const Batch& bt, const TileRef* tiles, int64_t ntiles, int32_t* pw, int32_t* fsucc,
                      int32_t* vsucc, uint32_t* planes, int64_t PW, hipStream_t st);
-void launch_compact_lists(bool dual, const RankView& rv, int32_t* min_ids, float* min_val, int32_t* sad_ids,
-                          float* sad_val, int32_t* sad_field, int32_t* max_ids, float* max_val, hipStream_t st);
+// min_key / max_key: pack_key(value, cell id) per extremum
+void launch_compact_lists(bool dual, const RankView& rv, int32_t* min_ids, uint64_t* min_key, int32_t* sad_ids,
+                          float* sad_val, int32_t* sad_field, int32_t* max_ids, uint64_t* max_key, hipStream_t st);
 void launch_saddle_records(const RankView& rv, const int32_t* sad_ids, const int32_t* sad_field, int64_t nsad,
                            const uint64_t* pj_face, const uint64_t* pj_vert, int32_t* cmax, int32_t* cmin,
                            int32_t* lmax, int32_t* lmin, hipStream_t st);

@@ -132,6 +132,11 @@ void launch_cancel_finish(int64_t nc, const int32_t* slot, const int32_t* cd, in
 }
 
 // ------------------------------------------------------------------ geometry rounds
+// Root of x in the geometry forest and the summed link length on the way, by path halving over
+// gjump: (ancestor J, length to J) pairs. The forest only grows at roots (a link is only ever
+// added from a root), so a shortcut stays valid once written; finds run in their own kernel, so
+// concurrent shortcuts are all consistent 64-bit pairs.
+constexpr uint32_t kVoid = 0xFFFFFFFFu;
 __device__ __forceinline__ void geo_root(const GeoState& g, int x, int& root, int32_t& len) {
   int32_t acc = 0;
   for (int it = 0; x >= 0; ++it) {
@@ -139,10 +144,27 @@ __device__ __forceinline__ void geo_root(const GeoState& g, int x, int& root, in
       report(g.err, ERR_GEO_CAP);
       break;
     }
-    const int p = g.gparent[x];
-    if (p == x) break;
-    acc = sat_add(acc, g.linklen[x]);
-    x = p;
+    const uint64_t cur = __hip_atomic_load(&g.gjump[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t J = (uint32_t)cur;
+    const int32_t L = (int32_t)(cur >> 32);
+    if (J == (uint32_t)x) break;
+    if (J == kVoid) {
+      acc = sat_add(acc, L);
+      x = -1;
+      break;
+    }
+    const uint64_t nx = __hip_atomic_load(&g.gjump[J], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t JJ = (uint32_t)nx;
+    if (JJ == J) {  // J is the root
+      acc = sat_add(acc, L);
+      x = (int)J;
+      break;
+    }
+    const int32_t L2 = sat_add(L, (int32_t)(nx >> 32));
+    __hip_atomic_store(&g.gjump[x], (uint64_t)JJ | ((uint64_t)(uint32_t)L2 << 32), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+    acc = sat_add(acc, L2);
+    x = JJ == kVoid ? -1 : (int)JJ;
   }
   root = x;
   len = acc;
@@ -195,29 +217,46 @@ __device__ __forceinline__ bool geo_apply_one(const ArcSide& A, const GeoState& 
     const int s = g.cs[c], ko = 1 - kd;
     const int Ro = kd == 0 ? R1 : R0;
     const int32_t Ld = kd == 0 ? L0 : L1, Lo = kd == 0 ? L1 : L0;
+    const int32_t ll = sat_add(sat_add(Ld, A.blen[2 * s + kd]), sat_add(A.blen[2 * s + ko], Lo));
     g.gparent[d] = Ro;
-    g.linklen[d] = sat_add(sat_add(Ld, A.blen[2 * s + kd]), sat_add(A.blen[2 * s + ko], Lo));
+    g.linklen[d] = ll;
+    g.gjump[d] = (uint64_t)(Ro < 0 ? kVoid : (uint32_t)Ro) | ((uint64_t)(uint32_t)ll << 32);
     g.linkc[d] = c;
     g.linkkd[d] = kd;
   }
   return false;
 }
 
-__global__ void k_geo_apply(ArcSide A, GeoState g, int round) {
+constexpr int kGeoItems = 4, kGeoChunk = 256 * kGeoItems;
+
+__global__ __launch_bounds__(256) void k_geo_apply(ArcSide A, GeoState g, int round) {
   const int cur = round & 1;
   const int n = g.cnt[cur];
   const uint64_t tag = (uint64_t)(0xFFFFFFFFu - (uint32_t)round) << 32;
-  const int lane = threadIdx.x & 63;
-  DMSC_WAVE_LOOP(j, (int64_t)n) {
-    const int64_t jj = j_base + lane;
-    bool wait = false;
-    int c = 0;
-    if (jj < n) {
-      c = g.wl[cur][jj];
-      wait = geo_apply_one(A, g, c, round, tag);
+  DMSC_BLOCK_LOOP(start, (int64_t)n, kGeoChunk) {
+    const int64_t j0 = start + (int64_t)threadIdx.x * kGeoItems;
+    int c[kGeoItems];
+    int32_t nwait = 0;
+    uint32_t waitmask = 0;
+#pragma unroll
+    for (int u = 0; u < kGeoItems; ++u) {
+      c[u] = 0;
+      if (j0 + u < n) {
+        c[u] = g.wl[cur][j0 + u];
+        if (geo_apply_one(A, g, c[u], round, tag)) {
+          waitmask |= 1u << u;
+          ++nwait;
+        }
+      }
     }
-    const int pos = wave_append(&g.cnt[cur ^ 1], wait);
-    if (wait) g.wl[cur ^ 1][pos] = c;
+    int32_t* const cnt[1] = {&g.cnt[cur ^ 1]};
+    const int32_t req[1] = {nwait};
+    int32_t first[1];
+    block_reserve<1>(cnt, req, first);
+    int32_t o = first[0];
+#pragma unroll
+    for (int u = 0; u < kGeoItems; ++u)
+      if (waitmask >> u & 1) g.wl[cur ^ 1][o++] = c[u];
   }
 }
 
@@ -226,7 +265,7 @@ void launch_geo_round(const ArcSide& a, const GeoState& g, int round, int64_t ma
   const int grid = grid_for(max_items, 256, 2048);
   hipLaunchKernelGGL(k_geo_find, dim3(grid), dim3(256), 0, st, a, g, round);
   DMSC_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_geo_apply, dim3(grid), dim3(256), 0, st, a, g, round);
+  hipLaunchKernelGGL(k_geo_apply, dim3(grid_for(max_items, kGeoChunk, 2048)), dim3(256), 0, st, a, g, round);
   DMSC_LAUNCH_CHECK();
 }
 
@@ -307,16 +346,15 @@ __device__ __forceinline__ int32_t cum_count(const uint64_t* __restrict__ cum, i
 
 // Seeds: per surviving saddle arc, its raw arc then CHAIN(target -> final root); also the field-local
 // CSR offsets.
-__global__ void k_seed_segments(int64_t ns, const int32_t* __restrict__ ord, const uint8_t* __restrict__ removed,
+__global__ __launch_bounds__(256) void k_seed_segments(int64_t ns, const int32_t* __restrict__ ord, const uint8_t* __restrict__ removed,
                                 ArcSide A, const uint64_t* __restrict__ pjq, int64_t next,
                                 const int64_t* __restrict__ flen, const int64_t* __restrict__ pos_pre,
                                 const int32_t* __restrict__ sad_to_out, Seg* __restrict__ chains,
                                 int32_t* __restrict__ nchain, Seg* __restrict__ raw, int32_t* __restrict__ nraw,
                                 int64_t cap, int32_t* __restrict__ g_err, int32_t* __restrict__ out_off,
                                 int32_t* __restrict__ out_arc_off, const int64_t* __restrict__ arc_base) {
-  const int lane = threadIdx.x & 63;
-  DMSC_WAVE_LOOP(k, ns) {
-    const int64_t kk0 = k_base + lane;
+  DMSC_BLOCK_LOOP(kb, ns, 256) {
+    const int64_t kk0 = kb + threadIdx.x;
     bool has_raw[2] = {false, false}, has_chain[2] = {false, false};
     Seg rs[2], cs[2];
     if (kk0 < ns) {
@@ -328,6 +366,7 @@ __global__ void k_seed_segments(int64_t ns, const int32_t* __restrict__ ord, con
         const int64_t local = base - arc_base[fi];  // field-local output offset
         out_off[o + fi] = (int32_t)local;
         out_arc_off[o] = (int32_t)(local + flen[2 * kk0]);
+#pragma unroll
         for (int kk = 0; kk < 2; ++kk) {
           const int a = 2 * i + kk, T = A.tinit[a];
           if (T < 0) continue;
@@ -342,18 +381,18 @@ __global__ void k_seed_segments(int64_t ns, const int32_t* __restrict__ ord, con
         }
       }
     }
+    int32_t* const cnt[2] = {nraw, nchain};
+    const int32_t req[2] = {(has_raw[0] ? 1 : 0) + (has_raw[1] ? 1 : 0), (has_chain[0] ? 1 : 0) + (has_chain[1] ? 1 : 0)};
+    int32_t first[2];
+    block_reserve<2>(cnt, req, first);
+    if (first[0] + req[0] > cap || first[1] + req[1] > cap) {
+      if (req[0] + req[1] > 0) report(g_err, ERR_SEG_CAP);
+      continue;
+    }
 #pragma unroll
     for (int kk = 0; kk < 2; ++kk) {
-      const int pr = wave_append(nraw, has_raw[kk]);
-      if (has_raw[kk]) {
-        if (pr < cap) raw[pr] = rs[kk];
-        else report(g_err, ERR_SEG_CAP);
-      }
-      const int pc = wave_append(nchain, has_chain[kk]);
-      if (has_chain[kk]) {
-        if (pc < cap) chains[pc] = cs[kk];
-        else report(g_err, ERR_SEG_CAP);
-      }
+      if (has_raw[kk]) raw[first[0]++] = rs[kk];
+      if (has_chain[kk]) chains[first[1]++] = cs[kk];
     }
   }
 }
@@ -420,14 +459,13 @@ void launch_link_counts(const ArcSide& a, const GeoState& g, const uint64_t* pjq
 
 // One expansion level. Each chain (a -> b) reserves exactly its links' pieces (path-sum
 // differences of cum_r / cum_c) with one atomic per wave, then walks its links writing them.
-__global__ void k_expand(ArcSide A, GeoState g, const uint64_t* __restrict__ pjq, const uint64_t* __restrict__ cum_r,
+__global__ __launch_bounds__(256) void k_expand(ArcSide A, GeoState g, const uint64_t* __restrict__ pjq, const uint64_t* __restrict__ cum_r,
                          const uint64_t* __restrict__ cum_c, const Seg* __restrict__ in, const int32_t* __restrict__ nin,
                          Seg* __restrict__ out, int32_t* __restrict__ nout, Seg* __restrict__ raw,
                          int32_t* __restrict__ nraw, int64_t cap) {
   const int n = *nin;
-  const int lane = threadIdx.x & 63;
-  DMSC_WAVE_LOOP(j, (int64_t)n) {
-    const int64_t jj = j_base + lane;
+  DMSC_BLOCK_LOOP(jb, (int64_t)n, 256) {
+    const int64_t jj = jb + threadIdx.x;
     Seg sg{0, 0, 0};
     int32_t nr = 0, nc = 0;
     if (jj < n) {
@@ -435,8 +473,11 @@ __global__ void k_expand(ArcSide A, GeoState g, const uint64_t* __restrict__ pjq
       nr = cum_count(cum_r, sg.a) - cum_count(cum_r, sg.b);
       nc = cum_count(cum_c, sg.a) - cum_count(cum_c, sg.b);
     }
-    int32_t pr = wave_reserve(nraw, nr);
-    int32_t pc = wave_reserve(nout, nc);
+    int32_t* const cnt[2] = {nraw, nout};
+    const int32_t req[2] = {nr, nc};
+    int32_t first[2];
+    block_reserve<2>(cnt, req, first);
+    int32_t pr = first[0], pc = first[1];
     if (jj >= n) continue;
     if ((int64_t)pr + nr > cap || (int64_t)pc + nc > cap) {
       report(g.err, ERR_SEG_CAP);

@@ -184,10 +184,10 @@ __device__ __forceinline__ int rank_min_pixel(const RankView& rv, int f, int p) 
 // thread order, critical_points.hxx:14-65), with the extremum / saddle value. Ids are field-local;
 // lists are concatenated in field order.
 template <bool DUAL>
-__global__ void k_compact_lists(RankView rv, int32_t* __restrict__ min_ids, float* __restrict__ min_val,
+__global__ void k_compact_lists(RankView rv, int32_t* __restrict__ min_ids, uint64_t* __restrict__ min_key,
                                 int32_t* __restrict__ sad_ids, float* __restrict__ sad_val,
                                 int32_t* __restrict__ sad_field, int32_t* __restrict__ max_ids,
-                                float* __restrict__ max_val) {
+                                uint64_t* __restrict__ max_key) {
   for (int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; w < rv.PW; w += (int64_t)gridDim.x * blockDim.x) {
     const int fi = field_of(rv.bt.word_b, rv.bt.B, w);
     const FieldDesc& fd = rv.bt.fd[fi];
@@ -202,7 +202,7 @@ __global__ void k_compact_lists(RankView rv, int32_t* __restrict__ min_ids, floa
       m0 &= m0 - 1;
       int x = xb + b;
       min_ids[o] = 4 * (y * Nx + x);
-      min_val[o] = f[(int64_t)y * W + x];
+      min_key[o] = pack_key(f[(int64_t)y * W + x], 4 * (y * Nx + x));
       ++o;
     }
     o = rv.pre_max[w];
@@ -211,7 +211,7 @@ __global__ void k_compact_lists(RankView rv, int32_t* __restrict__ min_ids, floa
       m3 &= m3 - 1;
       int x = xb + b;
       max_ids[o] = 4 * (y * Nx + x) + 3;
-      max_val[o] = f[top_pixel<DUAL>(3, y, x, H, W, f)];
+      max_key[o] = pack_key(f[top_pixel<DUAL>(3, y, x, H, W, f)], 4 * (y * Nx + x) + 3);
       ++o;
     }
     o = rv.pre_sad[w];
@@ -236,17 +236,17 @@ __global__ void k_compact_lists(RankView rv, int32_t* __restrict__ min_ids, floa
   }
 }
 
-void launch_compact_lists(bool dual, const RankView& rv, int32_t* min_ids, float* min_val, int32_t* sad_ids,
-                          float* sad_val, int32_t* sad_field, int32_t* max_ids, float* max_val, hipStream_t st) {
+void launch_compact_lists(bool dual, const RankView& rv, int32_t* min_ids, uint64_t* min_key, int32_t* sad_ids,
+                          float* sad_val, int32_t* sad_field, int32_t* max_ids, uint64_t* max_key, hipStream_t st) {
   KScope ks_("compact_lists", st);
   if (rv.PW == 0) return;
   int g = grid_for(rv.PW, 256, 8192);
   if (dual)
-    hipLaunchKernelGGL(k_compact_lists<true>, dim3(g), dim3(256), 0, st, rv, min_ids, min_val, sad_ids, sad_val, sad_field,
-                       max_ids, max_val);
+    hipLaunchKernelGGL(k_compact_lists<true>, dim3(g), dim3(256), 0, st, rv, min_ids, min_key, sad_ids, sad_val, sad_field,
+                       max_ids, max_key);
   else
-    hipLaunchKernelGGL(k_compact_lists<false>, dim3(g), dim3(256), 0, st, rv, min_ids, min_val, sad_ids, sad_val,
-                       sad_field, max_ids, max_val);
+    hipLaunchKernelGGL(k_compact_lists<false>, dim3(g), dim3(256), 0, st, rv, min_ids, min_key, sad_ids, sad_val,
+                       sad_field, max_ids, max_key);
   DMSC_LAUNCH_CHECK();
 }
 

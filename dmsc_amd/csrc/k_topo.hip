@@ -3,14 +3,18 @@
 //
 // Sequential reference: persistence.hxx:50-163 visits each side's saddles in order and merges the
 // younger of the two extremum components into the older one. Here every round:
-//   A) each pending saddle finds the two current component roots (the root of a component is its
-//      oldest extremum); equal roots => a cycle (pair -1, persistence -1). Otherwise it posts its
-//      order position into minkey[root] of both roots (64-bit atomicMin, tagged by round);
+//   A) each pending saddle finds its two current component roots (the root of a component is its
+//      oldest extremum); equal roots => a cycle (pair -1, persistence -1). Otherwise it decides
+//      which root dies (the younger) and posts its order position into minkey[root] of BOTH roots
+//      (64-bit atomicMin, tagged by round);
 //   B) a saddle is applied when it holds minkey of the root that dies at it. No earlier pending
 //      saddle touches that component, and the other component can only grow older before this
 //      saddle, so the dying extremum, the persistence and the thresholded merge are exactly those
 //      of the sequential sweep. Dying roots are distinct within a round.
-// The smallest pending saddle is always applicable, so every round makes progress.
+// Posting to both roots is necessary: an earlier pending saddle may see an incomplete component
+// on its other side (a merge it depends on is still pending) and so misjudge which root it kills;
+// it must still block every later saddle that touches either of its current roots.
+// The smallest pending saddle always holds its dying root, so every round makes progress.
 #include "internal.hpp"
 #include "k_topo.hpp"
 
@@ -34,6 +38,8 @@ __device__ __forceinline__ int uf_find(int32_t* __restrict__ p, int x, int32_t* 
   }
 }
 
+// rr[2i] = dying root (-2: cycle); rr[2i+1] = (surviving root + 1) << 1 | (dying root came from c[2i+1])
+template <bool DUAL>
 __device__ __forceinline__ void persist_find_one(const PersistSide& s, int i, uint64_t tag) {
   const int ra = uf_find(s.par_g, s.c[2 * i], s.err);
   const int rb = uf_find(s.par_g, s.c[2 * i + 1], s.err);
@@ -43,13 +49,27 @@ __device__ __forceinline__ void persist_find_one(const PersistSide& s, int i, ui
     s.rr[2 * i] = -2;
     return;
   }
-  s.rr[2 * i] = ra;
-  s.rr[2 * i + 1] = rb;
+  bool b_dies;
+  if (ra == -1) {
+    b_dies = true;
+  } else if (rb == -1) {
+    b_dies = false;
+  } else {
+    // min side: value_greater -> r1 dies (persistence.hxx:73-80); max side: value_less (:130-137)
+    const uint64_t ka = s.ekey[ra], kb = s.ekey[rb];
+    const bool a_dies = s.maxside ? higher<DUAL>(key_value(kb), key_id(kb), key_value(ka), key_id(ka))
+                                  : higher<DUAL>(key_value(ka), key_id(ka), key_value(kb), key_id(kb));
+    b_dies = !a_dies;
+  }
+  const int dying = b_dies ? rb : ra, surv = b_dies ? ra : rb;
+  s.rr[2 * i] = dying;
+  s.rr[2 * i + 1] = ((surv + 1) << 1) | (b_dies ? 1 : 0);
   const uint64_t key = tag | (uint32_t)s.pos[i];
-  if (ra >= 0) atomicMin((unsigned long long*)&s.minkey[ra], (unsigned long long)key);
-  if (rb >= 0) atomicMin((unsigned long long*)&s.minkey[rb], (unsigned long long)key);
+  atomicMin((unsigned long long*)&s.minkey[dying], (unsigned long long)key);
+  if (surv >= 0) atomicMin((unsigned long long*)&s.minkey[surv], (unsigned long long)key);
 }
 
+template <bool DUAL>
 __global__ void k_persist_find(PersistSide s0, PersistSide s1, int round) {
   const int cur = round & 1;
   const int n0 = s0.cnt[cur], n1 = s1.cnt[cur];
@@ -60,8 +80,8 @@ __global__ void k_persist_find(PersistSide s0, PersistSide s1, int round) {
   const uint64_t tag = (uint64_t)(0xFFFFFFFFu - (uint32_t)round) << 32;
   for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n0 + n1; j += (int64_t)gridDim.x * blockDim.x) {
     // worklists hold saddle indices (cell-id order); the sweep position is the merge key
-    if (j >= n0) persist_find_one(s1, s1.wl[cur][j - n0], tag);
-    else persist_find_one(s0, s0.wl[cur][j], tag);
+    if (j >= n0) persist_find_one<DUAL>(s1, s1.wl[cur][j - n0], tag);
+    else persist_find_one<DUAL>(s0, s0.wl[cur][j], tag);
   }
 }
 
@@ -69,81 +89,75 @@ __global__ void k_persist_find(PersistSide s0, PersistSide s1, int round) {
 template <bool DUAL>
 __device__ __forceinline__ bool persist_apply_one(const PersistSide& s, int i, uint64_t tag,
                                                   uint8_t* __restrict__ removed) {
-  const int ra = s.rr[2 * i];
-  if (ra == -2) return false;
-  const int rb = s.rr[2 * i + 1];
-  const int k = s.pos[i];
-  int dying, surv;
-  if (ra == -1) {
-    dying = rb;
-    surv = -1;
-  } else if (rb == -1) {
-    dying = ra;
-    surv = -1;
-  } else {
-    // min side: value_greater -> r1 dies (persistence.hxx:73-80); max side: value_less (:130-137)
-    const bool r1_dies = s.maxside ? higher<DUAL>(s.ev[rb], s.eid[rb], s.ev[ra], s.eid[ra])
-                                   : higher<DUAL>(s.ev[ra], s.eid[ra], s.ev[rb], s.eid[rb]);
-    dying = r1_dies ? ra : rb;
-    surv = r1_dies ? rb : ra;
-  }
-  const uint64_t key = tag | (uint32_t)k;
-  if (s.minkey[dying] != key) return true;
-  {
-    const float sv = s.sval[i], dv = s.ev[dying];
-    float p;
-    if (s.maxside) p = DUAL ? sv - dv : dv - sv;
-    else p = DUAL ? dv - sv : sv - dv;
-    s.pers[i] = p;
-    s.pair[i] = dying;
-    s.par_g[dying] = surv;
-    const float thr = s.thr[s.sfield ? s.sfield[i] : 0];
-    if (p <= thr) {
-      s.alive[dying] = 0;
-      removed[i] = 1;
-      const int a = s.c[2 * i], b = s.c[2 * i + 1];
-      int d, t;
-      if (dying == rb) {
-        d = uf_find(s.par_t, b, s.err);
-        t = uf_find(s.par_t, a, s.err);
-      } else {
-        d = uf_find(s.par_t, a, s.err);
-        t = uf_find(s.par_t, b, s.err);
-      }
-      if (d != t && d != -1) s.par_t[d] = t;
-      if (s.cancel_d) s.cancel_d[i] = d;
-    }
+  const int dying = s.rr[2 * i];
+  if (dying == -2) return false;
+  if (s.minkey[dying] != (tag | (uint32_t)s.pos[i])) return true;
+  const int code = s.rr[2 * i + 1];
+  const int surv = (code >> 1) - 1;
+  const bool b_dies = code & 1;
+  const float sv = s.sval[i], dv = key_value(s.ekey[dying]);
+  float p;
+  if (s.maxside) p = DUAL ? sv - dv : dv - sv;
+  else p = DUAL ? dv - sv : sv - dv;
+  s.pers[i] = p;
+  s.pair[i] = dying;
+  s.par_g[dying] = surv;
+  const float thr = s.thr[s.sfield ? s.sfield[i] : 0];
+  if (p <= thr) {
+    s.alive[dying] = 0;
+    removed[i] = 1;
+    const int a = s.c[2 * i], b = s.c[2 * i + 1];
+    const int d = uf_find(s.par_t, b_dies ? b : a, s.err);
+    const int t = uf_find(s.par_t, b_dies ? a : b, s.err);
+    if (d != t && d != -1) s.par_t[d] = t;
+    if (s.cancel_d) s.cancel_d[i] = d;
   }
   return false;
 }
 
+// One chunk of a side's worklist: kApplyItems consecutive items per thread; waiting items are
+// appended to the next worklist with one block-wide reservation (order preserved within the chunk).
+constexpr int kApplyItems = 4, kApplyChunk = 256 * kApplyItems;
+
 template <bool DUAL>
-__global__ void k_persist_apply(PersistSide s0, PersistSide s1, int round, uint8_t* __restrict__ removed) {
-  const int cur = round & 1;
-  const int n0 = s0.cnt[cur], n1 = s1.cnt[cur];
-  const uint64_t tag = (uint64_t)(0xFFFFFFFFu - (uint32_t)round) << 32;
-  const int lane = threadIdx.x & 63;
-  DMSC_WAVE_LOOP(j, (int64_t)n0 + n1) {
-    const int64_t jj = j_base + lane;
-    const bool side1 = jj >= n0;
-    bool wait = false;
-    int k = 0;
-    if (jj < (int64_t)n0 + n1) {
-      // two inlined copies instead of selecting a struct at run time (which would spill it)
-      if (side1) {
-        k = s1.wl[cur][jj - n0];
-        wait = persist_apply_one<DUAL>(s1, k, tag, removed);
-      } else {
-        k = s0.wl[cur][jj];
-        wait = persist_apply_one<DUAL>(s0, k, tag, removed);
+__device__ __forceinline__ void persist_apply_chunk(const PersistSide& s, int cur, int64_t start, int n, uint64_t tag,
+                                                    uint8_t* __restrict__ removed) {
+  const int64_t j0 = start + (int64_t)threadIdx.x * kApplyItems;
+  int k[kApplyItems];
+  int32_t nwait = 0;
+  uint32_t waitmask = 0;
+#pragma unroll
+  for (int u = 0; u < kApplyItems; ++u) {
+    k[u] = 0;
+    if (j0 + u < n) {
+      k[u] = s.wl[cur][j0 + u];
+      if (persist_apply_one<DUAL>(s, k[u], tag, removed)) {
+        waitmask |= 1u << u;
+        ++nwait;
       }
     }
-    const int p0 = wave_append(&s0.cnt[cur ^ 1], wait && !side1);
-    const int p1 = wave_append(&s1.cnt[cur ^ 1], wait && side1);
-    if (wait) {
-      if (side1) s1.wl[cur ^ 1][p1] = k;
-      else s0.wl[cur ^ 1][p0] = k;
-    }
+  }
+  int32_t* const cnt[1] = {&s.cnt[cur ^ 1]};
+  const int32_t req[1] = {nwait};
+  int32_t first[1];
+  block_reserve<1>(cnt, req, first);
+  int32_t o = first[0];
+#pragma unroll
+  for (int u = 0; u < kApplyItems; ++u)
+    if (waitmask >> u & 1) s.wl[cur ^ 1][o++] = k[u];
+}
+
+// Chunks of side 0 come first, then those of side 1, so every chunk (and block) has one side.
+template <bool DUAL>
+__global__ __launch_bounds__(256) void k_persist_apply(PersistSide s0, PersistSide s1, int round,
+                                                       uint8_t* __restrict__ removed) {
+  const int cur = round & 1;
+  const int n0 = s0.cnt[cur], n1 = s1.cnt[cur];
+  const int64_t nc0 = ceil_div(n0, kApplyChunk), nc = nc0 + ceil_div(n1, kApplyChunk);
+  const uint64_t tag = (uint64_t)(0xFFFFFFFFu - (uint32_t)round) << 32;
+  for (int64_t c = blockIdx.x; c < nc; c += gridDim.x) {
+    if (c >= nc0) persist_apply_chunk<DUAL>(s1, cur, (c - nc0) * kApplyChunk, n1, tag, removed);
+    else persist_apply_chunk<DUAL>(s0, cur, c * kApplyChunk, n0, tag, removed);
   }
 }
 
@@ -151,12 +165,14 @@ void launch_persist_round(bool dual, const PersistSide& s0, const PersistSide& s
                           int64_t max_items, hipStream_t st) {
   KScope ks_("persist_round", st);
   const int g = grid_for(max_items, 256, 2048);
-  hipLaunchKernelGGL(k_persist_find, dim3(g), dim3(256), 0, st, s0, s1, round);
+  if (dual) hipLaunchKernelGGL(k_persist_find<true>, dim3(g), dim3(256), 0, st, s0, s1, round);
+  else hipLaunchKernelGGL(k_persist_find<false>, dim3(g), dim3(256), 0, st, s0, s1, round);
   DMSC_LAUNCH_CHECK();
+  const int ga = grid_for(max_items, kApplyChunk, 2048);
   if (dual)
-    hipLaunchKernelGGL(k_persist_apply<true>, dim3(g), dim3(256), 0, st, s0, s1, round, removed);
+    hipLaunchKernelGGL(k_persist_apply<true>, dim3(ga), dim3(256), 0, st, s0, s1, round, removed);
   else
-    hipLaunchKernelGGL(k_persist_apply<false>, dim3(g), dim3(256), 0, st, s0, s1, round, removed);
+    hipLaunchKernelGGL(k_persist_apply<false>, dim3(ga), dim3(256), 0, st, s0, s1, round, removed);
   DMSC_LAUNCH_CHECK();
 }
 

@@ -10,8 +10,7 @@ struct PersistSide {
   const int32_t* pos;   // saddle index -> position in the side's sweep order (the merge key)
   const int32_t* c;     // per saddle: c[2i], c[2i+1] = extremum indices of its two V-paths (-1 = none)
   const float* sval;    // per saddle value
-  const float* ev;      // per extremum value
-  const int32_t* eid;   // per extremum cell id
+  const uint64_t* ekey; // per extremum pack_key(value, cell id)
   int32_t* par_g;       // global union-find parents (-1 = joined the void)
   int32_t* par_t;       // thresholded union-find parents
   uint8_t* alive;       // per extremum
@@ -19,7 +18,7 @@ struct PersistSide {
   int32_t* pair;        // per saddle: dying extremum index, -1 for a cycle
   float* pers;          // per saddle persistence (-1 for a cycle)
   int32_t* cancel_d;    // per saddle thresholded dead extremum (null when arcs are not traced)
-  int32_t* rr;          // per saddle: the two roots seen this round (-2 = cycle)
+  int32_t* rr;          // per saddle: dying root (-2 = cycle), then (surviving root + 1) << 1 | side bit
   int32_t* wl[2];       // ping-pong worklists of pending saddle indices
   int32_t* cnt;         // [2] worklist sizes (device)
   int32_t* err;         // device error flags
@@ -83,6 +82,7 @@ struct GeoState {
   const int32_t* key;      // cancel -> sorted position
   const int32_t* deadc;    // extremum -> cancel whose dead it is (-1)
   int32_t* gparent;        // geometry union-find (-1 = void)
+  uint64_t* gjump;         // path-compressed view of gparent: (ancestor or kVoid, link length to it)
   int32_t* linklen;        // length of the link path of a linked extremum (saturating int32)
   int32_t* linkc;          // cancel that linked the extremum
   int32_t* linkkd;         // which arc (0/1) of that cancel's saddle led to the dead extremum
