@@ -423,15 +423,17 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
   int round = 0;
   ctx->pending_log.clear();
   if (nsad > 0) {
+    int64_t pend = 2 * nsad;  // pending items of both sides at the last readback (launch sizes)
     while (true) {
       const int batch = round < 4 ? 4 : 8;
-      for (int b = 0; b < batch; ++b, ++round) launch_persist_round(dual, ps[0], ps[1], round, removed, nsad, st);
+      for (int b = 0; b < batch; ++b, ++round) launch_persist_round(dual, ps[0], ps[1], round, removed, pend, st);
       DMSC_CHECK(hipMemcpyAsync(ctx->pinned, sb[0].cnt + (round & 1), 4, hipMemcpyDeviceToHost, st));
       DMSC_CHECK(hipMemcpyAsync((char*)ctx->pinned + 4, sb[1].cnt + (round & 1), 4, hipMemcpyDeviceToHost, st));
       DMSC_CHECK(hipStreamSynchronize(st));
       const int32_t* pc = (const int32_t*)ctx->pinned;
       ctx->pending_log.insert(ctx->pending_log.end(), {round, pc[0], pc[1]});
       if (pc[0] == 0 && pc[1] == 0) break;
+      pend = (int64_t)pc[0] + pc[1];
       if (round > 4 * nsad + 64) throw Fail{DMSC_ERR_INTERNAL, "persistence rounds did not converge"};
     }
   }
@@ -556,13 +558,14 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
   // geometry rounds of both sides, interleaved, one readback per batch of rounds
   {
     bool live[2] = {aw[0].on && aw[0].ncancel > 0, aw[1].on && aw[1].ncancel > 0};
+    int64_t pend[2] = {aw[0].ncancel, aw[1].ncancel};  // pending at the last readback (launch sizes)
     int r = 0;
     ctx->rounds_geo[0] = ctx->rounds_geo[1] = 0;
     while (live[0] || live[1]) {
       const int batch = r < 4 ? 4 : 8;
       for (int k = 0; k < batch; ++k, ++r)
         for (int s = 0; s < 2; ++s)
-          if (live[s]) launch_geo_round(aw[s].A, aw[s].G, r, aw[s].ncancel, st);
+          if (live[s]) launch_geo_round(aw[s].A, aw[s].G, r, pend[s], st);
       for (int s = 0; s < 2; ++s)
         if (live[s]) DMSC_CHECK(hipMemcpyAsync((char*)ctx->pinned + 4 * s, aw[s].G.cnt + (r & 1), 4,
                                                hipMemcpyDeviceToHost, st));
@@ -570,7 +573,8 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
       for (int s = 0; s < 2; ++s)
         if (live[s]) {
           ctx->rounds_geo[s] = r;
-          if (((const int32_t*)ctx->pinned)[s] == 0) live[s] = false;
+          pend[s] = ((const int32_t*)ctx->pinned)[s];
+          if (pend[s] == 0) live[s] = false;
           else if (r > 4 * aw[s].ncancel + 64) throw Fail{DMSC_ERR_INTERNAL, "geometry rounds did not converge"};
         }
     }

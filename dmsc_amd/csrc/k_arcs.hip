@@ -227,19 +227,19 @@ __device__ __forceinline__ bool geo_apply_one(const ArcSide& A, const GeoState& 
   return false;
 }
 
-constexpr int kGeoItems = 4, kGeoChunk = 256 * kGeoItems;
+// Small worklists (the tail of the rounds) are latency-bound: one item per thread there.
+constexpr int kGeoItems = 4, kGeoChunk = 256 * kGeoItems, kGeoSmall = 1 << 16;
 
-__global__ __launch_bounds__(256) void k_geo_apply(ArcSide A, GeoState g, int round) {
-  const int cur = round & 1;
-  const int n = g.cnt[cur];
-  const uint64_t tag = (uint64_t)(0xFFFFFFFFu - (uint32_t)round) << 32;
-  DMSC_BLOCK_LOOP(start, (int64_t)n, kGeoChunk) {
-    const int64_t j0 = start + (int64_t)threadIdx.x * kGeoItems;
-    int c[kGeoItems];
+template <int ITEMS>
+__device__ __forceinline__ void geo_apply_chunks(const ArcSide& A, const GeoState& g, int round, int cur, int n,
+                                                 uint64_t tag) {
+  DMSC_BLOCK_LOOP(start, (int64_t)n, 256 * ITEMS) {
+    const int64_t j0 = start + (int64_t)threadIdx.x * ITEMS;
+    int c[ITEMS];
     int32_t nwait = 0;
     uint32_t waitmask = 0;
 #pragma unroll
-    for (int u = 0; u < kGeoItems; ++u) {
+    for (int u = 0; u < ITEMS; ++u) {
       c[u] = 0;
       if (j0 + u < n) {
         c[u] = g.wl[cur][j0 + u];
@@ -255,9 +255,17 @@ __global__ __launch_bounds__(256) void k_geo_apply(ArcSide A, GeoState g, int ro
     block_reserve<1>(cnt, req, first);
     int32_t o = first[0];
 #pragma unroll
-    for (int u = 0; u < kGeoItems; ++u)
+    for (int u = 0; u < ITEMS; ++u)
       if (waitmask >> u & 1) g.wl[cur ^ 1][o++] = c[u];
   }
+}
+
+__global__ __launch_bounds__(256) void k_geo_apply(ArcSide A, GeoState g, int round) {
+  const int cur = round & 1;
+  const int n = g.cnt[cur];
+  const uint64_t tag = (uint64_t)(0xFFFFFFFFu - (uint32_t)round) << 32;
+  if (n <= kGeoSmall) geo_apply_chunks<1>(A, g, round, cur, n, tag);
+  else geo_apply_chunks<kGeoItems>(A, g, round, cur, n, tag);
 }
 
 void launch_geo_round(const ArcSide& a, const GeoState& g, int round, int64_t max_items, hipStream_t st) {
@@ -265,7 +273,8 @@ void launch_geo_round(const ArcSide& a, const GeoState& g, int round, int64_t ma
   const int grid = grid_for(max_items, 256, 2048);
   hipLaunchKernelGGL(k_geo_find, dim3(grid), dim3(256), 0, st, a, g, round);
   DMSC_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_geo_apply, dim3(grid_for(max_items, kGeoChunk, 2048)), dim3(256), 0, st, a, g, round);
+  const int per = max_items <= kGeoSmall ? 256 : kGeoChunk;
+  hipLaunchKernelGGL(k_geo_apply, dim3(grid_for(max_items, per, 2048)), dim3(256), 0, st, a, g, round);
   DMSC_LAUNCH_CHECK();
 }
 

@@ -117,17 +117,18 @@ __device__ __forceinline__ bool persist_apply_one(const PersistSide& s, int i, u
 
 // One chunk of a side's worklist: kApplyItems consecutive items per thread; waiting items are
 // appended to the next worklist with one block-wide reservation (order preserved within the chunk).
-constexpr int kApplyItems = 4, kApplyChunk = 256 * kApplyItems;
+// Small worklists (the tail of the sweep) are latency-bound: one item per thread there.
+constexpr int kApplyItems = 4, kApplyChunk = 256 * kApplyItems, kSmallList = 1 << 16;
 
-template <bool DUAL>
+template <bool DUAL, int ITEMS>
 __device__ __forceinline__ void persist_apply_chunk(const PersistSide& s, int cur, int64_t start, int n, uint64_t tag,
                                                     uint8_t* __restrict__ removed) {
-  const int64_t j0 = start + (int64_t)threadIdx.x * kApplyItems;
-  int k[kApplyItems];
+  const int64_t j0 = start + (int64_t)threadIdx.x * ITEMS;
+  int k[ITEMS];
   int32_t nwait = 0;
   uint32_t waitmask = 0;
 #pragma unroll
-  for (int u = 0; u < kApplyItems; ++u) {
+  for (int u = 0; u < ITEMS; ++u) {
     k[u] = 0;
     if (j0 + u < n) {
       k[u] = s.wl[cur][j0 + u];
@@ -143,7 +144,7 @@ __device__ __forceinline__ void persist_apply_chunk(const PersistSide& s, int cu
   block_reserve<1>(cnt, req, first);
   int32_t o = first[0];
 #pragma unroll
-  for (int u = 0; u < kApplyItems; ++u)
+  for (int u = 0; u < ITEMS; ++u)
     if (waitmask >> u & 1) s.wl[cur ^ 1][o++] = k[u];
 }
 
@@ -153,11 +154,19 @@ __global__ __launch_bounds__(256) void k_persist_apply(PersistSide s0, PersistSi
                                                        uint8_t* __restrict__ removed) {
   const int cur = round & 1;
   const int n0 = s0.cnt[cur], n1 = s1.cnt[cur];
-  const int64_t nc0 = ceil_div(n0, kApplyChunk), nc = nc0 + ceil_div(n1, kApplyChunk);
   const uint64_t tag = (uint64_t)(0xFFFFFFFFu - (uint32_t)round) << 32;
+  if ((int64_t)n0 + n1 <= kSmallList) {
+    const int64_t nc0 = ceil_div(n0, 256), nc = nc0 + ceil_div(n1, 256);
+    for (int64_t c = blockIdx.x; c < nc; c += gridDim.x) {
+      if (c >= nc0) persist_apply_chunk<DUAL, 1>(s1, cur, (c - nc0) * 256, n1, tag, removed);
+      else persist_apply_chunk<DUAL, 1>(s0, cur, c * 256, n0, tag, removed);
+    }
+    return;
+  }
+  const int64_t nc0 = ceil_div(n0, kApplyChunk), nc = nc0 + ceil_div(n1, kApplyChunk);
   for (int64_t c = blockIdx.x; c < nc; c += gridDim.x) {
-    if (c >= nc0) persist_apply_chunk<DUAL>(s1, cur, (c - nc0) * kApplyChunk, n1, tag, removed);
-    else persist_apply_chunk<DUAL>(s0, cur, c * kApplyChunk, n0, tag, removed);
+    if (c >= nc0) persist_apply_chunk<DUAL, kApplyItems>(s1, cur, (c - nc0) * kApplyChunk, n1, tag, removed);
+    else persist_apply_chunk<DUAL, kApplyItems>(s0, cur, c * kApplyChunk, n0, tag, removed);
   }
 }
 
@@ -168,7 +177,7 @@ void launch_persist_round(bool dual, const PersistSide& s0, const PersistSide& s
   if (dual) hipLaunchKernelGGL(k_persist_find<true>, dim3(g), dim3(256), 0, st, s0, s1, round);
   else hipLaunchKernelGGL(k_persist_find<false>, dim3(g), dim3(256), 0, st, s0, s1, round);
   DMSC_LAUNCH_CHECK();
-  const int ga = grid_for(max_items, kApplyChunk, 2048);
+  const int ga = grid_for(max_items, max_items <= kSmallList ? 256 : kApplyChunk, 2048);
   if (dual)
     hipLaunchKernelGGL(k_persist_apply<true>, dim3(ga), dim3(256), 0, st, s0, s1, round, removed);
   else
