@@ -155,6 +155,17 @@ struct ArcWork {
   std::vector<int64_t> h_arc_base;
 };
 
+constexpr int kCoopRounds = 1 << 12;  // rounds per cooperative launch before the host checks in
+
+// Round loops: one launch per phase with a host readback every few rounds. DMSC_ROUNDS=coop selects
+// one cooperative launch per loop instead; that path is off by default because a large-grid run
+// of it (about 2000 workgroups) stopped making progress and the cause has not been established.
+bool cooperative_rounds() {
+  const char* e = std::getenv("DMSC_ROUNDS");
+  return e && std::strcmp(e, "coop") == 0;
+}
+
+
 // One field of a call, with the caller's index (the allocator's `field` argument).
 struct FieldIn {
   const float* f;
@@ -422,14 +433,28 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
   }
   int round = 0;
   ctx->pending_log.clear();
+  bool coop = cooperative_rounds();
   if (nsad > 0) {
     int64_t pend = 2 * nsad;  // pending items of both sides at the last readback (launch sizes)
     while (true) {
-      const int batch = round < 4 ? 4 : 8;
-      for (int b = 0; b < batch; ++b, ++round) launch_persist_round(dual, ps[0], ps[1], round, removed, pend, st);
-      DMSC_CHECK(hipMemcpyAsync(ctx->pinned, sb[0].cnt + (round & 1), 4, hipMemcpyDeviceToHost, st));
-      DMSC_CHECK(hipMemcpyAsync((char*)ctx->pinned + 4, sb[1].cnt + (round & 1), 4, hipMemcpyDeviceToHost, st));
-      DMSC_CHECK(hipStreamSynchronize(st));
+      if (coop && !launch_persist_rounds(dual, ps[0], ps[1], round, kCoopRounds, removed, c32 + 48, pend, st))
+        coop = false;  // no cooperative launch on this device or at this size: one launch per phase
+      if (coop) {
+        DMSC_CHECK(hipMemcpyAsync(ctx->pinned, c32 + 48, 4, hipMemcpyDeviceToHost, st));
+        DMSC_CHECK(hipMemcpyAsync((char*)ctx->pinned + 8, sb[0].cnt, 8, hipMemcpyDeviceToHost, st));
+        DMSC_CHECK(hipMemcpyAsync((char*)ctx->pinned + 16, sb[1].cnt, 8, hipMemcpyDeviceToHost, st));
+        DMSC_CHECK(hipStreamSynchronize(st));
+        const int32_t* h = (const int32_t*)ctx->pinned;
+        round = h[0];
+        const int32_t left[2] = {h[2 + (round & 1)], h[4 + (round & 1)]};
+        std::memcpy(ctx->pinned, left, 8);
+      } else {
+        const int batch = round < 4 ? 4 : 8;
+        for (int b = 0; b < batch; ++b, ++round) launch_persist_round(dual, ps[0], ps[1], round, removed, pend, st);
+        DMSC_CHECK(hipMemcpyAsync(ctx->pinned, sb[0].cnt + (round & 1), 4, hipMemcpyDeviceToHost, st));
+        DMSC_CHECK(hipMemcpyAsync((char*)ctx->pinned + 4, sb[1].cnt + (round & 1), 4, hipMemcpyDeviceToHost, st));
+        DMSC_CHECK(hipStreamSynchronize(st));
+      }
       const int32_t* pc = (const int32_t*)ctx->pinned;
       ctx->pending_log.insert(ctx->pending_log.end(), {round, pc[0], pc[1]});
       if (pc[0] == 0 && pc[1] == 0) break;
@@ -561,15 +586,33 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     int64_t pend[2] = {aw[0].ncancel, aw[1].ncancel};  // pending at the last readback (launch sizes)
     int r = 0;
     ctx->rounds_geo[0] = ctx->rounds_geo[1] = 0;
+    bool gcoop = cooperative_rounds();
     while (live[0] || live[1]) {
-      const int batch = r < 4 ? 4 : 8;
-      for (int k = 0; k < batch; ++k, ++r)
+      const ArcSide as[2] = {aw[0].A, aw[1].A};
+      const GeoState gs[2] = {aw[0].G, aw[1].G};
+      const int mask = (live[0] ? 1 : 0) | (live[1] ? 2 : 0);
+      if (gcoop && !launch_geo_rounds(as, gs, mask, r, kCoopRounds, c32 + 49, std::max(pend[0], pend[1]), st))
+        gcoop = false;
+      if (gcoop) {
+        DMSC_CHECK(hipMemcpyAsync(ctx->pinned, c32 + 49, 4, hipMemcpyDeviceToHost, st));
         for (int s = 0; s < 2; ++s)
-          if (live[s]) launch_geo_round(aw[s].A, aw[s].G, r, pend[s], st);
-      for (int s = 0; s < 2; ++s)
-        if (live[s]) DMSC_CHECK(hipMemcpyAsync((char*)ctx->pinned + 4 * s, aw[s].G.cnt + (r & 1), 4,
-                                               hipMemcpyDeviceToHost, st));
-      DMSC_CHECK(hipStreamSynchronize(st));
+          if (live[s])
+            DMSC_CHECK(hipMemcpyAsync((char*)ctx->pinned + 8 + 8 * s, aw[s].G.cnt, 8, hipMemcpyDeviceToHost, st));
+        DMSC_CHECK(hipStreamSynchronize(st));
+        const int32_t* h = (const int32_t*)ctx->pinned;
+        r = h[0];
+        const int32_t left[2] = {live[0] ? h[2 + (r & 1)] : 0, live[1] ? h[4 + (r & 1)] : 0};
+        std::memcpy(ctx->pinned, left, 8);
+      } else {
+        const int batch = r < 4 ? 4 : 8;
+        for (int k = 0; k < batch; ++k, ++r)
+          for (int s = 0; s < 2; ++s)
+            if (live[s]) launch_geo_round(aw[s].A, aw[s].G, r, pend[s], st);
+        for (int s = 0; s < 2; ++s)
+          if (live[s]) DMSC_CHECK(hipMemcpyAsync((char*)ctx->pinned + 4 * s, aw[s].G.cnt + (r & 1), 4,
+                                                 hipMemcpyDeviceToHost, st));
+        DMSC_CHECK(hipStreamSynchronize(st));
+      }
       for (int s = 0; s < 2; ++s)
         if (live[s]) {
           ctx->rounds_geo[s] = r;

@@ -15,6 +15,8 @@
 // on its other side (a merge it depends on is still pending) and so misjudge which root it kills;
 // it must still block every later saddle that touches either of its current roots.
 // The smallest pending saddle always holds its dying root, so every round makes progress.
+#include <hip/hip_cooperative_groups.h>
+
 #include "internal.hpp"
 #include "k_topo.hpp"
 
@@ -70,7 +72,7 @@ __device__ __forceinline__ void persist_find_one(const PersistSide& s, int i, ui
 }
 
 template <bool DUAL>
-__global__ void k_persist_find(PersistSide s0, PersistSide s1, int round) {
+__device__ __forceinline__ void persist_find_phase(const PersistSide& s0, const PersistSide& s1, int round) {
   const int cur = round & 1;
   const int n0 = s0.cnt[cur], n1 = s1.cnt[cur];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -83,6 +85,10 @@ __global__ void k_persist_find(PersistSide s0, PersistSide s1, int round) {
     if (j >= n0) persist_find_one<DUAL>(s1, s1.wl[cur][j - n0], tag);
     else persist_find_one<DUAL>(s0, s0.wl[cur][j], tag);
   }
+}
+template <bool DUAL>
+__global__ void k_persist_find(PersistSide s0, PersistSide s1, int round) {
+  persist_find_phase<DUAL>(s0, s1, round);
 }
 
 // Decides one pending saddle i; returns true when it must wait for a later round.
@@ -150,8 +156,8 @@ __device__ __forceinline__ void persist_apply_chunk(const PersistSide& s, int cu
 
 // Chunks of side 0 come first, then those of side 1, so every chunk (and block) has one side.
 template <bool DUAL>
-__global__ __launch_bounds__(256) void k_persist_apply(PersistSide s0, PersistSide s1, int round,
-                                                       uint8_t* __restrict__ removed) {
+__device__ __forceinline__ void persist_apply_phase(const PersistSide& s0, const PersistSide& s1, int round,
+                                                    uint8_t* __restrict__ removed) {
   const int cur = round & 1;
   const int n0 = s0.cnt[cur], n1 = s1.cnt[cur];
   const uint64_t tag = (uint64_t)(0xFFFFFFFFu - (uint32_t)round) << 32;
@@ -168,6 +174,61 @@ __global__ __launch_bounds__(256) void k_persist_apply(PersistSide s0, PersistSi
     if (c >= nc0) persist_apply_chunk<DUAL, kApplyItems>(s1, cur, (c - nc0) * kApplyChunk, n1, tag, removed);
     else persist_apply_chunk<DUAL, kApplyItems>(s0, cur, c * kApplyChunk, n0, tag, removed);
   }
+}
+template <bool DUAL>
+__global__ __launch_bounds__(256) void k_persist_apply(PersistSide s0, PersistSide s1, int round,
+                                                       uint8_t* __restrict__ removed) {
+  persist_apply_phase<DUAL>(s0, s1, round, removed);
+}
+
+// All rounds in one cooperative launch: find, grid barrier, apply, grid barrier, until both
+// worklists are empty or max_rounds have run. Every block takes the same path: the phase bounds
+// and the exit test read counters that no block writes between the barrier and the read.
+template <bool DUAL>
+__global__ __launch_bounds__(256) void k_persist_rounds(PersistSide s0, PersistSide s1, int round0, int max_rounds,
+                                                        uint8_t* __restrict__ removed, int32_t* __restrict__ round_out) {
+  cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+  int r = round0;
+  while (r < round0 + max_rounds) {
+    persist_find_phase<DUAL>(s0, s1, r);
+    grid.sync();
+    persist_apply_phase<DUAL>(s0, s1, r, removed);
+    grid.sync();
+    ++r;
+    const int p0 = __hip_atomic_load(&s0.cnt[r & 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int p1 = __hip_atomic_load(&s1.cnt[r & 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (p0 == 0 && p1 == 0) break;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) *round_out = r;
+}
+
+// Blocks a cooperative launch of `kernel` may use on the current device (all co-resident).
+static int coop_blocks(const void* kernel) {
+  int dev = 0, per_cu = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0) != hipSuccess || !prop.cooperativeLaunch)
+    return 0;
+  return per_cu * prop.multiProcessorCount;
+}
+
+bool launch_persist_rounds(bool dual, const PersistSide& s0, const PersistSide& s1, int round0, int max_rounds,
+                           uint8_t* removed, int32_t* round_out, int64_t max_items, hipStream_t st) {
+  KScope ks_("persist_round", st);
+  static const int cap[2] = {coop_blocks(reinterpret_cast<const void*>(&k_persist_rounds<false>)),
+                             coop_blocks(reinterpret_cast<const void*>(&k_persist_rounds<true>))};
+  const int maxb = cap[dual ? 1 : 0];
+  if (maxb <= 0) return false;
+  const int blocks = std::min<int64_t>(maxb, std::max<int64_t>(1, grid_for(max_items, 256, 1 << 20)));
+  PersistSide a = s0, b = s1;
+  void* args[] = {&a, &b, &round0, &max_rounds, &removed, &round_out};
+  const void* fn = dual ? reinterpret_cast<const void*>(&k_persist_rounds<true>)
+                        : reinterpret_cast<const void*>(&k_persist_rounds<false>);
+  if (hipLaunchCooperativeKernel(fn, dim3(blocks), dim3(256), args, 0, st) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return true;
 }
 
 void launch_persist_round(bool dual, const PersistSide& s0, const PersistSide& s1, int round, uint8_t* removed,

@@ -29,6 +29,11 @@ struct PersistSide {
 
 void launch_persist_round(bool dual, const PersistSide& s0, const PersistSide& s1, int round, uint8_t* removed,
                           int64_t max_items, hipStream_t st);
+// Every round up to max_rounds (or until both worklists drain) in one cooperative launch; the
+// round reached goes to *round_out. Returns false, having launched nothing, when the device or
+// the launch size does not allow a cooperative launch.
+bool launch_persist_rounds(bool dual, const PersistSide& s0, const PersistSide& s1, int round0, int max_rounds,
+                           uint8_t* removed, int32_t* round_out, int64_t max_items, hipStream_t st);
 void launch_uf_flatten(int32_t* par, int32_t* root, int64_t n, hipStream_t st);
 void launch_u8_to_i32(const uint8_t* a, int32_t* o, int64_t n, bool invert, hipStream_t st);
 void launch_mask_prefix(const uint8_t* alive, int32_t* pre, int64_t n, hipStream_t st);
@@ -94,6 +99,10 @@ struct GeoState {
   int32_t* err;
 };
 void launch_geo_round(const ArcSide& a, const GeoState& g, int round, int64_t max_items, hipStream_t st);
+// All geometry rounds of the traced sides (bit s of `live`) in one cooperative launch; see
+// launch_persist_rounds.
+bool launch_geo_rounds(const ArcSide* a, const GeoState* g, int live, int round0, int max_rounds, int32_t* round_out,
+                       int64_t max_items, hipStream_t st);
 void launch_cancel_flags(int64_t ns, const int32_t* cancel_d, int32_t* flag, hipStream_t st);
 void launch_cancel_compact(int64_t ns, const int32_t* cancel_d, const float* pers, const int32_t* pre,
                            int32_t* cs, int32_t* cd, uint32_t* keys, hipStream_t st);
