@@ -47,6 +47,10 @@ def parse():
     p.add_argument("--profile", action="store_true", help="print per-kernel event timings to stderr")
     p.add_argument("--copy-calibration", action="store_true",
                    help="only run plain 256 MiB device-to-device copies (reference kernel for PMC byte counters)")
+    p.add_argument("--summarize-profile", nargs=5, metavar=("TRACE_CSV", "FETCH_CSV", "WRITE_CSV", "COPY_FETCH_CSV",
+                                                            "COPY_WRITE_CSV"),
+                   help="print a per-kernel table (duration, calibrated HBM bytes, GB/s vs peak) from rocprofv3 "
+                        "kernel-trace and FETCH_SIZE / WRITE_SIZE counter CSVs of the same command")
     return p.parse_args()
 
 
@@ -83,12 +87,68 @@ def workload(name):
     return fields, list(zip([False] * len(fields), thrs)), desc
 
 
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch from the committed rocprofv3 PMC summary (profiles/traffic.json), if any."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as fh:
+            return json.load(fh).get(workload, {}).get(kernel, {}).get("traffic_bytes_per_launch")
+    except OSError:
+        return None
+
+
 def out_bytes(ms):
     return sum(t.numel() * t.element_size() for t in ms)
 
 
+def summarize_profile(trace_csv, fetch_csv, write_csv, cfetch_csv, cwrite_csv):
+    """Per-kernel table from rocprofv3 output: average duration from the kernel trace, bytes per
+    dispatch from the FETCH_SIZE / WRITE_SIZE counters (KiB), each scaled so that a plain 256 MiB
+    copy reads and writes 256 MiB, and the resulting GB/s against the 8 TB/s HBM peak."""
+    import csv
+    from collections import defaultdict
+
+    def short(name):
+        name = name.replace("void ", "").replace("dmsc::", "")
+        return name.split("(")[0]
+
+    def counters(path):
+        acc = defaultdict(lambda: [0.0, set()])
+        for r in csv.DictReader(open(path)):
+            a = acc[short(r["Kernel_Name"])]
+            a[0] += float(r["Counter_Value"])
+            a[1].add(r["Dispatch_Id"])
+        return {k: (v[0], len(v[1])) for k, v in acc.items()}
+
+    copy_kib = 4 * 64 * 1024 * 1024 / 1024
+    fetch_scale = copy_kib / (counters(cfetch_csv)["__amd_rocclr_copyBuffer"][0] /
+                              counters(cfetch_csv)["__amd_rocclr_copyBuffer"][1])
+    write_scale = copy_kib / (counters(cwrite_csv)["__amd_rocclr_copyBuffer"][0] /
+                              counters(cwrite_csv)["__amd_rocclr_copyBuffer"][1])
+    dur = defaultdict(lambda: [0, 0])
+    for r in csv.DictReader(open(trace_csv)):
+        d = dur[short(r["Kernel_Name"])]
+        d[0] += int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+        d[1] += 1
+    fetch, write = counters(fetch_csv), counters(write_csv)
+    total_ns = sum(v[0] for v in dur.values())
+    print(f"calibration: FETCH_SIZE x{fetch_scale:.3f}, WRITE_SIZE x{write_scale:.3f} (256 MiB copy)")
+    print("| kernel | calls | total ms | % time | avg us | MB / dispatch (read + write) | GB/s | % of 8 TB/s |")
+    print("|---|---:|---:|---:|---:|---:|---:|---:|")
+    for k, (ns, n) in sorted(dur.items(), key=lambda kv: -kv[1][0]):
+        fb = fetch.get(k, (0.0, 1))
+        wb = write.get(k, (0.0, 1))
+        mb = (fb[0] / max(fb[1], 1) * fetch_scale + wb[0] / max(wb[1], 1) * write_scale) * 1024 / 1e6
+        avg_us = ns / n / 1e3
+        gbs = mb * 1e6 / (avg_us * 1e3) if avg_us > 0 else 0.0
+        print(f"| `{k}` | {n} | {ns / 1e6:.3f} | {100 * ns / total_ns:.2f} | {avg_us:.1f} | {mb:.2f} | {gbs:.1f} | "
+              f"{100 * gbs / HBM_PEAK_GBS:.2f} |")
+
+
 def main():
     args = parse()
+    if args.summarize_profile:
+        summarize_profile(*args.summarize_profile)
+        return
     if args.copy_calibration:
         copy_calibration()
         return
@@ -180,7 +240,8 @@ def main():
         ach = b_sweep / sweep_s / 1e9
         roof = {"bound": "hbm", "kernel": "persistence sweep (k_persist_find + k_persist_apply rounds)",
                 "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 5),
-                "traffic": None, "bytes_per_launch": int(b_sweep), "avg_launch_us": round(sweep_s * 1e6, 1),
+                "traffic": measured_traffic(args.workload, "persistence_sweep"), "bytes_per_launch": int(b_sweep),
+                "avg_launch_us": round(sweep_s * 1e6, 1),
                 "launch_unit": "one sweep (all rounds) per call",
                 "share_of_kernel_time": round(pers_ms / total_ms, 3), "dominant_kernel": dom}
     grad_ms, grad_n = prof.get("gradient", (0.0, 0))
@@ -190,12 +251,7 @@ def main():
         per_launch = sum(KERNEL_MODELS["gradient"](int(sh[0]), int(sh[1])) for sh, _ in sizes) * args.steps / grad_n
         avg_s = grad_ms / grad_n / 1e3
         ach = per_launch / avg_s / 1e9
-        measured = None
-        try:
-            with open(os.path.join(ROOT, "profiles", "traffic.json")) as fh:
-                measured = json.load(fh).get(args.workload, {}).get("gradient", {}).get("traffic_bytes_per_launch")
-        except OSError:
-            pass
+        measured = measured_traffic(args.workload, "gradient")
         g = {"achieved": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4), "avg_launch_us": round(avg_s * 1e6, 2),
              "bytes_per_launch": int(per_launch), "traffic": measured, "share_of_kernel_time": round(grad_ms / total_ms, 4)}
         if roof is None:
