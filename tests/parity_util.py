@@ -18,8 +18,9 @@ def gpu_extract(field, thr, return_gradient=True, is_dual=False, tmax=True, tmin
     return [r[i].cpu().numpy() for i in range(19)]
 
 
-def oracle_extract(field, thr, return_gradient=True, is_dual=False, tmax=True, tmin=True, gmax=True, gmin=True):
-    return oracle.extract(field, thr, return_gradient, is_dual, tmax, tmin, gmax, gmin)
+def oracle_extract(field, thr, return_gradient=True, is_dual=False, tmax=True, tmin=True, gmax=True, gmin=True,
+                   nthreads=1):
+    return oracle.extract(field, thr, return_gradient, is_dual, tmax, tmin, gmax, gmin, nthreads=nthreads)
 
 
 def float_equal(a, b):

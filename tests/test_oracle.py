@@ -226,3 +226,16 @@ def test_raw_labels_and_euler_characteristic(dual, seed):
 
 def test_fixture_files_present():
     assert len(glob.glob(os.path.join(GOLDEN, "ms_*.npz"))) == 12
+
+
+def test_thread_count_does_not_change_outputs():
+    # the oracle's parallel stages keep the single-thread output order, so full-size parity checks
+    # may run it on all cores
+    from dmsc_amd import synthetic
+
+    f = synthetic.gaussian_mixture(200, 170, 12, (4.0, 20.0), 0.05, seed=5).numpy()
+    for thr, dual in ((0.05, False), (0.2, True), (-1.0, False)):
+        a = oracle.extract(f, thr, True, dual, True, True, True, True, nthreads=1)
+        b = oracle.extract(f, thr, True, dual, True, True, True, True, nthreads=8)
+        for x, y in zip(a, b):
+            assert np.array_equal(np.asarray(x), np.asarray(y), equal_nan=np.asarray(x).dtype.kind == "f")
