@@ -605,9 +605,7 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
         std::memcpy(ctx->pinned, left, 8);
       } else {
         const int batch = r < 4 ? 4 : 8;
-        for (int k = 0; k < batch; ++k, ++r)
-          for (int s = 0; s < 2; ++s)
-            if (live[s]) launch_geo_round(aw[s].A, aw[s].G, r, pend[s], st);
+        for (int k = 0; k < batch; ++k, ++r) launch_geo_round2(as, gs, mask, r, pend[0] + pend[1], st);
         for (int s = 0; s < 2; ++s)
           if (live[s]) DMSC_CHECK(hipMemcpyAsync((char*)ctx->pinned + 4 * s, aw[s].G.cnt + (r & 1), 4,
                                                  hipMemcpyDeviceToHost, st));

@@ -172,25 +172,43 @@ __device__ __forceinline__ void geo_root(const GeoState& g, int x, int& root, in
   len = acc;
 }
 
+__device__ __forceinline__ void geo_find_one(const ArcSide& A, const GeoState& g, int c, uint64_t tag) {
+  const int s = g.cs[c];
+  int R0, R1;
+  int32_t L0, L1;
+  geo_root(g, A.tinit[2 * s], R0, L0);
+  geo_root(g, A.tinit[2 * s + 1], R1, L1);
+  g.rr[4 * c] = R0;
+  g.rr[4 * c + 1] = R1;
+  g.rr[4 * c + 2] = L0;
+  g.rr[4 * c + 3] = L1;
+  const uint64_t key = tag | (uint32_t)g.key[c];
+  if (R0 >= 0) atomicMin((unsigned long long*)&g.watch[R0], (unsigned long long)key);
+  if (R1 >= 0) atomicMin((unsigned long long*)&g.watch[R1], (unsigned long long)key);
+}
+
 __device__ __forceinline__ void geo_find_phase(const ArcSide& A, const GeoState& g, int round) {
   const int cur = round & 1;
   const int n = g.cnt[cur];
   if (blockIdx.x == 0 && threadIdx.x == 0) g.cnt[cur ^ 1] = 0;
   const uint64_t tag = (uint64_t)(0xFFFFFFFFu - (uint32_t)round) << 32;
-  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
-    const int c = g.wl[cur][j];
-    const int s = g.cs[c];
-    int R0, R1;
-    int32_t L0, L1;
-    geo_root(g, A.tinit[2 * s], R0, L0);
-    geo_root(g, A.tinit[2 * s + 1], R1, L1);
-    g.rr[4 * c] = R0;
-    g.rr[4 * c + 1] = R1;
-    g.rr[4 * c + 2] = L0;
-    g.rr[4 * c + 3] = L1;
-    const uint64_t key = tag | (uint32_t)g.key[c];
-    if (R0 >= 0) atomicMin((unsigned long long*)&g.watch[R0], (unsigned long long)key);
-    if (R1 >= 0) atomicMin((unsigned long long*)&g.watch[R1], (unsigned long long)key);
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x)
+    geo_find_one(A, g, g.wl[cur][j], tag);
+}
+
+// Both traced sides of one round in one launch: side 0's items first, then side 1's.
+__global__ void k_geo_find2(ArcSide A0, GeoState G0, ArcSide A1, GeoState G1, int live, int round) {
+  const int cur = round & 1;
+  const int n0 = (live & 1) ? G0.cnt[cur] : 0, n1 = (live & 2) ? G1.cnt[cur] : 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (live & 1) G0.cnt[cur ^ 1] = 0;
+    if (live & 2) G1.cnt[cur ^ 1] = 0;
+  }
+  const uint64_t tag = (uint64_t)(0xFFFFFFFFu - (uint32_t)round) << 32;
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < (int64_t)n0 + n1;
+       j += (int64_t)gridDim.x * blockDim.x) {
+    if (j >= n0) geo_find_one(A1, G1, G1.wl[cur][j - n0], tag);
+    else geo_find_one(A0, G0, G0.wl[cur][j], tag);
   }
 }
 __global__ void k_geo_find(ArcSide A, GeoState g, int round) { geo_find_phase(A, g, round); }
@@ -271,6 +289,65 @@ __device__ __forceinline__ void geo_apply_phase(const ArcSide& A, const GeoState
   else geo_apply_chunks<kGeoItems>(A, g, round, cur, n, tag);
 }
 __global__ __launch_bounds__(256) void k_geo_apply(ArcSide A, GeoState g, int round) { geo_apply_phase(A, g, round); }
+
+// Both traced sides of one round in one launch; chunks of side 0 come first, so every chunk (and
+// block) has one side.
+template <int ITEMS>
+__device__ __forceinline__ void geo_apply_chunk(const ArcSide& A, const GeoState& g, int round, int cur, int64_t start,
+                                                int n, uint64_t tag) {
+  const int64_t j0 = start + (int64_t)threadIdx.x * ITEMS;
+  int c[ITEMS];
+  int32_t nwait = 0;
+  uint32_t waitmask = 0;
+#pragma unroll
+  for (int u = 0; u < ITEMS; ++u) {
+    c[u] = 0;
+    if (j0 + u < n) {
+      c[u] = g.wl[cur][j0 + u];
+      if (geo_apply_one(A, g, c[u], round, tag)) {
+        waitmask |= 1u << u;
+        ++nwait;
+      }
+    }
+  }
+  int32_t* const cnt[1] = {&g.cnt[cur ^ 1]};
+  const int32_t req[1] = {nwait};
+  int32_t first[1];
+  block_reserve<1>(cnt, req, first);
+  int32_t o = first[0];
+#pragma unroll
+  for (int u = 0; u < ITEMS; ++u)
+    if (waitmask >> u & 1) g.wl[cur ^ 1][o++] = c[u];
+}
+template <int ITEMS>
+__device__ __forceinline__ void geo_apply_both(const ArcSide& A0, const GeoState& G0, const ArcSide& A1,
+                                               const GeoState& G1, int round, int cur, int n0, int n1, uint64_t tag) {
+  constexpr int CH = 256 * ITEMS;
+  const int64_t nc0 = ceil_div(n0, CH), nc = nc0 + ceil_div(n1, CH);
+  for (int64_t ch = blockIdx.x; ch < nc; ch += gridDim.x) {
+    if (ch >= nc0) geo_apply_chunk<ITEMS>(A1, G1, round, cur, (ch - nc0) * CH, n1, tag);
+    else geo_apply_chunk<ITEMS>(A0, G0, round, cur, ch * CH, n0, tag);
+  }
+}
+__global__ __launch_bounds__(256) void k_geo_apply2(ArcSide A0, GeoState G0, ArcSide A1, GeoState G1, int live,
+                                                    int round) {
+  const int cur = round & 1;
+  const int n0 = (live & 1) ? G0.cnt[cur] : 0, n1 = (live & 2) ? G1.cnt[cur] : 0;
+  const uint64_t tag = (uint64_t)(0xFFFFFFFFu - (uint32_t)round) << 32;
+  if ((int64_t)n0 + n1 <= kGeoSmall) geo_apply_both<1>(A0, G0, A1, G1, round, cur, n0, n1, tag);
+  else geo_apply_both<kGeoItems>(A0, G0, A1, G1, round, cur, n0, n1, tag);
+}
+
+void launch_geo_round2(const ArcSide* a, const GeoState* g, int live, int round, int64_t max_items, hipStream_t st) {
+  KScope ks_("geo_round", st);
+  hipLaunchKernelGGL(k_geo_find2, dim3(grid_for(max_items, 256, 2048)), dim3(256), 0, st, a[0], g[0], a[1], g[1], live,
+                     round);
+  DMSC_LAUNCH_CHECK();
+  const int per = max_items <= kGeoSmall ? 256 : kGeoChunk;
+  hipLaunchKernelGGL(k_geo_apply2, dim3(grid_for(max_items, per, 2048)), dim3(256), 0, st, a[0], g[0], a[1], g[1], live,
+                     round);
+  DMSC_LAUNCH_CHECK();
+}
 
 // Both traced sides, every round, in one cooperative launch (same structure and exit rule as the
 // persistence rounds kernel): find, grid barrier, apply, grid barrier, until both drain.

@@ -99,6 +99,8 @@ struct GeoState {
   int32_t* err;
 };
 void launch_geo_round(const ArcSide& a, const GeoState& g, int round, int64_t max_items, hipStream_t st);
+// one round of both traced sides (bit s of `live`) in one find and one apply launch
+void launch_geo_round2(const ArcSide* a, const GeoState* g, int live, int round, int64_t max_items, hipStream_t st);
 // All geometry rounds of the traced sides (bit s of `live`) in one cooperative launch; see
 // launch_persist_rounds.
 bool launch_geo_rounds(const ArcSide* a, const GeoState* g, int live, int round0, int max_rounds, int32_t* round_out,
