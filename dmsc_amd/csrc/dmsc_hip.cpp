@@ -449,7 +449,7 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
         const int32_t left[2] = {h[2 + (round & 1)], h[4 + (round & 1)]};
         std::memcpy(ctx->pinned, left, 8);
       } else {
-        const int batch = round < 4 ? 4 : 8;
+        const int batch = round < 4 ? 4 : (round < 12 ? 8 : 16);  // rounds per host check
         for (int b = 0; b < batch; ++b, ++round) launch_persist_round(dual, ps[0], ps[1], round, removed, pend, st);
         DMSC_CHECK(hipMemcpyAsync(ctx->pinned, sb[0].cnt + (round & 1), 4, hipMemcpyDeviceToHost, st));
         DMSC_CHECK(hipMemcpyAsync((char*)ctx->pinned + 4, sb[1].cnt + (round & 1), 4, hipMemcpyDeviceToHost, st));
@@ -604,7 +604,7 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
         const int32_t left[2] = {live[0] ? h[2 + (r & 1)] : 0, live[1] ? h[4 + (r & 1)] : 0};
         std::memcpy(ctx->pinned, left, 8);
       } else {
-        const int batch = r < 4 ? 4 : 8;
+        const int batch = r < 4 ? 4 : (r < 12 ? 8 : 16);  // rounds per host check
         for (int k = 0; k < batch; ++k, ++r) launch_geo_round2(as, gs, mask, r, pend[0] + pend[1], st);
         for (int s = 0; s < 2; ++s)
           if (live[s]) DMSC_CHECK(hipMemcpyAsync((char*)ctx->pinned + 4 * s, aw[s].G.cnt + (r & 1), 4,
@@ -708,31 +708,31 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     Seg* sa = a2.get<Seg>(hb2 + H_SEGA, cap);
     Seg* sbuf = a2.get<Seg>(hb2 + H_SEGB, cap);
     Seg* raw = a2.get<Seg>(hb2 + H_RAW, cap);
-    int32_t* nseg = a2.get<int32_t>(hb2 + H_NSEG, 4);  // [0] list A, [1] list B, [2] raw
+    // chain lists alternate between two buffers; their counters rotate over three (level L reads
+    // nseg[L%3], appends to nseg[(L+1)%3] and clears nseg[(L+2)%3]); nseg[3] counts raw pieces
+    int32_t* nseg = a2.get<int32_t>(hb2 + H_NSEG, 4);
     fill_i32(nseg, 0, 4, st);
     launch_seed_segments(nsad, ord_min, removed, w.A, w.pjq, next, w.flen, w.pos_pre, sad_to_out, sa, nseg, raw,
-                         nseg + 2, cap, d_err, w.out_off, w.out_arc_off, w.arc_base, row32(5), B, st);
-    readback(ctx, nseg, 3 * sizeof(int32_t), st);
+                         nseg + 3, cap, d_err, w.out_off, w.out_arc_off, w.arc_base, row32(5), B, st);
+    readback(ctx, nseg, 4 * sizeof(int32_t), st);
     int32_t n_in = ((const int32_t*)ctx->pinned)[0];
-    Seg* in = sa;
-    Seg* nxt = sbuf;
-    int32_t* cnt_in = nseg;
-    int32_t* cnt_out = nseg + 1;
-    int guard = 0;
+    Seg* lists[2] = {sa, sbuf};
+    int level = 0;
     while (n_in > 0) {
       if (n_in > cap) throw Fail{DMSC_ERR_INTERNAL, "arc segment list overflow"};
-      fill_i32(cnt_out, 0, 1, st);
-      launch_expand_segments(w.A, w.G, w.pjq, w.cum_r, w.cum_c, in, cnt_in, nxt, cnt_out, raw, nseg + 2, n_in, cap,
-                             st);
-      readback(ctx, nseg, 3 * sizeof(int32_t), st);
-      n_in = ((const int32_t*)ctx->pinned)[cnt_out - nseg];
-      std::swap(in, nxt);
-      std::swap(cnt_in, cnt_out);
-      if (++guard > 1000000) throw Fail{DMSC_ERR_INTERNAL, "arc expansion did not terminate"};
+      // a batch of levels per host check; a level past the last one finds no chains and only clears
+      const int batch = level < 4 ? 2 : 8;
+      for (int k = 0; k < batch; ++k, ++level)
+        launch_expand_segments(w.A, w.G, w.pjq, w.cum_r, w.cum_c, lists[level & 1], nseg + level % 3,
+                               lists[(level + 1) & 1], nseg + (level + 1) % 3, nseg + (level + 2) % 3, raw, nseg + 3,
+                               std::max<int64_t>(n_in, 2048 * 256), cap, st);
+      readback(ctx, nseg, 4 * sizeof(int32_t), st);
+      n_in = ((const int32_t*)ctx->pinned)[level % 3];
+      if (level > 1000000) throw Fail{DMSC_ERR_INTERNAL, "arc expansion did not terminate"};
     }
-    const int32_t nraw = ((const int32_t*)ctx->pinned)[2];
+    const int32_t nraw = ((const int32_t*)ctx->pinned)[3];
     if (nraw > cap) throw Fail{DMSC_ERR_INTERNAL, "arc segment list overflow"};
-    if (nraw > 0) launch_copy_raw(w.A, raw, nseg + 2, nraw, w.out_cells, w.out_total, d_err, st);
+    if (nraw > 0) launch_copy_raw(w.A, raw, nseg + 3, std::min<int64_t>(nraw, cap), w.out_cells, w.out_total, d_err, st);
   }
   readback(ctx, d_err, sizeof(int32_t), st);
   if (ctx->times.on) ctx->times.collect();

@@ -598,9 +598,13 @@ void launch_link_counts(const ArcSide& a, const GeoState& g, const uint64_t* pjq
 // differences of cum_r / cum_c) with one atomic per wave, then walks its links writing them.
 __global__ __launch_bounds__(256) void k_expand(ArcSide A, GeoState g, const uint64_t* __restrict__ pjq, const uint64_t* __restrict__ cum_r,
                          const uint64_t* __restrict__ cum_c, const Seg* __restrict__ in, const int32_t* __restrict__ nin,
-                         Seg* __restrict__ out, int32_t* __restrict__ nout, Seg* __restrict__ raw,
-                         int32_t* __restrict__ nraw, int64_t cap) {
-  const int n = *nin;
+                         Seg* __restrict__ out, int32_t* __restrict__ nout, int32_t* __restrict__ nreset,
+                         Seg* __restrict__ raw, int32_t* __restrict__ nraw, int64_t cap) {
+  // levels are launched back to back without host checks: the input count comes from the device
+  // (clamped to the list capacity; an overflow was already reported) and this level clears the
+  // counter the next level will append to (the one the previous level read)
+  const int n = (int)(*nin < cap ? *nin : cap);
+  if (blockIdx.x == 0 && threadIdx.x == 0) *nreset = 0;
   DMSC_BLOCK_LOOP(jb, (int64_t)n, 256) {
     const int64_t jj = jb + threadIdx.x;
     Seg sg{0, 0, 0};
@@ -663,11 +667,11 @@ __global__ __launch_bounds__(256) void k_expand(ArcSide A, GeoState g, const uin
   }
 }
 void launch_expand_segments(const ArcSide& a, const GeoState& g, const uint64_t* pjq, const uint64_t* cum_r,
-                            const uint64_t* cum_c, const Seg* in, const int32_t* nin, Seg* out, int32_t* nout, Seg* raw,
-                            int32_t* nraw, int64_t max_in, int64_t cap, hipStream_t st) {
+                            const uint64_t* cum_c, const Seg* in, const int32_t* nin, Seg* out, int32_t* nout,
+                            int32_t* nreset, Seg* raw, int32_t* nraw, int64_t max_in, int64_t cap, hipStream_t st) {
   KScope ks_("expand_segments", st);
   hipLaunchKernelGGL(k_expand, dim3(grid_for(max_in, 256, 8192)), dim3(256), 0, st, a, g, pjq, cum_r, cum_c, in, nin,
-                     out, nout, raw, nraw, cap);
+                     out, nout, nreset, raw, nraw, cap);
   DMSC_LAUNCH_CHECK();
 }
 
@@ -675,9 +679,10 @@ void launch_expand_segments(const ArcSide& a, const GeoState& g, const uint64_t*
 // lengths) and copies that run 64 cells at a time, so reads and writes are contiguous within each
 // piece whatever the piece lengths.
 __global__ __launch_bounds__(256) void k_copy_raw(ArcSide A, const Seg* __restrict__ raw,
-                                                  const int32_t* __restrict__ nraw, int32_t* __restrict__ out,
-                                                  int64_t out_total, int32_t* __restrict__ err) {
-  const int n = *nraw;
+                                                  const int32_t* __restrict__ nraw, int64_t max_n,
+                                                  int32_t* __restrict__ out, int64_t out_total,
+                                                  int32_t* __restrict__ err) {
+  const int n = (int)(*nraw < max_n ? *nraw : max_n);
   const int lane = threadIdx.x & 63;
   DMSC_WAVE_LOOP(j, (int64_t)n) {
     const int64_t jj = j_base + lane;
@@ -719,7 +724,7 @@ __global__ __launch_bounds__(256) void k_copy_raw(ArcSide A, const Seg* __restri
 void launch_copy_raw(const ArcSide& a, const Seg* raw, const int32_t* nraw, int64_t max_raw, int32_t* out,
                      int64_t out_total, int32_t* err, hipStream_t st) {
   KScope ks_("copy_raw", st);
-  hipLaunchKernelGGL(k_copy_raw, dim3(grid_for(max_raw, 256, 16384)), dim3(256), 0, st, a, raw, nraw, out, out_total,
+  hipLaunchKernelGGL(k_copy_raw, dim3(grid_for(max_raw, 256, 16384)), dim3(256), 0, st, a, raw, nraw, max_raw, out, out_total,
                      err);
   DMSC_LAUNCH_CHECK();
 }

@@ -135,9 +135,10 @@ void launch_seed_segments(int64_t ns, const int32_t* ord, const uint8_t* removed
                           int32_t* err, int32_t* out_offsets, int32_t* out_arc_offsets, const int64_t* arc_base,
                           const int32_t* sad_out, int B, hipStream_t st);
 // One level of expansion: every chain becomes the raw pieces and sub-chains of its links.
+// n = min(*nin, cap) chains are read; *nreset is cleared (the counter the following level appends to).
 void launch_expand_segments(const ArcSide& a, const GeoState& g, const uint64_t* pjq, const uint64_t* cum_r,
-                            const uint64_t* cum_c, const Seg* in, const int32_t* nin, Seg* out, int32_t* nout, Seg* raw,
-                            int32_t* nraw, int64_t max_in, int64_t cap, hipStream_t st);
+                            const uint64_t* cum_c, const Seg* in, const int32_t* nin, Seg* out, int32_t* nout,
+                            int32_t* nreset, Seg* raw, int32_t* nraw, int64_t max_in, int64_t cap, hipStream_t st);
 void launch_copy_raw(const ArcSide& a, const Seg* raw, const int32_t* nraw, int64_t max_raw, int32_t* out,
                      int64_t out_total, int32_t* err, hipStream_t st);
 
