@@ -552,31 +552,29 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     GeoState& g = w.G;
     g.ncancel = nc;
     int32_t* ckey = ar.get<int32_t>(b + G_CS, nc);
-    int32_t* deadc = ar.get<int32_t>(b + G_DEADC, next);
-    fill_i32(deadc, -1, next, st);
-    launch_cancel_finish(nc, cslot, cd0, ckey, deadc, st);
+    uint64_t* kill = ar.get<uint64_t>(b + G_DEADC, next);
+    fill_u64(kill, ~0ull, next, st);
+    launch_cancel_finish(nc, cslot, cd0, ckey, kill, st);
     g.cs = cs0;
     g.cd = cd0;
     g.key = ckey;
-    g.deadc = deadc;
+    g.kill = kill;
     g.gparent = ar.get<int32_t>(b + G_GPAR, next);
     g.linklen = ar.get<int32_t>(b + G_LLEN, next);
     g.linkc = ar.get<int32_t>(b + G_LC, next);
     g.linkkd = ar.get<int32_t>(b + G_LKD, next);
-    g.done = ar.get<int32_t>(b + G_DONE, nc);
     g.watch = ar.get<uint64_t>(b + G_WATCH, next);
     g.rr = ar.get<int32_t>(b + G_RR, 4 * nc);
-    g.wl[0] = ar.get<int32_t>(b + G_WL0, nc);
-    g.wl[1] = ar.get<int32_t>(b + G_WL1, nc);
+    g.wl[0] = ar.get<int4>(b + G_WL0, nc);
+    g.wl[1] = ar.get<int4>(b + G_WL1, nc);
     g.cnt = ar.get<int32_t>(b + G_CNT, 2);
     g.err = d_err;
     iota_i32(g.gparent, next, st);
     g.gjump = ar.get<uint64_t>(b + G_JUMP, next);
     pointer_jump_init(g.gparent, g.gjump, next, st);  // (self, 0) for every extremum
     fill_i32(g.linklen, 0, next, st);
-    fill_i32(g.done, INT_MAX, nc, st);
     fill_u64(g.watch, ~0ull, next, st);
-    iota_i32(g.wl[0], nc, st);
+    launch_geo_items(nc, cs0, ckey, w.A.tinit, g.wl[0], st);
     fill_i32(g.cnt, (int32_t)nc, 1, st);
     fill_i32(g.cnt + 1, 0, 1, st);
   }

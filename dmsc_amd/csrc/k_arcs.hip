@@ -117,23 +117,40 @@ void launch_cancel_compact(int64_t ns, const int32_t* cancel_d, const float* per
   DMSC_LAUNCH_CHECK();
 }
 // Cancellations stay in saddle-id order (index j, spatially coherent); their sorted position is the
-// merge key: key[j] = position of j in (persistence, saddle id) order; deadc[d] = j.
+// merge key: key[j] = position of j in (persistence, saddle id) order; kill[d] = (pending, key of
+// the cancellation whose dead extremum d is).
 __global__ void k_cancel_finish(int64_t nc, const int32_t* __restrict__ slot, const int32_t* __restrict__ cd,
-                                int32_t* __restrict__ key, int32_t* __restrict__ deadc) {
+                                int32_t* __restrict__ key, uint64_t* __restrict__ kill) {
   for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x) {
     const int j = slot[c];
     key[j] = (int)c;
-    deadc[cd[j]] = j;
+    kill[cd[j]] = kKillPending | (uint32_t)c;
   }
 }
-void launch_cancel_finish(int64_t nc, const int32_t* slot, const int32_t* cd, int32_t* key, int32_t* deadc,
+void launch_cancel_finish(int64_t nc, const int32_t* slot, const int32_t* cd, int32_t* key, uint64_t* kill,
                           hipStream_t st) {
   if (nc == 0) return;
-  hipLaunchKernelGGL(k_cancel_finish, dim3(grid_for(nc, 256, 16384)), dim3(256), 0, st, nc, slot, cd, key, deadc);
+  hipLaunchKernelGGL(k_cancel_finish, dim3(grid_for(nc, 256, 16384)), dim3(256), 0, st, nc, slot, cd, key, kill);
   DMSC_LAUNCH_CHECK();
 }
 
 // ------------------------------------------------------------------ geometry rounds
+// initial worklist: every cancellation (saddle-id order) with its key and its two arc targets
+__global__ void k_geo_items(int64_t nc, const int32_t* __restrict__ cs, const int32_t* __restrict__ key,
+                            const int32_t* __restrict__ tinit, int4* __restrict__ wl) {
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x) {
+    const int s = cs[c];
+    const int2 t = *reinterpret_cast<const int2*>(tinit + 2 * s);
+    wl[c] = make_int4((int)c, key[c], t.x, t.y);
+  }
+}
+void launch_geo_items(int64_t nc, const int32_t* cs, const int32_t* key, const int32_t* tinit, int4* wl,
+                      hipStream_t st) {
+  if (nc == 0) return;
+  hipLaunchKernelGGL(k_geo_items, dim3(grid_for(nc, 256, 16384)), dim3(256), 0, st, nc, cs, key, tinit, wl);
+  DMSC_LAUNCH_CHECK();
+}
+
 // Root of x in the geometry forest and the summed link length on the way, by path halving over
 // gjump: (ancestor J, length to J) pairs. The forest only grows at roots (a link is only ever
 // added from a root), so a shortcut stays valid once written; finds run in their own kernel, so
@@ -172,17 +189,16 @@ __device__ __forceinline__ void geo_root(const GeoState& g, int x, int& root, in
   len = acc;
 }
 
-__device__ __forceinline__ void geo_find_one(const ArcSide& A, const GeoState& g, int c, uint64_t tag) {
-  const int s = g.cs[c];
+// worklist item: (cancellation, key, arc target 0, arc target 1); results go to rr at the item's
+// list position
+__device__ __forceinline__ void geo_find_one(const ArcSide& A, const GeoState& g, const int4 it, int64_t j,
+                                             uint64_t tag) {
   int R0, R1;
   int32_t L0, L1;
-  geo_root(g, A.tinit[2 * s], R0, L0);
-  geo_root(g, A.tinit[2 * s + 1], R1, L1);
-  g.rr[4 * c] = R0;
-  g.rr[4 * c + 1] = R1;
-  g.rr[4 * c + 2] = L0;
-  g.rr[4 * c + 3] = L1;
-  const uint64_t key = tag | (uint32_t)g.key[c];
+  geo_root(g, it.z, R0, L0);
+  geo_root(g, it.w, R1, L1);
+  *reinterpret_cast<int4*>(g.rr + 4 * j) = make_int4(R0, R1, L0, L1);
+  const uint64_t key = tag | (uint32_t)it.y;
   if (R0 >= 0) atomicMin((unsigned long long*)&g.watch[R0], (unsigned long long)key);
   if (R1 >= 0) atomicMin((unsigned long long*)&g.watch[R1], (unsigned long long)key);
 }
@@ -193,7 +209,7 @@ __device__ __forceinline__ void geo_find_phase(const ArcSide& A, const GeoState&
   if (blockIdx.x == 0 && threadIdx.x == 0) g.cnt[cur ^ 1] = 0;
   const uint64_t tag = (uint64_t)(0xFFFFFFFFu - (uint32_t)round) << 32;
   for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x)
-    geo_find_one(A, g, g.wl[cur][j], tag);
+    geo_find_one(A, g, g.wl[cur][j], j, tag);
 }
 
 // Both traced sides of one round in one launch: side 0's items first, then side 1's.
@@ -207,8 +223,8 @@ __global__ void k_geo_find2(ArcSide A0, GeoState G0, ArcSide A1, GeoState G1, in
   const uint64_t tag = (uint64_t)(0xFFFFFFFFu - (uint32_t)round) << 32;
   for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < (int64_t)n0 + n1;
        j += (int64_t)gridDim.x * blockDim.x) {
-    if (j >= n0) geo_find_one(A1, G1, G1.wl[cur][j - n0], tag);
-    else geo_find_one(A0, G0, G0.wl[cur][j], tag);
+    if (j >= n0) geo_find_one(A1, G1, G1.wl[cur][j - n0], j - n0, tag);
+    else geo_find_one(A0, G0, G0.wl[cur][j], j, tag);
   }
 }
 __global__ void k_geo_find(ArcSide A, GeoState g, int round) { geo_find_phase(A, g, round); }
@@ -217,23 +233,26 @@ __global__ void k_geo_find(ArcSide A, GeoState g, int round) { geo_find_phase(A,
 // absent, or comes later in (persistence, saddle id) order.
 __device__ __forceinline__ bool root_settled(const GeoState& g, int R, int kc, int round) {
   if (R < 0) return true;
-  const int dk = g.deadc[R];
-  return dk < 0 || g.done[dk] < round || g.key[dk] >= kc;
+  const uint64_t k = g.kill[R];
+  return (uint32_t)k == 0xFFFFFFFFu || (uint32_t)(k >> 32) < (uint32_t)round || (int)(uint32_t)k >= kc;
 }
 
 // Decides one pending cancellation (index c, saddle-id order); returns true when it must wait.
-__device__ __forceinline__ bool geo_apply_one(const ArcSide& A, const GeoState& g, int c, int round, uint64_t tag) {
-  const int R0 = g.rr[4 * c], R1 = g.rr[4 * c + 1];
-  const int32_t L0 = g.rr[4 * c + 2], L1 = g.rr[4 * c + 3];
+__device__ __forceinline__ bool geo_apply_one(const ArcSide& A, const GeoState& g, const int4 it, int64_t j, int round,
+                                              uint64_t tag) {
+  const int4 r = *reinterpret_cast<const int4*>(g.rr + 4 * j);
+  const int R0 = r.x, R1 = r.y;
+  const int32_t L0 = r.z, L1 = r.w;
+  const int c = it.x;
   const int d = g.cd[c];
-  const int kc = g.key[c];
+  const int kc = it.y;
   int kd = -1;
   if (R0 == d && R0 != R1) kd = 0;
   else if (R1 == d && R0 != R1) kd = 1;
   bool ready = root_settled(g, R0, kc, round) && root_settled(g, R1, kc, round);
   if (ready && kd >= 0) ready = g.watch[d] == (tag | (uint32_t)kc);
   if (!ready) return true;
-  g.done[c] = round;
+  g.kill[d] = ((uint64_t)(uint32_t)round << 32) | (uint32_t)kc;  // c is decided (d is c's dead extremum)
   if (kd >= 0) {
     const int s = g.cs[c], ko = 1 - kd;
     const int Ro = kd == 0 ? R1 : R0;
@@ -256,15 +275,15 @@ __device__ __forceinline__ void geo_apply_chunks(const ArcSide& A, const GeoStat
                                                  uint64_t tag) {
   DMSC_BLOCK_LOOP(start, (int64_t)n, 256 * ITEMS) {
     const int64_t j0 = start + (int64_t)threadIdx.x * ITEMS;
-    int c[ITEMS];
+    int4 c[ITEMS];
     int32_t nwait = 0;
     uint32_t waitmask = 0;
 #pragma unroll
     for (int u = 0; u < ITEMS; ++u) {
-      c[u] = 0;
+      c[u] = make_int4(0, 0, 0, 0);
       if (j0 + u < n) {
         c[u] = g.wl[cur][j0 + u];
-        if (geo_apply_one(A, g, c[u], round, tag)) {
+        if (geo_apply_one(A, g, c[u], j0 + u, round, tag)) {
           waitmask |= 1u << u;
           ++nwait;
         }
@@ -296,15 +315,15 @@ template <int ITEMS>
 __device__ __forceinline__ void geo_apply_chunk(const ArcSide& A, const GeoState& g, int round, int cur, int64_t start,
                                                 int n, uint64_t tag) {
   const int64_t j0 = start + (int64_t)threadIdx.x * ITEMS;
-  int c[ITEMS];
+  int4 c[ITEMS];
   int32_t nwait = 0;
   uint32_t waitmask = 0;
 #pragma unroll
   for (int u = 0; u < ITEMS; ++u) {
-    c[u] = 0;
+    c[u] = make_int4(0, 0, 0, 0);
     if (j0 + u < n) {
       c[u] = g.wl[cur][j0 + u];
-      if (geo_apply_one(A, g, c[u], round, tag)) {
+      if (geo_apply_one(A, g, c[u], j0 + u, round, tag)) {
         waitmask |= 1u << u;
         ++nwait;
       }

@@ -42,13 +42,14 @@ __device__ __forceinline__ int uf_find(int32_t* __restrict__ p, int x, int32_t* 
 
 // rr[2i] = dying root (-2: cycle); rr[2i+1] = (surviving root + 1) << 1 | (dying root came from c[2i+1])
 template <bool DUAL>
-__device__ __forceinline__ void persist_find_one(const PersistSide& s, int i, uint64_t tag) {
-  const int ra = uf_find(s.par_g, s.c[2 * i], s.err);
-  const int rb = uf_find(s.par_g, s.c[2 * i + 1], s.err);
+__device__ __forceinline__ void persist_find_one(const PersistSide& s, const int i, int64_t j, uint64_t tag) {
+  const int2 cc = *reinterpret_cast<const int2*>(s.c + 2 * i);
+  const int ra = uf_find(s.par_g, cc.x, s.err);
+  const int rb = uf_find(s.par_g, cc.y, s.err);
   if (ra == rb) {
     s.pair[i] = -1;
     s.pers[i] = -1.0f;
-    s.rr[2 * i] = -2;
+    s.rr[2 * j] = -2;
     return;
   }
   bool b_dies;
@@ -64,8 +65,7 @@ __device__ __forceinline__ void persist_find_one(const PersistSide& s, int i, ui
     b_dies = !a_dies;
   }
   const int dying = b_dies ? rb : ra, surv = b_dies ? ra : rb;
-  s.rr[2 * i] = dying;
-  s.rr[2 * i + 1] = ((surv + 1) << 1) | (b_dies ? 1 : 0);
+  *reinterpret_cast<int2*>(s.rr + 2 * j) = make_int2(dying, ((surv + 1) << 1) | (b_dies ? 1 : 0));
   const uint64_t key = tag | (uint32_t)s.pos[i];
   atomicMin((unsigned long long*)&s.minkey[dying], (unsigned long long)key);
   if (surv >= 0) atomicMin((unsigned long long*)&s.minkey[surv], (unsigned long long)key);
@@ -81,9 +81,10 @@ __device__ __forceinline__ void persist_find_phase(const PersistSide& s0, const 
   }
   const uint64_t tag = (uint64_t)(0xFFFFFFFFu - (uint32_t)round) << 32;
   for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n0 + n1; j += (int64_t)gridDim.x * blockDim.x) {
-    // worklists hold saddle indices (cell-id order); the sweep position is the merge key
-    if (j >= n0) persist_find_one<DUAL>(s1, s1.wl[cur][j - n0], tag);
-    else persist_find_one<DUAL>(s0, s0.wl[cur][j], tag);
+    // worklists hold saddle indices in saddle-id order (the sweep position is the merge key); the
+    // round's decisions go to rr at the item's list position
+    if (j >= n0) persist_find_one<DUAL>(s1, s1.wl[cur][j - n0], j - n0, tag);
+    else persist_find_one<DUAL>(s0, s0.wl[cur][j], j, tag);
   }
 }
 template <bool DUAL>
@@ -93,12 +94,13 @@ __global__ void k_persist_find(PersistSide s0, PersistSide s1, int round) {
 
 // Decides one pending saddle i; returns true when it must wait for a later round.
 template <bool DUAL>
-__device__ __forceinline__ bool persist_apply_one(const PersistSide& s, int i, uint64_t tag,
+__device__ __forceinline__ bool persist_apply_one(const PersistSide& s, const int i, int64_t j, uint64_t tag,
                                                   uint8_t* __restrict__ removed) {
-  const int dying = s.rr[2 * i];
+  const int2 r = *reinterpret_cast<const int2*>(s.rr + 2 * j);
+  const int dying = r.x;
   if (dying == -2) return false;
   if (s.minkey[dying] != (tag | (uint32_t)s.pos[i])) return true;
-  const int code = s.rr[2 * i + 1];
+  const int code = r.y;
   const int surv = (code >> 1) - 1;
   const bool b_dies = code & 1;
   const float sv = s.sval[i], dv = key_value(s.ekey[dying]);
@@ -112,7 +114,8 @@ __device__ __forceinline__ bool persist_apply_one(const PersistSide& s, int i, u
   if (p <= thr) {
     s.alive[dying] = 0;
     removed[i] = 1;
-    const int a = s.c[2 * i], b = s.c[2 * i + 1];
+    const int2 cc = *reinterpret_cast<const int2*>(s.c + 2 * i);
+    const int a = cc.x, b = cc.y;
     const int d = uf_find(s.par_t, b_dies ? b : a, s.err);
     const int t = uf_find(s.par_t, b_dies ? a : b, s.err);
     if (d != t && d != -1) s.par_t[d] = t;
@@ -138,7 +141,7 @@ __device__ __forceinline__ void persist_apply_chunk(const PersistSide& s, int cu
     k[u] = 0;
     if (j0 + u < n) {
       k[u] = s.wl[cur][j0 + u];
-      if (persist_apply_one<DUAL>(s, k[u], tag, removed)) {
+      if (persist_apply_one<DUAL>(s, k[u], j0 + u, tag, removed)) {
         waitmask |= 1u << u;
         ++nwait;
       }

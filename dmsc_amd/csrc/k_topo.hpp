@@ -18,7 +18,7 @@ struct PersistSide {
   int32_t* pair;        // per saddle: dying extremum index, -1 for a cycle
   float* pers;          // per saddle persistence (-1 for a cycle)
   int32_t* cancel_d;    // per saddle thresholded dead extremum (null when arcs are not traced)
-  int32_t* rr;          // per saddle: dying root (-2 = cycle), then (surviving root + 1) << 1 | side bit
+  int32_t* rr;          // per worklist position: dying root (-2 = cycle), (surviving root + 1) << 1 | side bit
   int32_t* wl[2];       // ping-pong worklists of pending saddle indices
   int32_t* cnt;         // [2] worklist sizes (device)
   int32_t* err;         // device error flags
@@ -35,6 +35,7 @@ void launch_persist_round(bool dual, const PersistSide& s0, const PersistSide& s
 bool launch_persist_rounds(bool dual, const PersistSide& s0, const PersistSide& s1, int round0, int max_rounds,
                            uint8_t* removed, int32_t* round_out, int64_t max_items, hipStream_t st);
 void launch_uf_flatten(int32_t* par, int32_t* root, int64_t n, hipStream_t st);
+
 void launch_u8_to_i32(const uint8_t* a, int32_t* o, int64_t n, bool invert, hipStream_t st);
 void launch_mask_prefix(const uint8_t* alive, int32_t* pre, int64_t n, hipStream_t st);
 void launch_side_counts(int64_t ns, const int32_t* ord, const uint8_t* removed, const int32_t* c, const int32_t* root_t,
@@ -85,16 +86,16 @@ struct GeoState {
   const int32_t* cs;       // cancel -> saddle index
   const int32_t* cd;       // cancel -> dead extremum
   const int32_t* key;      // cancel -> sorted position
-  const int32_t* deadc;    // extremum -> cancel whose dead it is (-1)
+  uint64_t* kill;          // per extremum: (round its killing cancellation was decided, or all ones while
+                           // pending) << 32 | that cancellation's key (all ones: no killer)
   int32_t* gparent;        // geometry union-find (-1 = void)
   uint64_t* gjump;         // path-compressed view of gparent: (ancestor or kVoid, link length to it)
   int32_t* linklen;        // length of the link path of a linked extremum (saturating int32)
   int32_t* linkc;          // cancel that linked the extremum
   int32_t* linkkd;         // which arc (0/1) of that cancel's saddle led to the dead extremum
-  int32_t* done;           // per cancel: round processed (INT_MAX while pending)
   uint64_t* watch;         // per extremum: (tag << 32 | cancel) atomic min
-  int32_t* rr;             // per cancel: R0, R1, len0, len1
-  int32_t* wl[2];
+  int32_t* rr;             // per worklist position: R0, R1, len0, len1
+  int4* wl[2];             // worklists: (cancellation, key, arc target 0, arc target 1)
   int32_t* cnt;
   int32_t* err;
 };
@@ -108,7 +109,10 @@ bool launch_geo_rounds(const ArcSide* a, const GeoState* g, int live, int round0
 void launch_cancel_flags(int64_t ns, const int32_t* cancel_d, int32_t* flag, hipStream_t st);
 void launch_cancel_compact(int64_t ns, const int32_t* cancel_d, const float* pers, const int32_t* pre,
                            int32_t* cs, int32_t* cd, uint32_t* keys, hipStream_t st);
-void launch_cancel_finish(int64_t nc, const int32_t* slot, const int32_t* cd, int32_t* key, int32_t* deadc,
+constexpr uint64_t kKillPending = 0xFFFFFFFFull << 32;
+void launch_geo_items(int64_t nc, const int32_t* cs, const int32_t* key, const int32_t* tinit, int4* wl,
+                      hipStream_t st);
+void launch_cancel_finish(int64_t nc, const int32_t* slot, const int32_t* cd, int32_t* key, uint64_t* kill,
                           hipStream_t st);
 void launch_geo_q(const GeoState& g, int64_t next, uint64_t* pjq, hipStream_t st);
 
