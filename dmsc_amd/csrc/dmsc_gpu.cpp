@@ -277,6 +277,16 @@ pybind11::object debug_fetch(int device, const std::string& name, int64_t capaci
 
 void profile_enable(int device, bool on) { dmsc_profile_enable(context_for(device), on ? 1 : 0); }
 
+void set_param(int device, const std::string& name, int64_t value) {
+  TORCH_CHECK(dmsc_set_param(context_for(device), name.c_str(), value) == DMSC_OK,
+              "dmsc: unknown parameter or value out of range: ", name, "=", value);
+}
+int64_t get_param(int device, const std::string& name) {
+  int64_t v = 0;
+  TORCH_CHECK(dmsc_get_param(context_for(device), name.c_str(), &v) == DMSC_OK, "dmsc: unknown parameter: ", name);
+  return v;
+}
+
 pybind11::dict profile_read(int device) {
   dmsc_context* ctx = context_for(device);
   pybind11::dict d;
@@ -294,6 +304,8 @@ pybind11::dict profile_read(int device) {
 PYBIND11_MODULE(dmsc_gpu, m) {
   m.def("profile_enable", &profile_enable, pybind11::arg("device"), pybind11::arg("on"));
   m.def("profile_read", &profile_read, pybind11::arg("device"));
+  m.def("set_param", &set_param, pybind11::arg("device"), pybind11::arg("name"), pybind11::arg("value"));
+  m.def("get_param", &get_param, pybind11::arg("device"), pybind11::arg("name"));
   auto cls = pybind11::class_<DMSComplex>(m, "DMSComplex", pybind11::module_local(),
                                           "Discrete Morse-Smale complex (19 tensors, reference slot order)");
   for (int i = 0; i < DMSC_NUM_OUTPUTS; ++i)

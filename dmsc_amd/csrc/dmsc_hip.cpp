@@ -54,6 +54,7 @@ enum ArcSlot2 { H_PJQ = 0, H_FLEN, H_PERPOS, H_POSPRE, H_SEGA, H_SEGB, H_RAW, H_
 
 namespace dmsc {
 thread_local KernelTimes* tl_times = nullptr;
+thread_local const Tunables* tl_tune = nullptr;
 }
 
 using namespace dmsc;
@@ -61,6 +62,7 @@ using namespace dmsc;
 struct dmsc_context {
   int device = 0;
   KernelTimes times;
+  Tunables tune;
   Arena ar;
   Arena ar2;  // arc emission scratch
   int64_t* pinned = nullptr;
@@ -156,6 +158,15 @@ struct ArcWork {
 };
 
 constexpr int kCoopRounds = 1 << 12;  // rounds per cooperative launch before the host checks in
+
+// Rounds to launch before the next host check, after `done` rounds: rounds_first, then doubling
+// up to rounds_max (a round past convergence finds an empty worklist and only costs its launch).
+int round_batch(int done) {
+  const Tunables& t = tune();
+  int b = t.rounds_first;
+  for (int sum = b; sum <= done && b < t.rounds_max; sum += b) b = std::min(2 * b, t.rounds_max);
+  return b;
+}
 
 // Round loops: one launch per phase with a host readback every few rounds. DMSC_ROUNDS=coop selects
 // one cooperative launch per loop instead; that path is off by default because a large-grid run
@@ -449,7 +460,7 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
         const int32_t left[2] = {h[2 + (round & 1)], h[4 + (round & 1)]};
         std::memcpy(ctx->pinned, left, 8);
       } else {
-        const int batch = round < 4 ? 4 : (round < 12 ? 8 : 16);  // rounds per host check
+        const int batch = round_batch(round);
         for (int b = 0; b < batch; ++b, ++round) launch_persist_round(dual, ps[0], ps[1], round, removed, pend, st);
         DMSC_CHECK(hipMemcpyAsync(ctx->pinned, sb[0].cnt + (round & 1), 4, hipMemcpyDeviceToHost, st));
         DMSC_CHECK(hipMemcpyAsync((char*)ctx->pinned + 4, sb[1].cnt + (round & 1), 4, hipMemcpyDeviceToHost, st));
@@ -602,7 +613,7 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
         const int32_t left[2] = {live[0] ? h[2 + (r & 1)] : 0, live[1] ? h[4 + (r & 1)] : 0};
         std::memcpy(ctx->pinned, left, 8);
       } else {
-        const int batch = r < 4 ? 4 : (r < 12 ? 8 : 16);  // rounds per host check
+        const int batch = round_batch(r);
         for (int k = 0; k < batch; ++k, ++r) launch_geo_round2(as, gs, mask, r, pend[0] + pend[1], st);
         for (int s = 0; s < 2; ++s)
           if (live[s]) DMSC_CHECK(hipMemcpyAsync((char*)ctx->pinned + 4 * s, aw[s].G.cnt + (r & 1), 4,
@@ -719,7 +730,7 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     while (n_in > 0) {
       if (n_in > cap) throw Fail{DMSC_ERR_INTERNAL, "arc segment list overflow"};
       // a batch of levels per host check; a level past the last one finds no chains and only clears
-      const int batch = level < 4 ? 2 : 8;
+      const int batch = level < 4 ? std::min(2, ctx->tune.expand_levels) : ctx->tune.expand_levels;
       for (int k = 0; k < batch; ++k, ++level)
         launch_expand_segments(w.A, w.G, w.pjq, w.cum_r, w.cum_c, lists[level & 1], nseg + level % 3,
                                lists[(level + 1) & 1], nseg + (level + 1) % 3, nseg + (level + 2) % 3, raw, nseg + 3,
@@ -796,6 +807,7 @@ dmsc_status guarded(dmsc_context* ctx, const std::function<void()>& fn) {
   if (!ctx) return DMSC_ERR_INVALID_ARGUMENT;
   ctx->err.clear();
   tl_times = &ctx->times;
+  tl_tune = &ctx->tune;
   try {
     DMSC_CHECK(hipSetDevice(ctx->device));
     fn();
@@ -888,6 +900,36 @@ dmsc_status dmsc_persistence_backward(dmsc_context* ctx, int32_t B, const float*
       ctx->times.collect();
     }
   });
+}
+
+namespace {
+int* param_slot(dmsc_context* ctx, const std::string& n, int64_t& lo, int64_t& hi) {
+  Tunables& t = ctx->tune;
+  if (n == "round_grid") { lo = 64; hi = 65536; return &t.round_grid; }
+  if (n == "small_list") { lo = 0; hi = 1 << 30; return &t.small_list; }
+  if (n == "rounds_first") { lo = 1; hi = 4096; return &t.rounds_first; }
+  if (n == "rounds_max") { lo = 1; hi = 4096; return &t.rounds_max; }
+  if (n == "expand_levels") { lo = 1; hi = 4096; return &t.expand_levels; }
+  return nullptr;
+}
+}  // namespace
+
+dmsc_status dmsc_set_param(dmsc_context* ctx, const char* name, int64_t value) {
+  if (!ctx || !name) return DMSC_ERR_INVALID_ARGUMENT;
+  int64_t lo = 0, hi = 0;
+  int* p = param_slot(ctx, name, lo, hi);
+  if (!p || value < lo || value > hi) return DMSC_ERR_INVALID_ARGUMENT;
+  *p = (int)value;
+  return DMSC_OK;
+}
+
+dmsc_status dmsc_get_param(dmsc_context* ctx, const char* name, int64_t* value) {
+  if (!ctx || !name || !value) return DMSC_ERR_INVALID_ARGUMENT;
+  int64_t lo = 0, hi = 0;
+  int* p = param_slot(ctx, name, lo, hi);
+  if (!p) return DMSC_ERR_INVALID_ARGUMENT;
+  *value = *p;
+  return DMSC_OK;
 }
 
 const char* dmsc_last_error(dmsc_context* ctx) { return ctx ? ctx->err.c_str() : "null context"; }

@@ -113,6 +113,20 @@ struct KernelTimes {
 };
 extern thread_local KernelTimes* tl_times;
 
+// Scheduling parameters of a context (set through dmsc_set_param; results never depend on them).
+struct Tunables {
+  int round_grid = 2048;     // most blocks a round kernel (persistence, geometry) is launched with
+  int small_list = 1 << 16;  // worklists up to this size are processed one item per thread
+  int rounds_first = 4;      // rounds launched before the first host check of a round loop
+  int rounds_max = 16;       // most rounds between host checks (the batch doubles up to this)
+  int expand_levels = 8;     // arc expansion levels between host checks
+};
+extern thread_local const Tunables* tl_tune;
+inline const Tunables& tune() {
+  static const Tunables defaults;
+  return tl_tune ? *tl_tune : defaults;
+}
+
 struct KScope {
   KernelTimes* t;
   const char* name;

@@ -268,7 +268,7 @@ __device__ __forceinline__ bool geo_apply_one(const ArcSide& A, const GeoState& 
 }
 
 // Small worklists (the tail of the rounds) are latency-bound: one item per thread there.
-constexpr int kGeoItems = 4, kGeoChunk = 256 * kGeoItems, kGeoSmall = 1 << 16;
+constexpr int kGeoItems = 4, kGeoChunk = 256 * kGeoItems, kGeoSmall = 1 << 16;  // kGeoSmall: cooperative path
 
 template <int ITEMS>
 __device__ __forceinline__ void geo_apply_chunks(const ArcSide& A, const GeoState& g, int round, int cur, int n,
@@ -349,22 +349,23 @@ __device__ __forceinline__ void geo_apply_both(const ArcSide& A0, const GeoState
   }
 }
 __global__ __launch_bounds__(256) void k_geo_apply2(ArcSide A0, GeoState G0, ArcSide A1, GeoState G1, int live,
-                                                    int round) {
+                                                    int round, int small_list) {
   const int cur = round & 1;
   const int n0 = (live & 1) ? G0.cnt[cur] : 0, n1 = (live & 2) ? G1.cnt[cur] : 0;
   const uint64_t tag = (uint64_t)(0xFFFFFFFFu - (uint32_t)round) << 32;
-  if ((int64_t)n0 + n1 <= kGeoSmall) geo_apply_both<1>(A0, G0, A1, G1, round, cur, n0, n1, tag);
+  if ((int64_t)n0 + n1 <= small_list) geo_apply_both<1>(A0, G0, A1, G1, round, cur, n0, n1, tag);
   else geo_apply_both<kGeoItems>(A0, G0, A1, G1, round, cur, n0, n1, tag);
 }
 
 void launch_geo_round2(const ArcSide* a, const GeoState* g, int live, int round, int64_t max_items, hipStream_t st) {
   KScope ks_("geo_round", st);
-  hipLaunchKernelGGL(k_geo_find2, dim3(grid_for(max_items, 256, 2048)), dim3(256), 0, st, a[0], g[0], a[1], g[1], live,
-                     round);
+  const Tunables& tu = tune();
+  hipLaunchKernelGGL(k_geo_find2, dim3(grid_for(max_items, 256, tu.round_grid)), dim3(256), 0, st, a[0], g[0], a[1],
+                     g[1], live, round);
   DMSC_LAUNCH_CHECK();
-  const int per = max_items <= kGeoSmall ? 256 : kGeoChunk;
-  hipLaunchKernelGGL(k_geo_apply2, dim3(grid_for(max_items, per, 2048)), dim3(256), 0, st, a[0], g[0], a[1], g[1], live,
-                     round);
+  const int per = max_items <= tu.small_list ? 256 : kGeoChunk;
+  hipLaunchKernelGGL(k_geo_apply2, dim3(grid_for(max_items, per, tu.round_grid)), dim3(256), 0, st, a[0], g[0], a[1],
+                     g[1], live, round, tu.small_list);
   DMSC_LAUNCH_CHECK();
 }
 

@@ -127,7 +127,7 @@ __device__ __forceinline__ bool persist_apply_one(const PersistSide& s, const in
 // One chunk of a side's worklist: kApplyItems consecutive items per thread; waiting items are
 // appended to the next worklist with one block-wide reservation (order preserved within the chunk).
 // Small worklists (the tail of the sweep) are latency-bound: one item per thread there.
-constexpr int kApplyItems = 4, kApplyChunk = 256 * kApplyItems, kSmallList = 1 << 16;
+constexpr int kApplyItems = 4, kApplyChunk = 256 * kApplyItems;
 
 template <bool DUAL, int ITEMS>
 __device__ __forceinline__ void persist_apply_chunk(const PersistSide& s, int cur, int64_t start, int n, uint64_t tag,
@@ -160,11 +160,11 @@ __device__ __forceinline__ void persist_apply_chunk(const PersistSide& s, int cu
 // Chunks of side 0 come first, then those of side 1, so every chunk (and block) has one side.
 template <bool DUAL>
 __device__ __forceinline__ void persist_apply_phase(const PersistSide& s0, const PersistSide& s1, int round,
-                                                    uint8_t* __restrict__ removed) {
+                                                    uint8_t* __restrict__ removed, int small_list) {
   const int cur = round & 1;
   const int n0 = s0.cnt[cur], n1 = s1.cnt[cur];
   const uint64_t tag = (uint64_t)(0xFFFFFFFFu - (uint32_t)round) << 32;
-  if ((int64_t)n0 + n1 <= kSmallList) {
+  if ((int64_t)n0 + n1 <= small_list) {
     const int64_t nc0 = ceil_div(n0, 256), nc = nc0 + ceil_div(n1, 256);
     for (int64_t c = blockIdx.x; c < nc; c += gridDim.x) {
       if (c >= nc0) persist_apply_chunk<DUAL, 1>(s1, cur, (c - nc0) * 256, n1, tag, removed);
@@ -180,8 +180,8 @@ __device__ __forceinline__ void persist_apply_phase(const PersistSide& s0, const
 }
 template <bool DUAL>
 __global__ __launch_bounds__(256) void k_persist_apply(PersistSide s0, PersistSide s1, int round,
-                                                       uint8_t* __restrict__ removed) {
-  persist_apply_phase<DUAL>(s0, s1, round, removed);
+                                                       uint8_t* __restrict__ removed, int small_list) {
+  persist_apply_phase<DUAL>(s0, s1, round, removed, small_list);
 }
 
 // All rounds in one cooperative launch: find, grid barrier, apply, grid barrier, until both
@@ -195,7 +195,7 @@ __global__ __launch_bounds__(256) void k_persist_rounds(PersistSide s0, PersistS
   while (r < round0 + max_rounds) {
     persist_find_phase<DUAL>(s0, s1, r);
     grid.sync();
-    persist_apply_phase<DUAL>(s0, s1, r, removed);
+    persist_apply_phase<DUAL>(s0, s1, r, removed, 1 << 16);
     grid.sync();
     ++r;
     const int p0 = __hip_atomic_load(&s0.cnt[r & 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -237,15 +237,16 @@ bool launch_persist_rounds(bool dual, const PersistSide& s0, const PersistSide& 
 void launch_persist_round(bool dual, const PersistSide& s0, const PersistSide& s1, int round, uint8_t* removed,
                           int64_t max_items, hipStream_t st) {
   KScope ks_("persist_round", st);
-  const int g = grid_for(max_items, 256, 2048);
+  const Tunables& tu = tune();
+  const int g = grid_for(max_items, 256, tu.round_grid);
   if (dual) hipLaunchKernelGGL(k_persist_find<true>, dim3(g), dim3(256), 0, st, s0, s1, round);
   else hipLaunchKernelGGL(k_persist_find<false>, dim3(g), dim3(256), 0, st, s0, s1, round);
   DMSC_LAUNCH_CHECK();
-  const int ga = grid_for(max_items, max_items <= kSmallList ? 256 : kApplyChunk, 2048);
+  const int ga = grid_for(max_items, max_items <= tu.small_list ? 256 : kApplyChunk, tu.round_grid);
   if (dual)
-    hipLaunchKernelGGL(k_persist_apply<true>, dim3(ga), dim3(256), 0, st, s0, s1, round, removed);
+    hipLaunchKernelGGL(k_persist_apply<true>, dim3(ga), dim3(256), 0, st, s0, s1, round, removed, tu.small_list);
   else
-    hipLaunchKernelGGL(k_persist_apply<false>, dim3(ga), dim3(256), 0, st, s0, s1, round, removed);
+    hipLaunchKernelGGL(k_persist_apply<false>, dim3(ga), dim3(256), 0, st, s0, s1, round, removed, tu.small_list);
   DMSC_LAUNCH_CHECK();
 }
 

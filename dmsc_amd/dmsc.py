@@ -112,6 +112,13 @@ class MSComplex:
     plot_gradient = plot_complex = plot_barcode = plot
 
 
+def _tuned(device):
+    """Saved scheduling parameters of this GPU's architecture (autotune.py), applied once per device."""
+    from . import autotune
+
+    autotune.apply_saved(device.index if device.index is not None else torch.cuda.current_device())
+
+
 def _validate(img):
     # reference dmsc.py:245-254 (same exception types and messages)
     if img.dim() not in (2, 3):
@@ -133,6 +140,7 @@ def _compute_dmsc_on_device(img, persistence_threshold, return_gradient, is_dual
         raise RuntimeError(
             f"dmsc_amd computes on AMD Instinct GPUs only; got a tensor on '{img.device}'. Move it with .to('cuda')."
         )
+    _tuned(img.device)
     return dmsc_gpu.extract_dmsc(img, persistence_threshold, return_gradient, is_dual, trace_max_arcs, trace_min_arcs,
                                  trace_max_groups, trace_min_groups)
 
@@ -171,6 +179,7 @@ def compute_dmsc_batch(fields_list, persistence_thresholds, return_gradient=True
         persistence_thresholds = [float(persistence_thresholds)] * len(fields_list)
     if fields_list[0].device.type != "cuda":
         raise RuntimeError("dmsc_amd computes on AMD Instinct GPUs only; move the fields with .to('cuda').")
+    _tuned(fields_list[0].device)
     out = dmsc_gpu.extract_dmsc_list(list(fields_list), [float(t) for t in persistence_thresholds], return_gradient,
                                      is_dual, trace_ridges, trace_valleys, trace_peaks, trace_basins)
     return [MSComplex(*r) for r in out]

@@ -124,6 +124,17 @@ int32_t dmsc_profile_count(dmsc_context* ctx);
 dmsc_status dmsc_profile_entry(dmsc_context* ctx, int32_t index, const char** name, double* total_ms,
                                int64_t* launches);
 
+/* Scheduling parameters of a context; outputs never depend on them (no reference counterpart: the
+ * autotuner, dmsc_amd/autotune.py, picks them per GPU). Names and ranges:
+ *   "round_grid"    64..65536  most blocks of a persistence / geometry round kernel (default 2048)
+ *   "small_list"    0..2^30    worklists up to this size run one item per thread (default 65536)
+ *   "rounds_first"  1..4096    rounds before the first host check of a round loop (default 4)
+ *   "rounds_max"    1..4096    most rounds between host checks (default 16)
+ *   "expand_levels" 1..4096    arc expansion levels between host checks (default 8)
+ * Unknown names or values out of range return DMSC_ERR_INVALID_ARGUMENT. */
+dmsc_status dmsc_set_param(dmsc_context* ctx, const char* name, int64_t value);
+dmsc_status dmsc_get_param(dmsc_context* ctx, const char* name, int64_t* value);
+
 /* Message for the last failing call on this context (valid until the next call). */
 const char* dmsc_last_error(dmsc_context* ctx);
 

@@ -95,3 +95,41 @@ def test_batch_split_into_groups():
         check_batch(fields, thrs)
     finally:
         del os.environ["DMSC_GROUP_SLOTS"]
+
+
+def test_scheduling_parameters_do_not_change_results():
+    import torch
+
+    from dmsc_amd.csrc import dmsc_gpu
+
+    rng = np.random.default_rng(17)
+    fields = [rng.random(s).astype(np.float32) for s in [(40, 33), (64, 129), (7, 11)]]
+    fields.append(synthetic.quantized_noise(80, 60, 3, seed=4).numpy())
+    thrs = [0.0, 0.1, 0.3, 0.05]
+    names = ("round_grid", "small_list", "rounds_first", "rounds_max", "expand_levels")
+    saved = {k: dmsc_gpu.get_param(0, k) for k in names}
+    try:
+        for setting in ({"round_grid": 64, "small_list": 0, "rounds_first": 1, "rounds_max": 1, "expand_levels": 1},
+                        {"round_grid": 8192, "small_list": 1 << 20, "rounds_first": 64, "rounds_max": 64,
+                         "expand_levels": 64}):
+            for k, v in setting.items():
+                dmsc_gpu.set_param(0, k, v)
+            check_batch(fields, thrs, is_dual=False)
+            check_batch(fields, thrs, is_dual=True)
+        with pytest.raises(RuntimeError):
+            dmsc_gpu.set_param(0, "round_grid", 1)
+        with pytest.raises(RuntimeError):
+            dmsc_gpu.set_param(0, "no_such_parameter", 1)
+    finally:
+        for k, v in saved.items():
+            dmsc_gpu.set_param(0, k, v)
+    torch.cuda.synchronize()
+
+
+def test_autotune_smoke():
+    from dmsc_amd import autotune
+
+    res = autotune.tune(size=256, n_small=4, reps=1, save=False, log=lambda *_: None,
+                        candidates={"round_grid": [512], "rounds_max": [8]})
+    assert set(res["params"]) == set(autotune.DEFAULTS)
+    autotune.apply(autotune.DEFAULTS)
