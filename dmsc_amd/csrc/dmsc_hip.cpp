@@ -63,6 +63,7 @@ struct dmsc_context {
   int device = 0;
   KernelTimes times;
   Tunables tune;
+  hipStream_t cap_stream = nullptr;  // private stream that round batches are captured on
   Arena ar;
   Arena ar2;  // arc emission scratch
   int64_t* pinned = nullptr;
@@ -158,6 +159,46 @@ struct ArcWork {
 };
 
 constexpr int kCoopRounds = 1 << 12;  // rounds per cooperative launch before the host checks in
+
+// A batch of rounds recorded once and replayed on the work stream; its kernels read the round number
+// from device memory (set before each replay), so one graph serves every batch of the same size
+// while the launch sizes recorded in it still fit the worklists.
+struct CapturedRounds {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int batch = 0;
+  int64_t items = 0;  // worklist size the launch grids were sized for
+  void reset() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+  }
+  ~CapturedRounds() { reset(); }
+  bool fits(int b, int64_t n) const { return exec && batch == b && n <= items && 4 * n >= items; }
+  template <class F>
+  void capture(dmsc_context* ctx, int b, int64_t n, F&& record) {
+    reset();
+    if (!ctx->cap_stream) DMSC_CHECK(hipStreamCreateWithFlags(&ctx->cap_stream, hipStreamNonBlocking));
+    KernelTimes* saved = tl_times;
+    tl_times = nullptr;  // no timing events inside a graph
+    DMSC_CHECK(hipStreamBeginCapture(ctx->cap_stream, hipStreamCaptureModeThreadLocal));
+    try {
+      record(ctx->cap_stream);
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(ctx->cap_stream, &g);
+      if (g) (void)hipGraphDestroy(g);
+      tl_times = saved;
+      throw;
+    }
+    tl_times = saved;
+    DMSC_CHECK(hipStreamEndCapture(ctx->cap_stream, &graph));
+    DMSC_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    batch = b;
+    items = n;
+  }
+};
 
 // Rounds to launch before the next host check, after `done` rounds: rounds_first, then doubling
 // up to rounds_max (a round past convergence finds an empty worklist and only costs its launch).
@@ -444,6 +485,8 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
   }
   int round = 0;
   ctx->pending_log.clear();
+  int32_t* d_round = c32 + 50;  // round number for graph replays
+  CapturedRounds pgraph, ggraph;
   bool coop = cooperative_rounds();
   if (nsad > 0) {
     int64_t pend = 2 * nsad;  // pending items of both sides at the last readback (launch sizes)
@@ -461,7 +504,21 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
         std::memcpy(ctx->pinned, left, 8);
       } else {
         const int batch = round_batch(round);
-        for (int b = 0; b < batch; ++b, ++round) launch_persist_round(dual, ps[0], ps[1], round, removed, pend, st);
+        if (ctx->tune.graph_rounds) {
+          if (!pgraph.fits(batch, pend))
+            pgraph.capture(ctx, batch, pend, [&](hipStream_t cs) {
+              for (int b = 0; b < batch; ++b) {
+                launch_persist_round(dual, ps[0], ps[1], 0, removed, pend, cs, d_round);
+                launch_round_inc(d_round, cs);
+              }
+            });
+          fill_i32(d_round, round, 1, st);
+          KScope ks_("persist_round", st);
+          DMSC_CHECK(hipGraphLaunch(pgraph.exec, st));
+          round += batch;
+        } else {
+          for (int b = 0; b < batch; ++b, ++round) launch_persist_round(dual, ps[0], ps[1], round, removed, pend, st);
+        }
         DMSC_CHECK(hipMemcpyAsync(ctx->pinned, sb[0].cnt + (round & 1), 4, hipMemcpyDeviceToHost, st));
         DMSC_CHECK(hipMemcpyAsync((char*)ctx->pinned + 4, sb[1].cnt + (round & 1), 4, hipMemcpyDeviceToHost, st));
         DMSC_CHECK(hipStreamSynchronize(st));
@@ -596,6 +653,7 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     int r = 0;
     ctx->rounds_geo[0] = ctx->rounds_geo[1] = 0;
     bool gcoop = cooperative_rounds();
+    int ggraph_mask = -1;
     while (live[0] || live[1]) {
       const ArcSide as[2] = {aw[0].A, aw[1].A};
       const GeoState gs[2] = {aw[0].G, aw[1].G};
@@ -614,7 +672,23 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
         std::memcpy(ctx->pinned, left, 8);
       } else {
         const int batch = round_batch(r);
-        for (int k = 0; k < batch; ++k, ++r) launch_geo_round2(as, gs, mask, r, pend[0] + pend[1], st);
+        const int64_t items = pend[0] + pend[1];
+        if (ctx->tune.graph_rounds) {
+          if (!ggraph.fits(batch, items) || ggraph_mask != mask)
+            ggraph.capture(ctx, batch, items, [&](hipStream_t cs) {
+              for (int k = 0; k < batch; ++k) {
+                launch_geo_round2(as, gs, mask, 0, items, cs, d_round);
+                launch_round_inc(d_round, cs);
+              }
+            });
+          ggraph_mask = mask;
+          fill_i32(d_round, r, 1, st);
+          KScope ks_("geo_round", st);
+          DMSC_CHECK(hipGraphLaunch(ggraph.exec, st));
+          r += batch;
+        } else {
+          for (int k = 0; k < batch; ++k, ++r) launch_geo_round2(as, gs, mask, r, items, st);
+        }
         for (int s = 0; s < 2; ++s)
           if (live[s]) DMSC_CHECK(hipMemcpyAsync((char*)ctx->pinned + 4 * s, aw[s].G.cnt + (r & 1), 4,
                                                  hipMemcpyDeviceToHost, st));
@@ -848,6 +922,7 @@ void dmsc_context_destroy(dmsc_context* ctx) {
   ctx->ar.release();
   ctx->ar2.release();
   if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+  if (ctx->cap_stream) (void)hipStreamDestroy(ctx->cap_stream);
   delete ctx;
 }
 
@@ -910,6 +985,7 @@ int* param_slot(dmsc_context* ctx, const std::string& n, int64_t& lo, int64_t& h
   if (n == "rounds_first") { lo = 1; hi = 4096; return &t.rounds_first; }
   if (n == "rounds_max") { lo = 1; hi = 4096; return &t.rounds_max; }
   if (n == "expand_levels") { lo = 1; hi = 4096; return &t.expand_levels; }
+  if (n == "graph_rounds") { lo = 0; hi = 1; return &t.graph_rounds; }
   return nullptr;
 }
 }  // namespace

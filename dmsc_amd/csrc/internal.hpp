@@ -120,6 +120,7 @@ struct Tunables {
   int rounds_first = 4;      // rounds launched before the first host check of a round loop
   int rounds_max = 16;       // most rounds between host checks (the batch doubles up to this)
   int expand_levels = 8;     // arc expansion levels between host checks
+  int graph_rounds = 0;      // 1: each batch of rounds is a captured hipGraph, replayed while it fits
 };
 extern thread_local const Tunables* tl_tune;
 inline const Tunables& tune() {

@@ -213,7 +213,9 @@ __device__ __forceinline__ void geo_find_phase(const ArcSide& A, const GeoState&
 }
 
 // Both traced sides of one round in one launch: side 0's items first, then side 1's.
-__global__ void k_geo_find2(ArcSide A0, GeoState G0, ArcSide A1, GeoState G1, int live, int round) {
+__global__ void k_geo_find2(ArcSide A0, GeoState G0, ArcSide A1, GeoState G1, int live, int round,
+                            const int32_t* __restrict__ d_round) {
+  if (d_round) round = *d_round;  // graph replays take the round number from the device
   const int cur = round & 1;
   const int n0 = (live & 1) ? G0.cnt[cur] : 0, n1 = (live & 2) ? G1.cnt[cur] : 0;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -349,7 +351,8 @@ __device__ __forceinline__ void geo_apply_both(const ArcSide& A0, const GeoState
   }
 }
 __global__ __launch_bounds__(256) void k_geo_apply2(ArcSide A0, GeoState G0, ArcSide A1, GeoState G1, int live,
-                                                    int round, int small_list) {
+                                                    int round, const int32_t* __restrict__ d_round, int small_list) {
+  if (d_round) round = *d_round;
   const int cur = round & 1;
   const int n0 = (live & 1) ? G0.cnt[cur] : 0, n1 = (live & 2) ? G1.cnt[cur] : 0;
   const uint64_t tag = (uint64_t)(0xFFFFFFFFu - (uint32_t)round) << 32;
@@ -357,15 +360,16 @@ __global__ __launch_bounds__(256) void k_geo_apply2(ArcSide A0, GeoState G0, Arc
   else geo_apply_both<kGeoItems>(A0, G0, A1, G1, round, cur, n0, n1, tag);
 }
 
-void launch_geo_round2(const ArcSide* a, const GeoState* g, int live, int round, int64_t max_items, hipStream_t st) {
+void launch_geo_round2(const ArcSide* a, const GeoState* g, int live, int round, int64_t max_items, hipStream_t st,
+                       const int32_t* d_round) {
   KScope ks_("geo_round", st);
   const Tunables& tu = tune();
   hipLaunchKernelGGL(k_geo_find2, dim3(grid_for(max_items, 256, tu.round_grid)), dim3(256), 0, st, a[0], g[0], a[1],
-                     g[1], live, round);
+                     g[1], live, round, d_round);
   DMSC_LAUNCH_CHECK();
   const int per = max_items <= tu.small_list ? 256 : kGeoChunk;
   hipLaunchKernelGGL(k_geo_apply2, dim3(grid_for(max_items, per, tu.round_grid)), dim3(256), 0, st, a[0], g[0], a[1],
-                     g[1], live, round, tu.small_list);
+                     g[1], live, round, d_round, tu.small_list);
   DMSC_LAUNCH_CHECK();
 }
 

@@ -88,7 +88,8 @@ __device__ __forceinline__ void persist_find_phase(const PersistSide& s0, const 
   }
 }
 template <bool DUAL>
-__global__ void k_persist_find(PersistSide s0, PersistSide s1, int round) {
+__global__ void k_persist_find(PersistSide s0, PersistSide s1, int round, const int32_t* __restrict__ d_round) {
+  if (d_round) round = *d_round;  // graph replays take the round number from the device
   persist_find_phase<DUAL>(s0, s1, round);
 }
 
@@ -180,7 +181,9 @@ __device__ __forceinline__ void persist_apply_phase(const PersistSide& s0, const
 }
 template <bool DUAL>
 __global__ __launch_bounds__(256) void k_persist_apply(PersistSide s0, PersistSide s1, int round,
+                                                       const int32_t* __restrict__ d_round,
                                                        uint8_t* __restrict__ removed, int small_list) {
+  if (d_round) round = *d_round;
   persist_apply_phase<DUAL>(s0, s1, round, removed, small_list);
 }
 
@@ -234,19 +237,27 @@ bool launch_persist_rounds(bool dual, const PersistSide& s0, const PersistSide& 
   return true;
 }
 
+__global__ void k_round_inc(int32_t* d_round) { *d_round += 1; }
+void launch_round_inc(int32_t* d_round, hipStream_t st) {
+  hipLaunchKernelGGL(k_round_inc, dim3(1), dim3(1), 0, st, d_round);
+  DMSC_LAUNCH_CHECK();
+}
+
 void launch_persist_round(bool dual, const PersistSide& s0, const PersistSide& s1, int round, uint8_t* removed,
-                          int64_t max_items, hipStream_t st) {
+                          int64_t max_items, hipStream_t st, const int32_t* d_round) {
   KScope ks_("persist_round", st);
   const Tunables& tu = tune();
   const int g = grid_for(max_items, 256, tu.round_grid);
-  if (dual) hipLaunchKernelGGL(k_persist_find<true>, dim3(g), dim3(256), 0, st, s0, s1, round);
-  else hipLaunchKernelGGL(k_persist_find<false>, dim3(g), dim3(256), 0, st, s0, s1, round);
+  if (dual) hipLaunchKernelGGL(k_persist_find<true>, dim3(g), dim3(256), 0, st, s0, s1, round, d_round);
+  else hipLaunchKernelGGL(k_persist_find<false>, dim3(g), dim3(256), 0, st, s0, s1, round, d_round);
   DMSC_LAUNCH_CHECK();
   const int ga = grid_for(max_items, max_items <= tu.small_list ? 256 : kApplyChunk, tu.round_grid);
   if (dual)
-    hipLaunchKernelGGL(k_persist_apply<true>, dim3(ga), dim3(256), 0, st, s0, s1, round, removed, tu.small_list);
+    hipLaunchKernelGGL(k_persist_apply<true>, dim3(ga), dim3(256), 0, st, s0, s1, round, d_round, removed,
+                       tu.small_list);
   else
-    hipLaunchKernelGGL(k_persist_apply<false>, dim3(ga), dim3(256), 0, st, s0, s1, round, removed, tu.small_list);
+    hipLaunchKernelGGL(k_persist_apply<false>, dim3(ga), dim3(256), 0, st, s0, s1, round, d_round, removed,
+                       tu.small_list);
   DMSC_LAUNCH_CHECK();
 }
 

@@ -27,8 +27,10 @@ struct PersistSide {
   int maxside;
 };
 
+// d_round (optional): the kernels read the round number from there instead of `round` (graph replay)
 void launch_persist_round(bool dual, const PersistSide& s0, const PersistSide& s1, int round, uint8_t* removed,
-                          int64_t max_items, hipStream_t st);
+                          int64_t max_items, hipStream_t st, const int32_t* d_round = nullptr);
+void launch_round_inc(int32_t* d_round, hipStream_t st);
 // Every round up to max_rounds (or until both worklists drain) in one cooperative launch; the
 // round reached goes to *round_out. Returns false, having launched nothing, when the device or
 // the launch size does not allow a cooperative launch.
@@ -101,7 +103,8 @@ struct GeoState {
 };
 void launch_geo_round(const ArcSide& a, const GeoState& g, int round, int64_t max_items, hipStream_t st);
 // one round of both traced sides (bit s of `live`) in one find and one apply launch
-void launch_geo_round2(const ArcSide* a, const GeoState* g, int live, int round, int64_t max_items, hipStream_t st);
+void launch_geo_round2(const ArcSide* a, const GeoState* g, int live, int round, int64_t max_items, hipStream_t st,
+                       const int32_t* d_round = nullptr);
 // All geometry rounds of the traced sides (bit s of `live`) in one cooperative launch; see
 // launch_persist_rounds.
 bool launch_geo_rounds(const ArcSide* a, const GeoState* g, int live, int round0, int max_rounds, int32_t* round_out,
