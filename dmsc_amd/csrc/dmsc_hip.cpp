@@ -375,7 +375,10 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
   int32_t* fsucc = ar.get<int32_t>(A_FSUCC, S);
   int32_t* vsucc = ar.get<int32_t>(A_VSUCC, N);
   uint32_t* planes = ar.get<uint32_t>(A_PLANES, 4 * PW);
-  launch_gradient(dual, bt, d_tiles, (int64_t)tiles.size(), grad ? o_grad : nullptr, fsucc, vsucc, planes, PW, st);
+  uint64_t* pjf = ar.get<uint64_t>(A_PJF, S);
+  uint64_t* pjv = ar.get<uint64_t>(A_PJV, N);
+  launch_gradient(dual, bt, d_tiles, (int64_t)tiles.size(), grad ? o_grad : nullptr, fsucc, vsucc, planes, PW, pjf, pjv,
+                  st);
 
   int32_t* c32 = ar.get<int32_t>(A_C32, 64);
   int64_t* c64 = ar.get<int64_t>(A_C64, 64);
@@ -409,11 +412,7 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
   uint64_t* max_key = ar.get<uint64_t>(A_MAXVAL, nmax);
   launch_compact_lists(dual, rv, min_ids, min_key, sad_ids, sad_val, sad_field, max_ids, max_key, st);
 
-  // ---- V-path forests: terminal extremum and depth of every face / vertex
-  uint64_t* pjf = ar.get<uint64_t>(A_PJF, S);
-  uint64_t* pjv = ar.get<uint64_t>(A_PJV, N);
-  pointer_jump_init(fsucc, pjf, S, st);
-  pointer_jump_init(vsucc, pjv, N, st);
+  // ---- V-path forests: terminal extremum and depth of every face / vertex (from the tile-resolved state)
   pointer_jump(pjf, S, d_err, st);
   pointer_jump(pjv, N, d_err, st);
 

@@ -292,8 +292,9 @@ template <class T, class S>
 void bases_at(const T* pre, const S* starts, int64_t n, const T* total, int B, T* out, hipStream_t st);
 
 // grid stages (over all fields of a batch)
+// also writes the initial pointer-jumping state of both V-path forests (pjf over slots, pjv over pixels)
 void launch_gradient(bool dual, const Batch& bt, const TileRef* tiles, int64_t ntiles, int32_t* pw, int32_t* fsucc,
-                     int32_t* vsucc, uint32_t* planes, int64_t PW, hipStream_t st);
+                     int32_t* vsucc, uint32_t* planes, int64_t PW, uint64_t* pjf, uint64_t* pjv, hipStream_t st);
 // min_key / max_key: pack_key(value, cell id) per extremum
 void launch_compact_lists(bool dual, const RankView& rv, int32_t* min_ids, uint64_t* min_key, int32_t* sad_ids,
                           float* sad_val, int32_t* sad_field, int32_t* max_ids, uint64_t* max_key, hipStream_t st);

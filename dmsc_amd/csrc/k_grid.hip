@@ -12,7 +12,10 @@ __constant__ constexpr PairLut c_lut = make_pair_lut();
 // LDS with a 2-pixel halo) and scatters the pairing of each lower-star cell into an LDS cell tile;
 // every valid cell lies in exactly one lower star, so there are no write conflicts. Phase 2 writes
 // each slot's four partners as one 16-byte store, the face / vertex V-path successors, and the
-// four critical bit planes via wave ballots.
+// four critical bit planes via wave ballots. Phase 3 follows every face and vertex V-path inside
+// the tile by pointer doubling in LDS (at most 1024 steps, so 10 doublings) and writes the initial
+// pointer-jumping state: (in-tile root or first node outside the tile, steps to it). The global
+// pointer jumping then moves one tile crossing per step instead of one cell.
 constexpr int GT_X = 64, GT_Y = 16;
 constexpr int GP_W = GT_X + 3, GP_H = GT_Y + 3;
 
@@ -20,9 +23,11 @@ template <bool DUAL>
 __global__ __launch_bounds__(256) void k_gradient(Batch bt, const TileRef* __restrict__ tiles,
                                                   int32_t* __restrict__ pw_all, int32_t* __restrict__ fsucc_all,
                                                   int32_t* __restrict__ vsucc_all, uint32_t* __restrict__ planes,
-                                                  int64_t PW) {
+                                                  int64_t PW, uint64_t* __restrict__ pjf_all,
+                                                  uint64_t* __restrict__ pjv_all) {
   __shared__ float pix[GP_H][GP_W];
-  __shared__ int32_t cell[GT_Y][GT_X][4];
+  __shared__ int32_t cell[GT_Y][GT_X][4];  // reused by phase 3 as its second buffers
+  __shared__ int32_t fJ[GT_Y * GT_X], fD[GT_Y * GT_X], vJ[GT_Y * GT_X], vD[GT_Y * GT_X];
   __shared__ uint8_t lut[kLutSize];
   const TileRef tr = tiles[blockIdx.x];
   const FieldDesc fd = bt.fd[tr.field];
@@ -112,29 +117,44 @@ __global__ __launch_bounds__(256) void k_gradient(Batch bt, const TileRef* __res
     const bool k1 = in && c1 < 0 && y < H;
     const bool k2 = in && c2 < 0 && x < W;
     const bool k3 = in && c3 < 0;
+    // phase-3 state: local successor index inside the tile, -(global id) - 1 outside, self = root
+    const int e = r * GT_X + lane;
+    int fje = e, fde = 0, vje = e, vde = 0;
     if (in) {
       const int q = y * Nx + x;
       if (pw) *reinterpret_cast<int4*>(pw + 4 * (int64_t)q) = make_int4(c0, c1, c2, c3);
-      int fs = q;
+      int fs = q, dy = 0, dx = 0;
       if (c3 >= 0) {
-        if (c3 == 4 * (q - Nx) + 1) fs = q - Nx;
-        else if (c3 == 4 * q + 1) fs = q + Nx;
-        else if (c3 == 4 * (q - 1) + 2) fs = q - 1;
-        else fs = q + 1;
+        if (c3 == 4 * (q - Nx) + 1) { fs = q - Nx; dy = -1; }
+        else if (c3 == 4 * q + 1) { fs = q + Nx; dy = 1; }
+        else if (c3 == 4 * (q - 1) + 2) { fs = q - 1; dx = -1; }
+        else { fs = q + 1; dx = 1; }
+        const int rr = r + dy, ll = lane + dx;
+        fde = 1;
+        fje = (rr >= 0 && rr < GT_Y && ll >= 0 && ll < GT_X) ? rr * GT_X + ll : -(sbase + fs) - 1;
       }
       fsucc[q] = sbase + fs;
       if (y < H && x < W) {
         const int p = y * W + x;
         int vs = p;
+        dy = 0;
+        dx = 0;
         if (c0 >= 0) {
-          if (c0 == 4 * q + 1) vs = p - 1;
-          else if (c0 == 4 * (q + 1) + 1) vs = p + 1;
-          else if (c0 == 4 * q + 2) vs = p - W;
-          else vs = p + W;
+          if (c0 == 4 * q + 1) { vs = p - 1; dx = -1; }
+          else if (c0 == 4 * (q + 1) + 1) { vs = p + 1; dx = 1; }
+          else if (c0 == 4 * q + 2) { vs = p - W; dy = -1; }
+          else { vs = p + W; dy = 1; }
+          const int rr = r + dy, ll = lane + dx;
+          vde = 1;
+          vje = (rr >= 0 && rr < GT_Y && ll >= 0 && ll < GT_X) ? rr * GT_X + ll : -(pbase + vs) - 1;
         }
         vsucc[p] = pbase + vs;
       }
     }
+    fJ[e] = fje;
+    fD[e] = fde;
+    vJ[e] = vje;
+    vD[e] = vde;
     const uint64_t b0 = __ballot(k0), b1 = __ballot(k1), b2 = __ballot(k2), b3 = __ballot(k3);
     if (y < Ny && lane < 8) {
       const int t = lane >> 1, half = lane & 1;
@@ -143,16 +163,75 @@ __global__ __launch_bounds__(256) void k_gradient(Batch bt, const TileRef* __res
       if (wi < WPR) planes[t * PW + wbase + (int64_t)y * WPR + wi] = (uint32_t)(bb >> (32 * half));
     }
   }
+  __syncthreads();
+
+  // phase 3: pointer doubling inside the tile (double-buffered through the phase-1 cell tile)
+  constexpr int TN = GT_Y * GT_X;
+  int32_t* fJb = &cell[0][0][0];
+  int32_t* fDb = fJb + TN;
+  int32_t* vJb = fJb + 2 * TN;
+  int32_t* vDb = fJb + 3 * TN;
+  int32_t *fJa = fJ, *fDa = fD, *vJa = vJ, *vDa = vD;
+#pragma unroll 1
+  for (int it = 0; it < 10; ++it) {
+    int moved = 0;  // a pass in which no pointer moves leaves every chain resolved
+    for (int e = tid; e < TN; e += 256) {
+      int j = fJa[e], d = fDa[e];
+      if (j >= 0 && j != e) {
+        const int jj = fJa[j];
+        moved |= jj != j;
+        d += fDa[j];
+        j = jj;
+      }
+      fJb[e] = j;
+      fDb[e] = d;
+      j = vJa[e];
+      d = vDa[e];
+      if (j >= 0 && j != e) {
+        const int jj = vJa[j];
+        moved |= jj != j;
+        d += vDa[j];
+        j = jj;
+      }
+      vJb[e] = j;
+      vDb[e] = d;
+    }
+    const int any = __syncthreads_or(moved);
+    int32_t* t;
+    t = fJa; fJa = fJb; fJb = t;
+    t = fDa; fDa = fDb; fDb = t;
+    t = vJa; vJa = vJb; vJb = t;
+    t = vDa; vDa = vDb; vDb = t;
+    if (!any) break;
+  }
+  uint64_t* __restrict__ pjf = pjf_all + fd.slot_base;
+  uint64_t* __restrict__ pjv = pjv_all + fd.pix_base;
+  for (int e = tid; e < TN; e += 256) {
+    const int r = e / GT_X, c = e - r * GT_X;
+    const int y = y0 + r, x = x0 + c;
+    if (y >= Ny || x >= Nx) continue;
+    const int q = y * Nx + x;
+    int j = fJa[e];
+    uint32_t target = j >= 0 ? (uint32_t)(sbase + (y0 + j / GT_X) * Nx + x0 + j % GT_X) : (uint32_t)(-j - 1);
+    pjf[q] = (uint64_t)target | ((uint64_t)(uint32_t)fDa[e] << 32);
+    if (y < H && x < W) {
+      j = vJa[e];
+      target = j >= 0 ? (uint32_t)(pbase + (y0 + j / GT_X) * W + x0 + j % GT_X) : (uint32_t)(-j - 1);
+      pjv[y * W + x] = (uint64_t)target | ((uint64_t)(uint32_t)vDa[e] << 32);
+    }
+  }
 }
 
 void launch_gradient(bool dual, const Batch& bt, const TileRef* tiles, int64_t ntiles, int32_t* pw, int32_t* fsucc,
-                     int32_t* vsucc, uint32_t* planes, int64_t PW, hipStream_t st) {
+                     int32_t* vsucc, uint32_t* planes, int64_t PW, uint64_t* pjf, uint64_t* pjv, hipStream_t st) {
   KScope ks_("gradient", st);
   if (ntiles == 0) return;
   if (dual)
-    hipLaunchKernelGGL(k_gradient<true>, dim3((unsigned)ntiles), dim3(256), 0, st, bt, tiles, pw, fsucc, vsucc, planes, PW);
+    hipLaunchKernelGGL(k_gradient<true>, dim3((unsigned)ntiles), dim3(256), 0, st, bt, tiles, pw, fsucc, vsucc, planes, PW,
+                       pjf, pjv);
   else
-    hipLaunchKernelGGL(k_gradient<false>, dim3((unsigned)ntiles), dim3(256), 0, st, bt, tiles, pw, fsucc, vsucc, planes, PW);
+    hipLaunchKernelGGL(k_gradient<false>, dim3((unsigned)ntiles), dim3(256), 0, st, bt, tiles, pw, fsucc, vsucc, planes, PW,
+                       pjf, pjv);
   DMSC_LAUNCH_CHECK();
 }
 
