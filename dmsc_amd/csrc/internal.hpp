@@ -263,6 +263,17 @@ __device__ __forceinline__ int field_of(const int64_t* __restrict__ base, int B,
   return lo;
 }
 
+// same over int32 bases (list positions)
+__device__ __forceinline__ int field_of64(const int32_t* __restrict__ base, int B, int64_t idx) {
+  int lo = 0, hi = B - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (base[mid] <= idx) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
 // Critical-cell rank structure over the batch: bit planes (type t at planes + t*PW, PW = total
 // words) plus exclusive popcount prefixes per word for minima (type 0), saddles (types 1+2) and
 // maxima (type 3). Ranks are global list indices (fields' lists are concatenated in field order).
@@ -295,9 +306,11 @@ void bases_at(const T* pre, const S* starts, int64_t n, const T* total, int B, T
 // also writes the initial pointer-jumping state of both V-path forests (pjf over slots, pjv over pixels)
 void launch_gradient(bool dual, const Batch& bt, const TileRef* tiles, int64_t ntiles, int32_t* pw, int32_t* fsucc,
                      int32_t* vsucc, uint32_t* planes, int64_t PW, uint64_t* pjf, uint64_t* pjv, hipStream_t st);
-// min_key / max_key: pack_key(value, cell id) per extremum
-void launch_compact_lists(bool dual, const RankView& rv, int32_t* min_ids, uint64_t* min_key, int32_t* sad_ids,
-                          float* sad_val, int32_t* sad_field, int32_t* max_ids, uint64_t* max_key, hipStream_t st);
+// min_key / max_key: pack_key(value, cell id) per extremum; list_b: rows of B+1 per-field list bases
+// (minima, saddles, maxima)
+void launch_compact_lists(bool dual, const RankView& rv, const int32_t* list_b, int32_t* min_ids, uint64_t* min_key,
+                          int32_t* sad_ids, float* sad_val, int32_t* sad_field, int32_t* max_ids, uint64_t* max_key,
+                          hipStream_t st);
 void launch_saddle_records(const RankView& rv, const int32_t* sad_ids, const int32_t* sad_field, int64_t nsad,
                            const uint64_t* pj_face, const uint64_t* pj_vert, int32_t* cmax, int32_t* cmin,
                            int32_t* lmax, int32_t* lmin, hipStream_t st);

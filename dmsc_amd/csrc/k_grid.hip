@@ -259,73 +259,105 @@ __device__ __forceinline__ int rank_min_pixel(const RankView& rv, int f, int p) 
 }
 
 // ------------------------------------------------------------------ critical lists
-// One thread per plane word; set bits are emitted in ascending cell id (the reference's single-
-// thread order, critical_points.hxx:14-65), with the extremum / saddle value. Ids are field-local;
-// lists are concatenated in field order.
+// Set bits in id order (the reference's single-thread order, critical_points.hxx:14-65), with the
+// extremum key / saddle value. A wave takes 64 consecutive plane words: each lane counts its word's
+// entries, a wave prefix sum places them, the lanes write the entries' cell ids into an LDS stage,
+// and the wave then writes the chunk's part of each list with consecutive lanes on consecutive
+// entries (one lane per word would scatter 4-byte stores over 64 cache lines). An entry's field
+// comes from its list position (per-field list bases), its value from its cell id.
+constexpr int kCompactChunk = 64;  // words per wave
+constexpr int kCompactWaves = 2;   // waves per block (the stage holds 2 saddles per slot: 16 KB a wave)
+
 template <bool DUAL>
-__global__ void k_compact_lists(RankView rv, int32_t* __restrict__ min_ids, uint64_t* __restrict__ min_key,
-                                int32_t* __restrict__ sad_ids, float* __restrict__ sad_val,
-                                int32_t* __restrict__ sad_field, int32_t* __restrict__ max_ids,
-                                uint64_t* __restrict__ max_key) {
-  for (int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; w < rv.PW; w += (int64_t)gridDim.x * blockDim.x) {
-    const int fi = field_of(rv.bt.word_b, rv.bt.B, w);
-    const FieldDesc& fd = rv.bt.fd[fi];
-    const float* __restrict__ f = fd.f;
-    const int H = fd.H, W = fd.W, Nx = fd.Nx;
-    const int64_t lw = w - fd.word_base;
-    const int y = (int)(lw / fd.WPR), xb = (int)(lw - (int64_t)y * fd.WPR) * 32;
-    uint32_t m0 = rv.planes[w], m1 = rv.planes[rv.PW + w], m2 = rv.planes[2 * rv.PW + w], m3 = rv.planes[3 * rv.PW + w];
-    int o = rv.pre_min[w];
-    while (m0) {
-      int b = __builtin_ctz(m0);
-      m0 &= m0 - 1;
-      int x = xb + b;
-      min_ids[o] = 4 * (y * Nx + x);
-      min_key[o] = pack_key(f[(int64_t)y * W + x], 4 * (y * Nx + x));
-      ++o;
+__global__ __launch_bounds__(64 * kCompactWaves) void k_compact_lists(RankView rv, const int32_t* __restrict__ list_b,
+                                                       int32_t* __restrict__ min_ids, uint64_t* __restrict__ min_key,
+                                                       int32_t* __restrict__ sad_ids, float* __restrict__ sad_val,
+                                                       int32_t* __restrict__ sad_field, int32_t* __restrict__ max_ids,
+                                                       uint64_t* __restrict__ max_key) {
+  __shared__ int32_t stage[kCompactWaves][kCompactChunk * 32 * 2];  // up to 2 saddles per slot
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int B = rv.bt.B;
+  const int64_t nchunks = (rv.PW + kCompactChunk - 1) / kCompactChunk;
+  // block-uniform loop: every wave runs the same iterations (the block synchronises in between)
+  for (int64_t cb = (int64_t)blockIdx.x * kCompactWaves; cb < nchunks; cb += (int64_t)gridDim.x * kCompactWaves) {
+    const int64_t chunk = cb + wid;
+    const int64_t w = chunk * kCompactChunk + lane;
+    const bool valid = chunk < nchunks && w < rv.PW;
+    uint32_t m[4] = {0, 0, 0, 0};
+    int Nx = 0, xb = 0, yrow = 0;
+    if (valid) {
+      const int fi = field_of(rv.bt.word_b, B, w);
+      const FieldDesc& fd = rv.bt.fd[fi];
+      const int64_t lw = w - fd.word_base;
+      yrow = (int)(lw / fd.WPR);
+      xb = (int)(lw - (int64_t)yrow * fd.WPR) * 32;
+      Nx = fd.Nx;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) m[t] = rv.planes[t * rv.PW + w];
     }
-    o = rv.pre_max[w];
-    while (m3) {
-      int b = __builtin_ctz(m3);
-      m3 &= m3 - 1;
-      int x = xb + b;
-      max_ids[o] = 4 * (y * Nx + x) + 3;
-      max_key[o] = pack_key(f[top_pixel<DUAL>(3, y, x, H, W, f)], 4 * (y * Nx + x) + 3);
-      ++o;
-    }
-    o = rv.pre_sad[w];
-    uint32_t m12 = m1 | m2;
-    while (m12) {
-      int b = __builtin_ctz(m12);
-      m12 &= m12 - 1;
-      int x = xb + b;
-      if (m1 >> b & 1) {
-        sad_ids[o] = 4 * (y * Nx + x) + 1;
-        sad_val[o] = f[top_pixel<DUAL>(1, y, x, H, W, f)];
-        if (sad_field) sad_field[o] = fi;
-        ++o;
+    const int64_t w0 = chunk * kCompactChunk;
+#pragma unroll
+    for (int list = 0; list < 3; ++list) {
+      // entries of this lane's word, in id order
+      const uint32_t ma = list == 0 ? m[0] : list == 1 ? m[1] : m[3];
+      const uint32_t mb = list == 1 ? m[2] : 0u;
+      const int cnt = __popc(ma) + __popc(mb);
+      const int incl = wave_inclusive_sum(cnt);
+      const int total = __shfl(incl, 63, 64);
+      int o = incl - cnt;
+      uint32_t mm = ma | mb;
+      while (mm) {
+        const int bit = __builtin_ctz(mm);
+        mm &= mm - 1;
+        const int q = yrow * Nx + xb + bit;
+        if (list == 0) stage[wid][o++] = 4 * q;
+        else if (list == 2) stage[wid][o++] = 4 * q + 3;
+        else {
+          if (ma >> bit & 1) stage[wid][o++] = 4 * q + 1;
+          if (mb >> bit & 1) stage[wid][o++] = 4 * q + 2;
+        }
       }
-      if (m2 >> b & 1) {
-        sad_ids[o] = 4 * (y * Nx + x) + 2;
-        sad_val[o] = f[top_pixel<DUAL>(2, y, x, H, W, f)];
-        if (sad_field) sad_field[o] = fi;
-        ++o;
+      __syncthreads();
+      if (chunk < nchunks && total > 0) {
+        const int32_t* pre = list == 0 ? rv.pre_min : list == 1 ? rv.pre_sad : rv.pre_max;
+        const int32_t* lb = list_b + list * (B + 1);
+        const int64_t base = pre[w0];
+        for (int t = lane; t < total; t += 64) {
+          const int id = stage[wid][t];
+          const int64_t oi = base + t;
+          const int fi = field_of64(lb, B, oi);
+          const FieldDesc& fd = rv.bt.fd[fi];
+          const int q = id >> 2, ty = q / fd.Nx, tx = q - ty * fd.Nx;
+          if (list == 0) {
+            min_ids[oi] = id;
+            min_key[oi] = pack_key(fd.f[(int64_t)ty * fd.W + tx], id);
+          } else if (list == 2) {
+            max_ids[oi] = id;
+            max_key[oi] = pack_key(fd.f[top_pixel<DUAL>(3, ty, tx, fd.H, fd.W, fd.f)], id);
+          } else {
+            sad_ids[oi] = id;
+            sad_val[oi] = fd.f[top_pixel<DUAL>(id & 3, ty, tx, fd.H, fd.W, fd.f)];
+            if (sad_field) sad_field[oi] = fi;
+          }
+        }
       }
+      __syncthreads();
     }
   }
 }
 
-void launch_compact_lists(bool dual, const RankView& rv, int32_t* min_ids, uint64_t* min_key, int32_t* sad_ids,
-                          float* sad_val, int32_t* sad_field, int32_t* max_ids, uint64_t* max_key, hipStream_t st) {
+void launch_compact_lists(bool dual, const RankView& rv, const int32_t* list_b, int32_t* min_ids, uint64_t* min_key,
+                          int32_t* sad_ids, float* sad_val, int32_t* sad_field, int32_t* max_ids, uint64_t* max_key,
+                          hipStream_t st) {
   KScope ks_("compact_lists", st);
   if (rv.PW == 0) return;
-  int g = grid_for(rv.PW, 256, 8192);
+  const int g = grid_for((rv.PW + kCompactChunk - 1) / kCompactChunk, kCompactWaves, 16384);
   if (dual)
-    hipLaunchKernelGGL(k_compact_lists<true>, dim3(g), dim3(256), 0, st, rv, min_ids, min_key, sad_ids, sad_val, sad_field,
-                       max_ids, max_key);
-  else
-    hipLaunchKernelGGL(k_compact_lists<false>, dim3(g), dim3(256), 0, st, rv, min_ids, min_key, sad_ids, sad_val,
+    hipLaunchKernelGGL(k_compact_lists<true>, dim3(g), dim3(64 * kCompactWaves), 0, st, rv, list_b, min_ids, min_key, sad_ids, sad_val,
                        sad_field, max_ids, max_key);
+  else
+    hipLaunchKernelGGL(k_compact_lists<false>, dim3(g), dim3(64 * kCompactWaves), 0, st, rv, list_b, min_ids, min_key, sad_ids,
+                       sad_val, sad_field, max_ids, max_key);
   DMSC_LAUNCH_CHECK();
 }
 
