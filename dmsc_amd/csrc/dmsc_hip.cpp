@@ -33,11 +33,11 @@ enum : int {
   A_FSUCC, A_VSUCC, A_PJF, A_PJV, A_PLANES, A_PREMIN, A_PRESAD, A_PREMAX,
   A_C32, A_C64,
   A_MINIDS, A_MINVAL, A_SADIDS, A_SADVAL, A_MAXIDS, A_MAXVAL,
-  A_CMAX, A_CMIN, A_LMAX, A_LMIN, A_META, A_BASES32, A_SADFIELD, A_CONTRIB,
+  A_CMAX, A_CMIN, A_LMAX, A_LMIN, A_META, A_BASES32, A_SADFIELD, A_CONTRIB, A_SURV1, A_SURVR1,
   A_KEYS, A_KEYS2, A_VALS2, A_ASC, A_ORDMIN, A_ORDMAX, A_REMOVED, A_POSMIN, A_POSMAX,
   A_SIDE0,  // per persistence side: 13 slots each
   A_SIDE_END = A_SIDE0 + 2 * 13,
-  A_ASM0 = A_SIDE_END,  // assembly: surv, surv_pre, rows x2, rows_pre x2, sad_to_out, to_out x2
+  A_ASM0 = A_SIDE_END,  // assembly: counts, prefixes, sad_to_out, survivor lists (12 slots)
   A_ASM_END = A_ASM0 + 12,
   A_ARC0 = A_ASM_END,  // per arc side: 25 slots each
   A_ARC_END = A_ARC0 + 2 * 25,
@@ -547,16 +547,23 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     bases_at<int32_t, int32_t>(sb[s].to_out, row32(s ? 2 : 0), next, c32 + 3 + s, B, row32(3 + s), st);
     launch_mask_prefix(sb[s].alive, sb[s].to_out, next, st);
   }
+  int32_t* surv_mx = ar.get<int32_t>(A_ASM0 + 8, nsad);
+  int32_t* surv_mx_pre = ar.get<int32_t>(A_ASM0 + 9, nsad);
   launch_side_counts(nsad, ord_min, removed, cmin, sb[0].root, sb[0].to_out, surv, rows[0], st);
-  launch_side_counts(nsad, ord_max, removed, cmax, sb[1].root, sb[1].to_out, nullptr, rows[1], st);
+  launch_side_counts(nsad, ord_max, removed, cmax, sb[1].root, sb[1].to_out, surv_mx, rows[1], st);
   scan_exclusive<int32_t>(surv, surv_pre, nsad, c32 + 5, ar, A_SCAN, st);
+  scan_exclusive<int32_t>(surv_mx, surv_mx_pre, nsad, c32 + 10, ar, A_SCAN, st);
   scan_exclusive<int32_t>(rows[0], rows_pre[0], nsad, c32 + 6, ar, A_SCAN, st);
   scan_exclusive<int32_t>(rows[1], rows_pre[1], nsad, c32 + 7, ar, A_SCAN, st);
   // orders are grouped by field, so field f owns positions [sad_b[f], sad_b[f+1]) of each order
   bases_at<int32_t, int32_t>(surv_pre, sad_b, nsad, c32 + 5, B, row32(5), st);
   bases_at<int32_t, int32_t>(rows_pre[0], sad_b, nsad, c32 + 6, B, row32(6), st);
   bases_at<int32_t, int32_t>(rows_pre[1], sad_b, nsad, c32 + 7, B, row32(7), st);
-  launch_sad_to_out(nsad, ord_min, removed, surv_pre, sad_to_out, st);
+  // survivor lists: min order (position = output saddle index) and max order, with their edge rows
+  int32_t* surv_list[2] = {ar.get<int32_t>(A_ASM0 + 10, nsad), ar.get<int32_t>(A_SURV1, nsad)};
+  int32_t* surv_rows[2] = {ar.get<int32_t>(A_ASM0 + 11, nsad), ar.get<int32_t>(A_SURVR1, nsad)};
+  launch_survivors(nsad, ord_min, surv, surv_pre, rows_pre[0], surv_list[0], surv_rows[0], sad_to_out, st);
+  launch_survivors(nsad, ord_max, surv_mx, surv_mx_pre, rows_pre[1], surv_list[1], surv_rows[1], nullptr, st);
 
   // ---- arc geometry, per traced side: raw arcs and cancellation lists
   ArcWork aw[2];  // 0 = vertices (min arcs), 1 = faces (max arcs)
@@ -596,6 +603,7 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
   std::vector<int32_t> hb(8 * (size_t)(B + 1));
   std::memcpy(hb.data(), (char*)ctx->pinned + 128, sizeof(int32_t) * hb.size());
   auto hrow = [&](int r, int f) -> int64_t { return hb[(size_t)r * (B + 1) + f]; };
+  const int64_t n_sad_total = h32[5];  // surviving saddles of all fields
 
   for (int s = 0; s < 2; ++s) {
     if (!aw[s].on) continue;
@@ -718,10 +726,11 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     w.flen = a2.get<int64_t>(hb2 + H_FLEN, 2 * nsad);
     int64_t* per_pos = a2.get<int64_t>(hb2 + H_PERPOS, nsad);
     w.pos_pre = a2.get<int64_t>(hb2 + H_POSPRE, nsad);
-    launch_final_lengths(nsad, ord_min, removed, w.A, w.pjq, next, w.flen, per_pos, st);
-    scan_exclusive<int64_t>(per_pos, w.pos_pre, nsad, c64 + 2 * s + 1, ar, A_SCAN, st);
+    // over the surviving saddles in output order (position = output saddle index)
+    launch_final_lengths(n_sad_total, surv_list[0], nullptr, w.A, w.pjq, next, w.flen, per_pos, st);
+    scan_exclusive<int64_t>(per_pos, w.pos_pre, n_sad_total, c64 + 2 * s + 1, ar, A_SCAN, st);
     w.arc_base = a2.get<int64_t>(hb2 + H_ARCB, B + 1);
-    bases_at<int64_t, int32_t>(w.pos_pre, sad_b, nsad, c64 + 2 * s + 1, B, w.arc_base, st);
+    bases_at<int64_t, int32_t>(w.pos_pre, row32(5), n_sad_total, c64 + 2 * s + 1, B, w.arc_base, st);
   }
   if (aw[0].on || aw[1].on) {
     for (int s = 0; s < 2; ++s)
@@ -773,10 +782,10 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
   launch_emit_points(nmin, min_ids, sb[0].to_out, o_pts[0], st);
   launch_emit_points(nmax, max_ids, sb[1].to_out, o_pts[1], st);
   const EmitBases eb[2] = {{sad_field, row32(5), row32(3)}, {sad_field, row32(5), row32(4)}};
-  launch_emit_side(nsad, ord_min, removed, sad_to_out, cmin, sb[0].root, sb[0].to_out, sb[0].pers, sb[0].pair,
-                   rows_pre[0], sad_ids, eb[0], o_p[0], o_pp[0], o_e[0], o_sad, st);
-  launch_emit_side(nsad, ord_max, removed, sad_to_out, cmax, sb[1].root, sb[1].to_out, sb[1].pers, sb[1].pair,
-                   rows_pre[1], sad_ids, eb[1], o_p[1], o_pp[1], o_e[1], nullptr, st);
+  launch_emit_side(n_sad_total, surv_list[0], nullptr, sad_to_out, cmin, sb[0].root, sb[0].to_out, sb[0].pers,
+                   sb[0].pair, surv_rows[0], sad_ids, eb[0], o_p[0], o_pp[0], o_e[0], o_sad, st);
+  launch_emit_side(n_sad_total, surv_list[1], nullptr, sad_to_out, cmax, sb[1].root, sb[1].to_out, sb[1].pers,
+                   sb[1].pair, surv_rows[1], sad_ids, eb[1], o_p[1], o_pp[1], o_e[1], nullptr, st);
   if (gmax) launch_labels_faces(rv, S, pjf, sb[1].root, sb[1].to_out, row32(4), o_flab, st);
   if (gmin) launch_labels_verts(rv, N, pjv, sb[0].root, sb[0].to_out, row32(3), o_vlab, st);
 
@@ -794,7 +803,7 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     // nseg[L%3], appends to nseg[(L+1)%3] and clears nseg[(L+2)%3]); nseg[3] counts raw pieces
     int32_t* nseg = a2.get<int32_t>(hb2 + H_NSEG, 4);
     fill_i32(nseg, 0, 4, st);
-    launch_seed_segments(nsad, ord_min, removed, w.A, w.pjq, next, w.flen, w.pos_pre, sad_to_out, sa, nseg, raw,
+    launch_seed_segments(n_sad_total, surv_list[0], nullptr, w.A, w.pjq, next, w.flen, w.pos_pre, sad_to_out, sa, nseg, raw,
                          nseg + 3, cap, d_err, w.out_off, w.out_arc_off, w.arc_base, row32(5), B, st);
     readback(ctx, nseg, 4 * sizeof(int32_t), st);
     int32_t n_in = ((const int32_t*)ctx->pinned)[0];

@@ -480,7 +480,7 @@ __global__ void k_final_lengths(int64_t ns, const int32_t* __restrict__ ord, con
   for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < ns; k += (int64_t)gridDim.x * blockDim.x) {
     const int i = ord[k];
     int64_t l[2] = {0, 0};
-    if (!removed[i]) {
+    if (!removed || !removed[i]) {
       for (int kk = 0; kk < 2; ++kk) {
         const int a = 2 * i + kk, T = A.tinit[a];
         if (T >= 0) l[kk] = (int64_t)A.blen[a] + qlen(pjq, T);
@@ -520,7 +520,7 @@ __global__ __launch_bounds__(256) void k_seed_segments(int64_t ns, const int32_t
     Seg rs[2], cs[2];
     if (kk0 < ns) {
       const int i = ord[kk0];
-      if (!removed[i]) {
+      if (!removed || !removed[i]) {
         const int o = sad_to_out[i];
         const int fi = A.sfield ? A.sfield[i] : 0;
         const int64_t base = pos_pre[kk0];

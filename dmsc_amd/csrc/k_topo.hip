@@ -323,18 +323,28 @@ void launch_side_counts(int64_t ns, const int32_t* ord, const uint8_t* removed, 
 }
 
 // sad_to_out[i] for surviving saddles, from the survivor prefix over the min order
-__global__ void k_sad_to_out(int64_t ns, const int32_t* __restrict__ ord, const uint8_t* __restrict__ removed,
-                             const int32_t* __restrict__ surv_pre, int32_t* __restrict__ sad_to_out) {
+// Surviving saddles of one side's order as a list: list[r] = saddle, row_list[r] = its first edge row,
+// r = its rank among the survivors in that order (for the min order r is the output saddle index).
+// sad_to_out (min order only): saddle -> output index (-1 when removed).
+__global__ void k_survivors(int64_t ns, const int32_t* __restrict__ ord, const int32_t* __restrict__ surv,
+                            const int32_t* __restrict__ surv_pre, const int32_t* __restrict__ row_pre,
+                            int32_t* __restrict__ list, int32_t* __restrict__ row_list,
+                            int32_t* __restrict__ sad_to_out) {
   for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < ns; k += (int64_t)gridDim.x * blockDim.x) {
     const int i = ord[k];
-    sad_to_out[i] = removed[i] ? -1 : surv_pre[k];
+    const int r = surv[k] ? surv_pre[k] : -1;
+    if (sad_to_out) sad_to_out[i] = r;
+    if (r >= 0) {
+      list[r] = i;
+      row_list[r] = row_pre[k];
+    }
   }
 }
-void launch_sad_to_out(int64_t ns, const int32_t* ord, const uint8_t* removed, const int32_t* surv_pre,
-                       int32_t* sad_to_out, hipStream_t st) {
+void launch_survivors(int64_t ns, const int32_t* ord, const int32_t* surv, const int32_t* surv_pre,
+                      const int32_t* row_pre, int32_t* list, int32_t* row_list, int32_t* sad_to_out, hipStream_t st) {
   if (ns == 0) return;
-  hipLaunchKernelGGL(k_sad_to_out, dim3(grid_for(ns, 256, 16384)), dim3(256), 0, st, ns, ord, removed, surv_pre,
-                     sad_to_out);
+  hipLaunchKernelGGL(k_survivors, dim3(grid_for(ns, 256, 16384)), dim3(256), 0, st, ns, ord, surv, surv_pre, row_pre,
+                     list, row_list, sad_to_out);
   DMSC_LAUNCH_CHECK();
 }
 
@@ -363,7 +373,7 @@ __global__ void k_emit_side(int64_t ns, const int32_t* __restrict__ ord, const u
                             int32_t* __restrict__ out_sad) {
   for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < ns; k += (int64_t)gridDim.x * blockDim.x) {
     const int i = ord[k];
-    if (removed[i]) continue;
+    if (removed && removed[i]) continue;
     int sbase = 0, ebase = 0;
     if (eb.sfield) {
       const int fi = eb.sfield[i];

@@ -42,10 +42,11 @@ void launch_u8_to_i32(const uint8_t* a, int32_t* o, int64_t n, bool invert, hipS
 void launch_mask_prefix(const uint8_t* alive, int32_t* pre, int64_t n, hipStream_t st);
 void launch_side_counts(int64_t ns, const int32_t* ord, const uint8_t* removed, const int32_t* c, const int32_t* root_t,
                         const int32_t* to_out, int32_t* surv, int32_t* rows, hipStream_t st);
-void launch_sad_to_out(int64_t ns, const int32_t* ord, const uint8_t* removed, const int32_t* surv_pre,
-                       int32_t* sad_to_out, hipStream_t st);
+void launch_survivors(int64_t ns, const int32_t* ord, const int32_t* surv, const int32_t* surv_pre,
+                      const int32_t* row_pre, int32_t* list, int32_t* row_list, int32_t* sad_to_out, hipStream_t st);
 void launch_emit_points(int64_t n, const int32_t* ids, const int32_t* to_out, int32_t* out, hipStream_t st);
-// Output indices are batch-wide positions; index values written into the outputs are made
+// ord / removed: a side order over all saddles with their removal flags, or a survivor list with
+// removed == null (row_pre indexed like ord). Output indices are batch-wide positions; index values written into the outputs are made
 // field-local with the per-field bases (sfield null: a single field, bases unused).
 struct EmitBases {
   const int32_t* sfield;    // per saddle field
