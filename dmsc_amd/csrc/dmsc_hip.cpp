@@ -35,15 +35,17 @@ enum : int {
   A_MINIDS, A_MINVAL, A_SADIDS, A_SADVAL, A_MAXIDS, A_MAXVAL,
   A_CMAX, A_CMIN, A_LMAX, A_LMIN, A_META, A_BASES32, A_SADFIELD, A_CONTRIB, A_SURV1, A_SURVR1,
   A_KEYS, A_KEYS2, A_VALS2, A_ASC, A_ORDMIN, A_ORDMAX, A_REMOVED, A_POSMIN, A_POSMAX,
-  A_SIDE0,  // per persistence side: 13 slots each
-  A_SIDE_END = A_SIDE0 + 2 * 13,
+  A_SIDE0,  // per persistence side: kSideSlots slots each
+  A_SIDE_END = A_SIDE0 + 2 * 14,
   A_ASM0 = A_SIDE_END,  // assembly: counts, prefixes, sad_to_out, survivor lists (12 slots)
   A_ASM_END = A_ASM0 + 12,
   A_ARC0 = A_ASM_END,  // per arc side: 25 slots each
   A_ARC_END = A_ARC0 + 2 * 25,
 };
 
-enum SideSlot { P_PARG, P_PART, P_ALIVE, P_MINKEY, P_PAIR, P_PERS, P_CANCEL, P_RR, P_WL0, P_WL1, P_CNT, P_ROOT, P_TOOUT };
+enum SideSlot { P_PARG, P_PART, P_ALIVE, P_MINKEY, P_PAIR, P_PERS, P_CANCEL, P_RR, P_WL0, P_WL1, P_CNT, P_ROOT, P_TOOUT,
+                P_CROOT, kSideSlots };
+static_assert(A_SIDE_END - A_SIDE0 == 2 * kSideSlots, "side slot count");
 enum ArcSlot {
   G_BLEN, G_BOFF, G_ARENA, G_FLAG, G_PRE, G_CS0, G_CD0, G_KEYS, G_SLOT, G_KEYS2, G_SLOT2, G_CS, G_CD, G_DEADC,
   G_GPAR, G_LLEN, G_LC, G_LKD, G_DONE, G_WATCH, G_RR, G_WL0, G_WL1, G_CNT, G_JUMP
@@ -115,14 +117,14 @@ void readback(dmsc_context* c, const void* dev, size_t bytes, hipStream_t st) {
 }
 
 struct SideBufs {
-  int32_t *par_g, *par_t, *pair, *cancel, *rr, *wl0, *wl1, *cnt, *root, *to_out;
+  int32_t *par_g, *par_t, *pair, *cancel, *rr, *wl0, *wl1, *cnt, *root, *to_out, *croot;
   uint8_t* alive;
   uint64_t* minkey;
   float* pers;
 };
 
 SideBufs side_bufs(Arena& ar, int side, int64_t next, int64_t ns) {
-  const int b = A_SIDE0 + 13 * side;
+  const int b = A_SIDE0 + kSideSlots * side;
   SideBufs s;
   s.par_g = ar.get<int32_t>(b + P_PARG, next);
   s.par_t = ar.get<int32_t>(b + P_PART, next);
@@ -137,6 +139,7 @@ SideBufs side_bufs(Arena& ar, int side, int64_t next, int64_t ns) {
   s.cnt = ar.get<int32_t>(b + P_CNT, 2);
   s.root = ar.get<int32_t>(b + P_ROOT, next);
   s.to_out = ar.get<int32_t>(b + P_TOOUT, next);
+  s.croot = ar.get<int32_t>(b + P_CROOT, 2 * ns);
   return s;
 }
 
@@ -474,6 +477,7 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     p.pers = sb[s].pers;
     p.cancel_d = record[s] ? sb[s].cancel : nullptr;
     p.rr = sb[s].rr;
+    p.croot = sb[s].croot;
     p.wl[0] = sb[s].wl0;
     p.wl[1] = sb[s].wl1;
     p.cnt = sb[s].cnt;

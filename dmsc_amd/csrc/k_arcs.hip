@@ -192,15 +192,15 @@ __device__ __forceinline__ void geo_root(const GeoState& g, int x, int& root, in
 // worklist item: (cancellation, key, arc target 0, arc target 1); results go to rr at the item's
 // list position
 __device__ __forceinline__ void geo_find_one(const ArcSide& A, const GeoState& g, const int4 it, int64_t j,
-                                             uint64_t tag) {
+                                             uint64_t tag, bool check) {
   int R0, R1;
   int32_t L0, L1;
   geo_root(g, it.z, R0, L0);
   geo_root(g, it.w, R1, L1);
   *reinterpret_cast<int4*>(g.rr + 4 * j) = make_int4(R0, R1, L0, L1);
   const uint64_t key = tag | (uint32_t)it.y;
-  if (R0 >= 0) atomicMin((unsigned long long*)&g.watch[R0], (unsigned long long)key);
-  if (R1 >= 0) atomicMin((unsigned long long*)&g.watch[R1], (unsigned long long)key);
+  if (R0 >= 0) post_min(&g.watch[R0], key, check);
+  if (R1 >= 0) post_min(&g.watch[R1], key, check);
 }
 
 __device__ __forceinline__ void geo_find_phase(const ArcSide& A, const GeoState& g, int round) {
@@ -209,7 +209,7 @@ __device__ __forceinline__ void geo_find_phase(const ArcSide& A, const GeoState&
   if (blockIdx.x == 0 && threadIdx.x == 0) g.cnt[cur ^ 1] = 0;
   const uint64_t tag = (uint64_t)(0xFFFFFFFFu - (uint32_t)round) << 32;
   for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x)
-    geo_find_one(A, g, g.wl[cur][j], j, tag);
+    geo_find_one(A, g, g.wl[cur][j], j, tag, round > 0);
 }
 
 // Both traced sides of one round in one launch: side 0's items first, then side 1's.
@@ -225,8 +225,8 @@ __global__ void k_geo_find2(ArcSide A0, GeoState G0, ArcSide A1, GeoState G1, in
   const uint64_t tag = (uint64_t)(0xFFFFFFFFu - (uint32_t)round) << 32;
   for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < (int64_t)n0 + n1;
        j += (int64_t)gridDim.x * blockDim.x) {
-    if (j >= n0) geo_find_one(A1, G1, G1.wl[cur][j - n0], j - n0, tag);
-    else geo_find_one(A0, G0, G0.wl[cur][j], j, tag);
+    if (j >= n0) geo_find_one(A1, G1, G1.wl[cur][j - n0], j - n0, tag, round > 0);
+    else geo_find_one(A0, G0, G0.wl[cur][j], j, tag, round > 0);
   }
 }
 __global__ void k_geo_find(ArcSide A, GeoState g, int round) { geo_find_phase(A, g, round); }

@@ -41,9 +41,12 @@ __device__ __forceinline__ int uf_find(int32_t* __restrict__ p, int x, int32_t* 
 }
 
 // rr[2i] = dying root (-2: cycle); rr[2i+1] = (surviving root + 1) << 1 | (dying root came from c[2i+1])
+// Round 0 starts from the saddle's extrema, later rounds from the roots it saw when it last waited
+// (the same components, fewer links to walk); from round 1 on posts read before their atomic.
 template <bool DUAL>
-__device__ __forceinline__ void persist_find_one(const PersistSide& s, const int i, int64_t j, uint64_t tag) {
-  const int2 cc = *reinterpret_cast<const int2*>(s.c + 2 * i);
+__device__ __forceinline__ void persist_find_one(const PersistSide& s, const int i, int64_t j, uint64_t tag,
+                                                 bool first) {
+  const int2 cc = *reinterpret_cast<const int2*>((first ? s.c : s.croot) + 2 * i);
   const int ra = uf_find(s.par_g, cc.x, s.err);
   const int rb = uf_find(s.par_g, cc.y, s.err);
   if (ra == rb) {
@@ -67,8 +70,8 @@ __device__ __forceinline__ void persist_find_one(const PersistSide& s, const int
   const int dying = b_dies ? rb : ra, surv = b_dies ? ra : rb;
   *reinterpret_cast<int2*>(s.rr + 2 * j) = make_int2(dying, ((surv + 1) << 1) | (b_dies ? 1 : 0));
   const uint64_t key = tag | (uint32_t)s.pos[i];
-  atomicMin((unsigned long long*)&s.minkey[dying], (unsigned long long)key);
-  if (surv >= 0) atomicMin((unsigned long long*)&s.minkey[surv], (unsigned long long)key);
+  post_min(&s.minkey[dying], key, !first);
+  if (surv >= 0) post_min(&s.minkey[surv], key, !first);
 }
 
 template <bool DUAL>
@@ -83,8 +86,8 @@ __device__ __forceinline__ void persist_find_phase(const PersistSide& s0, const 
   for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n0 + n1; j += (int64_t)gridDim.x * blockDim.x) {
     // worklists hold saddle indices in saddle-id order (the sweep position is the merge key); the
     // round's decisions go to rr at the item's list position
-    if (j >= n0) persist_find_one<DUAL>(s1, s1.wl[cur][j - n0], j - n0, tag);
-    else persist_find_one<DUAL>(s0, s0.wl[cur][j], j, tag);
+    if (j >= n0) persist_find_one<DUAL>(s1, s1.wl[cur][j - n0], j - n0, tag, round == 0);
+    else persist_find_one<DUAL>(s0, s0.wl[cur][j], j, tag, round == 0);
   }
 }
 template <bool DUAL>
@@ -100,10 +103,13 @@ __device__ __forceinline__ bool persist_apply_one(const PersistSide& s, const in
   const int2 r = *reinterpret_cast<const int2*>(s.rr + 2 * j);
   const int dying = r.x;
   if (dying == -2) return false;
-  if (s.minkey[dying] != (tag | (uint32_t)s.pos[i])) return true;
   const int code = r.y;
   const int surv = (code >> 1) - 1;
   const bool b_dies = code & 1;
+  if (s.minkey[dying] != (tag | (uint32_t)s.pos[i])) {
+    *reinterpret_cast<int2*>(s.croot + 2 * i) = b_dies ? make_int2(surv, dying) : make_int2(dying, surv);
+    return true;
+  }
   const float sv = s.sval[i], dv = key_value(s.ekey[dying]);
   float p;
   if (s.maxside) p = DUAL ? sv - dv : dv - sv;

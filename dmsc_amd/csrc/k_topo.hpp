@@ -19,6 +19,7 @@ struct PersistSide {
   float* pers;          // per saddle persistence (-1 for a cycle)
   int32_t* cancel_d;    // per saddle thresholded dead extremum (null when arcs are not traced)
   int32_t* rr;          // per worklist position: dying root (-2 = cycle), (surviving root + 1) << 1 | side bit
+  int32_t* croot;       // per saddle: its two component roots when it last had to wait (finds resume there)
   int32_t* wl[2];       // ping-pong worklists of pending saddle indices
   int32_t* cnt;         // [2] worklist sizes (device)
   int32_t* err;         // device error flags
