@@ -496,7 +496,10 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     while (true) {
       if (coop && !launch_persist_rounds(dual, ps[0], ps[1], round, kCoopRounds, removed, c32 + 48, pend, st))
         coop = false;  // no cooperative launch on this device or at this size: one launch per phase
-      if (coop) {
+      // few pending saddles: one workgroup runs every remaining round (at most one round per item)
+      const bool tail = !coop && ctx->tune.tail_items > 0 && pend <= ctx->tune.tail_items;
+      if (tail) launch_persist_tail(dual, ps[0], ps[1], round, (int)pend + 2, removed, c32 + 48, st);
+      if (coop || tail) {
         DMSC_CHECK(hipMemcpyAsync(ctx->pinned, c32 + 48, 4, hipMemcpyDeviceToHost, st));
         DMSC_CHECK(hipMemcpyAsync((char*)ctx->pinned + 8, sb[0].cnt, 8, hipMemcpyDeviceToHost, st));
         DMSC_CHECK(hipMemcpyAsync((char*)ctx->pinned + 16, sb[1].cnt, 8, hipMemcpyDeviceToHost, st));
@@ -671,7 +674,10 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
       const int mask = (live[0] ? 1 : 0) | (live[1] ? 2 : 0);
       if (gcoop && !launch_geo_rounds(as, gs, mask, r, kCoopRounds, c32 + 49, std::max(pend[0], pend[1]), st))
         gcoop = false;
-      if (gcoop) {
+      const int64_t gpend = (live[0] ? pend[0] : 0) + (live[1] ? pend[1] : 0);
+      const bool gtail = !gcoop && ctx->tune.tail_items > 0 && gpend <= ctx->tune.tail_items;
+      if (gtail) launch_geo_tail(as, gs, mask, r, (int)gpend + 64, c32 + 49, st);
+      if (gcoop || gtail) {
         DMSC_CHECK(hipMemcpyAsync(ctx->pinned, c32 + 49, 4, hipMemcpyDeviceToHost, st));
         for (int s = 0; s < 2; ++s)
           if (live[s])
@@ -998,6 +1004,7 @@ int* param_slot(dmsc_context* ctx, const std::string& n, int64_t& lo, int64_t& h
   if (n == "rounds_max") { lo = 1; hi = 4096; return &t.rounds_max; }
   if (n == "expand_levels") { lo = 1; hi = 4096; return &t.expand_levels; }
   if (n == "graph_rounds") { lo = 0; hi = 1; return &t.graph_rounds; }
+  if (n == "tail_items") { lo = 0; hi = 1 << 16; return &t.tail_items; }
   return nullptr;
 }
 }  // namespace

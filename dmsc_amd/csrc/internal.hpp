@@ -121,6 +121,8 @@ struct Tunables {
   int rounds_max = 16;       // most rounds between host checks (the batch doubles up to this)
   int expand_levels = 8;     // arc expansion levels between host checks
   int graph_rounds = 0;      // 1: each batch of rounds is a captured hipGraph, replayed while it fits
+  int tail_items = 2048;     // pending items at a host check up to which one workgroup runs every
+                             // remaining round in one launch (0: never)
 };
 extern thread_local const Tunables* tl_tune;
 inline const Tunables& tune() {
@@ -156,6 +158,18 @@ __device__ __forceinline__ void post_min(uint64_t* a, uint64_t key, bool check) 
   if (check && __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= key) return;
   atomicMin((unsigned long long*)a, (unsigned long long)key);
 }
+
+// Barrier for kernels whose phases hand data to each other through global memory inside one
+// workgroup: agent-scope release before and acquire after, so the next phase reads every store and
+// atomic of the previous one (the acquire drops lines the CU's L1 may still hold).
+__device__ __forceinline__ void phase_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
+// Threads of the one-workgroup kernels that run the last rounds of a round loop.
+constexpr int kTailThreads = 1024;
 
 // Inclusive prefix sum over the 64 lanes of a wave (every lane must take part).
 __device__ __forceinline__ int32_t wave_inclusive_sum(int32_t v) {

@@ -373,6 +373,52 @@ void launch_geo_round2(const ArcSide* a, const GeoState* g, int live, int round,
   DMSC_LAUNCH_CHECK();
 }
 
+// The last geometry rounds of both traced sides in one workgroup (same structure as the
+// persistence tail kernel): the pending counts end in cnt[r & 1], the round reached in *round_out.
+__global__ __launch_bounds__(kTailThreads) void k_geo_tail(ArcSide A0, GeoState G0, ArcSide A1, GeoState G1, int live,
+                                                           int round0, int max_rounds,
+                                                           int32_t* __restrict__ round_out) {
+  __shared__ int32_t s_next[2];
+  int n0 = (live & 1) ? __hip_atomic_load(&G0.cnt[round0 & 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+  int n1 = (live & 2) ? __hip_atomic_load(&G1.cnt[round0 & 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+  int r = round0;
+  for (; r < round0 + max_rounds && n0 + n1 > 0; ++r) {
+    const int cur = r & 1;
+    const uint64_t tag = (uint64_t)(0xFFFFFFFFu - (uint32_t)r) << 32;
+    if (threadIdx.x == 0) s_next[0] = s_next[1] = 0;
+    for (int j = threadIdx.x; j < n0 + n1; j += kTailThreads) {
+      if (j >= n0) geo_find_one(A1, G1, G1.wl[cur][j - n0], j - n0, tag, r > 0);
+      else geo_find_one(A0, G0, G0.wl[cur][j], j, tag, r > 0);
+    }
+    phase_barrier();
+    for (int j = threadIdx.x; j < n0 + n1; j += kTailThreads) {
+      const int side = j >= n0 ? 1 : 0;
+      const ArcSide& A = side ? A1 : A0;
+      const GeoState& g = side ? G1 : G0;
+      const int jj = side ? j - n0 : j;
+      const int4 it = g.wl[cur][jj];
+      if (geo_apply_one(A, g, it, jj, r, tag)) g.wl[cur ^ 1][atomicAdd(&s_next[side], 1)] = it;
+    }
+    phase_barrier();
+    n0 = s_next[0];
+    n1 = s_next[1];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    if (live & 1) G0.cnt[r & 1] = n0;
+    if (live & 2) G1.cnt[r & 1] = n1;
+    *round_out = r;
+  }
+}
+
+void launch_geo_tail(const ArcSide* a, const GeoState* g, int live, int round0, int max_rounds, int32_t* round_out,
+                     hipStream_t st) {
+  KScope ks_("geo_round", st);
+  hipLaunchKernelGGL(k_geo_tail, dim3(1), dim3(kTailThreads), 0, st, a[0], g[0], a[1], g[1], live, round0, max_rounds,
+                     round_out);
+  DMSC_LAUNCH_CHECK();
+}
+
 // Both traced sides, every round, in one cooperative launch (same structure and exit rule as the
 // persistence rounds kernel): find, grid barrier, apply, grid barrier, until both drain.
 __global__ __launch_bounds__(256) void k_geo_rounds(ArcSide A0, GeoState G0, ArcSide A1, GeoState G1, int live,

@@ -193,6 +193,59 @@ __global__ __launch_bounds__(256) void k_persist_apply(PersistSide s0, PersistSi
   persist_apply_phase<DUAL>(s0, s1, round, removed, small_list);
 }
 
+// The tail of the sweep in one workgroup: once few saddles are pending, a round would cost two
+// launches for a few hundred items. One block runs find, barrier, apply, barrier per round until both
+// worklists drain or max_rounds have run, then leaves the pending counts in cnt[r & 1] and the round
+// reached in *round_out. Waiting saddles take next-list slots from an LDS counter (the list order
+// affects no decision: every decision compares posted keys). Loop bounds are block-uniform (LDS).
+template <bool DUAL>
+__global__ __launch_bounds__(kTailThreads) void k_persist_tail(PersistSide s0, PersistSide s1, int round0,
+                                                               int max_rounds, uint8_t* __restrict__ removed,
+                                                               int32_t* __restrict__ round_out) {
+  __shared__ int32_t s_next[2];
+  int n0 = __hip_atomic_load(&s0.cnt[round0 & 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  int n1 = __hip_atomic_load(&s1.cnt[round0 & 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  int r = round0;
+  for (; r < round0 + max_rounds && n0 + n1 > 0; ++r) {
+    const int cur = r & 1;
+    const uint64_t tag = (uint64_t)(0xFFFFFFFFu - (uint32_t)r) << 32;
+    if (threadIdx.x == 0) s_next[0] = s_next[1] = 0;
+    for (int j = threadIdx.x; j < n0 + n1; j += kTailThreads) {
+      if (j >= n0) persist_find_one<DUAL>(s1, s1.wl[cur][j - n0], j - n0, tag, r == 0);
+      else persist_find_one<DUAL>(s0, s0.wl[cur][j], j, tag, r == 0);
+    }
+    phase_barrier();
+    for (int j = threadIdx.x; j < n0 + n1; j += kTailThreads) {
+      const int side = j >= n0 ? 1 : 0;
+      const PersistSide& s = side ? s1 : s0;
+      const int jj = side ? j - n0 : j;
+      const int i = s.wl[cur][jj];
+      if (persist_apply_one<DUAL>(s, i, jj, tag, removed)) s.wl[cur ^ 1][atomicAdd(&s_next[side], 1)] = i;
+    }
+    phase_barrier();
+    n0 = s_next[0];
+    n1 = s_next[1];
+    __syncthreads();  // every thread has read the counts before the next round clears them
+  }
+  if (threadIdx.x == 0) {
+    s0.cnt[r & 1] = n0;
+    s1.cnt[r & 1] = n1;
+    *round_out = r;
+  }
+}
+
+void launch_persist_tail(bool dual, const PersistSide& s0, const PersistSide& s1, int round0, int max_rounds,
+                         uint8_t* removed, int32_t* round_out, hipStream_t st) {
+  KScope ks_("persist_round", st);
+  if (dual)
+    hipLaunchKernelGGL(k_persist_tail<true>, dim3(1), dim3(kTailThreads), 0, st, s0, s1, round0, max_rounds, removed,
+                       round_out);
+  else
+    hipLaunchKernelGGL(k_persist_tail<false>, dim3(1), dim3(kTailThreads), 0, st, s0, s1, round0, max_rounds, removed,
+                       round_out);
+  DMSC_LAUNCH_CHECK();
+}
+
 // All rounds in one cooperative launch: find, grid barrier, apply, grid barrier, until both
 // worklists are empty or max_rounds have run. Every block takes the same path: the phase bounds
 // and the exit test read counters that no block writes between the barrier and the read.

@@ -35,6 +35,10 @@ void launch_round_inc(int32_t* d_round, hipStream_t st);
 // Every round up to max_rounds (or until both worklists drain) in one cooperative launch; the
 // round reached goes to *round_out. Returns false, having launched nothing, when the device or
 // the launch size does not allow a cooperative launch.
+// Every remaining round in one single-workgroup launch (for short worklists); pending counts are
+// left in cnt[r & 1] of each side and the round reached in *round_out (device).
+void launch_persist_tail(bool dual, const PersistSide& s0, const PersistSide& s1, int round0, int max_rounds,
+                         uint8_t* removed, int32_t* round_out, hipStream_t st);
 bool launch_persist_rounds(bool dual, const PersistSide& s0, const PersistSide& s1, int round0, int max_rounds,
                            uint8_t* removed, int32_t* round_out, int64_t max_items, hipStream_t st);
 void launch_uf_flatten(int32_t* par, int32_t* root, int64_t n, hipStream_t st);
@@ -109,6 +113,9 @@ void launch_geo_round2(const ArcSide* a, const GeoState* g, int live, int round,
                        const int32_t* d_round = nullptr);
 // All geometry rounds of the traced sides (bit s of `live`) in one cooperative launch; see
 // launch_persist_rounds.
+// Geometry counterpart of launch_persist_tail (both traced sides, `live` mask).
+void launch_geo_tail(const ArcSide* a, const GeoState* g, int live, int round0, int max_rounds, int32_t* round_out,
+                     hipStream_t st);
 bool launch_geo_rounds(const ArcSide* a, const GeoState* g, int live, int round0, int max_rounds, int32_t* round_out,
                        int64_t max_items, hipStream_t st);
 void launch_cancel_flags(int64_t ns, const int32_t* cancel_d, int32_t* flag, hipStream_t st);

@@ -132,6 +132,8 @@ dmsc_status dmsc_profile_entry(dmsc_context* ctx, int32_t index, const char** na
  *   "rounds_max"    1..4096    most rounds between host checks (default 16)
  *   "expand_levels" 1..4096    arc expansion levels between host checks (default 8)
  *   "graph_rounds"  0..1       1: each batch of rounds is captured once as a hipGraph and replayed (default 0)
+ *   "tail_items"    0..65536   at most this many pending items: one workgroup runs every remaining round
+ *                              of a round loop in one launch (default 2048; 0 = never)
  * Unknown names or values out of range return DMSC_ERR_INVALID_ARGUMENT. */
 dmsc_status dmsc_set_param(dmsc_context* ctx, const char* name, int64_t value);
 dmsc_status dmsc_get_param(dmsc_context* ctx, const char* name, int64_t* value);
