@@ -39,8 +39,8 @@ enum : int {
   A_SIDE_END = A_SIDE0 + 2 * 14,
   A_ASM0 = A_SIDE_END,  // assembly: counts, prefixes, sad_to_out, survivor lists (12 slots)
   A_ASM_END = A_ASM0 + 12,
-  A_ARC0 = A_ASM_END,  // per arc side: 25 slots each
-  A_ARC_END = A_ARC0 + 2 * 25,
+  A_ARC0 = A_ASM_END,  // per arc side: kArcSlots slots each
+  A_ARC_END = A_ARC0 + 2 * 22,
 };
 
 enum SideSlot { P_PARG, P_PART, P_ALIVE, P_MINKEY, P_PAIR, P_PERS, P_CANCEL, P_RR, P_WL0, P_WL1, P_CNT, P_ROOT, P_TOOUT,
@@ -48,8 +48,9 @@ enum SideSlot { P_PARG, P_PART, P_ALIVE, P_MINKEY, P_PAIR, P_PERS, P_CANCEL, P_R
 static_assert(A_SIDE_END - A_SIDE0 == 2 * kSideSlots, "side slot count");
 enum ArcSlot {
   G_BLEN, G_BOFF, G_ARENA, G_FLAG, G_PRE, G_CS0, G_CD0, G_KEYS, G_SLOT, G_KEYS2, G_SLOT2, G_CS, G_CD, G_DEADC,
-  G_GPAR, G_LLEN, G_LC, G_LKD, G_DONE, G_WATCH, G_RR, G_WL0, G_WL1, G_CNT, G_JUMP
+  G_LINK, G_DONE, G_WATCH, G_RR, G_WL0, G_WL1, G_CNT, G_JUMP, kArcSlots
 };
+static_assert(A_ARC_END - A_ARC0 == 2 * kArcSlots, "arc slot count");
 enum ArcSlot2 { H_PJQ = 0, H_FLEN, H_PERPOS, H_POSPRE, H_SEGA, H_SEGB, H_RAW, H_NSEG, H_CUMR, H_CUMC, H_ARCB };
 
 }  // namespace dmsc
@@ -579,7 +580,7 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     if (!traced[s]) continue;
     ArcWork& w = aw[s];
     w.on = true;
-    const int b = A_ARC0 + 25 * s;
+    const int b = A_ARC0 + kArcSlots * s;
     int32_t* blen = ar.get<int32_t>(b + G_BLEN, 2 * nsad);
     int64_t* boff = ar.get<int64_t>(b + G_BOFF, 2 * nsad);
     launch_arc_lengths(2 * nsad, s ? cmax : cmin, s ? lmax : lmin, blen, st);
@@ -615,7 +616,7 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
   for (int s = 0; s < 2; ++s) {
     if (!aw[s].on) continue;
     ArcWork& w = aw[s];
-    const int b = A_ARC0 + 25 * s;
+    const int b = A_ARC0 + kArcSlots * s;
     const int64_t next = s ? nmax : nmin;
     w.arena_total = h64[2 * s];
     w.ncancel = h32[8 + s];
@@ -641,20 +642,15 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     g.cd = cd0;
     g.key = ckey;
     g.kill = kill;
-    g.gparent = ar.get<int32_t>(b + G_GPAR, next);
-    g.linklen = ar.get<int32_t>(b + G_LLEN, next);
-    g.linkc = ar.get<int32_t>(b + G_LC, next);
-    g.linkkd = ar.get<int32_t>(b + G_LKD, next);
+    g.link = ar.get<int4>(b + G_LINK, next);
     g.watch = ar.get<uint64_t>(b + G_WATCH, next);
     g.rr = ar.get<int32_t>(b + G_RR, 4 * nc);
     g.wl[0] = ar.get<int4>(b + G_WL0, nc);
     g.wl[1] = ar.get<int4>(b + G_WL1, nc);
     g.cnt = ar.get<int32_t>(b + G_CNT, 2);
     g.err = d_err;
-    iota_i32(g.gparent, next, st);
     g.gjump = ar.get<uint64_t>(b + G_JUMP, next);
-    pointer_jump_init(g.gparent, g.gjump, next, st);  // (self, 0) for every extremum
-    fill_i32(g.linklen, 0, next, st);
+    launch_geo_init(g, next, st);  // every extremum its own root
     fill_u64(g.watch, ~0ull, next, st);
     launch_geo_items(nc, cs0, ckey, w.A.tinit, g.wl[0], st);
     fill_i32(g.cnt, (int32_t)nc, 1, st);

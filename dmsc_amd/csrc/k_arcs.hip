@@ -191,16 +191,19 @@ __device__ __forceinline__ void geo_root(const GeoState& g, int x, int& root, in
 
 // worklist item: (cancellation, key, arc target 0, arc target 1); results go to rr at the item's
 // list position
+// In round 0 the forest has no links yet: every target is its own root at length 0.
 __device__ __forceinline__ void geo_find_one(const ArcSide& A, const GeoState& g, const int4 it, int64_t j,
-                                             uint64_t tag, bool check) {
-  int R0, R1;
-  int32_t L0, L1;
-  geo_root(g, it.z, R0, L0);
-  geo_root(g, it.w, R1, L1);
+                                             uint64_t tag, bool first) {
+  int R0 = it.z, R1 = it.w;
+  int32_t L0 = 0, L1 = 0;
+  if (!first) {
+    geo_root(g, it.z, R0, L0);
+    geo_root(g, it.w, R1, L1);
+  }
   *reinterpret_cast<int4*>(g.rr + 4 * j) = make_int4(R0, R1, L0, L1);
   const uint64_t key = tag | (uint32_t)it.y;
-  if (R0 >= 0) post_min(&g.watch[R0], key, check);
-  if (R1 >= 0) post_min(&g.watch[R1], key, check);
+  if (R0 >= 0) post_min(&g.watch[R0], key, !first);
+  if (R1 >= 0) post_min(&g.watch[R1], key, !first);
 }
 
 __device__ __forceinline__ void geo_find_phase(const ArcSide& A, const GeoState& g, int round) {
@@ -209,7 +212,7 @@ __device__ __forceinline__ void geo_find_phase(const ArcSide& A, const GeoState&
   if (blockIdx.x == 0 && threadIdx.x == 0) g.cnt[cur ^ 1] = 0;
   const uint64_t tag = (uint64_t)(0xFFFFFFFFu - (uint32_t)round) << 32;
   for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x)
-    geo_find_one(A, g, g.wl[cur][j], j, tag, round > 0);
+    geo_find_one(A, g, g.wl[cur][j], j, tag, round == 0);
 }
 
 // Both traced sides of one round in one launch: side 0's items first, then side 1's.
@@ -225,8 +228,8 @@ __global__ void k_geo_find2(ArcSide A0, GeoState G0, ArcSide A1, GeoState G1, in
   const uint64_t tag = (uint64_t)(0xFFFFFFFFu - (uint32_t)round) << 32;
   for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < (int64_t)n0 + n1;
        j += (int64_t)gridDim.x * blockDim.x) {
-    if (j >= n0) geo_find_one(A1, G1, G1.wl[cur][j - n0], j - n0, tag, round > 0);
-    else geo_find_one(A0, G0, G0.wl[cur][j], j, tag, round > 0);
+    if (j >= n0) geo_find_one(A1, G1, G1.wl[cur][j - n0], j - n0, tag, round == 0);
+    else geo_find_one(A0, G0, G0.wl[cur][j], j, tag, round == 0);
   }
 }
 __global__ void k_geo_find(ArcSide A, GeoState g, int round) { geo_find_phase(A, g, round); }
@@ -260,11 +263,8 @@ __device__ __forceinline__ bool geo_apply_one(const ArcSide& A, const GeoState& 
     const int Ro = kd == 0 ? R1 : R0;
     const int32_t Ld = kd == 0 ? L0 : L1, Lo = kd == 0 ? L1 : L0;
     const int32_t ll = sat_add(sat_add(Ld, A.blen[2 * s + kd]), sat_add(A.blen[2 * s + ko], Lo));
-    g.gparent[d] = Ro;
-    g.linklen[d] = ll;
+    g.link[d] = make_int4(Ro, ll, s, kd);
     g.gjump[d] = (uint64_t)(Ro < 0 ? kVoid : (uint32_t)Ro) | ((uint64_t)(uint32_t)ll << 32);
-    g.linkc[d] = c;
-    g.linkkd[d] = kd;
   }
   return false;
 }
@@ -387,8 +387,8 @@ __global__ __launch_bounds__(kTailThreads) void k_geo_tail(ArcSide A0, GeoState 
     const uint64_t tag = (uint64_t)(0xFFFFFFFFu - (uint32_t)r) << 32;
     if (threadIdx.x == 0) s_next[0] = s_next[1] = 0;
     for (int j = threadIdx.x; j < n0 + n1; j += kTailThreads) {
-      if (j >= n0) geo_find_one(A1, G1, G1.wl[cur][j - n0], j - n0, tag, r > 0);
-      else geo_find_one(A0, G0, G0.wl[cur][j], j, tag, r > 0);
+      if (j >= n0) geo_find_one(A1, G1, G1.wl[cur][j - n0], j - n0, tag, r == 0);
+      else geo_find_one(A0, G0, G0.wl[cur][j], j, tag, r == 0);
     }
     phase_barrier();
     for (int j = threadIdx.x; j < n0 + n1; j += kTailThreads) {
@@ -476,6 +476,18 @@ void launch_geo_round(const ArcSide& a, const GeoState& g, int round, int64_t ma
   DMSC_LAUNCH_CHECK();
 }
 
+__global__ void k_geo_init(int4* __restrict__ link, uint64_t* __restrict__ gjump, int64_t next) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < next; e += (int64_t)gridDim.x * blockDim.x) {
+    link[e] = make_int4((int)e, 0, 0, 0);
+    gjump[e] = (uint64_t)(uint32_t)e;
+  }
+}
+void launch_geo_init(const GeoState& g, int64_t next, hipStream_t st) {
+  if (next == 0) return;
+  hipLaunchKernelGGL(k_geo_init, dim3(grid_for(next, 256, 16384)), dim3(256), 0, st, g.link, g.gjump, next);
+  DMSC_LAUNCH_CHECK();
+}
+
 // ------------------------------------------------------------------ path lengths to the final root
 // pjq[e] = (final root or `next` for the void, length of path(e -> root)), by pointer jumping with
 // saturating sums over the geometry forest.
@@ -483,10 +495,10 @@ __global__ void k_geo_q_init(GeoState g, int64_t next, uint64_t* __restrict__ pj
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e <= next; e += (int64_t)gridDim.x * blockDim.x) {
     uint32_t J = (uint32_t)e, D = 0;
     if (e < next) {
-      const int p = g.gparent[e];
-      if (p != (int)e) {
-        J = p < 0 ? (uint32_t)next : (uint32_t)p;
-        D = (uint32_t)g.linklen[e];
+      const int2 pl = *reinterpret_cast<const int2*>(&g.link[e]);
+      if (pl.x != (int)e) {
+        J = pl.x < 0 ? (uint32_t)next : (uint32_t)pl.x;
+        D = (uint32_t)pl.y;
       }
     }
     pj[e] = (uint64_t)J | ((uint64_t)D << 32);
@@ -618,16 +630,17 @@ struct LinkParts {
   int s, kd, ko, Td, To, Ro;
   int32_t Alen, Bk, Bo, Clen;
 };
-__device__ __forceinline__ LinkParts link_parts(const ArcSide& A, const GeoState& g, const uint64_t* __restrict__ pjq,
-                                                int d) {
+// lk = link[d] of a linked extremum d
+__device__ __forceinline__ LinkParts link_parts(const ArcSide& A, const uint64_t* __restrict__ pjq, int d,
+                                                const int4 lk) {
   LinkParts L;
-  const int c = g.linkc[d];
-  L.kd = g.linkkd[d];
+  L.kd = lk.w;
   L.ko = 1 - L.kd;
-  L.s = g.cs[c];
-  L.Td = A.tinit[2 * L.s + L.kd];
-  L.To = A.tinit[2 * L.s + L.ko];
-  L.Ro = g.gparent[d];
+  L.s = lk.z;
+  const int2 T = *reinterpret_cast<const int2*>(A.tinit + 2 * L.s);
+  L.Td = L.kd ? T.y : T.x;
+  L.To = L.kd ? T.x : T.y;
+  L.Ro = lk.x;
   L.Alen = qlen(pjq, L.Td) - qlen(pjq, d);
   L.Bk = A.blen[2 * L.s + L.kd];
   L.Bo = A.blen[2 * L.s + L.ko];
@@ -642,10 +655,10 @@ __global__ void k_link_counts(ArcSide A, GeoState g, const uint64_t* __restrict_
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e <= next; e += (int64_t)gridDim.x * blockDim.x) {
     uint32_t J = (uint32_t)e, nr = 0, nc = 0;
     if (e < next) {
-      const int p = g.gparent[e];
-      if (p != (int)e) {
-        J = p < 0 ? (uint32_t)next : (uint32_t)p;
-        const LinkParts L = link_parts(A, g, pjq, (int)e);
+      const int4 lk = g.link[e];
+      if (lk.x != (int)e) {
+        J = lk.x < 0 ? (uint32_t)next : (uint32_t)lk.x;
+        const LinkParts L = link_parts(A, pjq, (int)e, lk);
         nr = 1 + (L.Bo > 0 ? 1 : 0);
         nc = (L.Alen > 0 ? 1 : 0) + (L.Clen > 0 ? 1 : 0);
       }
@@ -701,13 +714,15 @@ __global__ __launch_bounds__(256) void k_expand(ArcSide A, GeoState g, const uin
     const int pr_end = pr + nr, pc_end = pc + nc;
     int y = sg.a;
     int64_t acc = 0;
-    for (int it = 0; y >= 0 && y != b && g.gparent[y] != y; ++it) {
+    for (int it = 0; y >= 0 && y != b; ++it) {
+      const int4 lk = g.link[y];
+      if (lk.x == y) break;
       if (it >= kLoopCap) {
         report(g.err, ERR_GEO_CAP);
         break;
       }
-      const int32_t len = g.linklen[y];
-      const LinkParts L = link_parts(A, g, pjq, y);
+      const int32_t len = lk.y;
+      const LinkParts L = link_parts(A, pjq, y, lk);
       const int need_r = 1 + (L.Bo > 0 ? 1 : 0), need_c = (L.Alen > 0 ? 1 : 0) + (L.Clen > 0 ? 1 : 0);
       if (pr + need_r > pr_end || pc + need_c > pc_end) {  // counts disagree with the walk: never overrun
         report(g.err, ERR_SEG_CAP);
@@ -732,7 +747,7 @@ __global__ __launch_bounds__(256) void k_expand(ArcSide A, GeoState g, const uin
         if (L.Alen > 0) out[pc++] = Seg{L.Td, y, seg_od(o, 1)};
       }
       acc += len;
-      y = g.gparent[y];
+      y = lk.x;
     }
   }
 }

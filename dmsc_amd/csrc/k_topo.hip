@@ -47,8 +47,9 @@ template <bool DUAL>
 __device__ __forceinline__ void persist_find_one(const PersistSide& s, const int i, int64_t j, uint64_t tag,
                                                  bool first) {
   const int2 cc = *reinterpret_cast<const int2*>((first ? s.c : s.croot) + 2 * i);
-  const int ra = uf_find(s.par_g, cc.x, s.err);
-  const int rb = uf_find(s.par_g, cc.y, s.err);
+  // round 0: no merges yet, every extremum is its own root
+  const int ra = first ? (cc.x < 0 ? -1 : cc.x) : uf_find(s.par_g, cc.x, s.err);
+  const int rb = first ? (cc.y < 0 ? -1 : cc.y) : uf_find(s.par_g, cc.y, s.err);
   if (ra == rb) {
     s.pair[i] = -1;
     s.pers[i] = -1.0f;

@@ -96,17 +96,18 @@ struct GeoState {
   const int32_t* key;      // cancel -> sorted position
   uint64_t* kill;          // per extremum: (round its killing cancellation was decided, or all ones while
                            // pending) << 32 | that cancellation's key (all ones: no killer)
-  int32_t* gparent;        // geometry union-find (-1 = void)
-  uint64_t* gjump;         // path-compressed view of gparent: (ancestor or kVoid, link length to it)
-  int32_t* linklen;        // length of the link path of a linked extremum (saturating int32)
-  int32_t* linkc;          // cancel that linked the extremum
-  int32_t* linkkd;         // which arc (0/1) of that cancel's saddle led to the dead extremum
+  int4* link;              // per extremum, one record: (parent in the geometry forest: self for a root,
+                           // -1 for the void; length of its link path (saturating); saddle of the
+                           // cancellation that linked it; which arc (0/1) of that saddle led to it)
+  uint64_t* gjump;         // path-compressed view of the parents: (ancestor or kVoid, link length to it)
   uint64_t* watch;         // per extremum: (tag << 32 | cancel) atomic min
   int32_t* rr;             // per worklist position: R0, R1, len0, len1
   int4* wl[2];             // worklists: (cancellation, key, arc target 0, arc target 1)
   int32_t* cnt;
   int32_t* err;
 };
+// link[e] = (e, 0, 0, 0), gjump[e] = (e, 0): every extremum its own root
+void launch_geo_init(const GeoState& g, int64_t next, hipStream_t st);
 void launch_geo_round(const ArcSide& a, const GeoState& g, int round, int64_t max_items, hipStream_t st);
 // one round of both traced sides (bit s of `live`) in one find and one apply launch
 void launch_geo_round2(const ArcSide* a, const GeoState* g, int live, int round, int64_t max_items, hipStream_t st,
