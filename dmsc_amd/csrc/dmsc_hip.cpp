@@ -67,6 +67,8 @@ struct dmsc_context {
   KernelTimes times;
   Tunables tune;
   hipStream_t cap_stream = nullptr;  // private stream that round batches are captured on
+  hipStream_t side = nullptr;        // output emission, concurrent with the arc geometry
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   Arena ar;
   Arena ar2;  // arc emission scratch
   int64_t* pinned = nullptr;
@@ -212,6 +214,35 @@ int round_batch(int done) {
   for (int sum = b; sum <= done && b < t.rounds_max; sum += b) b = std::min(2 * b, t.rounds_max);
   return b;
 }
+
+// Work forked from the caller's stream onto the context's side stream (default priority) and joined
+// back when the scope ends, on every path out of it, errors included: the caller's stream then
+// orders everything after the side work (outputs written there are never released early).
+struct StreamFork {
+  dmsc_context* ctx;
+  hipStream_t st;
+  bool on;
+  StreamFork(dmsc_context* c, hipStream_t s, bool enable) : ctx(c), st(s), on(enable) {
+    if (!on) return;
+    if (!ctx->side) DMSC_CHECK(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
+    if (!ctx->ev_fork) DMSC_CHECK(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
+    if (!ctx->ev_join) DMSC_CHECK(hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
+    DMSC_CHECK(hipEventRecord(ctx->ev_fork, st));
+    DMSC_CHECK(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
+  }
+  hipStream_t stream() const { return on ? ctx->side : st; }
+  void join() {
+    if (!on) return;
+    on = false;
+    DMSC_CHECK(hipEventRecord(ctx->ev_join, ctx->side));
+    DMSC_CHECK(hipStreamWaitEvent(st, ctx->ev_join, 0));
+  }
+  ~StreamFork() {
+    if (!on) return;
+    if (hipEventRecord(ctx->ev_join, ctx->side) != hipSuccess || hipStreamWaitEvent(st, ctx->ev_join, 0) != hipSuccess)
+      (void)hipStreamSynchronize(ctx->side);
+  }
+};
 
 // Round loops: one launch per phase with a host readback every few rounds. DMSC_ROUNDS=coop selects
 // one cooperative launch per loop instead; that path is off by default because a large-grid run
@@ -613,6 +644,40 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
   auto hrow = [&](int r, int f) -> int64_t { return hb[(size_t)r * (B + 1) + f]; };
   const int64_t n_sad_total = h32[5];  // surviving saddles of all fields
 
+  // ---- outputs that do not depend on the arc geometry (cpu/workspace.hxx:291-362 layout, dual swap by
+  // slot). Their kernels only read persistence and assembly results, which nothing below writes, so
+  // they run on the side stream while the arc geometry (latency-bound rounds) runs on `st`.
+  auto n_out_ext = [&](int s, int f) { return hrow(3 + s, f + 1) - hrow(3 + s, f); };
+  auto n_sad_out = [&](int f) { return hrow(5, f + 1) - hrow(5, f); };
+  auto n_rows = [&](int s, int f) { return hrow(6 + s, f + 1) - hrow(6 + s, f); };
+  auto one = [](int64_t n) { return std::make_pair(n, (int64_t)0); };
+  int32_t* o_pts[2];
+  for (int s = 0; s < 2; ++s)
+    o_pts[s] = (int32_t*)oa.slot(slot_pts(s == 1, dual), 1, [&](int f) { return one(n_out_ext(s, f)); },
+                                 DMSC_DTYPE_INT32);
+  int32_t* o_sad = (int32_t*)oa.slot(3, 1, [&](int f) { return one(n_sad_out(f)); }, DMSC_DTYPE_INT32);
+  int32_t *o_e[2], *o_pp[2];
+  float* o_p[2];
+  for (int s = 0; s < 2; ++s) {
+    o_e[s] = (int32_t*)oa.slot(slot_e(s == 1, dual), 2, [&](int f) { return std::make_pair(n_rows(s, f), (int64_t)2); },
+                               DMSC_DTYPE_INT32);
+    o_p[s] = (float*)oa.slot(slot_p(s == 1, dual), 1, [&](int f) { return one(n_sad_out(f)); }, DMSC_DTYPE_FLOAT32);
+    o_pp[s] = (int32_t*)oa.slot(slot_pp(s == 1, dual), 1, [&](int f) { return one(n_sad_out(f)); }, DMSC_DTYPE_INT32);
+  }
+  StreamFork fork(ctx, st, tmin || tmax);
+  {
+    const hipStream_t es = fork.stream();
+    launch_emit_points(nmin, min_ids, sb[0].to_out, o_pts[0], es);
+    launch_emit_points(nmax, max_ids, sb[1].to_out, o_pts[1], es);
+    const EmitBases eb[2] = {{sad_field, row32(5), row32(3)}, {sad_field, row32(5), row32(4)}};
+    launch_emit_side(n_sad_total, surv_list[0], nullptr, sad_to_out, cmin, sb[0].root, sb[0].to_out, sb[0].pers,
+                     sb[0].pair, surv_rows[0], sad_ids, eb[0], o_p[0], o_pp[0], o_e[0], o_sad, es);
+    launch_emit_side(n_sad_total, surv_list[1], nullptr, sad_to_out, cmax, sb[1].root, sb[1].to_out, sb[1].pers,
+                     sb[1].pair, surv_rows[1], sad_ids, eb[1], o_p[1], o_pp[1], o_e[1], nullptr, es);
+    if (gmax) launch_labels_faces(rv, S, pjf, sb[1].root, sb[1].to_out, row32(4), o_flab, es);
+    if (gmin) launch_labels_verts(rv, N, pjv, sb[0].root, sb[0].to_out, row32(3), o_vlab, es);
+  }
+
   for (int s = 0; s < 2; ++s) {
     if (!aw[s].on) continue;
     ArcWork& w = aw[s];
@@ -758,24 +823,7 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     }
   }
 
-  // ---- outputs (cpu/workspace.hxx:291-362 layout, dual swap by slot)
-  auto n_out_ext = [&](int s, int f) { return hrow(3 + s, f + 1) - hrow(3 + s, f); };
-  auto n_sad_out = [&](int f) { return hrow(5, f + 1) - hrow(5, f); };
-  auto n_rows = [&](int s, int f) { return hrow(6 + s, f + 1) - hrow(6 + s, f); };
-  auto one = [](int64_t n) { return std::make_pair(n, (int64_t)0); };
-  int32_t* o_pts[2];
-  for (int s = 0; s < 2; ++s)
-    o_pts[s] = (int32_t*)oa.slot(slot_pts(s == 1, dual), 1, [&](int f) { return one(n_out_ext(s, f)); },
-                                 DMSC_DTYPE_INT32);
-  int32_t* o_sad = (int32_t*)oa.slot(3, 1, [&](int f) { return one(n_sad_out(f)); }, DMSC_DTYPE_INT32);
-  int32_t *o_e[2], *o_pp[2];
-  float* o_p[2];
-  for (int s = 0; s < 2; ++s) {
-    o_e[s] = (int32_t*)oa.slot(slot_e(s == 1, dual), 2, [&](int f) { return std::make_pair(n_rows(s, f), (int64_t)2); },
-                               DMSC_DTYPE_INT32);
-    o_p[s] = (float*)oa.slot(slot_p(s == 1, dual), 1, [&](int f) { return one(n_sad_out(f)); }, DMSC_DTYPE_FLOAT32);
-    o_pp[s] = (int32_t*)oa.slot(slot_pp(s == 1, dual), 1, [&](int f) { return one(n_sad_out(f)); }, DMSC_DTYPE_INT32);
-  }
+  // ---- arc outputs (cpu/workspace.hxx:291-362 layout, dual swap by slot)
   for (int s = 0; s < 2; ++s) {
     const int sl = slot_arcs(s == 1, dual);
     ArcWork& w = aw[s];
@@ -784,16 +832,6 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     w.out_off = (int32_t*)oa.slot(sl + 1, 1, [&](int f) { return one(w.on ? n_sad_out(f) + 1 : 0); }, DMSC_DTYPE_INT32);
     w.out_arc_off = (int32_t*)oa.slot(sl + 2, 1, [&](int f) { return one(w.on ? n_sad_out(f) : 0); }, DMSC_DTYPE_INT32);
   }
-
-  launch_emit_points(nmin, min_ids, sb[0].to_out, o_pts[0], st);
-  launch_emit_points(nmax, max_ids, sb[1].to_out, o_pts[1], st);
-  const EmitBases eb[2] = {{sad_field, row32(5), row32(3)}, {sad_field, row32(5), row32(4)}};
-  launch_emit_side(n_sad_total, surv_list[0], nullptr, sad_to_out, cmin, sb[0].root, sb[0].to_out, sb[0].pers,
-                   sb[0].pair, surv_rows[0], sad_ids, eb[0], o_p[0], o_pp[0], o_e[0], o_sad, st);
-  launch_emit_side(n_sad_total, surv_list[1], nullptr, sad_to_out, cmax, sb[1].root, sb[1].to_out, sb[1].pers,
-                   sb[1].pair, surv_rows[1], sad_ids, eb[1], o_p[1], o_pp[1], o_e[1], nullptr, st);
-  if (gmax) launch_labels_faces(rv, S, pjf, sb[1].root, sb[1].to_out, row32(4), o_flab, st);
-  if (gmin) launch_labels_verts(rv, N, pjv, sb[0].root, sb[0].to_out, row32(3), o_vlab, st);
 
   for (int s = 0; s < 2; ++s) {
     ArcWork& w = aw[s];
@@ -831,6 +869,7 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     if (nraw > cap) throw Fail{DMSC_ERR_INTERNAL, "arc segment list overflow"};
     if (nraw > 0) launch_copy_raw(w.A, raw, nseg + 3, std::min<int64_t>(nraw, cap), w.out_cells, w.out_total, d_err, st);
   }
+  fork.join();
   readback(ctx, d_err, sizeof(int32_t), st);
   if (ctx->times.on) ctx->times.collect();
   if (*(const int32_t*)ctx->pinned != 0)
@@ -937,6 +976,9 @@ void dmsc_context_destroy(dmsc_context* ctx) {
   ctx->ar2.release();
   if (ctx->pinned) (void)hipHostFree(ctx->pinned);
   if (ctx->cap_stream) (void)hipStreamDestroy(ctx->cap_stream);
+  if (ctx->side) (void)hipStreamDestroy(ctx->side);
+  if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+  if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
   delete ctx;
 }
 
