@@ -68,7 +68,7 @@ struct dmsc_context {
   Tunables tune;
   hipStream_t cap_stream = nullptr;  // private stream that round batches are captured on
   hipStream_t side = nullptr;        // output emission, concurrent with the arc geometry
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_mark = nullptr;
   Arena ar;
   Arena ar2;  // arc emission scratch
   int64_t* pinned = nullptr;
@@ -227,10 +227,24 @@ struct StreamFork {
     if (!ctx->side) DMSC_CHECK(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
     if (!ctx->ev_fork) DMSC_CHECK(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
     if (!ctx->ev_join) DMSC_CHECK(hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
+    if (!ctx->ev_mark) DMSC_CHECK(hipEventCreateWithFlags(&ctx->ev_mark, hipEventDisableTiming));
     DMSC_CHECK(hipEventRecord(ctx->ev_fork, st));
     DMSC_CHECK(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
   }
   hipStream_t stream() const { return on ? ctx->side : st; }
+  // the side stream also waits for everything enqueued on `st` so far
+  void follow() {
+    if (!on) return;
+    DMSC_CHECK(hipEventRecord(ctx->ev_fork, st));
+    DMSC_CHECK(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
+  }
+  // mark the side stream's current point; wait_mark() makes `st` wait for it
+  void mark() {
+    if (on) DMSC_CHECK(hipEventRecord(ctx->ev_mark, ctx->side));
+  }
+  void wait_mark() {
+    if (on) DMSC_CHECK(hipStreamWaitEvent(st, ctx->ev_mark, 0));
+  }
   void join() {
     if (!on) return;
     on = false;
@@ -479,6 +493,47 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
   if (!dual) launch_orders(nsad, asc, sad_field, sad_b, ord_min, ord_max, pos_min, pos_max, st);
   else launch_orders(nsad, asc, sad_field, sad_b, ord_max, ord_min, pos_max, pos_min, st);
 
+  // ---- raw arc lengths of the traced sides. The raw arcs need no persistence result: they are
+  // traced on the side stream while the persistence rounds run on `st`.
+  ArcWork aw[2];  // 0 = vertices (min arcs), 1 = faces (max arcs)
+  const bool traced[2] = {tmin, tmax};
+  for (int s = 0; s < 2; ++s) {
+    if (!traced[s]) continue;
+    ArcWork& w = aw[s];
+    w.on = true;
+    const int b = A_ARC0 + kArcSlots * s;
+    int32_t* blen = ar.get<int32_t>(b + G_BLEN, 2 * nsad);
+    int64_t* boff = ar.get<int64_t>(b + G_BOFF, 2 * nsad);
+    launch_arc_lengths(2 * nsad, s ? cmax : cmin, s ? lmax : lmin, blen, st);
+    scan_i32_to_i64(blen, boff, 2 * nsad, c64 + 2 * s, ar, A_SCAN, st);
+    w.A.faces = s;
+    w.A.succ = s ? fsucc : vsucc;
+    w.A.fd = bt.fd;
+    w.A.sfield = sad_field;
+    w.A.ns = nsad;
+    w.A.next = s ? nmax : nmin;
+    w.A.sad_ids = sad_ids;
+    w.A.tinit = s ? cmax : cmin;
+    w.A.blen = blen;
+    w.A.boff = boff;
+  }
+  StreamFork fork(ctx, st, tmin || tmax);
+  // arena sizes (c64[0], c64[2]) arrive with the first readback of the rounds, at pinned byte 256
+  constexpr int kArenaPin = 32;
+  bool raw_launched = false;
+  auto launch_raw = [&]() {
+    const int64_t* tot = ctx->pinned + kArenaPin;
+    for (int s = 0; s < 2; ++s) {
+      if (!aw[s].on) continue;
+      ArcWork& w = aw[s];
+      w.arena_total = tot[2 * s];
+      w.A.arena = ar.get<int32_t>(A_ARC0 + kArcSlots * s + G_ARENA, w.arena_total);
+      launch_raw_arcs(w.A, fork.stream());
+    }
+    fork.mark();  // the expansion waits for the raw arcs only
+    raw_launched = true;
+  };
+
   // ---- persistence rounds (both sides together)
   uint8_t* removed = ar.get<uint8_t>(A_REMOVED, nsad);
   fill_u8(removed, 0, nsad, st);
@@ -526,6 +581,8 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
   if (nsad > 0) {
     int64_t pend = 2 * nsad;  // pending items of both sides at the last readback (launch sizes)
     while (true) {
+      if (fork.on && !raw_launched)
+        DMSC_CHECK(hipMemcpyAsync(ctx->pinned + kArenaPin, c64, 4 * sizeof(int64_t), hipMemcpyDeviceToHost, st));
       if (coop && !launch_persist_rounds(dual, ps[0], ps[1], round, kCoopRounds, removed, c32 + 48, pend, st))
         coop = false;  // no cooperative launch on this device or at this size: one launch per phase
       // few pending saddles: one workgroup runs every remaining round (at most one round per item)
@@ -561,12 +618,18 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
         DMSC_CHECK(hipMemcpyAsync((char*)ctx->pinned + 4, sb[1].cnt + (round & 1), 4, hipMemcpyDeviceToHost, st));
         DMSC_CHECK(hipStreamSynchronize(st));
       }
+      if (fork.on && !raw_launched) launch_raw();
       const int32_t* pc = (const int32_t*)ctx->pinned;
       ctx->pending_log.insert(ctx->pending_log.end(), {round, pc[0], pc[1]});
       if (pc[0] == 0 && pc[1] == 0) break;
       pend = (int64_t)pc[0] + pc[1];
       if (round > 4 * nsad + 64) throw Fail{DMSC_ERR_INTERNAL, "persistence rounds did not converge"};
     }
+  }
+  if (fork.on && !raw_launched) {  // no rounds ran
+    readback(ctx, c64, 4 * sizeof(int64_t), st);
+    std::memcpy(ctx->pinned + kArenaPin, ctx->pinned, 4 * sizeof(int64_t));
+    launch_raw();
   }
   launch_uf_flatten(sb[0].par_t, sb[0].root, nmin, st);
   launch_uf_flatten(sb[1].par_t, sb[1].root, nmax, st);
@@ -604,41 +667,20 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
   launch_survivors(nsad, ord_min, surv, surv_pre, rows_pre[0], surv_list[0], surv_rows[0], sad_to_out, st);
   launch_survivors(nsad, ord_max, surv_mx, surv_mx_pre, rows_pre[1], surv_list[1], surv_rows[1], nullptr, st);
 
-  // ---- arc geometry, per traced side: raw arcs and cancellation lists
-  ArcWork aw[2];  // 0 = vertices (min arcs), 1 = faces (max arcs)
-  const bool traced[2] = {tmin, tmax};
+  // ---- cancellation lists of the traced sides
   for (int s = 0; s < 2; ++s) {
-    if (!traced[s]) continue;
-    ArcWork& w = aw[s];
-    w.on = true;
+    if (!aw[s].on) continue;
     const int b = A_ARC0 + kArcSlots * s;
-    int32_t* blen = ar.get<int32_t>(b + G_BLEN, 2 * nsad);
-    int64_t* boff = ar.get<int64_t>(b + G_BOFF, 2 * nsad);
-    launch_arc_lengths(2 * nsad, s ? cmax : cmin, s ? lmax : lmin, blen, st);
-    scan_i32_to_i64(blen, boff, 2 * nsad, c64 + 2 * s, ar, A_SCAN, st);
     int32_t* flag = ar.get<int32_t>(b + G_FLAG, nsad);
     int32_t* pre = ar.get<int32_t>(b + G_PRE, nsad);
     launch_cancel_flags(nsad, sb[s].cancel, flag, st);
     scan_exclusive<int32_t>(flag, pre, nsad, c32 + 8 + s, ar, A_SCAN, st);
-    w.A.faces = s;
-    w.A.succ = s ? fsucc : vsucc;
-    w.A.fd = bt.fd;
-    w.A.sfield = sad_field;
-    w.A.ns = nsad;
-    w.A.next = s ? nmax : nmin;
-    w.A.sad_ids = sad_ids;
-    w.A.tinit = s ? cmax : cmin;
-    w.A.blen = blen;
-    w.A.boff = boff;
   }
   DMSC_CHECK(hipMemcpyAsync(ctx->pinned, c32, 16 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
-  DMSC_CHECK(hipMemcpyAsync((char*)ctx->pinned + 64, c64, 4 * sizeof(int64_t), hipMemcpyDeviceToHost, st));
   DMSC_CHECK(hipMemcpyAsync((char*)ctx->pinned + 128, b32, 8 * sizeof(int32_t) * (B + 1), hipMemcpyDeviceToHost, st));
   DMSC_CHECK(hipStreamSynchronize(st));
   int32_t h32[16];
-  int64_t h64[4];
   std::memcpy(h32, ctx->pinned, sizeof(h32));
-  std::memcpy(h64, (char*)ctx->pinned + 64, sizeof(h64));
   std::vector<int32_t> hb(8 * (size_t)(B + 1));
   std::memcpy(hb.data(), (char*)ctx->pinned + 128, sizeof(int32_t) * hb.size());
   auto hrow = [&](int r, int f) -> int64_t { return hb[(size_t)r * (B + 1) + f]; };
@@ -664,7 +706,7 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     o_p[s] = (float*)oa.slot(slot_p(s == 1, dual), 1, [&](int f) { return one(n_sad_out(f)); }, DMSC_DTYPE_FLOAT32);
     o_pp[s] = (int32_t*)oa.slot(slot_pp(s == 1, dual), 1, [&](int f) { return one(n_sad_out(f)); }, DMSC_DTYPE_INT32);
   }
-  StreamFork fork(ctx, st, tmin || tmax);
+  fork.follow();  // the side stream also waits for the persistence and assembly results
   {
     const hipStream_t es = fork.stream();
     launch_emit_points(nmin, min_ids, sb[0].to_out, o_pts[0], es);
@@ -683,10 +725,7 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     ArcWork& w = aw[s];
     const int b = A_ARC0 + kArcSlots * s;
     const int64_t next = s ? nmax : nmin;
-    w.arena_total = h64[2 * s];
     w.ncancel = h32[8 + s];
-    w.A.arena = ar.get<int32_t>(b + G_ARENA, w.arena_total);
-    launch_raw_arcs(w.A, st);
     // cancellations sorted by (persistence, saddle id) (persistence_struct.hxx:15-19); fields never
     // meet, so one batch-wide stable sort orders every field's cancellations correctly
     const int64_t nc = w.ncancel;
@@ -833,6 +872,7 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     w.out_arc_off = (int32_t*)oa.slot(sl + 2, 1, [&](int f) { return one(w.on ? n_sad_out(f) : 0); }, DMSC_DTYPE_INT32);
   }
 
+  fork.wait_mark();  // raw arcs traced
   for (int s = 0; s < 2; ++s) {
     ArcWork& w = aw[s];
     if (!w.on) continue;
@@ -979,6 +1019,7 @@ void dmsc_context_destroy(dmsc_context* ctx) {
   if (ctx->side) (void)hipStreamDestroy(ctx->side);
   if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
   if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
+  if (ctx->ev_mark) (void)hipEventDestroy(ctx->ev_mark);
   delete ctx;
 }
 
