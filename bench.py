@@ -204,7 +204,8 @@ def main():
     torch.cuda.synchronize()
     calls_per_step = len(counts)
 
-    dmsc_gpu.profile_enable(0, True)
+    # level 1: events only around the gradient launches and each batch of persistence rounds
+    dmsc_gpu.profile_enable(0, 1)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for s in range(args.warmup, total_steps):
@@ -212,7 +213,7 @@ def main():
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     prof = dmsc_gpu.profile_read(0)
-    dmsc_gpu.profile_enable(0, False)
+    dmsc_gpu.profile_enable(0, 0)
 
     elapsed = t1 - t0
     px_step = sum(int(sh[0]) * int(sh[1]) for sh, _ in sizes)
@@ -220,17 +221,19 @@ def main():
     b_alg_step = sum(4 * int(sh[0]) * int(sh[1]) + ob for sh, ob in sizes)
     e2e_gbs = b_alg_step * args.steps / elapsed / 1e9
 
-    # dominant kernel (largest total event time) and its roofline
-    ranked = sorted(prof.items(), key=lambda kv: -kv[1][0])
-    if args.profile:
-        for k, (ms_tot, n) in ranked:
+    if args.profile:  # every stage timed, on one extra untimed step (its events slow the round loops)
+        dmsc_gpu.profile_enable(0, 2)
+        step(args.warmup)
+        torch.cuda.synchronize()
+        full = dmsc_gpu.profile_read(0)
+        dmsc_gpu.profile_enable(0, 0)
+        for k, (ms_tot, n) in sorted(full.items(), key=lambda kv: -kv[1][0]):
             print(f"  {k:20s} total {ms_tot:9.3f} ms  launches {n:6d}  avg {ms_tot / max(n, 1) * 1e3:9.1f} us",
                   file=sys.stderr)
-    dom = ranked[0][0] if ranked else None
-    total_ms = max(sum(v[0] for v in prof.values()), 1e-9)
+    step_ms = elapsed * 1e3
     n_calls = args.steps * calls_per_step
     roof = None
-    pers_ms, _ = prof.get("persist_round", (0.0, 0))
+    pers_ms, _ = prof.get("persist_sweep", (0.0, 0))
     if pers_ms and counts:
         # Persistence sweep (all rounds of one call). Compulsory bytes per sweep: per saddle and
         # side 20 B (two extremum indices, saddle value in; pair, persistence out), per extremum
@@ -243,7 +246,7 @@ def main():
                 "traffic": measured_traffic(args.workload, "persistence_sweep"), "bytes_per_launch": int(b_sweep),
                 "avg_launch_us": round(sweep_s * 1e6, 1),
                 "launch_unit": "one sweep (all rounds) per call",
-                "share_of_kernel_time": round(pers_ms / total_ms, 3), "dominant_kernel": dom}
+                "share_of_step_time": round(pers_ms / step_ms, 3)}
     grad_ms, grad_n = prof.get("gradient", (0.0, 0))
     if grad_n:
         # gradient kernel: all fields of a launch (one field per plain call, the whole batch per
@@ -253,7 +256,7 @@ def main():
         ach = per_launch / avg_s / 1e9
         measured = measured_traffic(args.workload, "gradient")
         g = {"achieved": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4), "avg_launch_us": round(avg_s * 1e6, 2),
-             "bytes_per_launch": int(per_launch), "traffic": measured, "share_of_kernel_time": round(grad_ms / total_ms, 4)}
+             "bytes_per_launch": int(per_launch), "traffic": measured, "share_of_step_time": round(grad_ms / step_ms, 4)}
         if roof is None:
             roof = {"bound": "hbm", "kernel": "gradient", "peak": HBM_PEAK_GBS, "unit": "GB/s", **g}
         else:
@@ -314,14 +317,14 @@ def bench_diff(args, torch, dmsc_amd, dmsc_gpu):
     for s in range(max(args.warmup, 1)):
         step(s)
     torch.cuda.synchronize()
-    dmsc_gpu.profile_enable(0, True)
+    dmsc_gpu.profile_enable(0, 1)
     t0 = time.perf_counter()
     for s in range(max(args.warmup, 1), total_steps):
         step(s)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     prof = dmsc_gpu.profile_read(0)
-    dmsc_gpu.profile_enable(0, False)
+    dmsc_gpu.profile_enable(0, 0)
     px = B * H * W
     bw_ms, bw_n = prof.get("persistence_backward", (0.0, 0))
     # backward kernels per field: contributions zeroed + scattered (16 B per slot) and one gather

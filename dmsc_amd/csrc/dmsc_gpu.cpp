@@ -275,7 +275,9 @@ pybind11::object debug_fetch(int device, const std::string& name, int64_t capaci
   return pybind11::cast(t.slice(0, 0, std::min(n, capacity)));
 }
 
-void profile_enable(int device, bool on) { dmsc_profile_enable(context_for(device), on ? 1 : 0); }
+void profile_enable(int device, int level) {
+  TORCH_CHECK(dmsc_profile_enable(context_for(device), level) == DMSC_OK, "dmsc: profile level must be 0, 1 or 2");
+}
 
 void set_param(int device, const std::string& name, int64_t value) {
   TORCH_CHECK(dmsc_set_param(context_for(device), name.c_str(), value) == DMSC_OK,

@@ -583,12 +583,35 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     while (true) {
       if (fork.on && !raw_launched)
         DMSC_CHECK(hipMemcpyAsync(ctx->pinned + kArenaPin, c64, 4 * sizeof(int64_t), hipMemcpyDeviceToHost, st));
-      if (coop && !launch_persist_rounds(dual, ps[0], ps[1], round, kCoopRounds, removed, c32 + 48, pend, st))
-        coop = false;  // no cooperative launch on this device or at this size: one launch per phase
-      // few pending saddles: one workgroup runs every remaining round (at most one round per item)
-      const bool tail = !coop && ctx->tune.tail_items > 0 && pend <= ctx->tune.tail_items;
-      if (tail) launch_persist_tail(dual, ps[0], ps[1], round, (int)pend + 2, removed, c32 + 48, st);
-      if (coop || tail) {
+      bool tail = false, coop_ran = false;
+      {
+        KScope sweep_("persist_sweep", st, true);  // the rounds of this batch, back to back on `st`
+        coop_ran = coop && launch_persist_rounds(dual, ps[0], ps[1], round, kCoopRounds, removed, c32 + 48, pend, st);
+        coop = coop_ran;  // no cooperative launch on this device or at this size: one launch per phase
+        // few pending saddles: one workgroup runs every remaining round (at most one round per item)
+        tail = !coop_ran && ctx->tune.tail_items > 0 && pend <= ctx->tune.tail_items;
+        if (tail) {
+          launch_persist_tail(dual, ps[0], ps[1], round, (int)pend + 2, removed, c32 + 48, st);
+        } else if (!coop_ran) {
+          const int batch = round_batch(round);
+          if (ctx->tune.graph_rounds) {
+            if (!pgraph.fits(batch, pend))
+              pgraph.capture(ctx, batch, pend, [&](hipStream_t cs) {
+                for (int b = 0; b < batch; ++b) {
+                  launch_persist_round(dual, ps[0], ps[1], 0, removed, pend, cs, d_round);
+                  launch_round_inc(d_round, cs);
+                }
+              });
+            fill_i32(d_round, round, 1, st);
+            KScope ks_("persist_round", st);
+            DMSC_CHECK(hipGraphLaunch(pgraph.exec, st));
+            round += batch;
+          } else {
+            for (int b = 0; b < batch; ++b, ++round) launch_persist_round(dual, ps[0], ps[1], round, removed, pend, st);
+          }
+        }
+      }
+      if (coop_ran || tail) {
         DMSC_CHECK(hipMemcpyAsync(ctx->pinned, c32 + 48, 4, hipMemcpyDeviceToHost, st));
         DMSC_CHECK(hipMemcpyAsync((char*)ctx->pinned + 8, sb[0].cnt, 8, hipMemcpyDeviceToHost, st));
         DMSC_CHECK(hipMemcpyAsync((char*)ctx->pinned + 16, sb[1].cnt, 8, hipMemcpyDeviceToHost, st));
@@ -598,22 +621,6 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
         const int32_t left[2] = {h[2 + (round & 1)], h[4 + (round & 1)]};
         std::memcpy(ctx->pinned, left, 8);
       } else {
-        const int batch = round_batch(round);
-        if (ctx->tune.graph_rounds) {
-          if (!pgraph.fits(batch, pend))
-            pgraph.capture(ctx, batch, pend, [&](hipStream_t cs) {
-              for (int b = 0; b < batch; ++b) {
-                launch_persist_round(dual, ps[0], ps[1], 0, removed, pend, cs, d_round);
-                launch_round_inc(d_round, cs);
-              }
-            });
-          fill_i32(d_round, round, 1, st);
-          KScope ks_("persist_round", st);
-          DMSC_CHECK(hipGraphLaunch(pgraph.exec, st));
-          round += batch;
-        } else {
-          for (int b = 0; b < batch; ++b, ++round) launch_persist_round(dual, ps[0], ps[1], round, removed, pend, st);
-        }
         DMSC_CHECK(hipMemcpyAsync(ctx->pinned, sb[0].cnt + (round & 1), 4, hipMemcpyDeviceToHost, st));
         DMSC_CHECK(hipMemcpyAsync((char*)ctx->pinned + 4, sb[1].cnt + (round & 1), 4, hipMemcpyDeviceToHost, st));
         DMSC_CHECK(hipStreamSynchronize(st));
@@ -911,7 +918,7 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
   }
   fork.join();
   readback(ctx, d_err, sizeof(int32_t), st);
-  if (ctx->times.on) ctx->times.collect();
+  if (ctx->times.level) ctx->times.collect();
   if (*(const int32_t*)ctx->pinned != 0)
     throw Fail{DMSC_ERR_INTERNAL, "device consistency check failed (flags " +
                                       std::to_string(*(const int32_t*)ctx->pinned) + ")"};
@@ -1067,7 +1074,7 @@ dmsc_status dmsc_persistence_backward(dmsc_context* ctx, int32_t B, const float*
       launch_persistence_backward((flags & DMSC_DUAL) != 0, d_fields[b], H[b], W[b], pv, max_persistence,
                                   d_grad_scale + b, contrib, d_grad[b], (hipStream_t)stream);
     }
-    if (ctx->times.on) {
+    if (ctx->times.level) {
       DMSC_CHECK(hipStreamSynchronize((hipStream_t)stream));
       ctx->times.collect();
     }
@@ -1110,7 +1117,8 @@ const char* dmsc_last_error(dmsc_context* ctx) { return ctx ? ctx->err.c_str() :
 
 dmsc_status dmsc_profile_enable(dmsc_context* ctx, int32_t on) {
   if (!ctx) return DMSC_ERR_INVALID_ARGUMENT;
-  ctx->times.on = on != 0;
+  if (on < 0 || on > 2) return DMSC_ERR_INVALID_ARGUMENT;
+  ctx->times.level = on;
   ctx->times.acc.clear();
   return DMSC_OK;
 }

@@ -72,7 +72,7 @@ __device__ __forceinline__ void report(int32_t* err, int32_t code) {
 // Optional per-kernel timing with HIP events on the launch stream (enabled per context; used by
 // bench.py to measure the dominant kernel's average duration live). Events come from a reusable pool.
 struct KernelTimes {
-  bool on = false;
+  int level = 0;  // 0: off, 1: key scopes only (a few events per call), 2: every scope
   struct Rec {
     const char* name;
     hipEvent_t a, b;
@@ -130,12 +130,15 @@ inline const Tunables& tune() {
   return tl_tune ? *tl_tune : defaults;
 }
 
+// Events around a stretch of launches on one stream. Timing events are stream markers that cost a
+// few microseconds of GPU time each, so only `key` scopes record at level 1.
 struct KScope {
   KernelTimes* t;
   const char* name;
   hipStream_t st;
   hipEvent_t a{};
-  KScope(const char* n, hipStream_t s) : t(tl_times && tl_times->on ? tl_times : nullptr), name(n), st(s) {
+  KScope(const char* n, hipStream_t s, bool key = false)
+      : t(tl_times && tl_times->level >= (key ? 1 : 2) ? tl_times : nullptr), name(n), st(s) {
     if (t) {
       a = t->take();
       DMSC_CHECK(hipEventRecord(a, st));

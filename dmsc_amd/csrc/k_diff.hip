@@ -72,7 +72,7 @@ void fill_f32_zero(float* p, int64_t n, hipStream_t st);
 
 void launch_persistence_backward(bool dual, const float* f, int H, int W, const PairView& pv, float max_pers,
                                  const float* gscale, float* contrib, float* grad, hipStream_t st) {
-  KScope ks_("persistence_backward", st);
+  KScope ks_("persistence_backward", st, true);
   const int64_t S = (int64_t)(H + 1) * (W + 1), N = (int64_t)H * W;
   fill_f32_zero(contrib, 4 * S, st);
   if (pv.n_sad > 0) {

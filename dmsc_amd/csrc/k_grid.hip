@@ -224,7 +224,7 @@ __global__ __launch_bounds__(256) void k_gradient(Batch bt, const TileRef* __res
 
 void launch_gradient(bool dual, const Batch& bt, const TileRef* tiles, int64_t ntiles, int32_t* pw, int32_t* fsucc,
                      int32_t* vsucc, uint32_t* planes, int64_t PW, uint64_t* pjf, uint64_t* pjv, hipStream_t st) {
-  KScope ks_("gradient", st);
+  KScope ks_("gradient", st, true);
   if (ntiles == 0) return;
   if (dual)
     hipLaunchKernelGGL(k_gradient<true>, dim3((unsigned)ntiles), dim3(256), 0, st, bt, tiles, pw, fsucc, vsucc, planes, PW,

@@ -115,10 +115,13 @@ dmsc_status dmsc_persistence_backward(dmsc_context* ctx, int32_t B, const float*
                                       float max_persistence, const float* d_grad_scale, float* const* d_grad,
                                       dmsc_stream_t stream);
 
-/* Per-kernel timing with HIP events recorded on the work stream around each kernel launch (no
- * reference counterpart; the benchmark uses it to measure the dominant kernel's duration live).
- * dmsc_profile_enable(ctx, 1) clears and starts accumulating; entries are (kernel name, total ms,
- * launches) over all calls since. */
+/* Kernel timing with HIP events recorded on the work stream around launches (no reference
+ * counterpart; the benchmark uses it to measure the dominant kernels' durations live).
+ * dmsc_profile_enable(ctx, level) clears and starts accumulating: level 1 times only the key
+ * stretches ("gradient", "persist_sweep" = each batch of persistence rounds launched back to back,
+ * "persistence_backward"), level 2 times every stage (each timing event is a stream marker costing
+ * a few microseconds of GPU time, so level 2 slows the round loops down), 0 stops. Entries are
+ * (name, total ms, launches) over all calls since. */
 dmsc_status dmsc_profile_enable(dmsc_context* ctx, int32_t on);
 int32_t dmsc_profile_count(dmsc_context* ctx);
 dmsc_status dmsc_profile_entry(dmsc_context* ctx, int32_t index, const char** name, double* total_ms,
