@@ -162,6 +162,30 @@ __device__ __forceinline__ void post_min(uint64_t* a, uint64_t key, bool check) 
   atomicMin((unsigned long long*)a, (unsigned long long)key);
 }
 
+// The (up to) two root posts of one round item: atomic min of `key` into *a and *b (null: none).
+struct RootPosts {
+  uint64_t* a = nullptr;
+  uint64_t* b = nullptr;
+  uint64_t key = 0;
+};
+
+// Atomic min posts of a whole wave, combined first: a lane whose address equals the previous
+// lane's takes the smaller key (segmented min over each run of equal addresses, by doubling), and
+// only the last lane of a run issues the atomic. Pays off when neighbouring lanes often post to
+// one address. Every lane of the wave must call it; lanes with nothing to post pass a null address.
+__device__ __forceinline__ void wave_post_min(uint64_t* addr, uint64_t key) {
+  const int lane = threadIdx.x & 63;
+  const unsigned long long a = (unsigned long long)addr;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const unsigned long long pa = __shfl_up(a, o, 64);
+    const unsigned long long pk = __shfl_up((unsigned long long)key, o, 64);
+    if (lane >= o && pa == a && pk < key) key = pk;
+  }
+  const unsigned long long na = __shfl_down(a, 1, 64);
+  if (addr && (lane == 63 || na != a)) atomicMin((unsigned long long*)addr, (unsigned long long)key);
+}
+
 // Barrier for kernels whose phases hand data to each other through global memory inside one
 // workgroup: agent-scope release before and acquire after, so the next phase reads every store and
 // atomic of the previous one (the acquire drops lines the CU's L1 may still hold).

@@ -190,10 +190,10 @@ __device__ __forceinline__ void geo_root(const GeoState& g, int x, int& root, in
 }
 
 // worklist item: (cancellation, key, arc target 0, arc target 1); results go to rr at the item's
-// list position
-// In round 0 the forest has no links yet: every target is its own root at length 0.
-__device__ __forceinline__ void geo_find_one(const ArcSide& A, const GeoState& g, const int4 it, int64_t j,
-                                             uint64_t tag, bool first) {
+// list position. In round 0 the forest has no links yet: every target is its own root at length 0.
+// Returns the watch posts on the two roots.
+__device__ __forceinline__ RootPosts geo_find_one(const ArcSide& A, const GeoState& g, const int4 it, int64_t j,
+                                                  uint64_t tag, bool first) {
   int R0 = it.z, R1 = it.w;
   int32_t L0 = 0, L1 = 0;
   if (!first) {
@@ -201,9 +201,23 @@ __device__ __forceinline__ void geo_find_one(const ArcSide& A, const GeoState& g
     geo_root(g, it.w, R1, L1);
   }
   *reinterpret_cast<int4*>(g.rr + 4 * j) = make_int4(R0, R1, L0, L1);
-  const uint64_t key = tag | (uint32_t)it.y;
-  if (R0 >= 0) post_min(&g.watch[R0], key, !first);
-  if (R1 >= 0) post_min(&g.watch[R1], key, !first);
+  RootPosts p;
+  p.key = tag | (uint32_t)it.y;
+  if (R0 >= 0) p.a = &g.watch[R0];
+  if (R1 >= 0) p.b = &g.watch[R1];
+  return p;
+}
+
+// Round 0 visits the cancellations in saddle-id order: neighbouring lanes often post to the same
+// extremum, so a wave combines its posts first. Later rounds read before their atomics.
+__device__ __forceinline__ void geo_post(const RootPosts& p, bool first) {
+  if (first) {
+    wave_post_min(p.a, p.key);
+    wave_post_min(p.b, p.key);
+  } else {
+    if (p.a) post_min(p.a, p.key, true);
+    if (p.b) post_min(p.b, p.key, true);
+  }
 }
 
 __device__ __forceinline__ void geo_find_phase(const ArcSide& A, const GeoState& g, int round) {
@@ -211,8 +225,12 @@ __device__ __forceinline__ void geo_find_phase(const ArcSide& A, const GeoState&
   const int n = g.cnt[cur];
   if (blockIdx.x == 0 && threadIdx.x == 0) g.cnt[cur ^ 1] = 0;
   const uint64_t tag = (uint64_t)(0xFFFFFFFFu - (uint32_t)round) << 32;
-  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x)
-    geo_find_one(A, g, g.wl[cur][j], j, tag, round == 0);
+  DMSC_WAVE_LOOP(jw, (int64_t)n) {
+    const int64_t j = jw_base + (threadIdx.x & 63);
+    RootPosts p;
+    if (j < n) p = geo_find_one(A, g, g.wl[cur][j], j, tag, round == 0);
+    geo_post(p, round == 0);
+  }
 }
 
 // Both traced sides of one round in one launch: side 0's items first, then side 1's.
@@ -226,10 +244,14 @@ __global__ void k_geo_find2(ArcSide A0, GeoState G0, ArcSide A1, GeoState G1, in
     if (live & 2) G1.cnt[cur ^ 1] = 0;
   }
   const uint64_t tag = (uint64_t)(0xFFFFFFFFu - (uint32_t)round) << 32;
-  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < (int64_t)n0 + n1;
-       j += (int64_t)gridDim.x * blockDim.x) {
-    if (j >= n0) geo_find_one(A1, G1, G1.wl[cur][j - n0], j - n0, tag, round == 0);
-    else geo_find_one(A0, G0, G0.wl[cur][j], j, tag, round == 0);
+  DMSC_WAVE_LOOP(jw, (int64_t)n0 + n1) {
+    const int64_t j = jw_base + (threadIdx.x & 63);
+    RootPosts p;
+    if (j < (int64_t)n0 + n1) {
+      if (j >= n0) p = geo_find_one(A1, G1, G1.wl[cur][j - n0], j - n0, tag, round == 0);
+      else p = geo_find_one(A0, G0, G0.wl[cur][j], j, tag, round == 0);
+    }
+    geo_post(p, round == 0);
   }
 }
 __global__ void k_geo_find(ArcSide A, GeoState g, int round) { geo_find_phase(A, g, round); }
@@ -387,8 +409,10 @@ __global__ __launch_bounds__(kTailThreads) void k_geo_tail(ArcSide A0, GeoState 
     const uint64_t tag = (uint64_t)(0xFFFFFFFFu - (uint32_t)r) << 32;
     if (threadIdx.x == 0) s_next[0] = s_next[1] = 0;
     for (int j = threadIdx.x; j < n0 + n1; j += kTailThreads) {
-      if (j >= n0) geo_find_one(A1, G1, G1.wl[cur][j - n0], j - n0, tag, r == 0);
-      else geo_find_one(A0, G0, G0.wl[cur][j], j, tag, r == 0);
+      const RootPosts p = j >= n0 ? geo_find_one(A1, G1, G1.wl[cur][j - n0], j - n0, tag, r == 0)
+                                  : geo_find_one(A0, G0, G0.wl[cur][j], j, tag, r == 0);
+      if (p.a) post_min(p.a, p.key, r > 0);
+      if (p.b) post_min(p.b, p.key, r > 0);
     }
     phase_barrier();
     for (int j = threadIdx.x; j < n0 + n1; j += kTailThreads) {

@@ -42,10 +42,11 @@ __device__ __forceinline__ int uf_find(int32_t* __restrict__ p, int x, int32_t* 
 
 // rr[2i] = dying root (-2: cycle); rr[2i+1] = (surviving root + 1) << 1 | (dying root came from c[2i+1])
 // Round 0 starts from the saddle's extrema, later rounds from the roots it saw when it last waited
-// (the same components, fewer links to walk); from round 1 on posts read before their atomic.
+// (the same components, fewer links to walk). Returns the two minkey posts (null: none).
 template <bool DUAL>
-__device__ __forceinline__ void persist_find_one(const PersistSide& s, const int i, int64_t j, uint64_t tag,
-                                                 bool first) {
+__device__ __forceinline__ RootPosts persist_find_one(const PersistSide& s, const int i, int64_t j, uint64_t tag,
+                                                      bool first) {
+  RootPosts p;
   const int2 cc = *reinterpret_cast<const int2*>((first ? s.c : s.croot) + 2 * i);
   // round 0: no merges yet, every extremum is its own root
   const int ra = first ? (cc.x < 0 ? -1 : cc.x) : uf_find(s.par_g, cc.x, s.err);
@@ -54,7 +55,7 @@ __device__ __forceinline__ void persist_find_one(const PersistSide& s, const int
     s.pair[i] = -1;
     s.pers[i] = -1.0f;
     s.rr[2 * j] = -2;
-    return;
+    return p;
   }
   bool b_dies;
   if (ra == -1) {
@@ -70,11 +71,15 @@ __device__ __forceinline__ void persist_find_one(const PersistSide& s, const int
   }
   const int dying = b_dies ? rb : ra, surv = b_dies ? ra : rb;
   *reinterpret_cast<int2*>(s.rr + 2 * j) = make_int2(dying, ((surv + 1) << 1) | (b_dies ? 1 : 0));
-  const uint64_t key = tag | (uint32_t)s.pos[i];
-  post_min(&s.minkey[dying], key, !first);
-  if (surv >= 0) post_min(&s.minkey[surv], key, !first);
+  p.key = tag | (uint32_t)s.pos[i];
+  p.a = &s.minkey[dying];
+  if (surv >= 0) p.b = &s.minkey[surv];
+  return p;
 }
 
+// Round 0 visits the saddles in id order, so the lanes of a wave hold neighbouring saddles that
+// often share an extremum: their posts are combined per wave first. Later rounds read before
+// their atomics (hot roots of big components).
 template <bool DUAL>
 __device__ __forceinline__ void persist_find_phase(const PersistSide& s0, const PersistSide& s1, int round) {
   const int cur = round & 1;
@@ -84,11 +89,23 @@ __device__ __forceinline__ void persist_find_phase(const PersistSide& s0, const 
     s1.cnt[cur ^ 1] = 0;
   }
   const uint64_t tag = (uint64_t)(0xFFFFFFFFu - (uint32_t)round) << 32;
-  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n0 + n1; j += (int64_t)gridDim.x * blockDim.x) {
+  const bool first = round == 0;
+  DMSC_WAVE_LOOP(jw, (int64_t)n0 + n1) {
     // worklists hold saddle indices in saddle-id order (the sweep position is the merge key); the
     // round's decisions go to rr at the item's list position
-    if (j >= n0) persist_find_one<DUAL>(s1, s1.wl[cur][j - n0], j - n0, tag, round == 0);
-    else persist_find_one<DUAL>(s0, s0.wl[cur][j], j, tag, round == 0);
+    const int64_t j = jw_base + (threadIdx.x & 63);
+    RootPosts p;
+    if (j < (int64_t)n0 + n1) {
+      if (j >= n0) p = persist_find_one<DUAL>(s1, s1.wl[cur][j - n0], j - n0, tag, first);
+      else p = persist_find_one<DUAL>(s0, s0.wl[cur][j], j, tag, first);
+    }
+    if (first) {
+      wave_post_min(p.a, p.key);
+      wave_post_min(p.b, p.key);
+    } else {
+      if (p.a) post_min(p.a, p.key, true);
+      if (p.b) post_min(p.b, p.key, true);
+    }
   }
 }
 template <bool DUAL>
@@ -212,8 +229,10 @@ __global__ __launch_bounds__(kTailThreads) void k_persist_tail(PersistSide s0, P
     const uint64_t tag = (uint64_t)(0xFFFFFFFFu - (uint32_t)r) << 32;
     if (threadIdx.x == 0) s_next[0] = s_next[1] = 0;
     for (int j = threadIdx.x; j < n0 + n1; j += kTailThreads) {
-      if (j >= n0) persist_find_one<DUAL>(s1, s1.wl[cur][j - n0], j - n0, tag, r == 0);
-      else persist_find_one<DUAL>(s0, s0.wl[cur][j], j, tag, r == 0);
+      const RootPosts p = j >= n0 ? persist_find_one<DUAL>(s1, s1.wl[cur][j - n0], j - n0, tag, r == 0)
+                                  : persist_find_one<DUAL>(s0, s0.wl[cur][j], j, tag, r == 0);
+      if (p.a) post_min(p.a, p.key, r > 0);
+      if (p.b) post_min(p.b, p.key, r > 0);
     }
     phase_barrier();
     for (int j = threadIdx.x; j < n0 + n1; j += kTailThreads) {
