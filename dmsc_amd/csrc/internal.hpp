@@ -173,7 +173,8 @@ struct RootPosts {
 // lane's takes the smaller key (segmented min over each run of equal addresses, by doubling), and
 // only the last lane of a run issues the atomic. Pays off when neighbouring lanes often post to
 // one address. Every lane of the wave must call it; lanes with nothing to post pass a null address.
-__device__ __forceinline__ void wave_post_min(uint64_t* addr, uint64_t key) {
+// `check`: the posting lane reads first and skips an atomic that could not lower the value.
+__device__ __forceinline__ void wave_post_min(uint64_t* addr, uint64_t key, bool check) {
   const int lane = threadIdx.x & 63;
   const unsigned long long a = (unsigned long long)addr;
 #pragma unroll
@@ -183,7 +184,7 @@ __device__ __forceinline__ void wave_post_min(uint64_t* addr, uint64_t key) {
     if (lane >= o && pa == a && pk < key) key = pk;
   }
   const unsigned long long na = __shfl_down(a, 1, 64);
-  if (addr && (lane == 63 || na != a)) atomicMin((unsigned long long*)addr, (unsigned long long)key);
+  if (addr && (lane == 63 || na != a)) post_min(addr, key, check);
 }
 
 // Barrier for kernels whose phases hand data to each other through global memory inside one

@@ -208,16 +208,11 @@ __device__ __forceinline__ RootPosts geo_find_one(const ArcSide& A, const GeoSta
   return p;
 }
 
-// Round 0 visits the cancellations in saddle-id order: neighbouring lanes often post to the same
-// extremum, so a wave combines its posts first. Later rounds read before their atomics.
+// Worklists keep saddle-id order: neighbouring lanes often post to the same extremum, so a wave
+// combines its posts first; from round 1 on the posting lane also reads before its atomic.
 __device__ __forceinline__ void geo_post(const RootPosts& p, bool first) {
-  if (first) {
-    wave_post_min(p.a, p.key);
-    wave_post_min(p.b, p.key);
-  } else {
-    if (p.a) post_min(p.a, p.key, true);
-    if (p.b) post_min(p.b, p.key, true);
-  }
+  wave_post_min(p.a, p.key, !first);
+  wave_post_min(p.b, p.key, !first);
 }
 
 __device__ __forceinline__ void geo_find_phase(const ArcSide& A, const GeoState& g, int round) {

@@ -77,9 +77,9 @@ __device__ __forceinline__ RootPosts persist_find_one(const PersistSide& s, cons
   return p;
 }
 
-// Round 0 visits the saddles in id order, so the lanes of a wave hold neighbouring saddles that
-// often share an extremum: their posts are combined per wave first. Later rounds read before
-// their atomics (hot roots of big components).
+// Worklists keep saddle-id order, so the lanes of a wave hold nearby saddles that often share an
+// extremum: their posts are combined per wave first. From round 1 on the posting lane also reads
+// before its atomic (hot roots of big components).
 template <bool DUAL>
 __device__ __forceinline__ void persist_find_phase(const PersistSide& s0, const PersistSide& s1, int round) {
   const int cur = round & 1;
@@ -99,13 +99,8 @@ __device__ __forceinline__ void persist_find_phase(const PersistSide& s0, const 
       if (j >= n0) p = persist_find_one<DUAL>(s1, s1.wl[cur][j - n0], j - n0, tag, first);
       else p = persist_find_one<DUAL>(s0, s0.wl[cur][j], j, tag, first);
     }
-    if (first) {
-      wave_post_min(p.a, p.key);
-      wave_post_min(p.b, p.key);
-    } else {
-      if (p.a) post_min(p.a, p.key, true);
-      if (p.b) post_min(p.b, p.key, true);
-    }
+    wave_post_min(p.a, p.key, !first);
+    wave_post_min(p.b, p.key, !first);
   }
 }
 template <bool DUAL>
