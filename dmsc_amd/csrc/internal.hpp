@@ -154,11 +154,11 @@ struct KScope {
 };
 
 // 64-bit atomic min of a value that only ever decreases. With `check`, the current value is read
-// first (bypassing the CU's L1) and the atomic is skipped when it could not lower it: the atomics
-// of many saddles posting to one hot root then shrink to the few that still lower it. The read is
-// conservative: a stale value is never below the current one.
+// first (a plain load: it may come from a cache) and the atomic is skipped when it could not lower
+// it: the atomics of many saddles posting to one hot root then shrink to the few that still lower
+// it. The read is conservative: a stale value is never below the current one.
 __device__ __forceinline__ void post_min(uint64_t* a, uint64_t key, bool check) {
-  if (check && __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= key) return;
+  if (check && __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT) <= key) return;
   atomicMin((unsigned long long*)a, (unsigned long long)key);
 }
 

@@ -153,8 +153,9 @@ void launch_geo_items(int64_t nc, const int32_t* cs, const int32_t* key, const i
 
 // Root of x in the geometry forest and the summed link length on the way, by path halving over
 // gjump: (ancestor J, length to J) pairs. The forest only grows at roots (a link is only ever
-// added from a root), so a shortcut stays valid once written; finds run in their own kernel, so
-// concurrent shortcuts are all consistent 64-bit pairs.
+// added from a root), so a shortcut stays valid once written; finds run in their own kernel (or
+// phase), so concurrent shortcuts are all consistent 64-bit pairs, and an older pair read from a
+// cache is still valid: the accesses are single 64-bit loads and stores without a coherence scope.
 constexpr uint32_t kVoid = 0xFFFFFFFFu;
 __device__ __forceinline__ void geo_root(const GeoState& g, int x, int& root, int32_t& len) {
   int32_t acc = 0;
@@ -163,7 +164,7 @@ __device__ __forceinline__ void geo_root(const GeoState& g, int x, int& root, in
       report(g.err, ERR_GEO_CAP);
       break;
     }
-    const uint64_t cur = __hip_atomic_load(&g.gjump[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint64_t cur = __hip_atomic_load(&g.gjump[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
     const uint32_t J = (uint32_t)cur;
     const int32_t L = (int32_t)(cur >> 32);
     if (J == (uint32_t)x) break;
@@ -172,7 +173,7 @@ __device__ __forceinline__ void geo_root(const GeoState& g, int x, int& root, in
       x = -1;
       break;
     }
-    const uint64_t nx = __hip_atomic_load(&g.gjump[J], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint64_t nx = __hip_atomic_load(&g.gjump[J], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
     const uint32_t JJ = (uint32_t)nx;
     if (JJ == J) {  // J is the root
       acc = sat_add(acc, L);
@@ -181,7 +182,7 @@ __device__ __forceinline__ void geo_root(const GeoState& g, int x, int& root, in
     }
     const int32_t L2 = sat_add(L, (int32_t)(nx >> 32));
     __hip_atomic_store(&g.gjump[x], (uint64_t)JJ | ((uint64_t)(uint32_t)L2 << 32), __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
+                       __HIP_MEMORY_SCOPE_WAVEFRONT);
     acc = sat_add(acc, L2);
     x = JJ == kVoid ? -1 : (int)JJ;
   }
@@ -523,20 +524,25 @@ __global__ void k_geo_q_init(GeoState g, int64_t next, uint64_t* __restrict__ pj
     pj[e] = (uint64_t)J | ((uint64_t)D << 32);
   }
 }
-__global__ void k_pj_sat(uint64_t* __restrict__ pj, int64_t n) {
+// Saturating pointer jumping (same access rules as k_pj: unscoped single 64-bit accesses).
+__global__ void k_pj_sat(uint64_t* __restrict__ pj, int64_t n, int32_t* __restrict__ err) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    uint64_t cur = __hip_atomic_load(&pj[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint64_t cur = __hip_atomic_load(&pj[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
     uint32_t J = (uint32_t)cur;
     int32_t D = (int32_t)(cur >> 32);
     if (J == (uint32_t)i) continue;
-    while (true) {
-      const uint64_t nx = __hip_atomic_load(&pj[J], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int it = 0;; ++it) {
+      if (it >= kLoopCap) {  // only a cyclic forest can get here
+        report(err, ERR_GEO_CAP);
+        break;
+      }
+      const uint64_t nx = __hip_atomic_load(&pj[J], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
       const uint32_t JJ = (uint32_t)nx;
       if (JJ == J) break;
       D = sat_add(D, (int32_t)(nx >> 32));
       J = JJ;
       __hip_atomic_store(&pj[i], (uint64_t)J | ((uint64_t)(uint32_t)D << 32), __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
+                         __HIP_MEMORY_SCOPE_WAVEFRONT);
     }
   }
 }
@@ -544,7 +550,7 @@ void launch_geo_q(const GeoState& g, int64_t next, uint64_t* pjq, hipStream_t st
   KScope ks_("geo_q", st);
   hipLaunchKernelGGL(k_geo_q_init, dim3(grid_for(next + 1, 256, 16384)), dim3(256), 0, st, g, next, pjq);
   DMSC_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_pj_sat, dim3(grid_for(next + 1, 256, 16384)), dim3(256), 0, st, pjq, next + 1);
+  hipLaunchKernelGGL(k_pj_sat, dim3(grid_for(next + 1, 256, 16384)), dim3(256), 0, st, pjq, next + 1, g.err);
   DMSC_LAUNCH_CHECK();
 }
 

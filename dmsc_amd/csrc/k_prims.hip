@@ -263,9 +263,12 @@ __global__ void k_pj_init(const int32_t* __restrict__ succ, uint64_t* __restrict
   }
 }
 
+// The loads and stores are single 64-bit accesses without a coherence scope: a read may return an
+// older entry (from this CU's caches), which is still a valid (ancestor, distance) pair, so it only
+// costs extra steps; roots never change, so every walk still ends at the root.
 __global__ void k_pj(uint64_t* __restrict__ pj, int64_t n, int32_t* __restrict__ err) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    uint64_t cur = __hip_atomic_load(&pj[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint64_t cur = __hip_atomic_load(&pj[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
     uint32_t J = (uint32_t)cur, D = (uint32_t)(cur >> 32);
     if (J == (uint32_t)i) continue;
     for (int it = 0;; ++it) {
@@ -273,12 +276,12 @@ __global__ void k_pj(uint64_t* __restrict__ pj, int64_t n, int32_t* __restrict__
         report(err, ERR_PJ_CAP);
         break;
       }
-      const uint64_t nx = __hip_atomic_load(&pj[J], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const uint64_t nx = __hip_atomic_load(&pj[J], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
       const uint32_t JJ = (uint32_t)nx;
       if (JJ == J) break;
       D += (uint32_t)(nx >> 32);
       J = JJ;
-      __hip_atomic_store(&pj[i], (uint64_t)J | ((uint64_t)D << 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&pj[i], (uint64_t)J | ((uint64_t)D << 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
     }
   }
 }
