@@ -25,7 +25,7 @@ TUNING_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning")
 DEFAULTS = {"round_grid": 2048, "small_list": 65536, "rounds_first": 4, "rounds_max": 16, "expand_levels": 8,
             "graph_rounds": 0, "tail_items": 2048}
 CANDIDATES = {
-    "round_grid": [512, 1024, 2048, 4096, 8192],
+    "round_grid": [512, 1024, 2048, 4096, 8192, 16384],
     "small_list": [4096, 16384, 65536, 262144],
     "rounds_first": [2, 4, 8],
     "rounds_max": [8, 16, 32, 64],
@@ -134,4 +134,14 @@ def tune(device=0, size=2048, n_small=64, reps=3, save=True, candidates=None, lo
 
 
 if __name__ == "__main__":
-    print(json.dumps(tune()))
+    import argparse
+
+    ap = argparse.ArgumentParser(description="pick the scheduling parameters for this GPU")
+    ap.add_argument("--size", type=int, default=2048, help="side of the square gaussian-mixture field")
+    ap.add_argument("--n-small", type=int, default=64, help="fields in the mixed batch")
+    ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--no-save", action="store_true")
+    a = ap.parse_args()
+    res = tune(size=a.size, n_small=a.n_small, reps=a.reps, save=not a.no_save)
+    res["note"] = f"written by dmsc_amd.autotune.tune(size={a.size}, n_small={a.n_small}, reps={a.reps})"
+    print(json.dumps(res))
