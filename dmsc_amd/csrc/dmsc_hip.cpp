@@ -36,19 +36,19 @@ enum : int {
   A_CMAX, A_CMIN, A_LMAX, A_LMIN, A_META, A_BASES32, A_SADFIELD, A_CONTRIB, A_SURV1, A_SURVR1,
   A_KEYS, A_KEYS2, A_VALS2, A_ASC, A_ORDMIN, A_ORDMAX, A_REMOVED, A_POSMIN, A_POSMAX,
   A_SIDE0,  // per persistence side: kSideSlots slots each
-  A_SIDE_END = A_SIDE0 + 2 * 14,
+  A_SIDE_END = A_SIDE0 + 2 * 13,
   A_ASM0 = A_SIDE_END,  // assembly: counts, prefixes, sad_to_out, survivor lists (12 slots)
   A_ASM_END = A_ASM0 + 12,
   A_ARC0 = A_ASM_END,  // per arc side: kArcSlots slots each
-  A_ARC_END = A_ARC0 + 2 * 22,
+  A_ARC_END = A_ARC0 + 2 * 21,
 };
 
-enum SideSlot { P_PARG, P_PART, P_ALIVE, P_MINKEY, P_PAIR, P_PERS, P_CANCEL, P_RR, P_WL0, P_WL1, P_CNT, P_ROOT, P_TOOUT,
+enum SideSlot { P_PARG, P_PART, P_ALIVE, P_MINKEY, P_PAIR, P_PERS, P_CANCEL, P_RR, P_WL, P_CNT, P_ROOT, P_TOOUT,
                 P_CROOT, kSideSlots };
 static_assert(A_SIDE_END - A_SIDE0 == 2 * kSideSlots, "side slot count");
 enum ArcSlot {
   G_BLEN, G_BOFF, G_ARENA, G_FLAG, G_PRE, G_CS0, G_CD0, G_KEYS, G_SLOT, G_KEYS2, G_SLOT2, G_CS, G_CD, G_DEADC,
-  G_LINK, G_DONE, G_WATCH, G_RR, G_WL0, G_WL1, G_CNT, G_JUMP, kArcSlots
+  G_LINK, G_DONE, G_WATCH, G_RR, G_WL, G_CNT, G_JUMP, kArcSlots
 };
 static_assert(A_ARC_END - A_ARC0 == 2 * kArcSlots, "arc slot count");
 enum ArcSlot2 { H_PJQ = 0, H_FLEN, H_PERPOS, H_POSPRE, H_SEGA, H_SEGB, H_RAW, H_NSEG, H_CUMR, H_CUMC, H_ARCB };
@@ -120,7 +120,7 @@ void readback(dmsc_context* c, const void* dev, size_t bytes, hipStream_t st) {
 }
 
 struct SideBufs {
-  int32_t *par_g, *par_t, *pair, *cancel, *rr, *wl0, *wl1, *cnt, *root, *to_out, *croot;
+  int32_t *par_g, *par_t, *pair, *cancel, *rr, *wl, *cnt, *root, *to_out, *croot;
   uint8_t* alive;
   uint64_t* minkey;
   float* pers;
@@ -137,8 +137,7 @@ SideBufs side_bufs(Arena& ar, int side, int64_t next, int64_t ns) {
   s.pers = ar.get<float>(b + P_PERS, ns);
   s.cancel = ar.get<int32_t>(b + P_CANCEL, ns);
   s.rr = ar.get<int32_t>(b + P_RR, 2 * ns);
-  s.wl0 = ar.get<int32_t>(b + P_WL0, ns);
-  s.wl1 = ar.get<int32_t>(b + P_WL1, ns);
+  s.wl = ar.get<int32_t>(b + P_WL, 2 * ns);
   s.cnt = ar.get<int32_t>(b + P_CNT, 2);
   s.root = ar.get<int32_t>(b + P_ROOT, next);
   s.to_out = ar.get<int32_t>(b + P_TOOUT, next);
@@ -548,7 +547,7 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     fill_u64(sb[s].minkey, ~0ull, next, st);
     fill_i32(sb[s].pair, -1, nsad, st);
     fill_i32(sb[s].cancel, -1, nsad, st);
-    iota_i32(sb[s].wl0, nsad, st);
+    iota_i32(sb[s].wl, nsad, st);  // list 0: every saddle
     fill_i32(sb[s].cnt, (int32_t)nsad, 1, st);
     fill_i32(sb[s].cnt + 1, 0, 1, st);
     PersistSide& p = ps[s];
@@ -565,8 +564,8 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     p.cancel_d = record[s] ? sb[s].cancel : nullptr;
     p.rr = sb[s].rr;
     p.croot = sb[s].croot;
-    p.wl[0] = sb[s].wl0;
-    p.wl[1] = sb[s].wl1;
+    p.wl = sb[s].wl;
+    p.wls = nsad;
     p.cnt = sb[s].cnt;
     p.err = d_err;
     p.thr = d_thr;
@@ -756,14 +755,14 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     g.link = ar.get<int4>(b + G_LINK, next);
     g.watch = ar.get<uint64_t>(b + G_WATCH, next);
     g.rr = ar.get<int32_t>(b + G_RR, 4 * nc);
-    g.wl[0] = ar.get<int4>(b + G_WL0, nc);
-    g.wl[1] = ar.get<int4>(b + G_WL1, nc);
+    g.wl = ar.get<int4>(b + G_WL, 2 * nc);
+    g.wls = nc;
     g.cnt = ar.get<int32_t>(b + G_CNT, 2);
     g.err = d_err;
     g.gjump = ar.get<uint64_t>(b + G_JUMP, next);
     launch_geo_init(g, next, st);  // every extremum its own root
     fill_u64(g.watch, ~0ull, next, st);
-    launch_geo_items(nc, cs0, ckey, w.A.tinit, g.wl[0], st);
+    launch_geo_items(nc, cs0, ckey, w.A.tinit, g.list(0), st);
     fill_i32(g.cnt, (int32_t)nc, 1, st);
     fill_i32(g.cnt + 1, 0, 1, st);
   }

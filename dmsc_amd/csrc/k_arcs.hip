@@ -224,7 +224,7 @@ __device__ __forceinline__ void geo_find_phase(const ArcSide& A, const GeoState&
   DMSC_WAVE_LOOP(jw, (int64_t)n) {
     const int64_t j = jw_base + (threadIdx.x & 63);
     RootPosts p;
-    if (j < n) p = geo_find_one(A, g, g.wl[cur][j], j, tag, round == 0);
+    if (j < n) p = geo_find_one(A, g, g.list(cur)[j], j, tag, round == 0);
     geo_post(p, round == 0);
   }
 }
@@ -244,8 +244,8 @@ __global__ void k_geo_find2(ArcSide A0, GeoState G0, ArcSide A1, GeoState G1, in
     const int64_t j = jw_base + (threadIdx.x & 63);
     RootPosts p;
     if (j < (int64_t)n0 + n1) {
-      if (j >= n0) p = geo_find_one(A1, G1, G1.wl[cur][j - n0], j - n0, tag, round == 0);
-      else p = geo_find_one(A0, G0, G0.wl[cur][j], j, tag, round == 0);
+      if (j >= n0) p = geo_find_one(A1, G1, G1.list(cur)[j - n0], j - n0, tag, round == 0);
+      else p = geo_find_one(A0, G0, G0.list(cur)[j], j, tag, round == 0);
     }
     geo_post(p, round == 0);
   }
@@ -302,7 +302,7 @@ __device__ __forceinline__ void geo_apply_chunks(const ArcSide& A, const GeoStat
     for (int u = 0; u < ITEMS; ++u) {
       c[u] = make_int4(0, 0, 0, 0);
       if (j0 + u < n) {
-        c[u] = g.wl[cur][j0 + u];
+        c[u] = g.list(cur)[j0 + u];
         if (geo_apply_one(A, g, c[u], j0 + u, round, tag)) {
           waitmask |= 1u << u;
           ++nwait;
@@ -316,7 +316,7 @@ __device__ __forceinline__ void geo_apply_chunks(const ArcSide& A, const GeoStat
     int32_t o = first[0];
 #pragma unroll
     for (int u = 0; u < ITEMS; ++u)
-      if (waitmask >> u & 1) g.wl[cur ^ 1][o++] = c[u];
+      if (waitmask >> u & 1) g.list(cur ^ 1)[o++] = c[u];
   }
 }
 
@@ -342,7 +342,7 @@ __device__ __forceinline__ void geo_apply_chunk(const ArcSide& A, const GeoState
   for (int u = 0; u < ITEMS; ++u) {
     c[u] = make_int4(0, 0, 0, 0);
     if (j0 + u < n) {
-      c[u] = g.wl[cur][j0 + u];
+      c[u] = g.list(cur)[j0 + u];
       if (geo_apply_one(A, g, c[u], j0 + u, round, tag)) {
         waitmask |= 1u << u;
         ++nwait;
@@ -356,7 +356,7 @@ __device__ __forceinline__ void geo_apply_chunk(const ArcSide& A, const GeoState
   int32_t o = first[0];
 #pragma unroll
   for (int u = 0; u < ITEMS; ++u)
-    if (waitmask >> u & 1) g.wl[cur ^ 1][o++] = c[u];
+    if (waitmask >> u & 1) g.list(cur ^ 1)[o++] = c[u];
 }
 template <int ITEMS>
 __device__ __forceinline__ void geo_apply_both(const ArcSide& A0, const GeoState& G0, const ArcSide& A1,
@@ -391,6 +391,13 @@ void launch_geo_round2(const ArcSide* a, const GeoState* g, int live, int round,
   DMSC_LAUNCH_CHECK();
 }
 
+// one item of the tail kernel's apply phase; waiting items take next-list slots from an LDS counter
+__device__ __forceinline__ void geo_tail_apply(const ArcSide& A, const GeoState& g, int j, int cur, int r, uint64_t tag,
+                                               int32_t* s_next) {
+  const int4 it = g.list(cur)[j];
+  if (geo_apply_one(A, g, it, j, r, tag)) g.list(cur ^ 1)[atomicAdd(s_next, 1)] = it;
+}
+
 // The last geometry rounds of both traced sides in one workgroup (same structure as the
 // persistence tail kernel): the pending counts end in cnt[r & 1], the round reached in *round_out.
 __global__ __launch_bounds__(kTailThreads) void k_geo_tail(ArcSide A0, GeoState G0, ArcSide A1, GeoState G1, int live,
@@ -405,19 +412,15 @@ __global__ __launch_bounds__(kTailThreads) void k_geo_tail(ArcSide A0, GeoState 
     const uint64_t tag = (uint64_t)(0xFFFFFFFFu - (uint32_t)r) << 32;
     if (threadIdx.x == 0) s_next[0] = s_next[1] = 0;
     for (int j = threadIdx.x; j < n0 + n1; j += kTailThreads) {
-      const RootPosts p = j >= n0 ? geo_find_one(A1, G1, G1.wl[cur][j - n0], j - n0, tag, r == 0)
-                                  : geo_find_one(A0, G0, G0.wl[cur][j], j, tag, r == 0);
+      const RootPosts p = j >= n0 ? geo_find_one(A1, G1, G1.list(cur)[j - n0], j - n0, tag, r == 0)
+                                  : geo_find_one(A0, G0, G0.list(cur)[j], j, tag, r == 0);
       if (p.a) post_min(p.a, p.key, r > 0);
       if (p.b) post_min(p.b, p.key, r > 0);
     }
     phase_barrier();
     for (int j = threadIdx.x; j < n0 + n1; j += kTailThreads) {
-      const int side = j >= n0 ? 1 : 0;
-      const ArcSide& A = side ? A1 : A0;
-      const GeoState& g = side ? G1 : G0;
-      const int jj = side ? j - n0 : j;
-      const int4 it = g.wl[cur][jj];
-      if (geo_apply_one(A, g, it, jj, r, tag)) g.wl[cur ^ 1][atomicAdd(&s_next[side], 1)] = it;
+      if (j >= n0) geo_tail_apply(A1, G1, j - n0, cur, r, tag, &s_next[1]);
+      else geo_tail_apply(A0, G0, j, cur, r, tag, &s_next[0]);
     }
     phase_barrier();
     n0 = s_next[0];

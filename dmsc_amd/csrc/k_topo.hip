@@ -96,8 +96,8 @@ __device__ __forceinline__ void persist_find_phase(const PersistSide& s0, const 
     const int64_t j = jw_base + (threadIdx.x & 63);
     RootPosts p;
     if (j < (int64_t)n0 + n1) {
-      if (j >= n0) p = persist_find_one<DUAL>(s1, s1.wl[cur][j - n0], j - n0, tag, first);
-      else p = persist_find_one<DUAL>(s0, s0.wl[cur][j], j, tag, first);
+      if (j >= n0) p = persist_find_one<DUAL>(s1, s1.list(cur)[j - n0], j - n0, tag, first);
+      else p = persist_find_one<DUAL>(s0, s0.list(cur)[j], j, tag, first);
     }
     wave_post_min(p.a, p.key, !first);
     wave_post_min(p.b, p.key, !first);
@@ -160,7 +160,7 @@ __device__ __forceinline__ void persist_apply_chunk(const PersistSide& s, int cu
   for (int u = 0; u < ITEMS; ++u) {
     k[u] = 0;
     if (j0 + u < n) {
-      k[u] = s.wl[cur][j0 + u];
+      k[u] = s.list(cur)[j0 + u];
       if (persist_apply_one<DUAL>(s, k[u], j0 + u, tag, removed)) {
         waitmask |= 1u << u;
         ++nwait;
@@ -174,7 +174,7 @@ __device__ __forceinline__ void persist_apply_chunk(const PersistSide& s, int cu
   int32_t o = first[0];
 #pragma unroll
   for (int u = 0; u < ITEMS; ++u)
-    if (waitmask >> u & 1) s.wl[cur ^ 1][o++] = k[u];
+    if (waitmask >> u & 1) s.list(cur ^ 1)[o++] = k[u];
 }
 
 // Chunks of side 0 come first, then those of side 1, so every chunk (and block) has one side.
@@ -206,6 +206,14 @@ __global__ __launch_bounds__(256) void k_persist_apply(PersistSide s0, PersistSi
   persist_apply_phase<DUAL>(s0, s1, round, removed, small_list);
 }
 
+// one item of the tail kernel's apply phase; waiting saddles take next-list slots from an LDS counter
+template <bool DUAL>
+__device__ __forceinline__ void persist_tail_apply(const PersistSide& s, int j, int cur, uint64_t tag,
+                                                   uint8_t* __restrict__ removed, int32_t* s_next) {
+  const int i = s.list(cur)[j];
+  if (persist_apply_one<DUAL>(s, i, j, tag, removed)) s.list(cur ^ 1)[atomicAdd(s_next, 1)] = i;
+}
+
 // The tail of the sweep in one workgroup: once few saddles are pending, a round would cost two
 // launches for a few hundred items. One block runs find, barrier, apply, barrier per round until both
 // worklists drain or max_rounds have run, then leaves the pending counts in cnt[r & 1] and the round
@@ -224,18 +232,15 @@ __global__ __launch_bounds__(kTailThreads) void k_persist_tail(PersistSide s0, P
     const uint64_t tag = (uint64_t)(0xFFFFFFFFu - (uint32_t)r) << 32;
     if (threadIdx.x == 0) s_next[0] = s_next[1] = 0;
     for (int j = threadIdx.x; j < n0 + n1; j += kTailThreads) {
-      const RootPosts p = j >= n0 ? persist_find_one<DUAL>(s1, s1.wl[cur][j - n0], j - n0, tag, r == 0)
-                                  : persist_find_one<DUAL>(s0, s0.wl[cur][j], j, tag, r == 0);
+      const RootPosts p = j >= n0 ? persist_find_one<DUAL>(s1, s1.list(cur)[j - n0], j - n0, tag, r == 0)
+                                  : persist_find_one<DUAL>(s0, s0.list(cur)[j], j, tag, r == 0);
       if (p.a) post_min(p.a, p.key, r > 0);
       if (p.b) post_min(p.b, p.key, r > 0);
     }
     phase_barrier();
     for (int j = threadIdx.x; j < n0 + n1; j += kTailThreads) {
-      const int side = j >= n0 ? 1 : 0;
-      const PersistSide& s = side ? s1 : s0;
-      const int jj = side ? j - n0 : j;
-      const int i = s.wl[cur][jj];
-      if (persist_apply_one<DUAL>(s, i, jj, tag, removed)) s.wl[cur ^ 1][atomicAdd(&s_next[side], 1)] = i;
+      if (j >= n0) persist_tail_apply<DUAL>(s1, j - n0, cur, tag, removed, &s_next[1]);
+      else persist_tail_apply<DUAL>(s0, j, cur, tag, removed, &s_next[0]);
     }
     phase_barrier();
     n0 = s_next[0];

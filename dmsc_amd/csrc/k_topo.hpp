@@ -20,7 +20,11 @@ struct PersistSide {
   int32_t* cancel_d;    // per saddle thresholded dead extremum (null when arcs are not traced)
   int32_t* rr;          // per worklist position: dying root (-2 = cycle), (surviving root + 1) << 1 | side bit
   int32_t* croot;       // per saddle: its two component roots when it last had to wait (finds resume there)
-  int32_t* wl[2];       // ping-pong worklists of pending saddle indices
+  int32_t* wl;          // ping-pong worklists of pending saddle indices: list k at wl + k * wls
+  int64_t wls;
+  // (one buffer and an offset rather than an array of two pointers: a run-time index into a struct
+  // member array makes the compiler keep the kernel argument in scratch memory)
+  __host__ __device__ int32_t* list(int k) const { return wl + (int64_t)k * wls; }
   int32_t* cnt;         // [2] worklist sizes (device)
   int32_t* err;         // device error flags
   const float* thr;     // per field persistence threshold
@@ -102,7 +106,9 @@ struct GeoState {
   uint64_t* gjump;         // path-compressed view of the parents: (ancestor or kVoid, link length to it)
   uint64_t* watch;         // per extremum: (tag << 32 | cancel) atomic min
   int32_t* rr;             // per worklist position: R0, R1, len0, len1
-  int4* wl[2];             // worklists: (cancellation, key, arc target 0, arc target 1)
+  int4* wl;                // worklists (cancellation, key, arc target 0, arc target 1): list k at
+  int64_t wls;             // wl + k * wls
+  __host__ __device__ int4* list(int k) const { return wl + (int64_t)k * wls; }
   int32_t* cnt;
   int32_t* err;
 };
