@@ -34,7 +34,7 @@ enum : int {
   A_C32, A_C64,
   A_MINIDS, A_MINVAL, A_SADIDS, A_SADVAL, A_MAXIDS, A_MAXVAL,
   A_CMAX, A_CMIN, A_LMAX, A_LMIN, A_META, A_BASES32, A_SADFIELD, A_CONTRIB, A_SURV1, A_SURVR1,
-  A_KEYS, A_KEYS2, A_VALS2, A_ASC, A_ORDMIN, A_ORDMAX, A_REMOVED, A_POSMIN, A_POSMAX,
+  A_KEYS, A_KEYS2, A_VALS2, A_ASC, A_ORDMIN, A_ORDMAX, A_REMOVED, A_POSMIN, A_POSMAX, A_RSP0, A_RSP1,
   A_SIDE0,  // per persistence side: kSideSlots slots each
   A_SIDE_END = A_SIDE0 + 2 * 13,
   A_ASM0 = A_SIDE_END,  // assembly: counts, prefixes, sad_to_out, survivor lists (12 slots)
@@ -477,12 +477,12 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
   int32_t* vals2 = ar.get<int32_t>(A_VALS2, nsad);
   launch_sort_keys(nsad, sad_val, keys, st);
   iota_i32(asc, nsad, st);
-  radix_sort_pairs(keys, asc, keys2, vals2, nsad, ar, A_RS0, st);
+  radix_sort_pairs(keys, asc, keys2, vals2, nsad, ar, A_RS0, A_RSP0, st);
   if (B > 1) {  // stable second pass on the field: (field, value, id) order
     launch_gather_field(nsad, asc, sad_field, keys, st);
     int bits = 0;
     while ((1 << bits) < B) ++bits;
-    radix_sort_pairs_bits(keys, asc, keys2, vals2, nsad, bits, ar, A_RS0, st);
+    radix_sort_pairs_bits(keys, asc, keys2, vals2, nsad, bits, ar, A_RS0, A_RSP0, st);
   }
   int32_t* ord_min = ar.get<int32_t>(A_ORDMIN, nsad);
   int32_t* ord_max = ar.get<int32_t>(A_ORDMAX, nsad);
@@ -741,7 +741,7 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     int32_t* cslot = ar.get<int32_t>(b + G_SLOT, nc);
     launch_cancel_compact(nsad, sb[s].cancel, sb[s].pers, ar.get<int32_t>(b + G_PRE, nsad), cs0, cd0, ck, st);
     iota_i32(cslot, nc, st);
-    radix_sort_pairs(ck, cslot, ar.get<uint32_t>(b + G_KEYS2, nc), ar.get<int32_t>(b + G_SLOT2, nc), nc, ar, A_RS0, st);
+    radix_sort_pairs(ck, cslot, ar.get<uint32_t>(b + G_KEYS2, nc), ar.get<int32_t>(b + G_SLOT2, nc), nc, ar, A_RS0, A_RSP0, st);
     GeoState& g = w.G;
     g.ncancel = nc;
     int32_t* ckey = ar.get<int32_t>(b + G_CS, nc);

@@ -146,12 +146,21 @@ void scan_i32_to_i64(const int32_t* in, int64_t* out, int64_t n, int64_t* d_tota
 }
 
 // ------------------------------------------------------------------ radix sort
-// Stable LSD sort, 8-bit digits, 4 passes. Per pass: per-tile digit histograms (digit-major), one
-// exclusive scan, then a scatter that ranks equal digits in element order: chunks of 256 in
+// Stable LSD sort, 8-bit digits, up to 4 passes. Per pass: per-tile digit histograms (digit-major),
+// one exclusive scan, then a scatter that ranks equal digits in element order: chunks of 256 in
 // order, within a chunk wave-level peer masks from 8 ballots, waves in order via LDS counts.
+// Between passes the pairs travel packed as one 64-bit word (key << 32 | value): one scattered
+// 8-byte store per element instead of two 4-byte stores to two arrays.
 constexpr int RS_THREADS = 256, RS_ITEMS = 16, RS_TILE = RS_THREADS * RS_ITEMS;
 
-__global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint32_t* __restrict__ keys, int64_t n, int shift,
+template <bool PK>
+__device__ __forceinline__ uint32_t rs_key(const void* in, int64_t i) {
+  if (PK) return (uint32_t)(static_cast<const uint64_t*>(in)[i] >> 32);
+  return static_cast<const uint32_t*>(in)[i];
+}
+
+template <bool PK>
+__global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const void* __restrict__ keys, int64_t n, int shift,
                                                         int32_t* __restrict__ counts, int64_t nb) {
   __shared__ int32_t h[256];
   h[threadIdx.x] = 0;
@@ -159,14 +168,16 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint32_t* __restri
   const int64_t base = (int64_t)blockIdx.x * RS_TILE;
   for (int j = 0; j < RS_ITEMS; ++j) {
     int64_t i = base + j * RS_THREADS + threadIdx.x;
-    if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255], 1);
+    if (i < n) atomicAdd(&h[(rs_key<PK>(keys, i) >> shift) & 255], 1);
   }
   __syncthreads();
   counts[(int64_t)threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
 }
 
-__global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint32_t* __restrict__ kin, const int32_t* __restrict__ vin,
-                                                           uint32_t* __restrict__ kout, int32_t* __restrict__ vout,
+// PIN / POUT: input / output packed (one uint64 array) or separate (key array, value array)
+template <bool PIN, bool POUT>
+__global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const void* __restrict__ kin, const int32_t* __restrict__ vin,
+                                                           void* __restrict__ kout, int32_t* __restrict__ vout,
                                                            int64_t n, int shift, const int32_t* __restrict__ offs,
                                                            int64_t nb) {
   __shared__ int32_t base_off[256];
@@ -183,8 +194,18 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint32_t* __res
     __syncthreads();
     const int64_t i = base + j * RS_THREADS + tid;
     const bool valid = i < n;
-    uint32_t k = valid ? kin[i] : 0u;
-    int32_t v = valid ? vin[i] : 0;
+    uint32_t k = 0u;
+    int32_t v = 0;
+    if (valid) {
+      if (PIN) {
+        const uint64_t p = static_cast<const uint64_t*>(kin)[i];
+        k = (uint32_t)(p >> 32);
+        v = (int32_t)(uint32_t)p;
+      } else {
+        k = static_cast<const uint32_t*>(kin)[i];
+        v = vin[i];
+      }
+    }
     const int d = (k >> shift) & 255;
     uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -200,8 +221,12 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint32_t* __res
       int prior = 0;
       for (int w = 0; w < wid; ++w) prior += wcount[w][d];
       const int pos = base_off[d] + running[d] + prior + rank;
-      kout[pos] = k;
-      vout[pos] = v;
+      if (POUT) {
+        static_cast<uint64_t*>(kout)[pos] = ((uint64_t)k << 32) | (uint32_t)v;
+      } else {
+        static_cast<uint32_t*>(kout)[pos] = k;
+        vout[pos] = v;
+      }
     }
     __syncthreads();
     running[tid] += wcount[0][tid] + wcount[1][tid] + wcount[2][tid] + wcount[3][tid];
@@ -218,36 +243,42 @@ __global__ void k_copy_pairs(const uint32_t* __restrict__ ks, const int32_t* __r
 }
 
 // Sorts on the low `bits` bits of the keys (ceil(bits / 8) passes); the result is in (keys, vals).
+// Scratch: arena slots `slot` (counts), slot + 1 (their scan), pslot and pslot + 1 (packed pairs).
 void radix_sort_pairs_bits(uint32_t* keys, int32_t* vals, uint32_t* keys_tmp, int32_t* vals_tmp, int64_t n, int bits,
-                           Arena& ar, int slot, hipStream_t st) {
+                           Arena& ar, int slot, int pslot, hipStream_t st) {
   KScope ks_("radix_sort", st);
   if (n <= 1 || bits <= 0) return;
   const int npass = (bits + 7) / 8 > 4 ? 4 : (bits + 7) / 8;
   const int64_t nb = (n + RS_TILE - 1) / RS_TILE;
   int32_t* counts = ar.get<int32_t>(slot, (size_t)(256 * nb));
-  uint32_t* ka = keys;
-  int32_t* va = vals;
-  uint32_t* kb = keys_tmp;
-  int32_t* vb = vals_tmp;
-  for (int pass = 0; pass < npass; ++pass) {
-    const int shift = 8 * pass;
-    hipLaunchKernelGGL(k_rs_hist, dim3((unsigned)nb), dim3(RS_THREADS), 0, st, ka, n, shift, counts, nb);
+  auto pass = [&](int p, const void* kin, const int32_t* vin, void* kout, int32_t* vout, bool pin, bool pout) {
+    const int shift = 8 * p;
+    if (pin) hipLaunchKernelGGL(k_rs_hist<true>, dim3((unsigned)nb), dim3(RS_THREADS), 0, st, kin, n, shift, counts, nb);
+    else hipLaunchKernelGGL(k_rs_hist<false>, dim3((unsigned)nb), dim3(RS_THREADS), 0, st, kin, n, shift, counts, nb);
     DMSC_LAUNCH_CHECK();
     scan_exclusive<int32_t>(counts, counts, 256 * nb, nullptr, ar, slot + 1, st);
-    hipLaunchKernelGGL(k_rs_scatter, dim3((unsigned)nb), dim3(RS_THREADS), 0, st, ka, va, kb, vb, n, shift, counts, nb);
+    const dim3 g((unsigned)nb), b(RS_THREADS);
+    if (pin && pout) hipLaunchKernelGGL((k_rs_scatter<true, true>), g, b, 0, st, kin, vin, kout, vout, n, shift, counts, nb);
+    else if (pin) hipLaunchKernelGGL((k_rs_scatter<true, false>), g, b, 0, st, kin, vin, kout, vout, n, shift, counts, nb);
+    else if (pout) hipLaunchKernelGGL((k_rs_scatter<false, true>), g, b, 0, st, kin, vin, kout, vout, n, shift, counts, nb);
+    else hipLaunchKernelGGL((k_rs_scatter<false, false>), g, b, 0, st, kin, vin, kout, vout, n, shift, counts, nb);
     DMSC_LAUNCH_CHECK();
-    std::swap(ka, kb);
-    std::swap(va, vb);
-  }
-  if (ka != keys) {  // odd number of passes
-    hipLaunchKernelGGL(k_copy_pairs, dim3(grid_for(n, 256, 16384)), dim3(256), 0, st, ka, va, keys, vals, n);
+  };
+  if (npass == 1) {
+    pass(0, keys, vals, keys_tmp, vals_tmp, false, false);
+    hipLaunchKernelGGL(k_copy_pairs, dim3(grid_for(n, 256, 16384)), dim3(256), 0, st, keys_tmp, vals_tmp, keys, vals, n);
     DMSC_LAUNCH_CHECK();
+    return;
   }
+  uint64_t* pk[2] = {ar.get<uint64_t>(pslot, n), ar.get<uint64_t>(pslot + 1, n)};
+  pass(0, keys, vals, pk[0], nullptr, false, true);
+  for (int p = 1; p + 1 < npass; ++p) pass(p, pk[(p - 1) & 1], nullptr, pk[p & 1], nullptr, true, true);
+  pass(npass - 1, pk[(npass - 2) & 1], nullptr, keys, vals, true, false);
 }
 
 void radix_sort_pairs(uint32_t* keys, int32_t* vals, uint32_t* keys_tmp, int32_t* vals_tmp, int64_t n, Arena& ar,
-                      int slot, hipStream_t st) {
-  radix_sort_pairs_bits(keys, vals, keys_tmp, vals_tmp, n, 32, ar, slot, st);
+                      int slot, int pslot, hipStream_t st) {
+  radix_sort_pairs_bits(keys, vals, keys_tmp, vals_tmp, n, 32, ar, slot, pslot, st);
 }
 
 // ------------------------------------------------------------------ pointer jumping
