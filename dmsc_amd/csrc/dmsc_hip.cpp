@@ -535,21 +535,10 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
 
   // ---- persistence rounds (both sides together)
   uint8_t* removed = ar.get<uint8_t>(A_REMOVED, nsad);
-  fill_u8(removed, 0, nsad, st);
   SideBufs sb[2] = {side_bufs(ar, 0, nmin, nsad), side_bufs(ar, 1, nmax, nsad)};
   const bool record[2] = {tmin, tmax};
   PersistSide ps[2];
   for (int s = 0; s < 2; ++s) {
-    const int64_t next = s ? nmax : nmin;
-    iota_i32(sb[s].par_g, next, st);
-    iota_i32(sb[s].par_t, next, st);
-    fill_u8(sb[s].alive, 1, next, st);
-    fill_u64(sb[s].minkey, ~0ull, next, st);
-    fill_i32(sb[s].pair, -1, nsad, st);
-    fill_i32(sb[s].cancel, -1, nsad, st);
-    iota_i32(sb[s].wl, nsad, st);  // list 0: every saddle
-    fill_i32(sb[s].cnt, (int32_t)nsad, 1, st);
-    fill_i32(sb[s].cnt + 1, 0, 1, st);
     PersistSide& p = ps[s];
     p.pos = s ? pos_max : pos_min;
     p.c = s ? cmax : cmin;
@@ -572,6 +561,7 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     p.sfield = sad_field;
     p.maxside = s;
   }
+  launch_persist_init(ps[0], ps[1], nmin, nmax, nsad, sb[0].cancel, sb[1].cancel, removed, st);
   int round = 0;
   ctx->pending_log.clear();
   int32_t* d_round = c32 + 50;  // round number for graph replays
@@ -745,13 +735,10 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     GeoState& g = w.G;
     g.ncancel = nc;
     int32_t* ckey = ar.get<int32_t>(b + G_CS, nc);
-    uint64_t* kill = ar.get<uint64_t>(b + G_DEADC, next);
-    fill_u64(kill, ~0ull, next, st);
-    launch_cancel_finish(nc, cslot, cd0, ckey, kill, st);
     g.cs = cs0;
     g.cd = cd0;
     g.key = ckey;
-    g.kill = kill;
+    g.kill = ar.get<uint64_t>(b + G_DEADC, next);
     g.link = ar.get<int4>(b + G_LINK, next);
     g.watch = ar.get<uint64_t>(b + G_WATCH, next);
     g.rr = ar.get<int32_t>(b + G_RR, 4 * nc);
@@ -760,11 +747,9 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     g.cnt = ar.get<int32_t>(b + G_CNT, 2);
     g.err = d_err;
     g.gjump = ar.get<uint64_t>(b + G_JUMP, next);
-    launch_geo_init(g, next, st);  // every extremum its own root
-    fill_u64(g.watch, ~0ull, next, st);
-    launch_geo_items(nc, cs0, ckey, w.A.tinit, g.list(0), st);
-    fill_i32(g.cnt, (int32_t)nc, 1, st);
-    fill_i32(g.cnt + 1, 0, 1, st);
+    launch_geo_init(g, next, st);  // every extremum its own root, no killer, no watch
+    launch_cancel_finish(nc, cslot, cd0, ckey, g.kill, st);
+    launch_geo_items(nc, cs0, ckey, w.A.tinit, g.list(0), g.cnt, st);  // list 0 and its count
   }
   // geometry rounds of both sides, interleaved, one readback per batch of rounds
   {

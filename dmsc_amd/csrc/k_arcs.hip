@@ -137,17 +137,21 @@ void launch_cancel_finish(int64_t nc, const int32_t* slot, const int32_t* cd, in
 // ------------------------------------------------------------------ geometry rounds
 // initial worklist: every cancellation (saddle-id order) with its key and its two arc targets
 __global__ void k_geo_items(int64_t nc, const int32_t* __restrict__ cs, const int32_t* __restrict__ key,
-                            const int32_t* __restrict__ tinit, int4* __restrict__ wl) {
+                            const int32_t* __restrict__ tinit, int4* __restrict__ wl, int32_t* __restrict__ cnt) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    cnt[0] = (int32_t)nc;
+    cnt[1] = 0;
+  }
   for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x) {
     const int s = cs[c];
     const int2 t = *reinterpret_cast<const int2*>(tinit + 2 * s);
     wl[c] = make_int4((int)c, key[c], t.x, t.y);
   }
 }
-void launch_geo_items(int64_t nc, const int32_t* cs, const int32_t* key, const int32_t* tinit, int4* wl,
+void launch_geo_items(int64_t nc, const int32_t* cs, const int32_t* key, const int32_t* tinit, int4* wl, int32_t* cnt,
                       hipStream_t st) {
-  if (nc == 0) return;
-  hipLaunchKernelGGL(k_geo_items, dim3(grid_for(nc, 256, 16384)), dim3(256), 0, st, nc, cs, key, tinit, wl);
+  hipLaunchKernelGGL(k_geo_items, dim3(grid_for(std::max<int64_t>(nc, 1), 256, 16384)), dim3(256), 0, st, nc, cs, key,
+                     tinit, wl, cnt);
   DMSC_LAUNCH_CHECK();
 }
 
@@ -499,15 +503,19 @@ void launch_geo_round(const ArcSide& a, const GeoState& g, int round, int64_t ma
   DMSC_LAUNCH_CHECK();
 }
 
-__global__ void k_geo_init(int4* __restrict__ link, uint64_t* __restrict__ gjump, int64_t next) {
+__global__ void k_geo_init(int4* __restrict__ link, uint64_t* __restrict__ gjump, uint64_t* __restrict__ kill,
+                           uint64_t* __restrict__ watch, int64_t next) {
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < next; e += (int64_t)gridDim.x * blockDim.x) {
     link[e] = make_int4((int)e, 0, 0, 0);
     gjump[e] = (uint64_t)(uint32_t)e;
+    kill[e] = ~0ull;
+    watch[e] = ~0ull;
   }
 }
 void launch_geo_init(const GeoState& g, int64_t next, hipStream_t st) {
   if (next == 0) return;
-  hipLaunchKernelGGL(k_geo_init, dim3(grid_for(next, 256, 16384)), dim3(256), 0, st, g.link, g.gjump, next);
+  hipLaunchKernelGGL(k_geo_init, dim3(grid_for(next, 256, 16384)), dim3(256), 0, st, g.link, g.gjump, g.kill, g.watch,
+                     next);
   DMSC_LAUNCH_CHECK();
 }
 

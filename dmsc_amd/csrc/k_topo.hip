@@ -206,6 +206,50 @@ __global__ __launch_bounds__(256) void k_persist_apply(PersistSide s0, PersistSi
   persist_apply_phase<DUAL>(s0, s1, round, removed, small_list);
 }
 
+// Start state of both sides' rounds in one launch: every extremum its own root (both union-finds),
+// alive, no post; every saddle unpaired, not cancelled, not removed, in list 0.
+__global__ void k_persist_init(PersistSide s0, PersistSide s1, int64_t n0, int64_t n1, int64_t ns,
+                               int32_t* __restrict__ cancel0, int32_t* __restrict__ cancel1,
+                               uint8_t* __restrict__ removed) {
+  const int64_t m = max(max(n0, n1), ns);
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < m; e += (int64_t)gridDim.x * blockDim.x) {
+    if (e < n0) {
+      s0.par_g[e] = (int32_t)e;
+      s0.par_t[e] = (int32_t)e;
+      s0.alive[e] = 1;
+      s0.minkey[e] = ~0ull;
+    }
+    if (e < n1) {
+      s1.par_g[e] = (int32_t)e;
+      s1.par_t[e] = (int32_t)e;
+      s1.alive[e] = 1;
+      s1.minkey[e] = ~0ull;
+    }
+    if (e < ns) {
+      s0.pair[e] = -1;
+      s1.pair[e] = -1;
+      cancel0[e] = -1;
+      cancel1[e] = -1;
+      s0.list(0)[e] = (int32_t)e;
+      s1.list(0)[e] = (int32_t)e;
+      removed[e] = 0;
+    }
+    if (e == 0) {
+      s0.cnt[0] = (int32_t)ns;
+      s0.cnt[1] = 0;
+      s1.cnt[0] = (int32_t)ns;
+      s1.cnt[1] = 0;
+    }
+  }
+}
+void launch_persist_init(const PersistSide& s0, const PersistSide& s1, int64_t n0, int64_t n1, int64_t ns,
+                         int32_t* cancel0, int32_t* cancel1, uint8_t* removed, hipStream_t st) {
+  const int64_t m = std::max(std::max(n0, n1), std::max<int64_t>(ns, 1));
+  hipLaunchKernelGGL(k_persist_init, dim3(grid_for(m, 256, 16384)), dim3(256), 0, st, s0, s1, n0, n1, ns, cancel0,
+                     cancel1, removed);
+  DMSC_LAUNCH_CHECK();
+}
+
 // one item of the tail kernel's apply phase; waiting saddles take next-list slots from an LDS counter
 template <bool DUAL>
 __device__ __forceinline__ void persist_tail_apply(const PersistSide& s, int j, int cur, uint64_t tag,
@@ -375,6 +419,8 @@ void launch_mask_prefix(const uint8_t* alive, int32_t* pre, int64_t n, hipStream
 }
 
 // Per order position: survivor flag (min order) and edge-row count of one side.
+// Per position of the side's order (sequential writes; the per-saddle reads are gathers: visiting
+// saddles in id order and scattering the two counts measured slower except on heavy-tie noise).
 __global__ void k_side_counts(int64_t ns, const int32_t* __restrict__ ord, const uint8_t* __restrict__ removed,
                               const int32_t* __restrict__ c, const int32_t* __restrict__ root_t,
                               const int32_t* __restrict__ to_out, int32_t* __restrict__ surv,
@@ -385,8 +431,8 @@ __global__ void k_side_counts(int64_t ns, const int32_t* __restrict__ ord, const
     if (surv) surv[k] = alive ? 1 : 0;
     int n = 0;
     if (alive) {
-      const int a = c[2 * i], b = c[2 * i + 1];
-      const int r1 = a >= 0 ? root_t[a] : -1, r2 = b >= 0 ? root_t[b] : -1;
+      const int2 ab = *reinterpret_cast<const int2*>(c + 2 * i);
+      const int r1 = ab.x >= 0 ? root_t[ab.x] : -1, r2 = ab.y >= 0 ? root_t[ab.y] : -1;
       if (r1 != -1 && to_out[r1] != -1) ++n;
       if (r1 != r2 && r2 != -1 && to_out[r2] != -1) ++n;
     }

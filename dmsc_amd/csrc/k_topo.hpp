@@ -32,6 +32,9 @@ struct PersistSide {
   int maxside;
 };
 
+// start state of both sides' persistence rounds (one launch)
+void launch_persist_init(const PersistSide& s0, const PersistSide& s1, int64_t n0, int64_t n1, int64_t ns,
+                         int32_t* cancel0, int32_t* cancel1, uint8_t* removed, hipStream_t st);
 // d_round (optional): the kernels read the round number from there instead of `round` (graph replay)
 void launch_persist_round(bool dual, const PersistSide& s0, const PersistSide& s1, int round, uint8_t* removed,
                           int64_t max_items, hipStream_t st, const int32_t* d_round = nullptr);
@@ -112,7 +115,7 @@ struct GeoState {
   int32_t* cnt;
   int32_t* err;
 };
-// link[e] = (e, 0, 0, 0), gjump[e] = (e, 0): every extremum its own root
+// link[e] = (e, 0, 0, 0), gjump[e] = (e, 0): every extremum its own root; kill and watch all ones
 void launch_geo_init(const GeoState& g, int64_t next, hipStream_t st);
 void launch_geo_round(const ArcSide& a, const GeoState& g, int round, int64_t max_items, hipStream_t st);
 // one round of both traced sides (bit s of `live`) in one find and one apply launch
@@ -129,7 +132,7 @@ void launch_cancel_flags(int64_t ns, const int32_t* cancel_d, int32_t* flag, hip
 void launch_cancel_compact(int64_t ns, const int32_t* cancel_d, const float* pers, const int32_t* pre,
                            int32_t* cs, int32_t* cd, uint32_t* keys, hipStream_t st);
 constexpr uint64_t kKillPending = 0xFFFFFFFFull << 32;
-void launch_geo_items(int64_t nc, const int32_t* cs, const int32_t* key, const int32_t* tinit, int4* wl,
+void launch_geo_items(int64_t nc, const int32_t* cs, const int32_t* key, const int32_t* tinit, int4* wl, int32_t* cnt,
                       hipStream_t st);
 void launch_cancel_finish(int64_t nc, const int32_t* slot, const int32_t* cd, int32_t* key, uint64_t* kill,
                           hipStream_t st);
