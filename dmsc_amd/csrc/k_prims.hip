@@ -108,11 +108,51 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(Load ld, int64_t n, 
   }
 }
 
+// Down-sweep that adds up the block sums before its own tile itself (no separate scan of the block
+// sums: for a few thousand tiles that costs each block a few microseconds of L2 reads, less than a
+// launch); the last block writes the grand total.
+template <class T, class Load>
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_down2(Load ld, int64_t n, const T* __restrict__ bsums,
+                                                             T* __restrict__ out, T* __restrict__ total) {
+  __shared__ T lds4[4];
+  T pre = 0;
+  for (int64_t b = threadIdx.x; b < blockIdx.x; b += SCAN_THREADS) pre += bsums[b];
+  T base_sum;
+  block_exclusive_scan(pre, lds4, base_sum);
+  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+  T v[SCAN_ITEMS];
+  T s = 0;
+#pragma unroll
+  for (int j = 0; j < SCAN_ITEMS; ++j) {
+    v[j] = (base + j < n) ? ld(base + j) : (T)0;
+    s += v[j];
+  }
+  T tot;
+  T ex = block_exclusive_scan(s, lds4, tot) + base_sum;
+#pragma unroll
+  for (int j = 0; j < SCAN_ITEMS; ++j) {
+    if (base + j < n) out[base + j] = ex;
+    ex += v[j];
+  }
+  if (total && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *total = base_sum + tot;
+}
+constexpr int64_t kScanFewTiles = 4096;  // up to this many tiles: reduce + down2 (two launches)
+
 template <class T, class Load>
 static void scan_impl(Load ld, T* out, int64_t n, T* d_total, Arena& ar, int slot, hipStream_t st) {
   KScope ks_("scan", st);
   const int64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
   T* bsums = ar.get<T>(slot, (size_t)(nb > 0 ? nb : 1));
+  if (nb <= kScanFewTiles) {
+    if (n > 0) {
+      hipLaunchKernelGGL((k_scan_reduce<T, Load>), dim3((unsigned)nb), dim3(SCAN_THREADS), 0, st, ld, n, bsums);
+      DMSC_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL((k_scan_down2<T, Load>), dim3((unsigned)std::max<int64_t>(nb, 1)), dim3(SCAN_THREADS), 0, st, ld,
+                       n, bsums, out, d_total);
+    DMSC_LAUNCH_CHECK();
+    return;
+  }
   if (n > 0) {
     hipLaunchKernelGGL((k_scan_reduce<T, Load>), dim3((unsigned)nb), dim3(SCAN_THREADS), 0, st, ld, n, bsums);
     DMSC_LAUNCH_CHECK();
