@@ -864,41 +864,77 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
   }
 
   fork.wait_mark();  // raw arcs traced
+  // Expansion of both sides in lockstep: side 0 on `st`, side 1 on the side stream when both are
+  // traced (the side stream first waits for the geometry results), one host check per batch of levels.
+  const bool two = aw[0].on && aw[1].on;
+  if (two) fork.follow();
+  struct Expansion {
+    hipStream_t es = nullptr;
+    Seg* lists[2] = {nullptr, nullptr};
+    Seg* raw = nullptr;
+    int32_t* nseg = nullptr;
+    int64_t cap = 0;
+    int level = 0;
+    int32_t n_in = 0;
+  } ex[2];
+  auto pin32 = [&](int s) { return (int32_t*)((char*)ctx->pinned + 64 * s); };
   for (int s = 0; s < 2; ++s) {
     ArcWork& w = aw[s];
     if (!w.on) continue;
+    Expansion& e = ex[s];
+    e.es = (two && s == 1) ? fork.stream() : st;
     Arena& a2 = ctx->ar2;
     const int hb2 = 16 * s;
     const int64_t next = s ? nmax : nmin;
-    const int64_t cap = w.out_total + 16;
-    Seg* sa = a2.get<Seg>(hb2 + H_SEGA, cap);
-    Seg* sbuf = a2.get<Seg>(hb2 + H_SEGB, cap);
-    Seg* raw = a2.get<Seg>(hb2 + H_RAW, cap);
+    e.cap = w.out_total + 16;
+    e.lists[0] = a2.get<Seg>(hb2 + H_SEGA, e.cap);
+    e.lists[1] = a2.get<Seg>(hb2 + H_SEGB, e.cap);
+    e.raw = a2.get<Seg>(hb2 + H_RAW, e.cap);
     // chain lists alternate between two buffers; their counters rotate over three (level L reads
     // nseg[L%3], appends to nseg[(L+1)%3] and clears nseg[(L+2)%3]); nseg[3] counts raw pieces
-    int32_t* nseg = a2.get<int32_t>(hb2 + H_NSEG, 4);
-    fill_i32(nseg, 0, 4, st);
-    launch_seed_segments(n_sad_total, surv_list[0], nullptr, w.A, w.pjq, next, w.flen, w.pos_pre, sad_to_out, sa, nseg, raw,
-                         nseg + 3, cap, d_err, w.out_off, w.out_arc_off, w.arc_base, row32(5), B, st);
-    readback(ctx, nseg, 4 * sizeof(int32_t), st);
-    int32_t n_in = ((const int32_t*)ctx->pinned)[0];
-    Seg* lists[2] = {sa, sbuf};
-    int level = 0;
-    while (n_in > 0) {
-      if (n_in > cap) throw Fail{DMSC_ERR_INTERNAL, "arc segment list overflow"};
-      // a batch of levels per host check; a level past the last one finds no chains and only clears
-      const int batch = level < 4 ? std::min(2, ctx->tune.expand_levels) : ctx->tune.expand_levels;
-      for (int k = 0; k < batch; ++k, ++level)
-        launch_expand_segments(w.A, w.G, w.pjq, w.cum_r, w.cum_c, lists[level & 1], nseg + level % 3,
-                               lists[(level + 1) & 1], nseg + (level + 1) % 3, nseg + (level + 2) % 3, raw, nseg + 3,
-                               std::max<int64_t>(n_in, 2048 * 256), cap, st);
-      readback(ctx, nseg, 4 * sizeof(int32_t), st);
-      n_in = ((const int32_t*)ctx->pinned)[level % 3];
-      if (level > 1000000) throw Fail{DMSC_ERR_INTERNAL, "arc expansion did not terminate"};
+    e.nseg = a2.get<int32_t>(hb2 + H_NSEG, 4);
+    fill_i32(e.nseg, 0, 4, e.es);
+    launch_seed_segments(n_sad_total, surv_list[0], nullptr, w.A, w.pjq, next, w.flen, w.pos_pre, sad_to_out,
+                         e.lists[0], e.nseg, e.raw, e.nseg + 3, e.cap, d_err, w.out_off, w.out_arc_off, w.arc_base,
+                         row32(5), B, e.es);
+    DMSC_CHECK(hipMemcpyAsync(pin32(s), e.nseg, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, e.es));
+  }
+  for (int s = 0; s < 2; ++s)
+    if (aw[s].on) {
+      DMSC_CHECK(hipStreamSynchronize(ex[s].es));
+      ex[s].n_in = pin32(s)[0];
     }
-    const int32_t nraw = ((const int32_t*)ctx->pinned)[3];
-    if (nraw > cap) throw Fail{DMSC_ERR_INTERNAL, "arc segment list overflow"};
-    if (nraw > 0) launch_copy_raw(w.A, raw, nseg + 3, std::min<int64_t>(nraw, cap), w.out_cells, w.out_total, d_err, st);
+  while ((aw[0].on && ex[0].n_in > 0) || (aw[1].on && ex[1].n_in > 0)) {
+    bool ran[2] = {false, false};
+    for (int s = 0; s < 2; ++s) {
+      ArcWork& w = aw[s];
+      Expansion& e = ex[s];
+      if (!w.on || e.n_in <= 0) continue;
+      if (e.n_in > e.cap) throw Fail{DMSC_ERR_INTERNAL, "arc segment list overflow"};
+      // a batch of levels per host check; a level past the last one finds no chains and only clears
+      const int batch = e.level < 4 ? std::min(2, ctx->tune.expand_levels) : ctx->tune.expand_levels;
+      for (int k = 0; k < batch; ++k, ++e.level)
+        launch_expand_segments(w.A, w.G, w.pjq, w.cum_r, w.cum_c, e.lists[e.level & 1], e.nseg + e.level % 3,
+                               e.lists[(e.level + 1) & 1], e.nseg + (e.level + 1) % 3, e.nseg + (e.level + 2) % 3, e.raw,
+                               e.nseg + 3, std::max<int64_t>(e.n_in, 2048 * 256), e.cap, e.es);
+      DMSC_CHECK(hipMemcpyAsync(pin32(s), e.nseg, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, e.es));
+      ran[s] = true;
+      if (e.level > 1000000) throw Fail{DMSC_ERR_INTERNAL, "arc expansion did not terminate"};
+    }
+    for (int s = 0; s < 2; ++s)
+      if (ran[s]) {
+        DMSC_CHECK(hipStreamSynchronize(ex[s].es));
+        ex[s].n_in = pin32(s)[ex[s].level % 3];
+      }
+  }
+  for (int s = 0; s < 2; ++s) {
+    ArcWork& w = aw[s];
+    Expansion& e = ex[s];
+    if (!w.on) continue;
+    const int32_t nraw = pin32(s)[3];
+    if (nraw > e.cap) throw Fail{DMSC_ERR_INTERNAL, "arc segment list overflow"};
+    if (nraw > 0)
+      launch_copy_raw(w.A, e.raw, e.nseg + 3, std::min<int64_t>(nraw, e.cap), w.out_cells, w.out_total, d_err, e.es);
   }
   fork.join();
   readback(ctx, d_err, sizeof(int32_t), st);
