@@ -109,20 +109,13 @@ __global__ void k_persist_find(PersistSide s0, PersistSide s1, int round, const 
   persist_find_phase<DUAL>(s0, s1, round);
 }
 
-// Decides one pending saddle i; returns true when it must wait for a later round.
+// Applies saddle i, which holds minkey of its dying root (r = its rr pair).
 template <bool DUAL>
-__device__ __forceinline__ bool persist_apply_one(const PersistSide& s, const int i, int64_t j, uint64_t tag,
-                                                  uint8_t* __restrict__ removed) {
-  const int2 r = *reinterpret_cast<const int2*>(s.rr + 2 * j);
+__device__ __forceinline__ void persist_apply_winner(const PersistSide& s, const int i, const int2 r,
+                                                     uint8_t* __restrict__ removed) {
   const int dying = r.x;
-  if (dying == -2) return false;
-  const int code = r.y;
-  const int surv = (code >> 1) - 1;
-  const bool b_dies = code & 1;
-  if (s.minkey[dying] != (tag | (uint32_t)s.pos[i])) {
-    *reinterpret_cast<int2*>(s.croot + 2 * i) = b_dies ? make_int2(surv, dying) : make_int2(dying, surv);
-    return true;
-  }
+  const int surv = (r.y >> 1) - 1;
+  const bool b_dies = r.y & 1;
   const float sv = s.sval[i], dv = key_value(s.ekey[dying]);
   float p;
   if (s.maxside) p = DUAL ? sv - dv : dv - sv;
@@ -141,30 +134,70 @@ __device__ __forceinline__ bool persist_apply_one(const PersistSide& s, const in
     if (d != t && d != -1) s.par_t[d] = t;
     if (s.cancel_d) s.cancel_d[i] = d;
   }
+}
+
+// saddle i waits with its decision r: the next find resumes from its roots
+__device__ __forceinline__ void persist_wait(const PersistSide& s, const int i, const int2 r) {
+  const int surv = (r.y >> 1) - 1;
+  *reinterpret_cast<int2*>(s.croot + 2 * i) = (r.y & 1) ? make_int2(surv, r.x) : make_int2(r.x, surv);
+}
+
+// Decides one pending saddle i; returns true when it must wait for a later round.
+template <bool DUAL>
+__device__ __forceinline__ bool persist_apply_one(const PersistSide& s, const int i, int64_t j, uint64_t tag,
+                                                  uint8_t* __restrict__ removed) {
+  const int2 r = *reinterpret_cast<const int2*>(s.rr + 2 * j);
+  if (r.x == -2) return false;
+  if (s.minkey[r.x] != (tag | (uint32_t)s.pos[i])) {
+    persist_wait(s, i, r);
+    return true;
+  }
+  persist_apply_winner<DUAL>(s, i, r, removed);
   return false;
 }
 
 // One chunk of a side's worklist: kApplyItems consecutive items per thread; waiting items are
 // appended to the next worklist with one block-wide reservation (order preserved within the chunk).
-// Small worklists (the tail of the sweep) are latency-bound: one item per thread there.
+// The loads of the thread's items are issued stage by stage (list entries and decisions, then the
+// holder tests) so that their latencies overlap. Small worklists (the tail of the sweep) are
+// latency-bound: one item per thread there.
 constexpr int kApplyItems = 4, kApplyChunk = 256 * kApplyItems;
 
 template <bool DUAL, int ITEMS>
 __device__ __forceinline__ void persist_apply_chunk(const PersistSide& s, int cur, int64_t start, int n, uint64_t tag,
                                                     uint8_t* __restrict__ removed) {
   const int64_t j0 = start + (int64_t)threadIdx.x * ITEMS;
+  const int32_t* const in = s.list(cur);
   int k[ITEMS];
+  int2 r[ITEMS];
+  uint64_t mk[ITEMS];
+  uint32_t ps[ITEMS];
+#pragma unroll
+  for (int u = 0; u < ITEMS; ++u) {
+    const bool v = j0 + u < n;
+    k[u] = v ? in[j0 + u] : 0;
+    r[u] = v ? *reinterpret_cast<const int2*>(s.rr + 2 * (j0 + u)) : make_int2(-3, 0);  // -3: past the list
+  }
+#pragma unroll
+  for (int u = 0; u < ITEMS; ++u) {
+    mk[u] = 0;
+    ps[u] = 0;
+    if (r[u].x >= 0) {
+      mk[u] = s.minkey[r[u].x];
+      ps[u] = (uint32_t)s.pos[k[u]];
+    }
+  }
   int32_t nwait = 0;
   uint32_t waitmask = 0;
 #pragma unroll
   for (int u = 0; u < ITEMS; ++u) {
-    k[u] = 0;
-    if (j0 + u < n) {
-      k[u] = s.list(cur)[j0 + u];
-      if (persist_apply_one<DUAL>(s, k[u], j0 + u, tag, removed)) {
-        waitmask |= 1u << u;
-        ++nwait;
-      }
+    if (r[u].x < 0) continue;  // a cycle (decided in the find) or past the list
+    if (mk[u] != (tag | ps[u])) {
+      persist_wait(s, k[u], r[u]);
+      waitmask |= 1u << u;
+      ++nwait;
+    } else {
+      persist_apply_winner<DUAL>(s, k[u], r[u], removed);
     }
   }
   int32_t* const cnt[1] = {&s.cnt[cur ^ 1]};
