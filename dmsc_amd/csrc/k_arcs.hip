@@ -264,30 +264,40 @@ __device__ __forceinline__ bool root_settled(const GeoState& g, int R, int kc, i
   return (uint32_t)k == 0xFFFFFFFFu || (uint32_t)(k >> 32) < (uint32_t)round || (int)(uint32_t)k >= kc;
 }
 
-// Decides one pending cancellation (index c, saddle-id order); returns true when it must wait.
-__device__ __forceinline__ bool geo_apply_one(const ArcSide& A, const GeoState& g, const int4 it, int64_t j, int round,
-                                              uint64_t tag) {
-  const int4 r = *reinterpret_cast<const int4*>(g.rr + 4 * j);
-  const int R0 = r.x, R1 = r.y;
-  const int32_t L0 = r.z, L1 = r.w;
-  const int c = it.x;
-  const int d = g.cd[c];
-  const int kc = it.y;
-  int kd = -1;
-  if (R0 == d && R0 != R1) kd = 0;
-  else if (R1 == d && R0 != R1) kd = 1;
-  bool ready = root_settled(g, R0, kc, round) && root_settled(g, R1, kc, round);
-  if (ready && kd >= 0) ready = g.watch[d] == (tag | (uint32_t)kc);
-  if (!ready) return true;
+// Cancellation c (item it, round results r = (R0, R1, L0, L1), dead extremum d) is decided: record
+// it, and link d when it is one of c's roots (kd).
+__device__ __forceinline__ void geo_decide(const ArcSide& A, const GeoState& g, const int4 it, const int4 r, int d,
+                                           int kd, int round) {
+  const int c = it.x, kc = it.y;
   g.kill[d] = ((uint64_t)(uint32_t)round << 32) | (uint32_t)kc;  // c is decided (d is c's dead extremum)
   if (kd >= 0) {
     const int s = g.cs[c], ko = 1 - kd;
-    const int Ro = kd == 0 ? R1 : R0;
-    const int32_t Ld = kd == 0 ? L0 : L1, Lo = kd == 0 ? L1 : L0;
+    const int Ro = kd == 0 ? r.y : r.x;
+    const int32_t Ld = kd == 0 ? r.z : r.w, Lo = kd == 0 ? r.w : r.z;
     const int32_t ll = sat_add(sat_add(Ld, A.blen[2 * s + kd]), sat_add(A.blen[2 * s + ko], Lo));
     g.link[d] = make_int4(Ro, ll, s, kd);
     g.gjump[d] = (uint64_t)(Ro < 0 ? kVoid : (uint32_t)Ro) | ((uint64_t)(uint32_t)ll << 32);
   }
+}
+
+// which of c's current roots is its dead extremum d (-1: neither)
+__device__ __forceinline__ int geo_kd(const int4 r, int d) {
+  if (r.x == d && r.x != r.y) return 0;
+  if (r.y == d && r.x != r.y) return 1;
+  return -1;
+}
+
+// Decides one pending cancellation (index c, saddle-id order); returns true when it must wait.
+__device__ __forceinline__ bool geo_apply_one(const ArcSide& A, const GeoState& g, const int4 it, int64_t j, int round,
+                                              uint64_t tag) {
+  const int4 r = *reinterpret_cast<const int4*>(g.rr + 4 * j);
+  const int d = g.cd[it.x];
+  const int kc = it.y;
+  const int kd = geo_kd(r, d);
+  bool ready = root_settled(g, r.x, kc, round) && root_settled(g, r.y, kc, round);
+  if (ready && kd >= 0) ready = g.watch[d] == (tag | (uint32_t)kc);
+  if (!ready) return true;
+  geo_decide(A, g, it, r, d, kd, round);
   return false;
 }
 
@@ -335,22 +345,48 @@ __global__ __launch_bounds__(256) void k_geo_apply(ArcSide A, GeoState g, int ro
 
 // Both traced sides of one round in one launch; chunks of side 0 come first, so every chunk (and
 // block) has one side.
+// ITEMS consecutive items per thread, their loads issued stage by stage (items and round results;
+// dead extrema and the killers of both roots; the watch words) so that the latencies overlap.
 template <int ITEMS>
 __device__ __forceinline__ void geo_apply_chunk(const ArcSide& A, const GeoState& g, int round, int cur, int64_t start,
                                                 int n, uint64_t tag) {
   const int64_t j0 = start + (int64_t)threadIdx.x * ITEMS;
-  int4 c[ITEMS];
+  const int4* const in = g.list(cur);
+  int4 c[ITEMS], r[ITEMS];
+  int d[ITEMS];
+  bool ready[ITEMS];
+#pragma unroll
+  for (int u = 0; u < ITEMS; ++u) {
+    const bool v = j0 + u < n;
+    c[u] = v ? in[j0 + u] : make_int4(-1, 0, 0, 0);
+    r[u] = v ? *reinterpret_cast<const int4*>(g.rr + 4 * (j0 + u)) : make_int4(-1, -1, 0, 0);
+  }
+#pragma unroll
+  for (int u = 0; u < ITEMS; ++u) {
+    d[u] = -1;
+    ready[u] = false;
+    if (c[u].x >= 0) {
+      d[u] = g.cd[c[u].x];
+      const bool s0 = root_settled(g, r[u].x, c[u].y, round), s1 = root_settled(g, r[u].y, c[u].y, round);
+      ready[u] = s0 & s1;
+    }
+  }
+  int kd[ITEMS];
+#pragma unroll
+  for (int u = 0; u < ITEMS; ++u) {
+    kd[u] = c[u].x >= 0 ? geo_kd(r[u], d[u]) : -1;
+    if (ready[u] && kd[u] >= 0) ready[u] = g.watch[d[u]] == (tag | (uint32_t)c[u].y);
+  }
   int32_t nwait = 0;
   uint32_t waitmask = 0;
 #pragma unroll
   for (int u = 0; u < ITEMS; ++u) {
-    c[u] = make_int4(0, 0, 0, 0);
-    if (j0 + u < n) {
-      c[u] = g.list(cur)[j0 + u];
-      if (geo_apply_one(A, g, c[u], j0 + u, round, tag)) {
-        waitmask |= 1u << u;
-        ++nwait;
-      }
+    if (c[u].x < 0) continue;
+    if (!ready[u]) {
+      waitmask |= 1u << u;
+      ++nwait;
+    } else {
+      geo_decide(A, g, c[u], r[u], d[u], kd[u], round);
     }
   }
   int32_t* const cnt[1] = {&g.cnt[cur ^ 1]};
