@@ -41,13 +41,13 @@ __device__ __forceinline__ int uf_find(int32_t* __restrict__ p, int x, int32_t* 
 }
 
 // rr[2i] = dying root (-2: cycle); rr[2i+1] = (surviving root + 1) << 1 | (dying root came from c[2i+1])
-// Round 0 starts from the saddle's extrema, later rounds from the roots it saw when it last waited
-// (the same components, fewer links to walk). Returns the two minkey posts (null: none).
+// Round 0 starts from the saddle's extrema (cc = c[i]), later rounds from the roots it saw when it
+// last waited (cc = croot[i]: the same components, fewer links to walk); ps = pos[i]. Returns the
+// two minkey posts (null: none).
 template <bool DUAL>
-__device__ __forceinline__ RootPosts persist_find_one(const PersistSide& s, const int i, int64_t j, uint64_t tag,
-                                                      bool first) {
+__device__ __forceinline__ RootPosts persist_find_decide(const PersistSide& s, const int i, int64_t j, uint64_t tag,
+                                                         bool first, const int2 cc, uint32_t ps) {
   RootPosts p;
-  const int2 cc = *reinterpret_cast<const int2*>((first ? s.c : s.croot) + 2 * i);
   // round 0: no merges yet, every extremum is its own root
   const int ra = first ? (cc.x < 0 ? -1 : cc.x) : uf_find(s.par_g, cc.x, s.err);
   const int rb = first ? (cc.y < 0 ? -1 : cc.y) : uf_find(s.par_g, cc.y, s.err);
@@ -71,15 +71,59 @@ __device__ __forceinline__ RootPosts persist_find_one(const PersistSide& s, cons
   }
   const int dying = b_dies ? rb : ra, surv = b_dies ? ra : rb;
   *reinterpret_cast<int2*>(s.rr + 2 * j) = make_int2(dying, ((surv + 1) << 1) | (b_dies ? 1 : 0));
-  p.key = tag | (uint32_t)s.pos[i];
+  p.key = tag | ps;
   p.a = &s.minkey[dying];
   if (surv >= 0) p.b = &s.minkey[surv];
   return p;
 }
+template <bool DUAL>
+__device__ __forceinline__ RootPosts persist_find_one(const PersistSide& s, const int i, int64_t j, uint64_t tag,
+                                                      bool first) {
+  const int2 cc = *reinterpret_cast<const int2*>((first ? s.c : s.croot) + 2 * i);
+  return persist_find_decide<DUAL>(s, i, j, tag, first, cc, (uint32_t)s.pos[i]);
+}
 
-// Worklists keep saddle-id order, so the lanes of a wave hold nearby saddles that often share an
-// extremum: their posts are combined per wave first. From round 1 on the posting lane also reads
-// before its atomic (hot roots of big components).
+// One side's list, ITEMS items per lane: a wave takes 64 * ITEMS consecutive items, item u of a lane
+// at base + 64 u + lane, and issues each stage's loads (list entries; extrema or cached roots and
+// sweep positions) for all its items before the next, so that their latencies overlap. Worklists
+// keep saddle-id order, so the lanes of a wave hold nearby saddles that often share an extremum:
+// their posts are combined per wave first; from round 1 on the posting lane also reads before its
+// atomic (hot roots of big components).
+template <bool DUAL, int ITEMS>
+__device__ __forceinline__ void persist_find_list(const PersistSide& s, int cur, int64_t n, uint64_t tag, bool first) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  const int32_t* const in = s.list(cur);
+  const int32_t* const cs = first ? s.c : s.croot;
+  for (int64_t base = wave * 64 * ITEMS; base < n; base += nwaves * 64 * ITEMS) {
+    int i[ITEMS];
+    int2 cc[ITEMS];
+    uint32_t ps[ITEMS];
+#pragma unroll
+    for (int u = 0; u < ITEMS; ++u) {
+      const int64_t j = base + 64 * u + lane;
+      i[u] = j < n ? (first ? (int)j : in[j]) : -1;  // list 0 holds every saddle in id order
+    }
+#pragma unroll
+    for (int u = 0; u < ITEMS; ++u) {
+      cc[u] = make_int2(-1, -1);
+      ps[u] = 0;
+      if (i[u] >= 0) {
+        cc[u] = *reinterpret_cast<const int2*>(cs + 2 * i[u]);
+        ps[u] = (uint32_t)s.pos[i[u]];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < ITEMS; ++u) {
+      RootPosts p;
+      if (i[u] >= 0) p = persist_find_decide<DUAL>(s, i[u], base + 64 * u + lane, tag, first, cc[u], ps[u]);
+      wave_post_min(p.a, p.key, !first);
+      wave_post_min(p.b, p.key, !first);
+    }
+  }
+}
+
 template <bool DUAL>
 __device__ __forceinline__ void persist_find_phase(const PersistSide& s0, const PersistSide& s1, int round) {
   const int cur = round & 1;
@@ -90,17 +134,13 @@ __device__ __forceinline__ void persist_find_phase(const PersistSide& s0, const 
   }
   const uint64_t tag = (uint64_t)(0xFFFFFFFFu - (uint32_t)round) << 32;
   const bool first = round == 0;
-  DMSC_WAVE_LOOP(jw, (int64_t)n0 + n1) {
-    // worklists hold saddle indices in saddle-id order (the sweep position is the merge key); the
-    // round's decisions go to rr at the item's list position
-    const int64_t j = jw_base + (threadIdx.x & 63);
-    RootPosts p;
-    if (j < (int64_t)n0 + n1) {
-      if (j >= n0) p = persist_find_one<DUAL>(s1, s1.list(cur)[j - n0], j - n0, tag, first);
-      else p = persist_find_one<DUAL>(s0, s0.list(cur)[j], j, tag, first);
-    }
-    wave_post_min(p.a, p.key, !first);
-    wave_post_min(p.b, p.key, !first);
+  // the round's decisions go to rr at the item's list position
+  if ((int64_t)n0 + n1 > (1 << 16)) {
+    persist_find_list<DUAL, 4>(s0, cur, n0, tag, first);
+    persist_find_list<DUAL, 4>(s1, cur, n1, tag, first);
+  } else {
+    persist_find_list<DUAL, 1>(s0, cur, n0, tag, first);
+    persist_find_list<DUAL, 1>(s1, cur, n1, tag, first);
   }
 }
 template <bool DUAL>
