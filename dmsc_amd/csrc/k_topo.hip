@@ -149,21 +149,20 @@ __global__ void k_persist_find(PersistSide s0, PersistSide s1, int round, const 
   persist_find_phase<DUAL>(s0, s1, round);
 }
 
-// Applies saddle i, which holds minkey of its dying root (r = its rr pair).
+// Applies saddle i, which holds minkey of its dying root (r = its rr pair; sv its value, dv the dying
+// extremum's, thr its field's threshold).
 template <bool DUAL>
-__device__ __forceinline__ void persist_apply_winner(const PersistSide& s, const int i, const int2 r,
-                                                     uint8_t* __restrict__ removed) {
+__device__ __forceinline__ void persist_apply_values(const PersistSide& s, const int i, const int2 r, float sv, float dv,
+                                                     float thr, uint8_t* __restrict__ removed) {
   const int dying = r.x;
   const int surv = (r.y >> 1) - 1;
   const bool b_dies = r.y & 1;
-  const float sv = s.sval[i], dv = key_value(s.ekey[dying]);
   float p;
   if (s.maxside) p = DUAL ? sv - dv : dv - sv;
   else p = DUAL ? dv - sv : sv - dv;
   s.pers[i] = p;
   s.pair[i] = dying;
   s.par_g[dying] = surv;
-  const float thr = s.thr[s.sfield ? s.sfield[i] : 0];
   if (p <= thr) {
     s.alive[dying] = 0;
     removed[i] = 1;
@@ -174,6 +173,11 @@ __device__ __forceinline__ void persist_apply_winner(const PersistSide& s, const
     if (d != t && d != -1) s.par_t[d] = t;
     if (s.cancel_d) s.cancel_d[i] = d;
   }
+}
+template <bool DUAL>
+__device__ __forceinline__ void persist_apply_winner(const PersistSide& s, const int i, const int2 r,
+                                                     uint8_t* __restrict__ removed) {
+  persist_apply_values<DUAL>(s, i, r, s.sval[i], key_value(s.ekey[r.x]), s.thr[s.sfield ? s.sfield[i] : 0], removed);
 }
 
 // saddle i waits with its decision r: the next find resumes from its roots
@@ -228,7 +232,7 @@ __device__ __forceinline__ void persist_apply_chunk(const PersistSide& s, int cu
     }
   }
   int32_t nwait = 0;
-  uint32_t waitmask = 0;
+  uint32_t waitmask = 0, winmask = 0;
 #pragma unroll
   for (int u = 0; u < ITEMS; ++u) {
     if (r[u].x < 0) continue;  // a cycle (decided in the find) or past the list
@@ -237,9 +241,23 @@ __device__ __forceinline__ void persist_apply_chunk(const PersistSide& s, int cu
       waitmask |= 1u << u;
       ++nwait;
     } else {
-      persist_apply_winner<DUAL>(s, k[u], r[u], removed);
+      winmask |= 1u << u;
     }
   }
+  // winners: their value loads for all items first, then the decisions
+  float sv[ITEMS], dv[ITEMS], th[ITEMS];
+#pragma unroll
+  for (int u = 0; u < ITEMS; ++u) {
+    sv[u] = dv[u] = th[u] = 0.f;
+    if (winmask >> u & 1) {
+      sv[u] = s.sval[k[u]];
+      dv[u] = key_value(s.ekey[r[u].x]);
+      th[u] = s.thr[s.sfield ? s.sfield[k[u]] : 0];
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < ITEMS; ++u)
+    if (winmask >> u & 1) persist_apply_values<DUAL>(s, k[u], r[u], sv[u], dv[u], th[u], removed);
   int32_t* const cnt[1] = {&s.cnt[cur ^ 1]};
   const int32_t req[1] = {nwait};
   int32_t first[1];
