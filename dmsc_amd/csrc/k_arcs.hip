@@ -194,6 +194,13 @@ __device__ __forceinline__ void geo_root(const GeoState& g, int x, int& root, in
   len = acc;
 }
 
+// watch[R] is only ever read by R's killer (condition B of its decision), so cancellation kc posts on
+// root R only while R's killer is pending and does not come before kc (an earlier killer is not
+// affected by a larger key, and a decided or absent killer reads nothing).
+__device__ __forceinline__ bool watch_needed(uint64_t kill_word, int kc) {
+  return (kill_word >> 32) == 0xFFFFFFFFull && (uint32_t)kill_word != 0xFFFFFFFFu && (int)(uint32_t)kill_word >= kc;
+}
+
 // worklist item: (cancellation, key, arc target 0, arc target 1); results go to rr at the item's
 // list position. In round 0 the forest has no links yet: every target is its own root at length 0.
 // Returns the watch posts on the two roots.
@@ -208,8 +215,8 @@ __device__ __forceinline__ RootPosts geo_find_one(const ArcSide& A, const GeoSta
   *reinterpret_cast<int4*>(g.rr + 4 * j) = make_int4(R0, R1, L0, L1);
   RootPosts p;
   p.key = tag | (uint32_t)it.y;
-  if (R0 >= 0) p.a = &g.watch[R0];
-  if (R1 >= 0) p.b = &g.watch[R1];
+  if (R0 >= 0 && watch_needed(g.kill[R0], it.y)) p.a = &g.watch[R0];
+  if (R1 >= 0 && watch_needed(g.kill[R1], it.y)) p.b = &g.watch[R1];
   return p;
 }
 
