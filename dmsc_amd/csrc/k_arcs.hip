@@ -201,9 +201,18 @@ __device__ __forceinline__ bool watch_needed(uint64_t kill_word, int kc) {
   return (kill_word >> 32) == 0xFFFFFFFFull && (uint32_t)kill_word != 0xFFFFFFFFu && (int)(uint32_t)kill_word >= kc;
 }
 
+// R (kill word k) stays a root until after cancellation `kc` (key): the cancellation that could link
+// it is done, absent, or comes later in (persistence, saddle id) order. Evaluated in the find: a
+// killer decided during this round's apply carries this round's number, so the answer there is the
+// same as here.
+__device__ __forceinline__ bool root_settled(uint64_t k, int kc, int round) {
+  return (uint32_t)k == 0xFFFFFFFFu || (uint32_t)(k >> 32) < (uint32_t)round || (int)(uint32_t)k >= kc;
+}
+
 // worklist item: (cancellation, key, arc target 0, arc target 1); results go to rr at the item's
-// list position. In round 0 the forest has no links yet: every target is its own root at length 0.
-// Returns the watch posts on the two roots.
+// list position: (R0, R1, L0, L1) with bit 31 of a length set when that root is not settled. In
+// round 0 the forest has no links yet: every target is its own root at length 0. Returns the watch
+// posts on the two roots.
 __device__ __forceinline__ RootPosts geo_find_one(const ArcSide& A, const GeoState& g, const int4 it, int64_t j,
                                                   uint64_t tag, bool first) {
   int R0 = it.z, R1 = it.w;
@@ -212,11 +221,14 @@ __device__ __forceinline__ RootPosts geo_find_one(const ArcSide& A, const GeoSta
     geo_root(g, it.z, R0, L0);
     geo_root(g, it.w, R1, L1);
   }
-  *reinterpret_cast<int4*>(g.rr + 4 * j) = make_int4(R0, R1, L0, L1);
+  const int round = (int)(0xFFFFFFFFu - (uint32_t)(tag >> 32));
+  const uint64_t k0 = R0 >= 0 ? g.kill[R0] : ~0ull, k1 = R1 >= 0 ? g.kill[R1] : ~0ull;
+  const int32_t f0 = root_settled(k0, it.y, round) ? 0 : INT32_MIN, f1 = root_settled(k1, it.y, round) ? 0 : INT32_MIN;
+  *reinterpret_cast<int4*>(g.rr + 4 * j) = make_int4(R0, R1, L0 | f0, L1 | f1);
   RootPosts p;
   p.key = tag | (uint32_t)it.y;
-  if (R0 >= 0 && watch_needed(g.kill[R0], it.y)) p.a = &g.watch[R0];
-  if (R1 >= 0 && watch_needed(g.kill[R1], it.y)) p.b = &g.watch[R1];
+  if (R0 >= 0 && watch_needed(k0, it.y)) p.a = &g.watch[R0];
+  if (R1 >= 0 && watch_needed(k1, it.y)) p.b = &g.watch[R1];
   return p;
 }
 
@@ -263,16 +275,9 @@ __global__ void k_geo_find2(ArcSide A0, GeoState G0, ArcSide A1, GeoState G1, in
 }
 __global__ void k_geo_find(ArcSide A, GeoState g, int round) { geo_find_phase(A, g, round); }
 
-// R stays a root until after cancellation `kc` (key): the cancellation that could link it is done,
-// absent, or comes later in (persistence, saddle id) order.
-__device__ __forceinline__ bool root_settled(const GeoState& g, int R, int kc, int round) {
-  if (R < 0) return true;
-  const uint64_t k = g.kill[R];
-  return (uint32_t)k == 0xFFFFFFFFu || (uint32_t)(k >> 32) < (uint32_t)round || (int)(uint32_t)k >= kc;
-}
-
-// Cancellation c (item it, round results r = (R0, R1, L0, L1), dead extremum d) is decided: record
-// it, and link d when it is one of c's roots (kd).
+// Cancellation c (item it, round results r = (R0, R1, L0, L1) with both roots settled, so the
+// lengths carry no flag; dead extremum d) is decided: record it, and link d when it is one of c's
+// roots (kd).
 __device__ __forceinline__ void geo_decide(const ArcSide& A, const GeoState& g, const int4 it, const int4 r, int d,
                                            int kd, int round) {
   const int c = it.x, kc = it.y;
@@ -301,7 +306,7 @@ __device__ __forceinline__ bool geo_apply_one(const ArcSide& A, const GeoState& 
   const int d = g.cd[it.x];
   const int kc = it.y;
   const int kd = geo_kd(r, d);
-  bool ready = root_settled(g, r.x, kc, round) && root_settled(g, r.y, kc, round);
+  bool ready = r.z >= 0 && r.w >= 0;  // both roots settled (flags of the find)
   if (ready && kd >= 0) ready = g.watch[d] == (tag | (uint32_t)kc);
   if (!ready) return true;
   geo_decide(A, g, it, r, d, kd, round);
@@ -374,8 +379,7 @@ __device__ __forceinline__ void geo_apply_chunk(const ArcSide& A, const GeoState
     ready[u] = false;
     if (c[u].x >= 0) {
       d[u] = g.cd[c[u].x];
-      const bool s0 = root_settled(g, r[u].x, c[u].y, round), s1 = root_settled(g, r[u].y, c[u].y, round);
-      ready[u] = s0 & s1;
+      ready[u] = r[u].z >= 0 && r[u].w >= 0;  // both roots settled (flags of the find)
     }
   }
   int kd[ITEMS];
