@@ -71,6 +71,7 @@ __device__ __forceinline__ RootPosts persist_find_decide(const PersistSide& s, c
   }
   const int dying = b_dies ? rb : ra, surv = b_dies ? ra : rb;
   *reinterpret_cast<int2*>(s.rr + 2 * j) = make_int2(dying, ((surv + 1) << 1) | (b_dies ? 1 : 0));
+  s.rkey[j] = ps;
   p.key = tag | ps;
   p.a = &s.minkey[dying];
   if (surv >= 0) p.b = &s.minkey[surv];
@@ -192,7 +193,7 @@ __device__ __forceinline__ bool persist_apply_one(const PersistSide& s, const in
                                                   uint8_t* __restrict__ removed) {
   const int2 r = *reinterpret_cast<const int2*>(s.rr + 2 * j);
   if (r.x == -2) return false;
-  if (s.minkey[r.x] != (tag | (uint32_t)s.pos[i])) {
+  if (s.minkey[r.x] != (tag | s.rkey[j])) {
     persist_wait(s, i, r);
     return true;
   }
@@ -221,15 +222,12 @@ __device__ __forceinline__ void persist_apply_chunk(const PersistSide& s, int cu
     const bool v = j0 + u < n;
     k[u] = v ? in[j0 + u] : 0;
     r[u] = v ? *reinterpret_cast<const int2*>(s.rr + 2 * (j0 + u)) : make_int2(-3, 0);  // -3: past the list
+    ps[u] = v ? s.rkey[j0 + u] : 0u;
   }
 #pragma unroll
   for (int u = 0; u < ITEMS; ++u) {
     mk[u] = 0;
-    ps[u] = 0;
-    if (r[u].x >= 0) {
-      mk[u] = s.minkey[r[u].x];
-      ps[u] = (uint32_t)s.pos[k[u]];
-    }
+    if (r[u].x >= 0) mk[u] = s.minkey[r[u].x];
   }
   int32_t nwait = 0;
   uint32_t waitmask = 0, winmask = 0;

@@ -20,6 +20,7 @@ struct PersistSide {
   int32_t* cancel_d;    // per saddle thresholded dead extremum (null when arcs are not traced)
   int32_t* rr;          // per worklist position: dying root (-2 = cycle), (surviving root + 1) << 1 | side bit
   int32_t* croot;       // per saddle: its two component roots when it last had to wait (finds resume there)
+  uint32_t* rkey;       // per worklist position: the saddle's sweep position (its merge key), from the find
   int32_t* wl;          // ping-pong worklists of pending saddle indices: list k at wl + k * wls
   int64_t wls;
   // (one buffer and an offset rather than an array of two pointers: a run-time index into a struct
