@@ -36,7 +36,7 @@ enum : int {
   A_CMAX, A_CMIN, A_LMAX, A_LMIN, A_META, A_BASES32, A_SADFIELD, A_CONTRIB, A_SURV1, A_SURVR1,
   A_KEYS, A_KEYS2, A_VALS2, A_ASC, A_ORDMIN, A_ORDMAX, A_REMOVED, A_POSMIN, A_POSMAX, A_RSP0, A_RSP1,
   A_SIDE0,  // per persistence side: kSideSlots slots each
-  A_SIDE_END = A_SIDE0 + 2 * 14,
+  A_SIDE_END = A_SIDE0 + 2 * 15,
   A_ASM0 = A_SIDE_END,  // assembly: counts, prefixes, sad_to_out, survivor lists (12 slots)
   A_ASM_END = A_ASM0 + 12,
   A_ARC0 = A_ASM_END,  // per arc side: kArcSlots slots each
@@ -44,7 +44,7 @@ enum : int {
 };
 
 enum SideSlot { P_PARG, P_PART, P_ALIVE, P_MINKEY, P_PAIR, P_PERS, P_CANCEL, P_RR, P_WL, P_CNT, P_ROOT, P_TOOUT,
-                P_CROOT, P_RKEY, kSideSlots };
+                P_CROOT, P_RKEY, P_WKEY, kSideSlots };
 static_assert(A_SIDE_END - A_SIDE0 == 2 * kSideSlots, "side slot count");
 enum ArcSlot {
   G_BLEN, G_BOFF, G_ARENA, G_FLAG, G_PRE, G_CS0, G_CD0, G_KEYS, G_SLOT, G_KEYS2, G_SLOT2, G_CS, G_CD, G_DEADC,
@@ -121,7 +121,7 @@ void readback(dmsc_context* c, const void* dev, size_t bytes, hipStream_t st) {
 
 struct SideBufs {
   int32_t *par_g, *par_t, *pair, *cancel, *rr, *wl, *cnt, *root, *to_out, *croot;
-  uint32_t* rkey;
+  uint32_t *rkey, *wkey;
   uint8_t* alive;
   uint64_t* minkey;
   float* pers;
@@ -142,7 +142,8 @@ SideBufs side_bufs(Arena& ar, int side, int64_t next, int64_t ns) {
   s.cnt = ar.get<int32_t>(b + P_CNT, 2);
   s.root = ar.get<int32_t>(b + P_ROOT, next);
   s.to_out = ar.get<int32_t>(b + P_TOOUT, next);
-  s.croot = ar.get<int32_t>(b + P_CROOT, 2 * ns);
+  s.croot = ar.get<int32_t>(b + P_CROOT, 4 * ns);
+  s.wkey = ar.get<uint32_t>(b + P_WKEY, 2 * ns);
   s.rkey = ar.get<uint32_t>(b + P_RKEY, ns);
   return s;
 }
@@ -555,6 +556,7 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     p.cancel_d = record[s] ? sb[s].cancel : nullptr;
     p.rr = sb[s].rr;
     p.croot = sb[s].croot;
+    p.wkey = sb[s].wkey;
     p.rkey = sb[s].rkey;
     p.wl = sb[s].wl;
     p.wls = nsad;

@@ -303,12 +303,11 @@ __device__ __forceinline__ int geo_kd(const int4 r, int d) {
 __device__ __forceinline__ bool geo_apply_one(const ArcSide& A, const GeoState& g, const int4 it, int64_t j, int round,
                                               uint64_t tag) {
   const int4 r = *reinterpret_cast<const int4*>(g.rr + 4 * j);
+  if (r.z < 0 || r.w < 0) return true;  // a root is not settled (flags of the find)
   const int d = g.cd[it.x];
   const int kc = it.y;
   const int kd = geo_kd(r, d);
-  bool ready = r.z >= 0 && r.w >= 0;  // both roots settled (flags of the find)
-  if (ready && kd >= 0) ready = g.watch[d] == (tag | (uint32_t)kc);
-  if (!ready) return true;
+  if (kd >= 0 && g.watch[d] != (tag | (uint32_t)kc)) return true;
   geo_decide(A, g, it, r, d, kd, round);
   return false;
 }
@@ -376,16 +375,13 @@ __device__ __forceinline__ void geo_apply_chunk(const ArcSide& A, const GeoState
 #pragma unroll
   for (int u = 0; u < ITEMS; ++u) {
     d[u] = -1;
-    ready[u] = false;
-    if (c[u].x >= 0) {
-      d[u] = g.cd[c[u].x];
-      ready[u] = r[u].z >= 0 && r[u].w >= 0;  // both roots settled (flags of the find)
-    }
+    ready[u] = c[u].x >= 0 && r[u].z >= 0 && r[u].w >= 0;  // both roots settled (flags of the find)
+    if (ready[u]) d[u] = g.cd[c[u].x];  // the dead extremum matters only then
   }
   int kd[ITEMS];
 #pragma unroll
   for (int u = 0; u < ITEMS; ++u) {
-    kd[u] = c[u].x >= 0 ? geo_kd(r[u], d[u]) : -1;
+    kd[u] = ready[u] ? geo_kd(r[u], d[u]) : -1;
     if (ready[u] && kd[u] >= 0) ready[u] = g.watch[d[u]] == (tag | (uint32_t)c[u].y);
   }
   int32_t nwait = 0;

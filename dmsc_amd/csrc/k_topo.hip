@@ -78,10 +78,11 @@ __device__ __forceinline__ RootPosts persist_find_decide(const PersistSide& s, c
   return p;
 }
 template <bool DUAL>
-__device__ __forceinline__ RootPosts persist_find_one(const PersistSide& s, const int i, int64_t j, uint64_t tag,
+__device__ __forceinline__ RootPosts persist_find_one(const PersistSide& s, const int i, int64_t j, int cur, uint64_t tag,
                                                       bool first) {
-  const int2 cc = *reinterpret_cast<const int2*>((first ? s.c : s.croot) + 2 * i);
-  return persist_find_decide<DUAL>(s, i, j, tag, first, cc, (uint32_t)s.pos[i]);
+  if (first) return persist_find_decide<DUAL>(s, i, j, tag, first, *reinterpret_cast<const int2*>(s.c + 2 * i),
+                                              (uint32_t)s.pos[i]);
+  return persist_find_decide<DUAL>(s, i, j, tag, first, s.roots(cur)[j], s.keys(cur)[j]);
 }
 
 // One side's list, ITEMS items per lane: a wave takes 64 * ITEMS consecutive items, item u of a lane
@@ -96,7 +97,6 @@ __device__ __forceinline__ void persist_find_list(const PersistSide& s, int cur,
   const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
   const int32_t* const in = s.list(cur);
-  const int32_t* const cs = first ? s.c : s.croot;
   for (int64_t base = wave * 64 * ITEMS; base < n; base += nwaves * 64 * ITEMS) {
     int i[ITEMS];
     int2 cc[ITEMS];
@@ -111,8 +111,14 @@ __device__ __forceinline__ void persist_find_list(const PersistSide& s, int cur,
       cc[u] = make_int2(-1, -1);
       ps[u] = 0;
       if (i[u] >= 0) {
-        cc[u] = *reinterpret_cast<const int2*>(cs + 2 * i[u]);
-        ps[u] = (uint32_t)s.pos[i[u]];
+        if (first) {
+          cc[u] = *reinterpret_cast<const int2*>(s.c + 2 * i[u]);
+          ps[u] = (uint32_t)s.pos[i[u]];
+        } else {  // by list position: written where the apply appended the waiting saddle
+          const int64_t j = base + 64 * u + lane;
+          cc[u] = s.roots(cur)[j];
+          ps[u] = s.keys(cur)[j];
+        }
       }
     }
 #pragma unroll
@@ -181,10 +187,14 @@ __device__ __forceinline__ void persist_apply_winner(const PersistSide& s, const
   persist_apply_values<DUAL>(s, i, r, s.sval[i], key_value(s.ekey[r.x]), s.thr[s.sfield ? s.sfield[i] : 0], removed);
 }
 
-// saddle i waits with its decision r: the next find resumes from its roots
-__device__ __forceinline__ void persist_wait(const PersistSide& s, const int i, const int2 r) {
+// saddle i, waiting with decision r and sweep position ps, takes slot o of list k: the next find
+// resumes from its roots
+__device__ __forceinline__ void persist_park(const PersistSide& s, int k, int o, const int i, const int2 r,
+                                             uint32_t ps) {
   const int surv = (r.y >> 1) - 1;
-  *reinterpret_cast<int2*>(s.croot + 2 * i) = (r.y & 1) ? make_int2(surv, r.x) : make_int2(r.x, surv);
+  s.list(k)[o] = i;
+  s.roots(k)[o] = (r.y & 1) ? make_int2(surv, r.x) : make_int2(r.x, surv);
+  s.keys(k)[o] = ps;
 }
 
 // Decides one pending saddle i; returns true when it must wait for a later round.
@@ -193,10 +203,7 @@ __device__ __forceinline__ bool persist_apply_one(const PersistSide& s, const in
                                                   uint8_t* __restrict__ removed) {
   const int2 r = *reinterpret_cast<const int2*>(s.rr + 2 * j);
   if (r.x == -2) return false;
-  if (s.minkey[r.x] != (tag | s.rkey[j])) {
-    persist_wait(s, i, r);
-    return true;
-  }
+  if (s.minkey[r.x] != (tag | s.rkey[j])) return true;
   persist_apply_winner<DUAL>(s, i, r, removed);
   return false;
 }
@@ -235,7 +242,6 @@ __device__ __forceinline__ void persist_apply_chunk(const PersistSide& s, int cu
   for (int u = 0; u < ITEMS; ++u) {
     if (r[u].x < 0) continue;  // a cycle (decided in the find) or past the list
     if (mk[u] != (tag | ps[u])) {
-      persist_wait(s, k[u], r[u]);
       waitmask |= 1u << u;
       ++nwait;
     } else {
@@ -263,7 +269,7 @@ __device__ __forceinline__ void persist_apply_chunk(const PersistSide& s, int cu
   int32_t o = first[0];
 #pragma unroll
   for (int u = 0; u < ITEMS; ++u)
-    if (waitmask >> u & 1) s.list(cur ^ 1)[o++] = k[u];
+    if (waitmask >> u & 1) persist_park(s, cur ^ 1, o++, k[u], r[u], ps[u]);
 }
 
 // Chunks of side 0 come first, then those of side 1, so every chunk (and block) has one side.
@@ -344,7 +350,8 @@ template <bool DUAL>
 __device__ __forceinline__ void persist_tail_apply(const PersistSide& s, int j, int cur, uint64_t tag,
                                                    uint8_t* __restrict__ removed, int32_t* s_next) {
   const int i = s.list(cur)[j];
-  if (persist_apply_one<DUAL>(s, i, j, tag, removed)) s.list(cur ^ 1)[atomicAdd(s_next, 1)] = i;
+  if (persist_apply_one<DUAL>(s, i, j, tag, removed))
+    persist_park(s, cur ^ 1, atomicAdd(s_next, 1), i, *reinterpret_cast<const int2*>(s.rr + 2 * j), s.rkey[j]);
 }
 
 // The tail of the sweep in one workgroup: once few saddles are pending, a round would cost two
@@ -365,8 +372,8 @@ __global__ __launch_bounds__(kTailThreads) void k_persist_tail(PersistSide s0, P
     const uint64_t tag = (uint64_t)(0xFFFFFFFFu - (uint32_t)r) << 32;
     if (threadIdx.x == 0) s_next[0] = s_next[1] = 0;
     for (int j = threadIdx.x; j < n0 + n1; j += kTailThreads) {
-      const RootPosts p = j >= n0 ? persist_find_one<DUAL>(s1, s1.list(cur)[j - n0], j - n0, tag, r == 0)
-                                  : persist_find_one<DUAL>(s0, s0.list(cur)[j], j, tag, r == 0);
+      const RootPosts p = j >= n0 ? persist_find_one<DUAL>(s1, s1.list(cur)[j - n0], j - n0, cur, tag, r == 0)
+                                  : persist_find_one<DUAL>(s0, s0.list(cur)[j], j, cur, tag, r == 0);
       if (p.a) post_min(p.a, p.key, r > 0);
       if (p.b) post_min(p.b, p.key, r > 0);
     }

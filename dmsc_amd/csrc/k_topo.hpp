@@ -19,13 +19,17 @@ struct PersistSide {
   float* pers;          // per saddle persistence (-1 for a cycle)
   int32_t* cancel_d;    // per saddle thresholded dead extremum (null when arcs are not traced)
   int32_t* rr;          // per worklist position: dying root (-2 = cycle), (surviving root + 1) << 1 | side bit
-  int32_t* croot;       // per saddle: its two component roots when it last had to wait (finds resume there)
+  int32_t* croot;       // per position of each worklist (2 ints, list k at croot + 2 k wls): the two component
+                        // roots the saddle saw when it last had to wait (the next find resumes there)
+  uint32_t* wkey;       // per position of each worklist (list k at wkey + k wls): the saddle's sweep position
   uint32_t* rkey;       // per worklist position: the saddle's sweep position (its merge key), from the find
   int32_t* wl;          // ping-pong worklists of pending saddle indices: list k at wl + k * wls
   int64_t wls;
   // (one buffer and an offset rather than an array of two pointers: a run-time index into a struct
   // member array makes the compiler keep the kernel argument in scratch memory)
   __host__ __device__ int32_t* list(int k) const { return wl + (int64_t)k * wls; }
+  __host__ __device__ int2* roots(int k) const { return reinterpret_cast<int2*>(croot) + (int64_t)k * wls; }
+  __host__ __device__ uint32_t* keys(int k) const { return wkey + (int64_t)k * wls; }
   int32_t* cnt;         // [2] worklist sizes (device)
   int32_t* err;         // device error flags
   const float* thr;     // per field persistence threshold
