@@ -40,7 +40,7 @@ enum : int {
   A_ASM0 = A_SIDE_END,  // assembly: counts, prefixes, sad_to_out, survivor lists (12 slots)
   A_ASM_END = A_ASM0 + 12,
   A_ARC0 = A_ASM_END,  // per arc side: kArcSlots slots each
-  A_ARC_END = A_ARC0 + 2 * 21,
+  A_ARC_END = A_ARC0 + 2 * 22,
 };
 
 enum SideSlot { P_PARG, P_PART, P_ALIVE, P_MINKEY, P_PAIR, P_PERS, P_CANCEL, P_RR, P_WL, P_CNT, P_ROOT, P_TOOUT,
@@ -48,7 +48,7 @@ enum SideSlot { P_PARG, P_PART, P_ALIVE, P_MINKEY, P_PAIR, P_PERS, P_CANCEL, P_R
 static_assert(A_SIDE_END - A_SIDE0 == 2 * kSideSlots, "side slot count");
 enum ArcSlot {
   G_BLEN, G_BOFF, G_ARENA, G_FLAG, G_PRE, G_CS0, G_CD0, G_KEYS, G_SLOT, G_KEYS2, G_SLOT2, G_CS, G_CD, G_DEADC,
-  G_LINK, G_DONE, G_WATCH, G_RR, G_WL, G_CNT, G_JUMP, kArcSlots
+  G_LINK, G_DONE, G_WATCH, G_RR, G_WL, G_CNT, G_JUMP, G_GLEN, kArcSlots
 };
 static_assert(A_ARC_END - A_ARC0 == 2 * kArcSlots, "arc slot count");
 enum ArcSlot2 { H_PJQ = 0, H_FLEN, H_PERPOS, H_POSPRE, H_SEGA, H_SEGB, H_RAW, H_NSEG, H_CUMR, H_CUMC, H_ARCB };
@@ -749,6 +749,7 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     g.rr = ar.get<int32_t>(b + G_RR, 4 * nc);
     g.wl = ar.get<int4>(b + G_WL, 2 * nc);
     g.wls = nc;
+    g.glen = ar.get<int2>(b + G_GLEN, 2 * nc);
     g.cnt = ar.get<int32_t>(b + G_CNT, 2);
     g.err = d_err;
     g.gjump = ar.get<uint64_t>(b + G_JUMP, next);

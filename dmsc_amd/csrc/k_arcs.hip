@@ -213,13 +213,16 @@ __device__ __forceinline__ bool root_settled(uint64_t k, int kc, int round) {
 // list position: (R0, R1, L0, L1) with bit 31 of a length set when that root is not settled. In
 // round 0 the forest has no links yet: every target is its own root at length 0. Returns the watch
 // posts on the two roots.
-__device__ __forceinline__ RootPosts geo_find_one(const ArcSide& A, const GeoState& g, const int4 it, int64_t j,
+__device__ __forceinline__ RootPosts geo_find_one(const ArcSide& A, const GeoState& g, const int4 it, int64_t j, int cur,
                                                   uint64_t tag, bool first) {
   int R0 = it.z, R1 = it.w;
   int32_t L0 = 0, L1 = 0;
-  if (!first) {
+  if (!first) {  // from round 1 on the item holds the roots it last waited with, lens(cur) the lengths to them
     geo_root(g, it.z, R0, L0);
     geo_root(g, it.w, R1, L1);
+    const int2 base = g.lens(cur)[j];
+    L0 = sat_add(base.x, L0);
+    L1 = sat_add(base.y, L1);
   }
   const int round = (int)(0xFFFFFFFFu - (uint32_t)(tag >> 32));
   const uint64_t k0 = R0 >= 0 ? g.kill[R0] : ~0ull, k1 = R1 >= 0 ? g.kill[R1] : ~0ull;
@@ -247,7 +250,7 @@ __device__ __forceinline__ void geo_find_phase(const ArcSide& A, const GeoState&
   DMSC_WAVE_LOOP(jw, (int64_t)n) {
     const int64_t j = jw_base + (threadIdx.x & 63);
     RootPosts p;
-    if (j < n) p = geo_find_one(A, g, g.list(cur)[j], j, tag, round == 0);
+    if (j < n) p = geo_find_one(A, g, g.list(cur)[j], j, cur, tag, round == 0);
     geo_post(p, round == 0);
   }
 }
@@ -267,8 +270,8 @@ __global__ void k_geo_find2(ArcSide A0, GeoState G0, ArcSide A1, GeoState G1, in
     const int64_t j = jw_base + (threadIdx.x & 63);
     RootPosts p;
     if (j < (int64_t)n0 + n1) {
-      if (j >= n0) p = geo_find_one(A1, G1, G1.list(cur)[j - n0], j - n0, tag, round == 0);
-      else p = geo_find_one(A0, G0, G0.list(cur)[j], j, tag, round == 0);
+      if (j >= n0) p = geo_find_one(A1, G1, G1.list(cur)[j - n0], j - n0, cur, tag, round == 0);
+      else p = geo_find_one(A0, G0, G0.list(cur)[j], j, cur, tag, round == 0);
     }
     geo_post(p, round == 0);
   }
@@ -290,6 +293,13 @@ __device__ __forceinline__ void geo_decide(const ArcSide& A, const GeoState& g, 
     g.link[d] = make_int4(Ro, ll, s, kd);
     g.gjump[d] = (uint64_t)(Ro < 0 ? kVoid : (uint32_t)Ro) | ((uint64_t)(uint32_t)ll << 32);
   }
+}
+
+// Waiting item `it` (round results r) takes slot o of list k with the roots it found and the
+// lengths to them, so that the next find resumes there.
+__device__ __forceinline__ void geo_park(const GeoState& g, int k, int o, const int4 it, const int4 r) {
+  g.list(k)[o] = make_int4(it.x, it.y, r.x, r.y);
+  g.lens(k)[o] = make_int2(r.z & INT32_MAX, r.w & INT32_MAX);  // without the settled flags
 }
 
 // which of c's current roots is its dead extremum d (-1: neither)
@@ -341,7 +351,9 @@ __device__ __forceinline__ void geo_apply_chunks(const ArcSide& A, const GeoStat
     int32_t o = first[0];
 #pragma unroll
     for (int u = 0; u < ITEMS; ++u)
-      if (waitmask >> u & 1) g.list(cur ^ 1)[o++] = c[u];
+      if (waitmask >> u & 1) {
+        geo_park(g, cur ^ 1, o++, c[u], *reinterpret_cast<const int4*>(g.rr + 4 * (j0 + u)));
+      }
   }
 }
 
@@ -403,7 +415,7 @@ __device__ __forceinline__ void geo_apply_chunk(const ArcSide& A, const GeoState
   int32_t o = first[0];
 #pragma unroll
   for (int u = 0; u < ITEMS; ++u)
-    if (waitmask >> u & 1) g.list(cur ^ 1)[o++] = c[u];
+    if (waitmask >> u & 1) geo_park(g, cur ^ 1, o++, c[u], r[u]);
 }
 template <int ITEMS>
 __device__ __forceinline__ void geo_apply_both(const ArcSide& A0, const GeoState& G0, const ArcSide& A1,
@@ -442,7 +454,8 @@ void launch_geo_round2(const ArcSide* a, const GeoState* g, int live, int round,
 __device__ __forceinline__ void geo_tail_apply(const ArcSide& A, const GeoState& g, int j, int cur, int r, uint64_t tag,
                                                int32_t* s_next) {
   const int4 it = g.list(cur)[j];
-  if (geo_apply_one(A, g, it, j, r, tag)) g.list(cur ^ 1)[atomicAdd(s_next, 1)] = it;
+  if (geo_apply_one(A, g, it, j, r, tag))
+    geo_park(g, cur ^ 1, atomicAdd(s_next, 1), it, *reinterpret_cast<const int4*>(g.rr + 4 * j));
 }
 
 // The last geometry rounds of both traced sides in one workgroup (same structure as the
@@ -459,8 +472,8 @@ __global__ __launch_bounds__(kTailThreads) void k_geo_tail(ArcSide A0, GeoState 
     const uint64_t tag = (uint64_t)(0xFFFFFFFFu - (uint32_t)r) << 32;
     if (threadIdx.x == 0) s_next[0] = s_next[1] = 0;
     for (int j = threadIdx.x; j < n0 + n1; j += kTailThreads) {
-      const RootPosts p = j >= n0 ? geo_find_one(A1, G1, G1.list(cur)[j - n0], j - n0, tag, r == 0)
-                                  : geo_find_one(A0, G0, G0.list(cur)[j], j, tag, r == 0);
+      const RootPosts p = j >= n0 ? geo_find_one(A1, G1, G1.list(cur)[j - n0], j - n0, cur, tag, r == 0)
+                                  : geo_find_one(A0, G0, G0.list(cur)[j], j, cur, tag, r == 0);
       if (p.a) post_min(p.a, p.key, r > 0);
       if (p.b) post_min(p.b, p.key, r > 0);
     }

@@ -117,6 +117,9 @@ struct GeoState {
   int4* wl;                // worklists (cancellation, key, arc target 0, arc target 1): list k at
   int64_t wls;             // wl + k * wls
   __host__ __device__ int4* list(int k) const { return wl + (int64_t)k * wls; }
+  int2* glen;              // per position of each worklist (list k at glen + k wls): path lengths from the
+                           // cancellation's arc targets to the two roots its item holds from round 1 on
+  __host__ __device__ int2* lens(int k) const { return glen + (int64_t)k * wls; }
   int32_t* cnt;
   int32_t* err;
 };
