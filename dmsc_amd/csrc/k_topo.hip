@@ -41,33 +41,50 @@ __device__ __forceinline__ int uf_find(int32_t* __restrict__ p, int x, int32_t* 
 }
 
 // rr[2i] = dying root (-2: cycle); rr[2i+1] = (surviving root + 1) << 1 | (dying root came from c[2i+1])
-// Round 0 starts from the saddle's extrema (cc = c[i]), later rounds from the roots it saw when it
-// last waited (cc = croot[i]: the same components, fewer links to walk); ps = pos[i]. Returns the
-// two minkey posts (null: none).
+// Round 0 starts from the saddle's extrema (cc = c[i]). Later rounds start from the decision the
+// saddle waited with (cc, in rr's format): while that dying root is still a root it still dies (the
+// other component can only have been merged into an older root, or the void), so only the surviving
+// root is looked up again and no keys are compared; otherwise both roots are found again. ps: the
+// saddle's sweep position. Returns the two minkey posts (null: none).
 template <bool DUAL>
 __device__ __forceinline__ RootPosts persist_find_decide(const PersistSide& s, const int i, int64_t j, uint64_t tag,
                                                          bool first, const int2 cc, uint32_t ps) {
   RootPosts p;
-  // round 0: no merges yet, every extremum is its own root
-  const int ra = first ? (cc.x < 0 ? -1 : cc.x) : uf_find(s.par_g, cc.x, s.err);
-  const int rb = first ? (cc.y < 0 ? -1 : cc.y) : uf_find(s.par_g, cc.y, s.err);
-  if (ra == rb) {
-    s.pair[i] = -1;
-    s.pers[i] = -1.0f;
-    s.rr[2 * j] = -2;
-    return p;
-  }
-  bool b_dies;
-  if (ra == -1) {
-    b_dies = true;
-  } else if (rb == -1) {
-    b_dies = false;
+  int ra, rb;
+  bool b_dies = false, decided = false;
+  if (first) {  // no merges yet, every extremum is its own root
+    ra = cc.x < 0 ? -1 : cc.x;
+    rb = cc.y < 0 ? -1 : cc.y;
   } else {
-    // min side: value_greater -> r1 dies (persistence.hxx:73-80); max side: value_less (:130-137)
-    const uint64_t ka = s.ekey[ra], kb = s.ekey[rb];
-    const bool a_dies = s.maxside ? higher<DUAL>(key_value(kb), key_id(kb), key_value(ka), key_id(ka))
-                                  : higher<DUAL>(key_value(ka), key_id(ka), key_value(kb), key_id(kb));
-    b_dies = !a_dies;
+    const int d0 = cc.x, s0 = (cc.y >> 1) - 1;
+    const bool b0 = cc.y & 1;
+    const int rd = uf_find(s.par_g, d0, s.err);
+    const int rs = uf_find(s.par_g, s0, s.err);
+    ra = b0 ? rs : rd;
+    rb = b0 ? rd : rs;
+    if (rd == d0 && rs != rd) {
+      b_dies = b0;
+      decided = true;
+    }
+  }
+  if (!decided) {
+    if (ra == rb) {
+      s.pair[i] = -1;
+      s.pers[i] = -1.0f;
+      s.rr[2 * j] = -2;
+      return p;
+    }
+    if (ra == -1) {
+      b_dies = true;
+    } else if (rb == -1) {
+      b_dies = false;
+    } else {
+      // min side: value_greater -> r1 dies (persistence.hxx:73-80); max side: value_less (:130-137)
+      const uint64_t ka = s.ekey[ra], kb = s.ekey[rb];
+      const bool a_dies = s.maxside ? higher<DUAL>(key_value(kb), key_id(kb), key_value(ka), key_id(ka))
+                                    : higher<DUAL>(key_value(ka), key_id(ka), key_value(kb), key_id(kb));
+      b_dies = !a_dies;
+    }
   }
   const int dying = b_dies ? rb : ra, surv = b_dies ? ra : rb;
   *reinterpret_cast<int2*>(s.rr + 2 * j) = make_int2(dying, ((surv + 1) << 1) | (b_dies ? 1 : 0));
@@ -191,9 +208,8 @@ __device__ __forceinline__ void persist_apply_winner(const PersistSide& s, const
 // resumes from its roots
 __device__ __forceinline__ void persist_park(const PersistSide& s, int k, int o, const int i, const int2 r,
                                              uint32_t ps) {
-  const int surv = (r.y >> 1) - 1;
   s.list(k)[o] = i;
-  s.roots(k)[o] = (r.y & 1) ? make_int2(surv, r.x) : make_int2(r.x, surv);
+  s.roots(k)[o] = r;  // the decision it waits with
   s.keys(k)[o] = ps;
 }
 

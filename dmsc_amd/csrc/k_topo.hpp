@@ -19,8 +19,8 @@ struct PersistSide {
   float* pers;          // per saddle persistence (-1 for a cycle)
   int32_t* cancel_d;    // per saddle thresholded dead extremum (null when arcs are not traced)
   int32_t* rr;          // per worklist position: dying root (-2 = cycle), (surviving root + 1) << 1 | side bit
-  int32_t* croot;       // per position of each worklist (2 ints, list k at croot + 2 k wls): the two component
-                        // roots the saddle saw when it last had to wait (the next find resumes there)
+  int32_t* croot;       // per position of each worklist (2 ints, list k at croot + 2 k wls): the decision the
+                        // saddle waited with, in rr's format (the next find resumes there)
   uint32_t* wkey;       // per position of each worklist (list k at wkey + k wls): the saddle's sweep position
   uint32_t* rkey;       // per worklist position: the saddle's sweep position (its merge key), from the find
   int32_t* wl;          // ping-pong worklists of pending saddle indices: list k at wl + k * wls
