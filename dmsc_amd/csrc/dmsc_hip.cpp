@@ -156,7 +156,7 @@ struct ArcWork {
   int64_t arena_total = 0, ncancel = 0, out_total = 0;
   int64_t* flen = nullptr;
   int64_t* pos_pre = nullptr;
-  int4* rec = nullptr;  // per saddle arc record (32 bytes)
+  int4* rec = nullptr;  // per surviving saddle (output order): arc record, 32 bytes
   int64_t* arc_base = nullptr;  // [B+1] device
   uint64_t* pjq = nullptr;
   uint64_t* cum_r = nullptr;  // per extremum: raw / chain pieces of the links on its path to the root
@@ -835,9 +835,8 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     int64_t* per_pos = a2.get<int64_t>(hb2 + H_PERPOS, nsad);
     w.pos_pre = a2.get<int64_t>(hb2 + H_POSPRE, nsad);
     // over the surviving saddles in output order (position = output saddle index)
-    w.rec = a2.get<int4>(hb2 + H_REC, 2 * nsad);  // 32 bytes per saddle
-    launch_arc_records(nsad, w.A, w.pjq, w.rec, st);
-    launch_final_lengths(n_sad_total, surv_list[0], nullptr, w.rec, w.flen, per_pos, st);
+    w.rec = a2.get<int4>(hb2 + H_REC, 2 * nsad);  // 32 bytes per surviving saddle
+    launch_final_lengths(n_sad_total, surv_list[0], w.A, w.pjq, w.rec, w.flen, per_pos, st);
     scan_exclusive<int64_t>(per_pos, w.pos_pre, n_sad_total, c64 + 2 * s + 1, ar, A_SCAN, st);
     w.arc_base = a2.get<int64_t>(hb2 + H_ARCB, B + 1);
     bases_at<int64_t, int32_t>(w.pos_pre, row32(5), n_sad_total, c64 + 2 * s + 1, B, w.arc_base, st);
@@ -903,9 +902,8 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     // nseg[L%3], appends to nseg[(L+1)%3] and clears nseg[(L+2)%3]); nseg[3] counts raw pieces
     e.nseg = a2.get<int32_t>(hb2 + H_NSEG, 4);
     fill_i32(e.nseg, 0, 4, e.es);
-    launch_seed_segments(n_sad_total, surv_list[0], nullptr, w.A, w.rec, next, w.flen, w.pos_pre, sad_to_out,
-                         e.lists[0], e.nseg, e.raw, e.nseg + 3, e.cap, d_err, w.out_off, w.out_arc_off, w.arc_base,
-                         row32(5), B, e.es);
+    launch_seed_segments(n_sad_total, surv_list[0], w.A, w.rec, next, w.flen, w.pos_pre, e.lists[0], e.nseg, e.raw,
+                         e.nseg + 3, e.cap, d_err, w.out_off, w.out_arc_off, w.arc_base, row32(5), B, e.es);
     DMSC_CHECK(hipMemcpyAsync(pin32(s), e.nseg, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, e.es));
   }
   for (int s = 0; s < 2; ++s)

@@ -624,9 +624,9 @@ void launch_geo_q(const GeoState& g, int64_t next, uint64_t* pjq, hipStream_t st
 __device__ __forceinline__ int32_t qlen(const uint64_t* pjq, int x) { return x >= 0 ? (int32_t)(pjq[x] >> 32) : 0; }
 
 // ------------------------------------------------------------------ output arcs
-// Per saddle (id order), what the output stages read about its two arcs, in one 32-byte record:
-// targets, raw lengths, path lengths from the targets to their final roots and those roots. The
-// output stages visit saddles in sweep order; one record read replaces four scattered gathers.
+// Per surviving saddle, in output order, what the output stages read about its two arcs, in one
+// 32-byte record: targets, raw lengths, path lengths from the targets to their final roots and those
+// roots. The lengths pass gathers them once by saddle id; the seeds pass then reads them in order.
 struct alignas(32) ArcRec {
   int32_t T[2];     // arc targets (-1: no arc)
   int32_t L[2];     // raw lengths
@@ -635,52 +635,36 @@ struct alignas(32) ArcRec {
 };
 static_assert(sizeof(ArcRec) == 32, "ArcRec is 32 bytes");
 
-__global__ void k_arc_records(int64_t ns, ArcSide A, const uint64_t* __restrict__ pjq, ArcRec* __restrict__ rec) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < ns; i += (int64_t)gridDim.x * blockDim.x) {
-    const int2 T = *reinterpret_cast<const int2*>(A.tinit + 2 * i);
-    const int2 L = *reinterpret_cast<const int2*>(A.blen + 2 * i);
+__global__ void k_final_lengths(int64_t ns, const int32_t* __restrict__ ord, ArcSide A,
+                                const uint64_t* __restrict__ pjq, ArcRec* __restrict__ rec,
+                                int64_t* __restrict__ flen, int64_t* __restrict__ per_pos) {
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < ns; k += (int64_t)gridDim.x * blockDim.x) {
+    const int i = ord[k];
+    const int2 T = *reinterpret_cast<const int2*>(A.tinit + 2 * (int64_t)i);
+    const int2 L = *reinterpret_cast<const int2*>(A.blen + 2 * (int64_t)i);
+    const uint64_t p0 = T.x >= 0 ? pjq[T.x] : 0, p1 = T.y >= 0 ? pjq[T.y] : 0;
     ArcRec r;
     r.T[0] = T.x;
     r.T[1] = T.y;
     r.L[0] = L.x;
     r.L[1] = L.y;
-    const uint64_t p0 = T.x >= 0 ? pjq[T.x] : 0, p1 = T.y >= 0 ? pjq[T.y] : 0;
     r.q[0] = (int32_t)(p0 >> 32);
     r.q[1] = (int32_t)(p1 >> 32);
     r.root[0] = (int32_t)(uint32_t)p0;
     r.root[1] = (int32_t)(uint32_t)p1;
-    rec[i] = r;
+    rec[k] = r;
+    const int64_t l0 = T.x >= 0 ? (int64_t)L.x + r.q[0] : 0, l1 = T.y >= 0 ? (int64_t)L.y + r.q[1] : 0;
+    flen[2 * k] = l0;
+    flen[2 * k + 1] = l1;
+    per_pos[k] = l0 + l1;
   }
 }
-void launch_arc_records(int64_t ns, const ArcSide& a, const uint64_t* pjq, void* rec, hipStream_t st) {
-  if (ns == 0) return;
-  hipLaunchKernelGGL(k_arc_records, dim3(grid_for(ns, 256, 16384)), dim3(256), 0, st, ns, a, pjq,
-                     static_cast<ArcRec*>(rec));
-  DMSC_LAUNCH_CHECK();
-}
-
-__global__ void k_final_lengths(int64_t ns, const int32_t* __restrict__ ord, const uint8_t* __restrict__ removed,
-                                const ArcRec* __restrict__ rec, int64_t* __restrict__ flen,
-                                int64_t* __restrict__ per_pos) {
-  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < ns; k += (int64_t)gridDim.x * blockDim.x) {
-    const int i = ord[k];
-    int64_t l[2] = {0, 0};
-    if (!removed || !removed[i]) {
-      const ArcRec r = rec[i];
-      for (int kk = 0; kk < 2; ++kk)
-        if (r.T[kk] >= 0) l[kk] = (int64_t)r.L[kk] + r.q[kk];
-    }
-    flen[2 * k] = l[0];
-    flen[2 * k + 1] = l[1];
-    per_pos[k] = l[0] + l[1];
-  }
-}
-void launch_final_lengths(int64_t ns, const int32_t* ord, const uint8_t* removed, const void* rec, int64_t* flen,
-                          int64_t* per_pos, hipStream_t st) {
+void launch_final_lengths(int64_t ns, const int32_t* ord, const ArcSide& a, const uint64_t* pjq, void* rec,
+                          int64_t* flen, int64_t* per_pos, hipStream_t st) {
   KScope ks_("final_lengths", st);
   if (ns == 0) return;
-  hipLaunchKernelGGL(k_final_lengths, dim3(grid_for(ns, 256, 16384)), dim3(256), 0, st, ns, ord, removed,
-                     static_cast<const ArcRec*>(rec), flen, per_pos);
+  hipLaunchKernelGGL(k_final_lengths, dim3(grid_for(ns, 256, 16384)), dim3(256), 0, st, ns, ord, a, pjq,
+                     static_cast<ArcRec*>(rec), flen, per_pos);
   DMSC_LAUNCH_CHECK();
 }
 
@@ -691,10 +675,10 @@ __device__ __forceinline__ int32_t cum_count(const uint64_t* __restrict__ cum, i
 
 // Seeds: per surviving saddle arc, its raw arc then CHAIN(target -> final root); also the field-local
 // CSR offsets.
-__global__ __launch_bounds__(256) void k_seed_segments(int64_t ns, const int32_t* __restrict__ ord, const uint8_t* __restrict__ removed,
+__global__ __launch_bounds__(256) void k_seed_segments(int64_t ns, const int32_t* __restrict__ ord,
                                 ArcSide A, const ArcRec* __restrict__ rec, int64_t next,
                                 const int64_t* __restrict__ flen, const int64_t* __restrict__ pos_pre,
-                                const int32_t* __restrict__ sad_to_out, Seg* __restrict__ chains,
+                                Seg* __restrict__ chains,
                                 int32_t* __restrict__ nchain, Seg* __restrict__ raw, int32_t* __restrict__ nraw,
                                 int64_t cap, int32_t* __restrict__ g_err, int32_t* __restrict__ out_off,
                                 int32_t* __restrict__ out_arc_off, const int64_t* __restrict__ arc_base) {
@@ -703,27 +687,26 @@ __global__ __launch_bounds__(256) void k_seed_segments(int64_t ns, const int32_t
     bool has_raw[2] = {false, false}, has_chain[2] = {false, false};
     Seg rs[2], cs[2];
     if (kk0 < ns) {
+      // position kk0 of the survivor list is output saddle kk0
       const int i = ord[kk0];
-      if (!removed || !removed[i]) {
-        const int o = sad_to_out[i];
-        const int fi = A.sfield ? A.sfield[i] : 0;
-        const int64_t base = pos_pre[kk0];
-        const int64_t local = base - arc_base[fi];  // field-local output offset
-        out_off[o + fi] = (int32_t)local;
-        out_arc_off[o] = (int32_t)(local + flen[2 * kk0]);
-        const ArcRec r = rec[i];
+      const int fi = A.sfield ? A.sfield[i] : 0;
+      const int64_t base = pos_pre[kk0];
+      const int64_t local = base - arc_base[fi];  // field-local output offset
+      const int64_t f0 = flen[2 * kk0];
+      out_off[kk0 + fi] = (int32_t)local;
+      out_arc_off[kk0] = (int32_t)(local + f0);
+      const ArcRec r = rec[kk0];
 #pragma unroll
-        for (int kk = 0; kk < 2; ++kk) {
-          const int a = 2 * i + kk, T = r.T[kk];
-          if (T < 0) continue;
-          const int64_t off = base + (kk ? flen[2 * kk0] : 0);
-          has_raw[kk] = true;
-          rs[kk] = Seg{a, 0, seg_od(off, 1)};
-          if (r.q[kk] > 0) {
-            const uint32_t root = (uint32_t)r.root[kk];
-            has_chain[kk] = true;
-            cs[kk] = Seg{T, root == (uint32_t)next ? -1 : (int)root, seg_od(off + r.L[kk], 1)};
-          }
+      for (int kk = 0; kk < 2; ++kk) {
+        const int a = 2 * i + kk, T = r.T[kk];
+        if (T < 0) continue;
+        const int64_t off = base + (kk ? f0 : 0);
+        has_raw[kk] = true;
+        rs[kk] = Seg{a, r.L[kk], seg_od(off, 1)};
+        if (r.q[kk] > 0) {
+          const uint32_t root = (uint32_t)r.root[kk];
+          has_chain[kk] = true;
+          cs[kk] = Seg{T, root == (uint32_t)next ? -1 : (int)root, seg_od(off + r.L[kk], 1)};
         }
       }
     }
@@ -859,17 +842,17 @@ __global__ __launch_bounds__(256) void k_expand(ArcSide A, GeoState g, const uin
       if (fwd) {
         if (L.Alen > 0) out[pc++] = Seg{L.Td, y, seg_od(o, 0)};
         o += L.Alen;
-        raw[pr++] = Seg{2 * L.s + L.kd, 0, seg_od(o, 0)};
+        raw[pr++] = Seg{2 * L.s + L.kd, L.Bk, seg_od(o, 0)};
         o += L.Bk;
-        if (L.Bo > 0) raw[pr++] = Seg{2 * L.s + L.ko, 0, seg_od(o, 1)};
+        if (L.Bo > 0) raw[pr++] = Seg{2 * L.s + L.ko, L.Bo, seg_od(o, 1)};
         o += L.Bo;
         if (L.Clen > 0) out[pc++] = Seg{L.To, L.Ro, seg_od(o, 1)};
       } else {
         if (L.Clen > 0) out[pc++] = Seg{L.To, L.Ro, seg_od(o, 0)};
         o += L.Clen;
-        if (L.Bo > 0) raw[pr++] = Seg{2 * L.s + L.ko, 0, seg_od(o, 0)};
+        if (L.Bo > 0) raw[pr++] = Seg{2 * L.s + L.ko, L.Bo, seg_od(o, 0)};
         o += L.Bo;
-        raw[pr++] = Seg{2 * L.s + L.kd, 0, seg_od(o, 1)};
+        raw[pr++] = Seg{2 * L.s + L.kd, L.Bk, seg_od(o, 1)};
         o += L.Bk;
         if (L.Alen > 0) out[pc++] = Seg{L.Td, y, seg_od(o, 1)};
       }
@@ -903,7 +886,7 @@ __global__ __launch_bounds__(256) void k_copy_raw(ArcSide A, const Seg* __restri
     int fwd = 1;
     if (jj < n) {
       const Seg sg = raw[jj];
-      len = A.blen[sg.a];
+      len = sg.b;  // raw pieces carry their length
       dst = sg.od >> 1;
       fwd = (int)(sg.od & 1);
       src = A.boff[sg.a];
@@ -941,16 +924,14 @@ void launch_copy_raw(const ArcSide& a, const Seg* raw, const int32_t* nraw, int6
   DMSC_LAUNCH_CHECK();
 }
 
-void launch_seed_segments(int64_t ns, const int32_t* ord, const uint8_t* removed, const ArcSide& a,
-                          const void* rec, int64_t next, const int64_t* flen, const int64_t* pos_pre,
-                          const int32_t* sad_to_out, Seg* chains, int32_t* nchain, Seg* raw, int32_t* nraw, int64_t cap,
+void launch_seed_segments(int64_t ns, const int32_t* ord, const ArcSide& a, const void* rec, int64_t next,
+                          const int64_t* flen, const int64_t* pos_pre, Seg* chains, int32_t* nchain, Seg* raw, int32_t* nraw, int64_t cap,
                           int32_t* err, int32_t* out_offsets, int32_t* out_arc_offsets, const int64_t* arc_base,
                           const int32_t* sad_out, int B, hipStream_t st) {
   KScope ks_("seed_segments", st);
   if (ns > 0) {
-    hipLaunchKernelGGL(k_seed_segments, dim3(grid_for(ns, 256, 16384)), dim3(256), 0, st, ns, ord, removed, a,
-                       static_cast<const ArcRec*>(rec), next,
-                       flen, pos_pre, sad_to_out, chains, nchain, raw, nraw, cap, err, out_offsets, out_arc_offsets,
+    hipLaunchKernelGGL(k_seed_segments, dim3(grid_for(ns, 256, 16384)), dim3(256), 0, st, ns, ord, a,
+                       static_cast<const ArcRec*>(rec), next, flen, pos_pre, chains, nchain, raw, nraw, cap, err, out_offsets, out_arc_offsets,
                        arc_base);
     DMSC_LAUNCH_CHECK();
   }

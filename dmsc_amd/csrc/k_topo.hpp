@@ -146,14 +146,13 @@ void launch_cancel_finish(int64_t nc, const int32_t* slot, const int32_t* cd, in
                           hipStream_t st);
 void launch_geo_q(const GeoState& g, int64_t next, uint64_t* pjq, hipStream_t st);
 
-// Output arc lengths for surviving saddles in min order; then segment expansion.
-// per saddle (id order) one 32-byte record of its arcs' targets, raw lengths, path lengths to the final roots
-// and those roots (rec: 32 * ns bytes)
-void launch_arc_records(int64_t ns, const ArcSide& a, const uint64_t* pjq, void* rec, hipStream_t st);
-void launch_final_lengths(int64_t ns, const int32_t* ord, const uint8_t* removed, const void* rec, int64_t* flen,
-                          int64_t* per_pos, hipStream_t st);
-// A piece of a merged arc: a raw arc (a = arc index) or a chain of links (from extremum a up to its
-// ancestor b, -1 = the void). od = output offset << 1 | forward bit.
+// Output arc lengths for surviving saddles in min order (ord: survivor list, position = output saddle);
+// rec receives one 32-byte record per position (targets, raw lengths, path lengths to the final roots,
+// those roots) for the seeds pass. Then segment expansion.
+void launch_final_lengths(int64_t ns, const int32_t* ord, const ArcSide& a, const uint64_t* pjq, void* rec,
+                          int64_t* flen, int64_t* per_pos, hipStream_t st);
+// A piece of a merged arc: a raw arc (a = arc index, b = its length) or a chain of links (from
+// extremum a up to its ancestor b, -1 = the void). od = output offset << 1 | forward bit.
 struct Seg {
   int32_t a, b;
   int64_t od;
@@ -165,9 +164,8 @@ void launch_link_counts(const ArcSide& a, const GeoState& g, const uint64_t* pjq
 // arc_base: [B+1] first output cell of each field; sad_out: [B+1] first output saddle of each field.
 // A field's offsets array has one more entry than its saddles, so field f's entries start at
 // sad_out[f] + f in the concatenated offsets output.
-void launch_seed_segments(int64_t ns, const int32_t* ord, const uint8_t* removed, const ArcSide& a,
-                          const void* rec, int64_t next, const int64_t* flen, const int64_t* pos_pre,
-                          const int32_t* sad_to_out, Seg* chains, int32_t* nchain, Seg* raw, int32_t* nraw, int64_t cap,
+void launch_seed_segments(int64_t ns, const int32_t* ord, const ArcSide& a, const void* rec, int64_t next,
+                          const int64_t* flen, const int64_t* pos_pre, Seg* chains, int32_t* nchain, Seg* raw, int32_t* nraw, int64_t cap,
                           int32_t* err, int32_t* out_offsets, int32_t* out_arc_offsets, const int64_t* arc_base,
                           const int32_t* sad_out, int B, hipStream_t st);
 // One level of expansion: every chain becomes the raw pieces and sub-chains of its links.
