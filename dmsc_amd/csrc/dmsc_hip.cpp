@@ -650,11 +650,12 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     // first output extremum of each field (sampled before dead extrema are masked)
     bases_at<int32_t, int32_t>(sb[s].to_out, row32(s ? 2 : 0), next, c32 + 3 + s, B, row32(3 + s), st);
     launch_mask_prefix(sb[s].alive, sb[s].to_out, next, st);
+    launch_root_out(sb[s].root, sb[s].to_out, next, st);  // root[] holds root output indices from here on
   }
   int32_t* surv_mx = ar.get<int32_t>(A_ASM0 + 8, nsad);
   int32_t* surv_mx_pre = ar.get<int32_t>(A_ASM0 + 9, nsad);
-  launch_side_counts(nsad, ord_min, removed, cmin, sb[0].root, sb[0].to_out, surv, rows[0], st);
-  launch_side_counts(nsad, ord_max, removed, cmax, sb[1].root, sb[1].to_out, surv_mx, rows[1], st);
+  launch_side_counts(nsad, ord_min, removed, cmin, sb[0].root, surv, rows[0], st);
+  launch_side_counts(nsad, ord_max, removed, cmax, sb[1].root, surv_mx, rows[1], st);
   scan_exclusive<int32_t>(surv, surv_pre, nsad, c32 + 5, ar, A_SCAN, st);
   scan_exclusive<int32_t>(surv_mx, surv_mx_pre, nsad, c32 + 10, ar, A_SCAN, st);
   scan_exclusive<int32_t>(rows[0], rows_pre[0], nsad, c32 + 6, ar, A_SCAN, st);
@@ -714,12 +715,12 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     launch_emit_points(nmin, min_ids, sb[0].to_out, o_pts[0], es);
     launch_emit_points(nmax, max_ids, sb[1].to_out, o_pts[1], es);
     const EmitBases eb[2] = {{sad_field, row32(5), row32(3)}, {sad_field, row32(5), row32(4)}};
-    launch_emit_side(n_sad_total, surv_list[0], nullptr, sad_to_out, cmin, sb[0].root, sb[0].to_out, sb[0].pers,
+    launch_emit_side(n_sad_total, surv_list[0], nullptr, nullptr, cmin, sb[0].root, sb[0].pers,
                      sb[0].pair, surv_rows[0], sad_ids, eb[0], o_p[0], o_pp[0], o_e[0], o_sad, es);
-    launch_emit_side(n_sad_total, surv_list[1], nullptr, sad_to_out, cmax, sb[1].root, sb[1].to_out, sb[1].pers,
+    launch_emit_side(n_sad_total, surv_list[1], nullptr, sad_to_out, cmax, sb[1].root, sb[1].pers,
                      sb[1].pair, surv_rows[1], sad_ids, eb[1], o_p[1], o_pp[1], o_e[1], nullptr, es);
-    if (gmax) launch_labels_faces(rv, S, pjf, sb[1].root, sb[1].to_out, row32(4), o_flab, es);
-    if (gmin) launch_labels_verts(rv, N, pjv, sb[0].root, sb[0].to_out, row32(3), o_vlab, es);
+    if (gmax) launch_labels_faces(rv, S, pjf, sb[1].root, row32(4), o_flab, es);
+    if (gmin) launch_labels_verts(rv, N, pjv, sb[0].root, row32(3), o_vlab, es);
   }
 
   for (int s = 0; s < 2; ++s) {

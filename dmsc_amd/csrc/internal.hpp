@@ -365,11 +365,12 @@ void launch_compact_lists(bool dual, const RankView& rv, const int32_t* list_b, 
 void launch_saddle_records(const RankView& rv, const int32_t* sad_ids, const int32_t* sad_field, int64_t nsad,
                            const uint64_t* pj_face, const uint64_t* pj_vert, int32_t* cmax, int32_t* cmin,
                            int32_t* lmax, int32_t* lmin, hipStream_t st);
-// labels: out_base[f] = output index of field f's first extremum of that side
-void launch_labels_faces(const RankView& rv, int64_t S, const uint64_t* pj_face, const int32_t* root_t,
-                         const int32_t* to_out, const int32_t* out_base, int32_t* out, hipStream_t st);
-void launch_labels_verts(const RankView& rv, int64_t N, const uint64_t* pj_vert, const int32_t* root_t,
-                         const int32_t* to_out, const int32_t* out_base, int32_t* out, hipStream_t st);
+// labels: root_out[x] = output index of extremum x's thresholded root (-1: none); out_base[f] = output
+// index of field f's first extremum of that side
+void launch_labels_faces(const RankView& rv, int64_t S, const uint64_t* pj_face, const int32_t* root_out,
+                         const int32_t* out_base, int32_t* out, hipStream_t st);
+void launch_labels_verts(const RankView& rv, int64_t N, const uint64_t* pj_vert, const int32_t* root_out,
+                         const int32_t* out_base, int32_t* out, hipStream_t st);
 // persistence pairs of one field (output slots: saddles, min / max persistence, paired extremum,
 // extremum cell ids) for the backward of persistence losses (k_diff.hip)
 struct PairView {

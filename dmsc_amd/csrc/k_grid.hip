@@ -418,40 +418,37 @@ void launch_saddle_records(const RankView& rv, const int32_t* sad_ids, const int
 // label(cell) = field-local output index of the thresholded-UF root of its V-path terminal, -1 for
 // the void or a cancelled root (reference cell_groups.hxx:96-229). The output is the concatenation
 // of the fields' label arrays (slot / pixel order).
-__device__ __forceinline__ int32_t label_of(int r, const int32_t* __restrict__ to_out, int32_t base) {
-  if (r < 0) return -1;
-  const int32_t o = to_out[r];
-  return o >= 0 ? o - base : -1;
-}
-__global__ void k_labels_faces(int64_t S, RankView rv, const uint64_t* __restrict__ pj, const int32_t* __restrict__ root_t,
-                               const int32_t* __restrict__ to_out, const int32_t* __restrict__ out_base,
+// o: output index of the extremum's thresholded root (-1: none)
+__device__ __forceinline__ int32_t label_of(int32_t o, int32_t base) { return o >= 0 ? o - base : -1; }
+__global__ void k_labels_faces(int64_t S, RankView rv, const uint64_t* __restrict__ pj,
+                               const int32_t* __restrict__ root_out, const int32_t* __restrict__ out_base,
                                int32_t* __restrict__ out) {
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < S; q += (int64_t)gridDim.x * blockDim.x) {
     const int fi = field_of(rv.bt.slot_b, rv.bt.B, q);
-    out[q] = label_of(root_t[rank_max_slot(rv, fi, (int)(uint32_t)pj[q])], to_out, out_base[fi]);
+    out[q] = label_of(root_out[rank_max_slot(rv, fi, (int)(uint32_t)pj[q])], out_base[fi]);
   }
 }
-__global__ void k_labels_verts(int64_t N, RankView rv, const uint64_t* __restrict__ pj, const int32_t* __restrict__ root_t,
-                               const int32_t* __restrict__ to_out, const int32_t* __restrict__ out_base,
+__global__ void k_labels_verts(int64_t N, RankView rv, const uint64_t* __restrict__ pj,
+                               const int32_t* __restrict__ root_out, const int32_t* __restrict__ out_base,
                                int32_t* __restrict__ out) {
   for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < N; p += (int64_t)gridDim.x * blockDim.x) {
     const int fi = field_of(rv.bt.pix_b, rv.bt.B, p);
-    out[p] = label_of(root_t[rank_min_pixel(rv, fi, (int)(uint32_t)pj[p])], to_out, out_base[fi]);
+    out[p] = label_of(root_out[rank_min_pixel(rv, fi, (int)(uint32_t)pj[p])], out_base[fi]);
   }
 }
-void launch_labels_faces(const RankView& rv, int64_t S, const uint64_t* pj_face, const int32_t* root_t,
-                         const int32_t* to_out, const int32_t* out_base, int32_t* out, hipStream_t st) {
+void launch_labels_faces(const RankView& rv, int64_t S, const uint64_t* pj_face, const int32_t* root_out,
+                         const int32_t* out_base, int32_t* out, hipStream_t st) {
   KScope ks_("labels_faces", st);
   if (S == 0) return;
-  hipLaunchKernelGGL(k_labels_faces, dim3(grid_for(S, 256, 16384)), dim3(256), 0, st, S, rv, pj_face, root_t, to_out,
+  hipLaunchKernelGGL(k_labels_faces, dim3(grid_for(S, 256, 16384)), dim3(256), 0, st, S, rv, pj_face, root_out,
                      out_base, out);
   DMSC_LAUNCH_CHECK();
 }
-void launch_labels_verts(const RankView& rv, int64_t N, const uint64_t* pj_vert, const int32_t* root_t,
-                         const int32_t* to_out, const int32_t* out_base, int32_t* out, hipStream_t st) {
+void launch_labels_verts(const RankView& rv, int64_t N, const uint64_t* pj_vert, const int32_t* root_out,
+                         const int32_t* out_base, int32_t* out, hipStream_t st) {
   KScope ks_("labels_verts", st);
   if (N == 0) return;
-  hipLaunchKernelGGL(k_labels_verts, dim3(grid_for(N, 256, 16384)), dim3(256), 0, st, N, rv, pj_vert, root_t, to_out,
+  hipLaunchKernelGGL(k_labels_verts, dim3(grid_for(N, 256, 16384)), dim3(256), 0, st, N, rv, pj_vert, root_out,
                      out_base, out);
   DMSC_LAUNCH_CHECK();
 }

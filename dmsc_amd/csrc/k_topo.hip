@@ -533,10 +533,23 @@ void launch_mask_prefix(const uint8_t* alive, int32_t* pre, int64_t n, hipStream
 // Per order position: survivor flag (min order) and edge-row count of one side.
 // Per position of the side's order (sequential writes; the per-saddle reads are gathers: visiting
 // saddles in id order and scattering the two counts measured slower except on heavy-tie noise).
+// root_out[x] <- output index of x's thresholded root (-1: no root or the root is not output). Output
+// indices of live extrema are distinct, so two extrema share a root output exactly when they share a
+// live root; every consumer below compares these instead of the roots.
+__global__ void k_root_out(int32_t* __restrict__ root, const int32_t* __restrict__ to_out, int64_t n) {
+  for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < n; x += (int64_t)gridDim.x * blockDim.x) {
+    const int r = root[x];
+    root[x] = r >= 0 ? to_out[r] : -1;
+  }
+}
+void launch_root_out(int32_t* root, const int32_t* to_out, int64_t n, hipStream_t st) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_root_out, dim3(grid_for(n, 256, 16384)), dim3(256), 0, st, root, to_out, n);
+  DMSC_LAUNCH_CHECK();
+}
 __global__ void k_side_counts(int64_t ns, const int32_t* __restrict__ ord, const uint8_t* __restrict__ removed,
-                              const int32_t* __restrict__ c, const int32_t* __restrict__ root_t,
-                              const int32_t* __restrict__ to_out, int32_t* __restrict__ surv,
-                              int32_t* __restrict__ rows) {
+                              const int32_t* __restrict__ c, const int32_t* __restrict__ root_out,
+                              int32_t* __restrict__ surv, int32_t* __restrict__ rows) {
   for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < ns; k += (int64_t)gridDim.x * blockDim.x) {
     const int i = ord[k];
     const bool alive = !removed[i];
@@ -544,18 +557,18 @@ __global__ void k_side_counts(int64_t ns, const int32_t* __restrict__ ord, const
     int n = 0;
     if (alive) {
       const int2 ab = *reinterpret_cast<const int2*>(c + 2 * i);
-      const int r1 = ab.x >= 0 ? root_t[ab.x] : -1, r2 = ab.y >= 0 ? root_t[ab.y] : -1;
-      if (r1 != -1 && to_out[r1] != -1) ++n;
-      if (r1 != r2 && r2 != -1 && to_out[r2] != -1) ++n;
+      const int o1 = ab.x >= 0 ? root_out[ab.x] : -1, o2 = ab.y >= 0 ? root_out[ab.y] : -1;
+      if (o1 >= 0) ++n;
+      if (o2 >= 0 && o2 != o1) ++n;
     }
     rows[k] = n;
   }
 }
-void launch_side_counts(int64_t ns, const int32_t* ord, const uint8_t* removed, const int32_t* c, const int32_t* root_t,
-                        const int32_t* to_out, int32_t* surv, int32_t* rows, hipStream_t st) {
+void launch_side_counts(int64_t ns, const int32_t* ord, const uint8_t* removed, const int32_t* c,
+                        const int32_t* root_out, int32_t* surv, int32_t* rows, hipStream_t st) {
   if (ns == 0) return;
-  hipLaunchKernelGGL(k_side_counts, dim3(grid_for(ns, 256, 16384)), dim3(256), 0, st, ns, ord, removed, c, root_t,
-                     to_out, surv, rows);
+  hipLaunchKernelGGL(k_side_counts, dim3(grid_for(ns, 256, 16384)), dim3(256), 0, st, ns, ord, removed, c, root_out,
+                     surv, rows);
   DMSC_LAUNCH_CHECK();
 }
 
@@ -603,8 +616,8 @@ void launch_emit_points(int64_t n, const int32_t* ids, const int32_t* to_out, in
 // the thresholded root), and edge rows (saddle output index, extremum output index).
 __global__ void k_emit_side(int64_t ns, const int32_t* __restrict__ ord, const uint8_t* __restrict__ removed,
                             const int32_t* __restrict__ sad_to_out, const int32_t* __restrict__ c,
-                            const int32_t* __restrict__ root_t, const int32_t* __restrict__ to_out,
-                            const float* __restrict__ pers, const int32_t* __restrict__ pair,
+                            const int32_t* __restrict__ root_out, const float* __restrict__ pers,
+                            const int32_t* __restrict__ pair,
                             const int32_t* __restrict__ row_pre, const int32_t* __restrict__ sad_ids, EmitBases eb,
                             float* __restrict__ out_p, int32_t* __restrict__ out_pp, int32_t* __restrict__ out_e,
                             int32_t* __restrict__ out_sad) {
@@ -617,38 +630,34 @@ __global__ void k_emit_side(int64_t ns, const int32_t* __restrict__ ord, const u
       sbase = eb.sad_out[fi];
       ebase = eb.ext_out[fi];
     }
-    const int so = sad_to_out[i];
+    const int so = sad_to_out ? sad_to_out[i] : (int)k;  // no map: ord is the output order
     if (out_sad) out_sad[so] = sad_ids[i];
     out_p[so] = pers[i];
     const int pe = pair[i];
-    int pp = -1;
-    if (pe >= 0) {
-      const int r = root_t[pe];
-      pp = (r >= 0 && to_out[r] >= 0) ? to_out[r] - ebase : -1;
-    }
-    out_pp[so] = pp;
-    const int a = c[2 * i], b = c[2 * i + 1];
-    const int r1 = a >= 0 ? root_t[a] : -1, r2 = b >= 0 ? root_t[b] : -1;
+    const int po = pe >= 0 ? root_out[pe] : -1;
+    out_pp[so] = po >= 0 ? po - ebase : -1;
+    const int2 ab = *reinterpret_cast<const int2*>(c + 2 * (int64_t)i);
+    const int o1 = ab.x >= 0 ? root_out[ab.x] : -1, o2 = ab.y >= 0 ? root_out[ab.y] : -1;
     int row = row_pre[k];
-    if (r1 != -1 && to_out[r1] != -1) {
+    if (o1 >= 0) {
       out_e[2 * (int64_t)row] = so - sbase;
-      out_e[2 * (int64_t)row + 1] = to_out[r1] - ebase;
+      out_e[2 * (int64_t)row + 1] = o1 - ebase;
       ++row;
     }
-    if (r1 != r2 && r2 != -1 && to_out[r2] != -1) {
+    if (o2 >= 0 && o2 != o1) {
       out_e[2 * (int64_t)row] = so - sbase;
-      out_e[2 * (int64_t)row + 1] = to_out[r2] - ebase;
+      out_e[2 * (int64_t)row + 1] = o2 - ebase;
     }
   }
 }
 void launch_emit_side(int64_t ns, const int32_t* ord, const uint8_t* removed, const int32_t* sad_to_out, const int32_t* c,
-                      const int32_t* root_t, const int32_t* to_out, const float* pers, const int32_t* pair,
+                      const int32_t* root_out, const float* pers, const int32_t* pair,
                       const int32_t* row_pre, const int32_t* sad_ids, const EmitBases& eb, float* out_p, int32_t* out_pp,
                       int32_t* out_e, int32_t* out_sad, hipStream_t st) {
   KScope ks_("emit_side", st);
   if (ns == 0) return;
   hipLaunchKernelGGL(k_emit_side, dim3(grid_for(ns, 256, 16384)), dim3(256), 0, st, ns, ord, removed, sad_to_out, c,
-                     root_t, to_out, pers, pair, row_pre, sad_ids, eb, out_p, out_pp, out_e, out_sad);
+                     root_out, pers, pair, row_pre, sad_ids, eb, out_p, out_pp, out_e, out_sad);
   DMSC_LAUNCH_CHECK();
 }
 
