@@ -37,25 +37,103 @@ __device__ __forceinline__ T block_exclusive_scan(T v, T* lds4, T& total) {
   return base + inc - v;
 }
 
+// Scan inputs. Each thread of a tile owns SCAN_ITEMS consecutive elements; load8 fetches them with
+// 16-byte loads when the source is 16-byte aligned and the run is whole (vec is set by the host).
+static bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
+
 template <class T>
 struct LoadArr {
   const T* p;
+  bool vec;
   __device__ T operator()(int64_t i) const { return p[i]; }
+  __device__ void load8(int64_t base, int64_t n, T (&v)[8]) const {
+    if (vec && base + 8 <= n) {
+      if constexpr (sizeof(T) == 4) {
+        const int4 a = *reinterpret_cast<const int4*>(p + base), b = *reinterpret_cast<const int4*>(p + base + 4);
+        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+      } else {
+#pragma unroll
+        for (int j = 0; j < 8; j += 2) {
+          const longlong2 a = *reinterpret_cast<const longlong2*>(p + base + j);
+          v[j] = a.x;
+          v[j + 1] = a.y;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) v[j] = base + j < n ? p[base + j] : (T)0;
+    }
+  }
 };
 struct LoadPop {
   const uint32_t* a;
   const uint32_t* b;
+  bool vec;
   __device__ int32_t operator()(int64_t i) const { return __popc(a[i]) + (b ? __popc(b[i]) : 0); }
+  __device__ void load8(int64_t base, int64_t n, int32_t (&v)[8]) const {
+    if (vec && base + 8 <= n) {
+      const uint4 x = *reinterpret_cast<const uint4*>(a + base), y = *reinterpret_cast<const uint4*>(a + base + 4);
+      v[0] = __popc(x.x); v[1] = __popc(x.y); v[2] = __popc(x.z); v[3] = __popc(x.w);
+      v[4] = __popc(y.x); v[5] = __popc(y.y); v[6] = __popc(y.z); v[7] = __popc(y.w);
+      if (b) {
+        const uint4 z = *reinterpret_cast<const uint4*>(b + base), w = *reinterpret_cast<const uint4*>(b + base + 4);
+        v[0] += __popc(z.x); v[1] += __popc(z.y); v[2] += __popc(z.z); v[3] += __popc(z.w);
+        v[4] += __popc(w.x); v[5] += __popc(w.y); v[6] += __popc(w.z); v[7] += __popc(w.w);
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) v[j] = base + j < n ? (*this)(base + j) : 0;
+    }
+  }
 };
+struct LoadWiden {
+  const int32_t* p;
+  bool vec;
+  __device__ int64_t operator()(int64_t i) const { return (int64_t)p[i]; }
+  __device__ void load8(int64_t base, int64_t n, int64_t (&v)[8]) const {
+    if (vec && base + 8 <= n) {
+      const int4 a = *reinterpret_cast<const int4*>(p + base), b = *reinterpret_cast<const int4*>(p + base + 4);
+      v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+    } else {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) v[j] = base + j < n ? (int64_t)p[base + j] : 0;
+    }
+  }
+};
+static_assert(SCAN_ITEMS == 8, "load8 / store8 move 8 elements per thread");
+
+// out[base + j] = v[j] for the thread's run, with 16-byte stores when allowed
+template <class T>
+__device__ __forceinline__ void store8(T* __restrict__ out, int64_t base, int64_t n, const T (&v)[8], bool vec) {
+  if (vec && base + 8 <= n) {
+    if constexpr (sizeof(T) == 4) {
+      *reinterpret_cast<int4*>(out + base) = make_int4(v[0], v[1], v[2], v[3]);
+      *reinterpret_cast<int4*>(out + base + 4) = make_int4(v[4], v[5], v[6], v[7]);
+    } else {
+#pragma unroll
+      for (int j = 0; j < 8; j += 2) {
+        longlong2 a;
+        a.x = v[j];
+        a.y = v[j + 1];
+        *reinterpret_cast<longlong2*>(out + base + j) = a;
+      }
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      if (base + j < n) out[base + j] = v[j];
+  }
+}
 
 template <class T, class Load>
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(Load ld, int64_t n, T* __restrict__ bsums) {
   __shared__ T lds4[4];
   const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+  T v[SCAN_ITEMS];
+  ld.load8(base, n, v);
   T s = 0;
 #pragma unroll
-  for (int j = 0; j < SCAN_ITEMS; ++j)
-    if (base + j < n) s += ld(base + j);
+  for (int j = 0; j < SCAN_ITEMS; ++j) s += v[j];
   T tot;
   block_exclusive_scan(s, lds4, tot);
   if (threadIdx.x == 0) bsums[blockIdx.x] = tot;
@@ -89,23 +167,23 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_sums(T* __restrict__ bsum
 
 template <class T, class Load>
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(Load ld, int64_t n, const T* __restrict__ bsums,
-                                                            T* __restrict__ out) {
+                                                            T* __restrict__ out, bool vec_out) {
   __shared__ T lds4[4];
   const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
   T v[SCAN_ITEMS];
+  ld.load8(base, n, v);
   T s = 0;
 #pragma unroll
-  for (int j = 0; j < SCAN_ITEMS; ++j) {
-    v[j] = (base + j < n) ? ld(base + j) : (T)0;
-    s += v[j];
-  }
+  for (int j = 0; j < SCAN_ITEMS; ++j) s += v[j];
   T tot;
   T ex = block_exclusive_scan(s, lds4, tot) + bsums[blockIdx.x];
 #pragma unroll
   for (int j = 0; j < SCAN_ITEMS; ++j) {
-    if (base + j < n) out[base + j] = ex;
-    ex += v[j];
+    const T x = v[j];
+    v[j] = ex;
+    ex += x;
   }
+  store8(out, base, n, v, vec_out);
 }
 
 // Down-sweep that adds up the block sums before its own tile itself (no separate scan of the block
@@ -113,7 +191,8 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(Load ld, int64_t n, 
 // launch); the last block writes the grand total.
 template <class T, class Load>
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_down2(Load ld, int64_t n, const T* __restrict__ bsums,
-                                                             T* __restrict__ out, T* __restrict__ total) {
+                                                             T* __restrict__ out, T* __restrict__ total,
+                                                             bool vec_out) {
   __shared__ T lds4[4];
   T pre = 0;
   for (int64_t b = threadIdx.x; b < blockIdx.x; b += SCAN_THREADS) pre += bsums[b];
@@ -121,19 +200,19 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_down2(Load ld, int64_t n,
   block_exclusive_scan(pre, lds4, base_sum);
   const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
   T v[SCAN_ITEMS];
+  ld.load8(base, n, v);
   T s = 0;
 #pragma unroll
-  for (int j = 0; j < SCAN_ITEMS; ++j) {
-    v[j] = (base + j < n) ? ld(base + j) : (T)0;
-    s += v[j];
-  }
+  for (int j = 0; j < SCAN_ITEMS; ++j) s += v[j];
   T tot;
   T ex = block_exclusive_scan(s, lds4, tot) + base_sum;
 #pragma unroll
   for (int j = 0; j < SCAN_ITEMS; ++j) {
-    if (base + j < n) out[base + j] = ex;
-    ex += v[j];
+    const T x = v[j];
+    v[j] = ex;
+    ex += x;
   }
+  store8(out, base, n, v, vec_out);
   if (total && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *total = base_sum + tot;
 }
 constexpr int64_t kScanFewTiles = 4096;  // up to this many tiles: reduce + down2 (two launches)
@@ -149,7 +228,7 @@ static void scan_impl(Load ld, T* out, int64_t n, T* d_total, Arena& ar, int slo
       DMSC_LAUNCH_CHECK();
     }
     hipLaunchKernelGGL((k_scan_down2<T, Load>), dim3((unsigned)std::max<int64_t>(nb, 1)), dim3(SCAN_THREADS), 0, st, ld,
-                       n, bsums, out, d_total);
+                       n, bsums, out, d_total, aligned16(out));
     DMSC_LAUNCH_CHECK();
     return;
   }
@@ -160,29 +239,27 @@ static void scan_impl(Load ld, T* out, int64_t n, T* d_total, Arena& ar, int slo
   hipLaunchKernelGGL(k_scan_sums<T>, dim3(1), dim3(SCAN_THREADS), 0, st, bsums, nb, d_total);
   DMSC_LAUNCH_CHECK();
   if (n > 0) {
-    hipLaunchKernelGGL((k_scan_down<T, Load>), dim3((unsigned)nb), dim3(SCAN_THREADS), 0, st, ld, n, bsums, out);
+    hipLaunchKernelGGL((k_scan_down<T, Load>), dim3((unsigned)nb), dim3(SCAN_THREADS), 0, st, ld, n, bsums, out,
+                       aligned16(out));
     DMSC_LAUNCH_CHECK();
   }
 }
 
 template <class T>
 void scan_exclusive(const T* in, T* out, int64_t n, T* d_total, Arena& ar, int slot, hipStream_t st) {
-  scan_impl<T>(LoadArr<T>{in}, out, n, d_total, ar, slot, st);
+  scan_impl<T>(LoadArr<T>{in, aligned16(in)}, out, n, d_total, ar, slot, st);
 }
 template void scan_exclusive<int32_t>(const int32_t*, int32_t*, int64_t, int32_t*, Arena&, int, hipStream_t);
 template void scan_exclusive<int64_t>(const int64_t*, int64_t*, int64_t, int64_t*, Arena&, int, hipStream_t);
 
 void scan_plane_popcounts(const uint32_t* plane_a, const uint32_t* plane_b, int32_t* out, int64_t n, int32_t* d_total,
                           Arena& ar, int slot, hipStream_t st) {
-  scan_impl<int32_t>(LoadPop{plane_a, plane_b}, out, n, d_total, ar, slot, st);
+  scan_impl<int32_t>(LoadPop{plane_a, plane_b, aligned16(plane_a) && (!plane_b || aligned16(plane_b))}, out, n,
+                     d_total, ar, slot, st);
 }
 
-struct LoadWiden {
-  const int32_t* p;
-  __device__ int64_t operator()(int64_t i) const { return (int64_t)p[i]; }
-};
 void scan_i32_to_i64(const int32_t* in, int64_t* out, int64_t n, int64_t* d_total, Arena& ar, int slot, hipStream_t st) {
-  scan_impl<int64_t>(LoadWiden{in}, out, n, d_total, ar, slot, st);
+  scan_impl<int64_t>(LoadWiden{in, aligned16(in)}, out, n, d_total, ar, slot, st);
 }
 
 // ------------------------------------------------------------------ radix sort
