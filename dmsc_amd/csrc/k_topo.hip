@@ -240,12 +240,32 @@ __device__ __forceinline__ void persist_apply_chunk(const PersistSide& s, int cu
   int2 r[ITEMS];
   uint64_t mk[ITEMS];
   uint32_t ps[ITEMS];
+  if (ITEMS == 4 && j0 + 4 <= n) {
+    // a whole run: 16-byte loads (rr and rkey are indexed by list position, so j0 % 4 == 0 aligns
+    // them; the list itself sits at an offset of the worklist buffer)
+    const int4 kv = (reinterpret_cast<uintptr_t>(in + j0) & 15) == 0
+                        ? *reinterpret_cast<const int4*>(in + j0)
+                        : make_int4(in[j0], in[j0 + 1], in[j0 + 2], in[j0 + 3]);
+    const int4 ra = *reinterpret_cast<const int4*>(s.rr + 2 * j0);
+    const int4 rb = *reinterpret_cast<const int4*>(s.rr + 2 * j0 + 4);
+    const uint4 pv = *reinterpret_cast<const uint4*>(s.rkey + j0);
+    const int kk[4] = {kv.x, kv.y, kv.z, kv.w};
+    const int2 rv[4] = {make_int2(ra.x, ra.y), make_int2(ra.z, ra.w), make_int2(rb.x, rb.y), make_int2(rb.z, rb.w)};
+    const uint32_t pp[4] = {pv.x, pv.y, pv.z, pv.w};
 #pragma unroll
-  for (int u = 0; u < ITEMS; ++u) {
-    const bool v = j0 + u < n;
-    k[u] = v ? in[j0 + u] : 0;
-    r[u] = v ? *reinterpret_cast<const int2*>(s.rr + 2 * (j0 + u)) : make_int2(-3, 0);  // -3: past the list
-    ps[u] = v ? s.rkey[j0 + u] : 0u;
+    for (int u = 0; u < ITEMS; ++u) {
+      k[u] = kk[u & 3];
+      r[u] = rv[u & 3];
+      ps[u] = pp[u & 3];
+    }
+  } else {
+#pragma unroll
+    for (int u = 0; u < ITEMS; ++u) {
+      const bool v = j0 + u < n;
+      k[u] = v ? in[j0 + u] : 0;
+      r[u] = v ? *reinterpret_cast<const int2*>(s.rr + 2 * (j0 + u)) : make_int2(-3, 0);  // -3: past the list
+      ps[u] = v ? s.rkey[j0 + u] : 0u;
+    }
   }
 #pragma unroll
   for (int u = 0; u < ITEMS; ++u) {
