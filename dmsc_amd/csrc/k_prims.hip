@@ -291,62 +291,86 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const void* __restrict__
   counts[(int64_t)threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
 }
 
-// PIN / POUT: input / output packed (one uint64 array) or separate (key array, value array)
+// PIN / POUT: input / output packed (one uint64 array) or separate (key array, value array).
+// A tile is RS_ITEMS chunks of 256 elements in order; chunks are ranked RS_GROUP at a time: each wave
+// ranks its 64 elements of each chunk (peer masks from 8 ballots) and records per-digit counts per
+// (chunk, wave); one thread per digit then turns the counts into output offsets in element order;
+// four barriers per group of chunks.
+constexpr int RS_GROUP = 4;
+static_assert(RS_ITEMS % RS_GROUP == 0, "whole groups of chunks per tile");
 template <bool PIN, bool POUT>
 __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const void* __restrict__ kin, const int32_t* __restrict__ vin,
                                                            void* __restrict__ kout, int32_t* __restrict__ vout,
                                                            int64_t n, int shift, const int32_t* __restrict__ offs,
                                                            int64_t nb) {
-  __shared__ int32_t base_off[256];
-  __shared__ int32_t running[256];
-  __shared__ int32_t wcount[4][256];
+  __shared__ int32_t next_off[256];                  // per digit: output offset of the next element
+  __shared__ int32_t slot[RS_GROUP][4][256];         // per (chunk, wave, digit): count, then offset
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  base_off[tid] = offs[(int64_t)tid * nb + blockIdx.x];
-  running[tid] = 0;
+  next_off[tid] = offs[(int64_t)tid * nb + blockIdx.x];
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const int64_t base = (int64_t)blockIdx.x * RS_TILE;
-  for (int j = 0; j < RS_ITEMS; ++j) {
+  for (int g = 0; g < RS_ITEMS; g += RS_GROUP) {
 #pragma unroll
-    for (int w = 0; w < 4; ++w) wcount[w][tid] = 0;
+    for (int c = 0; c < RS_GROUP; ++c)
+#pragma unroll
+      for (int w = 0; w < 4; ++w) slot[c][w][tid] = 0;
     __syncthreads();
-    const int64_t i = base + j * RS_THREADS + tid;
-    const bool valid = i < n;
-    uint32_t k = 0u;
-    int32_t v = 0;
-    if (valid) {
-      if (PIN) {
-        const uint64_t p = static_cast<const uint64_t*>(kin)[i];
-        k = (uint32_t)(p >> 32);
-        v = (int32_t)(uint32_t)p;
-      } else {
-        k = static_cast<const uint32_t*>(kin)[i];
-        v = vin[i];
+    uint32_t k[RS_GROUP];
+    int32_t v[RS_GROUP];
+    int rank[RS_GROUP];
+#pragma unroll
+    for (int c = 0; c < RS_GROUP; ++c) {
+      const int64_t i = base + (int64_t)(g + c) * RS_THREADS + tid;
+      const bool valid = i < n;
+      k[c] = 0u;
+      v[c] = 0;
+      if (valid) {
+        if (PIN) {
+          const uint64_t p = static_cast<const uint64_t*>(kin)[i];
+          k[c] = (uint32_t)(p >> 32);
+          v[c] = (int32_t)(uint32_t)p;
+        } else {
+          k[c] = static_cast<const uint32_t*>(kin)[i];
+          v[c] = vin[i];
+        }
       }
-    }
-    const int d = (k >> shift) & 255;
-    uint64_t peers = __ballot(valid);
+      const int d = (k[c] >> shift) & 255;
+      uint64_t peers = __ballot(valid);
 #pragma unroll
-    for (int b = 0; b < 8; ++b) {
-      const bool bit = (d >> b) & 1;
-      const uint64_t bal = __ballot(bit);
-      peers &= bit ? bal : ~bal;
+      for (int b = 0; b < 8; ++b) {
+        const bool bit = (d >> b) & 1;
+        const uint64_t bal = __ballot(bit);
+        peers &= bit ? bal : ~bal;
+      }
+      rank[c] = valid ? __popcll(peers & lt) : -1;
+      if (valid && rank[c] == 0) slot[c][wid][d] = __popcll(peers);
     }
-    const int rank = __popcll(peers & lt);
-    if (valid && rank == 0) wcount[wid][d] = __popcll(peers);
     __syncthreads();
-    if (valid) {
-      int prior = 0;
-      for (int w = 0; w < wid; ++w) prior += wcount[w][d];
-      const int pos = base_off[d] + running[d] + prior + rank;
+    {  // thread tid owns digit tid: offsets in (chunk, wave) order
+      int32_t run = next_off[tid];
+#pragma unroll
+      for (int c = 0; c < RS_GROUP; ++c)
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+          const int32_t t = slot[c][w][tid];
+          slot[c][w][tid] = run;
+          run += t;
+        }
+      next_off[tid] = run;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < RS_GROUP; ++c) {
+      if (rank[c] < 0) continue;
+      const int d = (k[c] >> shift) & 255;
+      const int pos = slot[c][wid][d] + rank[c];
       if (POUT) {
-        static_cast<uint64_t*>(kout)[pos] = ((uint64_t)k << 32) | (uint32_t)v;
+        static_cast<uint64_t*>(kout)[pos] = ((uint64_t)k[c] << 32) | (uint32_t)v[c];
       } else {
-        static_cast<uint32_t*>(kout)[pos] = k;
-        vout[pos] = v;
+        static_cast<uint32_t*>(kout)[pos] = k[c];
+        vout[pos] = v[c];
       }
     }
-    __syncthreads();
-    running[tid] += wcount[0][tid] + wcount[1][tid] + wcount[2][tid] + wcount[3][tid];
     __syncthreads();
   }
 }
