@@ -681,6 +681,26 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
   }
   DMSC_CHECK(hipMemcpyAsync(ctx->pinned, c32, 16 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
   DMSC_CHECK(hipMemcpyAsync((char*)ctx->pinned + 128, b32, 8 * sizeof(int32_t) * (B + 1), hipMemcpyDeviceToHost, st));
+  // Work that needs no host-side count goes in behind the readback, so the GPU runs it while the host
+  // waits and then allocates the outputs: the cancellation compaction (lists sized for every saddle)
+  // and the geometry start state.
+  for (int s = 0; s < 2; ++s) {
+    if (!aw[s].on) continue;
+    const int b = A_ARC0 + kArcSlots * s;
+    const int64_t next = s ? nmax : nmin;
+    GeoState& g = aw[s].G;
+    int32_t* cs0 = ar.get<int32_t>(b + G_CS0, nsad);
+    int32_t* cd0 = ar.get<int32_t>(b + G_CD0, nsad);
+    uint32_t* ck = ar.get<uint32_t>(b + G_KEYS, nsad);
+    launch_cancel_compact(nsad, sb[s].cancel, sb[s].pers, ar.get<int32_t>(b + G_PRE, nsad), cs0, cd0, ck, st);
+    g.cs = cs0;
+    g.cd = cd0;
+    g.kill = ar.get<uint64_t>(b + G_DEADC, next);
+    g.link = ar.get<int4>(b + G_LINK, next);
+    g.watch = ar.get<uint64_t>(b + G_WATCH, next);
+    g.gjump = ar.get<uint64_t>(b + G_JUMP, next);
+    launch_geo_init(g, next, st);  // every extremum its own root, no killer, no watch
+  }
   DMSC_CHECK(hipStreamSynchronize(st));
   int32_t h32[16];
   std::memcpy(h32, ctx->pinned, sizeof(h32));
@@ -732,30 +752,22 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     // cancellations sorted by (persistence, saddle id) (persistence_struct.hxx:15-19); fields never
     // meet, so one batch-wide stable sort orders every field's cancellations correctly
     const int64_t nc = w.ncancel;
-    int32_t* cs0 = ar.get<int32_t>(b + G_CS0, nc);
-    int32_t* cd0 = ar.get<int32_t>(b + G_CD0, nc);
-    uint32_t* ck = ar.get<uint32_t>(b + G_KEYS, nc);
+    GeoState& g = w.G;
+    int32_t* cs0 = ar.get<int32_t>(b + G_CS0, nsad);  // compacted above (same slots and sizes)
+    int32_t* cd0 = ar.get<int32_t>(b + G_CD0, nsad);
+    uint32_t* ck = ar.get<uint32_t>(b + G_KEYS, nsad);
     int32_t* cslot = ar.get<int32_t>(b + G_SLOT, nc);
-    launch_cancel_compact(nsad, sb[s].cancel, sb[s].pers, ar.get<int32_t>(b + G_PRE, nsad), cs0, cd0, ck, st);
     iota_i32(cslot, nc, st);
     radix_sort_pairs(ck, cslot, ar.get<uint32_t>(b + G_KEYS2, nc), ar.get<int32_t>(b + G_SLOT2, nc), nc, ar, A_RS0, A_RSP0, st);
-    GeoState& g = w.G;
     g.ncancel = nc;
     int32_t* ckey = ar.get<int32_t>(b + G_CS, nc);
-    g.cs = cs0;
-    g.cd = cd0;
     g.key = ckey;
-    g.kill = ar.get<uint64_t>(b + G_DEADC, next);
-    g.link = ar.get<int4>(b + G_LINK, next);
-    g.watch = ar.get<uint64_t>(b + G_WATCH, next);
     g.rr = ar.get<int32_t>(b + G_RR, 4 * nc);
     g.wl = ar.get<int4>(b + G_WL, 2 * nc);
     g.wls = nc;
     g.glen = ar.get<int2>(b + G_GLEN, 2 * nc);
     g.cnt = ar.get<int32_t>(b + G_CNT, 2);
     g.err = d_err;
-    g.gjump = ar.get<uint64_t>(b + G_JUMP, next);
-    launch_geo_init(g, next, st);  // every extremum its own root, no killer, no watch
     launch_cancel_finish(nc, cslot, cd0, ckey, g.kill, st);
     launch_geo_items(nc, cs0, ckey, w.A.tinit, g.list(0), g.cnt, st);  // list 0 and its count
   }
