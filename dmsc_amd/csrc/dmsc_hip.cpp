@@ -357,10 +357,9 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
   // ---- field table, bases and tiles
   std::vector<FieldDesc> fd(B);
   std::vector<Dims> dims(B);
-  std::vector<int64_t> slot_b(B + 1), pix_b(B + 1), word_b(B + 1);
+  std::vector<int64_t> slot_b(B + 1), pix_b(B + 1), word_b(B + 1), tile_b(B + 1);
   std::vector<float> thr(B);
-  std::vector<TileRef> tiles;
-  int64_t S = 0, N = 0, PW = 0;
+  int64_t S = 0, N = 0, PW = 0, T = 0;
   for (int f = 0; f < B; ++f) {
     const Dims d = make_dims(fin[f].H, fin[f].W);
     dims[f] = d;
@@ -369,8 +368,8 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     pix_b[f] = N;
     word_b[f] = PW;
     thr[f] = fin[f].thr;
-    for (int ty = 0; ty < ceil_div(d.Ny, 16); ++ty)
-      for (int tx = 0; tx < ceil_div(d.Nx, 64); ++tx) tiles.push_back(TileRef{f, tx, ty});
+    tile_b[f] = T;
+    T += (int64_t)ceil_div(d.Ny, 16) * ceil_div(d.Nx, 64);  // gradient tiles of 64 x 16 cells, row-major
     S += d.S;
     N += d.N;
     PW += d.PW;
@@ -378,12 +377,13 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
   slot_b[B] = S;
   pix_b[B] = N;
   word_b[B] = PW;
+  tile_b[B] = T;
   if (S > (int64_t)INT32_MAX - 1) throw Fail{DMSC_ERR_TOO_LARGE, "batch group exceeds 32-bit cell indices"};
 
   const size_t o_fd = 0, o_sb = align_up(o_fd + sizeof(FieldDesc) * B, 16),
                o_pb = align_up(o_sb + 8 * (B + 1), 16), o_wb = align_up(o_pb + 8 * (B + 1), 16),
                o_thr = align_up(o_wb + 8 * (B + 1), 16), o_tl = align_up(o_thr + 4 * B, 16),
-               meta_bytes = align_up(o_tl + sizeof(TileRef) * tiles.size(), 16);
+               meta_bytes = align_up(o_tl + 8 * (B + 1), 16);
   ensure_pinned(ctx, std::max<size_t>(meta_bytes, 64 * (size_t)(B + 1) + 1024));
   {
     char* h = (char*)ctx->pinned;
@@ -392,14 +392,14 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     std::memcpy(h + o_pb, pix_b.data(), 8 * (B + 1));
     std::memcpy(h + o_wb, word_b.data(), 8 * (B + 1));
     std::memcpy(h + o_thr, thr.data(), 4 * B);
-    if (!tiles.empty()) std::memcpy(h + o_tl, tiles.data(), sizeof(TileRef) * tiles.size());
+    std::memcpy(h + o_tl, tile_b.data(), 8 * (B + 1));
   }
   char* meta = ar.get<char>(A_META, meta_bytes);
   DMSC_CHECK(hipMemcpyAsync(meta, ctx->pinned, meta_bytes, hipMemcpyHostToDevice, st));
   Batch bt{(const FieldDesc*)(meta + o_fd), (const int64_t*)(meta + o_sb), (const int64_t*)(meta + o_pb),
            (const int64_t*)(meta + o_wb), B};
   const float* d_thr = (const float*)(meta + o_thr);
-  const TileRef* d_tiles = (const TileRef*)(meta + o_tl);
+  const int64_t* d_tile_b = (const int64_t*)(meta + o_tl);
 
   for (int f = 0; f < B; ++f) {
     dmsc_outputs* o = &outs[fin[f].index];
@@ -429,7 +429,7 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
   uint32_t* planes = ar.get<uint32_t>(A_PLANES, 4 * PW);
   uint64_t* pjf = ar.get<uint64_t>(A_PJF, S);
   uint64_t* pjv = ar.get<uint64_t>(A_PJV, N);
-  launch_gradient(dual, bt, d_tiles, (int64_t)tiles.size(), grad ? o_grad : nullptr, fsucc, vsucc, planes, PW, pjf, pjv,
+  launch_gradient(dual, bt, d_tile_b, T, grad ? o_grad : nullptr, fsucc, vsucc, planes, PW, pjf, pjv,
                   st);
 
   int32_t* c32 = ar.get<int32_t>(A_C32, 64);

@@ -355,7 +355,8 @@ void bases_at(const T* pre, const S* starts, int64_t n, const T* total, int B, T
 
 // grid stages (over all fields of a batch)
 // also writes the initial pointer-jumping state of both V-path forests (pjf over slots, pjv over pixels)
-void launch_gradient(bool dual, const Batch& bt, const TileRef* tiles, int64_t ntiles, int32_t* pw, int32_t* fsucc,
+// tile_b: [B+1] first gradient tile of each field (64 x 16 cells per tile, row-major within a field)
+void launch_gradient(bool dual, const Batch& bt, const int64_t* tile_b, int64_t ntiles, int32_t* pw, int32_t* fsucc,
                      int32_t* vsucc, uint32_t* planes, int64_t PW, uint64_t* pjf, uint64_t* pjv, hipStream_t st);
 // min_key / max_key: pack_key(value, cell id) per extremum; list_b: rows of B+1 per-field list bases
 // (minima, saddles, maxima)

@@ -20,7 +20,7 @@ constexpr int GT_X = 64, GT_Y = 16;
 constexpr int GP_W = GT_X + 3, GP_H = GT_Y + 3;
 
 template <bool DUAL>
-__global__ __launch_bounds__(256) void k_gradient(Batch bt, const TileRef* __restrict__ tiles,
+__global__ __launch_bounds__(256) void k_gradient(Batch bt, const int64_t* __restrict__ tile_b,
                                                   int32_t* __restrict__ pw_all, int32_t* __restrict__ fsucc_all,
                                                   int32_t* __restrict__ vsucc_all, uint32_t* __restrict__ planes,
                                                   int64_t PW, uint64_t* __restrict__ pjf_all,
@@ -29,8 +29,11 @@ __global__ __launch_bounds__(256) void k_gradient(Batch bt, const TileRef* __res
   __shared__ int32_t cell[GT_Y][GT_X][4];  // reused by phase 3 as its second buffers
   __shared__ int32_t fJ[GT_Y * GT_X], fD[GT_Y * GT_X], vJ[GT_Y * GT_X], vD[GT_Y * GT_X];
   __shared__ uint8_t lut[kLutSize];
-  const TileRef tr = tiles[blockIdx.x];
-  const FieldDesc fd = bt.fd[tr.field];
+  const int tfield = field_of(tile_b, bt.B, blockIdx.x);
+  const FieldDesc fd = bt.fd[tfield];
+  const int64_t tlocal = blockIdx.x - tile_b[tfield];
+  const int ntx = (fd.Nx + GT_X - 1) / GT_X;
+  const TileRef tr{tfield, (int)(tlocal % ntx), (int)(tlocal / ntx)};
   const float* __restrict__ f = fd.f;
   const int H = fd.H, W = fd.W, WPR = fd.WPR;
   const int Nx = W + 1, Ny = H + 1;
@@ -222,15 +225,15 @@ __global__ __launch_bounds__(256) void k_gradient(Batch bt, const TileRef* __res
   }
 }
 
-void launch_gradient(bool dual, const Batch& bt, const TileRef* tiles, int64_t ntiles, int32_t* pw, int32_t* fsucc,
+void launch_gradient(bool dual, const Batch& bt, const int64_t* tile_b, int64_t ntiles, int32_t* pw, int32_t* fsucc,
                      int32_t* vsucc, uint32_t* planes, int64_t PW, uint64_t* pjf, uint64_t* pjv, hipStream_t st) {
   KScope ks_("gradient", st, true);
   if (ntiles == 0) return;
   if (dual)
-    hipLaunchKernelGGL(k_gradient<true>, dim3((unsigned)ntiles), dim3(256), 0, st, bt, tiles, pw, fsucc, vsucc, planes, PW,
+    hipLaunchKernelGGL(k_gradient<true>, dim3((unsigned)ntiles), dim3(256), 0, st, bt, tile_b, pw, fsucc, vsucc, planes, PW,
                        pjf, pjv);
   else
-    hipLaunchKernelGGL(k_gradient<false>, dim3((unsigned)ntiles), dim3(256), 0, st, bt, tiles, pw, fsucc, vsucc, planes, PW,
+    hipLaunchKernelGGL(k_gradient<false>, dim3((unsigned)ntiles), dim3(256), 0, st, bt, tile_b, pw, fsucc, vsucc, planes, PW,
                        pjf, pjv);
   DMSC_LAUNCH_CHECK();
 }
