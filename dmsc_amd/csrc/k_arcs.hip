@@ -155,43 +155,63 @@ void launch_geo_items(int64_t nc, const int32_t* cs, const int32_t* key, const i
   DMSC_LAUNCH_CHECK();
 }
 
-// Root of x in the geometry forest and the summed link length on the way, by path halving over
+// Roots in the geometry forest and the summed link lengths on the way, by path halving over
 // gjump: (ancestor J, length to J) pairs. The forest only grows at roots (a link is only ever
 // added from a root), so a shortcut stays valid once written; finds run in their own kernel (or
 // phase), so concurrent shortcuts are all consistent 64-bit pairs, and an older pair read from a
 // cache is still valid: the accesses are single 64-bit loads and stores without a coherence scope.
 constexpr uint32_t kVoid = 0xFFFFFFFFu;
-__device__ __forceinline__ void geo_root(const GeoState& g, int x, int& root, int32_t& len) {
-  int32_t acc = 0;
-  for (int it = 0; x >= 0; ++it) {
+// x[k] <- root of x[k] (-1: the void), acc[k] <- the length to it, for the two arc targets of one
+// item, walked in lockstep (each step issues both walks' loads before using either). Find rounds
+// only compress paths, so the walks are independent whatever their interleaving.
+__device__ __forceinline__ void geo_root2(const GeoState& g, int (&x)[2], int32_t (&acc)[2]) {
+  acc[0] = acc[1] = 0;
+  bool done[2] = {x[0] < 0, x[1] < 0};
+  for (int it = 0; !(done[0] && done[1]); ++it) {
     if (it >= kLoopCap) {
       report(g.err, ERR_GEO_CAP);
       break;
     }
-    const uint64_t cur = __hip_atomic_load(&g.gjump[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
-    const uint32_t J = (uint32_t)cur;
-    const int32_t L = (int32_t)(cur >> 32);
-    if (J == (uint32_t)x) break;
-    if (J == kVoid) {
-      acc = sat_add(acc, L);
-      x = -1;
-      break;
+    uint64_t cur[2] = {0, 0}, nx[2] = {0, 0};
+    uint32_t J[2] = {0, 0};
+    int32_t L[2] = {0, 0};
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+      if (!done[k]) cur[k] = __hip_atomic_load(&g.gjump[x[k]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      if (done[k]) continue;
+      J[k] = (uint32_t)cur[k];
+      L[k] = (int32_t)(cur[k] >> 32);
+      if (J[k] == (uint32_t)x[k]) {
+        done[k] = true;
+      } else if (J[k] == kVoid) {
+        acc[k] = sat_add(acc[k], L[k]);
+        x[k] = -1;
+        done[k] = true;
+      }
     }
-    const uint64_t nx = __hip_atomic_load(&g.gjump[J], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
-    const uint32_t JJ = (uint32_t)nx;
-    if (JJ == J) {  // J is the root
-      acc = sat_add(acc, L);
-      x = (int)J;
-      break;
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+      if (!done[k]) nx[k] = __hip_atomic_load(&g.gjump[J[k]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      if (done[k]) continue;
+      const uint32_t JJ = (uint32_t)nx[k];
+      if (JJ == J[k]) {  // J is the root
+        acc[k] = sat_add(acc[k], L[k]);
+        x[k] = (int)J[k];
+        done[k] = true;
+        continue;
+      }
+      const int32_t L2 = sat_add(L[k], (int32_t)(nx[k] >> 32));
+      __hip_atomic_store(&g.gjump[x[k]], (uint64_t)JJ | ((uint64_t)(uint32_t)L2 << 32), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_WAVEFRONT);
+      acc[k] = sat_add(acc[k], L2);
+      x[k] = JJ == kVoid ? -1 : (int)JJ;
+      if (x[k] < 0) done[k] = true;
     }
-    const int32_t L2 = sat_add(L, (int32_t)(nx >> 32));
-    __hip_atomic_store(&g.gjump[x], (uint64_t)JJ | ((uint64_t)(uint32_t)L2 << 32), __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_WAVEFRONT);
-    acc = sat_add(acc, L2);
-    x = JJ == kVoid ? -1 : (int)JJ;
   }
-  root = x;
-  len = acc;
 }
 
 // watch[R] is only ever read by R's killer (condition B of its decision), so cancellation kc posts on
@@ -218,11 +238,14 @@ __device__ __forceinline__ RootPosts geo_find_one(const ArcSide& A, const GeoSta
   int R0 = it.z, R1 = it.w;
   int32_t L0 = 0, L1 = 0;
   if (!first) {  // from round 1 on the item holds the roots it last waited with, lens(cur) the lengths to them
-    geo_root(g, it.z, R0, L0);
-    geo_root(g, it.w, R1, L1);
+    int x[2] = {it.z, it.w};
+    int32_t acc[2];
+    geo_root2(g, x, acc);
+    R0 = x[0];
+    R1 = x[1];
     const int2 base = g.lens(cur)[j];
-    L0 = sat_add(base.x, L0);
-    L1 = sat_add(base.y, L1);
+    L0 = sat_add(base.x, acc[0]);
+    L1 = sat_add(base.y, acc[1]);
   }
   const int round = (int)(0xFFFFFFFFu - (uint32_t)(tag >> 32));
   const uint64_t k0 = R0 >= 0 ? g.kill[R0] : ~0ull, k1 = R1 >= 0 ? g.kill[R1] : ~0ull;
