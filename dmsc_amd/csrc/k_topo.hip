@@ -40,6 +40,58 @@ __device__ __forceinline__ int uf_find(int32_t* __restrict__ p, int x, int32_t* 
   }
 }
 
+// uf_find of two elements in lockstep: each step issues both walks' loads before using either.
+// r[k] <- root of x[k] (-1: none). Halving writes only ever store ancestors, so interleaving the
+// two walks changes no result.
+__device__ __forceinline__ void uf_find2(int32_t* __restrict__ p, const int (&x0)[2], int (&r)[2],
+                                         int32_t* err = nullptr) {
+  int x[2] = {x0[0], x0[1]};
+  bool done[2];
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    r[k] = -1;
+    done[k] = x[k] < 0;
+  }
+  for (int it = 0; !(done[0] && done[1]); ++it) {
+    if (it >= kLoopCap) {  // only a cyclic parent array can get here
+      report(err, ERR_UF_CAP);
+#pragma unroll
+      for (int k = 0; k < 2; ++k)
+        if (!done[k]) r[k] = -1;
+      return;
+    }
+    int px[2] = {0, 0}, ppx[2] = {0, 0};
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+      if (!done[k]) px[k] = p[x[k]];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      if (done[k]) continue;
+      if (px[k] == x[k]) {
+        r[k] = x[k];
+        done[k] = true;
+      } else if (px[k] < 0) {
+        done[k] = true;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+      if (!done[k]) ppx[k] = p[px[k]];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      if (done[k]) continue;
+      if (ppx[k] == px[k]) {
+        r[k] = px[k];
+        done[k] = true;
+        continue;
+      }
+      p[x[k]] = ppx[k];
+      if (ppx[k] < 0) done[k] = true;
+      else x[k] = ppx[k];
+    }
+  }
+}
+
 // rr[2i] = dying root (-2: cycle); rr[2i+1] = (surviving root + 1) << 1 | (dying root came from c[2i+1])
 // Round 0 starts from the saddle's extrema (cc = c[i]). Later rounds start from the decision the
 // saddle waited with (cc, in rr's format): while that dying root is still a root it still dies (the
@@ -58,8 +110,9 @@ __device__ __forceinline__ RootPosts persist_find_decide(const PersistSide& s, c
   } else {
     const int d0 = cc.x, s0 = (cc.y >> 1) - 1;
     const bool b0 = cc.y & 1;
-    const int rd = uf_find(s.par_g, d0, s.err);
-    const int rs = uf_find(s.par_g, s0, s.err);
+    int r2[2];
+    uf_find2(s.par_g, {d0, s0}, r2, s.err);
+    const int rd = r2[0], rs = r2[1];
     ra = b0 ? rs : rd;
     rb = b0 ? rd : rs;
     if (rd == d0 && rs != rd) {
@@ -192,8 +245,9 @@ __device__ __forceinline__ void persist_apply_values(const PersistSide& s, const
     removed[i] = 1;
     const int2 cc = *reinterpret_cast<const int2*>(s.c + 2 * i);
     const int a = cc.x, b = cc.y;
-    const int d = uf_find(s.par_t, b_dies ? b : a, s.err);
-    const int t = uf_find(s.par_t, b_dies ? a : b, s.err);
+    int r2[2];
+    uf_find2(s.par_t, {b_dies ? b : a, b_dies ? a : b}, r2, s.err);
+    const int d = r2[0], t = r2[1];
     if (d != t && d != -1) s.par_t[d] = t;
     if (s.cancel_d) s.cancel_d[i] = d;
   }
