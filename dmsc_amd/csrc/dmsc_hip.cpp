@@ -480,8 +480,7 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
   int32_t* asc = ar.get<int32_t>(A_ASC, nsad);
   int32_t* vals2 = ar.get<int32_t>(A_VALS2, nsad);
   launch_sort_keys(nsad, sad_val, keys, st);
-  iota_i32(asc, nsad, st);
-  radix_sort_pairs(keys, asc, keys2, vals2, nsad, ar, A_RS0, A_RSP0, st);
+  radix_sort_pairs(keys, asc, keys2, vals2, nsad, ar, A_RS0, A_RSP0, st, true);  // (value key, saddle index)
   if (B > 1) {  // stable second pass on the field: (field, value, id) order
     launch_gather_field(nsad, asc, sad_field, keys, st);
     int bits = 0;
@@ -757,8 +756,8 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     int32_t* cd0 = ar.get<int32_t>(b + G_CD0, nsad);
     uint32_t* ck = ar.get<uint32_t>(b + G_KEYS, nsad);
     int32_t* cslot = ar.get<int32_t>(b + G_SLOT, nc);
-    iota_i32(cslot, nc, st);
-    radix_sort_pairs(ck, cslot, ar.get<uint32_t>(b + G_KEYS2, nc), ar.get<int32_t>(b + G_SLOT2, nc), nc, ar, A_RS0, A_RSP0, st);
+    radix_sort_pairs(ck, cslot, ar.get<uint32_t>(b + G_KEYS2, nc), ar.get<int32_t>(b + G_SLOT2, nc), nc, ar, A_RS0,
+                     A_RSP0, st, true);  // (persistence key, compacted slot)
     g.ncancel = nc;
     int32_t* ckey = ar.get<int32_t>(b + G_CS, nc);
     g.key = ckey;

@@ -343,8 +343,9 @@ template <class T>
 void scan_exclusive(const T* in, T* out, int64_t n, T* d_total, Arena& ar, int slot, hipStream_t st);
 void scan_plane_popcounts(const uint32_t* plane_a, const uint32_t* plane_b, int32_t* out, int64_t n, int32_t* d_total,
                           Arena& ar, int slot, hipStream_t st);
+// iota_vals: sort the pairs (keys[i], i); vals is then only written
 void radix_sort_pairs(uint32_t* keys, int32_t* vals, uint32_t* keys_tmp, int32_t* vals_tmp, int64_t n, Arena& ar,
-                      int slot, int pslot, hipStream_t st);
+                      int slot, int pslot, hipStream_t st, bool iota_vals = false);
 void pointer_jump(uint64_t* pj, int64_t n, int32_t* err, hipStream_t st);
 void pointer_jump_init(const int32_t* succ, uint64_t* pj, int64_t n, hipStream_t st);
 void fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t st);
@@ -387,6 +388,6 @@ struct PairView {
 void launch_persistence_backward(bool dual, const float* f, int H, int W, const PairView& pv, float max_pers,
                                  const float* gscale, float* contrib, float* grad, hipStream_t st);
 void radix_sort_pairs_bits(uint32_t* keys, int32_t* vals, uint32_t* keys_tmp, int32_t* vals_tmp, int64_t n, int bits,
-                           Arena& ar, int slot, int pslot, hipStream_t st);
+                           Arena& ar, int slot, int pslot, hipStream_t st, bool iota_vals = false);
 
 }  // namespace dmsc

@@ -331,7 +331,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const void* __restric
           v[c] = (int32_t)(uint32_t)p;
         } else {
           k[c] = static_cast<const uint32_t*>(kin)[i];
-          v[c] = vin[i];
+          v[c] = vin ? vin[i] : (int32_t)i;  // no value array: the values are the indices
         }
       }
       const int d = (k[c] >> shift) & 255;
@@ -384,11 +384,16 @@ __global__ void k_copy_pairs(const uint32_t* __restrict__ ks, const int32_t* __r
 }
 
 // Sorts on the low `bits` bits of the keys (ceil(bits / 8) passes); the result is in (keys, vals).
-// Scratch: arena slots `slot` (counts), slot + 1 (their scan), pslot and pslot + 1 (packed pairs).
+// iota_vals: the input values are 0 .. n-1 (vals is then only written). Scratch: arena slots `slot`
+// (counts), slot + 1 (their scan), pslot and pslot + 1 (packed pairs).
 void radix_sort_pairs_bits(uint32_t* keys, int32_t* vals, uint32_t* keys_tmp, int32_t* vals_tmp, int64_t n, int bits,
-                           Arena& ar, int slot, int pslot, hipStream_t st) {
+                           Arena& ar, int slot, int pslot, hipStream_t st, bool iota_vals) {
   KScope ks_("radix_sort", st);
-  if (n <= 1 || bits <= 0) return;
+  if (n <= 1 || bits <= 0) {
+    if (iota_vals) iota_i32(vals, n, st);
+    return;
+  }
+  const int32_t* const vals_in = iota_vals ? nullptr : vals;
   const int npass = (bits + 7) / 8 > 4 ? 4 : (bits + 7) / 8;
   const int64_t nb = (n + RS_TILE - 1) / RS_TILE;
   int32_t* counts = ar.get<int32_t>(slot, (size_t)(256 * nb));
@@ -406,20 +411,20 @@ void radix_sort_pairs_bits(uint32_t* keys, int32_t* vals, uint32_t* keys_tmp, in
     DMSC_LAUNCH_CHECK();
   };
   if (npass == 1) {
-    pass(0, keys, vals, keys_tmp, vals_tmp, false, false);
+    pass(0, keys, vals_in, keys_tmp, vals_tmp, false, false);
     hipLaunchKernelGGL(k_copy_pairs, dim3(grid_for(n, 256, 16384)), dim3(256), 0, st, keys_tmp, vals_tmp, keys, vals, n);
     DMSC_LAUNCH_CHECK();
     return;
   }
   uint64_t* pk[2] = {ar.get<uint64_t>(pslot, n), ar.get<uint64_t>(pslot + 1, n)};
-  pass(0, keys, vals, pk[0], nullptr, false, true);
+  pass(0, keys, vals_in, pk[0], nullptr, false, true);
   for (int p = 1; p + 1 < npass; ++p) pass(p, pk[(p - 1) & 1], nullptr, pk[p & 1], nullptr, true, true);
   pass(npass - 1, pk[(npass - 2) & 1], nullptr, keys, vals, true, false);
 }
 
 void radix_sort_pairs(uint32_t* keys, int32_t* vals, uint32_t* keys_tmp, int32_t* vals_tmp, int64_t n, Arena& ar,
-                      int slot, int pslot, hipStream_t st) {
-  radix_sort_pairs_bits(keys, vals, keys_tmp, vals_tmp, n, 32, ar, slot, pslot, st);
+                      int slot, int pslot, hipStream_t st, bool iota_vals) {
+  radix_sort_pairs_bits(keys, vals, keys_tmp, vals_tmp, n, 32, ar, slot, pslot, st, iota_vals);
 }
 
 // ------------------------------------------------------------------ pointer jumping
