@@ -490,10 +490,10 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
   int32_t* ord_min = ar.get<int32_t>(A_ORDMIN, nsad);
   int32_t* ord_max = ar.get<int32_t>(A_ORDMAX, nsad);
   // primal: min side ascending, max side descending; dual: the reverse (arcs_topology.hxx:178-181)
-  int32_t* pos_min = ar.get<int32_t>(A_POSMIN, nsad);
-  int32_t* pos_max = ar.get<int32_t>(A_POSMAX, nsad);
-  if (!dual) launch_orders(nsad, asc, sad_field, sad_b, ord_min, ord_max, pos_min, pos_max, st);
-  else launch_orders(nsad, asc, sad_field, sad_b, ord_max, ord_min, pos_max, pos_min, st);
+  // positions in the ascending order only: the descending side mirrors them within the field's range
+  int32_t* pos_asc = ar.get<int32_t>(A_POSMIN, nsad);
+  if (!dual) launch_orders(nsad, asc, sad_field, sad_b, ord_min, ord_max, pos_asc, nullptr, st);
+  else launch_orders(nsad, asc, sad_field, sad_b, ord_max, ord_min, pos_asc, nullptr, st);
 
   // ---- raw arc lengths of the traced sides. The raw arcs need no persistence result: they are
   // traced on the side stream while the persistence rounds run on `st`.
@@ -543,7 +543,9 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
   PersistSide ps[2];
   for (int s = 0; s < 2; ++s) {
     PersistSide& p = ps[s];
-    p.pos = s ? pos_max : pos_min;
+    p.pos = pos_asc;
+    p.sad_b = sad_b;
+    p.pos_rev = (s == 1) != dual;  // primal: the max side descends; dual: the min side
     p.c = s ? cmax : cmin;
     p.sval = sad_val;
     p.ekey = s ? max_key : min_key;

@@ -147,11 +147,24 @@ __device__ __forceinline__ RootPosts persist_find_decide(const PersistSide& s, c
   if (surv >= 0) p.b = &s.minkey[surv];
   return p;
 }
+// the saddle's position in this side's sweep order (its merge key)
+__device__ __forceinline__ uint32_t sweep_pos(const PersistSide& s, int i) {
+  const int32_t p = s.pos[i];
+  if (!s.pos_rev) return (uint32_t)p;
+  int64_t lo = 0, hi = s.wls;  // one field: every saddle
+  if (s.sfield) {
+    const int f = s.sfield[i];
+    lo = s.sad_b[f];
+    hi = s.sad_b[f + 1];
+  }
+  return (uint32_t)(lo + hi - 1 - p);
+}
+
 template <bool DUAL>
 __device__ __forceinline__ RootPosts persist_find_one(const PersistSide& s, const int i, int64_t j, int cur, uint64_t tag,
                                                       bool first) {
   if (first) return persist_find_decide<DUAL>(s, i, j, tag, first, *reinterpret_cast<const int2*>(s.c + 2 * i),
-                                              (uint32_t)s.pos[i]);
+                                              sweep_pos(s, i));
   return persist_find_decide<DUAL>(s, i, j, tag, first, s.roots(cur)[j], s.keys(cur)[j]);
 }
 
@@ -183,7 +196,7 @@ __device__ __forceinline__ void persist_find_list(const PersistSide& s, int cur,
       if (i[u] >= 0) {
         if (first) {
           cc[u] = *reinterpret_cast<const int2*>(s.c + 2 * i[u]);
-          ps[u] = (uint32_t)s.pos[i[u]];
+          ps[u] = sweep_pos(s, i[u]);
         } else {  // by list position: written where the apply appended the waiting saddle
           const int64_t j = base + 64 * u + lane;
           cc[u] = s.roots(cur)[j];
@@ -752,7 +765,7 @@ __global__ void k_orders(int64_t ns, const int32_t* __restrict__ asc, const int3
     fwd[k] = v;
     rev[r] = v;
     pos_fwd[v] = (int)k;
-    pos_rev[v] = (int)r;
+    if (pos_rev) pos_rev[v] = (int)r;
   }
 }
 void launch_orders(int64_t ns, const int32_t* asc, const int32_t* sfield, const int32_t* sad_b, int32_t* fwd,

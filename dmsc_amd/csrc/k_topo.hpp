@@ -7,7 +7,10 @@ namespace dmsc {
 
 // One side (vertex-minima or face-maxima) of the persistence sweep.
 struct PersistSide {
-  const int32_t* pos;   // saddle index -> position in the side's sweep order (the merge key)
+  const int32_t* pos;   // saddle index -> position in the ascending-value order
+  const int32_t* sad_b; // [B+1] first position of each field in the orders
+  int pos_rev;          // this side sweeps in descending order: its position is the mirror of pos within
+                        // the saddle's field range (the merge key)
   const int32_t* c;     // per saddle: c[2i], c[2i+1] = extremum indices of its two V-paths (-1 = none)
   const float* sval;    // per saddle value
   const uint64_t* ekey; // per extremum pack_key(value, cell id)
