@@ -37,7 +37,7 @@ enum : int {
   A_KEYS, A_KEYS2, A_VALS2, A_ASC, A_ORDMIN, A_ORDMAX, A_REMOVED, A_POSMIN, A_POSMAX, A_RSP0, A_RSP1,
   A_SIDE0,  // per persistence side: kSideSlots slots each
   A_SIDE_END = A_SIDE0 + 2 * 15,
-  A_ASM0 = A_SIDE_END,  // assembly: counts, prefixes, sad_to_out, survivor lists (12 slots)
+  A_ASM0 = A_SIDE_END,  // assembly: counts, prefixes, output map, survivor lists (12 slots)
   A_ASM_END = A_ASM0 + 12,
   A_ARC0 = A_ASM_END,  // per arc side: kArcSlots slots each
   A_ARC_END = A_ARC0 + 2 * 22,
@@ -643,7 +643,7 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
   int32_t* surv_pre = ar.get<int32_t>(A_ASM0 + 2, nsad);
   int32_t* rows[2] = {ar.get<int32_t>(A_ASM0 + 3, nsad), ar.get<int32_t>(A_ASM0 + 4, nsad)};
   int32_t* rows_pre[2] = {ar.get<int32_t>(A_ASM0 + 5, nsad), ar.get<int32_t>(A_ASM0 + 6, nsad)};
-  int32_t* sad_to_out = ar.get<int32_t>(A_ASM0 + 7, nsad);
+  int32_t* out_list_mx = ar.get<int32_t>(A_ASM0 + 7, nsad);  // per max-order survivor: its output index
   for (int s = 0; s < 2; ++s) {
     const int64_t next = s ? nmax : nmin;
     launch_u8_to_i32(sb[s].alive, tmp_i32, next, false, st);
@@ -668,8 +668,10 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
   // survivor lists: min order (position = output saddle index) and max order, with their edge rows
   int32_t* surv_list[2] = {ar.get<int32_t>(A_ASM0 + 10, nsad), ar.get<int32_t>(A_SURV1, nsad)};
   int32_t* surv_rows[2] = {ar.get<int32_t>(A_ASM0 + 11, nsad), ar.get<int32_t>(A_SURVR1, nsad)};
-  launch_survivors(nsad, ord_min, surv, surv_pre, rows_pre[0], surv_list[0], surv_rows[0], sad_to_out, st);
-  launch_survivors(nsad, ord_max, surv_mx, surv_mx_pre, rows_pre[1], surv_list[1], surv_rows[1], nullptr, st);
+  launch_survivors(nsad, ord_min, surv, surv_pre, rows_pre[0], surv_list[0], surv_rows[0], nullptr, sad_b, B, nullptr,
+                   st);
+  launch_survivors(nsad, ord_max, surv_mx, surv_mx_pre, rows_pre[1], surv_list[1], surv_rows[1], surv_pre, sad_b, B,
+                   out_list_mx, st);
 
   // ---- cancellation lists of the traced sides
   for (int s = 0; s < 2; ++s) {
@@ -738,7 +740,7 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     const EmitBases eb[2] = {{sad_field, row32(5), row32(3)}, {sad_field, row32(5), row32(4)}};
     launch_emit_side(n_sad_total, surv_list[0], nullptr, nullptr, cmin, sb[0].root, sb[0].pers,
                      sb[0].pair, surv_rows[0], sad_ids, eb[0], o_p[0], o_pp[0], o_e[0], o_sad, es);
-    launch_emit_side(n_sad_total, surv_list[1], nullptr, sad_to_out, cmax, sb[1].root, sb[1].pers,
+    launch_emit_side(n_sad_total, surv_list[1], nullptr, out_list_mx, cmax, sb[1].root, sb[1].pers,
                      sb[1].pair, surv_rows[1], sad_ids, eb[1], o_p[1], o_pp[1], o_e[1], nullptr, es);
     if (gmax) launch_labels_faces(rv, S, pjf, sb[1].root, row32(4), o_flab, es);
     if (gmin) launch_labels_verts(rv, N, pjv, sb[0].root, row32(3), o_vlab, es);

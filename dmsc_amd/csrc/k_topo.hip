@@ -659,29 +659,38 @@ void launch_side_counts(int64_t ns, const int32_t* ord, const uint8_t* removed, 
   DMSC_LAUNCH_CHECK();
 }
 
-// sad_to_out[i] for surviving saddles, from the survivor prefix over the min order
 // Surviving saddles of one side's order as a list: list[r] = saddle, row_list[r] = its first edge row,
 // r = its rank among the survivors in that order (for the min order r is the output saddle index).
-// sad_to_out (min order only): saddle -> output index (-1 when removed).
+// out_list (the max order only): out_list[r] = the saddle's output index. Within a field the max order
+// is the min order reversed, so position k mirrors position lo + hi - 1 - k of the min order, whose
+// survivor prefix min_pre holds that index (survival does not depend on the order).
 __global__ void k_survivors(int64_t ns, const int32_t* __restrict__ ord, const int32_t* __restrict__ surv,
                             const int32_t* __restrict__ surv_pre, const int32_t* __restrict__ row_pre,
                             int32_t* __restrict__ list, int32_t* __restrict__ row_list,
-                            int32_t* __restrict__ sad_to_out) {
+                            const int32_t* __restrict__ min_pre, const int32_t* __restrict__ sad_b, int B,
+                            int32_t* __restrict__ out_list) {
   for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < ns; k += (int64_t)gridDim.x * blockDim.x) {
-    const int i = ord[k];
-    const int r = surv[k] ? surv_pre[k] : -1;
-    if (sad_to_out) sad_to_out[i] = r;
-    if (r >= 0) {
-      list[r] = i;
-      row_list[r] = row_pre[k];
+    if (!surv[k]) continue;
+    const int r = surv_pre[k];
+    list[r] = ord[k];
+    row_list[r] = row_pre[k];
+    if (out_list) {
+      int64_t lo = 0, hi = ns;
+      if (B > 1) {
+        const int f = field_of64(sad_b, B, k);
+        lo = sad_b[f];
+        hi = sad_b[f + 1];
+      }
+      out_list[r] = min_pre[lo + hi - 1 - k];
     }
   }
 }
 void launch_survivors(int64_t ns, const int32_t* ord, const int32_t* surv, const int32_t* surv_pre,
-                      const int32_t* row_pre, int32_t* list, int32_t* row_list, int32_t* sad_to_out, hipStream_t st) {
+                      const int32_t* row_pre, int32_t* list, int32_t* row_list, const int32_t* min_pre,
+                      const int32_t* sad_b, int B, int32_t* out_list, hipStream_t st) {
   if (ns == 0) return;
   hipLaunchKernelGGL(k_survivors, dim3(grid_for(ns, 256, 16384)), dim3(256), 0, st, ns, ord, surv, surv_pre, row_pre,
-                     list, row_list, sad_to_out);
+                     list, row_list, min_pre, sad_b, B, out_list);
   DMSC_LAUNCH_CHECK();
 }
 
@@ -702,7 +711,7 @@ void launch_emit_points(int64_t n, const int32_t* ids, const int32_t* to_out, in
 // One side's per-saddle outputs, visiting that side's order: persistence, paired extremum (through
 // the thresholded root), and edge rows (saddle output index, extremum output index).
 __global__ void k_emit_side(int64_t ns, const int32_t* __restrict__ ord, const uint8_t* __restrict__ removed,
-                            const int32_t* __restrict__ sad_to_out, const int32_t* __restrict__ c,
+                            const int32_t* __restrict__ out_list, const int32_t* __restrict__ c,
                             const int32_t* __restrict__ root_out, const float* __restrict__ pers,
                             const int32_t* __restrict__ pair,
                             const int32_t* __restrict__ row_pre, const int32_t* __restrict__ sad_ids, EmitBases eb,
@@ -717,7 +726,7 @@ __global__ void k_emit_side(int64_t ns, const int32_t* __restrict__ ord, const u
       sbase = eb.sad_out[fi];
       ebase = eb.ext_out[fi];
     }
-    const int so = sad_to_out ? sad_to_out[i] : (int)k;  // no map: ord is the output order
+    const int so = out_list ? out_list[k] : (int)k;  // no list: ord is the output order
     if (out_sad) out_sad[so] = sad_ids[i];
     out_p[so] = pers[i];
     const int pe = pair[i];
@@ -737,13 +746,13 @@ __global__ void k_emit_side(int64_t ns, const int32_t* __restrict__ ord, const u
     }
   }
 }
-void launch_emit_side(int64_t ns, const int32_t* ord, const uint8_t* removed, const int32_t* sad_to_out, const int32_t* c,
+void launch_emit_side(int64_t ns, const int32_t* ord, const uint8_t* removed, const int32_t* out_list, const int32_t* c,
                       const int32_t* root_out, const float* pers, const int32_t* pair,
                       const int32_t* row_pre, const int32_t* sad_ids, const EmitBases& eb, float* out_p, int32_t* out_pp,
                       int32_t* out_e, int32_t* out_sad, hipStream_t st) {
   KScope ks_("emit_side", st);
   if (ns == 0) return;
-  hipLaunchKernelGGL(k_emit_side, dim3(grid_for(ns, 256, 16384)), dim3(256), 0, st, ns, ord, removed, sad_to_out, c,
+  hipLaunchKernelGGL(k_emit_side, dim3(grid_for(ns, 256, 16384)), dim3(256), 0, st, ns, ord, removed, out_list, c,
                      root_out, pers, pair, row_pre, sad_ids, eb, out_p, out_pp, out_e, out_sad);
   DMSC_LAUNCH_CHECK();
 }
