@@ -37,8 +37,8 @@ enum : int {
   A_KEYS, A_KEYS2, A_VALS2, A_ASC, A_ORDMIN, A_ORDMAX, A_REMOVED, A_POSMIN, A_POSMAX, A_RSP0, A_RSP1,
   A_SIDE0,  // per persistence side: kSideSlots slots each
   A_SIDE_END = A_SIDE0 + 2 * 15,
-  A_ASM0 = A_SIDE_END,  // assembly: counts, prefixes, output map, survivor lists (12 slots)
-  A_ASM_END = A_ASM0 + 12,
+  A_ASM0 = A_SIDE_END,  // assembly: counts, prefixes, output map, survivor lists, edge outputs (16 slots)
+  A_ASM_END = A_ASM0 + 16,
   A_ARC0 = A_ASM_END,  // per arc side: kArcSlots slots each
   A_ARC_END = A_ARC0 + 2 * 22,
 };
@@ -655,8 +655,10 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
   }
   int32_t* surv_mx = ar.get<int32_t>(A_ASM0 + 8, nsad);
   int32_t* surv_mx_pre = ar.get<int32_t>(A_ASM0 + 9, nsad);
-  launch_side_counts(nsad, ord_min, removed, cmin, sb[0].root, surv, rows[0], st);
-  launch_side_counts(nsad, ord_max, removed, cmax, sb[1].root, surv_mx, rows[1], st);
+  int2* edges[2] = {ar.get<int2>(A_ASM0 + 12, nsad), ar.get<int2>(A_ASM0 + 13, nsad)};  // by order position
+  int2* edge_list[2] = {ar.get<int2>(A_ASM0 + 14, nsad), ar.get<int2>(A_ASM0 + 15, nsad)};  // by survivor rank
+  launch_side_counts(nsad, ord_min, removed, cmin, sb[0].root, surv, rows[0], edges[0], st);
+  launch_side_counts(nsad, ord_max, removed, cmax, sb[1].root, surv_mx, rows[1], edges[1], st);
   scan_exclusive<int32_t>(surv, surv_pre, nsad, c32 + 5, ar, A_SCAN, st);
   scan_exclusive<int32_t>(surv_mx, surv_mx_pre, nsad, c32 + 10, ar, A_SCAN, st);
   scan_exclusive<int32_t>(rows[0], rows_pre[0], nsad, c32 + 6, ar, A_SCAN, st);
@@ -668,10 +670,10 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
   // survivor lists: min order (position = output saddle index) and max order, with their edge rows
   int32_t* surv_list[2] = {ar.get<int32_t>(A_ASM0 + 10, nsad), ar.get<int32_t>(A_SURV1, nsad)};
   int32_t* surv_rows[2] = {ar.get<int32_t>(A_ASM0 + 11, nsad), ar.get<int32_t>(A_SURVR1, nsad)};
-  launch_survivors(nsad, ord_min, surv, surv_pre, rows_pre[0], surv_list[0], surv_rows[0], nullptr, sad_b, B, nullptr,
-                   st);
-  launch_survivors(nsad, ord_max, surv_mx, surv_mx_pre, rows_pre[1], surv_list[1], surv_rows[1], surv_pre, sad_b, B,
-                   out_list_mx, st);
+  launch_survivors(nsad, ord_min, surv, surv_pre, rows_pre[0], edges[0], surv_list[0], surv_rows[0], edge_list[0],
+                   nullptr, sad_b, B, nullptr, st);
+  launch_survivors(nsad, ord_max, surv_mx, surv_mx_pre, rows_pre[1], edges[1], surv_list[1], surv_rows[1], edge_list[1],
+                   surv_pre, sad_b, B, out_list_mx, st);
 
   // ---- cancellation lists of the traced sides
   for (int s = 0; s < 2; ++s) {
@@ -738,9 +740,9 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     launch_emit_points(nmin, min_ids, sb[0].to_out, o_pts[0], es);
     launch_emit_points(nmax, max_ids, sb[1].to_out, o_pts[1], es);
     const EmitBases eb[2] = {{sad_field, row32(5), row32(3)}, {sad_field, row32(5), row32(4)}};
-    launch_emit_side(n_sad_total, surv_list[0], nullptr, nullptr, cmin, sb[0].root, sb[0].pers,
+    launch_emit_side(n_sad_total, surv_list[0], nullptr, nullptr, edge_list[0], sb[0].root, sb[0].pers,
                      sb[0].pair, surv_rows[0], sad_ids, eb[0], o_p[0], o_pp[0], o_e[0], o_sad, es);
-    launch_emit_side(n_sad_total, surv_list[1], nullptr, out_list_mx, cmax, sb[1].root, sb[1].pers,
+    launch_emit_side(n_sad_total, surv_list[1], nullptr, out_list_mx, edge_list[1], sb[1].root, sb[1].pers,
                      sb[1].pair, surv_rows[1], sad_ids, eb[1], o_p[1], o_pp[1], o_e[1], nullptr, es);
     if (gmax) launch_labels_faces(rv, S, pjf, sb[1].root, row32(4), o_flab, es);
     if (gmin) launch_labels_verts(rv, N, pjv, sb[0].root, row32(3), o_vlab, es);

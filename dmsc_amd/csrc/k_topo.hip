@@ -634,28 +634,31 @@ void launch_root_out(int32_t* root, const int32_t* to_out, int64_t n, hipStream_
   hipLaunchKernelGGL(k_root_out, dim3(grid_for(n, 256, 16384)), dim3(256), 0, st, root, to_out, n);
   DMSC_LAUNCH_CHECK();
 }
+// Per position k of one side's order: survival, edge rows, and the rows' extremum outputs
+// edges[k] = (first, second) (-1: no row; the second is dropped when it repeats the first).
 __global__ void k_side_counts(int64_t ns, const int32_t* __restrict__ ord, const uint8_t* __restrict__ removed,
                               const int32_t* __restrict__ c, const int32_t* __restrict__ root_out,
-                              int32_t* __restrict__ surv, int32_t* __restrict__ rows) {
+                              int32_t* __restrict__ surv, int32_t* __restrict__ rows, int2* __restrict__ edges) {
   for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < ns; k += (int64_t)gridDim.x * blockDim.x) {
     const int i = ord[k];
     const bool alive = !removed[i];
     if (surv) surv[k] = alive ? 1 : 0;
-    int n = 0;
+    int o1 = -1, o2 = -1;
     if (alive) {
       const int2 ab = *reinterpret_cast<const int2*>(c + 2 * i);
-      const int o1 = ab.x >= 0 ? root_out[ab.x] : -1, o2 = ab.y >= 0 ? root_out[ab.y] : -1;
-      if (o1 >= 0) ++n;
-      if (o2 >= 0 && o2 != o1) ++n;
+      o1 = ab.x >= 0 ? root_out[ab.x] : -1;
+      o2 = ab.y >= 0 ? root_out[ab.y] : -1;
+      if (o2 == o1) o2 = -1;
     }
-    rows[k] = n;
+    rows[k] = (o1 >= 0 ? 1 : 0) + (o2 >= 0 ? 1 : 0);
+    edges[k] = make_int2(o1, o2);
   }
 }
 void launch_side_counts(int64_t ns, const int32_t* ord, const uint8_t* removed, const int32_t* c,
-                        const int32_t* root_out, int32_t* surv, int32_t* rows, hipStream_t st) {
+                        const int32_t* root_out, int32_t* surv, int32_t* rows, int2* edges, hipStream_t st) {
   if (ns == 0) return;
   hipLaunchKernelGGL(k_side_counts, dim3(grid_for(ns, 256, 16384)), dim3(256), 0, st, ns, ord, removed, c, root_out,
-                     surv, rows);
+                     surv, rows, edges);
   DMSC_LAUNCH_CHECK();
 }
 
@@ -666,7 +669,8 @@ void launch_side_counts(int64_t ns, const int32_t* ord, const uint8_t* removed, 
 // survivor prefix min_pre holds that index (survival does not depend on the order).
 __global__ void k_survivors(int64_t ns, const int32_t* __restrict__ ord, const int32_t* __restrict__ surv,
                             const int32_t* __restrict__ surv_pre, const int32_t* __restrict__ row_pre,
-                            int32_t* __restrict__ list, int32_t* __restrict__ row_list,
+                            const int2* __restrict__ edges, int32_t* __restrict__ list,
+                            int32_t* __restrict__ row_list, int2* __restrict__ edge_list,
                             const int32_t* __restrict__ min_pre, const int32_t* __restrict__ sad_b, int B,
                             int32_t* __restrict__ out_list) {
   for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < ns; k += (int64_t)gridDim.x * blockDim.x) {
@@ -674,6 +678,7 @@ __global__ void k_survivors(int64_t ns, const int32_t* __restrict__ ord, const i
     const int r = surv_pre[k];
     list[r] = ord[k];
     row_list[r] = row_pre[k];
+    edge_list[r] = edges[k];
     if (out_list) {
       int64_t lo = 0, hi = ns;
       if (B > 1) {
@@ -686,11 +691,11 @@ __global__ void k_survivors(int64_t ns, const int32_t* __restrict__ ord, const i
   }
 }
 void launch_survivors(int64_t ns, const int32_t* ord, const int32_t* surv, const int32_t* surv_pre,
-                      const int32_t* row_pre, int32_t* list, int32_t* row_list, const int32_t* min_pre,
-                      const int32_t* sad_b, int B, int32_t* out_list, hipStream_t st) {
+                      const int32_t* row_pre, const int2* edges, int32_t* list, int32_t* row_list, int2* edge_list,
+                      const int32_t* min_pre, const int32_t* sad_b, int B, int32_t* out_list, hipStream_t st) {
   if (ns == 0) return;
   hipLaunchKernelGGL(k_survivors, dim3(grid_for(ns, 256, 16384)), dim3(256), 0, st, ns, ord, surv, surv_pre, row_pre,
-                     list, row_list, min_pre, sad_b, B, out_list);
+                     edges, list, row_list, edge_list, min_pre, sad_b, B, out_list);
   DMSC_LAUNCH_CHECK();
 }
 
@@ -711,7 +716,7 @@ void launch_emit_points(int64_t n, const int32_t* ids, const int32_t* to_out, in
 // One side's per-saddle outputs, visiting that side's order: persistence, paired extremum (through
 // the thresholded root), and edge rows (saddle output index, extremum output index).
 __global__ void k_emit_side(int64_t ns, const int32_t* __restrict__ ord, const uint8_t* __restrict__ removed,
-                            const int32_t* __restrict__ out_list, const int32_t* __restrict__ c,
+                            const int32_t* __restrict__ out_list, const int2* __restrict__ edge_list,
                             const int32_t* __restrict__ root_out, const float* __restrict__ pers,
                             const int32_t* __restrict__ pair,
                             const int32_t* __restrict__ row_pre, const int32_t* __restrict__ sad_ids, EmitBases eb,
@@ -732,27 +737,26 @@ __global__ void k_emit_side(int64_t ns, const int32_t* __restrict__ ord, const u
     const int pe = pair[i];
     const int po = pe >= 0 ? root_out[pe] : -1;
     out_pp[so] = po >= 0 ? po - ebase : -1;
-    const int2 ab = *reinterpret_cast<const int2*>(c + 2 * (int64_t)i);
-    const int o1 = ab.x >= 0 ? root_out[ab.x] : -1, o2 = ab.y >= 0 ? root_out[ab.y] : -1;
+    const int2 o = edge_list[k];
     int row = row_pre[k];
-    if (o1 >= 0) {
+    if (o.x >= 0) {
       out_e[2 * (int64_t)row] = so - sbase;
-      out_e[2 * (int64_t)row + 1] = o1 - ebase;
+      out_e[2 * (int64_t)row + 1] = o.x - ebase;
       ++row;
     }
-    if (o2 >= 0 && o2 != o1) {
+    if (o.y >= 0) {
       out_e[2 * (int64_t)row] = so - sbase;
-      out_e[2 * (int64_t)row + 1] = o2 - ebase;
+      out_e[2 * (int64_t)row + 1] = o.y - ebase;
     }
   }
 }
-void launch_emit_side(int64_t ns, const int32_t* ord, const uint8_t* removed, const int32_t* out_list, const int32_t* c,
-                      const int32_t* root_out, const float* pers, const int32_t* pair,
+void launch_emit_side(int64_t ns, const int32_t* ord, const uint8_t* removed, const int32_t* out_list,
+                      const int2* edge_list, const int32_t* root_out, const float* pers, const int32_t* pair,
                       const int32_t* row_pre, const int32_t* sad_ids, const EmitBases& eb, float* out_p, int32_t* out_pp,
                       int32_t* out_e, int32_t* out_sad, hipStream_t st) {
   KScope ks_("emit_side", st);
   if (ns == 0) return;
-  hipLaunchKernelGGL(k_emit_side, dim3(grid_for(ns, 256, 16384)), dim3(256), 0, st, ns, ord, removed, out_list, c,
+  hipLaunchKernelGGL(k_emit_side, dim3(grid_for(ns, 256, 16384)), dim3(256), 0, st, ns, ord, removed, out_list, edge_list,
                      root_out, pers, pair, row_pre, sad_ids, eb, out_p, out_pp, out_e, out_sad);
   DMSC_LAUNCH_CHECK();
 }

@@ -62,11 +62,12 @@ void launch_u8_to_i32(const uint8_t* a, int32_t* o, int64_t n, bool invert, hipS
 void launch_mask_prefix(const uint8_t* alive, int32_t* pre, int64_t n, hipStream_t st);
 // root[x] (thresholded root of extremum x) <- output index of that root, -1 when none (in place)
 void launch_root_out(int32_t* root, const int32_t* to_out, int64_t n, hipStream_t st);
+// per position: survival, edge rows and their extremum outputs (edges: int2 per position)
 void launch_side_counts(int64_t ns, const int32_t* ord, const uint8_t* removed, const int32_t* c,
-                        const int32_t* root_out, int32_t* surv, int32_t* rows, hipStream_t st);
+                        const int32_t* root_out, int32_t* surv, int32_t* rows, int2* edges, hipStream_t st);
 void launch_survivors(int64_t ns, const int32_t* ord, const int32_t* surv, const int32_t* surv_pre,
-                      const int32_t* row_pre, int32_t* list, int32_t* row_list, const int32_t* min_pre,
-                      const int32_t* sad_b, int B, int32_t* out_list, hipStream_t st);
+                      const int32_t* row_pre, const int2* edges, int32_t* list, int32_t* row_list, int2* edge_list,
+                      const int32_t* min_pre, const int32_t* sad_b, int B, int32_t* out_list, hipStream_t st);
 void launch_emit_points(int64_t n, const int32_t* ids, const int32_t* to_out, int32_t* out, hipStream_t st);
 // ord / removed: a side order over all saddles with their removal flags, or a survivor list with
 // removed == null (row_pre indexed like ord). Output indices are batch-wide positions; index values written into the outputs are made
@@ -77,8 +78,9 @@ struct EmitBases {
   const int32_t* ext_out;   // [B+1] first output extremum (this side) of each field
 };
 // out_list[k]: output index of the saddle at position k of ord (nullptr: position k is output saddle k)
-void launch_emit_side(int64_t ns, const int32_t* ord, const uint8_t* removed, const int32_t* out_list, const int32_t* c,
-                      const int32_t* root_out, const float* pers, const int32_t* pair,
+// edge_list[k]: the edge rows' extremum outputs of the saddle at position k (from the side counts)
+void launch_emit_side(int64_t ns, const int32_t* ord, const uint8_t* removed, const int32_t* out_list,
+                      const int2* edge_list, const int32_t* root_out, const float* pers, const int32_t* pair,
                       const int32_t* row_pre, const int32_t* sad_ids, const EmitBases& eb, float* out_p, int32_t* out_pp,
                       int32_t* out_e, int32_t* out_sad, hipStream_t st);
 // Sweep orders from the ascending order `asc`, grouped by field (sad_b: [B+1] saddle bases):
