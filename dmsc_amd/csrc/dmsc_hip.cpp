@@ -40,7 +40,7 @@ enum : int {
   A_ASM0 = A_SIDE_END,  // assembly: counts, prefixes, output map, survivor lists, edge outputs (16 slots)
   A_ASM_END = A_ASM0 + 16,
   A_ARC0 = A_ASM_END,  // per arc side: kArcSlots slots each
-  A_ARC_END = A_ARC0 + 2 * 22,
+  A_ARC_END = A_ARC0 + 2 * 23,
 };
 
 enum SideSlot { P_PARG, P_PART, P_ALIVE, P_MINKEY, P_PAIR, P_PERS, P_CANCEL, P_RR, P_WL, P_CNT, P_ROOT, P_TOOUT,
@@ -48,7 +48,7 @@ enum SideSlot { P_PARG, P_PART, P_ALIVE, P_MINKEY, P_PAIR, P_PERS, P_CANCEL, P_R
 static_assert(A_SIDE_END - A_SIDE0 == 2 * kSideSlots, "side slot count");
 enum ArcSlot {
   G_BLEN, G_BOFF, G_ARENA, G_FLAG, G_PRE, G_CS0, G_CD0, G_KEYS, G_SLOT, G_KEYS2, G_SLOT2, G_CS, G_CD, G_DEADC,
-  G_LINK, G_DONE, G_WATCH, G_RR, G_WL, G_CNT, G_JUMP, G_GLEN, kArcSlots
+  G_LINK, G_DONE, G_WATCH, G_RR, G_WL, G_CNT, G_JUMP, G_GLEN, G_TL, kArcSlots
 };
 static_assert(A_ARC_END - A_ARC0 == 2 * kArcSlots, "arc slot count");
 enum ArcSlot2 { H_PJQ = 0, H_FLEN, H_PERPOS, H_POSPRE, H_SEGA, H_SEGB, H_RAW, H_NSEG, H_CUMR, H_CUMC, H_ARCB, H_REC };
@@ -506,7 +506,8 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     const int b = A_ARC0 + kArcSlots * s;
     int32_t* blen = ar.get<int32_t>(b + G_BLEN, 2 * nsad);
     int64_t* boff = ar.get<int64_t>(b + G_BOFF, 2 * nsad);
-    launch_arc_lengths(2 * nsad, s ? cmax : cmin, s ? lmax : lmin, blen, st);
+    int4* tl = ar.get<int4>(b + G_TL, nsad);
+    launch_arc_lengths(nsad, s ? cmax : cmin, s ? lmax : lmin, blen, tl, st);
     scan_i32_to_i64(blen, boff, 2 * nsad, c64 + 2 * s, ar, A_SCAN, st);
     w.A.faces = s;
     w.A.succ = s ? fsucc : vsucc;
@@ -517,6 +518,7 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     w.A.sad_ids = sad_ids;
     w.A.tinit = s ? cmax : cmin;
     w.A.blen = blen;
+    w.A.tl = tl;
     w.A.boff = boff;
   }
   StreamFork fork(ctx, st, tmin || tmax);

@@ -29,14 +29,19 @@ __device__ __forceinline__ int32_t sat_add(int32_t a, int32_t b) {
 }
 
 // ------------------------------------------------------------------ raw arcs
-__global__ void k_arc_lengths(int64_t n2, const int32_t* __restrict__ c, const int32_t* __restrict__ steps,
-                              int32_t* __restrict__ blen) {
-  for (int64_t a = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; a < n2; a += (int64_t)gridDim.x * blockDim.x)
-    blen[a] = c[a] >= 0 ? steps[a] + 1 : 0;
+__global__ void k_arc_lengths(int64_t ns, const int32_t* __restrict__ c, const int32_t* __restrict__ steps,
+                              int32_t* __restrict__ blen, int4* __restrict__ tl) {
+  for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < ns; s += (int64_t)gridDim.x * blockDim.x) {
+    const int2 cc = *reinterpret_cast<const int2*>(c + 2 * s);
+    const int2 st = *reinterpret_cast<const int2*>(steps + 2 * s);
+    const int2 l = make_int2(cc.x >= 0 ? st.x + 1 : 0, cc.y >= 0 ? st.y + 1 : 0);
+    *reinterpret_cast<int2*>(blen + 2 * s) = l;
+    tl[s] = make_int4(cc.x, cc.y, l.x, l.y);
+  }
 }
-void launch_arc_lengths(int64_t n2, const int32_t* c, const int32_t* steps, int32_t* blen, hipStream_t st) {
-  if (n2 == 0) return;
-  hipLaunchKernelGGL(k_arc_lengths, dim3(grid_for(n2, 256, 16384)), dim3(256), 0, st, n2, c, steps, blen);
+void launch_arc_lengths(int64_t ns, const int32_t* c, const int32_t* steps, int32_t* blen, int4* tl, hipStream_t st) {
+  if (ns == 0) return;
+  hipLaunchKernelGGL(k_arc_lengths, dim3(grid_for(ns, 256, 16384)), dim3(256), 0, st, ns, c, steps, blen, tl);
   DMSC_LAUNCH_CHECK();
 }
 
@@ -663,8 +668,8 @@ __global__ void k_final_lengths(int64_t ns, const int32_t* __restrict__ ord, Arc
                                 int64_t* __restrict__ flen, int64_t* __restrict__ per_pos) {
   for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < ns; k += (int64_t)gridDim.x * blockDim.x) {
     const int i = ord[k];
-    const int2 T = *reinterpret_cast<const int2*>(A.tinit + 2 * (int64_t)i);
-    const int2 L = *reinterpret_cast<const int2*>(A.blen + 2 * (int64_t)i);
+    const int4 tl = A.tl[i];
+    const int2 T = make_int2(tl.x, tl.y), L = make_int2(tl.z, tl.w);
     const uint64_t p0 = T.x >= 0 ? pjq[T.x] : 0, p1 = T.y >= 0 ? pjq[T.y] : 0;
     ArcRec r;
     r.T[0] = T.x;
@@ -770,13 +775,13 @@ __device__ __forceinline__ LinkParts link_parts(const ArcSide& A, const uint64_t
   L.kd = lk.w;
   L.ko = 1 - L.kd;
   L.s = lk.z;
-  const int2 T = *reinterpret_cast<const int2*>(A.tinit + 2 * L.s);
-  L.Td = L.kd ? T.y : T.x;
-  L.To = L.kd ? T.x : T.y;
+  const int4 tl = A.tl[L.s];
+  L.Td = L.kd ? tl.y : tl.x;
+  L.To = L.kd ? tl.x : tl.y;
   L.Ro = lk.x;
   L.Alen = qlen(pjq, L.Td) - qlen(pjq, d);
-  L.Bk = A.blen[2 * L.s + L.kd];
-  L.Bo = A.blen[2 * L.s + L.ko];
+  L.Bk = L.kd ? tl.w : tl.z;
+  L.Bo = L.kd ? tl.z : tl.w;
   L.Clen = L.To >= 0 ? qlen(pjq, L.To) - qlen(pjq, L.Ro) : 0;
   return L;
 }

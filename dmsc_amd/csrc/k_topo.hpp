@@ -102,11 +102,14 @@ struct ArcSide {
   const int32_t* sad_ids;
   const int32_t* tinit;    // per arc (2s+k): initial target extremum, -1 = no arc
   const int32_t* blen;     // per arc: raw length in cells (steps + 1), 0 = no arc
+  const int4* tl;          // per saddle: both arcs' targets and raw lengths (tinit[2s], tinit[2s+1], blen[2s],
+                           // blen[2s+1]) in one 16-byte record for the stages that gather by saddle
   const int64_t* boff;     // per arc: offset in the raw arena
   int32_t* arena;          // raw arcs, cell ids from the saddle outwards
 };
 
-void launch_arc_lengths(int64_t n2, const int32_t* c, const int32_t* steps, int32_t* blen, hipStream_t st);
+// per saddle: blen[2s..2s+1] and tl[s] (targets c[2s..2s+1] with those lengths)
+void launch_arc_lengths(int64_t ns, const int32_t* c, const int32_t* steps, int32_t* blen, int4* tl, hipStream_t st);
 void launch_raw_arcs(const ArcSide& a, hipStream_t st);
 
 struct GeoState {
