@@ -636,11 +636,8 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     std::memcpy(ctx->pinned + kArenaPin, ctx->pinned, 4 * sizeof(int64_t));
     launch_raw();
   }
-  launch_uf_flatten(sb[0].par_t, sb[0].root, nmin, st);
-  launch_uf_flatten(sb[1].par_t, sb[1].root, nmax, st);
 
   // ---- assembly counts (cpu/workspace.hxx:107-168)
-  int32_t* tmp_i32 = ar.get<int32_t>(A_ASM0 + 0, std::max(nmin, nmax));
   int32_t* surv = ar.get<int32_t>(A_ASM0 + 1, nsad);
   int32_t* surv_pre = ar.get<int32_t>(A_ASM0 + 2, nsad);
   int32_t* rows[2] = {ar.get<int32_t>(A_ASM0 + 3, nsad), ar.get<int32_t>(A_ASM0 + 4, nsad)};
@@ -648,12 +645,11 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
   int32_t* out_list_mx = ar.get<int32_t>(A_ASM0 + 7, nsad);  // per max-order survivor: its output index
   for (int s = 0; s < 2; ++s) {
     const int64_t next = s ? nmax : nmin;
-    launch_u8_to_i32(sb[s].alive, tmp_i32, next, false, st);
-    scan_exclusive<int32_t>(tmp_i32, sb[s].to_out, next, c32 + 3 + s, ar, A_SCAN, st);
+    scan_u8_flags(sb[s].alive, sb[s].to_out, next, c32 + 3 + s, ar, A_SCAN, st);
     // first output extremum of each field (sampled before dead extrema are masked)
     bases_at<int32_t, int32_t>(sb[s].to_out, row32(s ? 2 : 0), next, c32 + 3 + s, B, row32(3 + s), st);
     launch_mask_prefix(sb[s].alive, sb[s].to_out, next, st);
-    launch_root_out(sb[s].root, sb[s].to_out, next, st);  // root[] holds root output indices from here on
+    launch_root_out(sb[s].par_t, sb[s].to_out, sb[s].root, next, st);  // root[]: root output indices
   }
   int32_t* surv_mx = ar.get<int32_t>(A_ASM0 + 8, nsad);
   int32_t* surv_mx_pre = ar.get<int32_t>(A_ASM0 + 9, nsad);

@@ -341,6 +341,8 @@ struct RankView {
 // primitives
 template <class T>
 void scan_exclusive(const T* in, T* out, int64_t n, T* d_total, Arena& ar, int slot, hipStream_t st);
+// exclusive scan of 0/1 flags stored as bytes (nonzero = 1)
+void scan_u8_flags(const uint8_t* in, int32_t* out, int64_t n, int32_t* d_total, Arena& ar, int slot, hipStream_t st);
 void scan_plane_popcounts(const uint32_t* plane_a, const uint32_t* plane_b, int32_t* out, int64_t n, int32_t* d_total,
                           Arena& ar, int slot, hipStream_t st);
 // iota_vals: sort the pairs (keys[i], i); vals is then only written

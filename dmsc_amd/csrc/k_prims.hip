@@ -100,6 +100,25 @@ struct LoadWiden {
     }
   }
 };
+// 0/1 flags from a byte array (nonzero = 1); 8-byte loads when aligned
+struct LoadU8 {
+  const uint8_t* p;
+  bool vec;
+  __device__ int32_t operator()(int64_t i) const { return p[i] ? 1 : 0; }
+  __device__ void load8(int64_t base, int64_t n, int32_t (&v)[8]) const {
+    if (vec && base + 8 <= n) {
+      const uint2 w = *reinterpret_cast<const uint2*>(p + base);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        v[j] = (w.x >> (8 * j)) & 0xFF ? 1 : 0;
+        v[4 + j] = (w.y >> (8 * j)) & 0xFF ? 1 : 0;
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) v[j] = base + j < n ? (*this)(base + j) : 0;
+    }
+  }
+};
 static_assert(SCAN_ITEMS == 8, "load8 / store8 move 8 elements per thread");
 
 // out[base + j] = v[j] for the thread's run, with 16-byte stores when allowed
@@ -256,6 +275,10 @@ void scan_plane_popcounts(const uint32_t* plane_a, const uint32_t* plane_b, int3
                           Arena& ar, int slot, hipStream_t st) {
   scan_impl<int32_t>(LoadPop{plane_a, plane_b, aligned16(plane_a) && (!plane_b || aligned16(plane_b))}, out, n,
                      d_total, ar, slot, st);
+}
+
+void scan_u8_flags(const uint8_t* in, int32_t* out, int64_t n, int32_t* d_total, Arena& ar, int slot, hipStream_t st) {
+  scan_impl<int32_t>(LoadU8{in, (reinterpret_cast<uintptr_t>(in) & 7) == 0}, out, n, d_total, ar, slot, st);
 }
 
 void scan_i32_to_i64(const int32_t* in, int64_t* out, int64_t n, int64_t* d_total, Arena& ar, int slot, hipStream_t st) {

@@ -584,28 +584,7 @@ void launch_persist_round(bool dual, const PersistSide& s0, const PersistSide& s
 }
 
 // root of every extremum in the thresholded union-find (reference flattening, cell_groups.hxx:128-146)
-__global__ void k_uf_flatten(int32_t* __restrict__ par, int32_t* __restrict__ root, int64_t n) {
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
-    root[e] = uf_find(par, (int)e);
-}
-void launch_uf_flatten(int32_t* par, int32_t* root, int64_t n, hipStream_t st) {
-  KScope ks_("uf_flatten", st);
-  if (n == 0) return;
-  hipLaunchKernelGGL(k_uf_flatten, dim3(grid_for(n, 256, 16384)), dim3(256), 0, st, par, root, n);
-  DMSC_LAUNCH_CHECK();
-}
-
 // ------------------------------------------------------------------ assembly (cpu/workspace.hxx:86-365)
-__global__ void k_u8_to_i32(const uint8_t* __restrict__ a, int32_t* __restrict__ o, int64_t n, int invert) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    o[i] = invert ? (a[i] ? 0 : 1) : (a[i] ? 1 : 0);
-}
-void launch_u8_to_i32(const uint8_t* a, int32_t* o, int64_t n, bool invert, hipStream_t st) {
-  if (n == 0) return;
-  hipLaunchKernelGGL(k_u8_to_i32, dim3(grid_for(n, 256, 16384)), dim3(256), 0, st, a, o, n, invert ? 1 : 0);
-  DMSC_LAUNCH_CHECK();
-}
-
 // to_out[e] = prefix[e] if alive else -1 (in place on the prefix array)
 __global__ void k_mask_prefix(const uint8_t* __restrict__ alive, int32_t* __restrict__ pre, int64_t n) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -620,18 +599,20 @@ void launch_mask_prefix(const uint8_t* alive, int32_t* pre, int64_t n, hipStream
 // Per order position: survivor flag (min order) and edge-row count of one side.
 // Per position of the side's order (sequential writes; the per-saddle reads are gathers: visiting
 // saddles in id order and scattering the two counts measured slower except on heavy-tie noise).
-// root_out[x] <- output index of x's thresholded root (-1: no root or the root is not output). Output
-// indices of live extrema are distinct, so two extrema share a root output exactly when they share a
-// live root; every consumer below compares these instead of the roots.
-__global__ void k_root_out(int32_t* __restrict__ root, const int32_t* __restrict__ to_out, int64_t n) {
+// root_out[x] <- output index of x's thresholded root (the root of x in par_t; -1: no root or the root
+// is not output). Output indices of live extrema are distinct, so two extrema share a root output
+// exactly when they share a live root; every consumer below compares these instead of the roots.
+__global__ void k_root_out(int32_t* __restrict__ par, const int32_t* __restrict__ to_out,
+                           int32_t* __restrict__ root_out, int64_t n) {
   for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < n; x += (int64_t)gridDim.x * blockDim.x) {
-    const int r = root[x];
-    root[x] = r >= 0 ? to_out[r] : -1;
+    const int r = uf_find(par, (int)x);
+    root_out[x] = r >= 0 ? to_out[r] : -1;
   }
 }
-void launch_root_out(int32_t* root, const int32_t* to_out, int64_t n, hipStream_t st) {
+void launch_root_out(int32_t* par, const int32_t* to_out, int32_t* root_out, int64_t n, hipStream_t st) {
+  KScope ks_("uf_flatten", st);
   if (n == 0) return;
-  hipLaunchKernelGGL(k_root_out, dim3(grid_for(n, 256, 16384)), dim3(256), 0, st, root, to_out, n);
+  hipLaunchKernelGGL(k_root_out, dim3(grid_for(n, 256, 16384)), dim3(256), 0, st, par, to_out, root_out, n);
   DMSC_LAUNCH_CHECK();
 }
 // Per position k of one side's order: survival, edge rows, and the rows' extremum outputs
