@@ -648,8 +648,7 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     scan_u8_flags(sb[s].alive, sb[s].to_out, next, c32 + 3 + s, ar, A_SCAN, st);
     // first output extremum of each field (sampled before dead extrema are masked)
     bases_at<int32_t, int32_t>(sb[s].to_out, row32(s ? 2 : 0), next, c32 + 3 + s, B, row32(3 + s), st);
-    launch_mask_prefix(sb[s].alive, sb[s].to_out, next, st);
-    launch_root_out(sb[s].par_t, sb[s].to_out, sb[s].root, next, st);  // root[]: root output indices
+    launch_root_out(sb[s].par_t, sb[s].alive, sb[s].to_out, sb[s].root, next, st);  // masks to_out too
   }
   int32_t* surv_mx = ar.get<int32_t>(A_ASM0 + 8, nsad);
   int32_t* surv_mx_pre = ar.get<int32_t>(A_ASM0 + 9, nsad);

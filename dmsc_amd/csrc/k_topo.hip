@@ -583,40 +583,32 @@ void launch_persist_round(bool dual, const PersistSide& s0, const PersistSide& s
   DMSC_LAUNCH_CHECK();
 }
 
-// root of every extremum in the thresholded union-find (reference flattening, cell_groups.hxx:128-146)
 // ------------------------------------------------------------------ assembly (cpu/workspace.hxx:86-365)
-// to_out[e] = prefix[e] if alive else -1 (in place on the prefix array)
-__global__ void k_mask_prefix(const uint8_t* __restrict__ alive, int32_t* __restrict__ pre, int64_t n) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    if (!alive[i]) pre[i] = -1;
-}
-void launch_mask_prefix(const uint8_t* alive, int32_t* pre, int64_t n, hipStream_t st) {
-  if (n == 0) return;
-  hipLaunchKernelGGL(k_mask_prefix, dim3(grid_for(n, 256, 16384)), dim3(256), 0, st, alive, pre, n);
-  DMSC_LAUNCH_CHECK();
-}
-
-// Per order position: survivor flag (min order) and edge-row count of one side.
-// Per position of the side's order (sequential writes; the per-saddle reads are gathers: visiting
-// saddles in id order and scattering the two counts measured slower except on heavy-tie noise).
-// root_out[x] <- output index of x's thresholded root (the root of x in par_t; -1: no root or the root
-// is not output). Output indices of live extrema are distinct, so two extrema share a root output
+// root_out[x] <- output index of x's thresholded root (the root of x in par_t, as the reference's
+// flattening finds it, cell_groups.hxx:128-146; -1: no root or the root is not output). Output indices of live extrema are distinct, so two extrema share a root output
 // exactly when they share a live root; every consumer below compares these instead of the roots.
-__global__ void k_root_out(int32_t* __restrict__ par, const int32_t* __restrict__ to_out,
+// The same pass masks the prefix into to_out (-1 for dead extrema, in place): a live root's entry
+// reads the same before and after its own thread masks, and a dead one is never read.
+__global__ void k_root_out(int32_t* __restrict__ par, const uint8_t* __restrict__ alive, int32_t* to_out,
                            int32_t* __restrict__ root_out, int64_t n) {
   for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < n; x += (int64_t)gridDim.x * blockDim.x) {
     const int r = uf_find(par, (int)x);
-    root_out[x] = r >= 0 ? to_out[r] : -1;
+    root_out[x] = (r >= 0 && alive[r]) ? to_out[r] : -1;
+    if (!alive[x]) to_out[x] = -1;
   }
 }
-void launch_root_out(int32_t* par, const int32_t* to_out, int32_t* root_out, int64_t n, hipStream_t st) {
+void launch_root_out(int32_t* par, const uint8_t* alive, int32_t* to_out, int32_t* root_out, int64_t n,
+                     hipStream_t st) {
   KScope ks_("uf_flatten", st);
   if (n == 0) return;
-  hipLaunchKernelGGL(k_root_out, dim3(grid_for(n, 256, 16384)), dim3(256), 0, st, par, to_out, root_out, n);
+  hipLaunchKernelGGL(k_root_out, dim3(grid_for(n, 256, 16384)), dim3(256), 0, st, par, alive, to_out, root_out, n);
   DMSC_LAUNCH_CHECK();
 }
-// Per position k of one side's order: survival, edge rows, and the rows' extremum outputs
-// edges[k] = (first, second) (-1: no row; the second is dropped when it repeats the first).
+
+// Per position k of one side's order (sequential writes; the per-saddle reads are gathers: visiting
+// saddles in id order and scattering the counts measured slower except on heavy-tie noise): survival
+// (min order), edge rows, and the rows' extremum outputs edges[k] = (first, second) (-1: no row; the
+// second is dropped when it repeats the first).
 __global__ void k_side_counts(int64_t ns, const int32_t* __restrict__ ord, const uint8_t* __restrict__ removed,
                               const int32_t* __restrict__ c, const int32_t* __restrict__ root_out,
                               int32_t* __restrict__ surv, int32_t* __restrict__ rows, int2* __restrict__ edges) {

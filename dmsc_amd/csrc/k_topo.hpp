@@ -57,9 +57,10 @@ void launch_persist_tail(bool dual, const PersistSide& s0, const PersistSide& s1
 bool launch_persist_rounds(bool dual, const PersistSide& s0, const PersistSide& s1, int round0, int max_rounds,
                            uint8_t* removed, int32_t* round_out, int64_t max_items, hipStream_t st);
 
-void launch_mask_prefix(const uint8_t* alive, int32_t* pre, int64_t n, hipStream_t st);
+// to_out: exclusive prefix of the alive flags, masked to -1 for dead extrema by this pass;
 // root_out[x] <- output index of the thresholded root of extremum x (its root in par), -1 when none
-void launch_root_out(int32_t* par, const int32_t* to_out, int32_t* root_out, int64_t n, hipStream_t st);
+void launch_root_out(int32_t* par, const uint8_t* alive, int32_t* to_out, int32_t* root_out, int64_t n,
+                     hipStream_t st);
 // per position: survival, edge rows and their extremum outputs (edges: int2 per position)
 void launch_side_counts(int64_t ns, const int32_t* ord, const uint8_t* removed, const int32_t* c,
                         const int32_t* root_out, int32_t* surv, int32_t* rows, int2* edges, hipStream_t st);
