@@ -898,9 +898,11 @@ void launch_expand_segments(const ArcSide& a, const GeoState& g, const uint64_t*
   DMSC_LAUNCH_CHECK();
 }
 
-// Raw pieces -> output cells. A wave takes 64 pieces, lays their cells end to end (prefix sum of
-// lengths) and copies that run 64 cells at a time, so reads and writes are contiguous within each
-// piece whatever the piece lengths.
+// Raw pieces -> output cells. A wave takes 64 pieces, lays the cells of its short pieces end to end
+// (prefix sum of lengths) and copies that run 64 cells at a time, so reads and writes are contiguous
+// within each piece; pieces of kLongPiece cells or more are then copied one at a time by the whole
+// wave, without the per-cell piece search.
+constexpr int32_t kLongPiece = 128;
 __global__ __launch_bounds__(256) void k_copy_raw(ArcSide A, const Seg* __restrict__ raw,
                                                   const int32_t* __restrict__ nraw, int64_t max_n,
                                                   int32_t* __restrict__ out, int64_t out_total,
@@ -925,8 +927,11 @@ __global__ __launch_bounds__(256) void k_copy_raw(ArcSide A, const Seg* __restri
     }
     // reversed pieces read src + len - 1 - t: encode as a negative start
     const int64_t srcv = fwd ? src : ~(src + len - 1);
-    const int32_t incl = wave_inclusive_sum(len);
-    const int32_t start = incl - len;
+    // long pieces are copied one at a time by the whole wave (below); the others are packed
+    const bool lng = len >= kLongPiece;
+    const int32_t slen = lng ? 0 : len;
+    const int32_t incl = wave_inclusive_sum(slen);
+    const int32_t start = incl - slen;
     const int32_t total = __shfl(incl, 63, 64);
     for (int32_t base = 0; base < total; base += 64) {
       const int32_t idx = base + lane;
@@ -940,6 +945,24 @@ __global__ __launch_bounds__(256) void k_copy_raw(ArcSide A, const Seg* __restri
       const int64_t d = __shfl(dst, k, 64);
       const int64_t sv = __shfl(srcv, k, 64);
       if (idx < total) out[d + t] = A.arena[sv >= 0 ? sv + t : ~sv - t];
+    }
+    for (uint64_t m = __ballot(lng); m; m &= m - 1) {  // wave-uniform
+      const int l = __builtin_ctzll(m);
+      const int32_t L = __shfl(len, l, 64);
+      const int64_t d = __shfl(dst, l, 64), sv = __shfl(srcv, l, 64);
+      const int32_t* __restrict__ a = A.arena;
+      int32_t t = lane;
+      for (; t + 192 < L; t += 256) {  // four loads in flight per lane
+        const int32_t v0 = a[sv >= 0 ? sv + t : ~sv - t];
+        const int32_t v1 = a[sv >= 0 ? sv + t + 64 : ~sv - t - 64];
+        const int32_t v2 = a[sv >= 0 ? sv + t + 128 : ~sv - t - 128];
+        const int32_t v3 = a[sv >= 0 ? sv + t + 192 : ~sv - t - 192];
+        out[d + t] = v0;
+        out[d + t + 64] = v1;
+        out[d + t + 128] = v2;
+        out[d + t + 192] = v3;
+      }
+      for (; t < L; t += 64) out[d + t] = a[sv >= 0 ? sv + t : ~sv - t];
     }
   }
 }
