@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <atomic>
 #include <climits>
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <functional>
@@ -215,6 +216,22 @@ int round_batch(int done) {
   const Tunables& t = tune();
   int b = t.rounds_first;
   for (int sum = b; sum <= done && b < t.rounds_max; sum += b) b = std::min(2 * b, t.rounds_max);
+  return b;
+}
+// The same, shortened once two readbacks show the pending count shrinking (pend after `done` rounds,
+// prev_pend after prev_done): only the rounds that the last batch's per-round decay needs to bring it
+// down to the tail threshold, after which the single-workgroup tail finishes. Rounds past the end
+// would each still cost two launches.
+int round_batch(int done, int64_t pend, int64_t prev_pend, int prev_done) {
+  const Tunables& t = tune();
+  int b = round_batch(done);
+  if (t.tail_items > 0 && pend > t.tail_items && prev_pend > pend && done > prev_done) {
+    const double q = std::pow((double)pend / (double)prev_pend, 1.0 / (double)(done - prev_done));
+    if (q > 0.0 && q < 1.0) {
+      const double need = std::log((double)t.tail_items / (double)pend) / std::log(q);
+      b = std::max(1, std::min(b, (int)std::ceil(need)));
+    }
+  }
   return b;
 }
 
@@ -578,9 +595,12 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
   bool coop = cooperative_rounds();
   if (nsad > 0) {
     int64_t pend = 2 * nsad;  // pending items of both sides at the last readback (launch sizes)
+    int64_t prev_pend = -1;   // ... and at the readback before, prev_round rounds in
+    int prev_round = 0;
     while (true) {
       if (fork.on && !raw_launched)
         DMSC_CHECK(hipMemcpyAsync(ctx->pinned + kArenaPin, c64, 4 * sizeof(int64_t), hipMemcpyDeviceToHost, st));
+      const int round_at_launch = round;  // pend was read back after this many rounds
       bool tail = false, coop_ran = false;
       {
         KScope sweep_("persist_sweep", st, true);  // the rounds of this batch, back to back on `st`
@@ -591,7 +611,7 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
         if (tail) {
           launch_persist_tail(dual, ps[0], ps[1], round, (int)pend + 2, removed, c32 + 48, st);
         } else if (!coop_ran) {
-          const int batch = round_batch(round);
+          const int batch = round_batch(round, pend, prev_pend, prev_round);
           if (ctx->tune.graph_rounds) {
             if (!pgraph.fits(batch, pend))
               pgraph.capture(ctx, batch, pend, [&](hipStream_t cs) {
@@ -627,6 +647,8 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
       const int32_t* pc = (const int32_t*)ctx->pinned;
       ctx->pending_log.insert(ctx->pending_log.end(), {round, pc[0], pc[1]});
       if (pc[0] == 0 && pc[1] == 0) break;
+      prev_pend = round > 0 ? pend : -1;
+      prev_round = round_at_launch;
       pend = (int64_t)pc[0] + pc[1];
       if (round > 4 * nsad + 64) throw Fail{DMSC_ERR_INTERNAL, "persistence rounds did not converge"};
     }
@@ -781,6 +803,8 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     ctx->rounds_geo[0] = ctx->rounds_geo[1] = 0;
     bool gcoop = cooperative_rounds();
     int ggraph_mask = -1;
+    int64_t gprev = -1;  // pending of the live sides at the readback before the last, gprev_r rounds in
+    int gprev_r = 0;
     while (live[0] || live[1]) {
       const ArcSide as[2] = {aw[0].A, aw[1].A};
       const GeoState gs[2] = {aw[0].G, aw[1].G};
@@ -801,7 +825,9 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
         const int32_t left[2] = {live[0] ? h[2 + (r & 1)] : 0, live[1] ? h[4 + (r & 1)] : 0};
         std::memcpy(ctx->pinned, left, 8);
       } else {
-        const int batch = round_batch(r);
+        const int batch = round_batch(r, gpend, gprev, gprev_r);
+        gprev = gpend;
+        gprev_r = r;
         const int64_t items = pend[0] + pend[1];
         if (ctx->tune.graph_rounds) {
           if (!ggraph.fits(batch, items) || ggraph_mask != mask)
