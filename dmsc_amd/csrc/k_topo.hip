@@ -599,7 +599,7 @@ __global__ void k_root_out(int32_t* __restrict__ par, const uint8_t* __restrict_
 }
 void launch_root_out(int32_t* par, const uint8_t* alive, int32_t* to_out, int32_t* root_out, int64_t n,
                      hipStream_t st) {
-  KScope ks_("uf_flatten", st);
+  KScope ks_("root_out", st);
   if (n == 0) return;
   hipLaunchKernelGGL(k_root_out, dim3(grid_for(n, 256, 16384)), dim3(256), 0, st, par, alive, to_out, root_out, n);
   DMSC_LAUNCH_CHECK();
