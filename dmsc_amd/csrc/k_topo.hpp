@@ -64,6 +64,9 @@ void launch_root_out(int32_t* par, const uint8_t* alive, int32_t* to_out, int32_
 // per position: survival, edge rows and their extremum outputs (edges: int2 per position)
 void launch_side_counts(int64_t ns, const int32_t* ord, const uint8_t* removed, const int32_t* c,
                         const int32_t* root_out, int32_t* surv, int32_t* rows, int2* edges, hipStream_t st);
+// Survivors of one order as lists by survivor rank (saddle, first edge row, edge outputs); for the max
+// order also out_list[r] = output index, read from the min order's survivor prefix min_pre at the
+// mirrored position (sad_b: [B+1] field bases of the orders). min_pre, out_list: null for the min order.
 void launch_survivors(int64_t ns, const int32_t* ord, const int32_t* surv, const int32_t* surv_pre,
                       const int32_t* row_pre, const int2* edges, int32_t* list, int32_t* row_list, int2* edge_list,
                       const int32_t* min_pre, const int32_t* sad_b, int B, int32_t* out_list, hipStream_t st);
