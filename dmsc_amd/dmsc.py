@@ -16,7 +16,7 @@ try:
 except ImportError as e:  # fail loudly: the product path has no CPU fallback
     raise ImportError(
         "dmsc_amd: the HIP extension is not built (run `python -c \"import __graft_entry__ as g; g.build()\"` "
-        f"or `python -m dmsc_amd._build`): {e}"
+        f"or `python dmsc_amd/_build.py`): {e}"
     ) from e
 
 
