@@ -13,6 +13,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <functional>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -64,6 +65,9 @@ thread_local const Tunables* tl_tune = nullptr;
 using namespace dmsc;
 
 struct dmsc_context {
+  // Every entry point holds this lock for its whole host-side run: calls on one context from several
+  // threads take turns (the arena, pinned buffer, side stream and events are shared state).
+  std::mutex mu;
   int device = 0;
   KernelTimes times;
   Tunables tune;
@@ -74,7 +78,7 @@ struct dmsc_context {
   Arena ar2;  // arc emission scratch
   int64_t* pinned = nullptr;
   size_t pinned_cap = 0;
-  std::string err;
+  std::map<std::thread::id, std::string> err;  // message of each thread's last failing call
   // last-call record for dmsc_debug_fetch
   int64_t S = 0, N = 0;
   int64_t nmin = 0, nsad = 0, nmax = 0;
@@ -167,8 +171,6 @@ struct ArcWork {
   int32_t* out_arc_off = nullptr;
   std::vector<int64_t> h_arc_base;
 };
-
-constexpr int kCoopRounds = 1 << 12;  // rounds per cooperative launch before the host checks in
 
 // A batch of rounds recorded once and replayed on the work stream; its kernels read the round number
 // from device memory (set before each replay), so one graph serves every batch of the same size
@@ -277,14 +279,6 @@ struct StreamFork {
       (void)hipStreamSynchronize(ctx->side);
   }
 };
-
-// Round loops: one launch per phase with a host readback every few rounds. DMSC_ROUNDS=coop selects
-// one cooperative launch per loop instead; that path is off by default because a large-grid run
-// of it (about 2000 workgroups) stopped making progress and the cause has not been established.
-bool cooperative_rounds() {
-  const char* e = std::getenv("DMSC_ROUNDS");
-  return e && std::strcmp(e, "coop") == 0;
-}
 
 
 // One field of a call, with the caller's index (the allocator's `field` argument).
@@ -592,7 +586,6 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
   ctx->pending_log.clear();
   int32_t* d_round = c32 + 50;  // round number for graph replays
   CapturedRounds pgraph, ggraph;
-  bool coop = cooperative_rounds();
   if (nsad > 0) {
     int64_t pend = 2 * nsad;  // pending items of both sides at the last readback (launch sizes)
     int64_t prev_pend = -1;   // ... and at the readback before, prev_round rounds in
@@ -601,16 +594,14 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
       if (fork.on && !raw_launched)
         DMSC_CHECK(hipMemcpyAsync(ctx->pinned + kArenaPin, c64, 4 * sizeof(int64_t), hipMemcpyDeviceToHost, st));
       const int round_at_launch = round;  // pend was read back after this many rounds
-      bool tail = false, coop_ran = false;
+      bool tail = false;
       {
         KScope sweep_("persist_sweep", st, true);  // the rounds of this batch, back to back on `st`
-        coop_ran = coop && launch_persist_rounds(dual, ps[0], ps[1], round, kCoopRounds, removed, c32 + 48, pend, st);
-        coop = coop_ran;  // no cooperative launch on this device or at this size: one launch per phase
         // few pending saddles: one workgroup runs every remaining round (at most one round per item)
-        tail = !coop_ran && ctx->tune.tail_items > 0 && pend <= ctx->tune.tail_items;
+        tail = ctx->tune.tail_items > 0 && pend <= ctx->tune.tail_items;
         if (tail) {
           launch_persist_tail(dual, ps[0], ps[1], round, (int)pend + 2, removed, c32 + 48, st);
-        } else if (!coop_ran) {
+        } else {
           const int batch = round_batch(round, pend, prev_pend, prev_round);
           if (ctx->tune.graph_rounds) {
             if (!pgraph.fits(batch, pend))
@@ -629,7 +620,7 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
           }
         }
       }
-      if (coop_ran || tail) {
+      if (tail) {
         DMSC_CHECK(hipMemcpyAsync(ctx->pinned, c32 + 48, 4, hipMemcpyDeviceToHost, st));
         DMSC_CHECK(hipMemcpyAsync((char*)ctx->pinned + 8, sb[0].cnt, 8, hipMemcpyDeviceToHost, st));
         DMSC_CHECK(hipMemcpyAsync((char*)ctx->pinned + 16, sb[1].cnt, 8, hipMemcpyDeviceToHost, st));
@@ -801,7 +792,6 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     int64_t pend[2] = {aw[0].ncancel, aw[1].ncancel};  // pending at the last readback (launch sizes)
     int r = 0;
     ctx->rounds_geo[0] = ctx->rounds_geo[1] = 0;
-    bool gcoop = cooperative_rounds();
     int ggraph_mask = -1;
     int64_t gprev = -1;  // pending of the live sides at the readback before the last, gprev_r rounds in
     int gprev_r = 0;
@@ -809,12 +799,10 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
       const ArcSide as[2] = {aw[0].A, aw[1].A};
       const GeoState gs[2] = {aw[0].G, aw[1].G};
       const int mask = (live[0] ? 1 : 0) | (live[1] ? 2 : 0);
-      if (gcoop && !launch_geo_rounds(as, gs, mask, r, kCoopRounds, c32 + 49, std::max(pend[0], pend[1]), st))
-        gcoop = false;
       const int64_t gpend = (live[0] ? pend[0] : 0) + (live[1] ? pend[1] : 0);
-      const bool gtail = !gcoop && ctx->tune.tail_items > 0 && gpend <= ctx->tune.tail_items;
-      if (gtail) launch_geo_tail(as, gs, mask, r, (int)gpend + 64, c32 + 49, st);
-      if (gcoop || gtail) {
+      const bool gtail = ctx->tune.tail_items > 0 && gpend <= ctx->tune.tail_items;
+      if (gtail) {
+        launch_geo_tail(as, gs, mask, r, (int)gpend + 64, c32 + 49, st);
         DMSC_CHECK(hipMemcpyAsync(ctx->pinned, c32 + 49, 4, hipMemcpyDeviceToHost, st));
         for (int s = 0; s < 2; ++s)
           if (live[s])
@@ -1045,25 +1033,33 @@ void run_fields(dmsc_context* ctx, const std::vector<FieldIn>& fields, uint32_t 
   }
 }
 
+// Runs one entry point under the context lock; exceptions become a status plus the calling
+// thread's error message. The HIP device of the calling thread is restored afterwards.
 dmsc_status guarded(dmsc_context* ctx, const std::function<void()>& fn) {
   if (!ctx) return DMSC_ERR_INVALID_ARGUMENT;
-  ctx->err.clear();
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  std::string& err = ctx->err[std::this_thread::get_id()];
+  err.clear();
   tl_times = &ctx->times;
   tl_tune = &ctx->tune;
+  int prev_dev = -1;
+  (void)hipGetDevice(&prev_dev);
+  dmsc_status st = DMSC_OK;
   try {
     DMSC_CHECK(hipSetDevice(ctx->device));
     fn();
-    return DMSC_OK;
   } catch (const Fail& e) {
-    ctx->err = e.msg;
-    return e.st;
+    err = e.msg;
+    st = e.st;
   } catch (const HipError& e) {
-    ctx->err = std::string(e.what) + ": " + hipGetErrorString(e.err);
-    return DMSC_ERR_HIP;
+    err = std::string(e.what) + ": " + hipGetErrorString(e.err);
+    st = DMSC_ERR_HIP;
   } catch (const std::exception& e) {
-    ctx->err = e.what();
-    return DMSC_ERR_INTERNAL;
+    err = e.what();
+    st = DMSC_ERR_INTERNAL;
   }
+  if (prev_dev >= 0 && prev_dev != ctx->device) (void)hipSetDevice(prev_dev);
+  return st;
 }
 }  // namespace
 
@@ -1165,6 +1161,7 @@ int* param_slot(dmsc_context* ctx, const std::string& n, int64_t& lo, int64_t& h
 
 dmsc_status dmsc_set_param(dmsc_context* ctx, const char* name, int64_t value) {
   if (!ctx || !name) return DMSC_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lock(ctx->mu);
   int64_t lo = 0, hi = 0;
   int* p = param_slot(ctx, name, lo, hi);
   if (!p || value < lo || value > hi) return DMSC_ERR_INVALID_ARGUMENT;
@@ -1174,6 +1171,7 @@ dmsc_status dmsc_set_param(dmsc_context* ctx, const char* name, int64_t value) {
 
 dmsc_status dmsc_get_param(dmsc_context* ctx, const char* name, int64_t* value) {
   if (!ctx || !name || !value) return DMSC_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lock(ctx->mu);
   int64_t lo = 0, hi = 0;
   int* p = param_slot(ctx, name, lo, hi);
   if (!p) return DMSC_ERR_INVALID_ARGUMENT;
@@ -1181,18 +1179,29 @@ dmsc_status dmsc_get_param(dmsc_context* ctx, const char* name, int64_t* value) 
   return DMSC_OK;
 }
 
-const char* dmsc_last_error(dmsc_context* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+const char* dmsc_last_error(dmsc_context* ctx) {
+  if (!ctx) return "null context";
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  return ctx->err[std::this_thread::get_id()].c_str();  // the entry stays until this thread's next call
+}
 
 dmsc_status dmsc_profile_enable(dmsc_context* ctx, int32_t on) {
   if (!ctx) return DMSC_ERR_INVALID_ARGUMENT;
   if (on < 0 || on > 2) return DMSC_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lock(ctx->mu);
   ctx->times.level = on;
   ctx->times.acc.clear();
   return DMSC_OK;
 }
-int32_t dmsc_profile_count(dmsc_context* ctx) { return ctx ? (int32_t)ctx->times.acc.size() : 0; }
+int32_t dmsc_profile_count(dmsc_context* ctx) {
+  if (!ctx) return 0;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  return (int32_t)ctx->times.acc.size();
+}
 dmsc_status dmsc_profile_entry(dmsc_context* ctx, int32_t i, const char** name, double* total_ms, int64_t* launches) {
-  if (!ctx || i < 0 || i >= (int32_t)ctx->times.acc.size()) return DMSC_ERR_INVALID_ARGUMENT;
+  if (!ctx) return DMSC_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  if (i < 0 || i >= (int32_t)ctx->times.acc.size()) return DMSC_ERR_INVALID_ARGUMENT;
   const auto& kv = ctx->times.acc[i];
   if (name) *name = kv.first.c_str();
   if (total_ms) *total_ms = kv.second.first;
@@ -1202,6 +1211,8 @@ dmsc_status dmsc_profile_entry(dmsc_context* ctx, int32_t i, const char** name, 
 
 int64_t dmsc_debug_fetch(dmsc_context* ctx, const char* name, void* dst, int64_t cap) {
   if (!ctx || !name) return -1;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  if (hipSetDevice(ctx->device) != hipSuccess) return -1;
   std::string n(name);
   auto copy = [&](const void* src, int64_t count, size_t elem) -> int64_t {
     int64_t m = std::min(count, cap);

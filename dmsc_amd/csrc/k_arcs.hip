@@ -17,7 +17,6 @@
 //      made out of order never sit below an undecided earlier cancellation).
 // Under (A) and (B) every decision and link path equals the sequential one; the smallest pending
 // cancellation always satisfies both.
-#include <hip/hip_cooperative_groups.h>
 
 #include "k_topo.hpp"
 
@@ -351,7 +350,7 @@ __device__ __forceinline__ bool geo_apply_one(const ArcSide& A, const GeoState& 
 }
 
 // Small worklists (the tail of the rounds) are latency-bound: one item per thread there.
-constexpr int kGeoItems = 4, kGeoChunk = 256 * kGeoItems, kGeoSmall = 1 << 16;  // kGeoSmall: cooperative path
+constexpr int kGeoItems = 4, kGeoChunk = 256 * kGeoItems, kGeoSmall = 1 << 16;
 
 template <int ITEMS>
 __device__ __forceinline__ void geo_apply_chunks(const ArcSide& A, const GeoState& g, int round, int cur, int n,
@@ -528,53 +527,6 @@ void launch_geo_tail(const ArcSide* a, const GeoState* g, int live, int round0, 
   hipLaunchKernelGGL(k_geo_tail, dim3(1), dim3(kTailThreads), 0, st, a[0], g[0], a[1], g[1], live, round0, max_rounds,
                      round_out);
   DMSC_LAUNCH_CHECK();
-}
-
-// Both traced sides, every round, in one cooperative launch (same structure and exit rule as the
-// persistence rounds kernel): find, grid barrier, apply, grid barrier, until both drain.
-__global__ __launch_bounds__(256) void k_geo_rounds(ArcSide A0, GeoState G0, ArcSide A1, GeoState G1, int live,
-                                                    int round0, int max_rounds, int32_t* __restrict__ round_out) {
-  cooperative_groups::grid_group grid = cooperative_groups::this_grid();
-  int r = round0;
-  while (r < round0 + max_rounds) {
-    if (live & 1) geo_find_phase(A0, G0, r);
-    if (live & 2) geo_find_phase(A1, G1, r);
-    grid.sync();
-    if (live & 1) geo_apply_phase(A0, G0, r);
-    if (live & 2) geo_apply_phase(A1, G1, r);
-    grid.sync();
-    ++r;
-    const int p0 = (live & 1) ? __hip_atomic_load(&G0.cnt[r & 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
-    const int p1 = (live & 2) ? __hip_atomic_load(&G1.cnt[r & 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
-    if (p0 == 0 && p1 == 0) break;
-  }
-  if (blockIdx.x == 0 && threadIdx.x == 0) *round_out = r;
-}
-
-bool launch_geo_rounds(const ArcSide* a, const GeoState* g, int live, int round0, int max_rounds, int32_t* round_out,
-                       int64_t max_items, hipStream_t st) {
-  KScope ks_("geo_round", st);
-  static const int maxb = [] {
-    int dev = 0, per_cu = 0;
-    hipDeviceProp_t prop;
-    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_geo_rounds), 256, 0) !=
-            hipSuccess ||
-        !prop.cooperativeLaunch)
-      return 0;
-    return per_cu * prop.multiProcessorCount;
-  }();
-  if (maxb <= 0) return false;
-  const int blocks = std::min<int64_t>(maxb, std::max<int64_t>(1, grid_for(max_items, 256, 1 << 20)));
-  ArcSide a0 = a[0], a1 = a[1];
-  GeoState g0 = g[0], g1 = g[1];
-  void* args[] = {&a0, &g0, &a1, &g1, &live, &round0, &max_rounds, &round_out};
-  if (hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_geo_rounds), dim3(blocks), dim3(256), args, 0, st) !=
-      hipSuccess) {
-    (void)hipGetLastError();
-    return false;
-  }
-  return true;
 }
 
 void launch_geo_round(const ArcSide& a, const GeoState& g, int round, int64_t max_items, hipStream_t st) {

@@ -15,7 +15,6 @@
 // on its other side (a merge it depends on is still pending) and so misjudge which root it kills;
 // it must still block every later saddle that touches either of its current roots.
 // The smallest pending saddle always holds its dying root, so every round makes progress.
-#include <hip/hip_cooperative_groups.h>
 
 #include "internal.hpp"
 #include "k_topo.hpp"
@@ -507,56 +506,6 @@ void launch_persist_tail(bool dual, const PersistSide& s0, const PersistSide& s1
     hipLaunchKernelGGL(k_persist_tail<false>, dim3(1), dim3(kTailThreads), 0, st, s0, s1, round0, max_rounds, removed,
                        round_out);
   DMSC_LAUNCH_CHECK();
-}
-
-// All rounds in one cooperative launch: find, grid barrier, apply, grid barrier, until both
-// worklists are empty or max_rounds have run. Every block takes the same path: the phase bounds
-// and the exit test read counters that no block writes between the barrier and the read.
-template <bool DUAL>
-__global__ __launch_bounds__(256) void k_persist_rounds(PersistSide s0, PersistSide s1, int round0, int max_rounds,
-                                                        uint8_t* __restrict__ removed, int32_t* __restrict__ round_out) {
-  cooperative_groups::grid_group grid = cooperative_groups::this_grid();
-  int r = round0;
-  while (r < round0 + max_rounds) {
-    persist_find_phase<DUAL>(s0, s1, r);
-    grid.sync();
-    persist_apply_phase<DUAL>(s0, s1, r, removed, 1 << 16);
-    grid.sync();
-    ++r;
-    const int p0 = __hip_atomic_load(&s0.cnt[r & 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const int p1 = __hip_atomic_load(&s1.cnt[r & 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (p0 == 0 && p1 == 0) break;
-  }
-  if (blockIdx.x == 0 && threadIdx.x == 0) *round_out = r;
-}
-
-// Blocks a cooperative launch of `kernel` may use on the current device (all co-resident).
-static int coop_blocks(const void* kernel) {
-  int dev = 0, per_cu = 0;
-  hipDeviceProp_t prop;
-  if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess ||
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0) != hipSuccess || !prop.cooperativeLaunch)
-    return 0;
-  return per_cu * prop.multiProcessorCount;
-}
-
-bool launch_persist_rounds(bool dual, const PersistSide& s0, const PersistSide& s1, int round0, int max_rounds,
-                           uint8_t* removed, int32_t* round_out, int64_t max_items, hipStream_t st) {
-  KScope ks_("persist_round", st);
-  static const int cap[2] = {coop_blocks(reinterpret_cast<const void*>(&k_persist_rounds<false>)),
-                             coop_blocks(reinterpret_cast<const void*>(&k_persist_rounds<true>))};
-  const int maxb = cap[dual ? 1 : 0];
-  if (maxb <= 0) return false;
-  const int blocks = std::min<int64_t>(maxb, std::max<int64_t>(1, grid_for(max_items, 256, 1 << 20)));
-  PersistSide a = s0, b = s1;
-  void* args[] = {&a, &b, &round0, &max_rounds, &removed, &round_out};
-  const void* fn = dual ? reinterpret_cast<const void*>(&k_persist_rounds<true>)
-                        : reinterpret_cast<const void*>(&k_persist_rounds<false>);
-  if (hipLaunchCooperativeKernel(fn, dim3(blocks), dim3(256), args, 0, st) != hipSuccess) {
-    (void)hipGetLastError();
-    return false;
-  }
-  return true;
 }
 
 __global__ void k_round_inc(int32_t* d_round) { *d_round += 1; }

@@ -47,15 +47,10 @@ void launch_persist_init(const PersistSide& s0, const PersistSide& s1, int64_t n
 void launch_persist_round(bool dual, const PersistSide& s0, const PersistSide& s1, int round, uint8_t* removed,
                           int64_t max_items, hipStream_t st, const int32_t* d_round = nullptr);
 void launch_round_inc(int32_t* d_round, hipStream_t st);
-// Every round up to max_rounds (or until both worklists drain) in one cooperative launch; the
-// round reached goes to *round_out. Returns false, having launched nothing, when the device or
-// the launch size does not allow a cooperative launch.
 // Every remaining round in one single-workgroup launch (for short worklists); pending counts are
 // left in cnt[r & 1] of each side and the round reached in *round_out (device).
 void launch_persist_tail(bool dual, const PersistSide& s0, const PersistSide& s1, int round0, int max_rounds,
                          uint8_t* removed, int32_t* round_out, hipStream_t st);
-bool launch_persist_rounds(bool dual, const PersistSide& s0, const PersistSide& s1, int round0, int max_rounds,
-                           uint8_t* removed, int32_t* round_out, int64_t max_items, hipStream_t st);
 
 // to_out: exclusive prefix of the alive flags, masked to -1 for dead extrema by this pass;
 // root_out[x] <- output index of the thresholded root of extremum x (its root in par), -1 when none
@@ -143,13 +138,9 @@ void launch_geo_round(const ArcSide& a, const GeoState& g, int round, int64_t ma
 // one round of both traced sides (bit s of `live`) in one find and one apply launch
 void launch_geo_round2(const ArcSide* a, const GeoState* g, int live, int round, int64_t max_items, hipStream_t st,
                        const int32_t* d_round = nullptr);
-// All geometry rounds of the traced sides (bit s of `live`) in one cooperative launch; see
-// launch_persist_rounds.
 // Geometry counterpart of launch_persist_tail (both traced sides, `live` mask).
 void launch_geo_tail(const ArcSide* a, const GeoState* g, int live, int round0, int max_rounds, int32_t* round_out,
                      hipStream_t st);
-bool launch_geo_rounds(const ArcSide* a, const GeoState* g, int live, int round0, int max_rounds, int32_t* round_out,
-                       int64_t max_items, hipStream_t st);
 void launch_cancel_flags(int64_t ns, const int32_t* cancel_d, int32_t* flag, hipStream_t st);
 void launch_cancel_compact(int64_t ns, const int32_t* cancel_d, const float* pers, const int32_t* pre,
                            int32_t* cs, int32_t* cd, uint32_t* keys, hipStream_t st);

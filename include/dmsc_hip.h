@@ -65,6 +65,9 @@ typedef struct {
   int32_t shape[2]; /* slot 0: {H, W} */
 } dmsc_outputs;
 
+/* Threading: every entry point locks its context for its whole host-side run, so calls on one
+ * context from several threads take turns and never share scratch. Error messages are kept per
+ * calling thread (dmsc_last_error returns the calling thread's). */
 typedef struct dmsc_context dmsc_context;
 
 /* Creates a context bound to HIP device `device`; it owns reusable device scratch (grow-only).
@@ -75,8 +78,7 @@ void dmsc_context_destroy(dmsc_context* ctx);
 /* Full pipeline on one contiguous H x W float32 field resident on the device.
  * Replaces extract_single_dmsc_gpu_t (reference dmsc_gpu.cpp:53-82) with the whole pipeline on
  * the GPU (gradient, critical cells, V-path forests, persistence, arc geometry, labels, assembly).
- * Synchronous: the outputs are complete when it returns (as in the reference). Calls on one
- * context must not overlap. */
+ * Synchronous: the outputs are complete when it returns (as in the reference). */
 dmsc_status dmsc_extract(dmsc_context* ctx, const float* d_field, int32_t H, int32_t W, float persistence_threshold,
                          uint32_t flags, dmsc_stream_t stream, dmsc_alloc_fn alloc, void* alloc_user,
                          dmsc_outputs* out);
@@ -141,7 +143,8 @@ dmsc_status dmsc_profile_entry(dmsc_context* ctx, int32_t index, const char** na
 dmsc_status dmsc_set_param(dmsc_context* ctx, const char* name, int64_t value);
 dmsc_status dmsc_get_param(dmsc_context* ctx, const char* name, int64_t* value);
 
-/* Message for the last failing call on this context (valid until the next call). */
+/* Message of the calling thread's last failing call on this context (valid until that thread's
+ * next call on it). */
 const char* dmsc_last_error(dmsc_context* ctx);
 
 /* Copies an internal intermediate of the last dmsc_extract call into host memory, for stage-by-
