@@ -16,7 +16,10 @@ def generate_noisy_landscape(H=20, W=20, with_loop=False, z_scale=1.0, noise_sca
     z = torch.exp(-((x - 1) ** 2 + y**2) / 0.5) + torch.exp(-((x + 1) ** 2 + y**2) / 0.5)
     z = z - 0.5 * torch.exp(-(x**2 + (y - 1) ** 2) / 0.5) - 1.0 * torch.exp(-(x**2 + (y + 1) ** 2) / 0.5)
     if with_loop:
-        z = z + torch.exp(-((x**2 + (y + 1) ** 2) ** 3) / 0.5) - y / 4
+        # added as one term, (peak + slope), in the reference's order: float addition does not associate
+        peak3 = torch.exp(-((x**2 + (y + 1) ** 2) ** 3) / 0.5)
+        slope = -y / 4
+        z = z + (peak3 + slope)
     noise = torch.rand(H, W) - 0.5
     return (z * z_scale + noise * noise_scale).to(torch.float32)
 
