@@ -8,7 +8,9 @@ product package ``dmsc_amd`` never does.
 
 Parity pins: the reference's 13 fixtures (tests/golden), exhaustive lower-star enumeration, an
 independent Kruskal/elder-rule persistence, brute-force V-path labels and invariants
-(tests/test_oracle.py).
+(tests/test_oracle.py); rank invariance of the gradient on tied fields, the arc layout (starts,
+adjacency, offsets) and an independent list-based arc merging with thresholded cancellations
+(tests/test_oracle_arcs.py, tests/arc_reference.py).
 """
 
 import ctypes
