@@ -66,3 +66,29 @@ def mixed_batch(n_fields, seed, sizes=tuple(range(64, 1025, 32)), bumps=(8, 64),
         fields.append(gaussian_mixture(H, W, k, (m / 32.0, m / 8.0), noise, sub_seed))
         thrs.append(thr)
     return fields, thrs
+
+
+def break_ties(field):
+    """A tie-free float32 field with the same (value, pixel id) order: visiting pixels in that order,
+    every value is raised to at least the next float32 above its predecessor. Values move by a few
+    ulps at most where they were tied, and the order the complex is built from is unchanged, so the
+    discrete gradient and every pairing equal those of the input."""
+    import numpy as np
+
+    f = np.ascontiguousarray(torch.as_tensor(field, dtype=torch.float32).numpy()).reshape(-1)
+    order = np.lexsort((np.arange(f.size), f))
+    v = f[order].copy()
+    bump = np.flatnonzero(v[1:] <= v[:-1]) + 1
+    while bump.size:  # runs of equal values need one pass per member
+        v[bump] = np.nextafter(v[bump - 1], np.float32(np.inf))
+        bump = np.flatnonzero(v[1:] <= v[:-1]) + 1
+    out = np.empty_like(f)
+    out[order] = v
+    return torch.from_numpy(out.reshape(tuple(torch.as_tensor(field).shape)))
+
+
+def loss_batch(n_fields=64, H=512, W=512, seed=5000):
+    """Fields of the persistence-loss workload: Gaussian mixtures (24 bumps, sigma 8-32 px, noise
+    0.05, seeds seed..seed+n-1) made tie-free by break_ties."""
+    return torch.stack([break_ties(gaussian_mixture(H, W, 24, (8.0, 32.0), 0.05, seed=seed + b))
+                        for b in range(n_fields)])

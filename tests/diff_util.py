@@ -48,23 +48,43 @@ def oracle_pairs(field, is_dual):
     return [np.asarray(out[k]) for k in PAIR_SLOTS]
 
 
+def value_pixels(field, cells, is_dual):
+    """value_pixel of many cells at once: the highest vertex by (value, pixel id) of each cell, the
+    lowest in the dual (the reference's cell value rule, cpu/cell_compare.hxx:33-87)."""
+    h, w = field.shape
+    ff = field.reshape(-1)
+    cells = np.asarray(cells, dtype=np.int64)
+    t, q = cells & 3, cells >> 2
+    y, x = q // (w + 1), q % (w + 1)
+    best = np.full(cells.shape, -1, dtype=np.int64)
+    for dy, dx, ok in ((0, 0, t == 0), (0, -1, t == 1), (0, 0, t == 1), (-1, 0, t == 2), (0, 0, t == 2),
+                       (-1, -1, t == 3), (-1, 0, t == 3), (0, -1, t == 3), (0, 0, t == 3)):
+        yy, xx = y + dy, x + dx
+        v = ok & (yy >= 0) & (yy < h) & (xx >= 0) & (xx < w)
+        cand = np.where(v, yy * w + xx, 0)
+        b = np.maximum(best, 0)
+        if is_dual:
+            better = (ff[cand] < ff[b]) | ((ff[cand] == ff[b]) & (cand < best))
+        else:
+            better = (ff[cand] > ff[b]) | ((ff[cand] == ff[b]) & (cand > best))
+        best = np.where(v & ((best < 0) | better), cand, best)
+    return best
+
+
 def loss_and_grad(field, max_pers, is_dual):
-    """(loss, d loss / d field) in float64 from the oracle's pairs."""
+    """(loss, d loss / d field) in float64 from the oracle's pairs: a min-side pair adds +2 pers at
+    the saddle's value pixel and -2 pers at the minimum's, a max-side pair +2 pers at the maximum's
+    and -2 pers at the saddle's."""
     sad, pmin, ppmin, mins, pmax, ppmax, maxs = oracle_pairs(field, is_dual)
     g = np.zeros(field.size)
+    sp = value_pixels(field, sad, is_dual)
     loss = 0.0
-    for k in range(sad.size):
-        sp = value_pixel(field, int(sad[k]), is_dual)
-        if ppmin[k] >= 0 and 0 <= pmin[k] < max_pers:
-            p = float(pmin[k])
-            loss += p * p
-            g[sp] += 2 * p
-            g[value_pixel(field, int(mins[ppmin[k]]), is_dual)] -= 2 * p
-        if ppmax[k] >= 0 and 0 <= pmax[k] < max_pers:
-            p = float(pmax[k])
-            loss += p * p
-            g[value_pixel(field, int(maxs[ppmax[k]]), is_dual)] += 2 * p
-            g[sp] -= 2 * p
+    for p, pp, ext, sgn in ((pmin, ppmin, mins, 1.0), (pmax, ppmax, maxs, -1.0)):
+        m = (pp >= 0) & (p >= 0) & (p < max_pers)
+        pv = p[m].astype(np.float64)
+        loss += float(np.sum(pv * pv))
+        np.add.at(g, sp[m], sgn * 2 * pv)
+        np.add.at(g, value_pixels(field, ext[pp[m]], is_dual), -sgn * 2 * pv)
     return loss, g.reshape(field.shape)
 
 

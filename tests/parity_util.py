@@ -48,3 +48,19 @@ def diff_outputs(got, exp):
             bad.append((name, f"{len(idx)} of {e.size} differ; first at {first}: "
                               f"{g.reshape(-1)[first] if first >= 0 else None} vs {e.reshape(-1)[first] if first >= 0 else None}"))
     return bad
+
+
+def pool_workers():
+    """Worker threads for oracle runs: the host's CPUs, at most 16 (one GPU's CPU share)."""
+    import os
+
+    return max(1, min(os.cpu_count() or 1, 16))
+
+
+def oracle_many(cases, **kw):
+    """Oracle outputs of many (field, thr) cases, one single-threaded oracle call per worker thread
+    (the oracle releases the GIL and keeps no global state), yielded in case order."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    with ThreadPoolExecutor(pool_workers()) as ex:
+        yield from ex.map(lambda c: oracle_extract(c[0], c[1], nthreads=1, **kw), cases)

@@ -1,10 +1,11 @@
 """Parity at the benchmark sizes: the HIP path against the oracle, all 19 outputs exact, on the exact
 inputs the benchmark times (same generators and seeds) -- the 256^2 Gaussian mixture at thr 0.1, the
-4096^2 Gaussian mixture primal at thr 0.05 and dual at thr 0.2, and the 8192^2 quantised noise at thr
-0.05 -- plus size-independent invariants of the outputs (Euler characteristic of the simplified
-complex, CSR offsets, survivors' persistence above the threshold, edge rows in range). The oracle
-runs on all host cores here; its output is independent of the thread count."""
+4096^2 Gaussian mixture primal and dual at every threshold of {0, 0.05, 0.2}, and the 8192^2
+quantised noise at thr 0.05 -- plus size-independent invariants of the outputs (Euler characteristic
+of the simplified complex, CSR offsets, survivors' persistence above the threshold, edge rows in
+range). The oracle runs on all host cores here; its output is independent of the thread count."""
 
+import functools
 import os
 
 import numpy as np
@@ -49,10 +50,15 @@ def test_gaussian_256_thr_0_1():
     run_case(f, 0.1, False)
 
 
-@pytest.mark.parametrize("is_dual,thr", [(False, 0.05), (True, 0.2)])
+@functools.lru_cache(maxsize=1)
+def gauss4096():
+    return synthetic.gaussian_mixture(4096, 4096, 256, (16.0, 256.0), 0.02, seed=4096).numpy()
+
+
+@pytest.mark.parametrize("is_dual", [False, True])
+@pytest.mark.parametrize("thr", [0.0, 0.05, 0.2])
 def test_gaussian_4096(is_dual, thr):
-    f = synthetic.gaussian_mixture(4096, 4096, 256, (16.0, 256.0), 0.02, seed=4096).numpy()
-    run_case(f, thr, is_dual)
+    run_case(gauss4096(), thr, is_dual)
 
 
 def test_noise_8192():
