@@ -205,37 +205,12 @@ std::vector<DMSComplex> extract_dmsc_list(std::vector<torch::Tensor> fields, std
   return run_batch(ctx, dev, ptrs, hs, ws, thr, flags, stream);
 }
 
-// Backward of the persistence loss for a list of fields; pairs[b] = (sad_pts, p_min, ppairs_min,
-// min_pts, p_max, ppairs_max, max_pts) of field b's unsimplified complex.
-std::vector<torch::Tensor> persistence_backward(std::vector<torch::Tensor> fields,
-                                                std::vector<std::vector<torch::Tensor>> pairs,
-                                                torch::Tensor grad_scale, double max_persistence, bool is_dual) {
-  const int64_t B = (int64_t)fields.size();
-  TORCH_CHECK(B > 0 && (int64_t)pairs.size() == B, "one set of pairs per field is required");
-  const auto dev = fields[0].device();
-  TORCH_CHECK(dev.is_cuda(), "persistence_backward requires GPU tensors");
-  TORCH_CHECK(grad_scale.device() == dev && grad_scale.scalar_type() == torch::kFloat32 && grad_scale.numel() == B,
-              "grad_scale must be a float32 tensor with one entry per field on the fields' device");
-  grad_scale = grad_scale.contiguous();
-  c10::hip::HIPGuard guard(dev.index());
-  dmsc_context* ctx = context_for(dev.index());
-  hipStream_t stream = c10::hip::getCurrentHIPStream(dev.index()).stream();
-  std::vector<torch::Tensor> keep, grads;
-  std::vector<const float*> ptrs;
-  std::vector<float*> gptrs;
-  std::vector<int32_t> hs, ws;
-  std::vector<dmsc_pairs> pv(B);
-  for (int64_t b = 0; b < B; ++b) {
-    torch::Tensor f = fields[b];
-    validate(f);
-    TORCH_CHECK(f.dim() == 2 && f.device() == dev, "each field must be a 2D tensor on the same device");
-    f = f.contiguous();
-    keep.push_back(f);
-    ptrs.push_back(f.data_ptr<float>());
-    hs.push_back((int32_t)f.size(0));
-    ws.push_back((int32_t)f.size(1));
-    grads.push_back(torch::empty_like(f));
-    gptrs.push_back(grads.back().data_ptr<float>());
+// The C-ABI view of per-field pair tensors (sad_pts, p_min, ppairs_min, min_pts, p_max, ppairs_max,
+// max_pts) of an unsimplified complex; contiguous copies are kept alive in `keep`.
+std::vector<dmsc_pairs> pair_views(std::vector<std::vector<torch::Tensor>>& pairs, torch::Device dev,
+                                   std::vector<torch::Tensor>& keep) {
+  std::vector<dmsc_pairs> pv(pairs.size());
+  for (size_t b = 0; b < pairs.size(); ++b) {
     auto& p = pairs[b];
     TORCH_CHECK(p.size() == 7, "pairs must hold (sad_pts, p_min, ppairs_min, min_pts, p_max, ppairs_max, max_pts)");
     for (int k = 0; k < 7; ++k) {
@@ -259,11 +234,69 @@ std::vector<torch::Tensor> persistence_backward(std::vector<torch::Tensor> field
                        ns ? p[5].data_ptr<int32_t>() : nullptr,
                        p[6].numel() ? p[6].data_ptr<int32_t>() : nullptr};
   }
-  const dmsc_status st = dmsc_persistence_backward(ctx, (int32_t)B, ptrs.data(), hs.data(), ws.data(),
-                                                   is_dual ? DMSC_DUAL : 0u, pv.data(), (float)max_persistence,
-                                                   grad_scale.data_ptr<float>(), gptrs.data(), (dmsc_stream_t)stream);
+  return pv;
+}
+
+// Backward of the persistence loss for a list of fields; pairs[b] = the pair tensors of field b's
+// unsimplified complex.
+std::vector<torch::Tensor> persistence_backward(std::vector<torch::Tensor> fields,
+                                                std::vector<std::vector<torch::Tensor>> pairs,
+                                                torch::Tensor grad_scale, double max_persistence, bool is_dual) {
+  const int64_t B = (int64_t)fields.size();
+  TORCH_CHECK(B > 0 && (int64_t)pairs.size() == B, "one set of pairs per field is required");
+  const auto dev = fields[0].device();
+  TORCH_CHECK(dev.is_cuda(), "persistence_backward requires GPU tensors");
+  TORCH_CHECK(grad_scale.device() == dev && grad_scale.scalar_type() == torch::kFloat32 && grad_scale.numel() == B,
+              "grad_scale must be a float32 tensor with one entry per field on the fields' device");
+  grad_scale = grad_scale.contiguous();
+  c10::hip::HIPGuard guard(dev.index());
+  dmsc_context* ctx = context_for(dev.index());
+  hipStream_t stream = c10::hip::getCurrentHIPStream(dev.index()).stream();
+  std::vector<torch::Tensor> keep, grads;
+  std::vector<const float*> ptrs;
+  std::vector<float*> gptrs;
+  std::vector<int32_t> hs, ws;
+  for (int64_t b = 0; b < B; ++b) {
+    torch::Tensor f = fields[b];
+    validate(f);
+    TORCH_CHECK(f.dim() == 2 && f.device() == dev, "each field must be a 2D tensor on the same device");
+    f = f.contiguous();
+    keep.push_back(f);
+    ptrs.push_back(f.data_ptr<float>());
+    hs.push_back((int32_t)f.size(0));
+    ws.push_back((int32_t)f.size(1));
+    grads.push_back(torch::empty_like(f));
+    gptrs.push_back(grads.back().data_ptr<float>());
+  }
+  std::vector<dmsc_pairs> pv = pair_views(pairs, dev, keep);
+  dmsc_status st;
+  {
+    pybind11::gil_scoped_release release;
+    st = dmsc_persistence_backward(ctx, (int32_t)B, ptrs.data(), hs.data(), ws.data(), is_dual ? DMSC_DUAL : 0u,
+                                   pv.data(), (float)max_persistence, grad_scale.data_ptr<float>(), gptrs.data(),
+                                   (dmsc_stream_t)stream);
+  }
   TORCH_CHECK(st == DMSC_OK, dmsc_last_error(ctx));
   return grads;
+}
+
+// Per-field loss sum(pers^2 for 0 <= pers < max_persistence) of the pairs of each field, as a [B]
+// float32 tensor on their device (one kernel for all fields).
+torch::Tensor persistence_loss(std::vector<std::vector<torch::Tensor>> pairs, double max_persistence) {
+  const int64_t B = (int64_t)pairs.size();
+  TORCH_CHECK(B > 0 && !pairs[0].empty(), "one set of pairs per field is required");
+  const auto dev = pairs[0][0].device();
+  TORCH_CHECK(dev.is_cuda(), "persistence_loss requires GPU tensors");
+  c10::hip::HIPGuard guard(dev.index());
+  dmsc_context* ctx = context_for(dev.index());
+  hipStream_t stream = c10::hip::getCurrentHIPStream(dev.index()).stream();
+  std::vector<torch::Tensor> keep;
+  std::vector<dmsc_pairs> pv = pair_views(pairs, dev, keep);
+  torch::Tensor loss = torch::empty({B}, torch::TensorOptions().dtype(torch::kFloat32).device(dev));
+  const dmsc_status st = dmsc_persistence_loss(ctx, (int32_t)B, pv.data(), (float)max_persistence,
+                                               loss.data_ptr<float>(), (dmsc_stream_t)stream);
+  TORCH_CHECK(st == DMSC_OK, dmsc_last_error(ctx));
+  return loss;
 }
 
 pybind11::object debug_fetch(int device, const std::string& name, int64_t capacity, bool as_float, bool as_u64) {
@@ -331,6 +364,8 @@ PYBIND11_MODULE(dmsc_gpu, m) {
   m.def("persistence_backward", &persistence_backward, "Gradient of sum(pers^2 for 0 <= pers < max) per field",
         pybind11::arg("fields"), pybind11::arg("pairs"), pybind11::arg("grad_scale"),
         pybind11::arg("max_persistence"), pybind11::arg("is_dual") = false);
+  m.def("persistence_loss", &persistence_loss, "sum(pers^2 for 0 <= pers < max) of each field's pairs",
+        pybind11::arg("pairs"), pybind11::arg("max_persistence"));
   m.def("debug_fetch", &debug_fetch, pybind11::arg("device"), pybind11::arg("name"), pybind11::arg("capacity"),
         pybind11::arg("as_float") = false, pybind11::arg("as_u64") = false);
 }

@@ -22,11 +22,13 @@
 #include <vector>
 
 #include "dmsc_hip.h"
+#include "k_diff.hpp"
 #include "k_topo.hpp"
 
 namespace dmsc {
 
 void fill_u8(uint8_t* p, uint8_t v, int64_t n, hipStream_t st);
+void fill_f32_zero(float* p, int64_t n, hipStream_t st);
 void scan_i32_to_i64(const int32_t* in, int64_t* out, int64_t n, int64_t* d_total, Arena& ar, int slot, hipStream_t st);
 
 // arena slots
@@ -35,7 +37,7 @@ enum : int {
   A_FSUCC, A_VSUCC, A_PJF, A_PJV, A_PLANES, A_PREMIN, A_PRESAD, A_PREMAX,
   A_C32, A_C64,
   A_MINIDS, A_MINVAL, A_SADIDS, A_SADVAL, A_MAXIDS, A_MAXVAL,
-  A_CMAX, A_CMIN, A_LMAX, A_LMIN, A_META, A_BASES32, A_SADFIELD, A_CONTRIB, A_SURV1, A_SURVR1,
+  A_CMAX, A_CMIN, A_LMAX, A_LMIN, A_META, A_BASES32, A_SADFIELD, A_SURV1, A_SURVR1,
   A_KEYS, A_KEYS2, A_VALS2, A_ASC, A_ORDMIN, A_ORDMAX, A_REMOVED, A_POSMIN, A_POSMAX, A_RSP0, A_RSP1,
   A_SIDE0,  // per persistence side: kSideSlots slots each
   A_SIDE_END = A_SIDE0 + 2 * 15,
@@ -89,6 +91,12 @@ struct dmsc_context {
   const int32_t *cmax = nullptr, *cmin = nullptr, *pair_min = nullptr, *pair_max = nullptr;
   const float *pers_min = nullptr, *pers_max = nullptr;
   const uint64_t *pjf = nullptr, *pjv = nullptr;
+  // persistence-loss backward scratch: grow-only, all zeros between calls unless bwd_dirty
+  float* bwd_scratch = nullptr;
+  int64_t bwd_cap = 0;  // floats
+  bool bwd_dirty = false;
+  hipEvent_t ev_bwd = nullptr;  // end of the last backward's work (orders calls on other streams)
+  bool bwd_pending = false;
 };
 
 namespace {
@@ -1035,6 +1043,82 @@ void run_fields(dmsc_context* ctx, const std::vector<FieldIn>& fields, uint32_t 
 
 // Runs one entry point under the context lock; exceptions become a status plus the calling
 // thread's error message. The HIP device of the calling thread is restored afterwards.
+// Persistence-loss fields in chunks of at most kBwdChunk fields and cap_cells scratch cells (a
+// field beyond cap_cells is an error). fields / H / W / gscale / grad may be null (loss only: no
+// field data is read).
+std::vector<BwdChunk> bwd_chunks(int B, const float* const* fields, const int32_t* H, const int32_t* W,
+                                 const dmsc_pairs* pairs, const float* gscale, float* const* grad, int64_t cap_cells) {
+  std::vector<BwdChunk> out;
+  BwdChunk cur{};
+  int64_t cells = 0, blocks = 0;
+  auto flush = [&]() {
+    if (cur.n == 0) return;
+    bwd_assign_blocks(cur);
+    out.push_back(cur);
+    cur = BwdChunk{};
+    cur.loss_base = out.back().loss_base + out.back().n;
+    cells = blocks = 0;
+  };
+  for (int b = 0; b < B; ++b) {
+    const dmsc_pairs& p = pairs[b];
+    if (p.n_sad < 0 || p.n_min < 0 || p.n_max < 0 ||
+        (p.n_sad > 0 && (!p.sad_pts || !p.p_min || !p.ppairs_min || !p.p_max || !p.ppairs_max)) ||
+        (p.n_min > 0 && !p.min_pts) || (p.n_max > 0 && !p.max_pts) || p.n_sad > INT32_MAX || p.n_min > INT32_MAX ||
+        p.n_max > INT32_MAX)
+      throw Fail{DMSC_ERR_INVALID_ARGUMENT, "persistence pairs of field " + std::to_string(b) + " are incomplete"};
+    BwdField F{};
+    F.sad_pts = p.sad_pts;
+    F.p_min = p.p_min;
+    F.pp_min = p.ppairs_min;
+    F.min_pts = p.min_pts;
+    F.p_max = p.p_max;
+    F.pp_max = p.ppairs_max;
+    F.max_pts = p.max_pts;
+    F.n_sad = (int32_t)p.n_sad;
+    F.n_min = (int32_t)p.n_min;
+    F.n_max = (int32_t)p.n_max;
+    int64_t fc = 0, fb = 0;
+    if (fields) {
+      validate_field(FieldIn{fields[b], H[b], W[b], 0.f, b});
+      if (!grad[b]) throw Fail{DMSC_ERR_INVALID_ARGUMENT, "gradient output must be defined"};
+      F.f = fields[b];
+      F.grad = grad[b];
+      F.gscale = gscale + b;
+      F.H = H[b];
+      F.W = W[b];
+      fc = 4 * ((int64_t)H[b] + 1) * ((int64_t)W[b] + 1);
+      fb = bwd_blocks_for(H[b], W[b], p.n_sad);
+      if (fc > cap_cells)
+        throw Fail{DMSC_ERR_TOO_LARGE, "field " + std::to_string(b) + " needs " + std::to_string(fc) +
+                                           " scratch cells; at most " + std::to_string(cap_cells) + " fit"};
+    }
+    if (cur.n == kBwdChunk || cells + fc > cap_cells || blocks + fb > INT32_MAX) flush();
+    cur.fld[cur.n++] = F;
+    cells += fc;
+    blocks += fb;
+  }
+  flush();
+  return out;
+}
+
+// Launches the backward of every chunk on `st` with `scratch` (cap floats): clears the whole
+// scratch first when it may hold leftovers, and orders the work after the previous backward of this
+// context (which used the same scratch, possibly on another stream).
+void run_backward(dmsc_context* ctx, const std::vector<BwdChunk>& chunks, uint32_t flags, float max_pers,
+                  float* scratch, int64_t cap, bool& dirty, hipStream_t st) {
+  if (ctx->bwd_pending) DMSC_CHECK(hipStreamWaitEvent(st, ctx->ev_bwd, 0));
+  if (dirty) fill_f32_zero(scratch, cap, st);
+  dirty = true;  // until every chunk's clear pass is queued
+  for (const BwdChunk& c : chunks) launch_persistence_backward_chunk((flags & DMSC_DUAL) != 0, c, max_pers, scratch, st);
+  dirty = false;
+  DMSC_CHECK(hipEventRecord(ctx->ev_bwd, st));
+  ctx->bwd_pending = true;
+  if (ctx->times.level) {
+    DMSC_CHECK(hipStreamSynchronize(st));
+    ctx->times.collect();
+  }
+}
+
 dmsc_status guarded(dmsc_context* ctx, const std::function<void()>& fn) {
   if (!ctx) return DMSC_ERR_INVALID_ARGUMENT;
   std::lock_guard<std::mutex> lock(ctx->mu);
@@ -1075,6 +1159,11 @@ dmsc_status dmsc_context_create(int device, dmsc_context** out) {
     return DMSC_ERR_HIP;
   }
   c->pinned_cap = 4096;
+  if (hipEventCreateWithFlags(&c->ev_bwd, hipEventDisableTiming) != hipSuccess) {
+    (void)hipHostFree(c->pinned);
+    delete c;
+    return DMSC_ERR_HIP;
+  }
   *out = c;
   return DMSC_OK;
 }
@@ -1091,6 +1180,8 @@ void dmsc_context_destroy(dmsc_context* ctx) {
   if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
   if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
   if (ctx->ev_mark) (void)hipEventDestroy(ctx->ev_mark);
+  if (ctx->ev_bwd) (void)hipEventDestroy(ctx->ev_bwd);
+  if (ctx->bwd_scratch) (void)hipFree(ctx->bwd_scratch);
   delete ctx;
 }
 
@@ -1119,29 +1210,31 @@ dmsc_status dmsc_persistence_backward(dmsc_context* ctx, int32_t B, const float*
                                       dmsc_stream_t stream) {
   if (B < 0 || (B > 0 && (!d_fields || !H || !W || !pairs || !d_grad_scale || !d_grad))) return DMSC_ERR_INVALID_ARGUMENT;
   return guarded(ctx, [&] {
-    int64_t smax = 0;
-    for (int b = 0; b < B; ++b) {
-      validate_field(FieldIn{d_fields[b], H[b], W[b], 0.f, b});
-      if (!d_grad[b]) throw Fail{DMSC_ERR_INVALID_ARGUMENT, "gradient output must be defined"};
-      const dmsc_pairs& p = pairs[b];
-      if (p.n_sad < 0 || p.n_min < 0 || p.n_max < 0 ||
-          (p.n_sad > 0 && (!p.sad_pts || !p.p_min || !p.ppairs_min || !p.p_max || !p.ppairs_max)) ||
-          (p.n_min > 0 && !p.min_pts) || (p.n_max > 0 && !p.max_pts))
-        throw Fail{DMSC_ERR_INVALID_ARGUMENT, "persistence pairs of field " + std::to_string(b) + " are incomplete"};
-      smax = std::max<int64_t>(smax, ((int64_t)H[b] + 1) * ((int64_t)W[b] + 1));
+    const hipStream_t st = (hipStream_t)stream;
+    const std::vector<BwdChunk> chunks = bwd_chunks(B, d_fields, H, W, pairs, d_grad_scale, d_grad, INT64_MAX);
+    int64_t need = 0;
+    for (const BwdChunk& c : chunks) need = std::max(need, c.cells);
+    if (chunks.empty()) return;
+    if (ctx->bwd_cap < need) {  // grow: the old buffer may still be in use by the last call's stream
+      if (ctx->bwd_pending) DMSC_CHECK(hipEventSynchronize(ctx->ev_bwd));
+      if (ctx->bwd_scratch) DMSC_CHECK(hipFree(ctx->bwd_scratch));
+      ctx->bwd_scratch = nullptr;
+      ctx->bwd_cap = 0;
+      const int64_t cap = need + need / 4 + 1024;
+      DMSC_CHECK(hipMalloc((void**)&ctx->bwd_scratch, (size_t)cap * sizeof(float)));
+      ctx->bwd_cap = cap;
+      ctx->bwd_dirty = true;
     }
-    float* contrib = ctx->ar.get<float>(A_CONTRIB, 4 * smax);  // one scratch reused field after field
-    for (int b = 0; b < B; ++b) {
-      const dmsc_pairs& p = pairs[b];
-      const PairView pv{p.n_sad, p.n_min, p.n_max, p.sad_pts, p.p_min, p.ppairs_min, p.min_pts,
-                        p.p_max,   p.ppairs_max, p.max_pts};
-      launch_persistence_backward((flags & DMSC_DUAL) != 0, d_fields[b], H[b], W[b], pv, max_persistence,
-                                  d_grad_scale + b, contrib, d_grad[b], (hipStream_t)stream);
-    }
-    if (ctx->times.level) {
-      DMSC_CHECK(hipStreamSynchronize((hipStream_t)stream));
-      ctx->times.collect();
-    }
+    run_backward(ctx, chunks, flags, max_persistence, ctx->bwd_scratch, ctx->bwd_cap, ctx->bwd_dirty, st);
+  });
+}
+
+dmsc_status dmsc_persistence_loss(dmsc_context* ctx, int32_t B, const dmsc_pairs* pairs, float max_persistence,
+                                  float* d_loss, dmsc_stream_t stream) {
+  if (B < 0 || (B > 0 && (!pairs || !d_loss))) return DMSC_ERR_INVALID_ARGUMENT;
+  return guarded(ctx, [&] {
+    const std::vector<BwdChunk> chunks = bwd_chunks(B, nullptr, nullptr, nullptr, pairs, nullptr, nullptr, INT64_MAX);
+    for (const BwdChunk& c : chunks) launch_persistence_loss_chunk(c, max_persistence, d_loss, (hipStream_t)stream);
   });
 }
 

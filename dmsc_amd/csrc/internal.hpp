@@ -375,20 +375,6 @@ void launch_labels_faces(const RankView& rv, int64_t S, const uint64_t* pj_face,
                          const int32_t* out_base, int32_t* out, hipStream_t st);
 void launch_labels_verts(const RankView& rv, int64_t N, const uint64_t* pj_vert, const int32_t* root_out,
                          const int32_t* out_base, int32_t* out, hipStream_t st);
-// persistence pairs of one field (output slots: saddles, min / max persistence, paired extremum,
-// extremum cell ids) for the backward of persistence losses (k_diff.hip)
-struct PairView {
-  int64_t n_sad, n_min, n_max;
-  const int32_t* sad_pts;
-  const float* p_min;    // -1: cycle
-  const int32_t* pp_min; // -1: none
-  const int32_t* min_pts;
-  const float* p_max;
-  const int32_t* pp_max;
-  const int32_t* max_pts;
-};
-void launch_persistence_backward(bool dual, const float* f, int H, int W, const PairView& pv, float max_pers,
-                                 const float* gscale, float* contrib, float* grad, hipStream_t st);
 void radix_sort_pairs_bits(uint32_t* keys, int32_t* vals, uint32_t* keys_tmp, int32_t* vals_tmp, int64_t n, int bits,
                            Arena& ar, int slot, int pslot, hipStream_t st, bool iota_vals = false);
 

@@ -22,11 +22,6 @@ except ImportError as e:  # fail loudly: there is no CPU fallback
 _PAIR_SLOTS = (3, 7, 9, 2, 6, 8, 1)
 
 
-def _side_loss(p, max_persistence):
-    keep = (p >= 0) & (p < max_persistence)
-    return torch.where(keep, p * p, torch.zeros_like(p)).sum()
-
-
 class PersistenceLoss(torch.autograd.Function):
     """Per-field loss of a [B, H, W] float32 GPU tensor; returns a [B] tensor."""
 
@@ -35,7 +30,7 @@ class PersistenceLoss(torch.autograd.Function):
         fields = fields.detach().contiguous()
         out = dmsc_gpu.extract_dmsc(fields, -1.0, False, bool(is_dual), False, False, False, False)
         pairs = [[r[k] for k in _PAIR_SLOTS] for r in out]
-        losses = torch.stack([_side_loss(p[1], max_persistence) + _side_loss(p[4], max_persistence) for p in pairs])
+        losses = dmsc_gpu.persistence_loss(pairs, float(max_persistence))  # one kernel for every field
         ctx.save_for_backward(fields)
         ctx.pairs = pairs
         ctx.max_persistence = float(max_persistence)
