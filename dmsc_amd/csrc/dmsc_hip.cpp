@@ -37,7 +37,7 @@ enum : int {
   A_FSUCC, A_VSUCC, A_PJF, A_PJV, A_PLANES, A_PREMIN, A_PRESAD, A_PREMAX,
   A_C32, A_C64,
   A_MINIDS, A_MINVAL, A_SADIDS, A_SADVAL, A_MAXIDS, A_MAXVAL,
-  A_CMAX, A_CMIN, A_LMAX, A_LMIN, A_META, A_BASES32, A_SADFIELD, A_SURV1, A_SURVR1,
+  A_CMAX, A_CMIN, A_LMAX, A_LMIN, A_META, A_BASES32, A_SADFIELD, A_LOSSPART, A_SURV1, A_SURVR1,
   A_KEYS, A_KEYS2, A_VALS2, A_ASC, A_ORDMIN, A_ORDMAX, A_REMOVED, A_POSMIN, A_POSMAX, A_RSP0, A_RSP1,
   A_SIDE0,  // per persistence side: kSideSlots slots each
   A_SIDE_END = A_SIDE0 + 2 * 15,
@@ -95,7 +95,7 @@ struct dmsc_context {
   float* bwd_scratch = nullptr;
   int64_t bwd_cap = 0;  // floats
   bool bwd_dirty = false;
-  hipEvent_t ev_bwd = nullptr;  // end of the last backward's work (orders calls on other streams)
+  hipEvent_t ev_bwd = nullptr;  // end of the last loss / backward work (orders calls on other streams)
   bool bwd_pending = false;
 };
 
@@ -1043,7 +1043,7 @@ void run_fields(dmsc_context* ctx, const std::vector<FieldIn>& fields, uint32_t 
 
 // Runs one entry point under the context lock; exceptions become a status plus the calling
 // thread's error message. The HIP device of the calling thread is restored afterwards.
-// Persistence-loss fields in chunks of at most kBwdChunk fields and cap_cells scratch cells (a
+// Persistence-loss fields in chunks of at most kBwdChunk fields and cap_cells scratch floats (a
 // field beyond cap_cells is an error). fields / H / W / gscale / grad may be null (loss only: no
 // field data is read).
 std::vector<BwdChunk> bwd_chunks(int B, const float* const* fields, const int32_t* H, const int32_t* W,
@@ -1086,11 +1086,12 @@ std::vector<BwdChunk> bwd_chunks(int B, const float* const* fields, const int32_
       F.gscale = gscale + b;
       F.H = H[b];
       F.W = W[b];
-      fc = 4 * ((int64_t)H[b] + 1) * ((int64_t)W[b] + 1);
+      fc = bwd_scratch_floats(H[b], W[b]);
       fb = bwd_blocks_for(H[b], W[b], p.n_sad);
       if (fc > cap_cells)
-        throw Fail{DMSC_ERR_TOO_LARGE, "field " + std::to_string(b) + " needs " + std::to_string(fc) +
-                                           " scratch cells; at most " + std::to_string(cap_cells) + " fit"};
+        throw Fail{DMSC_ERR_TOO_LARGE, "field " + std::to_string(b) + " (" + std::to_string(H[b]) + "x" +
+                                           std::to_string(W[b]) + ") needs " + std::to_string(fc) +
+                                           " scratch floats; at most " + std::to_string(cap_cells) + " fit"};
     }
     if (cur.n == kBwdChunk || cells + fc > cap_cells || blocks + fb > INT32_MAX) flush();
     cur.fld[cur.n++] = F;
@@ -1213,7 +1214,7 @@ dmsc_status dmsc_persistence_backward(dmsc_context* ctx, int32_t B, const float*
     const hipStream_t st = (hipStream_t)stream;
     const std::vector<BwdChunk> chunks = bwd_chunks(B, d_fields, H, W, pairs, d_grad_scale, d_grad, INT64_MAX);
     int64_t need = 0;
-    for (const BwdChunk& c : chunks) need = std::max(need, c.cells);
+    for (const BwdChunk& c : chunks) need = std::max(need, c.scratch);
     if (chunks.empty()) return;
     if (ctx->bwd_cap < need) {  // grow: the old buffer may still be in use by the last call's stream
       if (ctx->bwd_pending) DMSC_CHECK(hipEventSynchronize(ctx->ev_bwd));
@@ -1234,7 +1235,13 @@ dmsc_status dmsc_persistence_loss(dmsc_context* ctx, int32_t B, const dmsc_pairs
   if (B < 0 || (B > 0 && (!pairs || !d_loss))) return DMSC_ERR_INVALID_ARGUMENT;
   return guarded(ctx, [&] {
     const std::vector<BwdChunk> chunks = bwd_chunks(B, nullptr, nullptr, nullptr, pairs, nullptr, nullptr, INT64_MAX);
-    for (const BwdChunk& c : chunks) launch_persistence_loss_chunk(c, max_persistence, d_loss, (hipStream_t)stream);
+    // partial sums of one chunk at a time; chunks on the same stream reuse the buffer in order
+    const hipStream_t st = (hipStream_t)stream;
+    double* part = ctx->ar.get<double>(A_LOSSPART, loss_partials(kBwdChunk));
+    if (ctx->bwd_pending) DMSC_CHECK(hipStreamWaitEvent(st, ctx->ev_bwd, 0));
+    for (const BwdChunk& c : chunks) launch_persistence_loss_chunk(c, max_persistence, part, d_loss, st);
+    DMSC_CHECK(hipEventRecord(ctx->ev_bwd, st));
+    ctx->bwd_pending = true;
   });
 }
 

@@ -8,12 +8,12 @@
 //
 // Fields go through the kernels in chunks of up to kBwdChunk, their descriptors passed by value;
 // each kernel finds its field from block-uniform per-field block bases. No float atomics: the
-// scatter pass writes every critical cell's total contribution into a per-cell scratch array (each
-// cell has one writer: a saddle sums its two sides, an extremum dies in one pair when the pairs
-// come from the unsimplified complex); the gather pass lets every pixel add the contributions of
-// the cells whose value it carries, in a fixed order; the clear pass writes zeros back at exactly
-// the cells the scatter wrote. The scratch is therefore all zeros between calls and no pass over
-// the whole cell array is needed, and the result is deterministic.
+// scatter pass writes every critical cell's total contribution to its value pixel, into one of nine
+// per-pixel scratch planes chosen by where the cell lies around that pixel (each entry has one
+// writer: a saddle sums its two sides, an extremum dies in one pair when the pairs come from the
+// unsimplified complex); the gather pass lets every pixel add its nine entries in a fixed order and
+// zero them. The scratch is therefore all zeros between calls (no pass over the whole of it), and
+// the result is deterministic.
 #include "k_diff.hpp"
 
 namespace dmsc {
@@ -25,6 +25,7 @@ constexpr int kPixPerBlock = 4 * kBwdThreads;  // each thread of the gather take
 constexpr int kSadPerBlock = kBwdThreads;
 
 inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+__device__ __forceinline__ int64_t cdiv_d(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 // field of block b: the last field whose first block is <= b (block-uniform, from kernel arguments)
 template <int Which>
@@ -37,16 +38,46 @@ __device__ __forceinline__ int chunk_field(const BwdChunk& c, int b) {
 
 }  // namespace
 
-// CLEAR = false: write each critical cell's contribution; CLEAR = true: write zeros at those cells
-template <bool CLEAR>
+// A cell id of an H x W field that is not a padding slot (vertices need y < H and x < W, type-1
+// edges y < H, type-2 edges x < W; ids below 4 (H+1)(W+1) have y <= H): every such cell has at least
+// one pixel among its vertices, so its value pixel exists.
+__device__ __forceinline__ bool valid_cell(int cell, int H, int W) {
+  const int Nx = W + 1, t = cell & 3, q = cell >> 2, y = q / Nx, x = q - y * Nx;
+  return t == 3 || (t == 1 && y < H) || (t == 2 && x < W) || (t == 0 && y < H && x < W);
+}
+
+// Position of cell (t, cy, cx) among the nine cells that contain its value pixel (py, px), in the
+// order the gather adds them: the vertex, the edges (y, x) and (y, x+1) of type 1, the edges (y, x)
+// and (y+1, x) of type 2, the faces (y, x), (y, x+1), (y+1, x), (y+1, x+1).
+__device__ __forceinline__ int star_slot(int t, int cy, int cx, int py, int px) {
+  if (t == 0) return 0;
+  if (t == 1) return px == cx ? 1 : 2;
+  if (t == 2) return py == cy ? 3 : 4;
+  return py == cy ? (px == cx ? 5 : 6) : (px == cx ? 7 : 8);
+}
+
+// Writes a critical cell's contribution into plane star_slot of its value pixel.
+template <bool DUAL>
+__device__ __forceinline__ void put(const BwdField& F, float* planes, uint32_t N, int cell, float v) {
+  const int Nx = F.W + 1;
+  const int t = cell & 3, q = cell >> 2;
+  const int cy = q / Nx, cx = q - cy * Nx;
+  const int vp = top_pixel<DUAL>(t, cy, cx, F.H, F.W, F.f);
+  const int py = vp / F.W, px = vp - py * F.W;
+  planes[(uint32_t)star_slot(t, cy, cx, py, px) * N + (uint32_t)vp] = v;
+}
+
+// Each critical cell's total contribution (a saddle's two sides summed), placed by its value pixel.
+template <bool DUAL>
 __global__ __launch_bounds__(kBwdThreads) void k_pers_scatter(BwdChunk c, float max_pers, float* __restrict__ scratch) {
   const int fi = chunk_field<0>(c, blockIdx.x);
   const BwdField& F = c.fld[fi];
   const int64_t k = (int64_t)(blockIdx.x - F.blk_sad) * kSadPerBlock + threadIdx.x;
   if (k >= F.n_sad) return;
-  const int64_t ncells = 4 * (int64_t)(F.H + 1) * (F.W + 1);
-  float* contrib = scratch + F.cell_base;
-  const float g = CLEAR ? 0.0f : *F.gscale;
+  const int ncells = 4 * (F.H + 1) * (F.W + 1);
+  const uint32_t N = (uint32_t)F.H * (uint32_t)F.W;
+  float* planes = scratch + F.scratch_base;
+  const float g = *F.gscale;
   float cs = 0.0f;
   const float pm = F.p_min[k];
   const int em = F.pp_min[k];
@@ -54,7 +85,7 @@ __global__ __launch_bounds__(kBwdThreads) void k_pers_scatter(BwdChunk c, float 
     const float v = 2.0f * pm * g;
     cs += v;
     const int cell = F.min_pts[em];
-    if (cell >= 0 && cell < ncells) contrib[cell] = CLEAR ? 0.0f : -v;
+    if (cell >= 0 && cell < ncells && valid_cell(cell, F.H, F.W)) put<DUAL>(F, planes, N, cell, -v);
   }
   const float pM = F.p_max[k];
   const int eM = F.pp_max[k];
@@ -62,51 +93,54 @@ __global__ __launch_bounds__(kBwdThreads) void k_pers_scatter(BwdChunk c, float 
     const float v = 2.0f * pM * g;
     cs -= v;
     const int cell = F.max_pts[eM];
-    if (cell >= 0 && cell < ncells) contrib[cell] = CLEAR ? 0.0f : v;
+    if (cell >= 0 && cell < ncells && valid_cell(cell, F.H, F.W)) put<DUAL>(F, planes, N, cell, v);
   }
   const int sc = F.sad_pts[k];
-  if (sc >= 0 && sc < ncells) contrib[sc] = CLEAR ? 0.0f : cs;
+  if (sc >= 0 && sc < ncells && valid_cell(sc, F.H, F.W)) put<DUAL>(F, planes, N, sc, cs);
 }
 
-// pixel (y, x): its vertex, the four edges and four faces that contain it, in that order
-template <bool DUAL>
-__global__ __launch_bounds__(kBwdThreads) void k_pers_gather(BwdChunk c, const float* __restrict__ scratch) {
+// Pixel p adds its nine planes in order and zeroes what was set, so the scratch is all zeros again
+// when the kernel ends. Adding an unset (zero) entry leaves the sum unchanged: it starts at +0 and
+// can never become -0, since x + (-x) rounds to +0.
+__global__ __launch_bounds__(kBwdThreads) void k_pers_gather(BwdChunk c, float* __restrict__ scratch) {
   const int fi = chunk_field<1>(c, blockIdx.x);
   const BwdField& F = c.fld[fi];
-  const int H = F.H, W = F.W, Nx = W + 1;
-  const int64_t N = (int64_t)H * W;
-  const float* __restrict__ f = F.f;
-  const float* __restrict__ contrib = scratch + F.cell_base;
-  const int64_t first = (int64_t)(blockIdx.x - F.blk_pix) * kPixPerBlock + threadIdx.x;
+  const uint32_t N = (uint32_t)F.H * (uint32_t)F.W;
+  float* planes = scratch + F.scratch_base;
+  const uint32_t first = (uint32_t)(blockIdx.x - F.blk_pix) * kPixPerBlock + threadIdx.x;
+  constexpr int R = kPixPerBlock / kBwdThreads;
+  float v[R][9];
 #pragma unroll
-  for (int r = 0; r < kPixPerBlock / kBwdThreads; ++r) {
-    const int64_t p = first + (int64_t)r * kBwdThreads;
-    if (p >= N) break;
-    const int y = (int)(p / W), x = (int)(p - (int64_t)y * W);
-    float acc = 0.0f;
-    auto add = [&](int t, int cy, int cx) {
-      const float v = contrib[4 * ((int64_t)cy * Nx + cx) + t];
-      if (v != 0.0f && top_pixel<DUAL>(t, cy, cx, H, W, f) == (int)p) acc += v;
-    };
-    add(0, y, x);
-    add(1, y, x);
-    add(1, y, x + 1);
-    add(2, y, x);
-    add(2, y + 1, x);
-    add(3, y, x);
-    add(3, y, x + 1);
-    add(3, y + 1, x);
-    add(3, y + 1, x + 1);
-    F.grad[p] = acc;
+  for (int r = 0; r < R; ++r) {
+    const uint32_t p = first + (uint32_t)r * kBwdThreads;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) v[r][k] = p < N ? planes[(uint32_t)k * N + p] : 0.0f;
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint32_t p = first + (uint32_t)r * kBwdThreads;
+    if (p < N) {
+      float acc = 0.0f;
+#pragma unroll
+      for (int k = 0; k < 9; ++k) {
+        acc += v[r][k];
+        if (v[r][k] != 0.0f) planes[(uint32_t)k * N + p] = 0.0f;
+      }
+      F.grad[p] = acc;
+    }
   }
 }
 
-// loss of each field of the chunk: one block per field sums pers^2 over its pairs in double, in a
-// fixed order (thread strides, then a tree over the block), so the result is reproducible
-__global__ __launch_bounds__(kBwdThreads) void k_pers_loss(BwdChunk c, float max_pers, float* __restrict__ loss) {
-  const BwdField& F = c.fld[blockIdx.x];
+// Partial loss sums: kLossSplit blocks per field of the chunk, block k of a field summing pers^2 over
+// its k-th slice of the saddles in double (thread strides, then a tree over the block).
+constexpr int kLossSplit = 64;
+__global__ __launch_bounds__(kBwdThreads) void k_pers_loss_part(BwdChunk c, float max_pers, double* __restrict__ part) {
+  const BwdField& F = c.fld[blockIdx.x / kLossSplit];
+  const int split = blockIdx.x % kLossSplit;
+  const int64_t len = cdiv_d(F.n_sad, kLossSplit);
+  const int64_t k0 = split * len, k1 = min((int64_t)F.n_sad, k0 + len);
   double acc = 0.0;
-  for (int64_t k = threadIdx.x; k < F.n_sad; k += kBwdThreads) {
+  for (int64_t k = k0 + threadIdx.x; k < k1; k += kBwdThreads) {
     const float pm = F.p_min[k];
     if (F.pp_min[k] >= 0 && pm >= 0.0f && pm < max_pers) acc += (double)pm * (double)pm;
     const float pM = F.p_max[k];
@@ -119,7 +153,16 @@ __global__ __launch_bounds__(kBwdThreads) void k_pers_loss(BwdChunk c, float max
     if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) loss[c.loss_base + blockIdx.x] = (float)s[0];
+  if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+// loss of each field: its kLossSplit partial sums added in order
+__global__ void k_pers_loss_sum(BwdChunk c, const double* __restrict__ part, float* __restrict__ loss) {
+  const int fi = threadIdx.x;
+  if (fi >= c.n) return;
+  double acc = 0.0;
+  for (int k = 0; k < kLossSplit; ++k) acc += part[fi * kLossSplit + k];
+  loss[c.loss_base + fi] = (float)acc;
 }
 
 void bwd_assign_blocks(BwdChunk& c) {
@@ -129,14 +172,14 @@ void bwd_assign_blocks(BwdChunk& c) {
     BwdField& F = c.fld[k];
     F.blk_sad = bs;
     F.blk_pix = bp;
-    F.cell_base = cells;
+    F.scratch_base = cells;
     bs += (int)cdiv(F.n_sad, kSadPerBlock);
     bp += (int)cdiv((int64_t)F.H * F.W, kPixPerBlock);
-    cells += 4 * (int64_t)(F.H + 1) * (F.W + 1);
+    cells += bwd_scratch_floats(F.H, F.W);
   }
   c.blocks_sad = bs;
   c.blocks_pix = bp;
-  c.cells = cells;
+  c.scratch = cells;
 }
 
 int64_t bwd_blocks_for(int H, int W, int64_t n_sad) {
@@ -146,27 +189,25 @@ int64_t bwd_blocks_for(int H, int W, int64_t n_sad) {
 void launch_persistence_backward_chunk(bool dual, const BwdChunk& c, float max_pers, float* scratch, hipStream_t st) {
   KScope ks_("persistence_backward", st, true);
   if (c.blocks_sad > 0) {
-    hipLaunchKernelGGL(k_pers_scatter<false>, dim3(c.blocks_sad), dim3(kBwdThreads), 0, st, c, max_pers, scratch);
+    if (dual)
+      hipLaunchKernelGGL(k_pers_scatter<true>, dim3(c.blocks_sad), dim3(kBwdThreads), 0, st, c, max_pers, scratch);
+    else
+      hipLaunchKernelGGL(k_pers_scatter<false>, dim3(c.blocks_sad), dim3(kBwdThreads), 0, st, c, max_pers, scratch);
     DMSC_LAUNCH_CHECK();
   }
   if (c.blocks_pix > 0) {
-    if (dual)
-      hipLaunchKernelGGL(k_pers_gather<true>, dim3(c.blocks_pix), dim3(kBwdThreads), 0, st, c,
-                         (const float*)scratch);
-    else
-      hipLaunchKernelGGL(k_pers_gather<false>, dim3(c.blocks_pix), dim3(kBwdThreads), 0, st, c,
-                         (const float*)scratch);
-    DMSC_LAUNCH_CHECK();
-  }
-  if (c.blocks_sad > 0) {
-    hipLaunchKernelGGL(k_pers_scatter<true>, dim3(c.blocks_sad), dim3(kBwdThreads), 0, st, c, max_pers, scratch);
+    hipLaunchKernelGGL(k_pers_gather, dim3(c.blocks_pix), dim3(kBwdThreads), 0, st, c, scratch);
     DMSC_LAUNCH_CHECK();
   }
 }
 
-void launch_persistence_loss_chunk(const BwdChunk& c, float max_pers, float* loss, hipStream_t st) {
+int64_t loss_partials(int n_fields) { return (int64_t)n_fields * kLossSplit; }
+
+void launch_persistence_loss_chunk(const BwdChunk& c, float max_pers, double* part, float* loss, hipStream_t st) {
   if (c.n <= 0) return;
-  hipLaunchKernelGGL(k_pers_loss, dim3(c.n), dim3(kBwdThreads), 0, st, c, max_pers, loss);
+  hipLaunchKernelGGL(k_pers_loss_part, dim3(c.n * kLossSplit), dim3(kBwdThreads), 0, st, c, max_pers, part);
+  DMSC_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_pers_loss_sum, dim3(1), dim3(64), 0, st, c, (const double*)part, loss);
   DMSC_LAUNCH_CHECK();
 }
 

@@ -110,10 +110,10 @@ typedef struct {
  * persistence^2, scaled by d_grad_scale[b] (device float), written to d_grad[b] (H[b] x W[b]
  * float32, fully overwritten). The pairs must come from an unsimplified complex (threshold < 0),
  * in which every extremum dies in at most one pair; flags: DMSC_DUAL as in the extraction.
- * Deterministic (no float atomics) and asynchronous on `stream`: three launches per 16 fields
- * (scatter, gather, clear) over a context-owned scratch of 4 (H+1)(W+1) floats per field, which is
- * all zeros between calls; a call on another stream waits for the previous call's work. No
- * reference counterpart (the reference lists differentiation as future work, readme.md:237-242). */
+ * Deterministic (no float atomics) and asynchronous on `stream`: two launches per 16 fields
+ * (scatter, gather) over a context-owned scratch of 9 H W floats per field, which is all zeros
+ * between calls; a call on another stream waits for the previous call's work. No reference
+ * counterpart (the reference lists differentiation as future work, readme.md:237-242). */
 dmsc_status dmsc_persistence_backward(dmsc_context* ctx, int32_t B, const float* const* d_fields, const int32_t* H,
                                       const int32_t* W, uint32_t flags, const dmsc_pairs* pairs,
                                       float max_persistence, const float* d_grad_scale, float* const* d_grad,
