@@ -1041,23 +1041,22 @@ void run_fields(dmsc_context* ctx, const std::vector<FieldIn>& fields, uint32_t 
   }
 }
 
-// Runs one entry point under the context lock; exceptions become a status plus the calling
-// thread's error message. The HIP device of the calling thread is restored afterwards.
-// Persistence-loss fields in chunks of at most kBwdChunk fields and cap_cells scratch floats (a
-// field beyond cap_cells is an error). fields / H / W / gscale / grad may be null (loss only: no
-// field data is read).
-std::vector<BwdChunk> bwd_chunks(int B, const float* const* fields, const int32_t* H, const int32_t* W,
-                                 const dmsc_pairs* pairs, const float* gscale, float* const* grad, int64_t cap_cells) {
-  std::vector<BwdChunk> out;
+// Persistence-loss fields in chunks of at most kBwdChunk fields and cap scratch floats, handed to
+// fn(const BwdChunk&) one at a time (a field that needs more than cap is an error). fields / H / W /
+// gscale / grad may be null (loss only: no field data is read). Builds each chunk on the stack.
+template <class Fn>
+void for_each_bwd_chunk(int B, const float* const* fields, const int32_t* H, const int32_t* W, const dmsc_pairs* pairs,
+                        const float* gscale, float* const* grad, int64_t cap, Fn&& fn) {
   BwdChunk cur{};
-  int64_t cells = 0, blocks = 0;
+  int64_t used = 0, blocks = 0;
   auto flush = [&]() {
     if (cur.n == 0) return;
     bwd_assign_blocks(cur);
-    out.push_back(cur);
+    fn(static_cast<const BwdChunk&>(cur));
+    const int64_t next_base = cur.loss_base + cur.n;
     cur = BwdChunk{};
-    cur.loss_base = out.back().loss_base + out.back().n;
-    cells = blocks = 0;
+    cur.loss_base = next_base;
+    used = blocks = 0;
   };
   for (int b = 0; b < B; ++b) {
     const dmsc_pairs& p = pairs[b];
@@ -1088,38 +1087,48 @@ std::vector<BwdChunk> bwd_chunks(int B, const float* const* fields, const int32_
       F.W = W[b];
       fc = bwd_scratch_floats(H[b], W[b]);
       fb = bwd_blocks_for(H[b], W[b], p.n_sad);
-      if (fc > cap_cells)
+      if (fc > cap)
         throw Fail{DMSC_ERR_TOO_LARGE, "field " + std::to_string(b) + " (" + std::to_string(H[b]) + "x" +
                                            std::to_string(W[b]) + ") needs " + std::to_string(fc) +
-                                           " scratch floats; at most " + std::to_string(cap_cells) + " fit"};
+                                           " scratch floats; at most " + std::to_string(cap) + " fit"};
     }
-    if (cur.n == kBwdChunk || cells + fc > cap_cells || blocks + fb > INT32_MAX) flush();
+    if (cur.n == kBwdChunk || used + fc > cap || blocks + fb > INT32_MAX) flush();
     cur.fld[cur.n++] = F;
-    cells += fc;
+    used += fc;
     blocks += fb;
   }
   flush();
-  return out;
+}
+
+// Checks every field and returns the most scratch one chunk needs (nothing is launched).
+int64_t bwd_scratch_needed(int B, const float* const* fields, const int32_t* H, const int32_t* W,
+                           const dmsc_pairs* pairs, const float* gscale, float* const* grad, int64_t cap) {
+  int64_t need = 0;
+  for_each_bwd_chunk(B, fields, H, W, pairs, gscale, grad, cap,
+                     [&](const BwdChunk& c) { need = std::max(need, c.scratch); });
+  return need;
 }
 
 // Launches the backward of every chunk on `st` with `scratch` (cap floats): clears the whole
-// scratch first when it may hold leftovers, and orders the work after the previous backward of this
-// context (which used the same scratch, possibly on another stream).
-void run_backward(dmsc_context* ctx, const std::vector<BwdChunk>& chunks, uint32_t flags, float max_pers,
+// scratch first when it may hold leftovers, and orders the work after the previous loss / backward
+// of this context (possibly on another stream). The fields are checked before anything is queued.
+void run_backward(dmsc_context* ctx, int B, const float* const* fields, const int32_t* H, const int32_t* W,
+                  const dmsc_pairs* pairs, const float* gscale, float* const* grad, uint32_t flags, float max_pers,
                   float* scratch, int64_t cap, bool& dirty, hipStream_t st) {
+  (void)bwd_scratch_needed(B, fields, H, W, pairs, gscale, grad, cap);
   if (ctx->bwd_pending) DMSC_CHECK(hipStreamWaitEvent(st, ctx->ev_bwd, 0));
   if (dirty) fill_f32_zero(scratch, cap, st);
-  dirty = true;  // until every chunk's clear pass is queued
-  for (const BwdChunk& c : chunks) launch_persistence_backward_chunk((flags & DMSC_DUAL) != 0, c, max_pers, scratch, st);
+  dirty = true;  // until every chunk's gather (which re-zeroes the scratch) is queued
+  for_each_bwd_chunk(B, fields, H, W, pairs, gscale, grad, cap, [&](const BwdChunk& c) {
+    launch_persistence_backward_chunk((flags & DMSC_DUAL) != 0, c, max_pers, scratch, st);
+  });
   dirty = false;
   DMSC_CHECK(hipEventRecord(ctx->ev_bwd, st));
   ctx->bwd_pending = true;
-  if (ctx->times.level) {
-    DMSC_CHECK(hipStreamSynchronize(st));
-    ctx->times.collect();
-  }
 }
 
+// Runs one entry point under the context lock; exceptions become a status plus the calling
+// thread's error message. The HIP device of the calling thread is restored afterwards.
 dmsc_status guarded(dmsc_context* ctx, const std::function<void()>& fn) {
   if (!ctx) return DMSC_ERR_INVALID_ARGUMENT;
   std::lock_guard<std::mutex> lock(ctx->mu);
@@ -1212,10 +1221,8 @@ dmsc_status dmsc_persistence_backward(dmsc_context* ctx, int32_t B, const float*
   if (B < 0 || (B > 0 && (!d_fields || !H || !W || !pairs || !d_grad_scale || !d_grad))) return DMSC_ERR_INVALID_ARGUMENT;
   return guarded(ctx, [&] {
     const hipStream_t st = (hipStream_t)stream;
-    const std::vector<BwdChunk> chunks = bwd_chunks(B, d_fields, H, W, pairs, d_grad_scale, d_grad, INT64_MAX);
-    int64_t need = 0;
-    for (const BwdChunk& c : chunks) need = std::max(need, c.scratch);
-    if (chunks.empty()) return;
+    const int64_t need = bwd_scratch_needed(B, d_fields, H, W, pairs, d_grad_scale, d_grad, INT64_MAX);
+    if (need == 0) return;
     if (ctx->bwd_cap < need) {  // grow: the old buffer may still be in use by the last call's stream
       if (ctx->bwd_pending) DMSC_CHECK(hipEventSynchronize(ctx->ev_bwd));
       if (ctx->bwd_scratch) DMSC_CHECK(hipFree(ctx->bwd_scratch));
@@ -1226,7 +1233,12 @@ dmsc_status dmsc_persistence_backward(dmsc_context* ctx, int32_t B, const float*
       ctx->bwd_cap = cap;
       ctx->bwd_dirty = true;
     }
-    run_backward(ctx, chunks, flags, max_persistence, ctx->bwd_scratch, ctx->bwd_cap, ctx->bwd_dirty, st);
+    run_backward(ctx, B, d_fields, H, W, pairs, d_grad_scale, d_grad, flags, max_persistence, ctx->bwd_scratch,
+                 ctx->bwd_cap, ctx->bwd_dirty, st);
+    if (ctx->times.level) {
+      DMSC_CHECK(hipStreamSynchronize(st));
+      ctx->times.collect();
+    }
   });
 }
 
@@ -1234,12 +1246,13 @@ dmsc_status dmsc_persistence_loss(dmsc_context* ctx, int32_t B, const dmsc_pairs
                                   float* d_loss, dmsc_stream_t stream) {
   if (B < 0 || (B > 0 && (!pairs || !d_loss))) return DMSC_ERR_INVALID_ARGUMENT;
   return guarded(ctx, [&] {
-    const std::vector<BwdChunk> chunks = bwd_chunks(B, nullptr, nullptr, nullptr, pairs, nullptr, nullptr, INT64_MAX);
+    (void)bwd_scratch_needed(B, nullptr, nullptr, nullptr, pairs, nullptr, nullptr, INT64_MAX);  // checks
     // partial sums of one chunk at a time; chunks on the same stream reuse the buffer in order
     const hipStream_t st = (hipStream_t)stream;
     double* part = ctx->ar.get<double>(A_LOSSPART, loss_partials(kBwdChunk));
     if (ctx->bwd_pending) DMSC_CHECK(hipStreamWaitEvent(st, ctx->ev_bwd, 0));
-    for (const BwdChunk& c : chunks) launch_persistence_loss_chunk(c, max_persistence, part, d_loss, st);
+    for_each_bwd_chunk(B, nullptr, nullptr, nullptr, pairs, nullptr, nullptr, INT64_MAX,
+                       [&](const BwdChunk& c) { launch_persistence_loss_chunk(c, max_persistence, part, d_loss, st); });
     DMSC_CHECK(hipEventRecord(ctx->ev_bwd, st));
     ctx->bwd_pending = true;
   });
