@@ -66,6 +66,39 @@ thread_local const Tunables* tl_tune = nullptr;
 
 using namespace dmsc;
 
+// ---- scratch of the allocation-free backward (dmsc_persistence_backward_noalloc)
+// A fixed array in the code object: each GPU the library runs on holds its own copy (64 MiB) until
+// the process ends. Host code never uses the array's name as an address: dmsc_context_create looks
+// up each device's copy with hipGetSymbolAddress once and keeps that pointer here.
+constexpr int64_t kNoAllocScratchFloats = int64_t(1) << 24;
+static __device__ __align__(256) float scratch[kNoAllocScratchFloats];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;                      // guards the two tables below and the scratch's users
+float* g_scratch_ptr[kMaxDevices] = {};       // device address of `scratch` on each device (null: not set up)
+bool g_scratch_dirty[kMaxDevices] = {};       // true while the scratch may hold non-zero entries
+
+// Once per device, with that device current: the device address of its `scratch`, zeroed (the
+// backward kernels need it all zeros between calls).
+dmsc_status scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return DMSC_OK;  // the allocation-free entry refuses such devices
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return DMSC_OK;
+  void* p = nullptr;
+  if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return DMSC_ERR_HIP;
+  try {
+    fill_f32_zero(static_cast<float*>(p), kNoAllocScratchFloats, nullptr);
+  } catch (const HipError&) {
+    return DMSC_ERR_HIP;
+  }
+  if (hipStreamSynchronize(nullptr) != hipSuccess) return DMSC_ERR_HIP;
+  g_scratch_ptr[device] = static_cast<float*>(p);
+  g_scratch_dirty[device] = false;
+  return DMSC_OK;
+}
+}  // namespace
+
 struct dmsc_context {
   // Every entry point holds this lock for its whole host-side run: calls on one context from several
   // threads take turns (the arena, pinned buffer, side stream and events are shared state).
@@ -1162,17 +1195,24 @@ extern "C" {
 dmsc_status dmsc_context_create(int device, dmsc_context** out) {
   if (!out) return DMSC_ERR_INVALID_ARGUMENT;
   *out = nullptr;
+  int prev_dev = -1;
+  (void)hipGetDevice(&prev_dev);
   auto* c = new dmsc_context();
   c->device = device;
+  dmsc_status st = DMSC_OK;
   if (hipSetDevice(device) != hipSuccess || hipHostMalloc((void**)&c->pinned, 4096) != hipSuccess) {
-    delete c;
-    return DMSC_ERR_HIP;
+    st = DMSC_ERR_HIP;
+  } else {
+    c->pinned_cap = 4096;
+    if (hipEventCreateWithFlags(&c->ev_bwd, hipEventDisableTiming) != hipSuccess) st = DMSC_ERR_HIP;
+    if (st == DMSC_OK) st = scratch_setup(device);
   }
-  c->pinned_cap = 4096;
-  if (hipEventCreateWithFlags(&c->ev_bwd, hipEventDisableTiming) != hipSuccess) {
-    (void)hipHostFree(c->pinned);
+  if (prev_dev >= 0 && prev_dev != device) (void)hipSetDevice(prev_dev);
+  if (st != DMSC_OK) {
+    if (c->ev_bwd) (void)hipEventDestroy(c->ev_bwd);
+    if (c->pinned) (void)hipHostFree(c->pinned);
     delete c;
-    return DMSC_ERR_HIP;
+    return st;
   }
   *out = c;
   return DMSC_OK;
@@ -1239,6 +1279,30 @@ dmsc_status dmsc_persistence_backward(dmsc_context* ctx, int32_t B, const float*
       DMSC_CHECK(hipStreamSynchronize(st));
       ctx->times.collect();
     }
+  });
+}
+
+dmsc_status dmsc_persistence_backward_noalloc(dmsc_context* ctx, int32_t B, const float* const* d_fields,
+                                              const int32_t* H, const int32_t* W, uint32_t flags,
+                                              const dmsc_pairs* pairs, float max_persistence,
+                                              const float* d_grad_scale, float* const* d_grad, dmsc_stream_t stream) {
+  if (B < 0 || (B > 0 && (!d_fields || !H || !W || !pairs || !d_grad_scale || !d_grad))) return DMSC_ERR_INVALID_ARGUMENT;
+  return guarded(ctx, [&] {
+    if (ctx->device < 0 || ctx->device >= kMaxDevices)
+      throw Fail{DMSC_ERR_INVALID_ARGUMENT, "the allocation-free backward supports devices 0.." +
+                                                std::to_string(kMaxDevices - 1)};
+    // no timing events here: their pool may create events
+    KernelTimes* saved = tl_times;
+    tl_times = nullptr;
+    struct Restore {
+      KernelTimes* t;
+      ~Restore() { tl_times = t; }
+    } restore{saved};
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    float* s = g_scratch_ptr[ctx->device];
+    if (!s) throw Fail{DMSC_ERR_HIP, "the allocation-free scratch is not set up on this device"};
+    run_backward(ctx, B, d_fields, H, W, pairs, d_grad_scale, d_grad, flags, max_persistence, s,
+                 kNoAllocScratchFloats, g_scratch_dirty[ctx->device], (hipStream_t)stream);
   });
 }
 

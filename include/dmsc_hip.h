@@ -119,6 +119,22 @@ dmsc_status dmsc_persistence_backward(dmsc_context* ctx, int32_t B, const float*
                                       float max_persistence, const float* d_grad_scale, float* const* d_grad,
                                       dmsc_stream_t stream);
 
+/* dmsc_persistence_backward without any device allocation: no counterpart in the reference. For
+ * callers that cannot allocate device memory: code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc. Same arguments, results and ordering as
+ * dmsc_persistence_backward (the same kernels, bit for bit), but the scratch is the library's
+ * fixed per-device array of DMSC_NOALLOC_SCRATCH_FLOATS floats (64 MiB, held by every GPU the
+ * library runs on until the process ends), set up by dmsc_context_create. A field needs 9 H W of
+ * them (at most 1864135 pixels); a larger field returns DMSC_ERR_TOO_LARGE before anything is
+ * queued; fields are processed in as many chunks as the array needs. The array is all zeros again
+ * after each call. Every call on a GPU shares that GPU's array: calls on one context are ordered,
+ * but calls from different contexts that overlap on one GPU race. Devices 0-63 only. */
+enum { DMSC_NOALLOC_SCRATCH_FLOATS = 1 << 24 };
+dmsc_status dmsc_persistence_backward_noalloc(dmsc_context* ctx, int32_t B, const float* const* d_fields,
+                                              const int32_t* H, const int32_t* W, uint32_t flags,
+                                              const dmsc_pairs* pairs, float max_persistence,
+                                              const float* d_grad_scale, float* const* d_grad, dmsc_stream_t stream);
+
 /* Loss of each field: d_loss[b] = sum over the pairs of field b with 0 <= persistence <
  * max_persistence of persistence^2 (both sides; summed in double in a fixed order, so the result
  * is reproducible), as float32 on the device. One kernel launch per 16 fields, asynchronous on
