@@ -47,6 +47,8 @@ def parse():
     p.add_argument("--workload", default="mixed", choices=["mixed", "gauss4096", "noise8192", "diff"])
     p.add_argument("--headline-only", action="store_true",
                    help="mixed workload without the 8192^2 and 4096^2 configurations beside it")
+    p.add_argument("--no-plain", action="store_true",
+                   help="mixed workload without the per-field plain loop (for profiling the batched calls alone)")
     p.add_argument("--cpu-table", action="store_true",
                    help="only time the CPU oracle on every configuration (all allotted cores and 1 thread)")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
@@ -311,7 +313,8 @@ def main():
         bench_diff(args, torch, dmsc_amd, dmsc_gpu)
         return
     headline = args.workload == "mixed"
-    m = measure(args.workload, args.steps, args.warmup, plain_steps=3 if headline else 0, profile=args.profile)
+    m = measure(args.workload, args.steps, args.warmup, plain_steps=3 if headline and not args.no_plain else 0,
+                profile=args.profile)
     config = {"workload": m["desc"], "extractions_per_step": m["extractions_per_step"],
               "pixels_per_step": m["pixels_per_step"]}
     if "plain_loop_Mpix_s" in m:

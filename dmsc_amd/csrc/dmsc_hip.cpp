@@ -118,6 +118,7 @@ struct dmsc_context {
   int64_t S = 0, N = 0;
   int64_t nmin = 0, nsad = 0, nmax = 0;
   int rounds_persist = 0, rounds_geo[2] = {0, 0};
+  int64_t arc_stats[4] = {0, 0, 0, 0};  // per traced side (vertices, faces): raw pieces copied, output cells
   std::vector<int32_t> pending_log;  // (round, pending min side, pending max side) at each readback
   bool dual = false;
   const int32_t *min_ids = nullptr, *sad_ids = nullptr, *max_ids = nullptr, *asc = nullptr;
@@ -625,6 +626,7 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
   launch_persist_init(ps[0], ps[1], nmin, nmax, nsad, sb[0].cancel, sb[1].cancel, removed, st);
   int round = 0;
   ctx->pending_log.clear();
+  std::fill(std::begin(ctx->arc_stats), std::end(ctx->arc_stats), int64_t(0));
   int32_t* d_round = c32 + 50;  // round number for graph replays
   CapturedRounds pgraph, ggraph;
   if (nsad > 0) {
@@ -1009,6 +1011,8 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     Expansion& e = ex[s];
     if (!w.on) continue;
     const int32_t nraw = pin32(s)[3];
+    ctx->arc_stats[2 * s] = nraw;
+    ctx->arc_stats[2 * s + 1] = w.out_total;
     if (nraw > e.cap) throw Fail{DMSC_ERR_INTERNAL, "arc segment list overflow"};
     if (nraw > 0)
       launch_copy_raw(w.A, e.raw, e.nseg + 3, std::min<int64_t>(nraw, e.cap), w.out_cells, w.out_total, d_err, e.es);
@@ -1407,6 +1411,11 @@ int64_t dmsc_debug_fetch(dmsc_context* ctx, const char* name, void* dst, int64_t
     int32_t v[3] = {ctx->rounds_persist, ctx->rounds_geo[0], ctx->rounds_geo[1]};
     if (dst) std::memcpy(dst, v, sizeof(int32_t) * (size_t)std::min<int64_t>(3, cap));
     return 3;
+  }
+  if (n == "arc_stats") {
+    const int64_t m = std::min<int64_t>(4, cap);
+    if (dst && m > 0) std::memcpy(dst, ctx->arc_stats, (size_t)m * sizeof(int64_t));
+    return 4;
   }
   if (n == "persist_pending") {
     int64_t m = std::min<int64_t>((int64_t)ctx->pending_log.size(), cap);

@@ -178,7 +178,8 @@ const char* dmsc_last_error(dmsc_context* ctx);
  * or -1 for an unknown name; copies min(count, capacity) elements. Names: "counts" (n_min, n_sad,
  * n_max), "min_ids", "sad_ids", "max_ids", "face_term", "face_depth", "vert_term", "vert_depth",
  * "asc_order", "sad_cmax", "sad_cmin", "pers_min" (float), "pers_max" (float), "pair_min",
- * "pair_max", "rounds" (persistence rounds, geometry rounds). */
+ * "pair_max", "rounds" (persistence rounds, geometry rounds), "arc_stats" (int64: raw pieces copied
+ * and output arc cells of the vertex side, then of the face side). */
 int64_t dmsc_debug_fetch(dmsc_context* ctx, const char* name, void* host_dst, int64_t capacity);
 
 #ifdef __cplusplus
