@@ -1,6 +1,6 @@
-"""Autotuner for the scheduling parameters of the HIP pipeline (round kernel grid cap, one-item-per-
-thread threshold, host-check cadence of the round and expansion loops, hipGraph replay of round
-batches).
+"""Autotuner for the scheduling parameters of the HIP pipeline (gradient tile height, round kernel
+grid cap, one-item-per-thread threshold, host-check cadence of the round and expansion loops,
+hipGraph replay of round batches).
 
 No reference counterpart. The parameters change only how work is scheduled, never the results: the
 tuner checks that every candidate gives outputs identical to the defaults' and rejects it otherwise.
@@ -23,7 +23,7 @@ from .csrc import dmsc_gpu
 
 TUNING_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning")
 DEFAULTS = {"round_grid": 2048, "small_list": 65536, "rounds_first": 4, "rounds_max": 16, "expand_levels": 8,
-            "graph_rounds": 0, "tail_items": 2048}
+            "graph_rounds": 0, "tail_items": 2048, "grad_rows": 16}
 CANDIDATES = {
     "round_grid": [512, 1024, 2048, 4096, 8192, 16384],
     "small_list": [4096, 16384, 65536, 262144],
@@ -32,6 +32,7 @@ CANDIDATES = {
     "expand_levels": [4, 8, 16],
     "graph_rounds": [0, 1],
     "tail_items": [0, 512, 2048, 8192],
+    "grad_rows": [8, 16, 32],
 }
 _applied = set()
 

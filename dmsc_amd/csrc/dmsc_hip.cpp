@@ -422,7 +422,7 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     word_b[f] = PW;
     thr[f] = fin[f].thr;
     tile_b[f] = T;
-    T += (int64_t)ceil_div(d.Ny, 16) * ceil_div(d.Nx, 64);  // gradient tiles of 64 x 16 cells, row-major
+    T += gradient_tiles(d.H, d.W, ctx->tune.grad_rows);  // gradient tiles, row-major within the field
     S += d.S;
     N += d.N;
     PW += d.PW;
@@ -482,8 +482,8 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
   uint32_t* planes = ar.get<uint32_t>(A_PLANES, 4 * PW);
   uint64_t* pjf = ar.get<uint64_t>(A_PJF, S);
   uint64_t* pjv = ar.get<uint64_t>(A_PJV, N);
-  launch_gradient(dual, bt, d_tile_b, T, grad ? o_grad : nullptr, fsucc, vsucc, planes, PW, pjf, pjv,
-                  st);
+  launch_gradient(dual, ctx->tune.grad_rows, bt, d_tile_b, T, grad ? o_grad : nullptr, fsucc, vsucc, planes, PW, pjf,
+                  pjv, st);
 
   int32_t* c32 = ar.get<int32_t>(A_C32, 64);
   int64_t* c64 = ar.get<int64_t>(A_C64, 64);
@@ -1336,6 +1336,7 @@ int* param_slot(dmsc_context* ctx, const std::string& n, int64_t& lo, int64_t& h
   if (n == "expand_levels") { lo = 1; hi = 4096; return &t.expand_levels; }
   if (n == "graph_rounds") { lo = 0; hi = 1; return &t.graph_rounds; }
   if (n == "tail_items") { lo = 0; hi = 1 << 16; return &t.tail_items; }
+  if (n == "grad_rows") { lo = 8; hi = 32; return &t.grad_rows; }
   return nullptr;
 }
 }  // namespace
@@ -1346,6 +1347,7 @@ dmsc_status dmsc_set_param(dmsc_context* ctx, const char* name, int64_t value) {
   int64_t lo = 0, hi = 0;
   int* p = param_slot(ctx, name, lo, hi);
   if (!p || value < lo || value > hi) return DMSC_ERR_INVALID_ARGUMENT;
+  if (p == &ctx->tune.grad_rows && value != 8 && value != 16 && value != 32) return DMSC_ERR_INVALID_ARGUMENT;
   *p = (int)value;
   return DMSC_OK;
 }

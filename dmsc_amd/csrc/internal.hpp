@@ -123,6 +123,7 @@ struct Tunables {
   int graph_rounds = 0;      // 1: each batch of rounds is a captured hipGraph, replayed while it fits
   int tail_items = 2048;     // pending items at a host check up to which one workgroup runs every
                              // remaining round in one launch (0: never)
+  int grad_rows = 16;        // slot rows of a gradient tile (8, 16 or 32; tiles are 64 slots wide)
 };
 extern thread_local const Tunables* tl_tune;
 inline const Tunables& tune() {
@@ -358,9 +359,12 @@ void bases_at(const T* pre, const S* starts, int64_t n, const T* total, int B, T
 
 // grid stages (over all fields of a batch)
 // also writes the initial pointer-jumping state of both V-path forests (pjf over slots, pjv over pixels)
-// tile_b: [B+1] first gradient tile of each field (64 x 16 cells per tile, row-major within a field)
-void launch_gradient(bool dual, const Batch& bt, const int64_t* tile_b, int64_t ntiles, int32_t* pw, int32_t* fsucc,
-                     int32_t* vsucc, uint32_t* planes, int64_t PW, uint64_t* pjf, uint64_t* pjv, hipStream_t st);
+// tile_b: [B+1] first gradient tile of each field (64 x rows slots per tile, row-major within a field;
+// rows is 8, 16 or 32)
+int gradient_tiles(int H, int W, int rows);
+void launch_gradient(bool dual, int rows, const Batch& bt, const int64_t* tile_b, int64_t ntiles, int32_t* pw,
+                     int32_t* fsucc, int32_t* vsucc, uint32_t* planes, int64_t PW, uint64_t* pjf, uint64_t* pjv,
+                     hipStream_t st);
 // min_key / max_key: pack_key(value, cell id) per extremum; list_b: rows of B+1 per-field list bases
 // (minima, saddles, maxima)
 void launch_compact_lists(bool dual, const RankView& rv, const int32_t* list_b, int32_t* min_ids, uint64_t* min_key,

@@ -1,5 +1,7 @@
 // Grid-shaped stages: discrete gradient + critical bit planes (one LDS-tiled stencil pass),
 // order-preserving critical lists, saddle records and basin/peak labels.
+#include <type_traits>
+
 #include "internal.hpp"
 
 namespace dmsc {
@@ -7,28 +9,90 @@ namespace dmsc {
 __constant__ constexpr PairLut c_lut = make_pair_lut();
 
 // ------------------------------------------------------------------ K1: gradient
-// One 64 x 16 tile of slots per 256-thread block (a wave per slot row). Phase 1 evaluates the lower
-// star of every vertex whose star touches the tile ((16+1) x (64+1) vertices, pixels staged in
-// LDS with a 2-pixel halo) and scatters the pairing of each lower-star cell into an LDS cell tile;
-// every valid cell lies in exactly one lower star, so there are no write conflicts. Phase 2 writes
-// each slot's four partners as one 16-byte store, the face / vertex V-path successors, and the
-// four critical bit planes via wave ballots. Phase 3 follows every face and vertex V-path inside
-// the tile by pointer doubling in LDS (at most 1024 steps, so 10 doublings) and writes the initial
-// pointer-jumping state: (in-tile root or first node outside the tile, steps to it). The global
-// pointer jumping then moves one tile crossing per step instead of one cell.
-constexpr int GT_X = 64, GT_Y = 16;
-constexpr int GP_W = GT_X + 3, GP_H = GT_Y + 3;
+constexpr int GT_X = 64;
 
-template <bool DUAL>
+// In-tile V-path state of phase 3, one 32-bit word per node: the node it points to (local index),
+// an exit flag (that node's successor leaves the tile; its global id is kept in the tile's exit
+// table) and the steps to get there. A node is final when it points to itself (a root or an
+// invalid slot) or to an exit node. Both forests' words of a node share one 64-bit LDS entry.
+constexpr uint32_t kPjNodeBits = 11;  // tiles of up to 2048 slots
+constexpr uint32_t kPjNodeMask = (1u << kPjNodeBits) - 1u, kPjExit = 1u << kPjNodeBits;
+constexpr int kPjStepShift = kPjNodeBits + 1;
+__device__ __forceinline__ uint32_t pj_word(int node, bool exit, uint32_t steps) {
+  return (uint32_t)node | (exit ? kPjExit : 0u) | steps << kPjStepShift;
+}
+__device__ __forceinline__ int pj_node(uint32_t w) { return (int)(w & kPjNodeMask); }
+__device__ __forceinline__ bool pj_final(uint32_t w, int self) { return (w & kPjExit) || pj_node(w) == self; }
+// w points to node n whose word is w2: w now points where w2 does, steps added
+__device__ __forceinline__ uint32_t pj_join(uint32_t w, uint32_t w2) {
+  return (w2 & (kPjNodeMask | kPjExit)) | ((w >> kPjStepShift) + (w2 >> kPjStepShift)) << kPjStepShift;
+}
+
+constexpr int log2_exact(int n) { return n <= 1 ? 0 : 1 + log2_exact(n / 2); }
+
+// Lower star of one vertex as a 32-bit summary: edge mask (bits 0-3, L_edges order), face mask
+// (bits 4-7, L_faces order), paired edge estar (bits 8-10, 4 = none) and the face-queue result
+// (bits 11-18, 2 bits per face: 0 unpaired, 1 + which of its two edges). A vertex outside the image
+// has an empty star.
+constexpr uint32_t kEmptyStar = 4u << 8;
+__device__ __forceinline__ uint32_t star_word(const LowerStar& ls) {
+  const uint32_t idx = (uint32_t)(ls.emask | ls.fmask << 4 | ls.estar << 8);
+  return idx | (uint32_t)c_lut.v[idx] << 11;
+}
+// Partner of edge e of the star of vertex slot qo: the vertex itself when e is its paired edge, the
+// face the queue paired with e, or -1 (critical).
+__device__ __forceinline__ int star_edge_partner(uint32_t w, int e, int qo, int Nx) {
+  int partner = ((w >> 8) & 7) == (uint32_t)e ? 4 * qo : -1;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int cj = (int)(w >> (11 + 2 * j)) & 3;
+    if (cj && face_edge(j, cj - 1) == e) partner = 4 * (qo + (j & 1) + (j >> 1) * Nx) + 3;
+  }
+  return partner;
+}
+// Partner of vertex qo itself: its paired edge (the summary's estar is < 4).
+__device__ __forceinline__ int star_edge_partner_vertex(uint32_t w, int qo, int Nx) {
+  const int e = (int)(w >> 8) & 7;
+  return 4 * (qo + (e == 1 ? 1 : 0) + (e == 3 ? Nx : 0)) + (e < 2 ? 1 : 2);
+}
+// Partner of face j of the star of vertex slot qo: the edge the queue paired it with, or -1.
+__device__ __forceinline__ int star_face_partner(uint32_t w, int j, int qo, int Nx) {
+  const int cj = (int)(w >> (11 + 2 * j)) & 3;
+  if (!cj) return -1;
+  const int e = face_edge(j, cj - 1);
+  return 4 * (qo + (e == 1 ? 1 : 0) + (e == 3 ? Nx : 0)) + (e < 2 ? 1 : 2);
+}
+
+// One GT_X x TY tile of slots per 256-thread block (a wave per slot row). Phase 1 evaluates the lower
+// star of every vertex that can own a cell of the tile ((TY+1) x (GT_X+1) vertices, from pixels
+// staged in LDS with a 2-pixel halo) and keeps one summary word per vertex; tiles whose vertices'
+// neighbourhoods all lie inside the image (almost all of them) take a path with every existence
+// test folded away. Phase 2 gives each slot the partners of its four cells from the summaries of the
+// vertices that can own them (every valid cell lies in the lower star of exactly one vertex: its
+// vertex, its edges' and face's highest vertex among the slot's vertex and its left, upper and
+// upper-left neighbours) and writes them as one 16-byte store, the face / vertex V-path successors,
+// and the four critical bit planes via wave ballots. Phase 3 follows every face and vertex V-path
+// inside the tile by pointer doubling in LDS (log2 of the tile's slots at most) and writes the
+// initial pointer-jumping state: (in-tile root or first node outside the tile, steps to it). The
+// global pointer jumping then moves one tile crossing per step instead of one cell.
+template <bool DUAL, int TY>
 __global__ __launch_bounds__(256) void k_gradient(Batch bt, const int64_t* __restrict__ tile_b,
                                                   int32_t* __restrict__ pw_all, int32_t* __restrict__ fsucc_all,
                                                   int32_t* __restrict__ vsucc_all, uint32_t* __restrict__ planes,
                                                   int64_t PW, uint64_t* __restrict__ pjf_all,
                                                   uint64_t* __restrict__ pjv_all) {
-  __shared__ float pix[GP_H][GP_W];
-  __shared__ int32_t cell[GT_Y][GT_X][4];  // reused by phase 3 as its second buffers
-  __shared__ int32_t fJ[GT_Y * GT_X], fD[GT_Y * GT_X], vJ[GT_Y * GT_X], vD[GT_Y * GT_X];
-  __shared__ uint8_t lut[kLutSize];
+  constexpr int GP_W = GT_X + 3, GP_H = TY + 3, TN = TY * GT_X;
+  constexpr int SV_W = GT_X + 1, SV_H = TY + 1;
+  static_assert(TN <= (1 << kPjNodeBits), "tile too large for the packed phase-3 words");
+  // phases 1-2: staged pixels and star summaries; phase 3 reuses that space as its second buffer
+  constexpr int kStageBytes = (GP_H * GP_W + SV_H * SV_W) * 4;
+  constexpr int kBufBytes = TN * 8;
+  __shared__ __align__(16) unsigned char stage[kStageBytes > kBufBytes ? kStageBytes : kBufBytes];
+  __shared__ uint64_t pjA[TN];      // phase 3: (face word, vertex word) per node
+  __shared__ int32_t exits[2][TN];  // global successor of each exit node (faces, vertices)
+  float (*pix)[GP_W] = reinterpret_cast<float (*)[GP_W]>(stage);
+  uint32_t (*sv)[SV_W] = reinterpret_cast<uint32_t (*)[SV_W]>(stage + GP_H * GP_W * 4);
+
   const int tfield = field_of(tile_b, bt.B, blockIdx.x);
   const FieldDesc fd = bt.fd[tfield];
   const int64_t tlocal = blockIdx.x - tile_b[tfield];
@@ -37,92 +101,78 @@ __global__ __launch_bounds__(256) void k_gradient(Batch bt, const int64_t* __res
   const float* __restrict__ f = fd.f;
   const int H = fd.H, W = fd.W, WPR = fd.WPR;
   const int Nx = W + 1, Ny = H + 1;
-  const int x0 = tr.tx * GT_X, y0 = tr.ty * GT_Y;
+  const int x0 = tr.tx * GT_X, y0 = tr.ty * TY;
   const int sbase = (int)fd.slot_base, pbase = (int)fd.pix_base;  // batch totals fit int32 (host check)
   int32_t* __restrict__ pw = pw_all ? pw_all + 4 * fd.slot_base : nullptr;
   int32_t* __restrict__ fsucc = fsucc_all + fd.slot_base;
   int32_t* __restrict__ vsucc = vsucc_all + fd.pix_base;
   const int64_t wbase = fd.word_base;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
 
-  for (int i = tid; i < kLutSize; i += 256) lut[i] = c_lut.v[i];
-  for (int i = tid; i < GP_H * GP_W; i += 256) {
-    int r = i / GP_W, c = i - r * GP_W;
-    int py = y0 - 2 + r, px = x0 - 2 + c;
-    pix[r][c] = (py >= 0 && py < H && px >= 0 && px < W) ? f[(int64_t)py * W + px] : 0.0f;
+  // pixels (y0-2 .. y0+TY, x0-2 .. x0+GT_X), a wave per row
+  for (int r = wv; r < GP_H; r += 4) {
+    const int py = y0 - 2 + r;
+    for (int c = lane; c < GP_W; c += 64) {
+      const int px = x0 - 2 + c;
+      pix[r][c] = (py >= 0 && py < H && px >= 0 && px < W) ? f[(int64_t)py * W + px] : 0.0f;
+    }
   }
-  for (int i = tid; i < GT_Y * GT_X * 4; i += 256) (&cell[0][0][0])[i] = -1;
   __syncthreads();
 
-  for (int i = tid; i < (GT_Y + 1) * (GT_X + 1); i += 256) {
-    const int ly = i / (GT_X + 1), lx = i - ly * (GT_X + 1);
+  // phase 1: star summary of vertex (y0-1+ly, x0-1+lx), a wave per vertex row (lane = column) and the
+  // last column after
+  const bool interior = y0 >= 2 && x0 >= 2 && y0 + TY <= H - 1 && x0 + GT_X <= W - 1;
+  auto star_at = [&](int ly, int lx, auto interior_tag) -> uint32_t {
+    constexpr bool IN = decltype(interior_tag)::value;
     const int vy = y0 - 1 + ly, vx = x0 - 1 + lx;
-    if (vy < 0 || vy >= H || vx < 0 || vx >= W) continue;
+    if (!IN && (vy < 0 || vy >= H || vx < 0 || vx >= W)) return kEmptyStar;
     float nb[3][3];
     bool ex[3][3];
 #pragma unroll
     for (int dy = -1; dy <= 1; ++dy)
 #pragma unroll
       for (int dx = -1; dx <= 1; ++dx) {
-        int py = vy + dy, px = vx + dx;
-        ex[dy + 1][dx + 1] = (py >= 0 && py < H && px >= 0 && px < W);
+        const int py = vy + dy, px = vx + dx;
+        ex[dy + 1][dx + 1] = IN || (py >= 0 && py < H && px >= 0 && px < W);
         nb[dy + 1][dx + 1] = pix[ly + dy + 1][lx + dx + 1];
       }
-    const LowerStar ls = lower_star<DUAL>(nb, ex);
-    const int code = lut[ls.emask | ls.fmask << 4 | ls.estar << 8];
-    const int q = vy * Nx + vx;
-    // edge e of v: left H(y,x), right H(y,x+1), up V(y,x), down V(y+1,x); face j: UL, UR, DL, DR
-    auto eid = [&](int e) { return 4 * (q + (e == 1 ? 1 : 0) + (e == 3 ? Nx : 0)) + (e < 2 ? 1 : 2); };
-    auto fid = [&](int j) { return 4 * (q + (j & 1) + (j >> 1) * Nx) + 3; };
-    const int sy = vy - y0, sx = vx - x0;
-    if (sy >= 0 && sx >= 0) {
-      int p = -1;
-#pragma unroll
-      for (int e = 0; e < 4; ++e)
-        if (ls.estar == e) p = eid(e);
-      cell[sy][sx][0] = p;
-    }
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      if (!(ls.emask >> e & 1)) continue;
-      const int ty = sy + (e == 3 ? 1 : 0), tx = sx + (e == 1 ? 1 : 0);
-      if (ty < 0 || ty >= GT_Y || tx < 0 || tx >= GT_X) continue;
-      int partner = -1;
-      if (e == ls.estar) partner = 4 * q;
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int cj = (code >> (2 * j)) & 3;
-        if (cj && face_edge(j, cj - 1) == e) partner = fid(j);
-      }
-      cell[ty][tx][e < 2 ? 1 : 2] = partner;
-    }
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      if (!(ls.fmask >> j & 1)) continue;
-      const int ty = sy + (j >> 1), tx = sx + (j & 1);
-      if (ty < 0 || ty >= GT_Y || tx < 0 || tx >= GT_X) continue;
-      const int cj = (code >> (2 * j)) & 3;
-      int partner = -1;
-#pragma unroll
-      for (int e = 0; e < 4; ++e)
-        if (cj && face_edge(j, cj - 1) == e) partner = eid(e);
-      cell[ty][tx][3] = partner;
-    }
-  }
+    return star_word(lower_star<DUAL>(nb, ex));
+  };
+  auto phase1 = [&](auto interior_tag) {
+    for (int ly = wv; ly < SV_H; ly += 4) sv[ly][lane] = star_at(ly, lane, interior_tag);
+    if (tid < SV_H) sv[tid][GT_X] = star_at(tid, GT_X, interior_tag);
+  };
+  if (interior) phase1(std::true_type{});
+  else phase1(std::false_type{});
   __syncthreads();
 
-  const int lane = tid & 63, wv = tid >> 6;
-  for (int r = wv; r < GT_Y; r += 4) {
+  for (int r = wv; r < TY; r += 4) {
     const int y = y0 + r, x = x0 + lane;
     const bool in = (y < Ny) && (x < Nx);
-    const int c0 = cell[r][lane][0], c1 = cell[r][lane][1], c2 = cell[r][lane][2], c3 = cell[r][lane][3];
+    int c0 = -1, c1 = -1, c2 = -1, c3 = -1;
+    if (in) {
+      const int q = y * Nx + x;
+      // summaries of the vertices (y, x), (y, x-1), (y-1, x), (y-1, x-1)
+      const uint32_t w00 = sv[r + 1][lane + 1], wl = sv[r + 1][lane], wu = sv[r][lane + 1], wul = sv[r][lane];
+      c0 = ((w00 >> 8) & 7) < 4 ? star_edge_partner_vertex(w00, q, Nx) : -1;
+      // H-edge (y, x): the left edge of (y, x) or the right edge of (y, x-1)
+      if (w00 & 1) c1 = star_edge_partner(w00, 0, q, Nx);
+      else if (wl & 2) c1 = star_edge_partner(wl, 1, q - 1, Nx);
+      // V-edge (y, x): the up edge of (y, x) or the down edge of (y-1, x)
+      if (w00 & 4) c2 = star_edge_partner(w00, 2, q, Nx);
+      else if (wu & 8) c2 = star_edge_partner(wu, 3, q - Nx, Nx);
+      // face (y, x): UL of (y, x), UR of (y, x-1), DL of (y-1, x) or DR of (y-1, x-1)
+      if (w00 & 16) c3 = star_face_partner(w00, 0, q, Nx);
+      else if (wl & 32) c3 = star_face_partner(wl, 1, q - 1, Nx);
+      else if (wu & 64) c3 = star_face_partner(wu, 2, q - Nx, Nx);
+      else if (wul & 128) c3 = star_face_partner(wul, 3, q - Nx - 1, Nx);
+    }
     const bool k0 = in && c0 < 0 && y < H && x < W;
     const bool k1 = in && c1 < 0 && y < H;
     const bool k2 = in && c2 < 0 && x < W;
     const bool k3 = in && c3 < 0;
-    // phase-3 state: local successor index inside the tile, -(global id) - 1 outside, self = root
     const int e = r * GT_X + lane;
-    int fje = e, fde = 0, vje = e, vde = 0;
+    uint32_t fw = pj_word(e, false, 0), vw = pj_word(e, false, 0);  // itself: a root or an invalid slot
     if (in) {
       const int q = y * Nx + x;
       if (pw) *reinterpret_cast<int4*>(pw + 4 * (int64_t)q) = make_int4(c0, c1, c2, c3);
@@ -133,8 +183,12 @@ __global__ __launch_bounds__(256) void k_gradient(Batch bt, const int64_t* __res
         else if (c3 == 4 * (q - 1) + 2) { fs = q - 1; dx = -1; }
         else { fs = q + 1; dx = 1; }
         const int rr = r + dy, ll = lane + dx;
-        fde = 1;
-        fje = (rr >= 0 && rr < GT_Y && ll >= 0 && ll < GT_X) ? rr * GT_X + ll : -(sbase + fs) - 1;
+        if (rr >= 0 && rr < TY && ll >= 0 && ll < GT_X) {
+          fw = pj_word(rr * GT_X + ll, false, 1);
+        } else {
+          fw = pj_word(e, true, 1);
+          exits[0][e] = sbase + fs;
+        }
       }
       fsucc[q] = sbase + fs;
       if (y < H && x < W) {
@@ -148,16 +202,17 @@ __global__ __launch_bounds__(256) void k_gradient(Batch bt, const int64_t* __res
           else if (c0 == 4 * q + 2) { vs = p - W; dy = -1; }
           else { vs = p + W; dy = 1; }
           const int rr = r + dy, ll = lane + dx;
-          vde = 1;
-          vje = (rr >= 0 && rr < GT_Y && ll >= 0 && ll < GT_X) ? rr * GT_X + ll : -(pbase + vs) - 1;
+          if (rr >= 0 && rr < TY && ll >= 0 && ll < GT_X) {
+            vw = pj_word(rr * GT_X + ll, false, 1);
+          } else {
+            vw = pj_word(e, true, 1);
+            exits[1][e] = pbase + vs;
+          }
         }
         vsucc[p] = pbase + vs;
       }
     }
-    fJ[e] = fje;
-    fD[e] = fde;
-    vJ[e] = vje;
-    vD[e] = vde;
+    pjA[e] = (uint64_t)fw | (uint64_t)vw << 32;
     const uint64_t b0 = __ballot(k0), b1 = __ballot(k1), b2 = __ballot(k2), b3 = __ballot(k3);
     if (y < Ny && lane < 8) {
       const int t = lane >> 1, half = lane & 1;
@@ -168,73 +223,81 @@ __global__ __launch_bounds__(256) void k_gradient(Batch bt, const int64_t* __res
   }
   __syncthreads();
 
-  // phase 3: pointer doubling inside the tile (double-buffered through the phase-1 cell tile)
-  constexpr int TN = GT_Y * GT_X;
-  int32_t* fJb = &cell[0][0][0];
-  int32_t* fDb = fJb + TN;
-  int32_t* vJb = fJb + 2 * TN;
-  int32_t* vDb = fJb + 3 * TN;
-  int32_t *fJa = fJ, *fDa = fD, *vJa = vJ, *vDa = vD;
+  // phase 3: pointer doubling inside the tile, both forests per 64-bit word (double-buffered through
+  // the staging space)
+  uint64_t* A = pjA;
+  uint64_t* Bf = reinterpret_cast<uint64_t*>(stage);
 #pragma unroll 1
-  for (int it = 0; it < 10; ++it) {
-    int moved = 0;  // a pass in which no pointer moves leaves every chain resolved
-    for (int e = tid; e < TN; e += 256) {
-      int j = fJa[e], d = fDa[e];
-      if (j >= 0 && j != e) {
-        const int jj = fJa[j];
-        moved |= jj != j;
-        d += fDa[j];
-        j = jj;
+  for (int it = 0; it < log2_exact(TN); ++it) {
+    int moved = 0;  // a pass in which no word can advance further leaves every chain resolved
+#pragma unroll
+    for (int k = 0; k < TN / 256; ++k) {
+      const int e = tid + 256 * k;
+      const uint64_t w = A[e];
+      uint32_t fwd = (uint32_t)w, vwd = (uint32_t)(w >> 32);
+      if (!pj_final(fwd, e)) {
+        const int n = pj_node(fwd);
+        const uint32_t w2 = (uint32_t)A[n];
+        moved |= !pj_final(w2, n);
+        fwd = pj_join(fwd, w2);
       }
-      fJb[e] = j;
-      fDb[e] = d;
-      j = vJa[e];
-      d = vDa[e];
-      if (j >= 0 && j != e) {
-        const int jj = vJa[j];
-        moved |= jj != j;
-        d += vDa[j];
-        j = jj;
+      if (!pj_final(vwd, e)) {
+        const int n = pj_node(vwd);
+        const uint32_t w2 = (uint32_t)(A[n] >> 32);
+        moved |= !pj_final(w2, n);
+        vwd = pj_join(vwd, w2);
       }
-      vJb[e] = j;
-      vDb[e] = d;
+      Bf[e] = (uint64_t)fwd | (uint64_t)vwd << 32;
     }
     const int any = __syncthreads_or(moved);
-    int32_t* t;
-    t = fJa; fJa = fJb; fJb = t;
-    t = fDa; fDa = fDb; fDb = t;
-    t = vJa; vJa = vJb; vJb = t;
-    t = vDa; vDa = vDb; vDb = t;
+    uint64_t* t = A;
+    A = Bf;
+    Bf = t;
     if (!any) break;
   }
   uint64_t* __restrict__ pjf = pjf_all + fd.slot_base;
   uint64_t* __restrict__ pjv = pjv_all + fd.pix_base;
-  for (int e = tid; e < TN; e += 256) {
-    const int r = e / GT_X, c = e - r * GT_X;
-    const int y = y0 + r, x = x0 + c;
+  for (int r = wv; r < TY; r += 4) {
+    const int e = r * GT_X + lane;
+    const int y = y0 + r, x = x0 + lane;
     if (y >= Ny || x >= Nx) continue;
     const int q = y * Nx + x;
-    int j = fJa[e];
-    uint32_t target = j >= 0 ? (uint32_t)(sbase + (y0 + j / GT_X) * Nx + x0 + j % GT_X) : (uint32_t)(-j - 1);
-    pjf[q] = (uint64_t)target | ((uint64_t)(uint32_t)fDa[e] << 32);
+    const uint64_t w = A[e];
+    uint32_t wd = (uint32_t)w;
+    int n = pj_node(wd);
+    uint32_t target = (wd & kPjExit) ? (uint32_t)exits[0][n] : (uint32_t)(sbase + (y0 + n / GT_X) * Nx + x0 + n % GT_X);
+    pjf[q] = (uint64_t)target | (uint64_t)(wd >> kPjStepShift) << 32;
     if (y < H && x < W) {
-      j = vJa[e];
-      target = j >= 0 ? (uint32_t)(pbase + (y0 + j / GT_X) * W + x0 + j % GT_X) : (uint32_t)(-j - 1);
-      pjv[y * W + x] = (uint64_t)target | ((uint64_t)(uint32_t)vDa[e] << 32);
+      wd = (uint32_t)(w >> 32);
+      n = pj_node(wd);
+      target = (wd & kPjExit) ? (uint32_t)exits[1][n] : (uint32_t)(pbase + (y0 + n / GT_X) * W + x0 + n % GT_X);
+      pjv[y * W + x] = (uint64_t)target | (uint64_t)(wd >> kPjStepShift) << 32;
     }
   }
 }
 
-void launch_gradient(bool dual, const Batch& bt, const int64_t* tile_b, int64_t ntiles, int32_t* pw, int32_t* fsucc,
-                     int32_t* vsucc, uint32_t* planes, int64_t PW, uint64_t* pjf, uint64_t* pjv, hipStream_t st) {
+int gradient_tiles(int H, int W, int rows) { return ceil_div(H + 1, rows) * ceil_div(W + 1, GT_X); }
+
+template <bool DUAL>
+static void launch_gradient_rows(int rows, dim3 grid, hipStream_t st, const Batch& bt, const int64_t* tile_b,
+                                 int32_t* pw, int32_t* fsucc, int32_t* vsucc, uint32_t* planes, int64_t PW,
+                                 uint64_t* pjf, uint64_t* pjv) {
+  if (rows == 8)
+    hipLaunchKernelGGL((k_gradient<DUAL, 8>), grid, dim3(256), 0, st, bt, tile_b, pw, fsucc, vsucc, planes, PW, pjf, pjv);
+  else if (rows == 32)
+    hipLaunchKernelGGL((k_gradient<DUAL, 32>), grid, dim3(256), 0, st, bt, tile_b, pw, fsucc, vsucc, planes, PW, pjf, pjv);
+  else
+    hipLaunchKernelGGL((k_gradient<DUAL, 16>), grid, dim3(256), 0, st, bt, tile_b, pw, fsucc, vsucc, planes, PW, pjf, pjv);
+}
+
+void launch_gradient(bool dual, int rows, const Batch& bt, const int64_t* tile_b, int64_t ntiles, int32_t* pw,
+                     int32_t* fsucc, int32_t* vsucc, uint32_t* planes, int64_t PW, uint64_t* pjf, uint64_t* pjv,
+                     hipStream_t st) {
   KScope ks_("gradient", st, true);
   if (ntiles == 0) return;
-  if (dual)
-    hipLaunchKernelGGL(k_gradient<true>, dim3((unsigned)ntiles), dim3(256), 0, st, bt, tile_b, pw, fsucc, vsucc, planes, PW,
-                       pjf, pjv);
-  else
-    hipLaunchKernelGGL(k_gradient<false>, dim3((unsigned)ntiles), dim3(256), 0, st, bt, tile_b, pw, fsucc, vsucc, planes, PW,
-                       pjf, pjv);
+  const dim3 grid((unsigned)ntiles);
+  if (dual) launch_gradient_rows<true>(rows, grid, st, bt, tile_b, pw, fsucc, vsucc, planes, PW, pjf, pjv);
+  else launch_gradient_rows<false>(rows, grid, st, bt, tile_b, pw, fsucc, vsucc, planes, PW, pjf, pjv);
   DMSC_LAUNCH_CHECK();
 }
 

@@ -165,6 +165,7 @@ dmsc_status dmsc_profile_entry(dmsc_context* ctx, int32_t index, const char** na
  *   "graph_rounds"  0..1       1: each batch of rounds is captured once as a hipGraph and replayed (default 0)
  *   "tail_items"    0..65536   at most this many pending items: one workgroup runs every remaining round
  *                              of a round loop in one launch (default 2048; 0 = never)
+ *   "grad_rows"     8, 16, 32  slot rows of a gradient tile (tiles are 64 slots wide; default 16)
  * Unknown names or values out of range return DMSC_ERR_INVALID_ARGUMENT. */
 dmsc_status dmsc_set_param(dmsc_context* ctx, const char* name, int64_t value);
 dmsc_status dmsc_get_param(dmsc_context* ctx, const char* name, int64_t* value);
