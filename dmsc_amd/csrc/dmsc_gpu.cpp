@@ -350,6 +350,13 @@ PYBIND11_MODULE(dmsc_gpu, m) {
     return r.t[idx];
   });
   cls.def("__len__", [](const DMSComplex&) { return DMSC_NUM_OUTPUTS; });
+  // unpacking (MSComplex(*result)) takes the 19 tensors in one call instead of 19 __getitem__ calls
+  // and the IndexError that ends the sequence protocol
+  cls.def("__iter__", [](const DMSComplex& r) {
+    pybind11::tuple t((size_t)DMSC_NUM_OUTPUTS);
+    for (int i = 0; i < DMSC_NUM_OUTPUTS; ++i) t[i] = pybind11::cast(r.t[i]);
+    return pybind11::iter(t);
+  });
 
   m.def("extract_dmsc", &extract_dmsc, "MI355X Discrete Morse-Smale complex computation",
         pybind11::arg("scalar_field"), pybind11::arg("persistence_threshold"), pybind11::arg("return_gradient") = false,
