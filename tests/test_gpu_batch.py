@@ -106,15 +106,16 @@ def test_scheduling_parameters_do_not_change_results():
     fields = [rng.random(s).astype(np.float32) for s in [(40, 33), (64, 129), (7, 11)]]
     fields.append(synthetic.quantized_noise(80, 60, 3, seed=4).numpy())
     thrs = [0.0, 0.1, 0.3, 0.05]
-    names = ("round_grid", "small_list", "rounds_first", "rounds_max", "expand_levels", "graph_rounds", "tail_items")
+    names = ("round_grid", "small_list", "rounds_first", "rounds_max", "expand_levels", "graph_rounds", "tail_items",
+             "grad_rows")
     saved = {k: dmsc_gpu.get_param(0, k) for k in names}
     try:
         for setting in ({"round_grid": 64, "small_list": 0, "rounds_first": 1, "rounds_max": 1, "expand_levels": 1,
-                         "graph_rounds": 0, "tail_items": 0},
+                         "graph_rounds": 0, "tail_items": 0, "grad_rows": 8},
                         {"round_grid": 8192, "small_list": 1 << 20, "rounds_first": 64, "rounds_max": 64,
-                         "expand_levels": 64, "graph_rounds": 0, "tail_items": 1 << 16},
+                         "expand_levels": 64, "graph_rounds": 0, "tail_items": 1 << 16, "grad_rows": 32},
                         {"round_grid": 1024, "small_list": 4096, "rounds_first": 2, "rounds_max": 8, "expand_levels": 4,
-                         "graph_rounds": 1, "tail_items": 64}):
+                         "graph_rounds": 1, "tail_items": 64, "grad_rows": 16}):
             for k, v in setting.items():
                 dmsc_gpu.set_param(0, k, v)
             check_batch(fields, thrs, is_dual=False)
@@ -123,6 +124,8 @@ def test_scheduling_parameters_do_not_change_results():
             dmsc_gpu.set_param(0, "round_grid", 1)
         with pytest.raises(RuntimeError):
             dmsc_gpu.set_param(0, "no_such_parameter", 1)
+        with pytest.raises(RuntimeError):
+            dmsc_gpu.set_param(0, "grad_rows", 12)
     finally:
         for k, v in saved.items():
             dmsc_gpu.set_param(0, k, v)
@@ -136,3 +139,25 @@ def test_autotune_smoke():
                         candidates={"round_grid": [512], "rounds_max": [8]})
     assert set(res["params"]) == set(autotune.DEFAULTS)
     autotune.apply(autotune.DEFAULTS)
+
+
+def test_graph_replayed_round_batches():
+    """Round batches captured once as a hipGraph and replayed batch after batch within one call (host
+    readbacks and eager launches between the replays): every replay must leave exactly the state the
+    eager launches leave, which the outputs check field by field."""
+    from dmsc_amd.csrc import dmsc_gpu
+
+    fields = [synthetic.quantized_noise(300, 280, 6, seed=31).numpy(), synthetic.quantized_noise(250, 300, 4, seed=32).numpy()]
+    thrs = [0.2, 0.5]
+    names = ("graph_rounds", "rounds_first", "rounds_max", "tail_items")
+    saved = {k: dmsc_gpu.get_param(0, k) for k in names}
+    try:
+        for k, v in {"graph_rounds": 1, "rounds_first": 2, "rounds_max": 2, "tail_items": 0}.items():
+            dmsc_gpu.set_param(0, k, v)
+        for rep in range(2):
+            check_batch(fields, thrs, is_dual=bool(rep))
+            rounds = dmsc_gpu.debug_fetch(0, "rounds", 3).tolist()
+            assert rounds[0] >= 6, rounds  # at least three replays of the 2-round persistence batch
+    finally:
+        for k, v in saved.items():
+            dmsc_gpu.set_param(0, k, v)
