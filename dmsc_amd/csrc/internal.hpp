@@ -315,6 +315,16 @@ __device__ __forceinline__ int field_of(const int64_t* __restrict__ base, int B,
   return lo;
 }
 
+// field of idx given the field `from` of a smaller index of the same range (base[from] <= idx): a few
+// steps forward. Kernels whose block (or wave) covers consecutive indices search once for the first
+// index of the block (a uniform search) and step from there per thread.
+template <class T>
+__device__ __forceinline__ int field_from(const T* __restrict__ base, int B, int from, int64_t idx) {
+  int f = from;
+  while (f + 1 < B && base[f + 1] <= idx) ++f;
+  return f;
+}
+
 // same over int32 bases (list positions)
 __device__ __forceinline__ int field_of64(const int32_t* __restrict__ base, int B, int64_t idx) {
   int lo = 0, hi = B - 1;

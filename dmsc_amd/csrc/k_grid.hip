@@ -388,10 +388,11 @@ __global__ __launch_bounds__(64 * kCompactWaves) void k_compact_lists(RankView r
         const int32_t* pre = list == 0 ? rv.pre_min : list == 1 ? rv.pre_sad : rv.pre_max;
         const int32_t* lb = list_b + list * (B + 1);
         const int64_t base = pre[w0];
+        const int fw = field_of64(lb, B, base);  // the chunk's first entry: one wave-uniform search
         for (int t = lane; t < total; t += 64) {
           const int id = stage[wid][t];
           const int64_t oi = base + t;
-          const int fi = field_of64(lb, B, oi);
+          const int fi = field_from(lb, B, fw, oi);
           const FieldDesc& fd = rv.bt.fd[fi];
           const int q = id >> 2, ty = q / fd.Nx, tx = q - ty * fd.Nx;
           if (list == 0) {
@@ -489,16 +490,22 @@ __device__ __forceinline__ int32_t label_of(int32_t o, int32_t base) { return o 
 __global__ void k_labels_faces(int64_t S, RankView rv, const uint64_t* __restrict__ pj,
                                const int32_t* __restrict__ root_out, const int32_t* __restrict__ out_base,
                                int32_t* __restrict__ out) {
-  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < S; q += (int64_t)gridDim.x * blockDim.x) {
-    const int fi = field_of(rv.bt.slot_b, rv.bt.B, q);
+  for (int64_t q0 = (int64_t)blockIdx.x * blockDim.x; q0 < S; q0 += (int64_t)gridDim.x * blockDim.x) {
+    const int fb = field_of(rv.bt.slot_b, rv.bt.B, q0);  // the block's first slot: one uniform search
+    const int64_t q = q0 + threadIdx.x;
+    if (q >= S) continue;
+    const int fi = field_from(rv.bt.slot_b, rv.bt.B, fb, q);
     out[q] = label_of(root_out[rank_max_slot(rv, fi, (int)(uint32_t)pj[q])], out_base[fi]);
   }
 }
 __global__ void k_labels_verts(int64_t N, RankView rv, const uint64_t* __restrict__ pj,
                                const int32_t* __restrict__ root_out, const int32_t* __restrict__ out_base,
                                int32_t* __restrict__ out) {
-  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < N; p += (int64_t)gridDim.x * blockDim.x) {
-    const int fi = field_of(rv.bt.pix_b, rv.bt.B, p);
+  for (int64_t p0 = (int64_t)blockIdx.x * blockDim.x; p0 < N; p0 += (int64_t)gridDim.x * blockDim.x) {
+    const int fb = field_of(rv.bt.pix_b, rv.bt.B, p0);  // the block's first pixel: one uniform search
+    const int64_t p = p0 + threadIdx.x;
+    if (p >= N) continue;
+    const int fi = field_from(rv.bt.pix_b, rv.bt.B, fb, p);
     out[p] = label_of(root_out[rank_min_pixel(rv, fi, (int)(uint32_t)pj[p])], out_base[fi]);
   }
 }
