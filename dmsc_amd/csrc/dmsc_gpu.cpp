@@ -51,11 +51,22 @@ void* alloc_cb(void* user, int32_t field, int32_t slot, int32_t ndim, const int6
   return t.data_ptr();
 }
 
+// A contiguous view of `shape` at element offset `off` of `buf`, made directly on buf's storage
+// (sharing its version counter). A batch makes thousands of these; the dispatcher's narrow + view
+// pair costs a few microseconds each, this a fraction of one.
+torch::Tensor view_of(const torch::Tensor& buf, int64_t off, c10::IntArrayRef shape) {
+  auto impl = c10::make_intrusive<c10::TensorImpl>(c10::Storage(buf.storage()), buf.key_set(), buf.dtype());
+  impl->set_sizes_contiguous(shape);
+  impl->set_storage_offset(buf.storage_offset() + off);
+  impl->set_version_counter(buf.unsafeGetTensorImpl()->version_counter());
+  return torch::Tensor(std::move(impl));
+}
+
 // Outputs that live in a shared buffer become views of it (each field's slice, in its own shape).
 void attach_views(AllocUser& u, const dmsc_outputs& out, DMSComplex& r) {
   for (int k = 1; k < DMSC_NUM_OUTPUTS; ++k) {
     if (r.t[k].defined()) continue;
-    std::vector<int64_t> shape(out.dims[k], out.dims[k] + out.ndim[k]);
+    const c10::IntArrayRef shape(out.dims[k], out.ndim[k]);
     int64_t numel = 1;
     for (int64_t d : shape) numel *= d;
     if (numel == 0 || !out.data[k]) {
@@ -67,8 +78,10 @@ void attach_views(AllocUser& u, const dmsc_outputs& out, DMSComplex& r) {
     for (const torch::Tensor& buf : u.shared) {
       const char* b = static_cast<const char*>(buf.data_ptr());
       const int64_t bytes = buf.numel() * buf.element_size();
-      if (b && p >= b && p < b + bytes) {
-        r.t[k] = buf.narrow(0, (p - b) / buf.element_size(), numel).view(shape);
+      if (b && p >= b && p + numel * buf.element_size() <= b + bytes) {
+        TORCH_CHECK((p - b) % buf.element_size() == 0 && buf.scalar_type() == opts_for(out.dtype[k], u.dev).dtype(),
+                    "dmsc: output slot ", k, " does not match its buffer's type or alignment");
+        r.t[k] = view_of(buf, (p - b) / buf.element_size(), shape);
         found = true;
         break;
       }
@@ -144,9 +157,13 @@ std::vector<DMSComplex> run_batch(dmsc_context* ctx, torch::Device dev, const st
                             alloc_cb, &u, outs.data());
   }
   TORCH_CHECK(st == DMSC_OK, dmsc_last_error(ctx));
+  torch::Tensor shapes = torch::empty({B, 2}, torch::TensorOptions().dtype(torch::kInt32));
+  int32_t* sp = shapes.data_ptr<int32_t>();
   for (int b = 0; b < B; ++b) {
     attach_views(u, outs[b], res[b]);
-    res[b].t[0] = shape_tensor(outs[b]);
+    sp[2 * b] = outs[b].shape[0];
+    sp[2 * b + 1] = outs[b].shape[1];
+    res[b].t[0] = view_of(shapes, 2 * b, {2});
   }
   return res;
 }
