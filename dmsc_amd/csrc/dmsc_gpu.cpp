@@ -316,10 +316,18 @@ torch::Tensor persistence_loss(std::vector<std::vector<torch::Tensor>> pairs, do
   return loss;
 }
 
-pybind11::object debug_fetch(int device, const std::string& name, int64_t capacity, bool as_float, bool as_u64) {
+// Element type of each debug_fetch name: the host buffer is sized here, never by the caller, so a
+// name cannot write past it.
+torch::ScalarType debug_dtype(const std::string& name) {
+  if (name == "pers_min" || name == "pers_max") return torch::kFloat32;
+  if (name == "arc_stats" || name == "face_pj" || name == "vert_pj") return torch::kInt64;
+  return torch::kInt32;
+}
+
+pybind11::object debug_fetch(int device, const std::string& name, int64_t capacity) {
+  TORCH_CHECK(capacity >= 0, "dmsc: debug_fetch capacity must be >= 0");
   dmsc_context* ctx = context_for(device);
-  auto dt = as_u64 ? torch::kInt64 : (as_float ? torch::kFloat32 : torch::kInt32);
-  torch::Tensor t = torch::empty({capacity}, torch::TensorOptions().dtype(dt));
+  torch::Tensor t = torch::empty({capacity}, torch::TensorOptions().dtype(debug_dtype(name)));
   int64_t n = dmsc_debug_fetch(ctx, name.c_str(), t.data_ptr(), capacity);
   if (n < 0) return pybind11::none();
   return pybind11::cast(t.slice(0, 0, std::min(n, capacity)));
@@ -390,6 +398,5 @@ PYBIND11_MODULE(dmsc_gpu, m) {
         pybind11::arg("max_persistence"), pybind11::arg("is_dual") = false);
   m.def("persistence_loss", &persistence_loss, "sum(pers^2 for 0 <= pers < max) of each field's pairs",
         pybind11::arg("pairs"), pybind11::arg("max_persistence"));
-  m.def("debug_fetch", &debug_fetch, pybind11::arg("device"), pybind11::arg("name"), pybind11::arg("capacity"),
-        pybind11::arg("as_float") = false, pybind11::arg("as_u64") = false);
+  m.def("debug_fetch", &debug_fetch, pybind11::arg("device"), pybind11::arg("name"), pybind11::arg("capacity"));
 }

@@ -609,6 +609,7 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     p.alive = sb[s].alive;
     p.minkey = sb[s].minkey;
     p.pair = sb[s].pair;
+    p.pair_other = sb[1 - s].pair;
     p.pers = sb[s].pers;
     p.cancel_d = record[s] ? sb[s].cancel : nullptr;
     p.rr = sb[s].rr;

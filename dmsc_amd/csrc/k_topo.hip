@@ -15,6 +15,18 @@
 // on its other side (a merge it depends on is still pending) and so misjudge which root it kills;
 // it must still block every later saddle that touches either of its current roots.
 // The smallest pending saddle always holds its dying root, so every round makes progress.
+//
+// Both sides run together, and a saddle that merges on one side is a cycle on the other. With the
+// void vertex added, the complex is a 2-sphere: the minimum graph (minima + void, saddles joining the
+// ends of their two vertex V-paths) and the maximum graph (maxima, saddles joining the ends of their
+// two face V-paths) are dual graphs on it, and the two sweeps visit the saddles in opposite orders.
+// The merging saddles of an ascending union-find sweep form the minimum spanning forest of a graph,
+// those of the descending sweep of its dual the maximum spanning forest, and in a plane graph and
+// its dual these are complements (a spanning tree of one is the complement of a spanning tree of the
+// other). So from round 1 on, a pending saddle whose pair on the other side is already set resolves
+// as a cycle here (pair -1, persistence -1) without a root walk or a post. Most cycle saddles wait
+// many rounds for the merges that close them; dropping them early halves the visits of a sweep, and
+// a cycle is a no-op of the sequential sweep, so the decisions of the other saddles are unchanged.
 
 #include "internal.hpp"
 #include "k_topo.hpp"
@@ -159,11 +171,21 @@ __device__ __forceinline__ uint32_t sweep_pos(const PersistSide& s, int i) {
   return (uint32_t)(lo + hi - 1 - p);
 }
 
+// saddle i (list position j) merged on the other side, so it is a cycle on this one
+__device__ __forceinline__ void persist_prune(const PersistSide& s, const int i, int64_t j) {
+  s.pers[i] = -1.0f;  // pair[i] is still -1 from the start state
+  s.rr[2 * j] = -2;
+}
+
 template <bool DUAL>
 __device__ __forceinline__ RootPosts persist_find_one(const PersistSide& s, const int i, int64_t j, int cur, uint64_t tag,
                                                       bool first) {
   if (first) return persist_find_decide<DUAL>(s, i, j, tag, first, *reinterpret_cast<const int2*>(s.c + 2 * i),
                                               sweep_pos(s, i));
+  if (s.pair_other[i] >= 0) {
+    persist_prune(s, i, j);
+    return RootPosts{};
+  }
   return persist_find_decide<DUAL>(s, i, j, tag, first, s.roots(cur)[j], s.keys(cur)[j]);
 }
 
@@ -183,6 +205,7 @@ __device__ __forceinline__ void persist_find_list(const PersistSide& s, int cur,
     int i[ITEMS];
     int2 cc[ITEMS];
     uint32_t ps[ITEMS];
+    bool pruned[ITEMS];
 #pragma unroll
     for (int u = 0; u < ITEMS; ++u) {
       const int64_t j = base + 64 * u + lane;
@@ -192,21 +215,26 @@ __device__ __forceinline__ void persist_find_list(const PersistSide& s, int cur,
     for (int u = 0; u < ITEMS; ++u) {
       cc[u] = make_int2(-1, -1);
       ps[u] = 0;
+      pruned[u] = false;
       if (i[u] >= 0) {
-        if (first) {
+        if (first) {  // nothing has merged yet on either side
           cc[u] = *reinterpret_cast<const int2*>(s.c + 2 * i[u]);
           ps[u] = sweep_pos(s, i[u]);
         } else {  // by list position: written where the apply appended the waiting saddle
           const int64_t j = base + 64 * u + lane;
           cc[u] = s.roots(cur)[j];
           ps[u] = s.keys(cur)[j];
+          pruned[u] = s.pair_other[i[u]] >= 0;
         }
       }
     }
 #pragma unroll
     for (int u = 0; u < ITEMS; ++u) {
       RootPosts p;
-      if (i[u] >= 0) p = persist_find_decide<DUAL>(s, i[u], base + 64 * u + lane, tag, first, cc[u], ps[u]);
+      if (i[u] >= 0) {
+        if (pruned[u]) persist_prune(s, i[u], base + 64 * u + lane);
+        else p = persist_find_decide<DUAL>(s, i[u], base + 64 * u + lane, tag, first, cc[u], ps[u]);
+      }
       wave_post_min(p.a, p.key, !first);
       wave_post_min(p.b, p.key, !first);
     }

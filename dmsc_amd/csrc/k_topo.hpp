@@ -19,6 +19,8 @@ struct PersistSide {
   uint8_t* alive;       // per extremum
   uint64_t* minkey;     // per extremum: (round tag << 32 | order position), atomic min
   int32_t* pair;        // per saddle: dying extremum index, -1 for a cycle
+  const int32_t* pair_other;  // the other side's pair array: a saddle that merged there (>= 0) is a
+                              // cycle on this side
   float* pers;          // per saddle persistence (-1 for a cycle)
   int32_t* cancel_d;    // per saddle thresholded dead extremum (null when arcs are not traced)
   int32_t* rr;          // per worklist position: dying root (-2 = cycle), (surviving root + 1) << 1 | side bit

@@ -176,11 +176,14 @@ const char* dmsc_last_error(dmsc_context* ctx);
 
 /* Copies an internal intermediate of the last dmsc_extract call into host memory, for stage-by-
  * stage parity tests. No reference counterpart. Returns the element count (int32 unless stated),
- * or -1 for an unknown name; copies min(count, capacity) elements. Names: "counts" (n_min, n_sad,
- * n_max), "min_ids", "sad_ids", "max_ids", "face_term", "face_depth", "vert_term", "vert_depth",
- * "asc_order", "sad_cmax", "sad_cmin", "pers_min" (float), "pers_max" (float), "pair_min",
- * "pair_max", "rounds" (persistence rounds, geometry rounds), "arc_stats" (int64: raw pieces copied
- * and output arc cells of the vertex side, then of the face side). */
+ * or -1 for an unknown name; copies min(count, capacity) elements, so host_dst must hold capacity
+ * elements of the name's type. Names: "counts" (n_min, n_sad, n_max), "min_ids", "sad_ids",
+ * "max_ids", "asc_order", "sad_cmax", "sad_cmin", "pers_min" (float), "pers_max" (float),
+ * "pair_min", "pair_max", "face_pj", "vert_pj" (uint64 per face / vertex: V-path terminal in the
+ * low 32 bits, depth in the high 32), "rounds" (persistence rounds, then the geometry rounds of
+ * each side), "persist_pending" (per host check of the persistence rounds: round reached, pending
+ * saddles of each side), "arc_stats" (int64: raw pieces copied and output arc cells of the vertex
+ * side, then of the face side). */
 int64_t dmsc_debug_fetch(dmsc_context* ctx, const char* name, void* host_dst, int64_t capacity);
 
 #ifdef __cplusplus
