@@ -120,6 +120,7 @@ struct dmsc_context {
   int rounds_persist = 0, rounds_geo[2] = {0, 0};
   int64_t arc_stats[4] = {0, 0, 0, 0};  // per traced side (vertices, faces): raw pieces copied, output cells
   std::vector<int32_t> pending_log;  // (round, pending min side, pending max side) at each readback
+  std::vector<int32_t> geo_log;      // the same for the geometry rounds (first entry: round 0, cancellations)
   bool dual = false;
   const int32_t *min_ids = nullptr, *sad_ids = nullptr, *max_ids = nullptr, *asc = nullptr;
   const int32_t *cmax = nullptr, *cmin = nullptr, *pair_min = nullptr, *pair_max = nullptr;
@@ -836,6 +837,7 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     int64_t pend[2] = {aw[0].ncancel, aw[1].ncancel};  // pending at the last readback (launch sizes)
     int r = 0;
     ctx->rounds_geo[0] = ctx->rounds_geo[1] = 0;
+    ctx->geo_log.assign({0, (int32_t)pend[0], (int32_t)pend[1]});
     int ggraph_mask = -1;
     int64_t gprev = -1;  // pending of the live sides at the readback before the last, gprev_r rounds in
     int gprev_r = 0;
@@ -882,6 +884,8 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
                                                  hipMemcpyDeviceToHost, st));
         DMSC_CHECK(hipStreamSynchronize(st));
       }
+      ctx->geo_log.insert(ctx->geo_log.end(), {r, live[0] ? ((const int32_t*)ctx->pinned)[0] : 0,
+                                               live[1] ? ((const int32_t*)ctx->pinned)[1] : 0});
       for (int s = 0; s < 2; ++s)
         if (live[s]) {
           ctx->rounds_geo[s] = r;
@@ -1419,6 +1423,11 @@ int64_t dmsc_debug_fetch(dmsc_context* ctx, const char* name, void* dst, int64_t
     const int64_t m = std::min<int64_t>(4, cap);
     if (dst && m > 0) std::memcpy(dst, ctx->arc_stats, (size_t)m * sizeof(int64_t));
     return 4;
+  }
+  if (n == "geo_pending") {
+    int64_t m = std::min<int64_t>((int64_t)ctx->geo_log.size(), cap);
+    if (dst && m > 0) std::memcpy(dst, ctx->geo_log.data(), (size_t)m * 4);
+    return (int64_t)ctx->geo_log.size();
   }
   if (n == "persist_pending") {
     int64_t m = std::min<int64_t>((int64_t)ctx->pending_log.size(), cap);

@@ -23,10 +23,12 @@
 // The merging saddles of an ascending union-find sweep form the minimum spanning forest of a graph,
 // those of the descending sweep of its dual the maximum spanning forest, and in a plane graph and
 // its dual these are complements (a spanning tree of one is the complement of a spanning tree of the
-// other). So from round 1 on, a pending saddle whose pair on the other side is already set resolves
-// as a cycle here (pair -1, persistence -1) without a root walk or a post. Most cycle saddles wait
-// many rounds for the merges that close them; dropping them early halves the visits of a sweep, and
-// a cycle is a no-op of the sequential sweep, so the decisions of the other saddles are unchanged.
+// other). So a pending saddle whose pair on the other side is already set resolves as a cycle here
+// (pair -1, persistence -1) without a root walk or a post: in the find, and in the apply before a
+// waiting saddle is appended to the next worklist. Round 0 runs side 0 first and then side 1, which
+// drops every saddle side 0 merged in it. Most cycle saddles wait many rounds for the merges that
+// close them; dropping them early halves the visits of a sweep, and a cycle is a no-op of the
+// sequential sweep, so the decisions of the other saddles are unchanged.
 
 #include "internal.hpp"
 #include "k_topo.hpp"
@@ -196,7 +198,8 @@ __device__ __forceinline__ RootPosts persist_find_one(const PersistSide& s, cons
 // their posts are combined per wave first; from round 1 on the posting lane also reads before its
 // atomic (hot roots of big components).
 template <bool DUAL, int ITEMS>
-__device__ __forceinline__ void persist_find_list(const PersistSide& s, int cur, int64_t n, uint64_t tag, bool first) {
+__device__ __forceinline__ void persist_find_list(const PersistSide& s, int cur, int64_t n, uint64_t tag, bool first,
+                                                  bool prune) {
   const int lane = threadIdx.x & 63;
   const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
@@ -217,9 +220,10 @@ __device__ __forceinline__ void persist_find_list(const PersistSide& s, int cur,
       ps[u] = 0;
       pruned[u] = false;
       if (i[u] >= 0) {
-        if (first) {  // nothing has merged yet on either side
+        if (first) {  // nothing has merged yet on this side (the other side's round 0 may have run)
           cc[u] = *reinterpret_cast<const int2*>(s.c + 2 * i[u]);
           ps[u] = sweep_pos(s, i[u]);
+          pruned[u] = prune && s.pair_other[i[u]] >= 0;
         } else {  // by list position: written where the apply appended the waiting saddle
           const int64_t j = base + 64 * u + lane;
           cc[u] = s.roots(cur)[j];
@@ -241,29 +245,32 @@ __device__ __forceinline__ void persist_find_list(const PersistSide& s, int cur,
   }
 }
 
+// sides: bit 0 / bit 1 = side 0 / side 1 take part in this launch; bit 2 = the round is round 0 of
+// side 1 run after side 0's round 0, so side 1 already drops the saddles side 0 merged
 template <bool DUAL>
-__device__ __forceinline__ void persist_find_phase(const PersistSide& s0, const PersistSide& s1, int round) {
+__device__ __forceinline__ void persist_find_phase(const PersistSide& s0, const PersistSide& s1, int round, int sides) {
   const int cur = round & 1;
-  const int n0 = s0.cnt[cur], n1 = s1.cnt[cur];
+  const int n0 = (sides & 1) ? s0.cnt[cur] : 0, n1 = (sides & 2) ? s1.cnt[cur] : 0;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    s0.cnt[cur ^ 1] = 0;
-    s1.cnt[cur ^ 1] = 0;
+    if (sides & 1) s0.cnt[cur ^ 1] = 0;
+    if (sides & 2) s1.cnt[cur ^ 1] = 0;
   }
   const uint64_t tag = (uint64_t)(0xFFFFFFFFu - (uint32_t)round) << 32;
-  const bool first = round == 0;
+  const bool first = round == 0, prune = !first || (sides & 4);
   // the round's decisions go to rr at the item's list position
   if ((int64_t)n0 + n1 > (1 << 16)) {
-    persist_find_list<DUAL, 4>(s0, cur, n0, tag, first);
-    persist_find_list<DUAL, 4>(s1, cur, n1, tag, first);
+    persist_find_list<DUAL, 4>(s0, cur, n0, tag, first, prune);
+    persist_find_list<DUAL, 4>(s1, cur, n1, tag, first, prune);
   } else {
-    persist_find_list<DUAL, 1>(s0, cur, n0, tag, first);
-    persist_find_list<DUAL, 1>(s1, cur, n1, tag, first);
+    persist_find_list<DUAL, 1>(s0, cur, n0, tag, first, prune);
+    persist_find_list<DUAL, 1>(s1, cur, n1, tag, first, prune);
   }
 }
 template <bool DUAL>
-__global__ void k_persist_find(PersistSide s0, PersistSide s1, int round, const int32_t* __restrict__ d_round) {
+__global__ void k_persist_find(PersistSide s0, PersistSide s1, int round, const int32_t* __restrict__ d_round,
+                               int sides) {
   if (d_round) round = *d_round;  // graph replays take the round number from the device
-  persist_find_phase<DUAL>(s0, s1, round);
+  persist_find_phase<DUAL>(s0, s1, round, sides);
 }
 
 // Applies saddle i, which holds minkey of its dying root (r = its rr pair; sv its value, dv the dying
@@ -327,7 +334,7 @@ constexpr int kApplyItems = 4, kApplyChunk = 256 * kApplyItems;
 
 template <bool DUAL, int ITEMS>
 __device__ __forceinline__ void persist_apply_chunk(const PersistSide& s, int cur, int64_t start, int n, uint64_t tag,
-                                                    uint8_t* __restrict__ removed) {
+                                                    uint8_t* __restrict__ removed, bool prune) {
   const int64_t j0 = start + (int64_t)threadIdx.x * ITEMS;
   const int32_t* const in = s.list(cur);
   int k[ITEMS];
@@ -366,17 +373,26 @@ __device__ __forceinline__ void persist_apply_chunk(const PersistSide& s, int cu
     mk[u] = 0;
     if (r[u].x >= 0) mk[u] = s.minkey[r[u].x];
   }
-  int32_t nwait = 0;
   uint32_t waitmask = 0, winmask = 0;
 #pragma unroll
   for (int u = 0; u < ITEMS; ++u) {
     if (r[u].x < 0) continue;  // a cycle (decided in the find) or past the list
-    if (mk[u] != (tag | ps[u])) {
-      waitmask |= 1u << u;
-      ++nwait;
-    } else {
-      winmask |= 1u << u;
+    if (mk[u] != (tag | ps[u])) waitmask |= 1u << u;
+    else winmask |= 1u << u;
+  }
+  // a waiting saddle that the other side has merged by now is a cycle here: it leaves the worklist
+  // (a merge made elsewhere in this launch may not be seen yet; the next find catches that one)
+  int32_t po[ITEMS];
+#pragma unroll
+  for (int u = 0; u < ITEMS; ++u) po[u] = prune && (waitmask >> u & 1) ? s.pair_other[k[u]] : -1;
+  int32_t nwait = 0;
+#pragma unroll
+  for (int u = 0; u < ITEMS; ++u) {
+    if (po[u] >= 0) {
+      s.pers[k[u]] = -1.0f;
+      waitmask &= ~(1u << u);
     }
+    nwait += waitmask >> u & 1;
   }
   // winners: their value loads for all items first, then the decisions
   float sv[ITEMS], dv[ITEMS], th[ITEMS];
@@ -405,30 +421,31 @@ __device__ __forceinline__ void persist_apply_chunk(const PersistSide& s, int cu
 // Chunks of side 0 come first, then those of side 1, so every chunk (and block) has one side.
 template <bool DUAL>
 __device__ __forceinline__ void persist_apply_phase(const PersistSide& s0, const PersistSide& s1, int round,
-                                                    uint8_t* __restrict__ removed, int small_list) {
+                                                    uint8_t* __restrict__ removed, int small_list, int sides) {
   const int cur = round & 1;
-  const int n0 = s0.cnt[cur], n1 = s1.cnt[cur];
+  const int n0 = (sides & 1) ? s0.cnt[cur] : 0, n1 = (sides & 2) ? s1.cnt[cur] : 0;
   const uint64_t tag = (uint64_t)(0xFFFFFFFFu - (uint32_t)round) << 32;
+  const bool prune = round > 0 || (sides & 4);  // side 0's round 0 runs before anything merged on side 1
   if ((int64_t)n0 + n1 <= small_list) {
     const int64_t nc0 = ceil_div(n0, 256), nc = nc0 + ceil_div(n1, 256);
     for (int64_t c = blockIdx.x; c < nc; c += gridDim.x) {
-      if (c >= nc0) persist_apply_chunk<DUAL, 1>(s1, cur, (c - nc0) * 256, n1, tag, removed);
-      else persist_apply_chunk<DUAL, 1>(s0, cur, c * 256, n0, tag, removed);
+      if (c >= nc0) persist_apply_chunk<DUAL, 1>(s1, cur, (c - nc0) * 256, n1, tag, removed, prune);
+      else persist_apply_chunk<DUAL, 1>(s0, cur, c * 256, n0, tag, removed, prune);
     }
     return;
   }
   const int64_t nc0 = ceil_div(n0, kApplyChunk), nc = nc0 + ceil_div(n1, kApplyChunk);
   for (int64_t c = blockIdx.x; c < nc; c += gridDim.x) {
-    if (c >= nc0) persist_apply_chunk<DUAL, kApplyItems>(s1, cur, (c - nc0) * kApplyChunk, n1, tag, removed);
-    else persist_apply_chunk<DUAL, kApplyItems>(s0, cur, c * kApplyChunk, n0, tag, removed);
+    if (c >= nc0) persist_apply_chunk<DUAL, kApplyItems>(s1, cur, (c - nc0) * kApplyChunk, n1, tag, removed, prune);
+    else persist_apply_chunk<DUAL, kApplyItems>(s0, cur, c * kApplyChunk, n0, tag, removed, prune);
   }
 }
 template <bool DUAL>
 __global__ __launch_bounds__(256) void k_persist_apply(PersistSide s0, PersistSide s1, int round,
                                                        const int32_t* __restrict__ d_round,
-                                                       uint8_t* __restrict__ removed, int small_list) {
+                                                       uint8_t* __restrict__ removed, int small_list, int sides) {
   if (d_round) round = *d_round;
-  persist_apply_phase<DUAL>(s0, s1, round, removed, small_list);
+  persist_apply_phase<DUAL>(s0, s1, round, removed, small_list, sides);
 }
 
 // Start state of both sides' rounds in one launch: every extremum its own root (both union-finds),
@@ -542,22 +559,38 @@ void launch_round_inc(int32_t* d_round, hipStream_t st) {
   DMSC_LAUNCH_CHECK();
 }
 
-void launch_persist_round(bool dual, const PersistSide& s0, const PersistSide& s1, int round, uint8_t* removed,
-                          int64_t max_items, hipStream_t st, const int32_t* d_round) {
-  KScope ks_("persist_round", st);
+// one find and one apply launch over the sides in `sides` (persist_find_phase's mask)
+static void persist_round_launches(bool dual, const PersistSide& s0, const PersistSide& s1, int round,
+                                   uint8_t* removed, int64_t max_items, hipStream_t st, const int32_t* d_round,
+                                   int sides) {
   const Tunables& tu = tune();
   const int g = grid_for(max_items, 256, tu.round_grid);
-  if (dual) hipLaunchKernelGGL(k_persist_find<true>, dim3(g), dim3(256), 0, st, s0, s1, round, d_round);
-  else hipLaunchKernelGGL(k_persist_find<false>, dim3(g), dim3(256), 0, st, s0, s1, round, d_round);
+  if (dual) hipLaunchKernelGGL(k_persist_find<true>, dim3(g), dim3(256), 0, st, s0, s1, round, d_round, sides);
+  else hipLaunchKernelGGL(k_persist_find<false>, dim3(g), dim3(256), 0, st, s0, s1, round, d_round, sides);
   DMSC_LAUNCH_CHECK();
   const int ga = grid_for(max_items, max_items <= tu.small_list ? 256 : kApplyChunk, tu.round_grid);
   if (dual)
     hipLaunchKernelGGL(k_persist_apply<true>, dim3(ga), dim3(256), 0, st, s0, s1, round, d_round, removed,
-                       tu.small_list);
+                       tu.small_list, sides);
   else
     hipLaunchKernelGGL(k_persist_apply<false>, dim3(ga), dim3(256), 0, st, s0, s1, round, d_round, removed,
-                       tu.small_list);
+                       tu.small_list, sides);
   DMSC_LAUNCH_CHECK();
+}
+
+// Round 0 visits every saddle on both sides. Run side 0 (the vertex side, which has the more extrema
+// and so the more merges) first: side 1's round 0 then drops every saddle side 0 merged, with one
+// coalesced read and two writes instead of a visit. Later rounds run both sides in each launch.
+void launch_persist_round(bool dual, const PersistSide& s0, const PersistSide& s1, int round, uint8_t* removed,
+                          int64_t max_items, hipStream_t st, const int32_t* d_round) {
+  KScope ks_("persist_round", st);
+  if (round == 0 && !d_round) {
+    const int64_t per_side = (max_items + 1) / 2;
+    persist_round_launches(dual, s0, s1, 0, removed, per_side, st, nullptr, 1);
+    persist_round_launches(dual, s0, s1, 0, removed, per_side, st, nullptr, 2 | 4);
+    return;
+  }
+  persist_round_launches(dual, s0, s1, round, removed, max_items, st, d_round, 3);
 }
 
 // ------------------------------------------------------------------ assembly (cpu/workspace.hxx:86-365)

@@ -182,7 +182,8 @@ const char* dmsc_last_error(dmsc_context* ctx);
  * "pair_min", "pair_max", "face_pj", "vert_pj" (uint64 per face / vertex: V-path terminal in the
  * low 32 bits, depth in the high 32), "rounds" (persistence rounds, then the geometry rounds of
  * each side), "persist_pending" (per host check of the persistence rounds: round reached, pending
- * saddles of each side), "arc_stats" (int64: raw pieces copied and output arc cells of the vertex
+ * saddles of each side), "geo_pending" (the same for the geometry rounds, first entry round 0 with
+ * the cancellations of each side), "arc_stats" (int64: raw pieces copied and output arc cells of the vertex
  * side, then of the face side). */
 int64_t dmsc_debug_fetch(dmsc_context* ctx, const char* name, void* host_dst, int64_t capacity);
 

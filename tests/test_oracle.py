@@ -201,6 +201,46 @@ def test_persistence_matches_independent_elder_rule(dual, seed):
             assert np.float32(out["p_" + side_out][i]) == np.float32(p), (side_out, s)
 
 
+def _duality_fields():
+    rng = np.random.default_rng(77)
+    fields = []
+    for k in range(40):
+        H, W = (int(v) for v in rng.integers(1, 40, size=2))
+        f = rng.random((H, W)).astype(np.float32)
+        levels = (0, 2, 3, 16)[k % 4]
+        if levels:
+            f = (np.floor(f * levels) / levels).astype(np.float32)
+        if k % 5 == 1:
+            f[rng.random((H, W)) < 0.15] = np.inf
+        if k % 5 == 2:
+            f[rng.random((H, W)) < 0.15] = -np.inf
+        fields.append(f)
+    fields += [np.zeros((6, 9), np.float32), np.full((5, 4), np.nan, np.float32), np.full((3, 7), np.inf, np.float32)]
+    return fields
+
+
+@pytest.mark.parametrize("dual", [False, True])
+def test_every_saddle_merges_on_exactly_one_side(dual):
+    """The GPU persistence rounds drop a saddle from one side's sweep as soon as the other side has
+    merged it. That rests on planar duality: with the void vertex the complex is a 2-sphere, the
+    minimum and maximum saddle graphs are dual graphs on it, the ascending sweep's merges are a
+    minimum spanning forest and the descending sweep's merges of the dual a maximum one, and those
+    are complements. Checked on the independent elder-rule sweep and on the oracle, including ties,
+    infinities, constant and all-NaN fields and one-pixel-wide shapes."""
+    rng = np.random.default_rng(5)
+    for seed in range(20):
+        H, W = (int(v) for v in rng.integers(1, 12, size=2))
+        f = (rng.integers(0, 4, size=(H, W)) if seed % 2 else rng.random((H, W))).astype(np.float32)
+        ref = independent_pairs(f, dual)
+        assert ref["max"].keys() == ref["min"].keys()
+        for s in ref["max"]:
+            assert (ref["max"][s][0] != -1) != (ref["min"][s][0] != -1), (seed, s)
+    for f in _duality_fields():
+        out = oracle.extract(f, -1.0, False, dual, False, False, False, False)
+        merged_max, merged_min = out[8] != -1, out[9] != -1  # ppairs by output saddle, raw mode keeps all
+        assert len(out[3]) > 0 and np.all(merged_max ^ merged_min), f.shape
+
+
 @pytest.mark.parametrize("dual", [False, True])
 @pytest.mark.parametrize("seed", range(6))
 def test_raw_labels_and_euler_characteristic(dual, seed):
