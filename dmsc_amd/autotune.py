@@ -23,7 +23,7 @@ from .csrc import dmsc_gpu
 
 TUNING_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning")
 DEFAULTS = {"round_grid": 2048, "small_list": 65536, "rounds_first": 4, "rounds_max": 16, "expand_levels": 8,
-            "graph_rounds": 0, "tail_items": 2048, "grad_rows": 16}
+            "graph_rounds": 0, "tail_items": 2048, "grad_rows": 16, "side_rounds": 1}
 CANDIDATES = {
     "round_grid": [512, 1024, 2048, 4096, 8192, 16384],
     "small_list": [4096, 16384, 65536, 262144],
@@ -33,6 +33,7 @@ CANDIDATES = {
     "graph_rounds": [0, 1],
     "tail_items": [0, 512, 2048, 8192],
     "grad_rows": [8, 16, 32],
+    "side_rounds": [0, 1, 2, 3, 4],
 }
 _applied = set()
 

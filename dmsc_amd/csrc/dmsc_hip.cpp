@@ -34,6 +34,7 @@ void scan_i32_to_i64(const int32_t* in, int64_t* out, int64_t n, int64_t* d_tota
 // arena slots
 enum : int {
   A_SCAN = 0, A_RS0, A_RS1,
+  A_SCAN2, A_RS2, A_RS3, A_RSP2, A_RSP3,  // scratch of the per-side work run on the side stream
   A_FSUCC, A_VSUCC, A_PJF, A_PJV, A_PLANES, A_PREMIN, A_PRESAD, A_PREMAX,
   A_C32, A_C64,
   A_MINIDS, A_MINVAL, A_SADIDS, A_SADVAL, A_MAXIDS, A_MAXVAL,
@@ -108,7 +109,7 @@ struct dmsc_context {
   Tunables tune;
   hipStream_t cap_stream = nullptr;  // private stream that round batches are captured on
   hipStream_t side = nullptr;        // output emission, concurrent with the arc geometry
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_mark = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_mark = nullptr, ev_side = nullptr;
   Arena ar;
   Arena ar2;  // arc emission scratch
   int64_t* pinned = nullptr;
@@ -293,6 +294,7 @@ struct StreamFork {
     if (!ctx->ev_fork) DMSC_CHECK(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
     if (!ctx->ev_join) DMSC_CHECK(hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
     if (!ctx->ev_mark) DMSC_CHECK(hipEventCreateWithFlags(&ctx->ev_mark, hipEventDisableTiming));
+    if (!ctx->ev_side) DMSC_CHECK(hipEventCreateWithFlags(&ctx->ev_side, hipEventDisableTiming));
     DMSC_CHECK(hipEventRecord(ctx->ev_fork, st));
     DMSC_CHECK(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
   }
@@ -309,6 +311,14 @@ struct StreamFork {
   }
   void wait_mark() {
     if (on) DMSC_CHECK(hipStreamWaitEvent(st, ctx->ev_mark, 0));
+  }
+  // a second mark (per-side work run on the side stream): mark_side() records the side stream's
+  // current point, wait_side() makes `st` wait for it
+  void mark_side() {
+    if (on) DMSC_CHECK(hipEventRecord(ctx->ev_side, ctx->side));
+  }
+  void wait_side() {
+    if (on) DMSC_CHECK(hipStreamWaitEvent(st, ctx->ev_side, 0));
   }
   void join() {
     if (!on) return;
@@ -789,7 +799,44 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     o_p[s] = (float*)oa.slot(slot_p(s == 1, dual), 1, [&](int f) { return one(n_sad_out(f)); }, DMSC_DTYPE_FLOAT32);
     o_pp[s] = (int32_t*)oa.slot(slot_pp(s == 1, dual), 1, [&](int f) { return one(n_sad_out(f)); }, DMSC_DTYPE_INT32);
   }
+  // Per-side arc work between the assembly and the expansion runs side 0 on `st` and, when both sides are
+  // traced, side 1 on the side stream (with its own scan and sort scratch): the cancellation order here,
+  // the path lengths after the geometry rounds.
+  const bool split = fork.on && aw[0].on && aw[1].on;
+  auto side_stream = [&](int s) { return split && s == 1 ? fork.stream() : st; };
   fork.follow();  // the side stream also waits for the persistence and assembly results
+  auto prepare_cancels = [&](int s) {
+    ArcWork& w = aw[s];
+    const hipStream_t cst = side_stream(s);
+    const int b = A_ARC0 + kArcSlots * s;
+    w.ncancel = h32[8 + s];
+    // cancellations sorted by (persistence, saddle id) (persistence_struct.hxx:15-19); fields never
+    // meet, so one batch-wide stable sort orders every field's cancellations correctly
+    const int64_t nc = w.ncancel;
+    GeoState& g = w.G;
+    int32_t* cs0 = ar.get<int32_t>(b + G_CS0, nsad);  // compacted above (same slots and sizes)
+    int32_t* cd0 = ar.get<int32_t>(b + G_CD0, nsad);
+    uint32_t* ck = ar.get<uint32_t>(b + G_KEYS, nsad);
+    int32_t* cslot = ar.get<int32_t>(b + G_SLOT, nc);
+    const bool on_side = cst != st;
+    radix_sort_pairs(ck, cslot, ar.get<uint32_t>(b + G_KEYS2, nc), ar.get<int32_t>(b + G_SLOT2, nc), nc, ar,
+                     on_side ? A_RS2 : A_RS0, on_side ? A_RSP2 : A_RSP0, cst, true);  // (persistence key, slot)
+    g.ncancel = nc;
+    int32_t* ckey = ar.get<int32_t>(b + G_CS, nc);
+    g.key = ckey;
+    g.rr = ar.get<int32_t>(b + G_RR, 4 * nc);
+    g.wl = ar.get<int4>(b + G_WL, 2 * nc);
+    g.wls = nc;
+    g.glen = ar.get<int2>(b + G_GLEN, 2 * nc);
+    g.cnt = ar.get<int32_t>(b + G_CNT, 2);
+    g.err = d_err;
+    launch_cancel_finish(nc, cslot, cd0, ckey, g.kill, cst);
+    launch_geo_items(nc, cs0, ckey, w.A.tinit, g.list(0), g.cnt, cst);  // list 0 and its count
+  };
+  if (split) {
+    prepare_cancels(1);
+    fork.mark_side();  // the geometry rounds wait for this point, not for the emission queued below
+  }
   {
     const hipStream_t es = fork.stream();
     launch_emit_points(nmin, min_ids, sb[0].to_out, o_pts[0], es);
@@ -802,42 +849,16 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     if (gmax) launch_labels_faces(rv, S, pjf, sb[1].root, row32(4), o_flab, es);
     if (gmin) launch_labels_verts(rv, N, pjv, sb[0].root, row32(3), o_vlab, es);
   }
-
-  for (int s = 0; s < 2; ++s) {
-    if (!aw[s].on) continue;
-    ArcWork& w = aw[s];
-    const int b = A_ARC0 + kArcSlots * s;
-    const int64_t next = s ? nmax : nmin;
-    w.ncancel = h32[8 + s];
-    // cancellations sorted by (persistence, saddle id) (persistence_struct.hxx:15-19); fields never
-    // meet, so one batch-wide stable sort orders every field's cancellations correctly
-    const int64_t nc = w.ncancel;
-    GeoState& g = w.G;
-    int32_t* cs0 = ar.get<int32_t>(b + G_CS0, nsad);  // compacted above (same slots and sizes)
-    int32_t* cd0 = ar.get<int32_t>(b + G_CD0, nsad);
-    uint32_t* ck = ar.get<uint32_t>(b + G_KEYS, nsad);
-    int32_t* cslot = ar.get<int32_t>(b + G_SLOT, nc);
-    radix_sort_pairs(ck, cslot, ar.get<uint32_t>(b + G_KEYS2, nc), ar.get<int32_t>(b + G_SLOT2, nc), nc, ar, A_RS0,
-                     A_RSP0, st, true);  // (persistence key, compacted slot)
-    g.ncancel = nc;
-    int32_t* ckey = ar.get<int32_t>(b + G_CS, nc);
-    g.key = ckey;
-    g.rr = ar.get<int32_t>(b + G_RR, 4 * nc);
-    g.wl = ar.get<int4>(b + G_WL, 2 * nc);
-    g.wls = nc;
-    g.glen = ar.get<int2>(b + G_GLEN, 2 * nc);
-    g.cnt = ar.get<int32_t>(b + G_CNT, 2);
-    g.err = d_err;
-    launch_cancel_finish(nc, cslot, cd0, ckey, g.kill, st);
-    launch_geo_items(nc, cs0, ckey, w.A.tinit, g.list(0), g.cnt, st);  // list 0 and its count
-  }
+  for (int s = 0; s < 2; ++s)
+    if (aw[s].on && !(split && s == 1)) prepare_cancels(s);
+  if (split) fork.wait_side();
   // geometry rounds of both sides, interleaved, one readback per batch of rounds
   {
     bool live[2] = {aw[0].on && aw[0].ncancel > 0, aw[1].on && aw[1].ncancel > 0};
     int64_t pend[2] = {aw[0].ncancel, aw[1].ncancel};  // pending at the last readback (launch sizes)
     int r = 0;
     ctx->rounds_geo[0] = ctx->rounds_geo[1] = 0;
-    ctx->geo_log.assign({0, (int32_t)pend[0], (int32_t)pend[1]});
+    ctx->geo_log.assign({0, (int32_t)aw[0].ncancel, (int32_t)aw[1].ncancel});
     int ggraph_mask = -1;
     int64_t gprev = -1;  // pending of the live sides at the readback before the last, gprev_r rounds in
     int gprev_r = 0;
@@ -884,46 +905,47 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
                                                  hipMemcpyDeviceToHost, st));
         DMSC_CHECK(hipStreamSynchronize(st));
       }
-      ctx->geo_log.insert(ctx->geo_log.end(), {r, live[0] ? ((const int32_t*)ctx->pinned)[0] : 0,
-                                               live[1] ? ((const int32_t*)ctx->pinned)[1] : 0});
+      const int32_t* hp = (const int32_t*)ctx->pinned;  // pending per side
+      ctx->geo_log.insert(ctx->geo_log.end(), {r, live[0] ? hp[0] : 0, live[1] ? hp[1] : 0});
       for (int s = 0; s < 2; ++s)
         if (live[s]) {
           ctx->rounds_geo[s] = r;
-          pend[s] = ((const int32_t*)ctx->pinned)[s];
+          pend[s] = hp[s];
           if (pend[s] == 0) live[s] = false;
           else if (r > 4 * aw[s].ncancel + 64) throw Fail{DMSC_ERR_INTERNAL, "geometry rounds did not converge"};
         }
     }
   }
+  if (split) fork.follow();  // the side stream waits for the geometry rounds
   for (int s = 0; s < 2; ++s) {
     if (!aw[s].on) continue;
     ArcWork& w = aw[s];
     GeoState& g = w.G;
     const int64_t next = s ? nmax : nmin;
     // path lengths to the final roots and per-output-arc lengths (min order)
+    const hipStream_t cst = side_stream(s);
     Arena& a2 = ctx->ar2;
     const int hb2 = 16 * s;
     w.pjq = a2.get<uint64_t>(hb2 + H_PJQ, next + 1);
-    launch_geo_q(g, next, w.pjq, st);
+    launch_geo_q(g, next, w.pjq, cst);
     w.cum_r = a2.get<uint64_t>(hb2 + H_CUMR, next + 1);
     w.cum_c = a2.get<uint64_t>(hb2 + H_CUMC, next + 1);
-    launch_link_counts(w.A, g, w.pjq, next, w.cum_r, w.cum_c, st);
+    launch_link_counts(w.A, g, w.pjq, next, w.cum_r, w.cum_c, cst);
     w.flen = a2.get<int64_t>(hb2 + H_FLEN, 2 * nsad);
     int64_t* per_pos = a2.get<int64_t>(hb2 + H_PERPOS, nsad);
     w.pos_pre = a2.get<int64_t>(hb2 + H_POSPRE, nsad);
     // over the surviving saddles in output order (position = output saddle index)
     w.rec = a2.get<int4>(hb2 + H_REC, 2 * nsad);  // 32 bytes per surviving saddle
-    launch_final_lengths(n_sad_total, surv_list[0], w.A, w.pjq, w.rec, w.flen, per_pos, st);
-    scan_exclusive<int64_t>(per_pos, w.pos_pre, n_sad_total, c64 + 2 * s + 1, ar, A_SCAN, st);
+    launch_final_lengths(n_sad_total, surv_list[0], w.A, w.pjq, w.rec, w.flen, per_pos, cst);
+    scan_exclusive<int64_t>(per_pos, w.pos_pre, n_sad_total, c64 + 2 * s + 1, ar, cst == st ? A_SCAN : A_SCAN2, cst);
     w.arc_base = a2.get<int64_t>(hb2 + H_ARCB, B + 1);
-    bases_at<int64_t, int32_t>(w.pos_pre, row32(5), n_sad_total, c64 + 2 * s + 1, B, w.arc_base, st);
+    bases_at<int64_t, int32_t>(w.pos_pre, row32(5), n_sad_total, c64 + 2 * s + 1, B, w.arc_base, cst);
+    DMSC_CHECK(hipMemcpyAsync((char*)ctx->pinned + 8 * (size_t)(B + 1) * s, w.arc_base, 8 * (size_t)(B + 1),
+                              hipMemcpyDeviceToHost, cst));
   }
   if (aw[0].on || aw[1].on) {
-    for (int s = 0; s < 2; ++s)
-      if (aw[s].on)
-        DMSC_CHECK(hipMemcpyAsync((char*)ctx->pinned + 8 * (size_t)(B + 1) * s, aw[s].arc_base, 8 * (size_t)(B + 1),
-                                  hipMemcpyDeviceToHost, st));
     DMSC_CHECK(hipStreamSynchronize(st));
+    if (split) DMSC_CHECK(hipStreamSynchronize(fork.stream()));
     for (int s = 0; s < 2; ++s) {
       if (!aw[s].on) continue;
       ArcWork& w = aw[s];
@@ -983,10 +1005,25 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
                          e.nseg + 3, e.cap, d_err, w.out_off, w.out_arc_off, w.arc_base, row32(5), B, e.es);
     DMSC_CHECK(hipMemcpyAsync(pin32(s), e.nseg, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, e.es));
   }
+  // a side's raw pieces are copied as soon as its own expansion has ended (on its stream), so the
+  // copy of the shorter side overlaps the other side's remaining levels
+  bool copied[2] = {false, false};
+  auto start_copy = [&](int s) {
+    ArcWork& w = aw[s];
+    Expansion& e = ex[s];
+    copied[s] = true;
+    const int32_t nraw = pin32(s)[3];
+    ctx->arc_stats[2 * s] = nraw;
+    ctx->arc_stats[2 * s + 1] = w.out_total;
+    if (nraw > e.cap) throw Fail{DMSC_ERR_INTERNAL, "arc segment list overflow"};
+    if (nraw > 0)
+      launch_copy_raw(w.A, e.raw, e.nseg + 3, std::min<int64_t>(nraw, e.cap), w.out_cells, w.out_total, d_err, e.es);
+  };
   for (int s = 0; s < 2; ++s)
     if (aw[s].on) {
       DMSC_CHECK(hipStreamSynchronize(ex[s].es));
       ex[s].n_in = pin32(s)[0];
+      if (ex[s].n_in <= 0) start_copy(s);
     }
   while ((aw[0].on && ex[0].n_in > 0) || (aw[1].on && ex[1].n_in > 0)) {
     bool ran[2] = {false, false};
@@ -1009,19 +1046,11 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
       if (ran[s]) {
         DMSC_CHECK(hipStreamSynchronize(ex[s].es));
         ex[s].n_in = pin32(s)[ex[s].level % 3];
+        if (ex[s].n_in <= 0) start_copy(s);
       }
   }
-  for (int s = 0; s < 2; ++s) {
-    ArcWork& w = aw[s];
-    Expansion& e = ex[s];
-    if (!w.on) continue;
-    const int32_t nraw = pin32(s)[3];
-    ctx->arc_stats[2 * s] = nraw;
-    ctx->arc_stats[2 * s + 1] = w.out_total;
-    if (nraw > e.cap) throw Fail{DMSC_ERR_INTERNAL, "arc segment list overflow"};
-    if (nraw > 0)
-      launch_copy_raw(w.A, e.raw, e.nseg + 3, std::min<int64_t>(nraw, e.cap), w.out_cells, w.out_total, d_err, e.es);
-  }
+  for (int s = 0; s < 2; ++s)
+    if (aw[s].on && !copied[s]) start_copy(s);
   fork.join();
   readback(ctx, d_err, sizeof(int32_t), st);
   if (ctx->times.level) ctx->times.collect();
@@ -1239,6 +1268,7 @@ void dmsc_context_destroy(dmsc_context* ctx) {
   if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
   if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
   if (ctx->ev_mark) (void)hipEventDestroy(ctx->ev_mark);
+  if (ctx->ev_side) (void)hipEventDestroy(ctx->ev_side);
   if (ctx->ev_bwd) (void)hipEventDestroy(ctx->ev_bwd);
   if (ctx->bwd_scratch) (void)hipFree(ctx->bwd_scratch);
   delete ctx;
@@ -1342,6 +1372,7 @@ int* param_slot(dmsc_context* ctx, const std::string& n, int64_t& lo, int64_t& h
   if (n == "graph_rounds") { lo = 0; hi = 1; return &t.graph_rounds; }
   if (n == "tail_items") { lo = 0; hi = 1 << 16; return &t.tail_items; }
   if (n == "grad_rows") { lo = 8; hi = 32; return &t.grad_rows; }
+  if (n == "side_rounds") { lo = 0; hi = 4096; return &t.side_rounds; }
   return nullptr;
 }
 }  // namespace

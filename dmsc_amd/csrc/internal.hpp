@@ -124,6 +124,9 @@ struct Tunables {
   int tail_items = 2048;     // pending items at a host check up to which one workgroup runs every
                              // remaining round in one launch (0: never)
   int grad_rows = 16;        // slot rows of a gradient tile (8, 16 or 32; tiles are 64 slots wide)
+  int side_rounds = 1;       // first persistence rounds run side by side (vertex side, then face side,
+                             // which drops what the vertex side just merged); later rounds run both
+                             // sides in each launch
 };
 extern thread_local const Tunables* tl_tune;
 inline const Tunables& tune() {

@@ -580,14 +580,15 @@ static void persist_round_launches(bool dual, const PersistSide& s0, const Persi
 
 // Round 0 visits every saddle on both sides. Run side 0 (the vertex side, which has the more extrema
 // and so the more merges) first: side 1's round 0 then drops every saddle side 0 merged, with one
-// coalesced read and two writes instead of a visit. Later rounds run both sides in each launch.
+// coalesced read and two writes instead of a visit. The first tune().side_rounds rounds run side by
+// side this way (the next round of side 0 drops what side 1 merged, and so on); later rounds run
+// both sides in each launch, half the launches for the short worklists of the tail.
 void launch_persist_round(bool dual, const PersistSide& s0, const PersistSide& s1, int round, uint8_t* removed,
                           int64_t max_items, hipStream_t st, const int32_t* d_round) {
   KScope ks_("persist_round", st);
-  if (round == 0 && !d_round) {
-    const int64_t per_side = (max_items + 1) / 2;
-    persist_round_launches(dual, s0, s1, 0, removed, per_side, st, nullptr, 1);
-    persist_round_launches(dual, s0, s1, 0, removed, per_side, st, nullptr, 2 | 4);
+  if (round < tune().side_rounds && !d_round) {
+    persist_round_launches(dual, s0, s1, round, removed, max_items, st, nullptr, 1);
+    persist_round_launches(dual, s0, s1, round, removed, max_items, st, nullptr, 2 | 4);
     return;
   }
   persist_round_launches(dual, s0, s1, round, removed, max_items, st, d_round, 3);

@@ -107,15 +107,16 @@ def test_scheduling_parameters_do_not_change_results():
     fields.append(synthetic.quantized_noise(80, 60, 3, seed=4).numpy())
     thrs = [0.0, 0.1, 0.3, 0.05]
     names = ("round_grid", "small_list", "rounds_first", "rounds_max", "expand_levels", "graph_rounds", "tail_items",
-             "grad_rows")
+             "grad_rows", "side_rounds")
     saved = {k: dmsc_gpu.get_param(0, k) for k in names}
     try:
         for setting in ({"round_grid": 64, "small_list": 0, "rounds_first": 1, "rounds_max": 1, "expand_levels": 1,
-                         "graph_rounds": 0, "tail_items": 0, "grad_rows": 8},
+                         "graph_rounds": 0, "tail_items": 0, "grad_rows": 8, "side_rounds": 4096},
                         {"round_grid": 8192, "small_list": 1 << 20, "rounds_first": 64, "rounds_max": 64,
-                         "expand_levels": 64, "graph_rounds": 0, "tail_items": 1 << 16, "grad_rows": 32},
+                         "expand_levels": 64, "graph_rounds": 0, "tail_items": 1 << 16, "grad_rows": 32,
+                         "side_rounds": 0},
                         {"round_grid": 1024, "small_list": 4096, "rounds_first": 2, "rounds_max": 8, "expand_levels": 4,
-                         "graph_rounds": 1, "tail_items": 64, "grad_rows": 16}):
+                         "graph_rounds": 1, "tail_items": 64, "grad_rows": 16, "side_rounds": 2}):
             for k, v in setting.items():
                 dmsc_gpu.set_param(0, k, v)
             check_batch(fields, thrs, is_dual=False)
@@ -161,3 +162,4 @@ def test_graph_replayed_round_batches():
     finally:
         for k, v in saved.items():
             dmsc_gpu.set_param(0, k, v)
+
