@@ -740,28 +740,33 @@ __device__ __forceinline__ LinkParts link_parts(const ArcSide& A, const uint64_t
 
 // cum_r / cum_c: (parent or the void node `next`, raw / chain pieces of the link) for linked
 // extrema, (self, 0) for roots; pointer jumping then turns them into path sums.
+// The link record of every extremum e < next (the expansion then reads one record per link instead of
+// the link, the saddle's targets and four path lengths).
 __global__ void k_link_counts(ArcSide A, GeoState g, const uint64_t* __restrict__ pjq, int64_t next,
-                              uint64_t* __restrict__ cum_r, uint64_t* __restrict__ cum_c) {
+                              uint64_t* __restrict__ cum_r, uint64_t* __restrict__ cum_c, LinkRec* __restrict__ xl) {
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e <= next; e += (int64_t)gridDim.x * blockDim.x) {
     uint32_t J = (uint32_t)e, nr = 0, nc = 0;
     if (e < next) {
       const int4 lk = g.link[e];
+      LinkRec R{(int)e, -1, -1, 0, 0, 0, 0, 0};
       if (lk.x != (int)e) {
         J = lk.x < 0 ? (uint32_t)next : (uint32_t)lk.x;
         const LinkParts L = link_parts(A, pjq, (int)e, lk);
         nr = 1 + (L.Bo > 0 ? 1 : 0);
         nc = (L.Alen > 0 ? 1 : 0) + (L.Clen > 0 ? 1 : 0);
+        R = LinkRec{lk.x, L.Td, L.To, 2 * L.s + L.kd, L.Alen, L.Bk, L.Bo, L.Clen};
       }
+      xl[e] = R;
     }
     cum_r[e] = (uint64_t)J | ((uint64_t)nr << 32);
     cum_c[e] = (uint64_t)J | ((uint64_t)nc << 32);
   }
 }
 void launch_link_counts(const ArcSide& a, const GeoState& g, const uint64_t* pjq, int64_t next, uint64_t* cum_r,
-                        uint64_t* cum_c, hipStream_t st) {
+                        uint64_t* cum_c, LinkRec* xl, hipStream_t st) {
   KScope ks_("link_counts", st);
   hipLaunchKernelGGL(k_link_counts, dim3(grid_for(next + 1, 256, 16384)), dim3(256), 0, st, a, g, pjq, next, cum_r,
-                     cum_c);
+                     cum_c, xl);
   DMSC_LAUNCH_CHECK();
   pointer_jump(cum_r, next + 1, g.err, st);
   pointer_jump(cum_c, next + 1, g.err, st);
@@ -770,7 +775,8 @@ void launch_link_counts(const ArcSide& a, const GeoState& g, const uint64_t* pjq
 // One expansion level. Each chain (a -> b) reserves exactly its links' pieces (path-sum
 // differences of cum_r / cum_c) with one atomic per wave, then walks its links writing them.
 __global__ __launch_bounds__(256) void k_expand(ArcSide A, GeoState g, const uint64_t* __restrict__ pjq, const uint64_t* __restrict__ cum_r,
-                         const uint64_t* __restrict__ cum_c, const Seg* __restrict__ in, const int32_t* __restrict__ nin,
+                         const uint64_t* __restrict__ cum_c, const LinkRec* __restrict__ xl,
+                         const Seg* __restrict__ in, const int32_t* __restrict__ nin,
                          Seg* __restrict__ out, int32_t* __restrict__ nout, int32_t* __restrict__ nreset,
                          Seg* __restrict__ raw, int32_t* __restrict__ nraw, int64_t cap) {
   // levels are launched back to back without host checks: the input count comes from the device
@@ -805,14 +811,15 @@ __global__ __launch_bounds__(256) void k_expand(ArcSide A, GeoState g, const uin
     int y = sg.a;
     int64_t acc = 0;
     for (int it = 0; y >= 0 && y != b; ++it) {
-      const int4 lk = g.link[y];
-      if (lk.x == y) break;
+      const LinkRec L = xl[y];
+      if (L.next == y) break;
       if (it >= kLoopCap) {
         report(g.err, ERR_GEO_CAP);
         break;
       }
-      const int32_t len = lk.y;
-      const LinkParts L = link_parts(A, pjq, y, lk);
+      // the link's length (the sum cannot overflow here: a saturated length on any expanded path
+      // makes the merged arcs' total exceed 32 bits, which the host refuses before expanding)
+      const int32_t len = L.Alen + L.Bk + L.Bo + L.Clen;
       const int need_r = 1 + (L.Bo > 0 ? 1 : 0), need_c = (L.Alen > 0 ? 1 : 0) + (L.Clen > 0 ? 1 : 0);
       if (pr + need_r > pr_end || pc + need_c > pc_end) {  // counts disagree with the walk: never overrun
         report(g.err, ERR_SEG_CAP);
@@ -822,31 +829,32 @@ __global__ __launch_bounds__(256) void k_expand(ArcSide A, GeoState g, const uin
       if (fwd) {
         if (L.Alen > 0) out[pc++] = Seg{L.Td, y, seg_od(o, 0)};
         o += L.Alen;
-        raw[pr++] = Seg{2 * L.s + L.kd, L.Bk, seg_od(o, 0)};
+        raw[pr++] = Seg{L.arc, L.Bk, seg_od(o, 0)};
         o += L.Bk;
-        if (L.Bo > 0) raw[pr++] = Seg{2 * L.s + L.ko, L.Bo, seg_od(o, 1)};
+        if (L.Bo > 0) raw[pr++] = Seg{L.arc ^ 1, L.Bo, seg_od(o, 1)};
         o += L.Bo;
-        if (L.Clen > 0) out[pc++] = Seg{L.To, L.Ro, seg_od(o, 1)};
+        if (L.Clen > 0) out[pc++] = Seg{L.To, L.next, seg_od(o, 1)};
       } else {
-        if (L.Clen > 0) out[pc++] = Seg{L.To, L.Ro, seg_od(o, 0)};
+        if (L.Clen > 0) out[pc++] = Seg{L.To, L.next, seg_od(o, 0)};
         o += L.Clen;
-        if (L.Bo > 0) raw[pr++] = Seg{2 * L.s + L.ko, L.Bo, seg_od(o, 0)};
+        if (L.Bo > 0) raw[pr++] = Seg{L.arc ^ 1, L.Bo, seg_od(o, 0)};
         o += L.Bo;
-        raw[pr++] = Seg{2 * L.s + L.kd, L.Bk, seg_od(o, 1)};
+        raw[pr++] = Seg{L.arc, L.Bk, seg_od(o, 1)};
         o += L.Bk;
         if (L.Alen > 0) out[pc++] = Seg{L.Td, y, seg_od(o, 1)};
       }
       acc += len;
-      y = lk.x;
+      y = L.next;
     }
   }
 }
 void launch_expand_segments(const ArcSide& a, const GeoState& g, const uint64_t* pjq, const uint64_t* cum_r,
-                            const uint64_t* cum_c, const Seg* in, const int32_t* nin, Seg* out, int32_t* nout,
-                            int32_t* nreset, Seg* raw, int32_t* nraw, int64_t max_in, int64_t cap, hipStream_t st) {
+                            const uint64_t* cum_c, const LinkRec* xl, const Seg* in, const int32_t* nin, Seg* out,
+                            int32_t* nout, int32_t* nreset, Seg* raw, int32_t* nraw, int64_t max_in, int64_t cap,
+                            hipStream_t st) {
   KScope ks_("expand_segments", st);
-  hipLaunchKernelGGL(k_expand, dim3(grid_for(max_in, 256, 8192)), dim3(256), 0, st, a, g, pjq, cum_r, cum_c, in, nin,
-                     out, nout, nreset, raw, nraw, cap);
+  hipLaunchKernelGGL(k_expand, dim3(grid_for(max_in, 256, 8192)), dim3(256), 0, st, a, g, pjq, cum_r, cum_c, xl, in,
+                     nin, out, nout, nreset, raw, nraw, cap);
   DMSC_LAUNCH_CHECK();
 }
 

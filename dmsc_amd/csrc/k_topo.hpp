@@ -164,10 +164,20 @@ struct Seg {
   int32_t a, b;
   int64_t od;
 };
+// Everything the expansion needs about the link of one extremum d, gathered once (32 bytes, one
+// aligned half cache line): the link expands to reverse(chain(Td -> d)) ++ reverse(raw arc) ++
+// raw arc ^ 1 ++ chain(To -> next) with those pieces' lengths Alen, Bk, Bo, Clen.
+struct alignas(32) LinkRec {
+  int32_t next;      // parent (-1: the void); d itself for a root
+  int32_t Td, To;    // the arc targets of the linking saddle on d's side and on the other side
+  int32_t arc;       // arc index 2 s + kd of the saddle's arc on d's side (the other one is arc ^ 1)
+  int32_t Alen, Bk, Bo, Clen;
+};
 // Per extremum, after the geometry rounds: cumulative counts of the raw / chain pieces that the
-// links on its path to the final root expand into (pointer-jumped (root, count) pairs).
+// links on its path to the final root expand into (pointer-jumped (root, count) pairs), and its
+// link record.
 void launch_link_counts(const ArcSide& a, const GeoState& g, const uint64_t* pjq, int64_t next, uint64_t* cum_r,
-                        uint64_t* cum_c, hipStream_t st);
+                        uint64_t* cum_c, LinkRec* xl, hipStream_t st);
 // arc_base: [B+1] first output cell of each field; sad_out: [B+1] first output saddle of each field.
 // A field's offsets array has one more entry than its saddles, so field f's entries start at
 // sad_out[f] + f in the concatenated offsets output.
@@ -178,8 +188,9 @@ void launch_seed_segments(int64_t ns, const int32_t* ord, const ArcSide& a, cons
 // One level of expansion: every chain becomes the raw pieces and sub-chains of its links.
 // n = min(*nin, cap) chains are read; *nreset is cleared (the counter the following level appends to).
 void launch_expand_segments(const ArcSide& a, const GeoState& g, const uint64_t* pjq, const uint64_t* cum_r,
-                            const uint64_t* cum_c, const Seg* in, const int32_t* nin, Seg* out, int32_t* nout,
-                            int32_t* nreset, Seg* raw, int32_t* nraw, int64_t max_in, int64_t cap, hipStream_t st);
+                            const uint64_t* cum_c, const LinkRec* xl, const Seg* in, const int32_t* nin, Seg* out,
+                            int32_t* nout, int32_t* nreset, Seg* raw, int32_t* nraw, int64_t max_in, int64_t cap,
+                            hipStream_t st);
 void launch_copy_raw(const ArcSide& a, const Seg* raw, const int32_t* nraw, int64_t max_raw, int32_t* out,
                      int64_t out_total, int32_t* err, hipStream_t st);
 
