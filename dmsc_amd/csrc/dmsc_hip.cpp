@@ -547,7 +547,7 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
   launch_sort_keys(nsad, sad_val, keys, st);
   radix_sort_pairs(keys, asc, keys2, vals2, nsad, ar, A_RS0, A_RSP0, st, true);  // (value key, saddle index)
   if (B > 1) {  // stable second pass on the field: (field, value, id) order
-    launch_gather_field(nsad, asc, sad_field, keys, st);
+    launch_gather_field(nsad, asc, B, sad_b, keys, st);
     int bits = 0;
     while ((1 << bits) < B) ++bits;
     radix_sort_pairs_bits(keys, asc, keys2, vals2, nsad, bits, ar, A_RS0, A_RSP0, st);
@@ -557,8 +557,8 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
   // primal: min side ascending, max side descending; dual: the reverse (arcs_topology.hxx:178-181)
   // positions in the ascending order only: the descending side mirrors them within the field's range
   int32_t* pos_asc = ar.get<int32_t>(A_POSMIN, nsad);
-  if (!dual) launch_orders(nsad, asc, sad_field, sad_b, ord_min, ord_max, pos_asc, nullptr, st);
-  else launch_orders(nsad, asc, sad_field, sad_b, ord_max, ord_min, pos_asc, nullptr, st);
+  if (!dual) launch_orders(nsad, asc, B, sad_b, ord_min, ord_max, pos_asc, nullptr, st);
+  else launch_orders(nsad, asc, B, sad_b, ord_max, ord_min, pos_asc, nullptr, st);
 
   // ---- raw arc lengths of the traced sides. The raw arcs need no persistence result: they are
   // traced on the side stream while the persistence rounds run on `st`.
@@ -831,8 +831,8 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     g.glen = ar.get<int2>(b + G_GLEN, 2 * nc);
     g.cnt = ar.get<int32_t>(b + G_CNT, 2);
     g.err = d_err;
-    launch_cancel_finish(nc, cslot, cd0, ckey, g.kill, cst);
-    launch_geo_items(nc, cs0, ckey, w.A.tinit, g.list(0), g.cnt, cst);  // list 0 and its count
+    launch_cancel_finish(nc, cslot, ckey, cst);
+    launch_geo_items(nc, cs0, cd0, ckey, w.A.tinit, g.list(0), g.kill, g.cnt, cst);  // list 0, its count, killers
   };
   if (split) {
     prepare_cancels(1);

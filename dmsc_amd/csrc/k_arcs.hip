@@ -121,27 +121,23 @@ void launch_cancel_compact(int64_t ns, const int32_t* cancel_d, const float* per
   DMSC_LAUNCH_CHECK();
 }
 // Cancellations stay in saddle-id order (index j, spatially coherent); their sorted position is the
-// merge key: key[j] = position of j in (persistence, saddle id) order; kill[d] = (pending, key of
-// the cancellation whose dead extremum d is).
-__global__ void k_cancel_finish(int64_t nc, const int32_t* __restrict__ slot, const int32_t* __restrict__ cd,
-                                int32_t* __restrict__ key, uint64_t* __restrict__ kill) {
-  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x) {
-    const int j = slot[c];
-    key[j] = (int)c;
-    kill[cd[j]] = kKillPending | (uint32_t)c;
-  }
+// merge key: key[j] = position of j in (persistence, saddle id) order.
+__global__ void k_cancel_finish(int64_t nc, const int32_t* __restrict__ slot, int32_t* __restrict__ key) {
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x)
+    key[slot[c]] = (int)c;
 }
-void launch_cancel_finish(int64_t nc, const int32_t* slot, const int32_t* cd, int32_t* key, uint64_t* kill,
-                          hipStream_t st) {
+void launch_cancel_finish(int64_t nc, const int32_t* slot, int32_t* key, hipStream_t st) {
   if (nc == 0) return;
-  hipLaunchKernelGGL(k_cancel_finish, dim3(grid_for(nc, 256, 16384)), dim3(256), 0, st, nc, slot, cd, key, kill);
+  hipLaunchKernelGGL(k_cancel_finish, dim3(grid_for(nc, 256, 16384)), dim3(256), 0, st, nc, slot, key);
   DMSC_LAUNCH_CHECK();
 }
 
 // ------------------------------------------------------------------ geometry rounds
-// initial worklist: every cancellation (saddle-id order) with its key and its two arc targets
-__global__ void k_geo_items(int64_t nc, const int32_t* __restrict__ cs, const int32_t* __restrict__ key,
-                            const int32_t* __restrict__ tinit, int4* __restrict__ wl, int32_t* __restrict__ cnt) {
+// initial worklist: every cancellation (saddle-id order) with its key and its two arc targets; and
+// kill[d] = (pending, key of the cancellation whose dead extremum d is)
+__global__ void k_geo_items(int64_t nc, const int32_t* __restrict__ cs, const int32_t* __restrict__ cd,
+                            const int32_t* __restrict__ key, const int32_t* __restrict__ tinit, int4* __restrict__ wl,
+                            uint64_t* __restrict__ kill, int32_t* __restrict__ cnt) {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     cnt[0] = (int32_t)nc;
     cnt[1] = 0;
@@ -149,13 +145,15 @@ __global__ void k_geo_items(int64_t nc, const int32_t* __restrict__ cs, const in
   for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x) {
     const int s = cs[c];
     const int2 t = *reinterpret_cast<const int2*>(tinit + 2 * s);
-    wl[c] = make_int4((int)c, key[c], t.x, t.y);
+    const int k = key[c];
+    wl[c] = make_int4((int)c, k, t.x, t.y);
+    kill[cd[c]] = kKillPending | (uint32_t)k;
   }
 }
-void launch_geo_items(int64_t nc, const int32_t* cs, const int32_t* key, const int32_t* tinit, int4* wl, int32_t* cnt,
-                      hipStream_t st) {
-  hipLaunchKernelGGL(k_geo_items, dim3(grid_for(std::max<int64_t>(nc, 1), 256, 16384)), dim3(256), 0, st, nc, cs, key,
-                     tinit, wl, cnt);
+void launch_geo_items(int64_t nc, const int32_t* cs, const int32_t* cd, const int32_t* key, const int32_t* tinit,
+                      int4* wl, uint64_t* kill, int32_t* cnt, hipStream_t st) {
+  hipLaunchKernelGGL(k_geo_items, dim3(grid_for(std::max<int64_t>(nc, 1), 256, 16384)), dim3(256), 0, st, nc, cs, cd,
+                     key, tinit, wl, kill, cnt);
   DMSC_LAUNCH_CHECK();
 }
 

@@ -747,14 +747,16 @@ void launch_emit_side(int64_t ns, const int32_t* ord, const uint8_t* removed, co
 
 // Both sweep orders (ascending / descending within each field) and their inverses (saddle index ->
 // position). `asc` is grouped by field, ascending (value, id) inside each group.
-__global__ void k_orders(int64_t ns, const int32_t* __restrict__ asc, const int32_t* __restrict__ sfield,
-                         const int32_t* __restrict__ sad_b, int32_t* __restrict__ fwd, int32_t* __restrict__ rev,
-                         int32_t* __restrict__ pos_fwd, int32_t* __restrict__ pos_rev) {
+// The orders are grouped by field, so position k belongs to the field whose position range holds
+// it: a binary search over the B + 1 field bases (cached) instead of a gather by saddle.
+__global__ void k_orders(int64_t ns, const int32_t* __restrict__ asc, int B, const int32_t* __restrict__ sad_b,
+                         int32_t* __restrict__ fwd, int32_t* __restrict__ rev, int32_t* __restrict__ pos_fwd,
+                         int32_t* __restrict__ pos_rev) {
   for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < ns; k += (int64_t)gridDim.x * blockDim.x) {
     const int v = asc[k];
     int64_t lo = 0, hi = ns;
-    if (sfield) {
-      const int fi = sfield[v];
+    if (B > 1) {
+      const int fi = field_of64(sad_b, B, k);
       lo = sad_b[fi];
       hi = sad_b[fi + 1];
     }
@@ -765,23 +767,25 @@ __global__ void k_orders(int64_t ns, const int32_t* __restrict__ asc, const int3
     if (pos_rev) pos_rev[v] = (int)r;
   }
 }
-void launch_orders(int64_t ns, const int32_t* asc, const int32_t* sfield, const int32_t* sad_b, int32_t* fwd,
-                   int32_t* rev, int32_t* pos_fwd, int32_t* pos_rev, hipStream_t st) {
+void launch_orders(int64_t ns, const int32_t* asc, int B, const int32_t* sad_b, int32_t* fwd, int32_t* rev,
+                   int32_t* pos_fwd, int32_t* pos_rev, hipStream_t st) {
   if (ns == 0) return;
-  hipLaunchKernelGGL(k_orders, dim3(grid_for(ns, 256, 16384)), dim3(256), 0, st, ns, asc, sfield, sad_b, fwd, rev,
+  hipLaunchKernelGGL(k_orders, dim3(grid_for(ns, 256, 16384)), dim3(256), 0, st, ns, asc, B, sad_b, fwd, rev,
                      pos_fwd, pos_rev);
   DMSC_LAUNCH_CHECK();
 }
 
-// keys[k] = field of saddle idx[k] (the second, field-major pass of the saddle sort)
-__global__ void k_gather_field(int64_t n, const int32_t* __restrict__ idx, const int32_t* __restrict__ sfield,
+// keys[k] = field of saddle idx[k] (the second, field-major pass of the saddle sort). Saddles are
+// numbered field by field, so that is a binary search over the B + 1 saddle bases (cached) rather than
+// a gather from a per-saddle array.
+__global__ void k_gather_field(int64_t n, const int32_t* __restrict__ idx, int B, const int32_t* __restrict__ sad_b,
                                uint32_t* __restrict__ keys) {
   for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x)
-    keys[k] = (uint32_t)sfield[idx[k]];
+    keys[k] = (uint32_t)field_of64(sad_b, B, idx[k]);
 }
-void launch_gather_field(int64_t n, const int32_t* idx, const int32_t* sfield, uint32_t* keys, hipStream_t st) {
+void launch_gather_field(int64_t n, const int32_t* idx, int B, const int32_t* sad_b, uint32_t* keys, hipStream_t st) {
   if (n == 0) return;
-  hipLaunchKernelGGL(k_gather_field, dim3(grid_for(n, 256, 16384)), dim3(256), 0, st, n, idx, sfield, keys);
+  hipLaunchKernelGGL(k_gather_field, dim3(grid_for(n, 256, 16384)), dim3(256), 0, st, n, idx, B, sad_b, keys);
   DMSC_LAUNCH_CHECK();
 }
 

@@ -84,9 +84,10 @@ void launch_emit_side(int64_t ns, const int32_t* ord, const uint8_t* removed, co
                       int32_t* out_e, int32_t* out_sad, hipStream_t st);
 // Sweep orders from the ascending order `asc`, grouped by field (sad_b: [B+1] saddle bases):
 // fwd = asc, rev = asc reversed within each field's range, and their inverses.
-void launch_orders(int64_t ns, const int32_t* asc, const int32_t* sfield, const int32_t* sad_b, int32_t* fwd,
-                   int32_t* rev, int32_t* pos_fwd, int32_t* pos_rev, hipStream_t st);
-void launch_gather_field(int64_t n, const int32_t* idx, const int32_t* sfield, uint32_t* keys, hipStream_t st);
+void launch_orders(int64_t ns, const int32_t* asc, int B, const int32_t* sad_b, int32_t* fwd, int32_t* rev,
+                   int32_t* pos_fwd, int32_t* pos_rev, hipStream_t st);
+// keys[k] = field of saddle idx[k] (saddles are numbered field by field: sad_b holds the B + 1 bases)
+void launch_gather_field(int64_t n, const int32_t* idx, int B, const int32_t* sad_b, uint32_t* keys, hipStream_t st);
 void launch_sort_keys(int64_t n, const float* v, uint32_t* keys, hipStream_t st);
 
 // ------------------------------------------------------------------ arc geometry (k_arcs.hip)
@@ -147,10 +148,12 @@ void launch_cancel_flags(int64_t ns, const int32_t* cancel_d, int32_t* flag, hip
 void launch_cancel_compact(int64_t ns, const int32_t* cancel_d, const float* pers, const int32_t* pre,
                            int32_t* cs, int32_t* cd, uint32_t* keys, hipStream_t st);
 constexpr uint64_t kKillPending = 0xFFFFFFFFull << 32;
-void launch_geo_items(int64_t nc, const int32_t* cs, const int32_t* key, const int32_t* tinit, int4* wl, int32_t* cnt,
-                      hipStream_t st);
-void launch_cancel_finish(int64_t nc, const int32_t* slot, const int32_t* cd, int32_t* key, uint64_t* kill,
-                          hipStream_t st);
+// list 0 (every cancellation with its key and arc targets, saddle-id order), its count, and the
+// killer word of every dead extremum
+void launch_geo_items(int64_t nc, const int32_t* cs, const int32_t* cd, const int32_t* key, const int32_t* tinit,
+                      int4* wl, uint64_t* kill, int32_t* cnt, hipStream_t st);
+// key[slot[c]] = c: each cancellation's position in (persistence, saddle id) order
+void launch_cancel_finish(int64_t nc, const int32_t* slot, int32_t* key, hipStream_t st);
 void launch_geo_q(const GeoState& g, int64_t next, uint64_t* pjq, hipStream_t st);
 
 // Output arc lengths for surviving saddles in min order (ord: survivor list, position = output saddle);
