@@ -86,3 +86,22 @@ def test_msc_helpers():
         assert torch.equal(torch.cat([r0, r1]), ms.get_ridge(i))
         v0, v1 = ms.get_valley(i, split_arcs=True)
         assert torch.equal(torch.cat([v0, v1]), ms.get_valley(i))
+
+
+@pytest.mark.parametrize("batched", [False, True])
+def test_cell_id_limit_of_the_binding(batched):
+    """The binding's 32-bit cell-id check (reference input_validation.hxx:24-26: H + 1 <= INT32_MAX / 4 /
+    (W + 1), integer division) runs before anything touches the device: the largest square that fits is
+    23169^2 (refused only as a CPU tensor), 23170^2 is refused as too large. Expanded views, no memory."""
+    from dmsc_amd.csrc import dmsc_gpu
+
+    def view(h, w):
+        t = torch.zeros(1, 1).expand(h, w)
+        return t.unsqueeze(0) if batched else t
+
+    with pytest.raises(RuntimeError, match="too large for 32-bit cell IDs"):
+        dmsc_gpu.extract_dmsc(view(23170, 23170), 0.1)
+    with pytest.raises(RuntimeError, match="too large for 32-bit cell IDs"):
+        dmsc_gpu.extract_dmsc(view(1, (2**31 - 1) // 8), 0.1)  # H + 1 = 2 > INT32_MAX / 4 / (W + 1) = 1
+    with pytest.raises(RuntimeError, match="GPU backend requires"):
+        dmsc_gpu.extract_dmsc(view(23169, 23169), 0.1)
