@@ -45,14 +45,14 @@ enum : int {
   A_ASM0 = A_SIDE_END,  // assembly: counts, prefixes, output map, survivor lists, edge outputs (16 slots)
   A_ASM_END = A_ASM0 + 16,
   A_ARC0 = A_ASM_END,  // per arc side: kArcSlots slots each
-  A_ARC_END = A_ARC0 + 2 * 23,
+  A_ARC_END = A_ARC0 + 2 * 20,
 };
 
 enum SideSlot { P_PARG, P_PART, P_ALIVE, P_MINKEY, P_PAIR, P_PERS, P_CANCEL, P_RR, P_WL, P_CNT, P_ROOT, P_TOOUT,
                 P_CROOT, P_RKEY, P_WKEY, kSideSlots };
 static_assert(A_SIDE_END - A_SIDE0 == 2 * kSideSlots, "side slot count");
 enum ArcSlot {
-  G_BLEN, G_BOFF, G_ARENA, G_FLAG, G_PRE, G_CS0, G_CD0, G_KEYS, G_SLOT, G_KEYS2, G_SLOT2, G_CS, G_CD, G_DEADC,
+  G_BLEN, G_BOFF, G_ARENA, G_FLAG, G_PRE, G_CS0, G_CD0, G_KEYS, G_CS, G_CD, G_DEADC,
   G_LINK, G_DONE, G_WATCH, G_RR, G_WL, G_CNT, G_JUMP, G_GLEN, G_TL, kArcSlots
 };
 static_assert(A_ARC_END - A_ARC0 == 2 * kArcSlots, "arc slot count");
@@ -706,6 +706,36 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     launch_raw();
   }
 
+  // ---- cancellation lists of the traced sides, ahead of the assembly: their counts come back first,
+  // so the cancellation order (a sort, the merge keys, the first geometry worklist) is queued on the
+  // side stream while the assembly below runs on `st`
+  for (int s = 0; s < 2; ++s) {
+    if (!aw[s].on) continue;
+    const int b = A_ARC0 + kArcSlots * s;
+    const int64_t next = s ? nmax : nmin;
+    int32_t* flag = ar.get<int32_t>(b + G_FLAG, nsad);
+    int32_t* pre = ar.get<int32_t>(b + G_PRE, nsad);
+    launch_cancel_flags(nsad, sb[s].cancel, flag, st);
+    scan_exclusive<int32_t>(flag, pre, nsad, c32 + 8 + s, ar, A_SCAN, st);
+    GeoState& g = aw[s].G;
+    int32_t* cs0 = ar.get<int32_t>(b + G_CS0, nsad);
+    int32_t* cd0 = ar.get<int32_t>(b + G_CD0, nsad);
+    uint32_t* ck = ar.get<uint32_t>(b + G_KEYS, nsad);
+    launch_cancel_compact(nsad, sb[s].cancel, sb[s].pers, pre, cs0, cd0, ck, st);
+    g.cs = cs0;
+    g.cd = cd0;
+    g.kill = ar.get<uint64_t>(b + G_DEADC, next);
+    g.link = ar.get<int4>(b + G_LINK, next);
+    g.watch = ar.get<uint64_t>(b + G_WATCH, next);
+    g.gjump = ar.get<uint64_t>(b + G_JUMP, next);
+    launch_geo_init(g, next, st);  // every extremum its own root, no killer, no watch
+  }
+  constexpr size_t kCancelPin = 64;  // pinned bytes 64..71: the two cancellation counts
+  if (fork.on) {
+    DMSC_CHECK(hipMemcpyAsync((char*)ctx->pinned + kCancelPin, c32 + 8, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    fork.follow();  // the side stream waits for the compacted lists (and, with them, the counts)
+  }
+
   // ---- assembly counts (cpu/workspace.hxx:107-168)
   int32_t* surv = ar.get<int32_t>(A_ASM0 + 1, nsad);
   int32_t* surv_pre = ar.get<int32_t>(A_ASM0 + 2, nsad);
@@ -741,36 +771,37 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
   launch_survivors(nsad, ord_max, surv_mx, surv_mx_pre, rows_pre[1], edges[1], surv_list[1], surv_rows[1], edge_list[1],
                    surv_pre, sad_b, B, out_list_mx, st);
 
-  // ---- cancellation lists of the traced sides
-  for (int s = 0; s < 2; ++s) {
-    if (!aw[s].on) continue;
-    const int b = A_ARC0 + kArcSlots * s;
-    int32_t* flag = ar.get<int32_t>(b + G_FLAG, nsad);
-    int32_t* pre = ar.get<int32_t>(b + G_PRE, nsad);
-    launch_cancel_flags(nsad, sb[s].cancel, flag, st);
-    scan_exclusive<int32_t>(flag, pre, nsad, c32 + 8 + s, ar, A_SCAN, st);
-  }
   DMSC_CHECK(hipMemcpyAsync(ctx->pinned, c32, 16 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
   DMSC_CHECK(hipMemcpyAsync((char*)ctx->pinned + 128, b32, 8 * sizeof(int32_t) * (B + 1), hipMemcpyDeviceToHost, st));
-  // Work that needs no host-side count goes in behind the readback, so the GPU runs it while the host
-  // waits and then allocates the outputs: the cancellation compaction (lists sized for every saddle)
-  // and the geometry start state.
-  for (int s = 0; s < 2; ++s) {
-    if (!aw[s].on) continue;
-    const int b = A_ARC0 + kArcSlots * s;
-    const int64_t next = s ? nmax : nmin;
-    GeoState& g = aw[s].G;
-    int32_t* cs0 = ar.get<int32_t>(b + G_CS0, nsad);
-    int32_t* cd0 = ar.get<int32_t>(b + G_CD0, nsad);
-    uint32_t* ck = ar.get<uint32_t>(b + G_KEYS, nsad);
-    launch_cancel_compact(nsad, sb[s].cancel, sb[s].pers, ar.get<int32_t>(b + G_PRE, nsad), cs0, cd0, ck, st);
-    g.cs = cs0;
-    g.cd = cd0;
-    g.kill = ar.get<uint64_t>(b + G_DEADC, next);
-    g.link = ar.get<int4>(b + G_LINK, next);
-    g.watch = ar.get<uint64_t>(b + G_WATCH, next);
-    g.gjump = ar.get<uint64_t>(b + G_JUMP, next);
-    launch_geo_init(g, next, st);  // every extremum its own root, no killer, no watch
+  // Per traced side, on the side stream while the assembly runs: cancellations ordered by (persistence,
+  // saddle id) (persistence_struct.hxx:15-19), as the merge key of each (position in that order; the
+  // lists stay in saddle-id order, which keeps them spatially coherent), and the first geometry
+  // worklist. Fields never meet, so one batch-wide stable sort orders every field's cancellations.
+  if (fork.on) {
+    DMSC_CHECK(hipEventSynchronize(ctx->ev_fork));  // the counts copied before the assembly
+    int32_t hc[2];
+    std::memcpy(hc, (const char*)ctx->pinned + kCancelPin, sizeof(hc));
+    const hipStream_t cst = fork.stream();
+    for (int s = 0; s < 2; ++s) {
+      ArcWork& w = aw[s];
+      if (!w.on) continue;
+      const int b = A_ARC0 + kArcSlots * s;
+      const int64_t nc = hc[s];
+      w.ncancel = nc;
+      GeoState& g = w.G;
+      g.ncancel = nc;
+      int32_t* ckey = ar.get<int32_t>(b + G_CS, nc);
+      g.key = ckey;
+      g.rr = ar.get<int32_t>(b + G_RR, 4 * nc);
+      g.wl = ar.get<int4>(b + G_WL, 2 * nc);
+      g.wls = nc;
+      g.glen = ar.get<int2>(b + G_GLEN, 2 * nc);
+      g.cnt = ar.get<int32_t>(b + G_CNT, 2);
+      g.err = d_err;
+      radix_rank(ar.get<uint32_t>(b + G_KEYS, nsad), ckey, nc, ar, A_RS2, A_RSP2, cst);
+      launch_geo_items(nc, g.cs, g.cd, ckey, w.A.tinit, g.list(0), g.kill, g.cnt, cst);  // list 0, its count, killers
+    }
+    fork.mark_side();  // the geometry rounds wait for this point, not for the emission queued below
   }
   DMSC_CHECK(hipStreamSynchronize(st));
   int32_t h32[16];
@@ -806,38 +837,6 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
   const bool split = fork.on && aw[0].on && aw[1].on;
   auto side_stream = [&](int s) { return split && s == 1 ? fork.stream() : st; };
   fork.follow();  // the side stream also waits for the persistence and assembly results
-  auto prepare_cancels = [&](int s) {
-    ArcWork& w = aw[s];
-    const hipStream_t cst = side_stream(s);
-    const int b = A_ARC0 + kArcSlots * s;
-    w.ncancel = h32[8 + s];
-    // cancellations sorted by (persistence, saddle id) (persistence_struct.hxx:15-19); fields never
-    // meet, so one batch-wide stable sort orders every field's cancellations correctly
-    const int64_t nc = w.ncancel;
-    GeoState& g = w.G;
-    int32_t* cs0 = ar.get<int32_t>(b + G_CS0, nsad);  // compacted above (same slots and sizes)
-    int32_t* cd0 = ar.get<int32_t>(b + G_CD0, nsad);
-    uint32_t* ck = ar.get<uint32_t>(b + G_KEYS, nsad);
-    int32_t* cslot = ar.get<int32_t>(b + G_SLOT, nc);
-    const bool on_side = cst != st;
-    radix_sort_pairs(ck, cslot, ar.get<uint32_t>(b + G_KEYS2, nc), ar.get<int32_t>(b + G_SLOT2, nc), nc, ar,
-                     on_side ? A_RS2 : A_RS0, on_side ? A_RSP2 : A_RSP0, cst, true);  // (persistence key, slot)
-    g.ncancel = nc;
-    int32_t* ckey = ar.get<int32_t>(b + G_CS, nc);
-    g.key = ckey;
-    g.rr = ar.get<int32_t>(b + G_RR, 4 * nc);
-    g.wl = ar.get<int4>(b + G_WL, 2 * nc);
-    g.wls = nc;
-    g.glen = ar.get<int2>(b + G_GLEN, 2 * nc);
-    g.cnt = ar.get<int32_t>(b + G_CNT, 2);
-    g.err = d_err;
-    launch_cancel_finish(nc, cslot, ckey, cst);
-    launch_geo_items(nc, cs0, cd0, ckey, w.A.tinit, g.list(0), g.kill, g.cnt, cst);  // list 0, its count, killers
-  };
-  if (split) {
-    prepare_cancels(1);
-    fork.mark_side();  // the geometry rounds wait for this point, not for the emission queued below
-  }
   {
     const hipStream_t es = fork.stream();
     launch_emit_points(nmin, min_ids, sb[0].to_out, o_pts[0], es);
@@ -850,9 +849,7 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     if (gmax) launch_labels_faces(rv, S, pjf, sb[1].root, row32(4), o_flab, es);
     if (gmin) launch_labels_verts(rv, N, pjv, sb[0].root, row32(3), o_vlab, es);
   }
-  for (int s = 0; s < 2; ++s)
-    if (aw[s].on && !(split && s == 1)) prepare_cancels(s);
-  if (split) fork.wait_side();
+  fork.wait_side();
   // geometry rounds of both sides, interleaved, one readback per batch of rounds
   {
     bool live[2] = {aw[0].on && aw[0].ncancel > 0, aw[1].on && aw[1].ncancel > 0};

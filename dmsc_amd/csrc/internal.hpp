@@ -362,6 +362,8 @@ void scan_plane_popcounts(const uint32_t* plane_a, const uint32_t* plane_b, int3
 // iota_vals: sort the pairs (keys[i], i); vals is then only written
 void radix_sort_pairs(uint32_t* keys, int32_t* vals, uint32_t* keys_tmp, int32_t* vals_tmp, int64_t n, Arena& ar,
                       int slot, int pslot, hipStream_t st, bool iota_vals = false);
+// rank[i] = position of element i in the stable ascending order of the 32-bit keys (keys only read)
+void radix_rank(const uint32_t* keys, int32_t* rank, int64_t n, Arena& ar, int slot, int pslot, hipStream_t st);
 void pointer_jump(uint64_t* pj, int64_t n, int32_t* err, hipStream_t st);
 void pointer_jump_init(const int32_t* succ, uint64_t* pj, int64_t n, hipStream_t st);
 void fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t st);

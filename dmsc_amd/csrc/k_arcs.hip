@@ -120,18 +120,6 @@ void launch_cancel_compact(int64_t ns, const int32_t* cancel_d, const float* per
                      keys);
   DMSC_LAUNCH_CHECK();
 }
-// Cancellations stay in saddle-id order (index j, spatially coherent); their sorted position is the
-// merge key: key[j] = position of j in (persistence, saddle id) order.
-__global__ void k_cancel_finish(int64_t nc, const int32_t* __restrict__ slot, int32_t* __restrict__ key) {
-  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x)
-    key[slot[c]] = (int)c;
-}
-void launch_cancel_finish(int64_t nc, const int32_t* slot, int32_t* key, hipStream_t st) {
-  if (nc == 0) return;
-  hipLaunchKernelGGL(k_cancel_finish, dim3(grid_for(nc, 256, 16384)), dim3(256), 0, st, nc, slot, key);
-  DMSC_LAUNCH_CHECK();
-}
-
 // ------------------------------------------------------------------ geometry rounds
 // initial worklist: every cancellation (saddle-id order) with its key and its two arc targets; and
 // kill[d] = (pending, key of the cancellation whose dead extremum d is)

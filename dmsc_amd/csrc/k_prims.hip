@@ -315,13 +315,14 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const void* __restrict__
 }
 
 // PIN / POUT: input / output packed (one uint64 array) or separate (key array, value array).
+// RANK: no sorted output; vout[value] = output position instead (the inverse permutation).
 // A tile is RS_ITEMS chunks of 256 elements in order; chunks are ranked RS_GROUP at a time: each wave
 // ranks its 64 elements of each chunk (peer masks from 8 ballots) and records per-digit counts per
 // (chunk, wave); one thread per digit then turns the counts into output offsets in element order;
 // four barriers per group of chunks.
 constexpr int RS_GROUP = 4;
 static_assert(RS_ITEMS % RS_GROUP == 0, "whole groups of chunks per tile");
-template <bool PIN, bool POUT>
+template <bool PIN, bool POUT, bool RANK = false>
 __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const void* __restrict__ kin, const int32_t* __restrict__ vin,
                                                            void* __restrict__ kout, int32_t* __restrict__ vout,
                                                            int64_t n, int shift, const int32_t* __restrict__ offs,
@@ -387,7 +388,9 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const void* __restric
       if (rank[c] < 0) continue;
       const int d = (k[c] >> shift) & 255;
       const int pos = slot[c][wid][d] + rank[c];
-      if (POUT) {
+      if (RANK) {
+        vout[v[c]] = pos;
+      } else if (POUT) {
         static_cast<uint64_t*>(kout)[pos] = ((uint64_t)k[c] << 32) | (uint32_t)v[c];
       } else {
         static_cast<uint32_t*>(kout)[pos] = k[c];
@@ -443,6 +446,34 @@ void radix_sort_pairs_bits(uint32_t* keys, int32_t* vals, uint32_t* keys_tmp, in
   pass(0, keys, vals_in, pk[0], nullptr, false, true);
   for (int p = 1; p + 1 < npass; ++p) pass(p, pk[(p - 1) & 1], nullptr, pk[p & 1], nullptr, true, true);
   pass(npass - 1, pk[(npass - 2) & 1], nullptr, keys, vals, true, false);
+}
+
+// rank[i] = position of element i in the stable ascending order of keys (keys are only read); 32-bit
+// keys, four passes, the last one scattering positions instead of pairs. Scratch as above.
+void radix_rank(const uint32_t* keys, int32_t* rank, int64_t n, Arena& ar, int slot, int pslot, hipStream_t st) {
+  KScope ks_("radix_sort", st);
+  if (n <= 1) {
+    iota_i32(rank, n, st);
+    return;
+  }
+  const int64_t nb = (n + RS_TILE - 1) / RS_TILE;
+  int32_t* counts = ar.get<int32_t>(slot, (size_t)(256 * nb));
+  uint64_t* pk[2] = {ar.get<uint64_t>(pslot, n), ar.get<uint64_t>(pslot + 1, n)};
+  const dim3 g((unsigned)nb), b(RS_THREADS);
+  for (int p = 0; p < 4; ++p) {
+    const void* kin = p == 0 ? (const void*)keys : (const void*)pk[(p - 1) & 1];
+    if (p == 0) hipLaunchKernelGGL(k_rs_hist<false>, g, b, 0, st, kin, n, 0, counts, nb);
+    else hipLaunchKernelGGL(k_rs_hist<true>, g, b, 0, st, kin, n, 8 * p, counts, nb);
+    DMSC_LAUNCH_CHECK();
+    scan_exclusive<int32_t>(counts, counts, 256 * nb, nullptr, ar, slot + 1, st);
+    if (p == 0)
+      hipLaunchKernelGGL((k_rs_scatter<false, true>), g, b, 0, st, kin, nullptr, pk[0], nullptr, n, 0, counts, nb);
+    else if (p < 3)
+      hipLaunchKernelGGL((k_rs_scatter<true, true>), g, b, 0, st, kin, nullptr, pk[p & 1], nullptr, n, 8 * p, counts, nb);
+    else
+      hipLaunchKernelGGL((k_rs_scatter<true, false, true>), g, b, 0, st, kin, nullptr, nullptr, rank, n, 24, counts, nb);
+    DMSC_LAUNCH_CHECK();
+  }
 }
 
 void radix_sort_pairs(uint32_t* keys, int32_t* vals, uint32_t* keys_tmp, int32_t* vals_tmp, int64_t n, Arena& ar,

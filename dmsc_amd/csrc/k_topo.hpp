@@ -152,8 +152,6 @@ constexpr uint64_t kKillPending = 0xFFFFFFFFull << 32;
 // killer word of every dead extremum
 void launch_geo_items(int64_t nc, const int32_t* cs, const int32_t* cd, const int32_t* key, const int32_t* tinit,
                       int4* wl, uint64_t* kill, int32_t* cnt, hipStream_t st);
-// key[slot[c]] = c: each cancellation's position in (persistence, saddle id) order
-void launch_cancel_finish(int64_t nc, const int32_t* slot, int32_t* key, hipStream_t st);
 void launch_geo_q(const GeoState& g, int64_t next, uint64_t* pjq, hipStream_t st);
 
 // Output arc lengths for surviving saddles in min order (ord: survivor list, position = output saddle);
