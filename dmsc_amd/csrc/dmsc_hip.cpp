@@ -489,8 +489,9 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
       DMSC_DTYPE_INT32);
 
   // ---- gradient, critical bit planes, successor forests
-  int32_t* fsucc = ar.get<int32_t>(A_FSUCC, S);
-  int32_t* vsucc = ar.get<int32_t>(A_VSUCC, N);
+  // successor arrays only for the traced sides (the raw-arc walks are their only readers)
+  int32_t* fsucc = tmax ? ar.get<int32_t>(A_FSUCC, S) : nullptr;
+  int32_t* vsucc = tmin ? ar.get<int32_t>(A_VSUCC, N) : nullptr;
   uint32_t* planes = ar.get<uint32_t>(A_PLANES, 4 * PW);
   uint64_t* pjf = ar.get<uint64_t>(A_PJF, S);
   uint64_t* pjv = ar.get<uint64_t>(A_PJV, N);

@@ -375,7 +375,7 @@ void bases_at(const T* pre, const S* starts, int64_t n, const T* total, int B, T
 // grid stages (over all fields of a batch)
 // also writes the initial pointer-jumping state of both V-path forests (pjf over slots, pjv over pixels)
 // tile_b: [B+1] first gradient tile of each field (64 x rows slots per tile, row-major within a field;
-// rows is 8, 16 or 32)
+// rows is 8, 16 or 32); pw, fsucc and vsucc may be null (not written)
 int gradient_tiles(int H, int W, int rows);
 void launch_gradient(bool dual, int rows, const Batch& bt, const int64_t* tile_b, int64_t ntiles, int32_t* pw,
                      int32_t* fsucc, int32_t* vsucc, uint32_t* planes, int64_t PW, uint64_t* pjf, uint64_t* pjv,

@@ -543,15 +543,17 @@ void launch_geo_init(const GeoState& g, int64_t next, hipStream_t st) {
 
 // ------------------------------------------------------------------ path lengths to the final root
 // pjq[e] = (final root or `next` for the void, length of path(e -> root)), by pointer jumping with
-// saturating sums over the geometry forest.
+// saturating sums over the geometry forest. It starts from the rounds' own shortcuts: gjump[e] is an
+// ancestor of e (its parent when nothing compressed it) with the saturated length to it.
 __global__ void k_geo_q_init(GeoState g, int64_t next, uint64_t* __restrict__ pj) {
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e <= next; e += (int64_t)gridDim.x * blockDim.x) {
     uint32_t J = (uint32_t)e, D = 0;
     if (e < next) {
-      const int2 pl = *reinterpret_cast<const int2*>(&g.link[e]);
-      if (pl.x != (int)e) {
-        J = pl.x < 0 ? (uint32_t)next : (uint32_t)pl.x;
-        D = (uint32_t)pl.y;
+      const uint64_t gj = g.gjump[e];
+      const uint32_t a = (uint32_t)gj;
+      if (a != (uint32_t)e) {
+        J = a == kVoid ? (uint32_t)next : a;
+        D = (uint32_t)(gj >> 32);
       }
     }
     pj[e] = (uint64_t)J | ((uint64_t)D << 32);

@@ -30,6 +30,13 @@ __device__ __forceinline__ uint32_t pj_join(uint32_t w, uint32_t w2) {
 
 constexpr int log2_exact(int n) { return n <= 1 ? 0 : 1 + log2_exact(n / 2); }
 
+// Bijection of block ids onto [0, n): the blocks of XCD k (ids k, k + 8, ...) get the k-th run of
+// consecutive ids, in order.
+__device__ __forceinline__ int xcd_run(int b, int n) {
+  const int q = n >> 3, r = n & 7, k = b & 7, j = b >> 3;
+  return k * q + (k < r ? k : r) + j;
+}
+
 // Lower star of one vertex as a 32-bit summary: edge mask (bits 0-3, L_edges order), face mask
 // (bits 4-7, L_faces order), paired edge estar (bits 8-10, 4 = none) and the face-queue result
 // (bits 11-18, 2 bits per face: 0 unpaired, 1 + which of its two edges). A vertex outside the image
@@ -93,9 +100,13 @@ __global__ __launch_bounds__(256) void k_gradient(Batch bt, const int64_t* __res
   float (*pix)[GP_W] = reinterpret_cast<float (*)[GP_W]>(stage);
   uint32_t (*sv)[SV_W] = reinterpret_cast<uint32_t (*)[SV_W]>(stage + GP_H * GP_W * 4);
 
-  const int tfield = field_of(tile_b, bt.B, blockIdx.x);
+  // Workgroups go to the 8 XCDs in turn (block b to XCD b mod 8); each XCD takes one contiguous run
+  // of tiles instead, so the tile rows' partial cache lines at tile edges (rows of Nx = W + 1 slots
+  // are not line-aligned) are completed in the same L2 by the neighbouring tile.
+  const int tile = xcd_run(blockIdx.x, gridDim.x);
+  const int tfield = field_of(tile_b, bt.B, tile);
   const FieldDesc fd = bt.fd[tfield];
-  const int64_t tlocal = blockIdx.x - tile_b[tfield];
+  const int64_t tlocal = tile - tile_b[tfield];
   const int ntx = (fd.Nx + GT_X - 1) / GT_X;
   const TileRef tr{tfield, (int)(tlocal % ntx), (int)(tlocal / ntx)};
   const float* __restrict__ f = fd.f;
@@ -104,8 +115,8 @@ __global__ __launch_bounds__(256) void k_gradient(Batch bt, const int64_t* __res
   const int x0 = tr.tx * GT_X, y0 = tr.ty * TY;
   const int sbase = (int)fd.slot_base, pbase = (int)fd.pix_base;  // batch totals fit int32 (host check)
   int32_t* __restrict__ pw = pw_all ? pw_all + 4 * fd.slot_base : nullptr;
-  int32_t* __restrict__ fsucc = fsucc_all + fd.slot_base;
-  int32_t* __restrict__ vsucc = vsucc_all + fd.pix_base;
+  int32_t* __restrict__ fsucc = fsucc_all ? fsucc_all + fd.slot_base : nullptr;  // null: not wanted
+  int32_t* __restrict__ vsucc = vsucc_all ? vsucc_all + fd.pix_base : nullptr;
   const int64_t wbase = fd.word_base;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
 
@@ -190,7 +201,7 @@ __global__ __launch_bounds__(256) void k_gradient(Batch bt, const int64_t* __res
           exits[0][e] = sbase + fs;
         }
       }
-      fsucc[q] = sbase + fs;
+      if (fsucc) fsucc[q] = sbase + fs;
       if (y < H && x < W) {
         const int p = y * W + x;
         int vs = p;
@@ -209,7 +220,7 @@ __global__ __launch_bounds__(256) void k_gradient(Batch bt, const int64_t* __res
             exits[1][e] = pbase + vs;
           }
         }
-        vsucc[p] = pbase + vs;
+        if (vsucc) vsucc[p] = pbase + vs;
       }
     }
     pjA[e] = (uint64_t)fw | (uint64_t)vw << 32;
@@ -487,42 +498,56 @@ void launch_saddle_records(const RankView& rv, const int32_t* sad_ids, const int
 // of the fields' label arrays (slot / pixel order).
 // o: output index of the extremum's thresholded root (-1: none)
 __device__ __forceinline__ int32_t label_of(int32_t o, int32_t base) { return o >= 0 ? o - base : -1; }
-__global__ void k_labels_faces(int64_t S, RankView rv, const uint64_t* __restrict__ pj,
-                               const int32_t* __restrict__ root_out, const int32_t* __restrict__ out_base,
-                               int32_t* __restrict__ out) {
-  for (int64_t q0 = (int64_t)blockIdx.x * blockDim.x; q0 < S; q0 += (int64_t)gridDim.x * blockDim.x) {
-    const int fb = field_of(rv.bt.slot_b, rv.bt.B, q0);  // the block's first slot: one uniform search
-    const int64_t q = q0 + threadIdx.x;
-    if (q >= S) continue;
-    const int fi = field_from(rv.bt.slot_b, rv.bt.B, fb, q);
-    out[q] = label_of(root_out[rank_max_slot(rv, fi, (int)(uint32_t)pj[q])], out_base[fi]);
-  }
-}
-__global__ void k_labels_verts(int64_t N, RankView rv, const uint64_t* __restrict__ pj,
-                               const int32_t* __restrict__ root_out, const int32_t* __restrict__ out_base,
-                               int32_t* __restrict__ out) {
-  for (int64_t p0 = (int64_t)blockIdx.x * blockDim.x; p0 < N; p0 += (int64_t)gridDim.x * blockDim.x) {
-    const int fb = field_of(rv.bt.pix_b, rv.bt.B, p0);  // the block's first pixel: one uniform search
-    const int64_t p = p0 + threadIdx.x;
-    if (p >= N) continue;
-    const int fi = field_from(rv.bt.pix_b, rv.bt.B, fb, p);
-    out[p] = label_of(root_out[rank_min_pixel(rv, fi, (int)(uint32_t)pj[p])], out_base[fi]);
+// Each thread takes kLabelUnroll cells of its block's chunk per iteration, one per block-wide stride,
+// and issues every cell's loads before the dependent ones (successor entry, then the rank words, then
+// the root's output index), so each wave keeps several of those chains in flight.
+constexpr int kLabelUnroll = 4;
+template <bool FACES>
+__global__ __launch_bounds__(256) void k_labels(int64_t n, RankView rv, const uint64_t* __restrict__ pj,
+                                                const int32_t* __restrict__ root_out,
+                                                const int32_t* __restrict__ out_base, int32_t* __restrict__ out) {
+  const int64_t* const base = FACES ? rv.bt.slot_b : rv.bt.pix_b;
+  constexpr int U = kLabelUnroll;
+  for (int64_t c0 = (int64_t)blockIdx.x * (256 * U); c0 < n; c0 += (int64_t)gridDim.x * (256 * U)) {
+    const int fb = field_of(base, rv.bt.B, c0);  // the chunk's first cell: one uniform search
+    int fi[U];
+    uint64_t w[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t q = c0 + u * 256 + threadIdx.x;
+      fi[u] = q < n ? field_from(base, rv.bt.B, fb, q) : fb;
+      w[u] = q < n ? pj[q] : 0;
+    }
+    int32_t rk[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int t = (int)(uint32_t)w[u];
+      rk[u] = c0 + u * 256 + threadIdx.x < n ? (FACES ? rank_max_slot(rv, fi[u], t) : rank_min_pixel(rv, fi[u], t)) : 0;
+    }
+    int32_t o[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) o[u] = root_out[rk[u]];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t q = c0 + u * 256 + threadIdx.x;
+      if (q < n) out[q] = label_of(o[u], out_base[fi[u]]);
+    }
   }
 }
 void launch_labels_faces(const RankView& rv, int64_t S, const uint64_t* pj_face, const int32_t* root_out,
                          const int32_t* out_base, int32_t* out, hipStream_t st) {
   KScope ks_("labels_faces", st);
   if (S == 0) return;
-  hipLaunchKernelGGL(k_labels_faces, dim3(grid_for(S, 256, 16384)), dim3(256), 0, st, S, rv, pj_face, root_out,
-                     out_base, out);
+  hipLaunchKernelGGL(k_labels<true>, dim3(grid_for((S + kLabelUnroll - 1) / kLabelUnroll, 256, 16384)), dim3(256), 0, st,
+                     S, rv, pj_face, root_out, out_base, out);
   DMSC_LAUNCH_CHECK();
 }
 void launch_labels_verts(const RankView& rv, int64_t N, const uint64_t* pj_vert, const int32_t* root_out,
                          const int32_t* out_base, int32_t* out, hipStream_t st) {
   KScope ks_("labels_verts", st);
   if (N == 0) return;
-  hipLaunchKernelGGL(k_labels_verts, dim3(grid_for(N, 256, 16384)), dim3(256), 0, st, N, rv, pj_vert, root_out,
-                     out_base, out);
+  hipLaunchKernelGGL(k_labels<false>, dim3(grid_for((N + kLabelUnroll - 1) / kLabelUnroll, 256, 16384)), dim3(256), 0,
+                     st, N, rv, pj_vert, root_out, out_base, out);
   DMSC_LAUNCH_CHECK();
 }
 
