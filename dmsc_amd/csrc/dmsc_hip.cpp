@@ -29,7 +29,8 @@ namespace dmsc {
 
 void fill_u8(uint8_t* p, uint8_t v, int64_t n, hipStream_t st);
 void fill_f32_zero(float* p, int64_t n, hipStream_t st);
-void scan_i32_to_i64(const int32_t* in, int64_t* out, int64_t n, int64_t* d_total, Arena& ar, int slot, hipStream_t st);
+void scan_i32_to_i64(const int32_t* in, int64_t* out, int64_t n, int64_t* d_total, Arena& ar, int slot, hipStream_t st,
+                     int32_t min_len = 0);
 
 // arena slots
 enum : int {
@@ -45,7 +46,7 @@ enum : int {
   A_ASM0 = A_SIDE_END,  // assembly: counts, prefixes, output map, survivor lists, edge outputs (16 slots)
   A_ASM_END = A_ASM0 + 16,
   A_ARC0 = A_ASM_END,  // per arc side: kArcSlots slots each
-  A_ARC_END = A_ARC0 + 2 * 20,
+  A_ARC_END = A_ARC0 + 2 * 21,
 };
 
 enum SideSlot { P_PARG, P_PART, P_ALIVE, P_MINKEY, P_PAIR, P_PERS, P_CANCEL, P_RR, P_WL, P_CNT, P_ROOT, P_TOOUT,
@@ -53,7 +54,7 @@ enum SideSlot { P_PARG, P_PART, P_ALIVE, P_MINKEY, P_PAIR, P_PERS, P_CANCEL, P_R
 static_assert(A_SIDE_END - A_SIDE0 == 2 * kSideSlots, "side slot count");
 enum ArcSlot {
   G_BLEN, G_BOFF, G_ARENA, G_FLAG, G_PRE, G_CS0, G_CD0, G_KEYS, G_CS, G_CD, G_DEADC,
-  G_LINK, G_DONE, G_WATCH, G_RR, G_WL, G_CNT, G_JUMP, G_GLEN, G_TL, kArcSlots
+  G_LINK, G_DONE, G_WATCH, G_RR, G_WL, G_CNT, G_JUMP, G_GLEN, G_TL, G_HEAD, kArcSlots
 };
 static_assert(A_ARC_END - A_ARC0 == 2 * kArcSlots, "arc slot count");
 enum ArcSlot2 { H_PJQ = 0, H_FLEN, H_PERPOS, H_POSPRE, H_SEGA, H_SEGB, H_RAW, H_NSEG, H_CUMR, H_CUMC, H_ARCB, H_REC, H_XL };
@@ -574,7 +575,8 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     int64_t* boff = ar.get<int64_t>(b + G_BOFF, 2 * nsad);
     int4* tl = ar.get<int4>(b + G_TL, nsad);
     launch_arc_lengths(nsad, s ? cmax : cmin, s ? lmax : lmin, blen, tl, st);
-    scan_i32_to_i64(blen, boff, 2 * nsad, c64 + 2 * s, ar, A_SCAN, st);
+    // arena offsets of the arcs longer than kHeadCells (the shorter ones live in their head records)
+    scan_i32_to_i64(blen, boff, 2 * nsad, c64 + 2 * s, ar, A_SCAN, st, kHeadCells + 1);
     w.A.faces = s;
     w.A.succ = s ? fsucc : vsucc;
     w.A.fd = bt.fd;
@@ -586,6 +588,7 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     w.A.blen = blen;
     w.A.tl = tl;
     w.A.boff = boff;
+    w.A.head = ar.get<int4>(b + G_HEAD, 2 * nsad);
   }
   StreamFork fork(ctx, st, tmin || tmax);
   // arena sizes (c64[0], c64[2]) arrive with the first readback of the rounds, at pinned byte 256

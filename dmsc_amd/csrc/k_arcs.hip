@@ -67,7 +67,9 @@ __device__ __forceinline__ int32_t cell_of(const ArcSide& A, const FieldDesc& fd
   return 4 * (y * fd.Nx + x);
 }
 
-// one thread per arc: walk the successor forest from the start cell to the extremum
+// one thread per arc: walk the successor forest from the start cell to the extremum; an arc of at
+// most kHeadCells cells goes to its head record (one 16-byte store, indexed by arc), a longer one to
+// the arena
 __global__ void k_raw_arcs(ArcSide A) {
   for (int64_t a = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; a < 2 * A.ns; a += (int64_t)gridDim.x * blockDim.x) {
     const int len = A.blen[a];
@@ -76,6 +78,17 @@ __global__ void k_raw_arcs(ArcSide A) {
     const FieldDesc& fd = A.fd[A.sfield ? A.sfield[s] : 0];
     const int base = (int)(A.faces ? fd.slot_base : fd.pix_base);
     int cur = base + arc_start(A, fd, s, (int)(a & 1));
+    if (len <= kHeadCells) {
+      int32_t h[kHeadCells] = {-1, -1, -1, -1};
+#pragma unroll
+      for (int j = 0; j < kHeadCells; ++j)
+        if (j < len) {
+          h[j] = cell_of(A, fd, cur - base);
+          if (j + 1 < len) cur = A.succ[cur];
+        }
+      A.head[a] = make_int4(h[0], h[1], h[2], h[3]);
+      continue;
+    }
     int32_t* o = A.arena + A.boff[a];
     for (int j = 0; j < len; ++j) {
       o[j] = cell_of(A, fd, cur - base);
@@ -846,8 +859,9 @@ void launch_expand_segments(const ArcSide& a, const GeoState& g, const uint64_t*
   DMSC_LAUNCH_CHECK();
 }
 
-// Raw pieces -> output cells. A wave takes 64 pieces, lays the cells of its short pieces end to end
-// (prefix sum of lengths) and copies that run 64 cells at a time, so reads and writes are contiguous
+// Raw pieces -> output cells. A piece of at most kHeadCells cells is written by its own lane from the
+// arc's head record. A wave lays the cells of its other short pieces end to end (prefix sum of
+// lengths) and copies that run 64 cells at a time from the arena, so reads and writes are contiguous
 // within each piece; pieces of kLongPiece cells or more are then copied one at a time by the whole
 // wave, without the per-cell piece search.
 constexpr int32_t kLongPiece = 128;
@@ -860,24 +874,37 @@ __global__ __launch_bounds__(256) void k_copy_raw(ArcSide A, const Seg* __restri
   DMSC_WAVE_LOOP(j, (int64_t)n) {
     const int64_t jj = j_base + lane;
     int32_t len = 0;
-    int64_t dst = 0, src = 0;
-    int fwd = 1;
+    int64_t dst = 0;
+    int fwd = 1, arc = 0;
     if (jj < n) {
       const Seg sg = raw[jj];
       len = sg.b;  // raw pieces carry their length
       dst = sg.od >> 1;
       fwd = (int)(sg.od & 1);
-      src = A.boff[sg.a];
+      arc = sg.a;
       if (dst < 0 || dst + len > out_total) {  // inconsistent offsets: never write out of bounds
         report(err, ERR_SEG_CAP);
         len = 0;
       }
     }
+    const bool hd = len <= kHeadCells;
+    int64_t src = 0;
+    if (hd) {
+      if (len > 0) {
+        const int4 h4 = A.head[arc];
+        const int32_t h[kHeadCells] = {h4.x, h4.y, h4.z, h4.w};
+#pragma unroll
+        for (int t = 0; t < kHeadCells; ++t)
+          if (t < len) out[dst + (fwd ? t : len - 1 - t)] = h[t];
+      }
+    } else {
+      src = A.boff[arc];
+    }
     // reversed pieces read src + len - 1 - t: encode as a negative start
     const int64_t srcv = fwd ? src : ~(src + len - 1);
     // long pieces are copied one at a time by the whole wave (below); the others are packed
     const bool lng = len >= kLongPiece;
-    const int32_t slen = lng ? 0 : len;
+    const int32_t slen = (lng || hd) ? 0 : len;
     const int32_t incl = wave_inclusive_sum(slen);
     const int32_t start = incl - slen;
     const int32_t total = __shfl(incl, 63, 64);

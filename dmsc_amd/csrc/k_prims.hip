@@ -86,17 +86,21 @@ struct LoadPop {
     }
   }
 };
+// int32 -> int64; values below min_len count as 0
 struct LoadWiden {
   const int32_t* p;
   bool vec;
-  __device__ int64_t operator()(int64_t i) const { return (int64_t)p[i]; }
+  int32_t min_len;
+  __device__ int64_t keep(int32_t x) const { return x < min_len ? 0 : (int64_t)x; }
+  __device__ int64_t operator()(int64_t i) const { return keep(p[i]); }
   __device__ void load8(int64_t base, int64_t n, int64_t (&v)[8]) const {
     if (vec && base + 8 <= n) {
       const int4 a = *reinterpret_cast<const int4*>(p + base), b = *reinterpret_cast<const int4*>(p + base + 4);
-      v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+      v[0] = keep(a.x); v[1] = keep(a.y); v[2] = keep(a.z); v[3] = keep(a.w);
+      v[4] = keep(b.x); v[5] = keep(b.y); v[6] = keep(b.z); v[7] = keep(b.w);
     } else {
 #pragma unroll
-      for (int j = 0; j < 8; ++j) v[j] = base + j < n ? (int64_t)p[base + j] : 0;
+      for (int j = 0; j < 8; ++j) v[j] = base + j < n ? keep(p[base + j]) : 0;
     }
   }
 };
@@ -281,8 +285,9 @@ void scan_u8_flags(const uint8_t* in, int32_t* out, int64_t n, int32_t* d_total,
   scan_impl<int32_t>(LoadU8{in, (reinterpret_cast<uintptr_t>(in) & 7) == 0}, out, n, d_total, ar, slot, st);
 }
 
-void scan_i32_to_i64(const int32_t* in, int64_t* out, int64_t n, int64_t* d_total, Arena& ar, int slot, hipStream_t st) {
-  scan_impl<int64_t>(LoadWiden{in, aligned16(in)}, out, n, d_total, ar, slot, st);
+void scan_i32_to_i64(const int32_t* in, int64_t* out, int64_t n, int64_t* d_total, Arena& ar, int slot, hipStream_t st,
+                     int32_t min_len) {
+  scan_impl<int64_t>(LoadWiden{in, aligned16(in), min_len}, out, n, d_total, ar, slot, st);
 }
 
 // ------------------------------------------------------------------ radix sort

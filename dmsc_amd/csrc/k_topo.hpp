@@ -104,9 +104,11 @@ struct ArcSide {
   const int32_t* blen;     // per arc: raw length in cells (steps + 1), 0 = no arc
   const int4* tl;          // per saddle: both arcs' targets and raw lengths (tinit[2s], tinit[2s+1], blen[2s],
                            // blen[2s+1]) in one 16-byte record for the stages that gather by saddle
-  const int64_t* boff;     // per arc: offset in the raw arena
-  int32_t* arena;          // raw arcs, cell ids from the saddle outwards
+  const int64_t* boff;     // per arc longer than kHeadCells: offset in the raw arena
+  int32_t* arena;          // raw arcs longer than kHeadCells, cell ids from the saddle outwards
+  int4* head;              // per arc of at most kHeadCells cells: its cells (the arena holds the longer ones)
 };
+constexpr int kHeadCells = 4;
 
 // per saddle: blen[2s..2s+1] and tl[s] (targets c[2s..2s+1] with those lengths)
 void launch_arc_lengths(int64_t ns, const int32_t* c, const int32_t* steps, int32_t* blen, int4* tl, hipStream_t st);
