@@ -529,7 +529,7 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
   int32_t* sad_field = B > 1 ? ar.get<int32_t>(A_SADFIELD, nsad) : nullptr;
   int32_t* max_ids = ar.get<int32_t>(A_MAXIDS, nmax);
   uint64_t* max_key = ar.get<uint64_t>(A_MAXVAL, nmax);
-  launch_compact_lists(dual, rv, b32, min_ids, min_key, sad_ids, sad_val, sad_field, max_ids, max_key, st);
+  launch_compact_lists(dual, rv, min_ids, min_key, sad_ids, sad_val, sad_field, max_ids, max_key, st);
 
   // ---- V-path forests: terminal extremum and depth of every face / vertex (from the tile-resolved state)
   pointer_jump(pjf, S, d_err, st);

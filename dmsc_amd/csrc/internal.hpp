@@ -380,9 +380,9 @@ int gradient_tiles(int H, int W, int rows);
 void launch_gradient(bool dual, int rows, const Batch& bt, const int64_t* tile_b, int64_t ntiles, int32_t* pw,
                      int32_t* fsucc, int32_t* vsucc, uint32_t* planes, int64_t PW, uint64_t* pjf, uint64_t* pjv,
                      hipStream_t st);
-// min_key / max_key: pack_key(value, cell id) per extremum; list_b: rows of B+1 per-field list bases
-// (minima, saddles, maxima)
-void launch_compact_lists(bool dual, const RankView& rv, const int32_t* list_b, int32_t* min_ids, uint64_t* min_key,
+// min_key / max_key: pack_key(value, cell id) per extremum; sad_field: field of each saddle (null: not
+// wanted)
+void launch_compact_lists(bool dual, const RankView& rv, int32_t* min_ids, uint64_t* min_key,
                           int32_t* sad_ids, float* sad_val, int32_t* sad_field, int32_t* max_ids, uint64_t* max_key,
                           hipStream_t st);
 void launch_saddle_records(const RankView& rv, const int32_t* sad_ids, const int32_t* sad_field, int64_t nsad,

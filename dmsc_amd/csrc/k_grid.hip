@@ -338,20 +338,24 @@ __device__ __forceinline__ int rank_min_pixel(const RankView& rv, int f, int p) 
 // ------------------------------------------------------------------ critical lists
 // Set bits in id order (the reference's single-thread order, critical_points.hxx:14-65), with the
 // extremum key / saddle value. A wave takes 64 consecutive plane words: each lane counts its word's
-// entries, a wave prefix sum places them, the lanes write the entries' cell ids into an LDS stage,
-// and the wave then writes the chunk's part of each list with consecutive lanes on consecutive
-// entries (one lane per word would scatter 4-byte stores over 64 cache lines). An entry's field
-// comes from its list position (per-field list bases), its value from its cell id.
+// entries, a wave prefix sum places them, the lanes write the entries into an LDS stage as 16-bit
+// (word, bit, type) codes beside a table of each word's row, first column and field, and the wave
+// then writes the chunk's part of each list with consecutive lanes on consecutive entries (one lane
+// per word would scatter 4-byte stores over 64 cache lines). No entry needs a division or a field
+// search: its row, column and field come from its word's table entry.
 constexpr int kCompactChunk = 64;  // words per wave
-constexpr int kCompactWaves = 2;   // waves per block (the stage holds 2 saddles per slot: 16 KB a wave)
+constexpr int kCompactWaves = 2;   // waves per block (the stage holds 2 saddles per slot: 8 KB a wave)
 
 template <bool DUAL>
-__global__ __launch_bounds__(64 * kCompactWaves) void k_compact_lists(RankView rv, const int32_t* __restrict__ list_b,
-                                                       int32_t* __restrict__ min_ids, uint64_t* __restrict__ min_key,
-                                                       int32_t* __restrict__ sad_ids, float* __restrict__ sad_val,
-                                                       int32_t* __restrict__ sad_field, int32_t* __restrict__ max_ids,
-                                                       uint64_t* __restrict__ max_key) {
-  __shared__ int32_t stage[kCompactWaves][kCompactChunk * 32 * 2];  // up to 2 saddles per slot
+__global__ __launch_bounds__(64 * kCompactWaves) void k_compact_lists(RankView rv, int32_t* __restrict__ min_ids,
+                                                                    uint64_t* __restrict__ min_key,
+                                                                    int32_t* __restrict__ sad_ids, float* __restrict__ sad_val,
+                                                                    int32_t* __restrict__ sad_field,
+                                                                    int32_t* __restrict__ max_ids,
+                                                                    uint64_t* __restrict__ max_key) {
+  // code = word in chunk << 7 | bit << 2 | cell type
+  __shared__ uint16_t stage[kCompactWaves][kCompactChunk * 32 * 2];  // up to 2 saddles per slot
+  __shared__ int4 winfo[kCompactWaves][kCompactChunk];               // (row, first column, field, -)
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int B = rv.bt.B;
   const int64_t nchunks = (rv.PW + kCompactChunk - 1) / kCompactChunk;
@@ -361,14 +365,12 @@ __global__ __launch_bounds__(64 * kCompactWaves) void k_compact_lists(RankView r
     const int64_t w = chunk * kCompactChunk + lane;
     const bool valid = chunk < nchunks && w < rv.PW;
     uint32_t m[4] = {0, 0, 0, 0};
-    int Nx = 0, xb = 0, yrow = 0;
     if (valid) {
       const int fi = field_of(rv.bt.word_b, B, w);
       const FieldDesc& fd = rv.bt.fd[fi];
       const int64_t lw = w - fd.word_base;
-      yrow = (int)(lw / fd.WPR);
-      xb = (int)(lw - (int64_t)yrow * fd.WPR) * 32;
-      Nx = fd.Nx;
+      const int yrow = (int)(lw / fd.WPR);
+      winfo[wid][lane] = make_int4(yrow, (int)(lw - (int64_t)yrow * fd.WPR) * 32, fi, 0);
 #pragma unroll
       for (int t = 0; t < 4; ++t) m[t] = rv.planes[t * rv.PW + w];
     }
@@ -386,26 +388,25 @@ __global__ __launch_bounds__(64 * kCompactWaves) void k_compact_lists(RankView r
       while (mm) {
         const int bit = __builtin_ctz(mm);
         mm &= mm - 1;
-        const int q = yrow * Nx + xb + bit;
-        if (list == 0) stage[wid][o++] = 4 * q;
-        else if (list == 2) stage[wid][o++] = 4 * q + 3;
+        const int c = lane << 7 | bit << 2;
+        if (list == 0) stage[wid][o++] = (uint16_t)c;
+        else if (list == 2) stage[wid][o++] = (uint16_t)(c | 3);
         else {
-          if (ma >> bit & 1) stage[wid][o++] = 4 * q + 1;
-          if (mb >> bit & 1) stage[wid][o++] = 4 * q + 2;
+          if (ma >> bit & 1) stage[wid][o++] = (uint16_t)(c | 1);
+          if (mb >> bit & 1) stage[wid][o++] = (uint16_t)(c | 2);
         }
       }
       __syncthreads();
       if (chunk < nchunks && total > 0) {
         const int32_t* pre = list == 0 ? rv.pre_min : list == 1 ? rv.pre_sad : rv.pre_max;
-        const int32_t* lb = list_b + list * (B + 1);
         const int64_t base = pre[w0];
-        const int fw = field_of64(lb, B, base);  // the chunk's first entry: one wave-uniform search
         for (int t = lane; t < total; t += 64) {
-          const int id = stage[wid][t];
+          const int c = stage[wid][t];
+          const int4 wi = winfo[wid][c >> 7];
+          const int ty = wi.x, tx = wi.y + ((c >> 2) & 31), type = c & 3;
+          const FieldDesc& fd = rv.bt.fd[wi.z];
+          const int id = 4 * (ty * fd.Nx + tx) + type;
           const int64_t oi = base + t;
-          const int fi = field_from(lb, B, fw, oi);
-          const FieldDesc& fd = rv.bt.fd[fi];
-          const int q = id >> 2, ty = q / fd.Nx, tx = q - ty * fd.Nx;
           if (list == 0) {
             min_ids[oi] = id;
             min_key[oi] = pack_key(fd.f[(int64_t)ty * fd.W + tx], id);
@@ -414,8 +415,8 @@ __global__ __launch_bounds__(64 * kCompactWaves) void k_compact_lists(RankView r
             max_key[oi] = pack_key(fd.f[top_pixel<DUAL>(3, ty, tx, fd.H, fd.W, fd.f)], id);
           } else {
             sad_ids[oi] = id;
-            sad_val[oi] = fd.f[top_pixel<DUAL>(id & 3, ty, tx, fd.H, fd.W, fd.f)];
-            if (sad_field) sad_field[oi] = fi;
+            sad_val[oi] = fd.f[top_pixel<DUAL>(type, ty, tx, fd.H, fd.W, fd.f)];
+            if (sad_field) sad_field[oi] = wi.z;
           }
         }
       }
@@ -424,17 +425,16 @@ __global__ __launch_bounds__(64 * kCompactWaves) void k_compact_lists(RankView r
   }
 }
 
-void launch_compact_lists(bool dual, const RankView& rv, const int32_t* list_b, int32_t* min_ids, uint64_t* min_key,
-                          int32_t* sad_ids, float* sad_val, int32_t* sad_field, int32_t* max_ids, uint64_t* max_key,
-                          hipStream_t st) {
+void launch_compact_lists(bool dual, const RankView& rv, int32_t* min_ids, uint64_t* min_key, int32_t* sad_ids,
+                          float* sad_val, int32_t* sad_field, int32_t* max_ids, uint64_t* max_key, hipStream_t st) {
   KScope ks_("compact_lists", st);
   if (rv.PW == 0) return;
   const int g = grid_for((rv.PW + kCompactChunk - 1) / kCompactChunk, kCompactWaves, 16384);
   if (dual)
-    hipLaunchKernelGGL(k_compact_lists<true>, dim3(g), dim3(64 * kCompactWaves), 0, st, rv, list_b, min_ids, min_key, sad_ids, sad_val,
-                       sad_field, max_ids, max_key);
+    hipLaunchKernelGGL(k_compact_lists<true>, dim3(g), dim3(64 * kCompactWaves), 0, st, rv, min_ids, min_key, sad_ids,
+                       sad_val, sad_field, max_ids, max_key);
   else
-    hipLaunchKernelGGL(k_compact_lists<false>, dim3(g), dim3(64 * kCompactWaves), 0, st, rv, list_b, min_ids, min_key, sad_ids,
+    hipLaunchKernelGGL(k_compact_lists<false>, dim3(g), dim3(64 * kCompactWaves), 0, st, rv, min_ids, min_key, sad_ids,
                        sad_val, sad_field, max_ids, max_key);
   DMSC_LAUNCH_CHECK();
 }
