@@ -554,13 +554,13 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     while ((1 << bits) < B) ++bits;
     radix_sort_pairs_bits(keys, asc, keys2, vals2, nsad, bits, ar, A_RS0, A_RSP0, st);
   }
-  int32_t* ord_min = ar.get<int32_t>(A_ORDMIN, nsad);
-  int32_t* ord_max = ar.get<int32_t>(A_ORDMAX, nsad);
-  // primal: min side ascending, max side descending; dual: the reverse (arcs_topology.hxx:178-181)
+  // primal: min side ascending, max side descending; dual: the reverse (arcs_topology.hxx:178-181).
+  // The ascending side's order is `asc` itself.
   // positions in the ascending order only: the descending side mirrors them within the field's range
+  int32_t* const ord_min = dual ? ar.get<int32_t>(A_ORDMIN, nsad) : asc;
+  int32_t* const ord_max = dual ? asc : ar.get<int32_t>(A_ORDMAX, nsad);
   int32_t* pos_asc = ar.get<int32_t>(A_POSMIN, nsad);
-  if (!dual) launch_orders(nsad, asc, B, sad_b, ord_min, ord_max, pos_asc, nullptr, st);
-  else launch_orders(nsad, asc, B, sad_b, ord_max, ord_min, pos_asc, nullptr, st);
+  launch_orders(nsad, asc, B, sad_b, nullptr, dual ? ord_min : ord_max, pos_asc, nullptr, st);
 
   // ---- raw arc lengths of the traced sides. The raw arcs need no persistence result: they are
   // traced on the side stream while the persistence rounds run on `st`.

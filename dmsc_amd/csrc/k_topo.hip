@@ -749,19 +749,24 @@ void launch_emit_side(int64_t ns, const int32_t* ord, const uint8_t* removed, co
 // position). `asc` is grouped by field, ascending (value, id) inside each group.
 // The orders are grouped by field, so position k belongs to the field whose position range holds
 // it: a binary search over the B + 1 field bases (cached) instead of a gather by saddle.
+// Consecutive positions of a block share a field (orders are grouped by field): the block searches
+// the field bases once for its first position and each thread steps forward from there.
 __global__ void k_orders(int64_t ns, const int32_t* __restrict__ asc, int B, const int32_t* __restrict__ sad_b,
                          int32_t* __restrict__ fwd, int32_t* __restrict__ rev, int32_t* __restrict__ pos_fwd,
                          int32_t* __restrict__ pos_rev) {
-  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < ns; k += (int64_t)gridDim.x * blockDim.x) {
+  for (int64_t k0 = (int64_t)blockIdx.x * blockDim.x; k0 < ns; k0 += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t k = k0 + threadIdx.x;
+    const int fb = B > 1 ? field_of64(sad_b, B, k0) : 0;
+    if (k >= ns) continue;
     const int v = asc[k];
     int64_t lo = 0, hi = ns;
     if (B > 1) {
-      const int fi = field_of64(sad_b, B, k);
+      const int fi = field_from(sad_b, B, fb, k);
       lo = sad_b[fi];
       hi = sad_b[fi + 1];
     }
     const int64_t r = lo + hi - 1 - k;
-    fwd[k] = v;
+    if (fwd) fwd[k] = v;
     rev[r] = v;
     pos_fwd[v] = (int)k;
     if (pos_rev) pos_rev[v] = (int)r;
@@ -776,16 +781,24 @@ void launch_orders(int64_t ns, const int32_t* asc, int B, const int32_t* sad_b, 
 }
 
 // keys[k] = field of saddle idx[k] (the second, field-major pass of the saddle sort). Saddles are
-// numbered field by field, so that is a binary search over the B + 1 saddle bases (cached) rather than
-// a gather from a per-saddle array.
+// numbered field by field, so that is a binary search over the B + 1 saddle bases (in LDS when they
+// fit) rather than a gather from a per-saddle array.
+constexpr int kLdsBases = 4096;
 __global__ void k_gather_field(int64_t n, const int32_t* __restrict__ idx, int B, const int32_t* __restrict__ sad_b,
                                uint32_t* __restrict__ keys) {
+  __shared__ int32_t lb[kLdsBases];
+  const bool in_lds = B < kLdsBases;
+  if (in_lds) {
+    for (int i = threadIdx.x; i <= B; i += blockDim.x) lb[i] = sad_b[i];
+    __syncthreads();
+  }
+  const int32_t* const base = in_lds ? lb : sad_b;
   for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x)
-    keys[k] = (uint32_t)field_of64(sad_b, B, idx[k]);
+    keys[k] = (uint32_t)field_of64(base, B, idx[k]);
 }
 void launch_gather_field(int64_t n, const int32_t* idx, int B, const int32_t* sad_b, uint32_t* keys, hipStream_t st) {
   if (n == 0) return;
-  hipLaunchKernelGGL(k_gather_field, dim3(grid_for(n, 256, 16384)), dim3(256), 0, st, n, idx, B, sad_b, keys);
+  hipLaunchKernelGGL(k_gather_field, dim3(grid_for(n, 256, 4096)), dim3(256), 0, st, n, idx, B, sad_b, keys);
   DMSC_LAUNCH_CHECK();
 }
 
