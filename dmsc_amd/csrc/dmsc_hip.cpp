@@ -110,7 +110,9 @@ struct dmsc_context {
   Tunables tune;
   hipStream_t cap_stream = nullptr;  // private stream that round batches are captured on
   hipStream_t side = nullptr;        // output emission, concurrent with the arc geometry
+  hipStream_t late = nullptr;        // labels, concurrent with the last geometry rounds
   hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_mark = nullptr, ev_side = nullptr;
+  hipEvent_t ev_late = nullptr, ev_late_join = nullptr;
   Arena ar;
   Arena ar2;  // arc emission scratch
   int64_t* pinned = nullptr;
@@ -322,16 +324,42 @@ struct StreamFork {
   void wait_side() {
     if (on) DMSC_CHECK(hipStreamWaitEvent(st, ctx->ev_side, 0));
   }
+  // A third stream (default priority) for work that waits for `st` at the point late_mark() was
+  // called and is queued later, so that it runs beside whatever `st` runs by then.
+  bool late_on = false;
+  void late_mark() {
+    if (!on) return;
+    if (!ctx->ev_late) DMSC_CHECK(hipEventCreateWithFlags(&ctx->ev_late, hipEventDisableTiming));
+    DMSC_CHECK(hipEventRecord(ctx->ev_late, st));
+  }
+  hipStream_t late_stream() {
+    if (!on) return st;
+    if (!late_on) {
+      if (!ctx->late) DMSC_CHECK(hipStreamCreateWithFlags(&ctx->late, hipStreamNonBlocking));
+      if (!ctx->ev_late_join) DMSC_CHECK(hipEventCreateWithFlags(&ctx->ev_late_join, hipEventDisableTiming));
+      DMSC_CHECK(hipStreamWaitEvent(ctx->late, ctx->ev_late, 0));
+      late_on = true;
+    }
+    return ctx->late;
+  }
   void join() {
     if (!on) return;
     on = false;
     DMSC_CHECK(hipEventRecord(ctx->ev_join, ctx->side));
     DMSC_CHECK(hipStreamWaitEvent(st, ctx->ev_join, 0));
+    if (late_on) {
+      late_on = false;
+      DMSC_CHECK(hipEventRecord(ctx->ev_late_join, ctx->late));
+      DMSC_CHECK(hipStreamWaitEvent(st, ctx->ev_late_join, 0));
+    }
   }
   ~StreamFork() {
     if (!on) return;
     if (hipEventRecord(ctx->ev_join, ctx->side) != hipSuccess || hipStreamWaitEvent(st, ctx->ev_join, 0) != hipSuccess)
       (void)hipStreamSynchronize(ctx->side);
+    if (late_on && (hipEventRecord(ctx->ev_late_join, ctx->late) != hipSuccess ||
+                    hipStreamWaitEvent(st, ctx->ev_late_join, 0) != hipSuccess))
+      (void)hipStreamSynchronize(ctx->late);
   }
 };
 
@@ -850,9 +878,18 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
                      sb[0].pair, surv_rows[0], sad_ids, eb[0], o_p[0], o_pp[0], o_e[0], o_sad, es);
     launch_emit_side(n_sad_total, surv_list[1], nullptr, out_list_mx, edge_list[1], sb[1].root, sb[1].pers,
                      sb[1].pair, surv_rows[1], sad_ids, eb[1], o_p[1], o_pp[1], o_e[1], nullptr, es);
-    if (gmax) launch_labels_faces(rv, S, pjf, sb[1].root, row32(4), o_flab, es);
-    if (gmin) launch_labels_verts(rv, N, pjv, sb[0].root, row32(3), o_vlab, es);
   }
+  // The labels read only persistence and assembly results. They are bandwidth-bound, so they wait
+  // for the geometry rounds to fall to a sixteenth of their cancellations (the latency-bound rounds)
+  // and then run on a stream of their own beside them.
+  fork.late_mark();
+  bool labels_queued = false;
+  auto queue_labels = [&](hipStream_t ls) {
+    if (labels_queued) return;
+    labels_queued = true;
+    if (gmax) launch_labels_faces(rv, S, pjf, sb[1].root, row32(4), o_flab, ls);
+    if (gmin) launch_labels_verts(rv, N, pjv, sb[0].root, row32(3), o_vlab, ls);
+  };
   fork.wait_side();
   // geometry rounds of both sides, interleaved, one readback per batch of rounds
   {
@@ -909,6 +946,8 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
       }
       const int32_t* hp = (const int32_t*)ctx->pinned;  // pending per side
       ctx->geo_log.insert(ctx->geo_log.end(), {r, live[0] ? hp[0] : 0, live[1] ? hp[1] : 0});
+      if (16 * ((live[0] ? (int64_t)hp[0] : 0) + (live[1] ? (int64_t)hp[1] : 0)) <= aw[0].ncancel + aw[1].ncancel)
+        queue_labels(fork.late_stream());
       for (int s = 0; s < 2; ++s)
         if (live[s]) {
           ctx->rounds_geo[s] = r;
@@ -918,6 +957,7 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
         }
     }
   }
+  queue_labels(fork.late_stream());  // when no geometry round ran
   if (split) fork.follow();  // the side stream waits for the geometry rounds
   for (int s = 0; s < 2; ++s) {
     if (!aw[s].on) continue;
@@ -1268,6 +1308,9 @@ void dmsc_context_destroy(dmsc_context* ctx) {
   if (ctx->pinned) (void)hipHostFree(ctx->pinned);
   if (ctx->cap_stream) (void)hipStreamDestroy(ctx->cap_stream);
   if (ctx->side) (void)hipStreamDestroy(ctx->side);
+  if (ctx->late) (void)hipStreamDestroy(ctx->late);
+  if (ctx->ev_late) (void)hipEventDestroy(ctx->ev_late);
+  if (ctx->ev_late_join) (void)hipEventDestroy(ctx->ev_late_join);
   if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
   if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
   if (ctx->ev_mark) (void)hipEventDestroy(ctx->ev_mark);
