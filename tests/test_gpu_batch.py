@@ -97,6 +97,41 @@ def test_batch_split_into_groups():
         del os.environ["DMSC_GROUP_SLOTS"]
 
 
+@pytest.mark.parametrize("is_dual", [False, True])
+def test_pipelined_halves(is_dual):
+    """A batch run as two halves side by side (the second on a worker thread with its own context and
+    stream) against the oracle, with each half split into several groups too, and bit for bit against
+    the same batch as one pipeline."""
+    import torch
+
+    from dmsc_amd.csrc import dmsc_gpu
+
+    rng = np.random.default_rng(23)
+    fields = [rng.random(s).astype(np.float32) for s in [(40, 33), (1, 1), (64, 129), (7, 11), (90, 20), (3, 80)]]
+    fields.append(synthetic.quantized_noise(80, 60, 3, seed=4).numpy())
+    thrs = [0.0, 0.1, 0.3, 0.05, 0.2, 0.0, 0.1]
+    saved = {k: dmsc_gpu.get_param(0, k) for k in ("pipe_groups", "pipe_min_slots")}
+    try:
+        dmsc_gpu.set_param(0, "pipe_groups", 2)
+        dmsc_gpu.set_param(0, "pipe_min_slots", 0)
+        check_batch(fields, thrs, is_dual=is_dual)
+        os.environ["DMSC_GROUP_SLOTS"] = "3000"
+        try:
+            check_batch(fields, thrs, is_dual=is_dual)
+        finally:
+            del os.environ["DMSC_GROUP_SLOTS"]
+        two = run_list(fields, thrs, is_dual=is_dual)
+        dmsc_gpu.set_param(0, "pipe_groups", 1)
+        one = run_list(fields, thrs, is_dual=is_dual)
+    finally:
+        for k, v in saved.items():
+            dmsc_gpu.set_param(0, k, v)
+    torch.cuda.synchronize()
+    for j, (a, b) in enumerate(zip(two, one)):
+        for i in range(19):
+            assert a[i].shape == b[i].shape and a[i].tobytes() == b[i].tobytes(), (j, i)
+
+
 def test_scheduling_parameters_do_not_change_results():
     import torch
 
