@@ -105,3 +105,23 @@ def test_cell_id_limit_of_the_binding(batched):
         dmsc_gpu.extract_dmsc(view(1, (2**31 - 1) // 8), 0.1)  # H + 1 = 2 > INT32_MAX / 4 / (W + 1) = 1
     with pytest.raises(RuntimeError, match="GPU backend requires"):
         dmsc_gpu.extract_dmsc(view(23169, 23169), 0.1)
+
+
+def test_result_objects_made_from_the_native_record():
+    """compute_dmsc wraps the extension's 19-tensor record (MSComplex._wrap): every field, iteration,
+    equality, printing and dataclass helpers behave as for an object built from the 19 tensors."""
+    import dataclasses
+
+    from dmsc_amd import MSComplex
+
+    record = [torch.arange(k + 1, dtype=torch.int32) for k in range(19)]
+    lazy, eager = MSComplex._wrap(record), MSComplex(*record)
+    names = [f.name for f in dataclasses.fields(MSComplex)]
+    assert all(getattr(lazy, n) is record[i] for i, n in enumerate(names))
+    assert all(a is b for a, b in zip(lazy, eager, strict=True))
+    assert str(MSComplex._wrap(record)) == str(eager)
+    assert all(torch.equal(a, b) for a, b in zip(dataclasses.astuple(MSComplex._wrap(record)), record, strict=True))
+    assert dataclasses.replace(MSComplex._wrap(record), shape=record[0]).ridges is record[12]
+    assert not hasattr(lazy, "no_such_field")
+    with pytest.raises(AttributeError):
+        lazy.no_such_field
