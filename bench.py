@@ -20,6 +20,8 @@ step). --cpu-table times the CPU oracle on every configuration.
 """
 
 import argparse
+import contextlib
+import gc
 import json
 import os
 import sys
@@ -52,6 +54,9 @@ def parse():
     p.add_argument("--cpu-table", action="store_true",
                    help="only time the CPU oracle on every configuration (all allotted cores and 1 thread)")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    p.add_argument("--pipe-groups", type=int, choices=(1, 2), default=None,
+                   help="run large batches as one pipeline (1) or two halves side by side (2, the library's "
+                        "default) for the whole run, e.g. so that a rocprofv3 run matches the roofline pass")
     p.add_argument("--profile", action="store_true", help="print per-kernel event timings to stderr")
     p.add_argument("--copy-calibration", action="store_true",
                    help="only run plain 256 MiB device-to-device copies (reference kernel for PMC byte counters)")
@@ -60,6 +65,21 @@ def parse():
                    help="print a per-kernel table (duration, calibrated HBM bytes, GB/s vs peak) from rocprofv3 "
                         "kernel-trace and FETCH_SIZE / WRITE_SIZE counter CSVs of the same command")
     return p.parse_args()
+
+
+@contextlib.contextmanager
+def gc_paused():
+    """Python's cyclic garbage collector paused over a timed region, as timeit does: a full collection
+    of this process's heap (torch, numpy, the result objects) takes about 45 ms, a host pause unrelated
+    to the GPU work that would otherwise land in one timed step at random. One collection runs first."""
+    gc.collect()
+    was_enabled = gc.isenabled()
+    gc.disable()
+    try:
+        yield
+    finally:
+        if was_enabled:
+            gc.enable()
 
 
 def copy_calibration():
@@ -203,16 +223,37 @@ def measure(name, steps, warmup, plain_steps=0, profile=False):
     torch.cuda.synchronize()
     calls_per_step = len(counts)
 
-    # level 1: events only around the gradient launches and each batch of persistence rounds
-    dmsc_gpu.profile_enable(0, 1)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for s in range(max(warmup, 1), total_steps):
-        step(s)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    prof = dmsc_gpu.profile_read(0)
-    dmsc_gpu.profile_enable(0, 0)
+    def timed(idx, level):
+        """Wall time of the steps idx, and the per-scope event times at profile `level` (1: events only
+        around the gradient launches and each batch of persistence rounds; 0: none)."""
+        dmsc_gpu.profile_enable(0, level)
+        torch.cuda.synchronize()
+        with gc_paused():
+            t0 = time.perf_counter()
+            for s in idx:
+                step(s)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+        prof = dmsc_gpu.profile_read(0)
+        dmsc_gpu.profile_enable(0, 0)
+        return dt, prof
+
+    # A large batch runs as two halves side by side (pipe_groups 2), whose sweeps and gradient
+    # launches then share the GPU with the other half's work. The kernels' roofline figures come from
+    # a separate pass over the same steps with the batch as one pipeline (pipe_groups 1), so that each
+    # launch's time is its own; the headline value is the timed region's, as the library runs it.
+    split_roof = mixed and dmsc_gpu.get_param(0, "pipe_groups") > 1
+    elapsed, prof = timed(range(max(warmup, 1), total_steps), 0 if split_roof else 1)
+    roof_steps, roof_elapsed = steps, elapsed
+    if split_roof:
+        pipe = dmsc_gpu.get_param(0, "pipe_groups")
+        dmsc_gpu.set_param(0, "pipe_groups", 1)
+        try:
+            roof_steps = min(steps, 3)
+            step(max(warmup, 1))  # warmup of the single pipeline (its scratch)
+            roof_elapsed, prof = timed(range(max(warmup, 1), max(warmup, 1) + roof_steps), 1)
+        finally:
+            dmsc_gpu.set_param(0, "pipe_groups", pipe)
 
     px_step = sum(int(sh[0]) * int(sh[1]) for sh, _ in sizes)
     value = px_step * steps / elapsed / 1e6
@@ -230,7 +271,8 @@ def measure(name, steps, warmup, plain_steps=0, profile=False):
             print(f"  {k:20s} total {ms_tot:9.3f} ms  launches {n:6d}  avg {ms_tot / max(n, 1) * 1e3:9.1f} us",
                   file=sys.stderr)
     step_ms = elapsed / steps * 1e3
-    n_calls = steps * calls_per_step
+    roof_step_ms = roof_elapsed / roof_steps * 1e3
+    n_calls = roof_steps * calls_per_step
     roof = None
     pers_ms, _ = prof.get("persist_sweep", (0.0, 0))
     if pers_ms and counts:
@@ -244,17 +286,20 @@ def measure(name, steps, warmup, plain_steps=0, profile=False):
                 "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 5),
                 "traffic": measured_traffic(name, "persistence_sweep"), "bytes_per_launch": int(b_sweep),
                 "avg_launch_us": round(sweep_s * 1e6, 1), "launch_unit": "one sweep (all rounds) per call",
-                "share_of_step_time": round(pers_ms / steps / step_ms, 3)}
+                "share_of_step_time": round(pers_ms / roof_steps / roof_step_ms, 3)}
+        if split_roof:
+            roof["measured_on"] = (f"{roof_steps} extra steps with the batch as one pipeline (pipe_groups 1, "
+                                   f"{roof_step_ms:.1f} ms per step); the timed steps run it as two halves side by side")
     grad_ms, grad_n = prof.get("gradient", (0.0, 0))
     if grad_n:
         # gradient kernel: all fields of a launch (one field per plain call, the whole batch per
         # batched call): 4 B field read per pixel + 16 B gradient record per slot
-        per_launch = sum(KERNEL_MODELS["gradient"](int(sh[0]), int(sh[1])) for sh, _ in sizes) * steps / grad_n
+        per_launch = sum(KERNEL_MODELS["gradient"](int(sh[0]), int(sh[1])) for sh, _ in sizes) * roof_steps / grad_n
         avg_s = grad_ms / grad_n / 1e3
         ach = per_launch / avg_s / 1e9
         g = {"achieved": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4), "avg_launch_us": round(avg_s * 1e6, 2),
              "bytes_per_launch": int(per_launch), "traffic": measured_traffic(name, "gradient"),
-             "share_of_step_time": round(grad_ms / steps / step_ms, 4)}
+             "share_of_step_time": round(grad_ms / roof_steps / roof_step_ms, 4)}
         if roof is None:
             roof = {"bound": "hbm", "kernel": "gradient", "peak": HBM_PEAK_GBS, "unit": "GB/s", **g}
         else:
@@ -266,11 +311,12 @@ def measure(name, steps, warmup, plain_steps=0, profile=False):
     if mixed and plain_steps > 0:  # the same fields through the plain per-field path
         step(0, plain=True)  # warmup
         torch.cuda.synchronize()
-        tp = time.perf_counter()
-        for s in range(plain_steps):
-            step(1 + s % (total_steps - 1), plain=True)
-        torch.cuda.synchronize()
-        plain = px_step * plain_steps / (time.perf_counter() - tp) / 1e6
+        with gc_paused():
+            tp = time.perf_counter()
+            for s in range(plain_steps):
+                step(1 + s % (total_steps - 1), plain=True)
+            torch.cuda.synchronize()
+            plain = px_step * plain_steps / (time.perf_counter() - tp) / 1e6
         out["plain_loop_Mpix_s"] = plain
         out["plain_steps"] = plain_steps
     del copies
@@ -309,6 +355,11 @@ def main():
     import dmsc_amd
     from dmsc_amd.csrc import dmsc_gpu
 
+    if args.pipe_groups is not None:
+        from dmsc_amd import autotune
+
+        autotune.apply_saved(0)  # the saved parameters first, so that they cannot override this one
+        dmsc_gpu.set_param(0, "pipe_groups", args.pipe_groups)
     if args.workload == "diff":
         bench_diff(args, torch, dmsc_amd, dmsc_gpu)
         return
@@ -372,11 +423,12 @@ def bench_diff(args, torch, dmsc_amd, dmsc_gpu):
         step(s)
     torch.cuda.synchronize()
     dmsc_gpu.profile_enable(0, 1)
-    t0 = time.perf_counter()
-    for s in range(max(args.warmup, 1), total_steps):
-        step(s)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
+    with gc_paused():
+        t0 = time.perf_counter()
+        for s in range(max(args.warmup, 1), total_steps):
+            step(s)
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
     prof = dmsc_gpu.profile_read(0)
     dmsc_gpu.profile_enable(0, 0)
     px = B * H * W
