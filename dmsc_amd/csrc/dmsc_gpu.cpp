@@ -31,6 +31,7 @@ const char* kNames[DMSC_NUM_OUTPUTS] = {
 
 struct AllocUser {
   torch::Device dev;
+  c10::hip::HIPStream stream;         // the caller's stream: outputs belong to it
   DMSComplex* res;                    // one per field
   std::vector<torch::Tensor> shared;  // buffers holding one slot of several fields (field = -1)
 };
@@ -39,8 +40,12 @@ torch::TensorOptions opts_for(int32_t dtype, torch::Device dev) {
   return torch::TensorOptions().dtype(dtype == DMSC_DTYPE_FLOAT32 ? torch::kFloat32 : torch::kInt32).device(dev);
 }
 
+// The library calls this one call at a time, for a large batch partly from its worker thread: the
+// guard makes every output a block of the caller's stream in the caching allocator, whichever thread
+// asks for it.
 void* alloc_cb(void* user, int32_t field, int32_t slot, int32_t ndim, const int64_t* dims, int32_t dtype) {
   auto* a = static_cast<AllocUser*>(user);
+  c10::hip::HIPStreamGuard guard(a->stream);
   std::vector<int64_t> shape(dims, dims + ndim);
   if (field < 0) {
     a->shared.push_back(torch::empty(shape, opts_for(dtype, a->dev)));
@@ -127,15 +132,16 @@ uint32_t make_flags(bool grad, bool dual, bool tmax, bool tmin, bool gmax, bool 
          (tmin ? DMSC_TRACE_MIN_ARCS : 0u) | (gmax ? DMSC_TRACE_MAX_GROUPS : 0u) | (gmin ? DMSC_TRACE_MIN_GROUPS : 0u);
 }
 
-DMSComplex run_field(dmsc_context* ctx, const torch::Tensor& f, float thr, uint32_t flags, hipStream_t stream) {
+DMSComplex run_field(dmsc_context* ctx, const torch::Tensor& f, float thr, uint32_t flags,
+                     const c10::hip::HIPStream& stream) {
   DMSComplex r;
-  AllocUser u{f.device(), &r, {}};
+  AllocUser u{f.device(), stream, &r, {}};
   dmsc_outputs out;
   dmsc_status st;
   {
     pybind11::gil_scoped_release release;
     st = dmsc_extract(ctx, f.data_ptr<float>(), (int32_t)f.size(0), (int32_t)f.size(1), thr, flags,
-                      (dmsc_stream_t)stream, alloc_cb, &u, &out);
+                      (dmsc_stream_t)stream.stream(), alloc_cb, &u, &out);
   }
   TORCH_CHECK(st == DMSC_OK, dmsc_last_error(ctx));
   r.t[0] = shape_tensor(out);
@@ -145,16 +151,16 @@ DMSComplex run_field(dmsc_context* ctx, const torch::Tensor& f, float thr, uint3
 // Fields of one device processed together by the C-ABI batch entry (one pipeline for all fields).
 std::vector<DMSComplex> run_batch(dmsc_context* ctx, torch::Device dev, const std::vector<const float*>& ptrs,
                                   const std::vector<int32_t>& hs, const std::vector<int32_t>& ws,
-                                  const std::vector<float>& thr, uint32_t flags, hipStream_t stream) {
+                                  const std::vector<float>& thr, uint32_t flags, const c10::hip::HIPStream& stream) {
   const int B = (int)ptrs.size();
   std::vector<DMSComplex> res(B);
-  AllocUser u{dev, res.data(), {}};
+  AllocUser u{dev, stream, res.data(), {}};
   std::vector<dmsc_outputs> outs(B);
   dmsc_status st;
   {
     pybind11::gil_scoped_release release;
-    st = dmsc_extract_batch(ctx, B, ptrs.data(), hs.data(), ws.data(), thr.data(), flags, (dmsc_stream_t)stream,
-                            alloc_cb, &u, outs.data());
+    st = dmsc_extract_batch(ctx, B, ptrs.data(), hs.data(), ws.data(), thr.data(), flags,
+                            (dmsc_stream_t)stream.stream(), alloc_cb, &u, outs.data());
   }
   TORCH_CHECK(st == DMSC_OK, dmsc_last_error(ctx));
   torch::Tensor shapes = torch::empty({B, 2}, torch::TensorOptions().dtype(torch::kInt32));
@@ -177,7 +183,7 @@ pybind11::object extract_dmsc(torch::Tensor scalar_field, float persistence_thre
   c10::hip::HIPGuard guard(scalar_field.device().index());
   scalar_field = scalar_field.contiguous();
   dmsc_context* ctx = context_for(scalar_field.device().index());
-  hipStream_t stream = c10::hip::getCurrentHIPStream(scalar_field.device().index()).stream();
+  const c10::hip::HIPStream stream = c10::hip::getCurrentHIPStream(scalar_field.device().index());
   const uint32_t flags = make_flags(return_gradient, is_dual, trace_max_arcs, trace_min_arcs, trace_max_groups,
                                     trace_min_groups);
   if (scalar_field.dim() == 2) return pybind11::cast(run_field(ctx, scalar_field, persistence_threshold, flags, stream));
@@ -206,7 +212,7 @@ std::vector<DMSComplex> extract_dmsc_list(std::vector<torch::Tensor> fields, std
   }
   c10::hip::HIPGuard guard(dev.index());
   dmsc_context* ctx = context_for(dev.index());
-  hipStream_t stream = c10::hip::getCurrentHIPStream(dev.index()).stream();
+  const c10::hip::HIPStream stream = c10::hip::getCurrentHIPStream(dev.index());
   const uint32_t flags = make_flags(return_gradient, is_dual, trace_max_arcs, trace_min_arcs, trace_max_groups,
                                     trace_min_groups);
   std::vector<torch::Tensor> contig;
