@@ -57,7 +57,7 @@ enum ArcSlot {
   G_LINK, G_DONE, G_WATCH, G_RR, G_WL, G_CNT, G_JUMP, G_GLEN, G_TL, G_HEAD, kArcSlots
 };
 static_assert(A_ARC_END - A_ARC0 == 2 * kArcSlots, "arc slot count");
-enum ArcSlot2 { H_PJQ = 0, H_FLEN, H_PERPOS, H_POSPRE, H_SEGA, H_SEGB, H_RAW, H_NSEG, H_CUMR, H_CUMC, H_ARCB, H_REC, H_XL };
+enum ArcSlot2 { H_PJQ = 0, H_FLEN, H_PERPOS, H_POSPRE, H_SEGA, H_SEGB, H_RAW, H_NSEG, H_CUMR, H_CUMC, H_ARCB, H_REC, H_XL, H_NCQ };
 
 }  // namespace dmsc
 
@@ -219,6 +219,7 @@ struct ArcWork {
   uint64_t* cum_r = nullptr;  // per extremum: raw / chain pieces of the links on its path to the root
   uint64_t* cum_c = nullptr;
   LinkRec* xl = nullptr;      // per extremum: its link record
+  int4* ncq = nullptr;        // per extremum: raw / chain pieces and path length to the root, packed
   int32_t* out_cells = nullptr;
   int32_t* out_off = nullptr;
   int32_t* out_arc_off = nullptr;
@@ -978,7 +979,8 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     w.cum_r = a2.get<uint64_t>(hb2 + H_CUMR, next + 1);
     w.cum_c = a2.get<uint64_t>(hb2 + H_CUMC, next + 1);
     w.xl = a2.get<LinkRec>(hb2 + H_XL, next + 1);
-    launch_link_counts(w.A, g, w.pjq, next, w.cum_r, w.cum_c, w.xl, cst);
+    w.ncq = a2.get<int4>(hb2 + H_NCQ, next + 1);
+    launch_link_counts(w.A, g, w.pjq, next, w.cum_r, w.cum_c, w.xl, w.ncq, cst);
     w.flen = a2.get<int64_t>(hb2 + H_FLEN, 2 * nsad);
     int64_t* per_pos = a2.get<int64_t>(hb2 + H_PERPOS, nsad);
     w.pos_pre = a2.get<int64_t>(hb2 + H_POSPRE, nsad);
@@ -1083,7 +1085,7 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
       // a batch of levels per host check; a level past the last one finds no chains and only clears
       const int batch = e.level < 4 ? std::min(2, ctx->tune.expand_levels) : ctx->tune.expand_levels;
       for (int k = 0; k < batch; ++k, ++e.level)
-        launch_expand_segments(w.A, w.G, w.pjq, w.cum_r, w.cum_c, w.xl, e.lists[e.level & 1], e.nseg + e.level % 3,
+        launch_expand_segments(w.A, w.G, w.ncq, w.xl, e.lists[e.level & 1], e.nseg + e.level % 3,
                                e.lists[(e.level + 1) & 1], e.nseg + (e.level + 1) % 3, e.nseg + (e.level + 2) % 3, e.raw,
                                e.nseg + 3, std::max<int64_t>(e.n_in, 2048 * 256), e.cap, e.es);
       DMSC_CHECK(hipMemcpyAsync(pin32(s), e.nseg, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, e.es));

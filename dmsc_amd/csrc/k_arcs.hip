@@ -650,9 +650,6 @@ void launch_final_lengths(int64_t ns, const int32_t* ord, const ArcSide& a, cons
 }
 
 __device__ __forceinline__ int64_t seg_od(int64_t off, int fwd) { return off << 1 | (fwd ? 1 : 0); }
-__device__ __forceinline__ int32_t cum_count(const uint64_t* __restrict__ cum, int x) {
-  return x >= 0 ? (int32_t)(cum[x] >> 32) : 0;
-}
 
 // Seeds: per surviving saddle arc, its raw arc then CHAIN(target -> final root); also the field-local
 // CSR offsets.
@@ -763,20 +760,29 @@ __global__ void k_link_counts(ArcSide A, GeoState g, const uint64_t* __restrict_
     cum_c[e] = (uint64_t)J | ((uint64_t)nc << 32);
   }
 }
+__global__ void k_pack_counts(const uint64_t* __restrict__ pjq, const uint64_t* __restrict__ cum_r,
+                              const uint64_t* __restrict__ cum_c, int64_t n, int4* __restrict__ ncq) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+    ncq[e] = make_int4((int32_t)(cum_r[e] >> 32), (int32_t)(cum_c[e] >> 32), (int32_t)(pjq[e] >> 32), 0);
+}
 void launch_link_counts(const ArcSide& a, const GeoState& g, const uint64_t* pjq, int64_t next, uint64_t* cum_r,
-                        uint64_t* cum_c, LinkRec* xl, hipStream_t st) {
+                        uint64_t* cum_c, LinkRec* xl, int4* ncq, hipStream_t st) {
   KScope ks_("link_counts", st);
   hipLaunchKernelGGL(k_link_counts, dim3(grid_for(next + 1, 256, 16384)), dim3(256), 0, st, a, g, pjq, next, cum_r,
                      cum_c, xl);
   DMSC_LAUNCH_CHECK();
   pointer_jump(cum_r, next + 1, g.err, st);
   pointer_jump(cum_c, next + 1, g.err, st);
+  hipLaunchKernelGGL(k_pack_counts, dim3(grid_for(next + 1, 256, 16384)), dim3(256), 0, st, pjq, cum_r, cum_c, next + 1,
+                     ncq);
+  DMSC_LAUNCH_CHECK();
 }
 
 // One expansion level. Each chain (a -> b) reserves exactly its links' pieces (path-sum
-// differences of cum_r / cum_c) with one atomic per wave, then walks its links writing them.
-__global__ __launch_bounds__(256) void k_expand(ArcSide A, GeoState g, const uint64_t* __restrict__ pjq, const uint64_t* __restrict__ cum_r,
-                         const uint64_t* __restrict__ cum_c, const LinkRec* __restrict__ xl,
+// differences of the raw / chain piece counts, read with its length from the ends' packed records)
+// with one atomic per wave, then walks its links writing them.
+__global__ __launch_bounds__(256) void k_expand(ArcSide A, GeoState g, const int4* __restrict__ ncq,
+                         const LinkRec* __restrict__ xl,
                          const Seg* __restrict__ in, const int32_t* __restrict__ nin,
                          Seg* __restrict__ out, int32_t* __restrict__ nout, int32_t* __restrict__ nreset,
                          Seg* __restrict__ raw, int32_t* __restrict__ nraw, int64_t cap) {
@@ -788,11 +794,13 @@ __global__ __launch_bounds__(256) void k_expand(ArcSide A, GeoState g, const uin
   DMSC_BLOCK_LOOP(jb, (int64_t)n, 256) {
     const int64_t jj = jb + threadIdx.x;
     Seg sg{0, 0, 0};
-    int32_t nr = 0, nc = 0;
+    int32_t nr = 0, nc = 0, Lc = 0;
     if (jj < n) {
       sg = in[jj];
-      nr = cum_count(cum_r, sg.a) - cum_count(cum_r, sg.b);
-      nc = cum_count(cum_c, sg.a) - cum_count(cum_c, sg.b);
+      const int4 ca = ncq[sg.a], cb = sg.b >= 0 ? ncq[sg.b] : make_int4(0, 0, 0, 0);
+      nr = ca.x - cb.x;
+      nc = ca.y - cb.y;
+      Lc = ca.z - cb.z;
     }
     int32_t* const cnt[2] = {nraw, nout};
     const int32_t req[2] = {nr, nc};
@@ -807,7 +815,6 @@ __global__ __launch_bounds__(256) void k_expand(ArcSide A, GeoState g, const uin
     const int b = sg.b;
     const int fwd = (int)(sg.od & 1);
     const int64_t off = sg.od >> 1;
-    const int32_t Lc = qlen(pjq, sg.a) - qlen(pjq, b);
     const int pr_end = pr + nr, pc_end = pc + nc;
     int y = sg.a;
     int64_t acc = 0;
@@ -849,13 +856,12 @@ __global__ __launch_bounds__(256) void k_expand(ArcSide A, GeoState g, const uin
     }
   }
 }
-void launch_expand_segments(const ArcSide& a, const GeoState& g, const uint64_t* pjq, const uint64_t* cum_r,
-                            const uint64_t* cum_c, const LinkRec* xl, const Seg* in, const int32_t* nin, Seg* out,
-                            int32_t* nout, int32_t* nreset, Seg* raw, int32_t* nraw, int64_t max_in, int64_t cap,
-                            hipStream_t st) {
+void launch_expand_segments(const ArcSide& a, const GeoState& g, const int4* ncq, const LinkRec* xl, const Seg* in,
+                            const int32_t* nin, Seg* out, int32_t* nout, int32_t* nreset, Seg* raw, int32_t* nraw,
+                            int64_t max_in, int64_t cap, hipStream_t st) {
   KScope ks_("expand_segments", st);
-  hipLaunchKernelGGL(k_expand, dim3(grid_for(max_in, 256, 8192)), dim3(256), 0, st, a, g, pjq, cum_r, cum_c, xl, in,
-                     nin, out, nout, nreset, raw, nraw, cap);
+  hipLaunchKernelGGL(k_expand, dim3(grid_for(max_in, 256, 8192)), dim3(256), 0, st, a, g, ncq, xl, in, nin, out, nout,
+                     nreset, raw, nraw, cap);
   DMSC_LAUNCH_CHECK();
 }
 

@@ -178,9 +178,10 @@ struct alignas(32) LinkRec {
 };
 // Per extremum, after the geometry rounds: cumulative counts of the raw / chain pieces that the
 // links on its path to the final root expand into (pointer-jumped (root, count) pairs), and its
-// link record.
+// link record; then ncq[e] = (raw pieces, chain pieces, path length to the final root, 0), the three
+// path sums a chain's two ends are read for, in one 16-byte record.
 void launch_link_counts(const ArcSide& a, const GeoState& g, const uint64_t* pjq, int64_t next, uint64_t* cum_r,
-                        uint64_t* cum_c, LinkRec* xl, hipStream_t st);
+                        uint64_t* cum_c, LinkRec* xl, int4* ncq, hipStream_t st);
 // arc_base: [B+1] first output cell of each field; sad_out: [B+1] first output saddle of each field.
 // A field's offsets array has one more entry than its saddles, so field f's entries start at
 // sad_out[f] + f in the concatenated offsets output.
@@ -190,8 +191,8 @@ void launch_seed_segments(int64_t ns, const int32_t* ord, const ArcSide& a, cons
                           const int32_t* sad_out, int B, hipStream_t st);
 // One level of expansion: every chain becomes the raw pieces and sub-chains of its links.
 // n = min(*nin, cap) chains are read; *nreset is cleared (the counter the following level appends to).
-void launch_expand_segments(const ArcSide& a, const GeoState& g, const uint64_t* pjq, const uint64_t* cum_r,
-                            const uint64_t* cum_c, const LinkRec* xl, const Seg* in, const int32_t* nin, Seg* out,
+void launch_expand_segments(const ArcSide& a, const GeoState& g, const int4* ncq, const LinkRec* xl, const Seg* in,
+                            const int32_t* nin, Seg* out,
                             int32_t* nout, int32_t* nreset, Seg* raw, int32_t* nraw, int64_t max_in, int64_t cap,
                             hipStream_t st);
 void launch_copy_raw(const ArcSide& a, const Seg* raw, const int32_t* nraw, int64_t max_raw, int32_t* out,
