@@ -762,10 +762,10 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     launch_cancel_compact(nsad, sb[s].cancel, sb[s].pers, pre, cs0, cd0, ck, st);
     g.cs = cs0;
     g.cd = cd0;
-    g.kill = ar.get<uint64_t>(b + G_DEADC, next);
+
     g.link = ar.get<int4>(b + G_LINK, next);
     g.watch = ar.get<uint64_t>(b + G_WATCH, next);
-    g.gjump = ar.get<uint64_t>(b + G_JUMP, next);
+    g.gk = ar.get<uint64_t>(b + G_JUMP, 2 * next);  // (forest entry, kill word) per extremum
     launch_geo_init(g, next, st);  // every extremum its own root, no killer, no watch
   }
   constexpr size_t kCancelPin = 64;  // pinned bytes 64..71: the two cancellation counts
@@ -837,7 +837,7 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
       g.cnt = ar.get<int32_t>(b + G_CNT, 2);
       g.err = d_err;
       radix_rank(ar.get<uint32_t>(b + G_KEYS, nsad), ckey, nc, ar, A_RS2, A_RSP2, cst);
-      launch_geo_items(nc, g.cs, g.cd, ckey, w.A.tinit, g.list(0), g.kill, g.cnt, cst);  // list 0, its count, killers
+      launch_geo_items(nc, g.cs, g.cd, ckey, w.A.tinit, g.list(0), g.gk, g.cnt, cst);  // list 0, its count, killers
     }
     fork.mark_side();  // the geometry rounds wait for this point, not for the emission queued below
   }

@@ -135,10 +135,10 @@ void launch_cancel_compact(int64_t ns, const int32_t* cancel_d, const float* per
 }
 // ------------------------------------------------------------------ geometry rounds
 // initial worklist: every cancellation (saddle-id order) with its key and its two arc targets; and
-// kill[d] = (pending, key of the cancellation whose dead extremum d is)
+// kill word of d = (pending, key of the cancellation whose dead extremum d is)
 __global__ void k_geo_items(int64_t nc, const int32_t* __restrict__ cs, const int32_t* __restrict__ cd,
                             const int32_t* __restrict__ key, const int32_t* __restrict__ tinit, int4* __restrict__ wl,
-                            uint64_t* __restrict__ kill, int32_t* __restrict__ cnt) {
+                            uint64_t* __restrict__ gk, int32_t* __restrict__ cnt) {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     cnt[0] = (int32_t)nc;
     cnt[1] = 0;
@@ -148,18 +148,18 @@ __global__ void k_geo_items(int64_t nc, const int32_t* __restrict__ cs, const in
     const int2 t = *reinterpret_cast<const int2*>(tinit + 2 * s);
     const int k = key[c];
     wl[c] = make_int4((int)c, k, t.x, t.y);
-    kill[cd[c]] = kKillPending | (uint32_t)k;
+    gk[2 * (int64_t)cd[c] + 1] = kKillPending | (uint32_t)k;
   }
 }
 void launch_geo_items(int64_t nc, const int32_t* cs, const int32_t* cd, const int32_t* key, const int32_t* tinit,
-                      int4* wl, uint64_t* kill, int32_t* cnt, hipStream_t st) {
+                      int4* wl, uint64_t* gk, int32_t* cnt, hipStream_t st) {
   hipLaunchKernelGGL(k_geo_items, dim3(grid_for(std::max<int64_t>(nc, 1), 256, 16384)), dim3(256), 0, st, nc, cs, cd,
-                     key, tinit, wl, kill, cnt);
+                     key, tinit, wl, gk, cnt);
   DMSC_LAUNCH_CHECK();
 }
 
-// Roots in the geometry forest and the summed link lengths on the way, by path halving over
-// gjump: (ancestor J, length to J) pairs. The forest only grows at roots (a link is only ever
+// Roots in the geometry forest and the summed link lengths on the way, by path halving over the
+// forest entries gk[2 e]: (ancestor J, length to J) pairs. The forest only grows at roots (a link is only ever
 // added from a root), so a shortcut stays valid once written; finds run in their own kernel (or
 // phase), so concurrent shortcuts are all consistent 64-bit pairs, and an older pair read from a
 // cache is still valid: the accesses are single 64-bit loads and stores without a coherence scope.
@@ -180,7 +180,7 @@ __device__ __forceinline__ void geo_root2(const GeoState& g, int (&x)[2], int32_
     int32_t L[2] = {0, 0};
 #pragma unroll
     for (int k = 0; k < 2; ++k)
-      if (!done[k]) cur[k] = __hip_atomic_load(&g.gjump[x[k]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+      if (!done[k]) cur[k] = __hip_atomic_load(&g.gk[2 * (int64_t)x[k]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
       if (done[k]) continue;
@@ -196,7 +196,7 @@ __device__ __forceinline__ void geo_root2(const GeoState& g, int (&x)[2], int32_
     }
 #pragma unroll
     for (int k = 0; k < 2; ++k)
-      if (!done[k]) nx[k] = __hip_atomic_load(&g.gjump[J[k]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+      if (!done[k]) nx[k] = __hip_atomic_load(&g.gk[2 * (int64_t)J[k]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
       if (done[k]) continue;
@@ -208,7 +208,7 @@ __device__ __forceinline__ void geo_root2(const GeoState& g, int (&x)[2], int32_
         continue;
       }
       const int32_t L2 = sat_add(L[k], (int32_t)(nx[k] >> 32));
-      __hip_atomic_store(&g.gjump[x[k]], (uint64_t)JJ | ((uint64_t)(uint32_t)L2 << 32), __ATOMIC_RELAXED,
+      __hip_atomic_store(&g.gk[2 * (int64_t)x[k]], (uint64_t)JJ | ((uint64_t)(uint32_t)L2 << 32), __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_WAVEFRONT);
       acc[k] = sat_add(acc[k], L2);
       x[k] = JJ == kVoid ? -1 : (int)JJ;
@@ -251,7 +251,7 @@ __device__ __forceinline__ RootPosts geo_find_one(const ArcSide& A, const GeoSta
     L1 = sat_add(base.y, acc[1]);
   }
   const int round = (int)(0xFFFFFFFFu - (uint32_t)(tag >> 32));
-  const uint64_t k0 = R0 >= 0 ? g.kill[R0] : ~0ull, k1 = R1 >= 0 ? g.kill[R1] : ~0ull;
+  const uint64_t k0 = R0 >= 0 ? g.gk[2 * (int64_t)R0 + 1] : ~0ull, k1 = R1 >= 0 ? g.gk[2 * (int64_t)R1 + 1] : ~0ull;
   const int32_t f0 = root_settled(k0, it.y, round) ? 0 : INT32_MIN, f1 = root_settled(k1, it.y, round) ? 0 : INT32_MIN;
   *reinterpret_cast<int4*>(g.rr + 4 * j) = make_int4(R0, R1, L0 | f0, L1 | f1);
   RootPosts p;
@@ -310,14 +310,14 @@ __global__ void k_geo_find(ArcSide A, GeoState g, int round) { geo_find_phase(A,
 __device__ __forceinline__ void geo_decide(const ArcSide& A, const GeoState& g, const int4 it, const int4 r, int d,
                                            int kd, int round) {
   const int c = it.x, kc = it.y;
-  g.kill[d] = ((uint64_t)(uint32_t)round << 32) | (uint32_t)kc;  // c is decided (d is c's dead extremum)
+  g.gk[2 * (int64_t)d + 1] = ((uint64_t)(uint32_t)round << 32) | (uint32_t)kc;  // c is decided (d is c's dead extremum)
   if (kd >= 0) {
     const int s = g.cs[c], ko = 1 - kd;
     const int Ro = kd == 0 ? r.y : r.x;
     const int32_t Ld = kd == 0 ? r.z : r.w, Lo = kd == 0 ? r.w : r.z;
     const int32_t ll = sat_add(sat_add(Ld, A.blen[2 * s + kd]), sat_add(A.blen[2 * s + ko], Lo));
     g.link[d] = make_int4(Ro, ll, s, kd);
-    g.gjump[d] = (uint64_t)(Ro < 0 ? kVoid : (uint32_t)Ro) | ((uint64_t)(uint32_t)ll << 32);
+    g.gk[2 * (int64_t)d] = (uint64_t)(Ro < 0 ? kVoid : (uint32_t)Ro) | ((uint64_t)(uint32_t)ll << 32);
   }
 }
 
@@ -538,31 +538,31 @@ void launch_geo_round(const ArcSide& a, const GeoState& g, int round, int64_t ma
   DMSC_LAUNCH_CHECK();
 }
 
-__global__ void k_geo_init(int4* __restrict__ link, uint64_t* __restrict__ gjump, uint64_t* __restrict__ kill,
+__global__ void k_geo_init(int4* __restrict__ link, uint64_t* __restrict__ gk,
                            uint64_t* __restrict__ watch, int64_t next) {
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < next; e += (int64_t)gridDim.x * blockDim.x) {
     link[e] = make_int4((int)e, 0, 0, 0);
-    gjump[e] = (uint64_t)(uint32_t)e;
-    kill[e] = ~0ull;
+    gk[2 * e] = (uint64_t)(uint32_t)e;
+    gk[2 * e + 1] = ~0ull;
     watch[e] = ~0ull;
   }
 }
 void launch_geo_init(const GeoState& g, int64_t next, hipStream_t st) {
   if (next == 0) return;
-  hipLaunchKernelGGL(k_geo_init, dim3(grid_for(next, 256, 16384)), dim3(256), 0, st, g.link, g.gjump, g.kill, g.watch,
+  hipLaunchKernelGGL(k_geo_init, dim3(grid_for(next, 256, 16384)), dim3(256), 0, st, g.link, g.gk, g.watch,
                      next);
   DMSC_LAUNCH_CHECK();
 }
 
 // ------------------------------------------------------------------ path lengths to the final root
 // pjq[e] = (final root or `next` for the void, length of path(e -> root)), by pointer jumping with
-// saturating sums over the geometry forest. It starts from the rounds' own shortcuts: gjump[e] is an
+// saturating sums over the geometry forest. It starts from the rounds' own shortcuts: gk[2 e] is an
 // ancestor of e (its parent when nothing compressed it) with the saturated length to it.
 __global__ void k_geo_q_init(GeoState g, int64_t next, uint64_t* __restrict__ pj) {
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e <= next; e += (int64_t)gridDim.x * blockDim.x) {
     uint32_t J = (uint32_t)e, D = 0;
     if (e < next) {
-      const uint64_t gj = g.gjump[e];
+      const uint64_t gj = g.gk[2 * e];
       const uint32_t a = (uint32_t)gj;
       if (a != (uint32_t)e) {
         J = a == kVoid ? (uint32_t)next : a;

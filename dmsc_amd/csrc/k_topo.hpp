@@ -120,12 +120,13 @@ struct GeoState {
   const int32_t* cs;       // cancel -> saddle index
   const int32_t* cd;       // cancel -> dead extremum
   const int32_t* key;      // cancel -> sorted position
-  uint64_t* kill;          // per extremum: (round its killing cancellation was decided, or all ones while
-                           // pending) << 32 | that cancellation's key (all ones: no killer)
   int4* link;              // per extremum, one record: (parent in the geometry forest: self for a root,
                            // -1 for the void; length of its link path (saturating); saddle of the
                            // cancellation that linked it; which arc (0/1) of that saddle led to it)
-  uint64_t* gjump;         // path-compressed view of the parents: (ancestor or kVoid, link length to it)
+  uint64_t* gk;            // per extremum e, one 16-byte record: gk[2e] = path-compressed view of the parents
+                           // (ancestor or kVoid, link length to it); gk[2e+1] = kill word: (round its killing
+                           // cancellation was decided, or all ones while pending) << 32 | that cancellation's
+                           // key (all ones: no killer). A root walk's last read brings the root's kill word.
   uint64_t* watch;         // per extremum: (tag << 32 | cancel) atomic min
   int32_t* rr;             // per worklist position: R0, R1, len0, len1
   int4* wl;                // worklists (cancellation, key, arc target 0, arc target 1): list k at
@@ -137,7 +138,7 @@ struct GeoState {
   int32_t* cnt;
   int32_t* err;
 };
-// link[e] = (e, 0, 0, 0), gjump[e] = (e, 0): every extremum its own root; kill and watch all ones
+// link[e] = (e, 0, 0, 0), forest entry (e, 0): every extremum its own root; kill and watch words all ones
 void launch_geo_init(const GeoState& g, int64_t next, hipStream_t st);
 void launch_geo_round(const ArcSide& a, const GeoState& g, int round, int64_t max_items, hipStream_t st);
 // one round of both traced sides (bit s of `live`) in one find and one apply launch
@@ -151,9 +152,9 @@ void launch_cancel_compact(int64_t ns, const int32_t* cancel_d, const float* per
                            int32_t* cs, int32_t* cd, uint32_t* keys, hipStream_t st);
 constexpr uint64_t kKillPending = 0xFFFFFFFFull << 32;
 // list 0 (every cancellation with its key and arc targets, saddle-id order), its count, and the
-// killer word of every dead extremum
+// kill word of every dead extremum (gk: the records of GeoState::gk)
 void launch_geo_items(int64_t nc, const int32_t* cs, const int32_t* cd, const int32_t* key, const int32_t* tinit,
-                      int4* wl, uint64_t* kill, int32_t* cnt, hipStream_t st);
+                      int4* wl, uint64_t* gk, int32_t* cnt, hipStream_t st);
 void launch_geo_q(const GeoState& g, int64_t next, uint64_t* pjq, hipStream_t st);
 
 // Output arc lengths for surviving saddles in min order (ord: survivor list, position = output saddle);
