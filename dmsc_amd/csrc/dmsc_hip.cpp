@@ -29,8 +29,8 @@ namespace dmsc {
 
 void fill_u8(uint8_t* p, uint8_t v, int64_t n, hipStream_t st);
 void fill_f32_zero(float* p, int64_t n, hipStream_t st);
-void scan_i32_to_i64(const int32_t* in, int64_t* out, int64_t n, int64_t* d_total, Arena& ar, int slot, hipStream_t st,
-                     int32_t min_len = 0);
+void scan_arc_offsets(const int32_t* blen, const uint8_t* removed, const int32_t* cancel, int32_t min_len, int64_t* out,
+                      int64_t n, int64_t* d_total, Arena& ar, int slot, hipStream_t st);
 
 // arena slots
 enum : int {
@@ -609,8 +609,6 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     int64_t* boff = ar.get<int64_t>(b + G_BOFF, 2 * nsad);
     int4* tl = ar.get<int4>(b + G_TL, nsad);
     launch_arc_lengths(nsad, s ? cmax : cmin, s ? lmax : lmin, blen, tl, st);
-    // arena offsets of the arcs longer than kHeadCells (the shorter ones live in their head records)
-    scan_i32_to_i64(blen, boff, 2 * nsad, c64 + 2 * s, ar, A_SCAN, st, kHeadCells + 1);
     w.A.faces = s;
     w.A.succ = s ? fsucc : vsucc;
     w.A.fd = bt.fd;
@@ -625,21 +623,6 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     w.A.head = ar.get<int4>(b + G_HEAD, 2 * nsad);
   }
   StreamFork fork(ctx, st, tmin || tmax);
-  // arena sizes (c64[0], c64[2]) arrive with the first readback of the rounds, at pinned byte 256
-  constexpr int kArenaPin = 32;
-  bool raw_launched = false;
-  auto launch_raw = [&]() {
-    const int64_t* tot = ctx->pinned + kArenaPin;
-    for (int s = 0; s < 2; ++s) {
-      if (!aw[s].on) continue;
-      ArcWork& w = aw[s];
-      w.arena_total = tot[2 * s];
-      w.A.arena = ar.get<int32_t>(A_ARC0 + kArcSlots * s + G_ARENA, w.arena_total);
-      launch_raw_arcs(w.A, fork.stream());
-    }
-    fork.mark();  // the expansion waits for the raw arcs only
-    raw_launched = true;
-  };
 
   // ---- persistence rounds (both sides together)
   uint8_t* removed = ar.get<uint8_t>(A_REMOVED, nsad);
@@ -685,8 +668,6 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     int64_t prev_pend = -1;   // ... and at the readback before, prev_round rounds in
     int prev_round = 0;
     while (true) {
-      if (fork.on && !raw_launched)
-        DMSC_CHECK(hipMemcpyAsync(ctx->pinned + kArenaPin, c64, 4 * sizeof(int64_t), hipMemcpyDeviceToHost, st));
       const int round_at_launch = round;  // pend was read back after this many rounds
       bool tail = false;
       {
@@ -728,7 +709,6 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
         DMSC_CHECK(hipMemcpyAsync((char*)ctx->pinned + 4, sb[1].cnt + (round & 1), 4, hipMemcpyDeviceToHost, st));
         DMSC_CHECK(hipStreamSynchronize(st));
       }
-      if (fork.on && !raw_launched) launch_raw();
       const int32_t* pc = (const int32_t*)ctx->pinned;
       ctx->pending_log.insert(ctx->pending_log.end(), {round, pc[0], pc[1]});
       if (pc[0] == 0 && pc[1] == 0) break;
@@ -737,11 +717,6 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
       pend = (int64_t)pc[0] + pc[1];
       if (round > 4 * nsad + 64) throw Fail{DMSC_ERR_INTERNAL, "persistence rounds did not converge"};
     }
-  }
-  if (fork.on && !raw_launched) {  // no rounds ran
-    readback(ctx, c64, 4 * sizeof(int64_t), st);
-    std::memcpy(ctx->pinned + kArenaPin, ctx->pinned, 4 * sizeof(int64_t));
-    launch_raw();
   }
 
   // ---- cancellation lists of the traced sides, ahead of the assembly: their counts come back first,
@@ -762,6 +737,12 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     launch_cancel_compact(nsad, sb[s].cancel, sb[s].pers, pre, cs0, cd0, ck, st);
     g.cs = cs0;
     g.cd = cd0;
+    // arena offsets of the arcs an output can use (the surviving saddles' and this side's
+    // cancellations') that are longer than kHeadCells (the shorter ones live in their head records)
+    aw[s].A.removed = removed;
+    aw[s].A.cancel = sb[s].cancel;
+    scan_arc_offsets(aw[s].A.blen, removed, sb[s].cancel, kHeadCells + 1, ar.get<int64_t>(b + G_BOFF, 2 * nsad),
+                     2 * nsad, c64 + 2 * s, ar, A_SCAN, st);
 
     g.link = ar.get<int4>(b + G_LINK, next);
     g.watch = ar.get<uint64_t>(b + G_WATCH, next);
@@ -769,8 +750,10 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
     launch_geo_init(g, next, st);  // every extremum its own root, no killer, no watch
   }
   constexpr size_t kCancelPin = 64;  // pinned bytes 64..71: the two cancellation counts
+  constexpr size_t kArenaPin = 72;   // pinned bytes 72..103: c64[0..3], the arena sizes at c64[0] and c64[2]
   if (fork.on) {
     DMSC_CHECK(hipMemcpyAsync((char*)ctx->pinned + kCancelPin, c32 + 8, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    DMSC_CHECK(hipMemcpyAsync((char*)ctx->pinned + kArenaPin, c64, 4 * sizeof(int64_t), hipMemcpyDeviceToHost, st));
     fork.follow();  // the side stream waits for the compacted lists (and, with them, the counts)
   }
 
@@ -840,6 +823,18 @@ void run_group(dmsc_context* ctx, const FieldIn* fin, int B, uint32_t flags, hip
       launch_geo_items(nc, g.cs, g.cd, ckey, w.A.tinit, g.list(0), g.gk, g.cnt, cst);  // list 0, its count, killers
     }
     fork.mark_side();  // the geometry rounds wait for this point, not for the emission queued below
+    // The raw arcs go on the same stream after that point: they run beside the assembly and the
+    // geometry rounds, and only the expansion waits for them.
+    int64_t tot[4];
+    std::memcpy(tot, (const char*)ctx->pinned + kArenaPin, sizeof(tot));
+    for (int s = 0; s < 2; ++s) {
+      if (!aw[s].on) continue;
+      ArcWork& w = aw[s];
+      w.arena_total = tot[2 * s];
+      w.A.arena = ar.get<int32_t>(A_ARC0 + kArcSlots * s + G_ARENA, w.arena_total);
+      launch_raw_arcs(w.A, cst);
+    }
+    fork.mark();  // the expansion waits for the raw arcs only
   }
   DMSC_CHECK(hipStreamSynchronize(st));
   int32_t h32[16];
@@ -1197,6 +1192,7 @@ dmsc_context* peer_of(dmsc_context* ctx) {
 void run_pipelined(dmsc_context* ctx, const std::vector<FieldIn>& fields, size_t k, uint32_t flags, hipStream_t st,
                    dmsc_alloc_fn alloc, void* user, dmsc_outputs* outs) {
   dmsc_context* peer = peer_of(ctx);
+  peer->tune = ctx->tune;  // the second half runs with the caller's scheduling parameters
   SerialAlloc sa{alloc, user, {}};
   DMSC_CHECK(hipEventRecord(ctx->ev_pin, st));
   DMSC_CHECK(hipStreamWaitEvent(ctx->peer_st, ctx->ev_pin, 0));  // the peer's inputs are ready
@@ -1206,7 +1202,7 @@ void run_pipelined(dmsc_context* ctx, const std::vector<FieldIn>& fields, size_t
     try {
       DMSC_CHECK(hipSetDevice(ctx->device));
       tl_times = &peer->times;
-      tl_tune = &ctx->tune;
+      tl_tune = &peer->tune;
       run_range(peer, fields.data() + k, fields.size() - k, flags, ctx->peer_st, serial_alloc, &sa, outs);
     } catch (...) {
       perr = std::current_exception();

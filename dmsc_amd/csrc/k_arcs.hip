@@ -67,14 +67,15 @@ __device__ __forceinline__ int32_t cell_of(const ArcSide& A, const FieldDesc& fd
   return 4 * (y * fd.Nx + x);
 }
 
-// one thread per arc: walk the successor forest from the start cell to the extremum; an arc of at
-// most kHeadCells cells goes to its head record (one 16-byte store, indexed by arc), a longer one to
-// the arena
+// one thread per arc that an output can use (its saddle survives or this side cancels it): walk the
+// successor forest from the start cell to the extremum; an arc of at most kHeadCells cells goes to its
+// head record (one 16-byte store, indexed by arc), a longer one to the arena
 __global__ void k_raw_arcs(ArcSide A) {
   for (int64_t a = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; a < 2 * A.ns; a += (int64_t)gridDim.x * blockDim.x) {
     const int len = A.blen[a];
     if (len == 0) continue;
     const int s = (int)(a >> 1);
+    if (A.removed[s] && A.cancel[s] < 0) continue;
     const FieldDesc& fd = A.fd[A.sfield ? A.sfield[s] : 0];
     const int base = (int)(A.faces ? fd.slot_base : fd.pix_base);
     int cur = base + arc_start(A, fd, s, (int)(a & 1));

@@ -86,22 +86,22 @@ struct LoadPop {
     }
   }
 };
-// int32 -> int64; values below min_len count as 0
-struct LoadWiden {
-  const int32_t* p;
-  bool vec;
+// Raw arc i = 2 s + k (length blen[i]) counts when it lies in the arena: longer than min_len - 1 cells
+// and used by the outputs (saddle s survives, or this side cancels it).
+struct LoadArcLen {
+  const int32_t* blen;
+  const uint8_t* removed;
+  const int32_t* cancel;
   int32_t min_len;
-  __device__ int64_t keep(int32_t x) const { return x < min_len ? 0 : (int64_t)x; }
-  __device__ int64_t operator()(int64_t i) const { return keep(p[i]); }
+  __device__ int64_t operator()(int64_t i) const {
+    const int32_t len = blen[i];
+    if (len < min_len) return 0;
+    const int64_t s = i >> 1;
+    return (!removed[s] || cancel[s] >= 0) ? (int64_t)len : 0;
+  }
   __device__ void load8(int64_t base, int64_t n, int64_t (&v)[8]) const {
-    if (vec && base + 8 <= n) {
-      const int4 a = *reinterpret_cast<const int4*>(p + base), b = *reinterpret_cast<const int4*>(p + base + 4);
-      v[0] = keep(a.x); v[1] = keep(a.y); v[2] = keep(a.z); v[3] = keep(a.w);
-      v[4] = keep(b.x); v[5] = keep(b.y); v[6] = keep(b.z); v[7] = keep(b.w);
-    } else {
 #pragma unroll
-      for (int j = 0; j < 8; ++j) v[j] = base + j < n ? keep(p[base + j]) : 0;
-    }
+    for (int j = 0; j < 8; ++j) v[j] = base + j < n ? (*this)(base + j) : 0;
   }
 };
 // 0/1 flags from a byte array (nonzero = 1); 8-byte loads when aligned
@@ -285,9 +285,9 @@ void scan_u8_flags(const uint8_t* in, int32_t* out, int64_t n, int32_t* d_total,
   scan_impl<int32_t>(LoadU8{in, (reinterpret_cast<uintptr_t>(in) & 7) == 0}, out, n, d_total, ar, slot, st);
 }
 
-void scan_i32_to_i64(const int32_t* in, int64_t* out, int64_t n, int64_t* d_total, Arena& ar, int slot, hipStream_t st,
-                     int32_t min_len) {
-  scan_impl<int64_t>(LoadWiden{in, aligned16(in), min_len}, out, n, d_total, ar, slot, st);
+void scan_arc_offsets(const int32_t* blen, const uint8_t* removed, const int32_t* cancel, int32_t min_len, int64_t* out,
+                      int64_t n, int64_t* d_total, Arena& ar, int slot, hipStream_t st) {
+  scan_impl<int64_t>(LoadArcLen{blen, removed, cancel, min_len}, out, n, d_total, ar, slot, st);
 }
 
 // ------------------------------------------------------------------ radix sort

@@ -104,7 +104,11 @@ struct ArcSide {
   const int32_t* blen;     // per arc: raw length in cells (steps + 1), 0 = no arc
   const int4* tl;          // per saddle: both arcs' targets and raw lengths (tinit[2s], tinit[2s+1], blen[2s],
                            // blen[2s+1]) in one 16-byte record for the stages that gather by saddle
-  const int64_t* boff;     // per arc longer than kHeadCells: offset in the raw arena
+  const uint8_t* removed;  // per saddle: cancelled (on either side)
+  const int32_t* cancel;   // per saddle: this side's dead extremum when this side cancels it, else -1
+                           // (an arc is traced when its saddle survives or this side cancels it: no
+                           // output reads the arcs of the other side's cancellations)
+  const int64_t* boff;     // per traced arc longer than kHeadCells: offset in the raw arena
   int32_t* arena;          // raw arcs longer than kHeadCells, cell ids from the saddle outwards
   int4* head;              // per arc of at most kHeadCells cells: its cells (the arena holds the longer ones)
 };
