@@ -88,20 +88,31 @@ struct LoadPop {
 };
 // Raw arc i = 2 s + k (length blen[i]) counts when it lies in the arena: longer than min_len - 1 cells
 // and used by the outputs (saddle s survives, or this side cancels it).
+// A thread's 8 arcs are the arcs of 4 consecutive saddles: with aligned arrays, two 16-byte loads of
+// lengths, one 4-byte load of removal flags and one 16-byte load of cancellations.
 struct LoadArcLen {
   const int32_t* blen;
   const uint8_t* removed;
   const int32_t* cancel;
   int32_t min_len;
-  __device__ int64_t operator()(int64_t i) const {
-    const int32_t len = blen[i];
-    if (len < min_len) return 0;
-    const int64_t s = i >> 1;
-    return (!removed[s] || cancel[s] >= 0) ? (int64_t)len : 0;
+  bool vec;
+  __device__ int64_t keep(int32_t len, uint32_t rm, int32_t cd) const {
+    return (len >= min_len && (!rm || cd >= 0)) ? (int64_t)len : 0;
   }
+  __device__ int64_t operator()(int64_t i) const { return keep(blen[i], removed[i >> 1], cancel[i >> 1]); }
   __device__ void load8(int64_t base, int64_t n, int64_t (&v)[8]) const {
+    if (vec && base + 8 <= n) {
+      const int4 a = *reinterpret_cast<const int4*>(blen + base), b = *reinterpret_cast<const int4*>(blen + base + 4);
+      const uint32_t rm = *reinterpret_cast<const uint32_t*>(removed + (base >> 1));
+      const int4 cd = *reinterpret_cast<const int4*>(cancel + (base >> 1));
+      v[0] = keep(a.x, rm & 0xFF, cd.x); v[1] = keep(a.y, rm & 0xFF, cd.x);
+      v[2] = keep(a.z, rm >> 8 & 0xFF, cd.y); v[3] = keep(a.w, rm >> 8 & 0xFF, cd.y);
+      v[4] = keep(b.x, rm >> 16 & 0xFF, cd.z); v[5] = keep(b.y, rm >> 16 & 0xFF, cd.z);
+      v[6] = keep(b.z, rm >> 24, cd.w); v[7] = keep(b.w, rm >> 24, cd.w);
+    } else {
 #pragma unroll
-    for (int j = 0; j < 8; ++j) v[j] = base + j < n ? (*this)(base + j) : 0;
+      for (int j = 0; j < 8; ++j) v[j] = base + j < n ? (*this)(base + j) : 0;
+    }
   }
 };
 // 0/1 flags from a byte array (nonzero = 1); 8-byte loads when aligned
@@ -287,7 +298,8 @@ void scan_u8_flags(const uint8_t* in, int32_t* out, int64_t n, int32_t* d_total,
 
 void scan_arc_offsets(const int32_t* blen, const uint8_t* removed, const int32_t* cancel, int32_t min_len, int64_t* out,
                       int64_t n, int64_t* d_total, Arena& ar, int slot, hipStream_t st) {
-  scan_impl<int64_t>(LoadArcLen{blen, removed, cancel, min_len}, out, n, d_total, ar, slot, st);
+  const bool vec = aligned16(blen) && aligned16(cancel) && (reinterpret_cast<uintptr_t>(removed) & 3) == 0;
+  scan_impl<int64_t>(LoadArcLen{blen, removed, cancel, min_len, vec}, out, n, d_total, ar, slot, st);
 }
 
 // ------------------------------------------------------------------ radix sort
