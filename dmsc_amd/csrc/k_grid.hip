@@ -120,13 +120,20 @@ __global__ __launch_bounds__(256) void k_gradient(Batch bt, const int64_t* __res
   const int64_t wbase = fd.word_base;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
 
-  // pixels (y0-2 .. y0+TY, x0-2 .. x0+GT_X), a wave per row
-  for (int r = wv; r < GP_H; r += 4) {
-    const int py = y0 - 2 + r;
-    for (int c = lane; c < GP_W; c += 64) {
-      const int px = x0 - 2 + c;
-      pix[r][c] = (py >= 0 && py < H && px >= 0 && px < W) ? f[(int64_t)py * W + px] : 0.0f;
+  // pixels (y0-2 .. y0+TY, x0-2 .. x0+GT_X): the GP_H x GP_W staging area as one flat range over the
+  // block's threads, every load issued before any is stored
+  {
+    constexpr int NP = GP_H * GP_W, KP = (NP + 255) / 256;
+    float v[KP];
+#pragma unroll
+    for (int k = 0; k < KP; ++k) {
+      const int i = tid + 256 * k, r = i / GP_W, c = i - r * GP_W;
+      const int py = y0 - 2 + r, px = x0 - 2 + c;
+      v[k] = (i < NP && py >= 0 && py < H && px >= 0 && px < W) ? f[(int64_t)py * W + px] : 0.0f;
     }
+#pragma unroll
+    for (int k = 0; k < KP; ++k)
+      if (tid + 256 * k < NP) (&pix[0][0])[tid + 256 * k] = v[k];
   }
   __syncthreads();
 
@@ -149,9 +156,14 @@ __global__ __launch_bounds__(256) void k_gradient(Batch bt, const int64_t* __res
       }
     return star_word(lower_star<DUAL>(nb, ex));
   };
+  // the SV_H x SV_W summaries as one flat range over the block's threads (a wave per row would leave
+  // the last row and the last column to one wave)
   auto phase1 = [&](auto interior_tag) {
-    for (int ly = wv; ly < SV_H; ly += 4) sv[ly][lane] = star_at(ly, lane, interior_tag);
-    if (tid < SV_H) sv[tid][GT_X] = star_at(tid, GT_X, interior_tag);
+    constexpr int NV = SV_H * SV_W;
+    for (int i = tid; i < NV; i += 256) {
+      const int ly = i / SV_W, lx = i - ly * SV_W;
+      sv[ly][lx] = star_at(ly, lx, interior_tag);
+    }
   };
   if (interior) phase1(std::true_type{});
   else phase1(std::false_type{});
