@@ -54,9 +54,10 @@ def parse():
     p.add_argument("--cpu-table", action="store_true",
                    help="only time the CPU oracle on every configuration (all allotted cores and 1 thread)")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
-    p.add_argument("--pipe-groups", type=int, choices=(1, 2), default=None,
-                   help="run large batches as one pipeline (1) or two halves side by side (2, the library's "
-                        "default) for the whole run, e.g. so that a rocprofv3 run matches the roofline pass")
+    p.add_argument("--pipe-groups", type=int, choices=tuple(range(1, 9)), default=None,
+                   help="run large batches as one pipeline (1) or as this many parts side by side (the "
+                        "library's default is 4) for the whole run, e.g. so that a rocprofv3 run matches the "
+                        "roofline pass")
     p.add_argument("--profile", action="store_true", help="print per-kernel event timings to stderr")
     p.add_argument("--copy-calibration", action="store_true",
                    help="only run plain 256 MiB device-to-device copies (reference kernel for PMC byte counters)")
@@ -238,8 +239,8 @@ def measure(name, steps, warmup, plain_steps=0, profile=False):
         dmsc_gpu.profile_enable(0, 0)
         return dt, prof
 
-    # A large batch runs as two halves side by side (pipe_groups 2), whose sweeps and gradient
-    # launches then share the GPU with the other half's work. The kernels' roofline figures come from
+    # A large batch runs as several parts side by side (pipe_groups, 4 by default), whose sweeps and
+    # gradient launches then share the GPU with the other parts' work. The kernels' roofline figures come from
     # a separate pass over the same steps with the batch as one pipeline (pipe_groups 1), so that each
     # launch's time is its own; the headline value is the timed region's, as the library runs it.
     split_roof = mixed and dmsc_gpu.get_param(0, "pipe_groups") > 1
@@ -289,7 +290,7 @@ def measure(name, steps, warmup, plain_steps=0, profile=False):
                 "share_of_step_time": round(pers_ms / roof_steps / roof_step_ms, 3)}
         if split_roof:
             roof["measured_on"] = (f"{roof_steps} extra steps with the batch as one pipeline (pipe_groups 1, "
-                                   f"{roof_step_ms:.1f} ms per step); the timed steps run it as two halves side by side")
+                                   f"{roof_step_ms:.1f} ms per step); the timed steps run it as {pipe} parts side by side")
     grad_ms, grad_n = prof.get("gradient", (0.0, 0))
     if grad_n:
         # gradient kernel: all fields of a launch (one field per plain call, the whole batch per

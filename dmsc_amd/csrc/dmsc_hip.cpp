@@ -136,11 +136,13 @@ struct dmsc_context {
   bool bwd_dirty = false;
   hipEvent_t ev_bwd = nullptr;  // end of the last loss / backward work (orders calls on other streams)
   bool bwd_pending = false;
-  // Second half of a large batch: its own context (scratch, pinned buffer, streams) driven by a worker
-  // thread on its own stream, ordered after the caller's stream by ev_pin and before it by ev_pout.
-  dmsc_context* peer = nullptr;
-  hipStream_t peer_st = nullptr;
-  hipEvent_t ev_pin = nullptr, ev_pout = nullptr;
+  // Further parts of a large batch (pipe_groups): each with its own context (scratch, pinned buffer,
+  // streams) driven by a worker thread on its own stream, ordered after the caller's stream by ev_pin
+  // and before it by its ev_pout.
+  static constexpr int kMaxPeers = 7;
+  dmsc_context* peer[kMaxPeers] = {};
+  hipStream_t peer_st[kMaxPeers] = {};
+  hipEvent_t ev_pin = nullptr, ev_pout[kMaxPeers] = {};
 };
 
 namespace {
@@ -1158,7 +1160,7 @@ void run_range(dmsc_context* ctx, const FieldIn* fields, size_t n, uint32_t flag
   }
 }
 
-// The output allocator of a pipelined batch: the two halves' threads call it one at a time.
+// The output allocator of a pipelined batch: the parts' threads call it one at a time.
 struct SerialAlloc {
   dmsc_alloc_fn fn;
   void* user;
@@ -1170,77 +1172,96 @@ void* serial_alloc(void* u, int32_t field, int32_t slot, int32_t ndim, const int
   return a->fn(a->user, field, slot, ndim, dims, dtype);
 }
 
-// The peer context, its stream and the two ordering events (created on first use, kept with ctx).
-dmsc_context* peer_of(dmsc_context* ctx) {
-  if (!ctx->peer) {
+// Peer context i, its stream and its ordering events (created on first use, kept with ctx).
+dmsc_context* peer_of(dmsc_context* ctx, int i) {
+  if (!ctx->peer[i]) {
     dmsc_context* p = nullptr;
     if (dmsc_context_create(ctx->device, &p) != DMSC_OK)
-      throw Fail{DMSC_ERR_HIP, "could not create the context of the batch's second half"};
-    ctx->peer = p;
+      throw Fail{DMSC_ERR_HIP, "could not create the context of a part of the batch"};
+    ctx->peer[i] = p;
   }
-  if (!ctx->peer_st) DMSC_CHECK(hipStreamCreateWithFlags(&ctx->peer_st, hipStreamNonBlocking));
+  if (!ctx->peer_st[i]) DMSC_CHECK(hipStreamCreateWithFlags(&ctx->peer_st[i], hipStreamNonBlocking));
   if (!ctx->ev_pin) DMSC_CHECK(hipEventCreateWithFlags(&ctx->ev_pin, hipEventDisableTiming));
-  if (!ctx->ev_pout) DMSC_CHECK(hipEventCreateWithFlags(&ctx->ev_pout, hipEventDisableTiming));
-  return ctx->peer;
+  if (!ctx->ev_pout[i]) DMSC_CHECK(hipEventCreateWithFlags(&ctx->ev_pout[i], hipEventDisableTiming));
+  return ctx->peer[i];
 }
 
-// A large batch as two halves of about equal slots run side by side: the first on the calling
-// thread and stream, the second on a worker thread with the peer context and stream. Each half is a
-// pipeline of latency-bound round loops and bandwidth-bound passes with host checks in between; run
-// together, one half's round tails and host waits overlap the other's passes (+6 % on the 512-field
-// batch). Results do not depend on the split: fields never meet.
-void run_pipelined(dmsc_context* ctx, const std::vector<FieldIn>& fields, size_t k, uint32_t flags, hipStream_t st,
-                   dmsc_alloc_fn alloc, void* user, dmsc_outputs* outs) {
-  dmsc_context* peer = peer_of(ctx);
-  peer->tune = ctx->tune;  // the second half runs with the caller's scheduling parameters
+// A large batch as parts of about equal slots run side by side: the first on the calling thread and
+// stream, each other one on a worker thread with a peer context and stream. Each part is a pipeline
+// of latency-bound round loops and bandwidth-bound passes with host checks in between; run together,
+// one part's round tails and host waits overlap the others' passes (+6 % on the 512-field batch with
+// two parts). Results do not depend on the split: fields never meet. Part g holds the fields
+// [bounds[g], bounds[g + 1]).
+void run_pipelined(dmsc_context* ctx, const std::vector<FieldIn>& fields, const std::vector<size_t>& bounds,
+                   uint32_t flags, hipStream_t st, dmsc_alloc_fn alloc, void* user, dmsc_outputs* outs) {
+  const int G = (int)bounds.size() - 1;
+  dmsc_context* peers[dmsc_context::kMaxPeers] = {};
+  for (int i = 0; i + 1 < G; ++i) peers[i] = peer_of(ctx, i);
   SerialAlloc sa{alloc, user, {}};
   DMSC_CHECK(hipEventRecord(ctx->ev_pin, st));
-  DMSC_CHECK(hipStreamWaitEvent(ctx->peer_st, ctx->ev_pin, 0));  // the peer's inputs are ready
-  peer->times.level = ctx->times.level;
-  std::exception_ptr perr, merr;
-  std::thread worker([&] {
-    try {
-      DMSC_CHECK(hipSetDevice(ctx->device));
-      tl_times = &peer->times;
-      tl_tune = &peer->tune;
-      run_range(peer, fields.data() + k, fields.size() - k, flags, ctx->peer_st, serial_alloc, &sa, outs);
-    } catch (...) {
-      perr = std::current_exception();
-    }
-  });
+  for (int i = 0; i + 1 < G; ++i) {
+    peers[i]->tune = ctx->tune;  // every part runs with the caller's scheduling parameters
+    peers[i]->times.level = ctx->times.level;
+    DMSC_CHECK(hipStreamWaitEvent(ctx->peer_st[i], ctx->ev_pin, 0));  // the peer's inputs are ready
+  }
+  std::exception_ptr merr, perr[dmsc_context::kMaxPeers];
+  std::vector<std::thread> workers;
+  for (int i = 0; i + 1 < G; ++i)
+    workers.emplace_back([&, i] {
+      try {
+        DMSC_CHECK(hipSetDevice(ctx->device));
+        tl_times = &peers[i]->times;
+        tl_tune = &peers[i]->tune;
+        run_range(peers[i], fields.data() + bounds[i + 1], bounds[i + 2] - bounds[i + 1], flags, ctx->peer_st[i],
+                  serial_alloc, &sa, outs);
+      } catch (...) {
+        perr[i] = std::current_exception();
+      }
+    });
   try {
-    run_range(ctx, fields.data(), k, flags, st, serial_alloc, &sa, outs);
+    run_range(ctx, fields.data(), bounds[1], flags, st, serial_alloc, &sa, outs);
   } catch (...) {
     merr = std::current_exception();
   }
-  worker.join();
-  // the caller's stream orders everything after the peer's work (its outputs, and its scratch reuse)
-  const bool joined = hipEventRecord(ctx->ev_pout, ctx->peer_st) == hipSuccess &&
-                      hipStreamWaitEvent(st, ctx->ev_pout, 0) == hipSuccess;
-  if (!joined) (void)hipStreamSynchronize(ctx->peer_st);
-  // per-stage times and the debug counts cover both halves (the debug arrays are the first half's)
-  for (const auto& kv : peer->times.acc) {
-    bool found = false;
-    for (auto& mine : ctx->times.acc)
-      if (mine.first == kv.first) {
-        mine.second.first += kv.second.first;
-        mine.second.second += kv.second.second;
-        found = true;
-      }
-    if (!found) ctx->times.acc.push_back(kv);
+  for (std::thread& w : workers) w.join();
+  // the caller's stream orders everything after the peers' work (their outputs, and their scratch reuse)
+  bool joined = true;
+  for (int i = 0; i + 1 < G; ++i) {
+    const bool ok = hipEventRecord(ctx->ev_pout[i], ctx->peer_st[i]) == hipSuccess &&
+                    hipStreamWaitEvent(st, ctx->ev_pout[i], 0) == hipSuccess;
+    if (!ok) (void)hipStreamSynchronize(ctx->peer_st[i]);
+    joined = joined && ok;
   }
-  peer->times.acc.clear();
+  // per-stage times and the debug counts cover every part (the debug arrays are the first part's)
+  for (int i = 0; i + 1 < G; ++i) {
+    dmsc_context* peer = peers[i];
+    for (const auto& kv : peer->times.acc) {
+      bool found = false;
+      for (auto& mine : ctx->times.acc)
+        if (mine.first == kv.first) {
+          mine.second.first += kv.second.first;
+          mine.second.second += kv.second.second;
+          found = true;
+        }
+      if (!found) ctx->times.acc.push_back(kv);
+    }
+    peer->times.acc.clear();
+  }
   if (merr) std::rethrow_exception(merr);
-  if (perr) std::rethrow_exception(perr);
-  ctx->nmin += peer->nmin;
-  ctx->nsad += peer->nsad;
-  ctx->nmax += peer->nmax;
-  ctx->S += peer->S;
-  ctx->N += peer->N;
-  for (int i = 0; i < 4; ++i) ctx->arc_stats[i] += peer->arc_stats[i];
-  ctx->rounds_persist = std::max(ctx->rounds_persist, peer->rounds_persist);
-  for (int s = 0; s < 2; ++s) ctx->rounds_geo[s] = std::max(ctx->rounds_geo[s], peer->rounds_geo[s]);
-  if (!joined) throw Fail{DMSC_ERR_HIP, "could not order the caller's stream after the second half of the batch"};
+  for (int i = 0; i + 1 < G; ++i)
+    if (perr[i]) std::rethrow_exception(perr[i]);
+  for (int i = 0; i + 1 < G; ++i) {
+    const dmsc_context* peer = peers[i];
+    ctx->nmin += peer->nmin;
+    ctx->nsad += peer->nsad;
+    ctx->nmax += peer->nmax;
+    ctx->S += peer->S;
+    ctx->N += peer->N;
+    for (int k = 0; k < 4; ++k) ctx->arc_stats[k] += peer->arc_stats[k];
+    ctx->rounds_persist = std::max(ctx->rounds_persist, peer->rounds_persist);
+    for (int s = 0; s < 2; ++s) ctx->rounds_geo[s] = std::max(ctx->rounds_geo[s], peer->rounds_geo[s]);
+  }
+  if (!joined) throw Fail{DMSC_ERR_HIP, "could not order the caller's stream after the other parts of the batch"};
 }
 
 void run_fields(dmsc_context* ctx, const std::vector<FieldIn>& fields, uint32_t flags, hipStream_t st,
@@ -1248,14 +1269,26 @@ void run_fields(dmsc_context* ctx, const std::vector<FieldIn>& fields, uint32_t 
   for (const FieldIn& f : fields) validate_field(f);
   int64_t total = 0;
   for (const FieldIn& f : fields) total += field_slots(f);
-  if (ctx->tune.pipe_groups >= 2 && fields.size() >= 2 && total >= ctx->tune.pipe_min_slots) {
-    size_t k = 1;  // first field of the second half: prefix slots reach half the total
-    for (int64_t pre = field_slots(fields[0]); k + 1 < fields.size() && 2 * pre < total; ++k)
-      pre += field_slots(fields[k]);
-    run_pipelined(ctx, fields, k, flags, st, alloc, user, outs);
+  // parts: pipe_groups, at most one per field and at least pipe_min_slots slots each
+  const size_t n = fields.size();
+  const int64_t by_size = ctx->tune.pipe_min_slots > 0 ? total / ctx->tune.pipe_min_slots : INT64_MAX;
+  const int G = (int)std::min<int64_t>(std::min<int64_t>(ctx->tune.pipe_groups, (int64_t)n), by_size);
+  if (G >= 2) {
+    // part g ends at the first field where the prefix slots reach g / G of the total, each part
+    // keeping at least one field
+    std::vector<size_t> bounds{0};
+    size_t j = 0;
+    int64_t pre = 0;
+    for (int g = 1; g < G; ++g) {
+      while (j < n - (size_t)(G - g) && (j == bounds.back() || (int64_t)G * pre < (int64_t)g * total))
+        pre += field_slots(fields[j++]);
+      bounds.push_back(j);
+    }
+    bounds.push_back(n);
+    run_pipelined(ctx, fields, bounds, flags, st, alloc, user, outs);
     return;
   }
-  run_range(ctx, fields.data(), fields.size(), flags, st, alloc, user, outs);
+  run_range(ctx, fields.data(), n, flags, st, alloc, user, outs);
 }
 
 // Persistence-loss fields in chunks of at most kBwdChunk fields and cap scratch floats, handed to
@@ -1406,10 +1439,12 @@ void dmsc_context_destroy(dmsc_context* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
   (void)hipDeviceSynchronize();
-  if (ctx->peer) dmsc_context_destroy(ctx->peer);
-  if (ctx->peer_st) (void)hipStreamDestroy(ctx->peer_st);
+  for (int i = 0; i < dmsc_context::kMaxPeers; ++i) {
+    if (ctx->peer[i]) dmsc_context_destroy(ctx->peer[i]);
+    if (ctx->peer_st[i]) (void)hipStreamDestroy(ctx->peer_st[i]);
+    if (ctx->ev_pout[i]) (void)hipEventDestroy(ctx->ev_pout[i]);
+  }
   if (ctx->ev_pin) (void)hipEventDestroy(ctx->ev_pin);
-  if (ctx->ev_pout) (void)hipEventDestroy(ctx->ev_pout);
   ctx->ar.release();
   ctx->ar2.release();
   if (ctx->pinned) (void)hipHostFree(ctx->pinned);
@@ -1526,7 +1561,7 @@ int* param_slot(dmsc_context* ctx, const std::string& n, int64_t& lo, int64_t& h
   if (n == "tail_items") { lo = 0; hi = 1 << 16; return &t.tail_items; }
   if (n == "grad_rows") { lo = 8; hi = 32; return &t.grad_rows; }
   if (n == "side_rounds") { lo = 0; hi = 4096; return &t.side_rounds; }
-  if (n == "pipe_groups") { lo = 1; hi = 2; return &t.pipe_groups; }
+  if (n == "pipe_groups") { lo = 1; hi = 1 + dmsc_context::kMaxPeers; return &t.pipe_groups; }
   if (n == "pipe_min_slots") { lo = 0; hi = INT32_MAX; return &t.pipe_min_slots; }
   return nullptr;
 }

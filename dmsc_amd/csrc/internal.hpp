@@ -127,8 +127,8 @@ struct Tunables {
   int side_rounds = 1;       // first persistence rounds run side by side (vertex side, then face side,
                              // which drops what the vertex side just merged); later rounds run both
                              // sides in each launch
-  int pipe_groups = 2;       // 2: a batch of at least pipe_min_slots slots runs as two halves side by side
-  int pipe_min_slots = 1 << 24;
+  int pipe_groups = 4;       // a large batch runs as this many parts side by side ...
+  int pipe_min_slots = 1 << 23;  // ... of at least this many slots each
 };
 extern thread_local const Tunables* tl_tune;
 inline const Tunables& tune() {

@@ -49,9 +49,9 @@ enum { DMSC_DTYPE_INT32 = 0, DMSC_DTYPE_FLOAT32 = 1 };
 /* Allocator callback: return device memory for output `slot` with the given shape and dtype (ndim
  * 1 or 2, row-major), possibly with zero elements (the pointer is then never dereferenced). Return
  * NULL on failure. Never called concurrently. dmsc_extract calls it from the calling thread; for a
- * large batch, dmsc_extract_batch runs its second half on a worker thread of the library (see
- * dmsc_extract_batch) and the calls for that half come from that thread while the calling thread
- * waits, one at a time with the calling thread's own calls.
+ * large batch, dmsc_extract_batch runs all parts but the first on worker threads of the library (see
+ * dmsc_extract_batch) and the calls for those parts come from those threads while the calling thread
+ * works too, one at a time with each other and with the calling thread's own calls.
  *   field >= 0: the outputs of one field (0 for dmsc_extract, the batch index when a batch group
  *     holds a single field); called once per slot 1..18 with that field's shape.
  *   field == -1 (dmsc_extract_batch): one buffer of dims[0] elements holding `slot` of several
@@ -91,12 +91,12 @@ dmsc_status dmsc_extract(dmsc_context* ctx, const float* d_field, int32_t H, int
  * requires one shape and one threshold. out must hold B entries. Fields are processed in groups
  * (up to 2^28 cells per group): all fields of a group share every kernel launch and every
  * persistence / geometry round, so a batch of small fields costs about as many launches as one
- * field. A batch of at least pipe_min_slots (H+1)(W+1) slots (2^24 by default) runs as two halves of
- * about equal size side by side: the first on the calling thread and `stream`, the second on a
- * library worker thread with a second context and stream of its own, ordered after the work queued
- * on `stream` before the call, with `stream` ordered after it when the call returns (dmsc_set_param
- * "pipe_groups" 1 keeps the whole batch on the calling thread). Results equal dmsc_extract on each
- * field alone. */
+ * field. A large batch runs as parts of about equal size side by side, as many as pipe_groups (4 by
+ * default) with at least pipe_min_slots (H+1)(W+1) slots (2^23 by default) each: the first on the
+ * calling thread and `stream`, each other one on a library worker thread with a context and stream
+ * of its own, ordered after the work queued on `stream` before the call, with `stream` ordered after
+ * them when the call returns (dmsc_set_param "pipe_groups" 1 keeps the whole batch on the calling
+ * thread). Results equal dmsc_extract on each field alone. */
 dmsc_status dmsc_extract_batch(dmsc_context* ctx, int32_t B, const float* const* d_fields, const int32_t* H,
                                const int32_t* W, const float* persistence_thresholds, uint32_t flags,
                                dmsc_stream_t stream, dmsc_alloc_fn alloc, void* alloc_user, dmsc_outputs* out);
@@ -157,8 +157,8 @@ dmsc_status dmsc_persistence_loss(dmsc_context* ctx, int32_t B, const dmsc_pairs
  * stretches ("gradient", "persist_sweep" = each batch of persistence rounds launched back to back,
  * "persistence_backward"), level 2 times every stage (each timing event is a stream marker costing
  * a few microseconds of GPU time, so level 2 slows the round loops down), 0 stops. Entries are
- * (name, total ms, launches) over all calls since; a batch run as two halves adds both halves'
- * times (each half on its own stream, so they overlap in time). */
+ * (name, total ms, launches) over all calls since; a batch run as several parts adds all parts'
+ * times (each part on its own stream, so they overlap in time). */
 dmsc_status dmsc_profile_enable(dmsc_context* ctx, int32_t on);
 int32_t dmsc_profile_count(dmsc_context* ctx);
 dmsc_status dmsc_profile_entry(dmsc_context* ctx, int32_t index, const char** name, double* total_ms,
@@ -176,8 +176,9 @@ dmsc_status dmsc_profile_entry(dmsc_context* ctx, int32_t index, const char** na
  *                              of a round loop in one launch (default 2048; 0 = never)
  *   "grad_rows"     8, 16, 32  slot rows of a gradient tile (tiles are 64 slots wide; default 16)
  *   "side_rounds"   0..4096    first persistence rounds run side by side, vertex side first (default 1)
- *   "pipe_groups"   1..2       2: a large batch runs as two halves side by side (default 2)
- *   "pipe_min_slots" 0..2^31-1 slots from which a batch is split into halves (default 2^24)
+ *   "pipe_groups"   1..8       parts of a large batch run side by side, each on its own thread and
+ *                              stream (default 4; 1 = one pipeline on the calling thread)
+ *   "pipe_min_slots" 0..2^31-1 fewest slots per part of a split batch (default 2^23; 0 = no limit)
  * Unknown names or values out of range return DMSC_ERR_INVALID_ARGUMENT. */
 dmsc_status dmsc_set_param(dmsc_context* ctx, const char* name, int64_t value);
 dmsc_status dmsc_get_param(dmsc_context* ctx, const char* name, int64_t* value);
@@ -196,8 +197,8 @@ const char* dmsc_last_error(dmsc_context* ctx);
  * each side), "persist_pending" (per host check of the persistence rounds: round reached, pending
  * saddles of each side), "geo_pending" (the same for the geometry rounds, first entry round 0 with
  * the cancellations of each side), "arc_stats" (int64: raw pieces copied and output arc cells of the vertex
- * side, then of the face side). After a batch run as two halves, "counts" and "arc_stats" cover the
- * whole batch, "rounds" is the larger of the halves', and the arrays and logs are the first half's. */
+ * side, then of the face side). After a batch run as several parts, "counts" and "arc_stats" cover
+ * the whole batch, "rounds" is the largest of the parts', and the arrays and logs are the first part's. */
 int64_t dmsc_debug_fetch(dmsc_context* ctx, const char* name, void* host_dst, int64_t capacity);
 
 #ifdef __cplusplus

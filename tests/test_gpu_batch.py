@@ -99,9 +99,9 @@ def test_batch_split_into_groups():
 
 @pytest.mark.parametrize("is_dual", [False, True])
 def test_pipelined_halves(is_dual):
-    """A batch run as two halves side by side (the second on a worker thread with its own context and
-    stream) against the oracle, with each half split into several groups too, and bit for bit against
-    the same batch as one pipeline."""
+    """A batch run as parts side by side (all but the first on worker threads with their own contexts
+    and streams) against the oracle, with each part split into several groups too, and as two, three
+    and four parts bit for bit against the same batch as one pipeline."""
     import torch
 
     from dmsc_amd.csrc import dmsc_gpu
@@ -120,16 +120,20 @@ def test_pipelined_halves(is_dual):
             check_batch(fields, thrs, is_dual=is_dual)
         finally:
             del os.environ["DMSC_GROUP_SLOTS"]
-        two = run_list(fields, thrs, is_dual=is_dual)
+        parts = []
+        for groups in (2, 3, 4):  # parts side by side, each on its own worker thread and stream
+            dmsc_gpu.set_param(0, "pipe_groups", groups)
+            parts.append(run_list(fields, thrs, is_dual=is_dual))
         dmsc_gpu.set_param(0, "pipe_groups", 1)
         one = run_list(fields, thrs, is_dual=is_dual)
     finally:
         for k, v in saved.items():
             dmsc_gpu.set_param(0, k, v)
     torch.cuda.synchronize()
-    for j, (a, b) in enumerate(zip(two, one)):
-        for i in range(19):
-            assert a[i].shape == b[i].shape and a[i].tobytes() == b[i].tobytes(), (j, i)
+    for split in parts:
+        for j, (a, b) in enumerate(zip(split, one)):
+            for i in range(19):
+                assert a[i].shape == b[i].shape and a[i].tobytes() == b[i].tobytes(), (j, i)
 
 
 def test_scheduling_parameters_do_not_change_results():
