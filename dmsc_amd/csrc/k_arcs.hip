@@ -67,40 +67,91 @@ __device__ __forceinline__ int32_t cell_of(const ArcSide& A, const FieldDesc& fd
   return 4 * (y * fd.Nx + x);
 }
 
-// one thread per arc that an output can use (its saddle survives or this side cancels it): walk the
-// successor forest from the start cell to the extremum; an arc of at most kHeadCells cells goes to its
-// head record (one 16-byte store, indexed by arc), a longer one to the arena
-__global__ void k_raw_arcs(ArcSide A) {
-  for (int64_t a = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; a < 2 * A.ns; a += (int64_t)gridDim.x * blockDim.x) {
-    const int len = A.blen[a];
-    if (len == 0) continue;
-    const int s = (int)(a >> 1);
-    if (A.removed[s] && A.cancel[s] < 0) continue;
-    const FieldDesc& fd = A.fd[A.sfield ? A.sfield[s] : 0];
-    const int base = (int)(A.faces ? fd.slot_base : fd.pix_base);
-    int cur = base + arc_start(A, fd, s, (int)(a & 1));
-    if (len <= kHeadCells) {
-      int32_t h[kHeadCells] = {-1, -1, -1, -1};
+// The arcs that an output can use (their saddle survives or this side cancels it): walk the successor
+// forest from the start cell to the extremum. An arc of at most kHeadCells cells goes to its head
+// record (one 16-byte store, indexed by arc), a longer one to the arena. A wave takes kRawChunk x 64
+// consecutive arcs: it writes the short ones at once and queues the long ones in LDS, then walks the
+// queue with every lane taking the next queued arc as soon as its own ends, so that one long arc no
+// longer holds 63 idle lanes (the walk is one dependent load per cell).
+constexpr int kRawChunk = 8, kRawWaves = 4;
+__global__ __launch_bounds__(64 * kRawWaves) void k_raw_arcs(ArcSide A) {
+  __shared__ int2 q[kRawWaves][64 * kRawChunk];  // per wave: (arc offset in the chunk, length) of its long arcs
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint64_t below = (1ull << lane) - 1;
+  const int64_t n = 2 * A.ns, chunk = 64 * kRawChunk;
+  for (int64_t c0 = ((int64_t)blockIdx.x * kRawWaves + wv) * chunk; c0 < n;
+       c0 += (int64_t)gridDim.x * kRawWaves * chunk) {
+    int nq = 0;
+    for (int u = 0; u < kRawChunk; ++u) {
+      const int64_t a = c0 + u * 64 + lane;
+      int len = 0;
+      if (a < n) {
+        len = A.blen[a];
+        if (len > 0 && A.removed[a >> 1] && A.cancel[a >> 1] < 0) len = 0;
+      }
+      if (len > 0 && len <= kHeadCells) {
+        const int s = (int)(a >> 1);
+        const FieldDesc& fd = A.fd[A.sfield ? A.sfield[s] : 0];
+        const int base = (int)(A.faces ? fd.slot_base : fd.pix_base);
+        int cur = base + arc_start(A, fd, s, (int)(a & 1));
+        int32_t h[kHeadCells] = {-1, -1, -1, -1};
 #pragma unroll
-      for (int j = 0; j < kHeadCells; ++j)
-        if (j < len) {
-          h[j] = cell_of(A, fd, cur - base);
-          if (j + 1 < len) cur = A.succ[cur];
+        for (int t = 0; t < kHeadCells; ++t)
+          if (t < len) {
+            h[t] = cell_of(A, fd, cur - base);
+            if (t + 1 < len) cur = A.succ[cur];
+          }
+        A.head[a] = make_int4(h[0], h[1], h[2], h[3]);
+      }
+      const bool lng = len > kHeadCells;
+      const uint64_t m = __ballot(lng);
+      if (lng) q[wv][nq + __popcll(m & below)] = make_int2(u * 64 + lane, len);
+      nq += __popcll(m);
+    }
+    __builtin_amdgcn_wave_barrier();
+    // queued arcs: rem cells left to write at o, the walk at cur (field-local cell ids from base)
+    int rem = 0, cur = 0, base = 0, W = 1, Nx = 1;
+    int32_t* o = nullptr;
+    for (int head = 0;;) {
+      const uint64_t idle = __ballot(rem == 0);
+      if (head >= nq && idle == ~0ull) break;  // wave-uniform
+      if (rem == 0) {
+        const int r = head + __popcll(idle & below);
+        if (r < nq) {
+          const int2 e = q[wv][r];
+          const int64_t a = c0 + e.x;
+          const int s = (int)(a >> 1);
+          const FieldDesc& fd = A.fd[A.sfield ? A.sfield[s] : 0];
+          base = (int)(A.faces ? fd.slot_base : fd.pix_base);
+          W = fd.W;
+          Nx = fd.Nx;
+          cur = base + arc_start(A, fd, s, (int)(a & 1));
+          o = A.arena + A.boff[a];
+          rem = e.y;
         }
-      A.head[a] = make_int4(h[0], h[1], h[2], h[3]);
-      continue;
+      }
+      head += __popcll(idle);
+      if (rem > 0) {
+        const int node = cur - base;
+        int32_t cell;
+        if (A.faces) {
+          cell = 4 * node + 3;
+        } else {
+          const int y = node / W;
+          cell = 4 * (y * Nx + node - y * W);
+        }
+        *o++ = cell;
+        if (--rem > 0) cur = A.succ[cur];
+      }
     }
-    int32_t* o = A.arena + A.boff[a];
-    for (int j = 0; j < len; ++j) {
-      o[j] = cell_of(A, fd, cur - base);
-      cur = A.succ[cur];
-    }
+    __builtin_amdgcn_wave_barrier();
   }
 }
 void launch_raw_arcs(const ArcSide& a, hipStream_t st) {
   KScope ks_("raw_arcs", st);
   if (a.ns == 0) return;
-  hipLaunchKernelGGL(k_raw_arcs, dim3(grid_for(2 * a.ns, 256, 16384)), dim3(256), 0, st, a);
+  const int64_t chunk = 64 * kRawChunk, waves = (2 * a.ns + chunk - 1) / chunk;
+  hipLaunchKernelGGL(k_raw_arcs, dim3(grid_for(waves, kRawWaves, 16384)), dim3(64 * kRawWaves), 0, st, a);
   DMSC_LAUNCH_CHECK();
 }
 
