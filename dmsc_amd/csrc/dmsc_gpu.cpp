@@ -40,7 +40,7 @@ torch::TensorOptions opts_for(int32_t dtype, torch::Device dev) {
   return torch::TensorOptions().dtype(dtype == DMSC_DTYPE_FLOAT32 ? torch::kFloat32 : torch::kInt32).device(dev);
 }
 
-// The library calls this one call at a time, for a large batch partly from its worker thread: the
+// The library calls this one call at a time, for a large batch partly from its worker threads: the
 // guard makes every output a block of the caller's stream in the caching allocator, whichever thread
 // asks for it.
 void* alloc_cb(void* user, int32_t field, int32_t slot, int32_t ndim, const int64_t* dims, int32_t dtype) {
