@@ -6,7 +6,7 @@
 
 namespace dmsc {
 
-constexpr int kBwdChunk = 16;  // fields per launch (the descriptors travel as kernel arguments)
+constexpr int kBwdChunk = 32;  // fields per launch (the descriptors travel as kernel arguments)
 
 struct BwdField {
   const float* f;      // H x W field (backward only)
